@@ -1,0 +1,148 @@
+/*
+ * ref_harness.c -- test-only checker around the reference OpenZL library.
+ *
+ * Linked against oracle/_ref/libopenzl_ref.so (the reference C library built
+ * from its own sources by oracle/ref.mk). It exposes a flat C interface that
+ * the Python tests load with ctypes in order to produce the reference's own
+ * frames for the graphs this project accelerates, and to decode frames with
+ * the reference's universal decompressor. Nothing on the product path loads
+ * this file.
+ *
+ * Graph construction follows the reference's public API:
+ *   include/openzl/zl_compressor.h:232 (registerStaticGraph_fromNode1o)
+ *   include/openzl/codecs/zl_transpose.h:43 (registerTransposeSplitGraph)
+ *   include/openzl/zl_graph_api.h:102 (registerFunctionGraph)
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "openzl/codecs/zl_bitpack.h"
+#include "openzl/codecs/zl_delta.h"
+#include "openzl/codecs/zl_entropy.h"
+#include "openzl/codecs/zl_transpose.h"
+#include "openzl/codecs/zl_zigzag.h"
+#include "openzl/openzl.h"
+#include "openzl/zl_graph_api.h"
+
+/* Graph kinds (mirrored by openzl_amd.graphs on the Python side). */
+enum {
+    REFH_STORE          = 0,  /* ZL_GRAPH_STORE                                  */
+    REFH_FSE            = 1,  /* ZL_GRAPH_FSE                                    */
+    REFH_HUFFMAN        = 2,  /* ZL_GRAPH_HUFFMAN                                */
+    REFH_BITPACK        = 3,  /* ZL_GRAPH_BITPACK                                */
+    REFH_DELTA_TS_FSE   = 4,  /* DELTA_INT -> transpose_split -> FSE per plane   */
+    REFH_DELTA_ZZ_BP    = 5,  /* DELTA_INT -> ZIGZAG -> BITPACK                  */
+    REFH_TS_HUF_FSE     = 6,  /* transpose_split(2): plane1 HUFFMAN, plane0 FSE  */
+    REFH_DELTA_FSE      = 7,  /* DELTA_INT -> FSE                                */
+    REFH_TS_FSE         = 8,  /* transpose_split -> FSE per plane                */
+    REFH_TS_HUF         = 9,  /* transpose_split -> HUFFMAN per plane            */
+    REFH_ZZ_BP          = 10, /* ZIGZAG -> BITPACK                               */
+    REFH_DELTA_BP       = 11, /* DELTA_INT -> BITPACK                            */
+};
+
+static ZL_Report splitHufFse(ZL_Graph* g, ZL_Edge* inputs[], size_t nbIns)
+{
+    (void)nbIns;
+    ZL_TRY_LET_T(ZL_EdgeList, planes, ZL_Edge_runTransposeSplit(inputs[0], g));
+    if (planes.nbEdges != 2) return ZL_REPORT_ERROR(GENERIC);
+    ZL_RET_R_IF_ERR(ZL_Edge_setDestination(planes.edges[0], ZL_GRAPH_FSE));
+    ZL_RET_R_IF_ERR(ZL_Edge_setDestination(planes.edges[1], ZL_GRAPH_HUFFMAN));
+    return ZL_returnSuccess();
+}
+
+static ZL_GraphID buildGraph(ZL_Compressor* c, int kind)
+{
+    switch (kind) {
+    case REFH_STORE: return ZL_GRAPH_STORE;
+    case REFH_FSE: return ZL_GRAPH_FSE;
+    case REFH_HUFFMAN: return ZL_GRAPH_HUFFMAN;
+    case REFH_BITPACK: return ZL_GRAPH_BITPACK;
+    case REFH_DELTA_TS_FSE: {
+        ZL_GraphID ts = ZL_Compressor_registerTransposeSplitGraph(c, ZL_GRAPH_FSE);
+        return ZL_Compressor_registerStaticGraph_fromNode1o(c, ZL_NODE_DELTA_INT, ts);
+    }
+    case REFH_DELTA_ZZ_BP: {
+        ZL_GraphID zz = ZL_Compressor_registerStaticGraph_fromNode1o(c, ZL_NODE_ZIGZAG, ZL_GRAPH_BITPACK);
+        return ZL_Compressor_registerStaticGraph_fromNode1o(c, ZL_NODE_DELTA_INT, zz);
+    }
+    case REFH_TS_HUF_FSE: {
+        static const ZL_Type mask[1] = { ZL_Type_struct };
+        ZL_FunctionGraphDesc d;
+        memset(&d, 0, sizeof(d));
+        d.name = "split_huf_fse";
+        d.graph_f = splitHufFse;
+        d.inputTypeMasks = mask;
+        d.nbInputs = 1;
+        return ZL_Compressor_registerFunctionGraph(c, &d);
+    }
+    case REFH_DELTA_FSE:
+        return ZL_Compressor_registerStaticGraph_fromNode1o(c, ZL_NODE_DELTA_INT, ZL_GRAPH_FSE);
+    case REFH_TS_FSE:
+        return ZL_Compressor_registerTransposeSplitGraph(c, ZL_GRAPH_FSE);
+    case REFH_TS_HUF:
+        return ZL_Compressor_registerTransposeSplitGraph(c, ZL_GRAPH_HUFFMAN);
+    case REFH_ZZ_BP:
+        return ZL_Compressor_registerStaticGraph_fromNode1o(c, ZL_NODE_ZIGZAG, ZL_GRAPH_BITPACK);
+    case REFH_DELTA_BP:
+        return ZL_Compressor_registerStaticGraph_fromNode1o(c, ZL_NODE_DELTA_INT, ZL_GRAPH_BITPACK);
+    default: return ZL_GRAPH_ILLEGAL;
+    }
+}
+
+/* type: 0 serial, 1 struct, 2 numeric (ZL_Type bit order is not used here).
+ * Returns the frame size, or -(error code) on failure. */
+long long refh_compress(int kind, int type, int width, const void* src, size_t nbElts,
+                        void* dst, size_t dstCapacity, int contentChecksum, int compressedChecksum)
+{
+    ZL_Compressor* c = ZL_Compressor_create();
+    if (!c) return -1;
+    long long ret = -1;
+    ZL_CCtx* cctx = NULL;
+    ZL_TypedRef* tr = NULL;
+    if (ZL_isError(ZL_Compressor_setParameter(c, ZL_CParam_formatVersion, ZL_MAX_FORMAT_VERSION))) goto out;
+    if (!contentChecksum)
+        ZL_Compressor_setParameter(c, ZL_CParam_contentChecksum, ZL_TernaryParam_disable);
+    if (!compressedChecksum)
+        ZL_Compressor_setParameter(c, ZL_CParam_compressedChecksum, ZL_TernaryParam_disable);
+    ZL_GraphID g = buildGraph(c, kind);
+    if (!ZL_GraphID_isValid(g)) goto out;
+    if (ZL_isError(ZL_Compressor_selectStartingGraphID(c, g))) goto out;
+    cctx = ZL_CCtx_create();
+    if (!cctx || ZL_isError(ZL_CCtx_refCompressor(cctx, c))) goto out;
+    switch (type) {
+    case 0: tr = ZL_TypedRef_createSerial(src, nbElts * (size_t)width); break;
+    case 1: tr = ZL_TypedRef_createStruct(src, (size_t)width, nbElts); break;
+    default: tr = ZL_TypedRef_createNumeric(src, (size_t)width, nbElts); break;
+    }
+    if (!tr) goto out;
+    {
+        ZL_Report r = ZL_CCtx_compressTypedRef(cctx, dst, dstCapacity, tr);
+        ret = ZL_isError(r) ? -(long long)ZL_errorCode(r) : (long long)ZL_validResult(r);
+    }
+out:
+    ZL_TypedRef_free(tr);
+    ZL_CCtx_free(cctx);
+    ZL_Compressor_free(c);
+    return ret;
+}
+
+/* Decodes a single-output frame into dst. Returns the decoded byte size, or
+ * -(error code). *outType receives the ZL_Type of the output, *outWidth its
+ * element width. */
+long long refh_decompress(const void* src, size_t srcSize, void* dst, size_t dstCapacity,
+                          int* outType, int* outWidth)
+{
+    ZL_DCtx* d = ZL_DCtx_create();
+    if (!d) return -1;
+    ZL_OutputInfo info;
+    memset(&info, 0, sizeof(info));
+    ZL_Report r = ZL_DCtx_decompressTyped(d, &info, dst, dstCapacity, src, srcSize);
+    ZL_DCtx_free(d);
+    if (ZL_isError(r)) return -(long long)ZL_errorCode(r);
+    if (outType) *outType = (int)info.type;
+    if (outWidth) *outWidth = (int)info.fixedWidth;
+    return (long long)ZL_validResult(r);
+}
+
+size_t refh_compressBound(size_t n) { return ZL_compressBound(n); }
