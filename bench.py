@@ -1,0 +1,285 @@
+#!/usr/bin/env python3
+"""Entropy-stage benchmark of the OpenZL codec hot path on one MI355X.
+
+Workload (the entropy-only stress configuration): 1 GiB of uint8 symbols drawn
+from Zipf(s=1.1) over 256 symbols (ranks permuted by the seed), cut into
+16,384 blocks of 64 KiB. Every block is compressed into its own frame twice,
+with ZL_GRAPH_FSE and with ZL_GRAPH_HUFFMAN, and every frame is decompressed
+again; each coding is one batched call for all blocks (libozl_hip.so C ABI).
+
+One step = FSE encode + Huffman encode + FSE decode + Huffman decode of the
+whole 1 GiB batch, inputs resident in HBM. value = (2 x 1 GiB) / step time.
+
+Also reported:
+  roofline      the kernel with the largest share of the timed region, its
+                algorithmic bytes per launch (input + payload bytes of its
+                blocks) over its mean duration (HIP events around every launch
+                during the timed steps), against 8 TB/s
+  cpu_baseline  the reference OpenZL library (oracle/_ref, compiled from the
+                reference sources) compressing and decompressing a bounded
+                sample of the same blocks with both graphs, one context per
+                thread on all host cores
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "encode+decode GB/s on 1 MI355X (% HBM roofline) vs CPU; ratio loss vs exhaustive"
+HBM_PEAK_GBS = 8000.0
+BLOCK = 65536
+N_BLOCKS = 16384
+SEED = 0x5EED0002
+
+
+# ---------------------------------------------------------------- data
+def zipf_table(seed, s=1.1, card=256):
+    """uint32 inverse-CDF thresholds and the rank -> symbol permutation."""
+    p = 1.0 / np.arange(1, card + 1, dtype=np.float64) ** s
+    cdf = np.cumsum(p / p.sum())
+    thr = np.minimum(np.floor(cdf * 4294967296.0), 4294967295).astype(np.uint64)
+    thr[-1] = 4294967296
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(card).astype(np.uint8)
+    return thr, perm
+
+
+def splitmix_np(idx, seed):
+    with np.errstate(over="ignore"):
+        z = idx.astype(np.uint64) + np.uint64((seed * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def gen_blocks_np(first_block, n_blocks, seed=SEED):
+    thr, perm = zipf_table(seed)
+    idx = np.arange(first_block * BLOCK, (first_block + n_blocks) * BLOCK, dtype=np.uint64)
+    u = splitmix_np(idx, seed) >> np.uint64(32)
+    rank = np.searchsorted(thr, u, side="right")
+    return perm[rank]
+
+
+def gen_blocks_torch(torch, n_blocks, seed=SEED, device="cuda"):
+    """Same values as gen_blocks_np, generated on the GPU in slices."""
+    thr, perm = zipf_table(seed)
+    thr_t = torch.from_numpy(thr.astype(np.int64)).to(device)
+    perm_t = torch.from_numpy(perm).to(device)
+    total = n_blocks * BLOCK
+    out = torch.empty(total, dtype=torch.uint8, device=device)
+    M64 = (1 << 64) - 1
+
+    def s64(v):
+        v &= M64
+        return v - (1 << 64) if v >= (1 << 63) else v
+
+    add = s64(seed * 0x9E3779B97F4A7C15)
+    m1 = s64(0xBF58476D1CE4E5B9)
+    m2 = s64(0x94D049BB133111EB)
+
+    def lsr(x, k):  # logical shift right on int64
+        return (x >> k) & ((1 << (64 - k)) - 1)
+
+    step = 1 << 26
+    for a in range(0, total, step):
+        b = min(total, a + step)
+        z = torch.arange(a, b, dtype=torch.int64, device=device) + add
+        z = (z ^ lsr(z, 30)) * m1
+        z = (z ^ lsr(z, 27)) * m2
+        z = z ^ lsr(z, 31)
+        u = lsr(z, 32)
+        rank = torch.searchsorted(thr_t, u, right=True)
+        out[a:b] = perm_t[rank]
+    return out
+
+
+# ---------------------------------------------------------------- CPU baseline
+def cpu_baseline(sample_blocks, threads, budget_s=12.0):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ozl_checkers as C
+    if C.refh() is None:
+        return None
+    data = gen_blocks_np(0, sample_blocks)
+    blocks = [data[i * BLOCK:(i + 1) * BLOCK] for i in range(sample_blocks)]
+    h = C.refh()
+    cap = h.refh_compressBound(BLOCK) + 4096
+
+    def worker(tid, res):
+        out = np.empty(cap, np.uint8)
+        back = np.empty(BLOCK + 64, np.uint8)
+        t_enc = t_dec = 0.0
+        nbytes = 0
+        tt = ctypes.c_int(0)
+        ww = ctypes.c_int(0)
+        for i in range(tid, sample_blocks, threads):
+            for kind in (C.FSE, C.HUFFMAN):
+                b = blocks[i]
+                t0 = time.perf_counter()
+                r = h.refh_compress(kind, 0, 1, b.ctypes.data, BLOCK, out.ctypes.data, cap, 1, 1)
+                t1 = time.perf_counter()
+                d = h.refh_decompress(out.ctypes.data, r, back.ctypes.data, back.size, ctypes.byref(tt),
+                                      ctypes.byref(ww))
+                t2 = time.perf_counter()
+                assert r > 0 and d == BLOCK
+                t_enc += t1 - t0
+                t_dec += t2 - t1
+                nbytes += BLOCK
+        res[tid] = (t_enc, t_dec, nbytes)
+
+    res = [None] * threads
+    ths = [threading.Thread(target=worker, args=(t, res)) for t in range(threads)]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    wall = time.perf_counter() - t0
+    total = sum(r[2] for r in res)
+    return {
+        "value": round(total / wall / 1e9, 4),
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": f"{sample_blocks} blocks x 64 KiB, each compressed+decompressed with ZL_GRAPH_FSE and "
+                  f"ZL_GRAPH_HUFFMAN by the reference library, one context per thread; "
+                  f"{total / 1e6:.0f} MB in {wall:.1f} s wall",
+    }
+
+
+# ---------------------------------------------------------------- main
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--blocks", type=int, default=N_BLOCKS)
+    ap.add_argument("--cpu-sample", type=int, default=1024, help="blocks in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from openzl_amd import batch, _lib
+
+    L = _lib.lib()
+    B = args.blocks
+    dev = "cuda"
+    print(f"[bench] generating {B} x 64 KiB Zipf(1.1) blocks", file=sys.stderr, flush=True)
+    src = gen_blocks_torch(torch, B)
+    off = torch.arange(0, (B + 1) * BLOCK, BLOCK, dtype=torch.int64, device=dev)
+    enc = {m: batch.EntropyBatchCompressor(B, BLOCK) for m in (batch.FSE, batch.HUFFMAN)}
+    dec = {m: batch.EntropyBatchDecompressor(B) for m in (batch.FSE, batch.HUFFMAN)}
+    out = {m: torch.empty(B * BLOCK, dtype=torch.uint8, device=dev) for m in (batch.FSE, batch.HUFFMAN)}
+    torch.cuda.synchronize()
+
+    def step():
+        frames = {}
+        for m in (batch.FSE, batch.HUFFMAN):
+            frames[m] = enc[m].run(m, src, off)
+        for m in (batch.FSE, batch.HUFFMAN):
+            dst, foff, _ = frames[m]
+            dec[m].run(dst, foff, out[m], off)
+        return frames
+
+    print("[bench] correctness pass", file=sys.stderr, flush=True)
+    frames = step()
+    torch.cuda.synchronize()
+    csize = {}
+    for m in (batch.FSE, batch.HUFFMAN):
+        dst, foff, st = frames[m]
+        assert int(st.abs().sum()) == 0, "compression reported errors"
+        assert int(dec[m].status[:B].abs().sum()) == 0, "decompression reported errors"
+        assert torch.equal(out[m], src), "round trip mismatch"
+        csize[m] = int(foff[B].item())
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    if not args.no_profile:
+        L.ozl_profile_begin()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = {}
+    if not args.no_profile:
+        buf = ctypes.create_string_buffer(1 << 16)
+        L.ozl_profile_end.restype = ctypes.c_size_t
+        L.ozl_profile_end.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+        L.ozl_profile_end(buf, len(buf))
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            prof[name] = (int(cnt), float(ms))
+    ms_step = (t1 - t0) * 1000 / args.steps
+    U = B * BLOCK
+    value = 2 * U / (ms_step / 1000) / 1e9
+
+    # payload bytes per coding (frames minus per-frame headers ~ same order; use frame bytes)
+    roofline = None
+    if prof:
+        dom = max(prof.items(), key=lambda kv: kv[1][1])
+        name, (cnt, tot_ms) = dom
+        avg_ms = tot_ms / cnt
+        per_launch = {
+            "k_fse_encode": U + csize[batch.FSE], "k_fse_decode": U + csize[batch.FSE],
+            "k_huf_encode": U + csize[batch.HUFFMAN], "k_huf_decode": U + csize[batch.HUFFMAN],
+            "k_plan": U, "k_content_hash": U, "k_dverify": U + (csize[0] + csize[1]) // 2,
+            "k_frame_write": (csize[0] + csize[1]) // 2, "k_frame_checksum": (csize[0] + csize[1]) // 2,
+            "k_pack_decode": 0, "k_pack_encode": 0, "k_dparse": 0,
+        }.get(name, U)
+        achieved = per_launch / (avg_ms / 1000) / 1e9
+        roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": per_launch,
+                    "kernel_ms_per_step": {k: round(v[1] / args.steps, 4) for k, v in sorted(prof.items())}}
+
+    cpu = None
+    if args.cpu_sample > 0:
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = max(1, min(threads, 16))
+        print(f"[bench] CPU baseline on {threads} threads", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_sample, threads)
+
+    rec = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded Zipf(1.1) over 256 symbols, ranks permuted)",
+        "config": {
+            "workload": "entropy stress: 16384 x 64 KiB uint8 Zipf(1.1) blocks, each framed with ZL_GRAPH_FSE "
+                        "and ZL_GRAPH_HUFFMAN, encode+decode, format v23, checksums on",
+            "blocks": B, "block_bytes": BLOCK, "input_bytes_per_coding": U,
+            "frame_bytes_fse": csize[batch.FSE], "frame_bytes_huffman": csize[batch.HUFFMAN],
+            "ratio_fse": round(U / csize[batch.FSE], 4), "ratio_huffman": round(U / csize[batch.HUFFMAN], 4),
+            "symbols_per_s": round(2 * U / (ms_step / 1000), 1),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,630 @@
+// Batched FSE / Huffman dynamic graphs on gfx950: one frame per input block.
+//
+// Pipeline for B blocks (each < 100000 bytes, so the graphs do not split it,
+// src/openzl/codecs/entropy/encode_entropy_binding.c:491-519):
+//   k_plan          one workgroup per block: LDS histogram, then one lane
+//                   decides the backend (:601-753) and builds the FSE tables or
+//                   Huffman code, the NCount, and the nested graph of the
+//                   Huffman weights (ozl_core.h)
+//   k_fse_encode    one 4-lane group per block, one lane per interleaved state;
+//                   the 16 groups of a wavefront share nothing but the issue
+//                   slot, their tables sit in LDS
+//   k_huf_encode    one workgroup per block: code lengths, a block-wide scan of
+//                   bit offsets in reverse symbol order, bits OR-ed into LDS
+//   k_pack_encode   flatpack / bitpack payloads, one output byte per lane
+//   k_content_hash  one wavefront per block (ozl_xxh3_wave)
+//   k_frame_size -> k_scan -> k_frame_write -> k_frame_checksum
+#include "ozl_internal.h"
+#include "ozl_core.h"
+#include "ozl_xxh3.h"
+#include "ozl_frame.h"
+#include "ozl_profile.h"
+
+#define OZL_TAB_BYTES 10240  // per block: FSE stateTable u16[4096] + tt u32[256]; HUF len[256] + val u16[256]
+
+struct ozl_ews {
+    ozl_eplan* plans;
+    uint8_t* tables;
+    uint8_t* aux;
+    uint8_t* payload;
+    uint64_t* frameOff;  // B+1
+    size_t payloadCap;
+};
+
+namespace {
+
+constexpr int kT = 256;
+
+__device__ inline void block_hist(const uint8_t* src, uint32_t n, uint32_t* h4 /*4*256*/, uint32_t* count)
+{
+    int tid = threadIdx.x, wave = tid >> 6;
+    for (int i = tid; i < 4 * 256; i += kT) h4[i] = 0;
+    __syncthreads();
+    uint32_t* h = h4 + wave * 256;
+    // aligned 16-byte body, byte head and tail
+    uint32_t head = (uint32_t)((16 - ((uintptr_t)src & 15)) & 15);
+    if (head > n) head = n;
+    for (uint32_t i = tid; i < head; i += kT) atomicAdd(&h[src[i]], 1u);
+    uint32_t nv = (n - head) / 16;
+    const uint4* v = reinterpret_cast<const uint4*>(src + head);
+    for (uint32_t i = tid; i < nv; i += kT) {
+        uint4 q = v[i];
+        uint32_t w[4] = { q.x, q.y, q.z, q.w };
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) atomicAdd(&h[(w[k] >> (8 * b)) & 255], 1u);
+    }
+    for (uint32_t i = head + nv * 16 + tid; i < n; i += kT) atomicAdd(&h[src[i]], 1u);
+    __syncthreads();
+    for (int s = tid; s < 256; s += kT) count[s] = h4[s] + h4[256 + s] + h4[512 + s] + h4[768 + s];
+    __syncthreads();
+}
+
+struct PlanLds {
+    uint32_t h4[4 * 256];
+    uint32_t count[256];
+    int16_t norm[256];
+    uint8_t cells[4096];
+    ozl_hnode hn[2 * 256 + 2];
+    uint8_t wts[256];
+    uint32_t wcount[256];
+    uint16_t wst[4096];
+    uint32_t wtt[256];
+};
+
+// Nested FSE graph on the Huffman weights (entropyCompressChunk in EBM_fse
+// mode on the num->serial converted weights stream).
+__device__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux)
+{
+    uint32_t m = p.maxSym + 1u;
+    if (m < OZL_MIN_STREAM_SIZE) { p.wbk = 255; return; }
+    for (int s = 0; s < 256; s++) L.wcount[s] = 0;
+    for (uint32_t i = 0; i < m; i++) L.wcount[L.wts[i]]++;
+    unsigned wms, wcard;
+    int wbk = ozl_decide_from_hist(L.wcount, m, 0, &wms, &wcard);
+    p.wbk = (uint8_t)wbk;
+    p.wMaxSym = (uint8_t)wms;
+    uint8_t* side = aux + OZL_AUX_W_SIDE;
+    uint8_t* pay = aux + OZL_AUX_W_PAYLOAD;
+    switch (wbk) {
+    case OZL_BK_CONSTANT: pay[0] = L.wts[0]; p.wPayload = 1; break;
+    case OZL_BK_FLATPACK: {
+        size_t ps;
+        p.wAux = (uint16_t)ozl_flatpack_encode_seq(side, pay, &ps, L.wts, m, L.wcount);
+        p.wPayload = (uint16_t)ps;
+        break;
+    }
+    case OZL_BK_BITPACK: {
+        int nb = 1 + (wms ? ozl_hb32(wms) : 0);
+        p.wAux8 = (uint8_t)nb;
+        p.wPayload = (uint16_t)ozl_bitpack8_seq(pay, L.wts, m, nb);
+        break;
+    }
+    case OZL_BK_FSE: {
+        unsigned tl = ozl_fse_optimal_tablelog(OZL_FSE_DEFAULT_TABLELOG, m, wms, 2);
+        p.wTl = (uint8_t)tl;
+        int16_t wn[256];
+        if (ozl_fse_normalize(wn, tl, L.wcount, m, wms) != 0) { p.status = OZL_ERR_GENERIC; return; }
+        if (ozl_fse_build_ctable(L.wst, L.wtt, L.cells, wn, wms, tl)) { p.status = OZL_ERR_GENERIC; return; }
+        size_t bits = ozl_fse_encode_seq(pay, OZL_AUX_BYTES - OZL_AUX_W_PAYLOAD, L.wts, m, L.wst, L.wtt, tl, 2);
+        if (bits == 0) { p.status = OZL_ERR_GENERIC; return; }
+        p.wPayload = (uint16_t)bits;
+        if (2u * (wms + 1u) < OZL_MIN_STREAM_SIZE) {
+            for (unsigned s = 0; s <= wms; s++) { side[2 * s] = (uint8_t)wn[s]; side[2 * s + 1] = (uint8_t)((uint16_t)wn[s] >> 8); }
+            p.wAux = (uint16_t)(2 * (wms + 1));
+        } else {
+            p.wAux = (uint16_t)ozl_fse_write_ncount(side, wn, wms, tl);
+        }
+        break;
+    }
+    default: break;
+    }
+}
+
+template <int HUF>
+__global__ void __launch_bounds__(kT) k_plan(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
+                                             uint32_t B, ozl_ews ws)
+{
+    __shared__ PlanLds L;
+    uint32_t b = blockIdx.x;
+    if (b >= B) return;
+    const uint8_t* in = src + off[b];
+    uint32_t n = (uint32_t)(off[b + 1] - off[b]);
+    block_hist(in, n, L.h4, L.count);
+    if (threadIdx.x != 0) return;
+    ozl_eplan p;
+    uint32_t* pw = reinterpret_cast<uint32_t*>(&p);
+    for (size_t k = 0; k < sizeof(p) / 4; k++) pw[k] = 0;
+    p.n = n;
+    uint8_t* aux = ws.aux + (size_t)b * OZL_AUX_BYTES;
+    uint8_t* tab = ws.tables + (size_t)b * OZL_TAB_BYTES;
+    unsigned maxSym, card;
+    int bk = ozl_decide_from_hist(L.count, n, HUF, &maxSym, &card);
+    if (n < OZL_MIN_STREAM_SIZE) bk = OZL_BK_STORE;
+    p.bk = (uint8_t)bk;
+    p.maxSym = (uint8_t)maxSym;
+    p.card = (uint16_t)card;
+    switch (bk) {
+    case OZL_BK_CONSTANT: aux[0] = (uint8_t)maxSym; p.aux = (uint16_t)maxSym; break;
+    case OZL_BK_FLATPACK: {
+        unsigned c = 0;
+        for (int s = 0; s < 256; s++)
+            if (L.count[s]) aux[c++] = (uint8_t)s;
+        p.aux = (uint16_t)c;
+        unsigned nb = c <= 1 ? c : 1 + (unsigned)ozl_hb32(c - 1);
+        p.payload = 1 + nb * n / 8;
+        break;
+    }
+    case OZL_BK_BITPACK: {
+        int nb = 1 + (maxSym ? ozl_hb32(maxSym) : 0);
+        p.aux = (uint16_t)nb;
+        p.payload = (uint32_t)(((size_t)n * nb + 7) / 8);
+        break;
+    }
+    case OZL_BK_FSE: {
+        unsigned tl = ozl_fse_optimal_tablelog(OZL_FSE_DEFAULT_TABLELOG, n, maxSym, 2);
+        p.tl = (uint8_t)tl;
+        p.nbStates = n < 1000 ? 2 : 4;
+        if (ozl_fse_normalize(L.norm, tl, L.count, n, maxSym) != 0) { p.status = OZL_ERR_GENERIC; break; }
+        uint16_t* st = reinterpret_cast<uint16_t*>(tab);
+        uint32_t* tt = reinterpret_cast<uint32_t*>(tab + 8192);
+        if (ozl_fse_build_ctable(st, tt, L.cells, L.norm, maxSym, tl)) { p.status = OZL_ERR_GENERIC; break; }
+        if (2u * (maxSym + 1u) < OZL_MIN_STREAM_SIZE) {
+            for (unsigned s = 0; s <= maxSym; s++) { aux[2 * s] = (uint8_t)L.norm[s]; aux[2 * s + 1] = (uint8_t)((uint16_t)L.norm[s] >> 8); }
+            p.aux = (uint16_t)(2 * (maxSym + 1));
+        } else {
+            p.aux = (uint16_t)ozl_fse_write_ncount(aux, L.norm, maxSym, tl);
+        }
+        break;
+    }
+    case OZL_BK_HUF: {
+        unsigned tl = ozl_fse_optimal_tablelog(OZL_HUF_DEFAULT_TABLELOG, n, maxSym, 1);
+        uint8_t* len = tab;
+        uint16_t* val = reinterpret_cast<uint16_t*>(tab + 256);
+        int mx = ozl_huf_build(len, val, L.count, maxSym, tl, L.hn);
+        if (mx < 0) { p.status = OZL_ERR_GENERIC; break; }
+        p.tl = (uint8_t)mx;
+        p.nbStates = n > 1000 ? 1 : 0;
+        for (unsigned s = 0; s <= maxSym; s++) {
+            uint8_t w = len[s] ? (uint8_t)(mx + 1 - len[s]) : 0;
+            L.wts[s] = w;
+            aux[s] = w;
+        }
+        plan_weights(p, L, aux);
+        break;
+    }
+    default: break;
+    }
+    ws.plans[b] = p;
+}
+
+// ------------------------------------------------------------------ FSE encode
+// 16 blocks per 64-lane workgroup; lane q of a 4-lane group owns encoder state
+// q (fse_compress.c:607-660). Emission e (e = 0.. n-S-1) encodes position
+// p = n-S-1-e with state (e - n) mod S; a round gives each lane one emission,
+// the group combines the four (value, nbBits) pairs in emission order and lane
+// 0 appends them to a 32-bit-word writer.
+constexpr int kFseGroups = 16;
+
+__global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
+                                                   uint32_t B, ozl_ews ws)
+{
+    __shared__ uint16_t sST[kFseGroups][2048];
+    __shared__ uint32_t sTT[kFseGroups][256];
+    const int lane = threadIdx.x & 63, g = lane >> 2, q = lane & 3;
+    const uint32_t b = blockIdx.x * kFseGroups + g;
+    bool active = b < B;
+    ozl_eplan p;
+    if (active) {
+        p = ws.plans[b];
+        active = p.bk == OZL_BK_FSE && p.status == 0 && p.tl <= 11;
+    }
+    const uint8_t* tab = ws.tables + (size_t)b * OZL_TAB_BYTES;
+    if (active) {
+        unsigned ts = 1u << p.tl;
+        const uint16_t* st = reinterpret_cast<const uint16_t*>(tab);
+        const uint32_t* tt = reinterpret_cast<const uint32_t*>(tab + 8192);
+        for (unsigned i = q; i < ts; i += 4) sST[g][i] = st[i];
+        for (unsigned i = q; i <= p.maxSym; i += 4) sTT[g][i] = tt[i];
+    }
+    __syncthreads();
+    if (!active) return;
+    const uint8_t* in = src + off[b];
+    const uint32_t n = p.n, S = p.nbStates, tl = p.tl;
+    const uint16_t* ST = sST[g];
+    const uint32_t* TT = sTT[g];
+    uint32_t* outw = reinterpret_cast<uint32_t*>(ws.payload + (size_t)b * ws.payloadCap);
+    const uint32_t capWords = (uint32_t)(ws.payloadCap / 4) - 2;
+    const bool mine = q < (int)S;
+    // init: the one position among the last S whose state index is q
+    uint32_t state = 0;
+    if (mine) {
+        uint32_t pinit = 0;
+        for (uint32_t k = 0; k < S; k++) {
+            uint32_t pp = n - 1 - k;
+            if ((S - 1 - (pp % S)) % S == (uint32_t)q) pinit = pp;
+        }
+        uint32_t t = TT[in[pinit]];
+        uint32_t dnb = ozl_tt_dnb(t);
+        uint32_t nbOut = (dnb + (1u << 15)) >> 16;
+        uint32_t v = (nbOut << 16) - dnb;
+        state = ST[(int)(v >> nbOut) + ozl_tt_dfs(t)];
+    }
+    const uint32_t nEmit = n - S;
+    const uint32_t kq = (uint32_t)(q + n) % S;  // this lane's slot inside a round
+    const int base = lane & ~3;
+    // for each slot, which lane owns it
+    int slotLane[4];
+    for (uint32_t k = 0; k < 4; k++) slotLane[k] = base + (int)((k + S - (n % S)) % S);
+    uint64_t acc = 0;
+    uint32_t accBits = 0, wpos = 0;
+    const uint32_t rounds = (nEmit + S - 1) / S;
+    for (uint32_t r = 0; r < rounds; r++) {
+        uint32_t e = r * S + kq;
+        uint32_t nb = 0, val = 0;
+        if (mine && e < nEmit) {
+            uint32_t t = TT[in[nEmit - 1 - e]];
+            nb = (state + ozl_tt_dnb(t)) >> 16;
+            val = state & ((1u << nb) - 1);
+            state = ST[(int)(state >> nb) + ozl_tt_dfs(t)];
+        }
+        // gather in emission order: two halves of at most 24 bits
+        uint32_t v0 = __shfl(val, slotLane[0], 64), n0 = __shfl(nb, slotLane[0], 64);
+        uint32_t v1 = __shfl(val, slotLane[1], 64), n1 = __shfl(nb, slotLane[1], 64);
+        uint32_t v2 = __shfl(val, slotLane[2], 64), n2 = __shfl(nb, slotLane[2], 64);
+        uint32_t v3 = __shfl(val, slotLane[3], 64), n3 = __shfl(nb, slotLane[3], 64);
+        if (S == 2) { n2 = n3 = 0; v2 = v3 = 0; }
+        acc |= (uint64_t)(v0 | (v1 << n0)) << accBits;
+        accBits += n0 + n1;
+        if (accBits >= 32) {
+            if (q == 0 && wpos < capWords) outw[wpos] = (uint32_t)acc;
+            wpos++;
+            acc >>= 32;
+            accBits -= 32;
+        }
+        acc |= (uint64_t)(v2 | (v3 << n2)) << accBits;
+        accBits += n2 + n3;
+        if (accBits >= 32) {
+            if (q == 0 && wpos < capWords) outw[wpos] = (uint32_t)acc;
+            wpos++;
+            acc >>= 32;
+            accBits -= 32;
+        }
+    }
+    // flush the states in index order, then the end mark
+    for (uint32_t i = 0; i < S; i++) {
+        uint32_t sv = __shfl(state, base + (int)i, 64) & ((1u << tl) - 1);
+        acc |= (uint64_t)sv << accBits;
+        accBits += tl;
+        if (accBits >= 32) {
+            if (q == 0 && wpos < capWords) outw[wpos] = (uint32_t)acc;
+            wpos++;
+            acc >>= 32;
+            accBits -= 32;
+        }
+    }
+    acc |= 1ull << accBits;
+    accBits += 1;
+    uint32_t tailBytes = (accBits + 7) / 8;
+    if (q == 0) {
+        if (wpos >= capWords) {
+            ws.plans[b].status = OZL_ERR_INTERNAL_BUFFER;
+        } else {
+            uint8_t* ob = reinterpret_cast<uint8_t*>(outw + wpos);
+            for (uint32_t k = 0; k < tailBytes; k++) ob[k] = (uint8_t)(acc >> (8 * k));
+        }
+        ws.plans[b].payload = wpos * 4 + tailBytes;
+    }
+}
+
+// ------------------------------------------------------------------ Huffman encode
+// One workgroup per block (huf_compress.c:1139-1196 for 4 streams, :1022-1086
+// per stream). Per stream: lane t takes a run of symbols in reverse order,
+// sums their code lengths, a block scan gives its starting bit, and it ORs its
+// codes into an LDS bit buffer; lane 0 then knows the stream size.
+constexpr int kHufMaxSeg = 25008;  // (100000+3)/4 symbols per stream
+
+template <typename T>
+__device__ T block_scan_excl(T v, T* lds, T& total)
+{
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        T y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    T off = 0, all = 0;
+    for (int w = 0; w < kT / 64; w++) {
+        T s = lds[w];
+        if (w < wave) off += s;
+        all += s;
+    }
+    __syncthreads();
+    total = all;
+    return off + x - v;
+}
+
+__global__ void __launch_bounds__(kT) k_huf_encode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
+                                                   uint32_t B, ozl_ews ws)
+{
+    __shared__ uint8_t sLen[256];
+    __shared__ uint16_t sVal[256];
+    __shared__ uint32_t sBits[(kHufMaxSeg * 12) / 32 + 64];
+    __shared__ uint32_t sScan[4];
+    __shared__ uint32_t sOut;
+    uint32_t b = blockIdx.x;
+    if (b >= B) return;
+    ozl_eplan p = ws.plans[b];
+    if (p.bk != OZL_BK_HUF || p.status != 0) return;
+    const uint8_t* tab = ws.tables + (size_t)b * OZL_TAB_BYTES;
+    for (int s = threadIdx.x; s < 256; s += kT) {
+        sLen[s] = s <= p.maxSym ? tab[s] : 0;
+        sVal[s] = s <= p.maxSym ? reinterpret_cast<const uint16_t*>(tab + 256)[s] : 0;
+    }
+    const uint8_t* in = src + off[b];
+    uint8_t* out = ws.payload + (size_t)b * ws.payloadCap;
+    const uint32_t n = p.n;
+    const int x4 = p.nbStates;
+    const uint32_t seg = x4 ? (n + 3) / 4 : n;
+    const int nbSeg = x4 ? 4 : 1;
+    uint32_t opos = x4 ? 6 : 0;
+    __syncthreads();
+    for (int k = 0; k < nbSeg; k++) {
+        uint32_t s0 = k * seg;
+        uint32_t len = (k < nbSeg - 1) ? seg : n - s0;
+        // lane t handles reverse indices [t*per, t*per+per): symbol s0+len-1-r
+        uint32_t per = (len + kT - 1) / kT;
+        uint32_t r0 = threadIdx.x * per, r1 = r0 + per < len ? r0 + per : len;
+        uint32_t mine = 0;
+        for (uint32_t r = r0; r < r1; r++) mine += sLen[in[s0 + len - 1 - r]];
+        uint32_t total;
+        uint32_t bit = block_scan_excl<uint32_t>(mine, sScan, total);
+        uint32_t words = (total + 1 + 31) / 32;
+        for (uint32_t w = threadIdx.x; w < words; w += kT) sBits[w] = 0;
+        __syncthreads();
+        for (uint32_t r = r0; r < r1; r++) {
+            uint8_t sy = in[s0 + len - 1 - r];
+            uint32_t l = sLen[sy], v = sVal[sy];
+            // code value bits go LSB-first at [bit, bit+l)
+            uint32_t w = bit >> 5, sh = bit & 31;
+            atomicOr(&sBits[w], v << sh);
+            if (sh + l > 32) atomicOr(&sBits[w + 1], v >> (32 - sh));
+            bit += l;
+        }
+        if (threadIdx.x == 0) {
+            uint32_t w = total >> 5, sh = total & 31;
+            atomicOr(&sBits[w], 1u << sh);  // end mark
+        }
+        __syncthreads();
+        uint32_t bytes = (total + 1 + 7) / 8;
+        if (opos + bytes > ws.payloadCap) {  // cannot happen for codes of <= 11 bits chosen by the estimate
+            if (threadIdx.x == 0) ws.plans[b].status = OZL_ERR_INTERNAL_BUFFER;
+            return;
+        }
+        for (uint32_t i = threadIdx.x; i < bytes; i += kT) out[opos + i] = (uint8_t)(sBits[i >> 2] >> (8 * (i & 3)));
+        if (threadIdx.x == 0 && x4 && k < 3) {
+            out[2 * k] = (uint8_t)bytes;
+            out[2 * k + 1] = (uint8_t)(bytes >> 8);
+        }
+        opos += bytes;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ws.plans[b].payload = opos;
+    (void)sOut;
+}
+
+// ------------------------------------------------------------------ flatpack / bitpack payloads
+__global__ void __launch_bounds__(kT) k_pack_encode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
+                                                    uint32_t B, ozl_ews ws)
+{
+    __shared__ uint8_t sMap[256];
+    uint32_t b = blockIdx.x;
+    if (b >= B) return;
+    ozl_eplan p = ws.plans[b];
+    if ((p.bk != OZL_BK_FLATPACK && p.bk != OZL_BK_BITPACK) || p.status != 0) return;
+    const uint8_t* in = src + off[b];
+    uint8_t* out = ws.payload + (size_t)b * ws.payloadCap;
+    const uint8_t* aux = ws.aux + (size_t)b * OZL_AUX_BYTES;
+    uint32_t n = p.n, nb;
+    bool flat = p.bk == OZL_BK_FLATPACK;
+    if (flat) {
+        for (int s = threadIdx.x; s < 256; s += kT) sMap[s] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < p.aux; i += kT) sMap[aux[i]] = (uint8_t)i;
+        nb = p.aux <= 1 ? p.aux : 1 + (uint32_t)ozl_hb32(p.aux - 1u);
+    } else {
+        for (int s = threadIdx.x; s < 256; s += kT) sMap[s] = (uint8_t)s;
+        nb = p.aux;
+    }
+    __syncthreads();
+    uint64_t totalBits = (uint64_t)n * nb;
+    uint32_t bytes = p.payload;
+    for (uint32_t o = threadIdx.x; o < bytes; o += kT) {
+        uint64_t bit0 = (uint64_t)o * 8, bit1 = bit0 + 8;
+        uint32_t v = 0;
+        if (nb && bit0 < totalBits) {
+            uint64_t e0 = bit0 / nb, e1 = ((bit1 < totalBits ? bit1 : totalBits) - 1) / nb;
+            for (uint64_t e = e0; e <= e1; e++) {
+                long long sh = (long long)(e * nb) - (long long)bit0;
+                uint32_t x = sMap[in[e]];
+                v |= sh >= 0 ? (x << sh) : (x >> (-sh));
+            }
+        }
+        if (flat && totalBits >= bit0 && totalBits < bit1) v |= 1u << (totalBits - bit0);  // terminating bit
+        out[o] = (uint8_t)v;
+    }
+}
+
+// ------------------------------------------------------------------ checksums and frames
+__global__ void __launch_bounds__(64) k_content_hash(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
+                                                     uint32_t B, ozl_ews ws)
+{
+    __shared__ uint64_t lds[8 * 128];
+    uint32_t b = blockIdx.x;
+    if (b >= B) return;
+    uint64_t h = ozl_xxh3_wave(src + off[b], off[b + 1] - off[b], lds, 128);
+    if (threadIdx.x == 0) ws.plans[b].contentHash = (uint32_t)h;
+}
+
+__device__ inline size_t stored_bytes(const ozl_fstream* st, const int* order, int nbStored)
+{
+    size_t t = 0;
+    for (int k = 0; k < nbStored; k++) t += st[order[k]].size;
+    return t;
+}
+
+__global__ void k_frame_size(uint32_t B, ozl_ews ws)
+{
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    ozl_eplan p = ws.plans[b];
+    if (p.status != 0) { ws.frameOff[b] = 0; return; }
+    ozl_fnode nodes[8];
+    ozl_fstream st[12];
+    int ns, order[12], nbStored;
+    int nn = ozl_entropy_layout(&p, nodes, st, &ns);
+    uint8_t hdr[512];
+    size_t fh = ozl_write_frame_header(hdr, OZL_T_SERIAL, p.n, 1, 1);
+    size_t ch = ozl_write_chunk_header(hdr, nodes, nn, st, ns, order, &nbStored);
+    ws.frameOff[b] = fh + ch + stored_bytes(st, order, nbStored) + 8 + 1;
+}
+
+// exclusive scan of B+1 entries (last = total) in one workgroup
+__global__ void __launch_bounds__(kT) k_scan_u64(uint64_t* v, uint32_t B)
+{
+    __shared__ uint64_t lds[4];
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < B; base += kT) {
+        uint32_t i = base + threadIdx.x;
+        uint64_t x = i < B ? v[i] : 0;
+        uint64_t total;
+        uint64_t ex = block_scan_excl<uint64_t>(x, lds, total);
+        if (i < B) v[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) v[B] = carry;
+}
+
+__global__ void __launch_bounds__(kT) k_frame_write(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
+                                                    uint32_t B, ozl_ews ws, uint8_t* __restrict__ dst)
+{
+    __shared__ uint8_t hdr[512];
+    __shared__ ozl_fstream st[12];
+    __shared__ int order[12];
+    __shared__ int sInfo[3];
+    uint32_t b = blockIdx.x;
+    if (b >= B) return;
+    ozl_eplan p = ws.plans[b];
+    if (p.status != 0) return;
+    if (threadIdx.x == 0) {
+        ozl_fnode nodes[8];
+        int ns, nbStored;
+        int nn = ozl_entropy_layout(&p, nodes, st, &ns);
+        size_t fh = ozl_write_frame_header(hdr, OZL_T_SERIAL, p.n, 1, 1);
+        size_t ch = ozl_write_chunk_header(hdr + fh, nodes, nn, st, ns, order, &nbStored);
+        sInfo[0] = (int)(fh + ch);
+        sInfo[1] = nbStored;
+        sInfo[2] = (int)fh;
+    }
+    __syncthreads();
+    uint8_t* out = dst + ws.frameOff[b];
+    int hlen = sInfo[0];
+    for (int i = threadIdx.x; i < hlen; i += kT) out[i] = hdr[i];
+    size_t pos = hlen;
+    const uint8_t* in = src + off[b];
+    const uint8_t* aux = ws.aux + (size_t)b * OZL_AUX_BYTES;
+    const uint8_t* pay = ws.payload + (size_t)b * ws.payloadCap;
+    for (int k = 0; k < sInfo[1]; k++) {
+        const ozl_fstream s = st[order[k]];
+        const uint8_t* from = s.src == OZL_SRC_INPUT ? in : s.src == OZL_SRC_PAYLOAD ? pay : aux + s.srcOff;
+        for (uint32_t i = threadIdx.x; i < s.size; i += kT) out[pos + i] = from[i];
+        pos += s.size;
+    }
+    if (threadIdx.x == 0) {
+        uint32_t h = p.contentHash;
+        for (int k = 0; k < 4; k++) out[pos + k] = (uint8_t)(h >> (8 * k));
+    }
+}
+
+// compressed checksum over [chunk start, end of content checksum) and the
+// end-of-frame byte; one wavefront per frame
+__global__ void __launch_bounds__(64) k_frame_checksum(uint32_t B, ozl_ews ws, uint8_t* __restrict__ dst, int* status)
+{
+    __shared__ uint64_t lds[8 * 128];
+    uint32_t b = blockIdx.x;
+    if (b >= B) return;
+    ozl_eplan p = ws.plans[b];
+    if (p.status != 0) { if (threadIdx.x == 0) status[b] = (int)p.status; return; }
+    if (threadIdx.x == 0) status[b] = 0;
+    uint8_t fhbuf[24];
+    size_t fh = ozl_write_frame_header(fhbuf, OZL_T_SERIAL, p.n, 1, 1);
+    uint8_t* fr = dst + ws.frameOff[b];
+    size_t flen = ws.frameOff[b + 1] - ws.frameOff[b];
+    size_t clen = flen - fh - 4 - 1;
+    uint64_t h = ozl_xxh3_wave(fr + fh, clen, lds, 128);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 4; k++) fr[fh + clen + k] = (uint8_t)(h >> (8 * k));
+        fr[flen - 1] = 0;
+    }
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+// Workspace bytes for ozl_entropy_compress_batch on B blocks of at most
+// maxBlock bytes.
+size_t ozl_entropy_compress_workspace(uint32_t B, size_t maxBlock)
+{
+    size_t cap = ((maxBlock + maxBlock / 128 + 256) + 15) & ~(size_t)15;
+    return (size_t)B * (sizeof(ozl_eplan) + OZL_TAB_BYTES + OZL_AUX_BYTES + cap) + (size_t)(B + 1) * 8 + 4096;
+}
+
+// Upper bound of one frame for a block of n bytes.
+size_t ozl_entropy_frame_bound(size_t n) { return n + n / 128 + 1024; }
+
+// Compresses B serial blocks (block b = src[off[b], off[b+1]), each below
+// 100000 bytes) into B frames with ZL_GRAPH_FSE (hufMode 0) or
+// ZL_GRAPH_HUFFMAN (hufMode 1), format version 23, both checksums on: the
+// frames the reference produces with ZL_CCtx_compressTypedRef on each block
+// (include/openzl/zl_compress.h:274). Frames are written back to back into
+// dst; frameOff (device, B+1 entries) receives their offsets and status
+// (device, B ints) 0 or the error code of each block (a failed block gets an
+// empty frame). All pointers are device pointers.
+int ozl_entropy_compress_batch(int hufMode, const void* src, const uint64_t* off, uint32_t B, size_t maxBlock,
+                               void* dst, size_t dstCapacity, uint64_t* frameOff, int* status, void* workspace,
+                               size_t wsSize, void* stream)
+{
+    if (B == 0) return OZL_OK;
+    if (maxBlock >= 100000) return OZL_ERR_PARAMETER_INVALID;
+    if (wsSize < ozl_entropy_compress_workspace(B, maxBlock)) return OZL_ERR_INTERNAL_BUFFER;
+    if (dstCapacity < (size_t)B * ozl_entropy_frame_bound(maxBlock)) return OZL_ERR_DSTCAPACITY_TOOSMALL;
+    hipStream_t st = (hipStream_t)stream;
+    uint8_t* w = (uint8_t*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
+    ozl_ews ws;
+    ws.payloadCap = ((maxBlock + maxBlock / 128 + 256) + 15) & ~(size_t)15;
+    ws.plans = (ozl_eplan*)w; w += (size_t)B * sizeof(ozl_eplan);
+    ws.tables = w; w += (size_t)B * OZL_TAB_BYTES;
+    ws.aux = w; w += (size_t)B * OZL_AUX_BYTES;
+    ws.payload = w; w += (size_t)B * ws.payloadCap;
+    ws.frameOff = frameOff;
+    const uint8_t* s = (const uint8_t*)src;
+    if (hufMode) OZL_LAUNCH("k_plan", k_plan<1>, dim3(B), dim3(kT), 0, st, s, off, B, ws);
+    else OZL_LAUNCH("k_plan", k_plan<0>, dim3(B), dim3(kT), 0, st, s, off, B, ws);
+    OZL_LAUNCH("k_content_hash", k_content_hash, dim3(B), dim3(64), 0, st, s, off, B, ws);
+    if (hufMode) OZL_LAUNCH("k_huf_encode", k_huf_encode, dim3(B), dim3(kT), 0, st, s, off, B, ws);
+    else OZL_LAUNCH("k_fse_encode", k_fse_encode, dim3((B + kFseGroups - 1) / kFseGroups), dim3(64), 0, st, s, off, B, ws);
+    OZL_LAUNCH("k_pack_encode", k_pack_encode, dim3(B), dim3(kT), 0, st, s, off, B, ws);
+    OZL_LAUNCH("k_frame_size", k_frame_size, dim3((B + 63) / 64), dim3(64), 0, st, B, ws);
+    OZL_LAUNCH("k_scan_u64", k_scan_u64, dim3(1), dim3(kT), 0, st, frameOff, B);
+    OZL_LAUNCH("k_frame_write", k_frame_write, dim3(B), dim3(kT), 0, st, s, off, B, ws, (uint8_t*)dst);
+    OZL_LAUNCH("k_frame_checksum", k_frame_checksum, dim3(B), dim3(64), 0, st, B, ws, (uint8_t*)dst, status);
+    OZL_HIP_CHECK(hipGetLastError());
+    return OZL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,41 @@
+// Internal helpers shared by the HIP translation units of libozl_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+// Error codes: the numeric values of the reference's ZL_ErrorCode
+// (include/openzl/zl_errors_types.h:41-99) so callers see the same codes.
+enum {
+    OZL_OK = 0,
+    OZL_ERR_GENERIC = 1,
+    OZL_ERR_SRCSIZE_TOOSMALL = 3,
+    OZL_ERR_DSTCAPACITY_TOOSMALL = 5,
+    OZL_ERR_HEADER_UNKNOWN = 10,
+    OZL_ERR_CORRUPTION = 12,
+    OZL_ERR_COMPRESSED_CHECKSUM = 13,
+    OZL_ERR_CONTENT_CHECKSUM = 14,
+    OZL_ERR_PARAMETER_INVALID = 21,
+    OZL_ERR_GRAPH_INVALID = 30,
+    OZL_ERR_INPUT_TYPE_UNSUPPORTED = 43,
+    OZL_ERR_NODE_INVALID_INPUT = 55,
+    OZL_ERR_FORMAT_VERSION_UNSUPPORTED = 60,
+    OZL_ERR_ALLOCATION = 70,
+    OZL_ERR_INTERNAL_BUFFER = 71,
+    OZL_ERR_LOGIC = 80,
+    OZL_ERR_HIP = 90,  // a HIP runtime call failed (no reference counterpart)
+};
+
+#define OZL_HIP_CHECK(expr)                                   \
+    do {                                                      \
+        hipError_t _e = (expr);                               \
+        if (_e != hipSuccess) return OZL_ERR_HIP;             \
+    } while (0)
+
+static inline unsigned ozl_blocks(size_t work, unsigned perBlock, unsigned cap = 1u << 20)
+{
+    size_t b = (work + perBlock - 1) / perBlock;
+    if (b == 0) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}

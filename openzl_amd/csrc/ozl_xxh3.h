@@ -1,0 +1,199 @@
+// ozl_xxh3.h -- XXH3-64 (seed 0, default secret) for host and device.
+//
+// The frame format stores XXH3-64 checksums (src/openzl/compress/cctx.c:
+// 1537-1580, encode_frameheader.c:624-630). Written from the published XXH3
+// algorithm; the 192-byte default secret is the specification's table.
+//   ozl_xxh3_seq   one lane, any length (headers, small streams)
+//   ozl_xxh3_wave  one 64-lane wavefront: the sums over the 16 stripes of every
+//                  1 KiB block are independent, so lanes compute them in
+//                  parallel; only the per-block scramble chain is sequential.
+#pragma once
+#include "ozl_core.h"
+
+OZL_CONST_TABLE uint8_t ozl_xxh3_secret[192] = {
+    0xb8, 0xfe, 0x6c, 0x39, 0x23, 0xa4, 0x4b, 0xbe, 0x7c, 0x01, 0x81, 0x2c, 0xf7, 0x21, 0xad, 0x1c,
+    0xde, 0xd4, 0x6d, 0xe9, 0x83, 0x90, 0x97, 0xdb, 0x72, 0x40, 0xa4, 0xa4, 0xb7, 0xb3, 0x67, 0x1f,
+    0xcb, 0x79, 0xe6, 0x4e, 0xcc, 0xc0, 0xe5, 0x78, 0x82, 0x5a, 0xd0, 0x7d, 0xcc, 0xff, 0x72, 0x21,
+    0xb8, 0x08, 0x46, 0x74, 0xf7, 0x43, 0x24, 0x8e, 0xe0, 0x35, 0x90, 0xe6, 0x81, 0x3a, 0x26, 0x4c,
+    0x3c, 0x28, 0x52, 0xbb, 0x91, 0xc3, 0x00, 0xcb, 0x88, 0xd0, 0x65, 0x8b, 0x1b, 0x53, 0x2e, 0xa3,
+    0x71, 0x64, 0x48, 0x97, 0xa2, 0x0d, 0xf9, 0x4e, 0x38, 0x19, 0xef, 0x46, 0xa9, 0xde, 0xac, 0xd8,
+    0xa8, 0xfa, 0x76, 0x3f, 0xe3, 0x9c, 0x34, 0x3f, 0xf9, 0xdc, 0xbb, 0xc7, 0xc7, 0x0b, 0x4f, 0x1d,
+    0x8a, 0x51, 0xe0, 0x4b, 0xcd, 0xb4, 0x59, 0x31, 0xc8, 0x9f, 0x7e, 0xc9, 0xd9, 0x78, 0x73, 0x64,
+    0xea, 0xc5, 0xac, 0x83, 0x34, 0xd3, 0xeb, 0xc3, 0xc5, 0x81, 0xa0, 0xff, 0xfa, 0x13, 0x63, 0xeb,
+    0x17, 0x0d, 0xdd, 0x51, 0xb7, 0xf0, 0xda, 0x49, 0xd3, 0x16, 0x55, 0x26, 0x29, 0xd4, 0x68, 0x9e,
+    0x2b, 0x16, 0xbe, 0x58, 0x7d, 0x47, 0xa1, 0xfc, 0x8f, 0xf8, 0xb8, 0xd1, 0x7a, 0xd0, 0x31, 0xce,
+    0x45, 0xcb, 0x3a, 0x8f, 0x95, 0x16, 0x04, 0x28, 0xaf, 0xd7, 0xfb, 0xca, 0xbb, 0x4b, 0x40, 0x7e,
+};
+
+#define OZL_P32_1 0x9E3779B1u
+#define OZL_P32_2 0x85EBCA77u
+#define OZL_P32_3 0xC2B2AE3Du
+#define OZL_P64_1 0x9E3779B185EBCA87ull
+#define OZL_P64_2 0xC2B2AE3D27D4EB4Full
+#define OZL_P64_3 0x165667B19E3779F9ull
+#define OZL_P64_4 0x85EBCA77C2B2AE63ull
+#define OZL_P64_5 0x27D4EB2F165667C5ull
+#define OZL_PMX1 0x165667919E3779F9ull
+#define OZL_PMX2 0x9FB21C651E98DF25ull
+
+// byte-wise little-endian reads: inputs may sit at any byte offset
+OZL_HD uint64_t ozl_le64(const uint8_t* p)
+{
+    uint64_t v = 0;
+    for (int k = 0; k < 8; k++) v |= (uint64_t)p[k] << (8 * k);
+    return v;
+}
+OZL_HD uint32_t ozl_le32(const uint8_t* p)
+{
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+OZL_HD uint64_t ozl_sec64(int off) { return ozl_le64(ozl_xxh3_secret + off); }
+OZL_HD uint64_t ozl_rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+OZL_HD uint64_t ozl_fold128(uint64_t a, uint64_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t lo = a * b, hi = __umul64hi(a, b);
+#else
+    unsigned __int128 m = (unsigned __int128)a * b;
+    uint64_t lo = (uint64_t)m, hi = (uint64_t)(m >> 64);
+#endif
+    return lo ^ hi;
+}
+OZL_HD uint64_t ozl_avalanche64(uint64_t h)
+{
+    h ^= h >> 33; h *= OZL_P64_2; h ^= h >> 29; h *= OZL_P64_3; h ^= h >> 32; return h;
+}
+OZL_HD uint64_t ozl_avalanche3(uint64_t h) { h ^= h >> 37; h *= OZL_PMX1; h ^= h >> 32; return h; }
+OZL_HD uint64_t ozl_mix16(const uint8_t* in, int so)
+{
+    return ozl_fold128(ozl_le64(in) ^ ozl_sec64(so), ozl_le64(in + 8) ^ ozl_sec64(so + 8));
+}
+OZL_HD uint64_t ozl_xxh3_short(const uint8_t* in, size_t len)
+{
+    if (len == 0) return ozl_avalanche64(ozl_sec64(56) ^ ozl_sec64(64));
+    if (len <= 3) {
+        uint32_t c1 = in[0], c2 = in[len >> 1], c3 = in[len - 1];
+        uint32_t comb = (c1 << 16) | (c2 << 24) | c3 | ((uint32_t)len << 8);
+        uint64_t flip = (uint64_t)(ozl_le32(ozl_xxh3_secret) ^ ozl_le32(ozl_xxh3_secret + 4));
+        return ozl_avalanche64((uint64_t)comb ^ flip);
+    }
+    if (len <= 8) {
+        uint32_t i1 = ozl_le32(in), i2 = ozl_le32(in + len - 4);
+        uint64_t flip = ozl_sec64(8) ^ ozl_sec64(16);
+        uint64_t h = (i2 + ((uint64_t)i1 << 32)) ^ flip;
+        h ^= ozl_rotl64(h, 49) ^ ozl_rotl64(h, 24);
+        h *= OZL_PMX2;
+        h ^= (h >> 35) + len;
+        h *= OZL_PMX2;
+        h ^= h >> 28;
+        return h;
+    }
+    if (len <= 16) {
+        uint64_t lo = ozl_le64(in) ^ (ozl_sec64(24) ^ ozl_sec64(32));
+        uint64_t hi = ozl_le64(in + len - 8) ^ (ozl_sec64(40) ^ ozl_sec64(48));
+        uint64_t acc = len + __builtin_bswap64(lo) + hi + ozl_fold128(lo, hi);
+        return ozl_avalanche3(acc);
+    }
+    if (len <= 128) {
+        uint64_t acc = len * OZL_P64_1;
+        if (len > 32) {
+            if (len > 64) {
+                if (len > 96) { acc += ozl_mix16(in + 48, 96); acc += ozl_mix16(in + len - 64, 112); }
+                acc += ozl_mix16(in + 32, 64); acc += ozl_mix16(in + len - 48, 80);
+            }
+            acc += ozl_mix16(in + 16, 32); acc += ozl_mix16(in + len - 32, 48);
+        }
+        acc += ozl_mix16(in, 0); acc += ozl_mix16(in + len - 16, 16);
+        return ozl_avalanche3(acc);
+    }
+    // 129..240
+    uint64_t acc = len * OZL_P64_1;
+    size_t rounds = len / 16;
+    for (size_t i = 0; i < 8; i++) acc += ozl_mix16(in + 16 * i, 16 * (int)i);
+    acc = ozl_avalanche3(acc);
+    for (size_t i = 8; i < rounds; i++) acc += ozl_mix16(in + 16 * i, 16 * (int)(i - 8) + 3);
+    acc += ozl_mix16(in + len - 16, 136 - 17);
+    return ozl_avalanche3(acc);
+}
+
+// contribution of one 64-byte stripe to accumulator lane i
+OZL_HD uint64_t ozl_stripe_term(const uint8_t* stripe, int secOff, int i)
+{
+    uint64_t dOther = ozl_le64(stripe + 8 * (i ^ 1));
+    uint64_t d = ozl_le64(stripe + 8 * i);
+    uint64_t k = d ^ ozl_sec64(secOff + 8 * i);
+    return dOther + (uint64_t)(uint32_t)k * (k >> 32);
+}
+OZL_HD uint64_t ozl_scramble(uint64_t a, int i)
+{
+    a ^= a >> 47;
+    a ^= ozl_sec64(192 - 64 + 8 * i);
+    a *= OZL_P32_1;
+    return a;
+}
+OZL_HD uint64_t ozl_xxh3_merge(const uint64_t* acc, size_t len)
+{
+    uint64_t res = len * OZL_P64_1;
+    for (int i = 0; i < 4; i++)
+        res += ozl_fold128(acc[2 * i] ^ ozl_sec64(11 + 16 * i), acc[2 * i + 1] ^ ozl_sec64(11 + 16 * i + 8));
+    return ozl_avalanche3(res);
+}
+
+OZL_HD uint64_t ozl_xxh3_seq(const uint8_t* in, size_t len)
+{
+    if (len <= 240) return ozl_xxh3_short(in, len);
+    uint64_t acc[8] = { OZL_P32_3, OZL_P64_1, OZL_P64_2, OZL_P64_3, OZL_P64_4, OZL_P32_2, OZL_P64_5, OZL_P32_1 };
+    size_t nbBlocks = (len - 1) / 1024;
+    for (size_t b = 0; b < nbBlocks; b++) {
+        for (int s = 0; s < 16; s++)
+            for (int i = 0; i < 8; i++) acc[i] += ozl_stripe_term(in + b * 1024 + s * 64, s * 8, i);
+        for (int i = 0; i < 8; i++) acc[i] = ozl_scramble(acc[i], i);
+    }
+    size_t nbStripes = ((len - 1) - 1024 * nbBlocks) / 64;
+    for (size_t s = 0; s < nbStripes; s++)
+        for (int i = 0; i < 8; i++) acc[i] += ozl_stripe_term(in + nbBlocks * 1024 + s * 64, (int)s * 8, i);
+    for (int i = 0; i < 8; i++) acc[i] += ozl_stripe_term(in + len - 64, 192 - 64 - 7, i);
+    return ozl_xxh3_merge(acc, len);
+}
+
+#if defined(__HIPCC__)
+// One wavefront hashes `len` bytes at `in`; every lane returns the hash.
+// Lane l handles blocks b = l, l+64, ...: it sums the 16 stripe terms of each
+// of its blocks for all 8 accumulators; then the 8 lanes < 8 run the scramble
+// chain over the blocks in order, reading the block sums from `lds`
+// (capacity: 8 * min(nbBlocks, chainCap) u64), in rounds of chainCap blocks.
+__device__ inline uint64_t ozl_xxh3_wave(const uint8_t* in, size_t len, uint64_t* lds, int chainCap)
+{
+    const int lane = threadIdx.x & 63;
+    if (len <= 240) return ozl_xxh3_short(in, len);
+    uint64_t acc = 0;  // accumulator lane (lane < 8)
+    const uint64_t init[8] = { OZL_P32_3, OZL_P64_1, OZL_P64_2, OZL_P64_3, OZL_P64_4, OZL_P32_2, OZL_P64_5, OZL_P32_1 };
+    if (lane < 8) acc = init[lane];
+    size_t nbBlocks = (len - 1) / 1024;
+    for (size_t b0 = 0; b0 < nbBlocks; b0 += chainCap) {
+        size_t nb = nbBlocks - b0 < (size_t)chainCap ? nbBlocks - b0 : (size_t)chainCap;
+        for (size_t bb = lane; bb < nb; bb += 64) {
+            const uint8_t* blk = in + (b0 + bb) * 1024;
+            uint64_t sum[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+            for (int s = 0; s < 16; s++)
+                for (int i = 0; i < 8; i++) sum[i] += ozl_stripe_term(blk + s * 64, s * 8, i);
+            for (int i = 0; i < 8; i++) lds[bb * 8 + i] = sum[i];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane < 8)
+            for (size_t bb = 0; bb < nb; bb++) acc = ozl_scramble(acc + lds[bb * 8 + lane], lane);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    // tail stripes and the last stripe: lanes < 8 each finish their accumulator
+    if (lane < 8) {
+        size_t nbStripes = ((len - 1) - 1024 * nbBlocks) / 64;
+        for (size_t s = 0; s < nbStripes; s++) acc += ozl_stripe_term(in + nbBlocks * 1024 + s * 64, (int)s * 8, lane);
+        acc += ozl_stripe_term(in + len - 64, 192 - 64 - 7, lane);
+    }
+    uint64_t a[8];
+    for (int i = 0; i < 8; i++) a[i] = __shfl(acc, i, 64);
+    return ozl_xxh3_merge(a, len);
+}
+#endif

@@ -1,0 +1,64 @@
+// CPU unit build of openzl_amd/csrc/ozl_core.h (test-only). The same source
+// that the HIP kernels run is compiled for the host so its table builders and
+// decisions can be checked against the oracle without a GPU.
+#include "ozl_core.h"
+
+extern "C" {
+int cu_decide(const uint8_t* src, size_t n, int hufMode)
+{
+    uint32_t count[256] = { 0 };
+    for (size_t i = 0; i < n; i++) count[src[i]]++;
+    unsigned ms, card;
+    return ozl_decide_from_hist(count, n, hufMode, &ms, &card);
+}
+double cu_entropy(const uint32_t* count, unsigned maxValue, size_t total) { return ozl_entropy(count, maxValue, total); }
+int cu_fse_normalize(int16_t* norm, unsigned tl, const uint32_t* count, size_t total, unsigned maxSym)
+{
+    return ozl_fse_normalize(norm, tl, count, total, maxSym);
+}
+unsigned cu_fse_optimal_tablelog(unsigned maxTL, size_t n, unsigned maxSym, unsigned minus)
+{
+    return ozl_fse_optimal_tablelog(maxTL, n, maxSym, minus);
+}
+int cu_fse_build_ctable(uint16_t* st, uint32_t* tt, const int16_t* norm, unsigned maxSym, unsigned tl)
+{
+    uint8_t cells[4096];
+    return ozl_fse_build_ctable(st, tt, cells, norm, maxSym, tl);
+}
+size_t cu_fse_encode(uint8_t* dst, size_t cap, const uint8_t* src, size_t n, const uint16_t* st, const uint32_t* tt,
+                     unsigned tl, unsigned nbStates)
+{
+    return ozl_fse_encode_seq(dst, cap, src, n, st, tt, tl, nbStates);
+}
+int cu_fse_decode(uint8_t* dst, size_t n, const uint8_t* src, size_t size, const int16_t* norm, unsigned maxSym,
+                  unsigned tl, unsigned nbStates)
+{
+    static uint32_t dt[4096];
+    uint8_t cells[4096];
+    if (ozl_fse_build_dtable(dt, cells, norm, maxSym, tl)) return -2;
+    return ozl_fse_decode_seq(dst, n, src, size, dt, tl, nbStates);
+}
+size_t cu_fse_write_ncount(uint8_t* dst, const int16_t* norm, unsigned maxSym, unsigned tl)
+{
+    return ozl_fse_write_ncount(dst, norm, maxSym, tl);
+}
+long cu_fse_read_ncount(int16_t* norm, unsigned* ms, unsigned* tl, const uint8_t* src, size_t size)
+{
+    return ozl_fse_read_ncount(norm, ms, tl, src, size);
+}
+int cu_huf_build(uint8_t* nb, uint16_t* val, const uint32_t* count, unsigned maxSym, unsigned maxNbBits)
+{
+    ozl_hnode tbl[2 * 256 + 2];
+    return ozl_huf_build(nb, val, count, maxSym, maxNbBits, tbl);
+}
+int cu_huf_from_weights(uint8_t* len, uint16_t* val, const uint8_t* w, unsigned nbW)
+{
+    return ozl_huf_from_weights(len, val, w, nbW);
+}
+size_t cu_flatpack(uint8_t* alphabet, uint8_t* packed, size_t* packedSize, const uint8_t* src, size_t n)
+{
+    uint32_t count[256] = { 0 };
+    for (size_t i = 0; i < n; i++) count[src[i]]++;
+    return ozl_flatpack_encode_seq(alphabet, packed, packedSize, src, n, count);
+}
+}
