@@ -1,0 +1,257 @@
+/*
+ * openzl_amd.h -- C ABI of libozl_hip.so, the MI355X (gfx950) implementation
+ * of OpenZL's data-parallel codec hot path.
+ *
+ * Two layers:
+ *
+ * 1. The reference's own public API, same names, same argument meaning, same
+ *    numeric node / graph / transform IDs and the same ZL_Report layout, for
+ *    the graphs of the hot path (ZL_NODE_DELTA_INT, ZL_NODE_ZIGZAG,
+ *    ZL_NODE_TRANSPOSE_SPLIT, ZL_GRAPH_BITPACK, ZL_GRAPH_FSE, ZL_GRAPH_HUFFMAN,
+ *    ZL_GRAPH_STORE, ZL_GRAPH_CONSTANT, ZL_GRAPH_FLATPACK, function graphs and
+ *    the brute-force selector). A program written against include/openzl/*.h
+ *    for these graphs links against this library instead; the codec work runs
+ *    on the GPU and frames are byte-identical to the reference's. Each
+ *    declaration cites the reference declaration it replaces.
+ *
+ * 2. Device-resident batch entry points (ozl_*): device pointers in and out,
+ *    many independent blocks per call. They have no counterpart in the
+ *    reference (whose contexts are single-threaded, src/openzl/compress/
+ *    cctx.c:1160-1168); a binding reaches them the same way as layer 1.
+ *
+ * All functions are thread-compatible: one ZL_CCtx / ZL_DCtx per thread.
+ */
+#ifndef OPENZL_AMD_H
+#define OPENZL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ errors */
+/* include/openzl/zl_errors_types.h:41-99 (subset, same values) */
+typedef enum {
+    ZL_ErrorCode_no_error = 0,
+    ZL_ErrorCode_GENERIC = 1,
+    ZL_ErrorCode_srcSize_tooSmall = 3,
+    ZL_ErrorCode_srcSize_tooLarge = 4,
+    ZL_ErrorCode_dstCapacity_tooSmall = 5,
+    ZL_ErrorCode_header_unknown = 10,
+    ZL_ErrorCode_frameParameter_unsupported = 11,
+    ZL_ErrorCode_corruption = 12,
+    ZL_ErrorCode_compressedChecksumWrong = 13,
+    ZL_ErrorCode_contentChecksumWrong = 14,
+    ZL_ErrorCode_parameter_invalid = 21,
+    ZL_ErrorCode_graph_invalid = 30,
+    ZL_ErrorCode_graph_invalidNumInputs = 33,
+    ZL_ErrorCode_successor_invalid = 40,
+    ZL_ErrorCode_inputType_unsupported = 43,
+    ZL_ErrorCode_node_invalid_input = 55,
+    ZL_ErrorCode_node_invalid = 56,
+    ZL_ErrorCode_formatVersion_unsupported = 60,
+    ZL_ErrorCode_formatVersion_notSet = 61,
+    ZL_ErrorCode_allocation = 70,
+    ZL_ErrorCode_internalBuffer_tooSmall = 71,
+    ZL_ErrorCode_logicError = 80,
+    ZL_ErrorCode_temporaryLibraryLimitation = 81,
+} ZL_ErrorCode;
+
+/* include/openzl/zl_errors.h:206-257: ZL_Report is a 16-byte union whose
+ * first member is the error code; on success the value follows. */
+typedef union {
+    ZL_ErrorCode _code;
+    struct {
+        ZL_ErrorCode _code;
+        size_t _value;
+    } _value;
+    struct {
+        ZL_ErrorCode _code;
+        void* _info;
+    } _error;
+} ZL_Report;
+
+static inline int ZL_isError(ZL_Report r) { return r._code != ZL_ErrorCode_no_error; }
+static inline size_t ZL_validResult(ZL_Report r) { return r._value._value; }
+static inline ZL_ErrorCode ZL_errorCode(ZL_Report r) { return r._code; }
+/* include/openzl/zl_errors.h: ZL_ErrorCode_toString */
+const char* ZL_ErrorCode_toString(ZL_ErrorCode code);
+
+/* ------------------------------------------------------------ types and IDs */
+/* include/openzl/zl_data.h: ZL_Type bit flags */
+typedef enum { ZL_Type_serial = 1, ZL_Type_struct = 2, ZL_Type_numeric = 4, ZL_Type_string = 8 } ZL_Type;
+
+typedef struct { unsigned nid; } ZL_NodeID;   /* include/openzl/zl_opaque_types.h */
+typedef struct { unsigned gid; } ZL_GraphID;
+
+/* include/openzl/zl_nodes.h:13-88 */
+#define ZL_NODE_ILLEGAL ((ZL_NodeID){ 0 })
+#define ZL_NODE_DELTA_INT ((ZL_NodeID){ 2 })
+#define ZL_NODE_TRANSPOSE_SPLIT ((ZL_NodeID){ 3 })
+#define ZL_NODE_ZIGZAG ((ZL_NodeID){ 4 })
+#define ZL_NODE_CONVERT_STRUCT_TO_SERIAL ((ZL_NodeID){ 10 })
+#define ZL_NODE_CONVERT_NUM_TO_STRUCT_LE ((ZL_NodeID){ 11 })
+#define ZL_NODE_CONVERT_NUM_TO_SERIAL_LE ((ZL_NodeID){ 12 })
+
+/* include/openzl/zl_graphs.h:11-42 */
+#define ZL_GRAPH_ILLEGAL ((ZL_GraphID){ 0 })
+#define ZL_GRAPH_STORE ((ZL_GraphID){ 2 })
+#define ZL_GRAPH_FSE ((ZL_GraphID){ 3 })
+#define ZL_GRAPH_HUFFMAN ((ZL_GraphID){ 4 })
+#define ZL_GRAPH_CONSTANT ((ZL_GraphID){ 6 })
+#define ZL_GRAPH_BITPACK ((ZL_GraphID){ 8 })
+#define ZL_GRAPH_FLATPACK ((ZL_GraphID){ 9 })
+
+/* include/openzl/zl_compress.h:64-131 (subset) */
+typedef enum {
+    ZL_CParam_compressionLevel = 2,
+    ZL_CParam_formatVersion = 4,
+    ZL_CParam_compressedChecksum = 6,
+    ZL_CParam_contentChecksum = 7,
+    ZL_CParam_minStreamSize = 11,
+} ZL_CParam;
+typedef enum { ZL_TernaryParam_auto = 0, ZL_TernaryParam_enable = 1, ZL_TernaryParam_disable = 2 } ZL_TernaryParam;
+#define ZL_MAX_FORMAT_VERSION 23 /* include/openzl/zl_version.h:57 */
+
+typedef struct ZL_Compressor_s ZL_Compressor;
+typedef struct ZL_CCtx_s ZL_CCtx;
+typedef struct ZL_DCtx_s ZL_DCtx;
+typedef struct ZL_TypedRef_s ZL_TypedRef;
+typedef struct ZL_Graph_s ZL_Graph;
+typedef struct ZL_Edge_s ZL_Edge;
+typedef struct ZL_Input_s ZL_Input;
+
+/* ------------------------------------------------------------ compressor */
+/* include/openzl/zl_compressor.h:69 */
+ZL_Compressor* ZL_Compressor_create(void);
+void ZL_Compressor_free(ZL_Compressor* compressor);
+/* include/openzl/zl_compressor.h:151 */
+ZL_Report ZL_Compressor_setParameter(ZL_Compressor* compressor, ZL_CParam param, int value);
+/* include/openzl/zl_compressor.h:232 */
+ZL_GraphID ZL_Compressor_registerStaticGraph_fromNode1o(ZL_Compressor* compressor, ZL_NodeID headNode,
+                                                        ZL_GraphID dstGraph);
+/* include/openzl/zl_compressor.h:281 */
+ZL_GraphID ZL_Compressor_registerStaticGraph_fromNode(ZL_Compressor* compressor, ZL_NodeID headNode,
+                                                      const ZL_GraphID* dstGraphs, size_t nbDstGraphs);
+/* include/openzl/codecs/zl_transpose.h:43 */
+ZL_GraphID ZL_Compressor_registerTransposeSplitGraph(ZL_Compressor* compressor, ZL_GraphID successor);
+/* src/openzl/compress/selectors/selector_brute_force.c:52-66 (declared in
+ * include/openzl/zl_compressor.h) */
+ZL_GraphID ZL_Compressor_registerBruteForceSelectorGraph(ZL_Compressor* compressor, const ZL_GraphID* successors,
+                                                         size_t numSuccessors);
+/* include/openzl/zl_compressor.h:555 */
+ZL_Report ZL_Compressor_selectStartingGraphID(ZL_Compressor* compressor, ZL_GraphID graph);
+
+/* Function graphs: include/openzl/zl_graph_api.h:58-120 (subset). */
+typedef ZL_Report (*ZL_FunctionGraphFn)(ZL_Graph* graph, ZL_Edge* inputs[], size_t nbInputs);
+typedef struct {
+    const char* name;
+    ZL_FunctionGraphFn graph_f;
+    void* validate_f;                 /* unused */
+    const ZL_Type* inputTypeMasks;
+    size_t nbInputs;
+    int lastInputIsVariable;
+    const ZL_GraphID* customGraphs;
+    size_t nbCustomGraphs;
+    const ZL_NodeID* customNodes;
+    size_t nbCustomNodes;
+    char localParams[48];             /* layout placeholder, unused */
+    char opaque[24];                  /* layout placeholder, unused */
+} ZL_FunctionGraphDesc;
+ZL_GraphID ZL_Compressor_registerFunctionGraph(ZL_Compressor* compressor, const ZL_FunctionGraphDesc* desc);
+typedef struct {
+    ZL_Edge** edges;
+    size_t nbEdges;
+} ZL_EdgeList;
+typedef union {
+    ZL_ErrorCode _code;
+    struct {
+        ZL_ErrorCode _code;
+        ZL_EdgeList _value;
+    } _value;
+} ZL_Result_ZL_EdgeList;
+/* include/openzl/zl_graph_api.h: runtime edge operations */
+ZL_Result_ZL_EdgeList ZL_Edge_runNode(ZL_Edge* input, ZL_NodeID node);
+ZL_Result_ZL_EdgeList ZL_Edge_runTransposeSplit(ZL_Edge* edge, const ZL_Graph* graph);
+ZL_Report ZL_Edge_setDestination(ZL_Edge* edge, ZL_GraphID destination);
+const ZL_Input* ZL_Edge_getData(const ZL_Edge* edge);
+size_t ZL_Input_numElts(const ZL_Input* input);
+size_t ZL_Input_eltWidth(const ZL_Input* input);
+size_t ZL_Input_contentSize(const ZL_Input* input);
+ZL_Type ZL_Input_type(const ZL_Input* input);
+
+/* ------------------------------------------------------------ compression */
+/* include/openzl/zl_compress.h:50 */
+size_t ZL_compressBound(size_t totalSrcSize);
+/* include/openzl/zl_compress.h / zl_compressor.h:597 */
+ZL_CCtx* ZL_CCtx_create(void);
+void ZL_CCtx_free(ZL_CCtx* cctx);
+ZL_Report ZL_CCtx_refCompressor(ZL_CCtx* cctx, const ZL_Compressor* compressor);
+ZL_Report ZL_CCtx_setParameter(ZL_CCtx* cctx, ZL_CParam param, int value);
+/* include/openzl/zl_compress.h:313-349 */
+ZL_TypedRef* ZL_TypedRef_createSerial(const void* src, size_t srcSize);
+ZL_TypedRef* ZL_TypedRef_createStruct(const void* start, size_t structWidth, size_t structCount);
+ZL_TypedRef* ZL_TypedRef_createNumeric(const void* start, size_t numWidth, size_t numCount);
+void ZL_TypedRef_free(ZL_TypedRef* tref);
+/* include/openzl/zl_compress.h:274 */
+ZL_Report ZL_CCtx_compressTypedRef(ZL_CCtx* cctx, void* dst, size_t dstCapacity, const ZL_TypedRef* input);
+/* include/openzl/zl_compress.h: serial convenience */
+ZL_Report ZL_CCtx_compress(ZL_CCtx* cctx, void* dst, size_t dstCapacity, const void* src, size_t srcSize);
+
+/* ------------------------------------------------------------ decompression */
+/* include/openzl/zl_decompress.h:355-366 */
+typedef struct {
+    ZL_Type type;
+    uint32_t fixedWidth;
+    uint64_t decompressedByteSize;
+    uint64_t numElts;
+} ZL_OutputInfo;
+/* include/openzl/zl_decompress.h:32 */
+ZL_Report ZL_decompress(void* dst, size_t dstCapacity, const void* src, size_t srcSize);
+/* include/openzl/zl_decompress.h: ZL_getDecompressedSize */
+ZL_Report ZL_getDecompressedSize(const void* compressed, size_t cSize);
+ZL_DCtx* ZL_DCtx_create(void);
+void ZL_DCtx_free(ZL_DCtx* dctx);
+/* include/openzl/zl_decompress.h:388 */
+ZL_Report ZL_DCtx_decompressTyped(ZL_DCtx* dctx, ZL_OutputInfo* outputInfo, void* dst, size_t dstByteCapacity,
+                                  const void* compressed, size_t cSize);
+
+/* ------------------------------------------------------------ device batch entry points */
+/* Numeric transforms on device buffers (no reference counterpart: the
+ * reference kernels ZS_deltaEncode etc. take host pointers,
+ * src/openzl/codecs/delta/encode_delta_kernel.c:80). `stream` is a hipStream_t
+ * (0 = default). Return 0 or a ZL_ErrorCode. */
+int ozl_delta_encode(const void* src, size_t n, int width, void* deltas, void* first, void* stream);
+size_t ozl_delta_decode_scratch_size(size_t n, int width);
+int ozl_delta_decode(const void* first, const void* deltas, size_t n, int width, void* dst, void* scratch,
+                     void* stream);
+int ozl_zigzag_encode(const void* src, size_t n, int width, void* dst, void* stream);
+int ozl_zigzag_decode(const void* src, size_t n, int width, void* dst, void* stream);
+int ozl_transpose_split(const void* src, size_t n, int width, void* planes, void* stream);
+int ozl_transpose_join(const void* planes, size_t n, int width, void* dst, void* stream);
+int ozl_max_async(const void* src, size_t n, int width, unsigned long long* dmax, void* stream);
+int ozl_bitpack_encode(const void* src, size_t n, int width, int nbBits, void* dst, void* stream);
+int ozl_bitpack_decode(const void* src, size_t n, int width, int nbBits, void* dst, void* stream);
+
+/* Batched ZL_GRAPH_FSE (hufMode 0) / ZL_GRAPH_HUFFMAN (hufMode 1) frames, one
+ * per block of < 100000 bytes. */
+size_t ozl_entropy_compress_workspace(uint32_t nbBlocks, size_t maxBlock);
+size_t ozl_entropy_frame_bound(size_t blockSize);
+int ozl_entropy_compress_batch(int hufMode, const void* src, const uint64_t* offsets, uint32_t nbBlocks,
+                               size_t maxBlock, void* dst, size_t dstCapacity, uint64_t* frameOffsets,
+                               int* status, void* workspace, size_t workspaceSize, void* stream);
+size_t ozl_entropy_decompress_workspace(uint32_t nbBlocks);
+int ozl_entropy_decompress_batch(const void* frames, const uint64_t* frameOffsets, uint32_t nbFrames, void* dst,
+                                 const uint64_t* dstOffsets, int* status, void* workspace, size_t workspaceSize,
+                                 void* stream);
+
+/* Per-kernel timing (HIP events around every launch while enabled). */
+void ozl_profile_begin(void);
+size_t ozl_profile_end(char* out, size_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

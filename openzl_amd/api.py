@@ -1,0 +1,208 @@
+"""Python view of the ZL_* C API of libozl_hip.so (the same functions a C
+program calls). Builds the graphs of the benchmark configurations exactly as a
+reference user would (include/openzl/zl_compressor.h), compresses host
+buffers and decompresses frames; every codec runs on the GPU."""
+import ctypes
+
+import numpy as np
+
+from ._lib import lib
+
+P = ctypes.c_void_p
+SZ = ctypes.c_size_t
+
+
+class ZLReport(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_int), ("value", ctypes.c_size_t)]
+
+
+class GraphID(ctypes.Structure):
+    _fields_ = [("gid", ctypes.c_uint)]
+
+
+class NodeID(ctypes.Structure):
+    _fields_ = [("nid", ctypes.c_uint)]
+
+
+class OutputInfo(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("fixedWidth", ctypes.c_uint32), ("decompressedByteSize", ctypes.c_uint64),
+                ("numElts", ctypes.c_uint64)]
+
+
+SERIAL, STRUCT, NUMERIC = 1, 2, 4
+G_STORE, G_FSE, G_HUFFMAN, G_BITPACK = GraphID(2), GraphID(3), GraphID(4), GraphID(8)
+N_DELTA, N_TRANSPOSE_SPLIT, N_ZIGZAG = NodeID(2), NodeID(3), NodeID(4)
+
+# graph kinds, same numbering as the reference harness (oracle/ref_harness.c)
+STORE, FSE, HUFFMAN, BITPACK, DELTA_TS_FSE, DELTA_ZZ_BP, TS_HUF_FSE, DELTA_FSE, TS_FSE, TS_HUF, ZZ_BP, DELTA_BP = range(12)
+
+_setup = False
+
+
+def _api():
+    global _setup
+    L = lib()
+    if _setup:
+        return L
+    sig = {
+        "ZL_Compressor_create": (P, []),
+        "ZL_Compressor_free": (None, [P]),
+        "ZL_Compressor_setParameter": (ZLReport, [P, ctypes.c_int, ctypes.c_int]),
+        "ZL_Compressor_registerStaticGraph_fromNode1o": (GraphID, [P, NodeID, GraphID]),
+        "ZL_Compressor_registerTransposeSplitGraph": (GraphID, [P, GraphID]),
+        "ZL_Compressor_registerBruteForceSelectorGraph": (GraphID, [P, ctypes.POINTER(GraphID), SZ]),
+        "ZL_Compressor_selectStartingGraphID": (ZLReport, [P, GraphID]),
+        "ZL_CCtx_create": (P, []),
+        "ZL_CCtx_free": (None, [P]),
+        "ZL_CCtx_refCompressor": (ZLReport, [P, P]),
+        "ZL_TypedRef_createSerial": (P, [P, SZ]),
+        "ZL_TypedRef_createStruct": (P, [P, SZ, SZ]),
+        "ZL_TypedRef_createNumeric": (P, [P, SZ, SZ]),
+        "ZL_TypedRef_free": (None, [P]),
+        "ZL_CCtx_compressTypedRef": (ZLReport, [P, P, SZ, P]),
+        "ZL_compressBound": (SZ, [SZ]),
+        "ZL_DCtx_create": (P, []),
+        "ZL_DCtx_free": (None, [P]),
+        "ZL_DCtx_decompressTyped": (ZLReport, [P, ctypes.POINTER(OutputInfo), P, SZ, P, SZ]),
+        "ZL_ErrorCode_toString": (ctypes.c_char_p, [ctypes.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _setup = True
+    return L
+
+
+class ZLError(RuntimeError):
+    def __init__(self, code, what):
+        L = _api()
+        super().__init__(f"{what}: {L.ZL_ErrorCode_toString(code).decode()} ({code})")
+        self.code = code
+
+
+def _chk(r, what):
+    if r.code:
+        raise ZLError(r.code, what)
+    return r.value
+
+
+_graphs = None
+
+
+def _graphs_lib():
+    """libozl_graphs.so: the user graphs written in C against the public API
+    (openzl_amd/graphs/ozl_graphs.c)."""
+    global _graphs
+    if _graphs is None:
+        import os
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libozl_graphs.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built: run openzl_amd/build.py")
+        _api()
+        g = ctypes.CDLL(path)
+        g.ozl_graph_weights_split.restype = GraphID
+        g.ozl_graph_weights_split.argtypes = [P]
+        _graphs = g
+    return _graphs
+
+
+def build_graph(L, c, kind):
+    if kind == STORE:
+        return G_STORE
+    if kind == FSE:
+        return G_FSE
+    if kind == HUFFMAN:
+        return G_HUFFMAN
+    if kind == BITPACK:
+        return G_BITPACK
+    if kind == DELTA_TS_FSE:
+        ts = L.ZL_Compressor_registerTransposeSplitGraph(c, G_FSE)
+        return L.ZL_Compressor_registerStaticGraph_fromNode1o(c, N_DELTA, ts)
+    if kind == DELTA_ZZ_BP:
+        zz = L.ZL_Compressor_registerStaticGraph_fromNode1o(c, N_ZIGZAG, G_BITPACK)
+        return L.ZL_Compressor_registerStaticGraph_fromNode1o(c, N_DELTA, zz)
+    if kind == TS_HUF_FSE:
+        # transpose_split, plane 0 -> FSE, plane 1 -> Huffman (a C function graph)
+        return _graphs_lib().ozl_graph_weights_split(c)
+    if kind == DELTA_FSE:
+        return L.ZL_Compressor_registerStaticGraph_fromNode1o(c, N_DELTA, G_FSE)
+    if kind == TS_FSE:
+        return L.ZL_Compressor_registerTransposeSplitGraph(c, G_FSE)
+    if kind == TS_HUF:
+        return L.ZL_Compressor_registerTransposeSplitGraph(c, G_HUFFMAN)
+    if kind == ZZ_BP:
+        return L.ZL_Compressor_registerStaticGraph_fromNode1o(c, N_ZIGZAG, G_BITPACK)
+    if kind == DELTA_BP:
+        return L.ZL_Compressor_registerStaticGraph_fromNode1o(c, N_DELTA, G_BITPACK)
+    raise ValueError(kind)
+
+
+class Compressor:
+    """A ZL_Compressor + ZL_CCtx pair for one graph kind (format version 23)."""
+
+    def __init__(self, kind, candidates=None, content_checksum=True, compressed_checksum=True):
+        L = _api()
+        self.L = L
+        self.c = L.ZL_Compressor_create()
+        _chk(L.ZL_Compressor_setParameter(self.c, 4, 23), "formatVersion")
+        if not content_checksum:
+            _chk(L.ZL_Compressor_setParameter(self.c, 7, 2), "contentChecksum")
+        if not compressed_checksum:
+            _chk(L.ZL_Compressor_setParameter(self.c, 6, 2), "compressedChecksum")
+        if candidates is not None:
+            gids = (GraphID * len(candidates))(*[build_graph(L, self.c, k) for k in candidates])
+            g = L.ZL_Compressor_registerBruteForceSelectorGraph(self.c, gids, len(candidates))
+        else:
+            g = build_graph(L, self.c, kind)
+        _chk(L.ZL_Compressor_selectStartingGraphID(self.c, g), "selectStartingGraphID")
+        self.cctx = L.ZL_CCtx_create()
+        _chk(L.ZL_CCtx_refCompressor(self.cctx, self.c), "refCompressor")
+
+    def compress(self, arr, typ, width):
+        L = self.L
+        arr = np.ascontiguousarray(arr)
+        nb = arr.nbytes
+        n = nb // width
+        if typ == SERIAL:
+            tr = L.ZL_TypedRef_createSerial(arr.ctypes.data, nb)
+        elif typ == STRUCT:
+            tr = L.ZL_TypedRef_createStruct(arr.ctypes.data, width, n)
+        else:
+            tr = L.ZL_TypedRef_createNumeric(arr.ctypes.data, width, n)
+        cap = L.ZL_compressBound(nb)
+        out = np.empty(cap, np.uint8)
+        try:
+            size = _chk(L.ZL_CCtx_compressTypedRef(self.cctx, out.ctypes.data, cap, tr), "compress")
+        finally:
+            L.ZL_TypedRef_free(tr)
+        return out[:size].tobytes()
+
+    def __del__(self):
+        try:
+            self.L.ZL_CCtx_free(self.cctx)
+            self.L.ZL_Compressor_free(self.c)
+        except Exception:
+            pass
+
+
+class Decompressor:
+    def __init__(self):
+        L = _api()
+        self.L = L
+        self.d = L.ZL_DCtx_create()
+
+    def decompress(self, frame, capacity):
+        """Returns (bytes, type, width)."""
+        src = np.frombuffer(frame, np.uint8)
+        out = np.empty(max(capacity, 1), np.uint8)
+        info = OutputInfo()
+        size = _chk(self.L.ZL_DCtx_decompressTyped(self.d, ctypes.byref(info), out.ctypes.data, out.size,
+                                                   src.ctypes.data, src.size), "decompress")
+        return out[:size].tobytes(), info.type, info.fixedWidth
+
+    def __del__(self):
+        try:
+            self.L.ZL_DCtx_free(self.d)
+        except Exception:
+            pass

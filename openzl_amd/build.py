@@ -11,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libozl_hip.so")
+GRAPHS_LIB = os.path.join(HERE, "libozl_graphs.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -55,7 +56,22 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout.decode(errors="replace"))
     os.replace(tmp, LIB)
+    build_graphs()
     return LIB
+
+
+def build_graphs():
+    """User graphs in plain C (graphs/ozl_graphs.c) against the public header,
+    linked to libozl_hip.so."""
+    tmp = GRAPHS_LIB + ".tmp"
+    cmd = ["gcc", "-std=gnu11", "-O2", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(HERE, "graphs", "ozl_graphs.c"), "-L", HERE, "-l:libozl_hip.so",
+           "-Wl,-rpath,$ORIGIN", "-Wl,--no-undefined", "-o", tmp]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("graphs build failed:\n" + r.stdout.decode(errors="replace"))
+    os.replace(tmp, GRAPHS_LIB)
+    return GRAPHS_LIB
 
 
 if __name__ == "__main__":

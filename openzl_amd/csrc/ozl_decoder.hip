@@ -1,0 +1,729 @@
+// Universal decompressor of libozl_hip.so for frames made of the hot-path
+// transforms (src/openzl/decompress/decompress2.c:1442-1874 restricted to
+// single-input, single-chunk frames of format version 23).
+//
+// The host reads the frame and chunk headers, rebuilds the stream numbering
+// of the encoder (nodes in creation order, singleton then variable outputs,
+// decode_frameheader.c), and runs the decoders in frame order on the GPU.
+// Independent fse_v2 / huffman_v2 decoders and NCount readers are batched
+// into one launch each until a later decoder needs their output.
+#include <string.h>
+
+#include <vector>
+
+#include "openzl_amd.h"
+#include "ozl_internal.h"
+#include "ozl_core.h"
+#include "ozl_xxh3.h"
+#include "ozl_frame.h"
+#include "ozl_profile.h"
+
+extern "C" {
+int ozl_delta_decode(const void* first, const void* deltas, size_t n, int width, void* dst, void* scratch, void* stream);
+size_t ozl_delta_decode_scratch_size(size_t n, int width);
+int ozl_zigzag_decode(const void* src, size_t n, int width, void* dst, void* stream);
+int ozl_transpose_join(const void* planes, size_t n, int width, void* dst, void* stream);
+int ozl_bitpack_decode(const void* src, size_t n, int width, int nbBits, void* dst, void* stream);
+}
+int ozl_xxh3_device(const uint8_t* data, size_t len, uint64_t* dOut, void* scratch, size_t scratchSize, hipStream_t st);
+size_t ozl_xxh3_scratch_size(size_t len);
+
+struct ozl_ditem {
+    uint64_t side;
+    uint64_t bits;
+    uint64_t out;
+    uint32_t sideN;
+    uint32_t bitsLen;
+    uint32_t n;
+    uint8_t huf;
+    uint8_t nbStates;
+    uint8_t pad[2];
+};
+size_t ozl_entropy_items_workspace(uint32_t B);
+int ozl_entropy_decode_items(const ozl_ditem* items, uint32_t B, int* status, void* workspace, size_t wsSize,
+                             hipStream_t st);
+
+namespace {
+
+constexpr int kT = 256;
+
+struct NcItem {
+    uint64_t src;
+    uint64_t out;   // int16[256]
+    uint32_t len;
+    uint32_t pad;
+};
+
+__global__ void k_ncount_items(const NcItem* __restrict__ items, uint32_t B, int* __restrict__ counts)
+{
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    NcItem it = items[b];
+    int16_t norm[256];
+    unsigned ms, tl;
+    long r = ozl_fse_read_ncount(norm, &ms, &tl, (const uint8_t*)(uintptr_t)it.src, it.len);
+    if (r < 0 || (size_t)r != it.len) { counts[b] = -1; return; }
+    uint8_t* o = (uint8_t*)(uintptr_t)it.out;
+    for (unsigned s = 0; s <= ms; s++) { o[2 * s] = (uint8_t)norm[s]; o[2 * s + 1] = (uint8_t)((uint16_t)norm[s] >> 8); }
+    counts[b] = (int)ms + 1;
+}
+
+__global__ void __launch_bounds__(kT) k_fill(uint8_t* __restrict__ dst, size_t n, const uint8_t* __restrict__ v)
+{
+    uint8_t x = *v;
+    for (size_t i = (size_t)blockIdx.x * kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) dst[i] = x;
+}
+
+// flatpack decode (decode_flatpack_kernel.c:359)
+__global__ void __launch_bounds__(kT) k_flatpack_decode(const uint8_t* __restrict__ alpha, uint32_t card,
+                                                        const uint8_t* __restrict__ pk, size_t n, uint32_t nb,
+                                                        uint8_t* __restrict__ dst)
+{
+    __shared__ uint8_t a[256];
+    for (uint32_t i = threadIdx.x; i < 256; i += kT) a[i] = i < card ? alpha[i] : 0;
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) {
+        uint64_t bit = (uint64_t)i * nb;
+        uint32_t v = pk[bit >> 3];
+        if ((bit & 7) + nb > 8) v |= (uint32_t)pk[(bit >> 3) + 1] << 8;
+        v = (v >> (bit & 7)) & ((1u << nb) - 1);
+        dst[i] = a[v & 255];
+    }
+}
+
+__global__ void __launch_bounds__(kT) k_copy(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) dst[i] = src[i];
+}
+
+ZL_Report rep_ok(size_t v)
+{
+    ZL_Report r;
+    memset(&r, 0, sizeof(r));
+    r._value._value = v;
+    return r;
+}
+ZL_Report rep_err(int code)
+{
+    ZL_Report r;
+    memset(&r, 0, sizeof(r));
+    r._code = (ZL_ErrorCode)code;
+    return r;
+}
+
+struct HostReader {
+    const uint8_t* p;
+    size_t pos, end;
+    bool bad;
+    uint32_t u8() { if (pos >= end) { bad = true; return 0; } return p[pos++]; }
+    uint64_t varint()
+    {
+        uint64_t v = 0;
+        for (int s = 0; s < 64; s += 7) {
+            uint32_t x = u8();
+            v |= (uint64_t)(x & 127) << s;
+            if (x < 128) return v;
+        }
+        bad = true;
+        return 0;
+    }
+    void unpack(std::vector<uint32_t>& out, size_t cnt, int nb)
+    {
+        out.assign(cnt, 0);
+        size_t nbytes = (cnt * (size_t)nb + 7) / 8;
+        if (pos + nbytes > end) { bad = true; return; }
+        size_t bit = 0;
+        for (size_t i = 0; i < cnt; i++) {
+            uint32_t v = 0;
+            for (int b = 0; b < nb; b++, bit++) v |= (uint32_t)((p[pos + (bit >> 3)] >> (bit & 7)) & 1) << b;
+            out[i] = v;
+        }
+        pos += nbytes;
+    }
+};
+
+struct DStream {
+    uint8_t* dev = nullptr;
+    size_t bytes = 0;
+    int type = 0;   // ZL_Type bit
+    int width = 1;
+    bool ready = false;
+    bool pending = false;
+};
+
+struct DNode {
+    uint8_t trid;
+    std::vector<uint8_t> hdr;
+    uint32_t nbVO;
+    uint32_t firstOut, nbOut, in;
+};
+
+int nb_singletons(uint8_t trid)
+{
+    switch (trid) {
+    case OZL_TR_DELTA_INT: case OZL_TR_ZIGZAG: case OZL_TR_CONV_STRUCT_TO_SERIAL: case OZL_TR_CONV_NUM_TO_STRUCT_LE:
+    case OZL_TR_CONV_NUM_TO_SERIAL_LE: case OZL_TR_BITPACK_SERIAL: case OZL_TR_BITPACK_INT:
+    case OZL_TR_CONSTANT_SERIAL: case OZL_TR_FSE_NCOUNT: return 1;
+    case OZL_TR_FLATPACK: case OZL_TR_FSE_V2: case OZL_TR_HUFFMAN_V2: return 2;
+    case OZL_TR_TRANSPOSE_SPLIT: case OZL_TR_SPLITN: return 0;
+    default: return -1;
+    }
+}
+
+}  // namespace
+
+struct ZL_DCtx_s {
+    hipStream_t st = nullptr;
+    uint8_t* base = nullptr;
+    size_t cap = 0, used = 0;
+    std::vector<uint8_t*> old;
+    ~ZL_DCtx_s()
+    {
+        if (base) (void)hipFree(base);
+        for (uint8_t* p : old) (void)hipFree(p);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    void reset()
+    {
+        used = 0;
+        for (uint8_t* p : old) (void)hipFree(p);
+        old.clear();
+    }
+    uint8_t* alloc(size_t n)
+    {
+        n = ((n ? n : 1) + 255) & ~(size_t)255;
+        if (used + n > cap) {
+            size_t ncap = cap * 2 > used + n ? cap * 2 : used + n + (64u << 20);
+            uint8_t* nb = nullptr;
+            if (hipMalloc(&nb, ncap) != hipSuccess) return nullptr;
+            if (base) old.push_back(base);
+            base = nb;
+            cap = ncap;
+            used = 0;
+        }
+        uint8_t* p = base + used;
+        used += n;
+        return p;
+    }
+};
+
+namespace {
+
+struct Decoder {
+    ZL_DCtx_s* d;
+    hipStream_t st;
+    std::vector<DStream> S;
+    std::vector<ozl_ditem> items;
+    std::vector<uint32_t> itemStream;
+    std::vector<NcItem> nc;
+    std::vector<uint32_t> ncStream;
+    int err = 0;
+
+    uint8_t* alloc(size_t n)
+    {
+        uint8_t* p = d->alloc(n + 16);
+        if (!p && !err) err = OZL_ERR_ALLOCATION;
+        return p;
+    }
+
+    int flush()
+    {
+        if (!nc.empty()) {
+            uint32_t B = (uint32_t)nc.size();
+            NcItem* dItems = (NcItem*)alloc(B * sizeof(NcItem));
+            int* dCounts = (int*)alloc(B * 4);
+            if (err) return err;
+            OZL_HIP_CHECK(hipMemcpyAsync(dItems, nc.data(), B * sizeof(NcItem), hipMemcpyHostToDevice, st));
+            OZL_LAUNCH("k_ncount_items", k_ncount_items, dim3((B + 63) / 64), dim3(64), 0, st, dItems, B, dCounts);
+            std::vector<int> counts(B);
+            OZL_HIP_CHECK(hipMemcpyAsync(counts.data(), dCounts, B * 4, hipMemcpyDeviceToHost, st));
+            OZL_HIP_CHECK(hipStreamSynchronize(st));
+            for (uint32_t k = 0; k < B; k++) {
+                if (counts[k] <= 0) return OZL_ERR_CORRUPTION;
+                DStream& o = S[ncStream[k]];
+                o.bytes = (size_t)counts[k] * 2;
+                o.pending = false;
+                o.ready = true;
+            }
+            nc.clear();
+            ncStream.clear();
+        }
+        if (!items.empty()) {
+            uint32_t B = (uint32_t)items.size();
+            // side streams may have been regenerated by the NCount batch
+            for (uint32_t k = 0; k < B; k++) {
+                (void)k;
+            }
+            ozl_ditem* dItems = (ozl_ditem*)alloc(B * sizeof(ozl_ditem));
+            int* dStatus = (int*)alloc(B * 4);
+            size_t wsz = ozl_entropy_items_workspace(B);
+            uint8_t* ws = alloc(wsz);
+            if (err) return err;
+            OZL_HIP_CHECK(hipMemcpyAsync(dItems, items.data(), B * sizeof(ozl_ditem), hipMemcpyHostToDevice, st));
+            int rc = ozl_entropy_decode_items(dItems, B, dStatus, ws, wsz, st);
+            if (rc) return rc;
+            std::vector<int> status(B);
+            OZL_HIP_CHECK(hipMemcpyAsync(status.data(), dStatus, B * 4, hipMemcpyDeviceToHost, st));
+            OZL_HIP_CHECK(hipStreamSynchronize(st));
+            for (uint32_t k = 0; k < B; k++) {
+                if (status[k]) return status[k];
+                S[itemStream[k]].pending = false;
+                S[itemStream[k]].ready = true;
+            }
+            items.clear();
+            itemStream.clear();
+        }
+        return 0;
+    }
+
+    // make sure the given streams hold their final content
+    int need(const DNode& nd)
+    {
+        for (uint32_t k = 0; k < nd.nbOut; k++) {
+            const DStream& s = S[nd.firstOut + k];
+            if (s.pending) return flush();
+            if (!s.ready) return OZL_ERR_CORRUPTION;
+        }
+        return 0;
+    }
+
+    int decode(const DNode& nd)
+    {
+        DStream& in = S[nd.in];
+        const uint32_t o0 = nd.firstOut;
+        switch (nd.trid) {
+        case OZL_TR_FSE_NCOUNT: {
+            int rc = need(nd);
+            if (rc) return rc;
+            in.dev = alloc(512);
+            if (err) return err;
+            in.type = T_NUMERIC_;
+            in.width = 2;
+            in.pending = true;
+            nc.push_back({ (uint64_t)(uintptr_t)S[o0].dev, (uint64_t)(uintptr_t)in.dev, (uint32_t)S[o0].bytes, 0 });
+            ncStream.push_back(nd.in);
+            return 0;
+        }
+        case OZL_TR_FSE_V2:
+        case OZL_TR_HUFFMAN_V2: {
+            int rc = need(nd);
+            if (rc) return rc;
+            if (nd.hdr.size() < 2 || nd.hdr.size() > 9) return OZL_ERR_CORRUPTION;
+            uint64_t n = 0;
+            for (size_t k = 1; k < nd.hdr.size(); k++) n |= (uint64_t)nd.hdr[k] << (8 * (k - 1));
+            if (n > 0xFFFFFFFFull) return OZL_ERR_CORRUPTION;
+            const DStream& side = S[o0];
+            const DStream& bits = S[o0 + 1];
+            in.dev = alloc(n);
+            if (err) return err;
+            in.bytes = n;
+            in.type = T_SERIAL_;
+            in.width = 1;
+            in.pending = true;
+            ozl_ditem it;
+            memset(&it, 0, sizeof(it));
+            it.side = (uint64_t)(uintptr_t)side.dev;
+            it.sideN = (uint32_t)(nd.trid == OZL_TR_FSE_V2 ? side.bytes / 2 : side.bytes);
+            it.bits = (uint64_t)(uintptr_t)bits.dev;
+            it.bitsLen = (uint32_t)bits.bytes;
+            it.out = (uint64_t)(uintptr_t)in.dev;
+            it.n = (uint32_t)n;
+            it.huf = nd.trid == OZL_TR_HUFFMAN_V2;
+            it.nbStates = nd.hdr[0];
+            items.push_back(it);
+            itemStream.push_back(nd.in);
+            return 0;
+        }
+        default: break;
+        }
+        int rc = need(nd);
+        if (rc) return rc;
+        switch (nd.trid) {
+        case OZL_TR_DELTA_INT: {  // decode_delta_binding.c:10-62
+            const DStream& dl = S[o0];
+            int w = dl.width;
+            if (dl.type != T_NUMERIC_) return OZL_ERR_CORRUPTION;
+            size_t m = dl.bytes / w;
+            size_t n = nd.hdr.empty() ? 0 : m + 1;
+            if (!nd.hdr.empty() && nd.hdr.size() != (size_t)w) return OZL_ERR_CORRUPTION;
+            in.dev = alloc(n * w);
+            uint8_t* first = alloc(8);
+            uint8_t* scratch = alloc(ozl_delta_decode_scratch_size(n, w));
+            if (err) return err;
+            if (n) {
+                OZL_HIP_CHECK(hipMemcpyAsync(first, nd.hdr.data(), w, hipMemcpyHostToDevice, st));
+                rc = ozl_delta_decode(first, dl.dev, n, w, in.dev, scratch, st);
+                if (rc) return rc;
+                OZL_HIP_CHECK(hipStreamSynchronize(st));  // header staging buffer
+            }
+            in.bytes = n * w;
+            in.type = T_NUMERIC_;
+            in.width = w;
+            break;
+        }
+        case OZL_TR_ZIGZAG: {
+            const DStream& z = S[o0];
+            if (z.type != T_NUMERIC_) return OZL_ERR_CORRUPTION;
+            in.dev = alloc(z.bytes);
+            if (err) return err;
+            rc = ozl_zigzag_decode(z.dev, z.bytes / z.width, z.width, in.dev, st);
+            if (rc) return rc;
+            in.bytes = z.bytes;
+            in.type = T_NUMERIC_;
+            in.width = z.width;
+            break;
+        }
+        case OZL_TR_TRANSPOSE_SPLIT: {
+            uint32_t W = nd.nbVO;
+            if (W == 0) return OZL_ERR_CORRUPTION;
+            size_t n = S[o0].bytes;
+            for (uint32_t p = 1; p < W; p++)
+                if (S[o0 + p].bytes != n) return OZL_ERR_CORRUPTION;  // decode_transpose_binding.c:54-57
+            uint8_t* planes = alloc(n * W);
+            in.dev = alloc(n * W);
+            if (err) return err;
+            for (uint32_t p = 0; p < W; p++)
+                if (n) OZL_HIP_CHECK(hipMemcpyAsync(planes + (size_t)p * n, S[o0 + p].dev, n, hipMemcpyDeviceToDevice, st));
+            if (n) {
+                rc = ozl_transpose_join(planes, n, (int)W, in.dev, st);
+                if (rc) return rc;
+            }
+            in.bytes = n * W;
+            in.type = T_STRUCT_;
+            in.width = (int)W;
+            break;
+        }
+        case OZL_TR_CONV_NUM_TO_STRUCT_LE: {
+            const DStream& s = S[o0];
+            in = s;
+            in.type = T_NUMERIC_;
+            break;
+        }
+        case OZL_TR_CONV_STRUCT_TO_SERIAL: {
+            const DStream& s = S[o0];
+            uint64_t w = 0;
+            for (size_t k = 0; k < nd.hdr.size() && k < 9; k++) w |= (uint64_t)(nd.hdr[k] & 127) << (7 * k);
+            if (w == 0 || s.bytes % w) return OZL_ERR_CORRUPTION;
+            in = s;
+            in.type = T_STRUCT_;
+            in.width = (int)w;
+            break;
+        }
+        case OZL_TR_CONV_NUM_TO_SERIAL_LE: {
+            const DStream& s = S[o0];
+            if (nd.hdr.size() != 1 || nd.hdr[0] > 3) return OZL_ERR_CORRUPTION;
+            int w = 1 << nd.hdr[0];
+            if (s.bytes % w) return OZL_ERR_CORRUPTION;
+            in = s;
+            in.type = T_NUMERIC_;
+            in.width = w;
+            break;
+        }
+        case OZL_TR_BITPACK_SERIAL:
+        case OZL_TR_BITPACK_INT: {  // decode_bitpack_binding.c:10-60
+            const DStream& s = S[o0];
+            if (nd.hdr.empty() || nd.hdr.size() > 2) return OZL_ERR_CORRUPTION;
+            int w = 1 << (nd.hdr[0] >> 6);
+            int nb = (nd.hdr[0] & 63) + 1;
+            if (nb > 8 * w || (nd.trid == OZL_TR_BITPACK_SERIAL && w != 1)) return OZL_ERR_CORRUPTION;
+            size_t extra = nd.hdr.size() == 2 ? nd.hdr[1] : 0;
+            size_t maxElts = s.bytes * 8 / nb;
+            if (extra > maxElts) return OZL_ERR_CORRUPTION;
+            size_t n = maxElts - extra;
+            if ((n * nb + 7) / 8 != s.bytes) return OZL_ERR_CORRUPTION;
+            in.dev = alloc(n * w);
+            if (err) return err;
+            if (n) {
+                rc = ozl_bitpack_decode(s.dev, n, w, nb, in.dev, st);
+                if (rc) return rc;
+            }
+            in.bytes = n * w;
+            in.type = nd.trid == OZL_TR_BITPACK_INT ? T_NUMERIC_ : T_SERIAL_;
+            in.width = nd.trid == OZL_TR_BITPACK_INT ? w : 1;
+            break;
+        }
+        case OZL_TR_FLATPACK: {
+            const DStream& a = S[o0];
+            const DStream& pk = S[o0 + 1];
+            size_t card = a.bytes;
+            if (card > 256) return OZL_ERR_CORRUPTION;
+            size_t n = 0;
+            uint32_t nb = card <= 1 ? (uint32_t)card : 1 + (uint32_t)ozl_hb32((uint32_t)card - 1);
+            if (pk.bytes) {
+                uint8_t last = 0;
+                OZL_HIP_CHECK(hipMemcpyAsync(&last, pk.dev + pk.bytes - 1, 1, hipMemcpyDeviceToHost, st));
+                OZL_HIP_CHECK(hipStreamSynchronize(st));
+                if (last == 0 || nb == 0) return OZL_ERR_CORRUPTION;
+                n = ((pk.bytes - 1) * 8 + ozl_hb32(last)) / nb;
+            }
+            in.dev = alloc(n);
+            if (err) return err;
+            if (n)
+                OZL_LAUNCH("k_flatpack_decode", k_flatpack_decode, dim3(ozl_blocks(n, kT, 4096)), dim3(kT), 0, st, a.dev,
+                           (uint32_t)card, pk.dev, n, nb, in.dev);
+            in.bytes = n;
+            in.type = T_SERIAL_;
+            in.width = 1;
+            break;
+        }
+        case OZL_TR_CONSTANT_SERIAL: {  // decode_constant_binding.c:10
+            const DStream& v = S[o0];
+            uint64_t n = 0;
+            for (size_t k = 0; k < nd.hdr.size() && k < 10; k++) n |= (uint64_t)(nd.hdr[k] & 127) << (7 * k);
+            if (v.bytes != 1) return OZL_ERR_CORRUPTION;
+            in.dev = alloc(n);
+            if (err) return err;
+            if (n) OZL_LAUNCH("k_fill", k_fill, dim3(ozl_blocks(n, kT, 4096)), dim3(kT), 0, st, in.dev, (size_t)n, v.dev);
+            in.bytes = n;
+            in.type = T_SERIAL_;
+            in.width = 1;
+            break;
+        }
+        case OZL_TR_SPLITN: {  // decode_splitN_binding.c:16 -- concatenation
+            size_t total = 0;
+            for (uint32_t k = 0; k < nd.nbVO; k++) total += S[o0 + k].bytes;
+            in.dev = alloc(total);
+            if (err) return err;
+            size_t pos = 0;
+            for (uint32_t k = 0; k < nd.nbVO; k++) {
+                if (S[o0 + k].bytes)
+                    OZL_HIP_CHECK(hipMemcpyAsync(in.dev + pos, S[o0 + k].dev, S[o0 + k].bytes, hipMemcpyDeviceToDevice, st));
+                pos += S[o0 + k].bytes;
+            }
+            in.bytes = total;
+            in.type = T_SERIAL_;
+            in.width = 1;
+            break;
+        }
+        default: return OZL_ERR_CORRUPTION;
+        }
+        in.ready = true;
+        in.pending = false;
+        return 0;
+    }
+
+    static constexpr int T_SERIAL_ = ZL_Type_serial, T_STRUCT_ = ZL_Type_struct, T_NUMERIC_ = ZL_Type_numeric;
+};
+
+struct ParsedFrame {
+    int flags, type;
+    uint64_t size;
+    size_t chunkStart, dataStart, csumPos, end;
+    std::vector<DNode> nodes;
+    std::vector<uint64_t> storedSize;
+    std::vector<uint32_t> storedId;
+    uint32_t nbStreams;
+};
+
+int parse_frame(const uint8_t* f, size_t size, ParsedFrame& P)
+{
+    HostReader r = { f, 0, size, false };
+    uint32_t magic = 0;
+    for (int k = 0; k < 4; k++) magic |= r.u8() << (8 * k);
+    if (r.bad || magic < OZL_MAGIC_BASE || magic > OZL_MAGIC_BASE + 64) return OZL_ERR_HEADER_UNKNOWN;
+    if (magic != OZL_MAGIC_BASE + OZL_FORMAT_VERSION) return OZL_ERR_FORMAT_VERSION_UNSUPPORTED;
+    P.flags = (int)r.u8();
+    if (P.flags & ~3) return ZL_ErrorCode_frameParameter_unsupported;
+    uint32_t tok = r.u8();
+    if ((tok & 15) != 1) return ZL_ErrorCode_frameParameter_unsupported;  // one input
+    P.type = (int)((tok >> 4) & 3);
+    uint64_t s1 = r.varint();
+    if (r.bad || s1 == 0) return OZL_ERR_CORRUPTION;
+    P.size = s1 - 1;
+    if (P.flags & 2) {
+        uint8_t hc = (uint8_t)(ozl_xxh3_short(f, r.pos) & 255);
+        if (r.u8() != hc) return OZL_ERR_CORRUPTION;
+    }
+    P.chunkStart = r.pos;
+    uint64_t nbDec1 = r.varint();
+    if (r.bad || nbDec1 == 0) return OZL_ERR_CORRUPTION;
+    size_t nn = (size_t)nbDec1 - 1;
+    uint64_t nbStored = r.varint();
+    if (r.bad || nn > 20000 || nbStored > 110000) return OZL_ERR_CORRUPTION;
+    std::vector<uint32_t> trt, ids, hf, vf, inf, dist;
+    r.unpack(trt, nn, 1);
+    for (uint32_t t : trt)
+        if (t) return ZL_ErrorCode_frameParameter_unsupported;  // custom transforms
+    r.unpack(ids, nn, 6);
+    P.nodes.assign(nn, DNode());
+    for (size_t k = 0; k < nn; k++) P.nodes[k].trid = (uint8_t)ids[nn - 1 - k];
+    r.unpack(hf, nn, 1);
+    std::vector<size_t> hlen(nn, 0);
+    for (size_t i = 0; i < nn; i++)
+        if (hf[i]) hlen[nn - 1 - i] = (size_t)r.varint() + 1;
+    r.unpack(vf, nn, 1);
+    for (size_t k = 0; k < nn; k++) P.nodes[k].nbVO = 0;
+    for (size_t i = 0; i < nn; i++)
+        if (vf[i]) P.nodes[nn - 1 - i].nbVO = (uint32_t)r.varint() + 1;
+    r.unpack(inf, nn, 1);
+    for (uint32_t x : inf)
+        if (x) return ZL_ErrorCode_frameParameter_unsupported;  // multi-input transforms
+    r.unpack(dist, nn, ozl_next_pow2((uint64_t)nn + nbStored));
+    P.storedSize.resize(nbStored);
+    for (size_t k = 0; k < nbStored; k++) P.storedSize[k] = r.varint();
+    if (r.bad) return OZL_ERR_CORRUPTION;
+    // node headers (decode order)
+    size_t hp = r.pos;
+    for (size_t i = 0; i < nn; i++) {
+        DNode& nd = P.nodes[nn - 1 - i];
+        if (hp + hlen[nn - 1 - i] > size) return OZL_ERR_CORRUPTION;
+        nd.hdr.assign(f + hp, f + hp + hlen[nn - 1 - i]);
+        hp += hlen[nn - 1 - i];
+    }
+    P.dataStart = hp;
+    // stream numbering (creation order)
+    uint32_t next = 1;
+    for (size_t k = 0; k < nn; k++) {
+        DNode& nd = P.nodes[k];
+        int s1n = nb_singletons(nd.trid);
+        if (s1n < 0) return ZL_ErrorCode_frameParameter_unsupported;
+        nd.firstOut = next;
+        nd.nbOut = (uint32_t)s1n + nd.nbVO;
+        next += nd.nbOut;
+        uint32_t dd = dist[nn - 1 - k] + 1;
+        if (dd > nd.firstOut) return OZL_ERR_CORRUPTION;
+        nd.in = nd.firstOut - dd;
+    }
+    P.nbStreams = next;
+    std::vector<uint8_t> consumed(next, 0);
+    for (auto& nd : P.nodes) {
+        if (consumed[nd.in]) return OZL_ERR_CORRUPTION;
+        consumed[nd.in] = 1;
+    }
+    for (int64_t s = (int64_t)next - 1; s >= 0; s--)
+        if (!consumed[s]) P.storedId.push_back((uint32_t)s);
+    if (P.storedId.size() != nbStored) return OZL_ERR_CORRUPTION;
+    size_t pos = P.dataStart;
+    for (size_t k = 0; k < nbStored; k++) pos += P.storedSize[k];
+    P.csumPos = pos;
+    P.end = pos + ((P.flags & 1) ? 4 : 0) + ((P.flags & 2) ? 4 : 0);
+    if (P.end + 1 > size || f[P.end] != 0) return OZL_ERR_CORRUPTION;
+    return 0;
+}
+
+int decompress_impl(ZL_DCtx_s* dc, const uint8_t* src, size_t srcSize, void* dst, size_t cap, ZL_OutputInfo* info)
+{
+    ParsedFrame P;
+    int rc = parse_frame(src, srcSize, P);
+    if (rc) return rc;
+    if (!dc->st && hipStreamCreate(&dc->st) != hipSuccess) return OZL_ERR_ALLOCATION;
+    dc->reset();
+    Decoder D;
+    D.d = dc;
+    D.st = dc->st;
+    D.S.assign(P.nbStreams, DStream());
+    uint8_t* dFrame = D.alloc(srcSize);
+    if (D.err) return D.err;
+    OZL_HIP_CHECK(hipMemcpyAsync(dFrame, src, srcSize, hipMemcpyHostToDevice, dc->st));
+    size_t pos = P.dataStart;
+    for (size_t k = 0; k < P.storedId.size(); k++) {
+        DStream& s = D.S[P.storedId[k]];
+        s.dev = dFrame + pos;
+        s.bytes = P.storedSize[k];
+        s.type = ZL_Type_serial;
+        s.width = 1;
+        s.ready = true;
+        pos += P.storedSize[k];
+    }
+    // stored numeric streams keep their type: side inputs of fse_v2 (int16)
+    // and huffman_v2 (u8) when stored raw
+    for (auto& nd : P.nodes) {
+        if (nd.trid == OZL_TR_FSE_V2 && D.S[nd.firstOut].ready) { D.S[nd.firstOut].type = ZL_Type_numeric; D.S[nd.firstOut].width = 2; }
+        if (nd.trid == OZL_TR_HUFFMAN_V2 && D.S[nd.firstOut].ready) { D.S[nd.firstOut].type = ZL_Type_numeric; }
+        if (nd.trid == OZL_TR_FSE_NCOUNT) { /* input regenerated as numeric */ }
+    }
+    for (int64_t k = (int64_t)P.nodes.size() - 1; k >= 0; k--) {
+        rc = D.decode(P.nodes[k]);
+        if (rc) return rc;
+    }
+    rc = D.flush();
+    if (rc) return rc;
+    DStream& out = D.S[0];
+    if (P.nodes.empty()) {  // stored input
+        out.type = P.type == OZL_T_SERIAL ? ZL_Type_serial : P.type == OZL_T_STRUCT ? ZL_Type_struct : ZL_Type_numeric;
+    }
+    if (out.bytes != P.size) return OZL_ERR_CORRUPTION;
+    int wantType = P.type == OZL_T_SERIAL ? ZL_Type_serial : P.type == OZL_T_STRUCT ? ZL_Type_struct : ZL_Type_numeric;
+    if (!P.nodes.empty() && out.type != wantType) {
+        // a stored input keeps the frame's type; a regenerated one must match
+        if (!(wantType == ZL_Type_serial)) return OZL_ERR_CORRUPTION;
+    }
+    if (out.bytes > cap) return OZL_ERR_DSTCAPACITY_TOOSMALL;
+    // checksums
+    uint64_t hashes[2] = { 0, 0 };
+    uint64_t* dH = (uint64_t*)D.alloc(16);
+    size_t hs = ozl_xxh3_scratch_size(out.bytes > srcSize ? out.bytes : srcSize);
+    uint8_t* scratch = D.alloc(hs);
+    if (D.err) return D.err;
+    if (P.flags & 1) {
+        rc = ozl_xxh3_device(out.dev ? out.dev : dFrame, out.bytes, dH, scratch, hs, dc->st);
+        if (rc) return rc;
+    }
+    if (P.flags & 2) {
+        size_t clen = P.csumPos + ((P.flags & 1) ? 4 : 0) - P.chunkStart;
+        rc = ozl_xxh3_device(dFrame + P.chunkStart, clen, dH + 1, scratch, hs, dc->st);
+        if (rc) return rc;
+    }
+    OZL_HIP_CHECK(hipMemcpyAsync(hashes, dH, 16, hipMemcpyDeviceToHost, dc->st));
+    if (out.bytes) OZL_HIP_CHECK(hipMemcpyAsync(dst, out.dev, out.bytes, hipMemcpyDeviceToHost, dc->st));
+    OZL_HIP_CHECK(hipStreamSynchronize(dc->st));
+    size_t cp = P.csumPos;
+    if (P.flags & 2) {
+        size_t at = cp + ((P.flags & 1) ? 4 : 0);
+        if ((uint32_t)hashes[1] != ozl_le32(src + at)) return OZL_ERR_COMPRESSED_CHECKSUM;
+    }
+    if (P.flags & 1) {
+        if ((uint32_t)hashes[0] != ozl_le32(src + cp)) return OZL_ERR_CONTENT_CHECKSUM;
+    }
+    if (info) {
+        info->type = (ZL_Type)wantType;
+        info->fixedWidth = (uint32_t)(wantType == ZL_Type_serial ? 1 : out.width);
+        info->decompressedByteSize = out.bytes;
+        info->numElts = out.bytes / info->fixedWidth;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+ZL_DCtx* ZL_DCtx_create(void) { return new (std::nothrow) ZL_DCtx_s(); }
+void ZL_DCtx_free(ZL_DCtx* d) { delete d; }
+
+ZL_Report ZL_DCtx_decompressTyped(ZL_DCtx* dctx, ZL_OutputInfo* info, void* dst, size_t cap, const void* src,
+                                  size_t srcSize)
+{
+    if (!dctx || !src) return rep_err(ZL_ErrorCode_parameter_invalid);
+    ZL_OutputInfo tmp;
+    int rc = decompress_impl(dctx, (const uint8_t*)src, srcSize, dst, cap, &tmp);
+    if (rc) return rep_err(rc);
+    if (info) *info = tmp;
+    return rep_ok(tmp.decompressedByteSize);
+}
+
+ZL_Report ZL_decompress(void* dst, size_t cap, const void* src, size_t srcSize)
+{
+    ZL_DCtx_s d;
+    ZL_OutputInfo info;
+    int rc = decompress_impl(&d, (const uint8_t*)src, srcSize, dst, cap, &info);
+    if (rc) return rep_err(rc);
+    return rep_ok(info.decompressedByteSize);
+}
+
+ZL_Report ZL_getDecompressedSize(const void* src, size_t srcSize)
+{
+    const uint8_t* f = (const uint8_t*)src;
+    HostReader r = { f, 0, srcSize, false };
+    uint32_t magic = 0;
+    for (int k = 0; k < 4; k++) magic |= r.u8() << (8 * k);
+    if (r.bad || magic != OZL_MAGIC_BASE + OZL_FORMAT_VERSION) return rep_err(ZL_ErrorCode_header_unknown);
+    r.u8();
+    uint32_t tok = r.u8();
+    if ((tok & 15) != 1) return rep_err(ZL_ErrorCode_frameParameter_unsupported);
+    uint64_t s1 = r.varint();
+    if (r.bad || s1 == 0) return rep_err(ZL_ErrorCode_corruption);
+    return rep_ok(s1 - 1);
+}
+
+}  // extern "C"

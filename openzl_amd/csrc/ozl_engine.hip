@@ -1,0 +1,976 @@
+// Graph engine of libozl_hip.so: the reference's public compression API
+// (ZL_Compressor / ZL_CCtx) for the graphs of the codec hot path, with every
+// codec running on the GPU.
+//
+// The engine restates the reference engine's execution order, because that
+// order defines the frame (src/openzl/compress/cctx.c:1593-1718):
+//   * a graph invocation first runs the nodes its function creates, then the
+//     successors of its edges in edge-creation order, depth first
+//     (cctx.c:697-790, 1116-1224);
+//   * a successor input smaller than minStreamSize is stored (cctx.c:1127-1140);
+//   * an input whose type the graph does not accept is converted first by an
+//     implicit conversion node (cctx.c:534-605, implicit_conversion.c:20-34);
+//   * a node's singleton outputs take consecutive stream ids when the node is
+//     created, its variable outputs follow (rtgraphs.c:87-132, 278-340).
+// Stream contents stay in device memory (one arena per context); the host
+// keeps only ids, sizes, node headers and the per-chunk entropy plans.
+#include <string.h>
+
+#include <memory>
+#include <vector>
+
+#include "openzl_amd.h"
+#include "ozl_internal.h"
+#include "ozl_core.h"
+#include "ozl_xxh3.h"
+#include "ozl_frame.h"
+#include "ozl_entropy_api.h"
+#include "ozl_profile.h"
+
+// numeric / serial device entry points (ozl_numeric.hip)
+extern "C" {
+int ozl_delta_encode(const void* src, size_t n, int width, void* deltas, void* first, void* stream);
+int ozl_zigzag_encode(const void* src, size_t n, int width, void* dst, void* stream);
+int ozl_transpose_split(const void* src, size_t n, int width, void* planes, void* stream);
+int ozl_max_async(const void* src, size_t n, int width, unsigned long long* dmax, void* stream);
+int ozl_bitpack_encode(const void* src, size_t n, int width, int nbBits, void* dst, void* stream);
+}
+int ozl_xxh3_device(const uint8_t* data, size_t len, uint64_t* dOut, void* scratch, size_t scratchSize, hipStream_t st);
+size_t ozl_xxh3_scratch_size(size_t len);
+int ozl_gather_sized(const uint8_t* const* srcs, const uint64_t* sizes, const uint64_t* dstOffs, int count,
+                     size_t maxSize, uint8_t* dst, hipStream_t st);
+
+namespace {
+
+enum { NID_DELTA = 2, NID_TRANSPOSE_SPLIT = 3, NID_ZIGZAG = 4, NID_CONV_STRUCT_TO_SERIAL = 10,
+       NID_CONV_NUM_TO_STRUCT = 11, NID_CONV_NUM_TO_SERIAL = 12 };
+enum { GID_SERIAL_STORE = 1, GID_STORE = 2, GID_FSE = 3, GID_HUF = 4, GID_CONSTANT = 6, GID_BITPACK = 8,
+       GID_FLATPACK = 9, GID_CUSTOM_BASE = 4096 };
+enum { GK_STATIC, GK_TRANSPOSE, GK_FUNCTION, GK_BRUTE };
+enum { T_SERIAL = 1, T_STRUCT = 2, T_NUMERIC = 4 };
+
+ZL_Report rep_ok(size_t v)
+{
+    ZL_Report r;
+    memset(&r, 0, sizeof(r));
+    r._value._code = ZL_ErrorCode_no_error;
+    r._value._value = v;
+    return r;
+}
+ZL_Report rep_err(int code)
+{
+    ZL_Report r;
+    memset(&r, 0, sizeof(r));
+    r._code = (ZL_ErrorCode)code;
+    return r;
+}
+int frame_type(int t) { return t == T_SERIAL ? OZL_T_SERIAL : t == T_STRUCT ? OZL_T_STRUCT : OZL_T_NUMERIC; }
+
+struct GDef {
+    int kind;
+    unsigned node;
+    std::vector<unsigned> succ;
+    ZL_FunctionGraphFn fn;
+    int mask;
+};
+
+// grow-only device arena, reset per compression
+struct DevArena {
+    uint8_t* base = nullptr;
+    size_t cap = 0, used = 0;
+    std::vector<uint8_t*> old;
+    ~DevArena()
+    {
+        if (base) (void)hipFree(base);
+        for (uint8_t* p : old) (void)hipFree(p);
+    }
+    void reset()
+    {
+        used = 0;
+        for (uint8_t* p : old) (void)hipFree(p);
+        old.clear();
+    }
+    uint8_t* alloc(size_t n)
+    {
+        n = (n + 255) & ~(size_t)255;
+        if (used + n > cap) {
+            size_t ncap = cap * 2 > used + n ? cap * 2 : used + n + (64u << 20);
+            uint8_t* nb = nullptr;
+            if (hipMalloc(&nb, ncap) != hipSuccess) return nullptr;
+            if (base) old.push_back(base);  // earlier allocations stay valid until reset()
+            base = nb;
+            cap = ncap;
+            used = 0;
+        }
+        uint8_t* p = base + used;
+        used += n;
+        return p;
+    }
+};
+
+struct EStream {
+    int type;
+    int width;
+    size_t n;
+    uint8_t* dev;
+    bool store;
+    size_t bytes() const { return n * (size_t)width; }
+};
+
+struct ENode {
+    uint8_t trid;
+    uint8_t hdrLen;
+    uint16_t nbVO;
+    uint32_t in;
+    uint32_t firstOut;
+    uint8_t hdr[12];
+};
+
+}  // namespace
+
+struct ZL_Compressor_s {
+    int formatVersion = 0;
+    int contentChecksum = ZL_TernaryParam_auto;
+    int compressedChecksum = ZL_TernaryParam_auto;
+    int minStreamSize = OZL_MIN_STREAM_SIZE;
+    std::vector<GDef> graphs;
+    unsigned start = 0;
+};
+
+struct ZL_TypedRef_s {
+    const void* ptr;
+    int type;
+    size_t width, n;
+};
+
+struct Session;
+struct ZL_Input_s {
+    Session* s;
+    uint32_t sid;
+};
+struct ZL_Edge_s {
+    ZL_Graph* g;
+    uint32_t sid;
+    unsigned dest;
+    bool destSet;
+    bool consumed;
+    ZL_Input_s input;
+};
+struct ZL_Graph_s {
+    Session* s;
+    std::vector<std::unique_ptr<ZL_Edge_s>> edges;
+    std::vector<std::unique_ptr<std::vector<ZL_Edge*>>> lists;
+};
+
+struct Session {
+    const ZL_Compressor_s* comp;
+    DevArena* arena;
+    hipStream_t st;
+    int minStreamSize;
+    std::vector<EStream> streams;
+    std::vector<ENode> nodes;
+    int err = 0;
+
+    uint8_t* alloc(size_t n)
+    {
+        uint8_t* p = arena->alloc(n ? n : 1);
+        if (!p && !err) err = OZL_ERR_ALLOCATION;
+        return p;
+    }
+    uint32_t newStream(int type, int width, size_t n, uint8_t* dev)
+    {
+        streams.push_back({ type, width, n, dev, false });
+        return (uint32_t)streams.size() - 1;
+    }
+    ENode& newNode(uint8_t trid, uint32_t in)
+    {
+        ENode nd;
+        memset(&nd, 0, sizeof(nd));
+        nd.trid = trid;
+        nd.in = in;
+        nd.firstOut = (uint32_t)streams.size();
+        nodes.push_back(nd);
+        return nodes.back();
+    }
+    const GDef* custom(unsigned gid) const
+    {
+        if (gid < GID_CUSTOM_BASE || gid - GID_CUSTOM_BASE >= comp->graphs.size()) return nullptr;
+        return &comp->graphs[gid - GID_CUSTOM_BASE];
+    }
+    int graphMask(unsigned gid) const
+    {
+        switch (gid) {
+        case GID_SERIAL_STORE: return T_SERIAL;
+        case GID_STORE: return T_SERIAL | T_STRUCT | T_NUMERIC;
+        case GID_FSE: return T_SERIAL;
+        case GID_HUF: return T_SERIAL | T_STRUCT | T_NUMERIC;
+        case GID_CONSTANT: return T_SERIAL | T_STRUCT;
+        case GID_BITPACK: return T_SERIAL | T_NUMERIC;
+        case GID_FLATPACK: return T_SERIAL;
+        default: break;
+        }
+        const GDef* g = custom(gid);
+        return g ? g->mask : 0;
+    }
+
+    // ---------------------------------------------------------------- nodes
+    int runConversion(uint32_t sid, int toMask, uint32_t* out)
+    {
+        EStream s = streams[sid];
+        if ((s.type & T_NUMERIC) && (toMask & T_STRUCT)) {
+            newNode(OZL_TR_CONV_NUM_TO_STRUCT_LE, sid);
+            *out = newStream(T_STRUCT, s.width, s.n, s.dev);
+            return 0;
+        }
+        if ((s.type & T_STRUCT) && (toMask & T_SERIAL)) {
+            ENode& nd = newNode(OZL_TR_CONV_STRUCT_TO_SERIAL, sid);
+            nd.hdrLen = (uint8_t)ozl_varint((uint64_t)s.width, nd.hdr);
+            *out = newStream(T_SERIAL, 1, s.bytes(), s.dev);
+            return 0;
+        }
+        if ((s.type & T_NUMERIC) && (toMask & T_SERIAL)) {
+            ENode& nd = newNode(OZL_TR_CONV_NUM_TO_SERIAL_LE, sid);
+            nd.hdr[0] = (uint8_t)ozl_next_pow2((uint64_t)s.width);
+            nd.hdrLen = 1;
+            *out = newStream(T_SERIAL, 1, s.bytes(), s.dev);
+            return 0;
+        }
+        return OZL_ERR_INPUT_TYPE_UNSUPPORTED;
+    }
+
+    // STORE1 (selector_store.c:19-36) sends every stream to serial_store, so a
+    // numeric or struct stream is first converted to serial
+    // (cctx.c:563-605); the converted stream is the one stored.
+    int storeStream(uint32_t sid)
+    {
+        if (streams[sid].type != T_SERIAL) {
+            uint32_t o;
+            int rc = runConversion(sid, T_SERIAL, &o);
+            if (rc) return rc;
+            sid = o;
+        }
+        streams[sid].store = true;
+        return 0;
+    }
+
+    // Runs a standard node; returns its outputs in order.
+    int runNode(unsigned nid, uint32_t sid, std::vector<uint32_t>& outs)
+    {
+        EStream s = streams[sid];
+        outs.clear();
+        switch (nid) {
+        case NID_DELTA: {  // encode_delta_binding.c:11-45
+            if (!(s.type & T_NUMERIC)) return OZL_ERR_INPUT_TYPE_UNSUPPORTED;
+            uint8_t* d = alloc(s.bytes());
+            uint8_t* first = alloc(8);
+            if (err) return err;
+            if (s.n) {
+                int rc = ozl_delta_encode(s.dev, s.n, s.width, d, first, st);
+                if (rc) return rc;
+            }
+            ENode& nd = newNode(OZL_TR_DELTA_INT, sid);
+            if (s.n) {
+                OZL_HIP_CHECK(hipMemcpyAsync(nd.hdr, first, s.width, hipMemcpyDeviceToHost, st));
+                OZL_HIP_CHECK(hipStreamSynchronize(st));
+                nd.hdrLen = (uint8_t)s.width;
+            }
+            outs.push_back(newStream(T_NUMERIC, s.width, s.n ? s.n - 1 : 0, d));
+            return 0;
+        }
+        case NID_ZIGZAG: {  // encode_zigzag_binding.c:8-40
+            if (!(s.type & T_NUMERIC)) return OZL_ERR_INPUT_TYPE_UNSUPPORTED;
+            uint8_t* d = alloc(s.bytes());
+            if (err) return err;
+            int rc = ozl_zigzag_encode(s.dev, s.n, s.width, d, st);
+            if (rc) return rc;
+            newNode(OZL_TR_ZIGZAG, sid);
+            outs.push_back(newStream(T_NUMERIC, s.width, s.n, d));
+            return 0;
+        }
+        case NID_TRANSPOSE_SPLIT: {  // encode_transpose_binding.c:42-79
+            if (!(s.type & T_STRUCT)) return OZL_ERR_INPUT_TYPE_UNSUPPORTED;
+            uint8_t* d = alloc(s.bytes());
+            if (err) return err;
+            int rc = ozl_transpose_split(s.dev, s.n, s.width, d, st);
+            if (rc) return rc;
+            ENode& nd = newNode(OZL_TR_TRANSPOSE_SPLIT, sid);
+            nd.nbVO = (uint16_t)s.width;
+            for (int p = 0; p < s.width; p++) outs.push_back(newStream(T_SERIAL, 1, s.n, d + (size_t)p * s.n));
+            return 0;
+        }
+        case NID_CONV_NUM_TO_STRUCT:
+        case NID_CONV_NUM_TO_SERIAL:
+        case NID_CONV_STRUCT_TO_SERIAL: {
+            int want = nid == NID_CONV_NUM_TO_STRUCT ? T_STRUCT : T_SERIAL;
+            int from = nid == NID_CONV_STRUCT_TO_SERIAL ? T_STRUCT : T_NUMERIC;
+            if (!(s.type & from)) return OZL_ERR_INPUT_TYPE_UNSUPPORTED;
+            uint32_t o;
+            int rc = runConversion(sid, want, &o);
+            if (rc) return rc;
+            outs.push_back(o);
+            return 0;
+        }
+        default: return OZL_ERR_GRAPH_INVALID;
+        }
+    }
+
+    // ZL_GRAPH_BITPACK: selector to bitpack_int (numeric) or bitpack_serial
+    // (encode_bitpack_binding.c:92-165), then serial_store.
+    int runBitpack(uint32_t sid)
+    {
+        EStream s = streams[sid];
+        unsigned long long* dmax = (unsigned long long*)alloc(8);
+        if (err) return err;
+        OZL_HIP_CHECK(hipMemsetAsync(dmax, 0, 8, st));
+        int rc = ozl_max_async(s.dev, s.n, s.width, dmax, st);
+        if (rc) return rc;
+        unsigned long long mx = 0;
+        OZL_HIP_CHECK(hipMemcpyAsync(&mx, dmax, 8, hipMemcpyDeviceToHost, st));
+        OZL_HIP_CHECK(hipStreamSynchronize(st));
+        int nb = 1 + (mx ? ozl_hb64(mx) : 0);
+        size_t size = (s.n * (size_t)nb + 7) / 8;
+        uint8_t* d = alloc(size + 8);
+        if (err) return err;
+        if (nb == 8 * s.width) {
+            if (size) OZL_HIP_CHECK(hipMemcpyAsync(d, s.dev, size, hipMemcpyDeviceToDevice, st));
+        } else {
+            rc = ozl_bitpack_encode(s.dev, s.n, s.width, nb, d, st);
+            if (rc) return rc;
+        }
+        ENode& nd = newNode((s.type & T_NUMERIC) ? OZL_TR_BITPACK_INT : OZL_TR_BITPACK_SERIAL, sid);
+        static const uint8_t lg[9] = { 0, 0, 1, 0, 2, 0, 0, 0, 3 };
+        size_t maxElts = size * 8 / (size_t)nb;
+        nd.hdr[0] = (uint8_t)((lg[s.width] << 6) | (nb - 1));
+        nd.hdrLen = 1;
+        if (maxElts > s.n) { nd.hdr[1] = (uint8_t)(maxElts - s.n); nd.hdrLen = 2; }
+        uint32_t o = newStream(T_SERIAL, 1, size, d);
+        streams[o].store = true;
+        return 0;
+    }
+
+    // ---------------------------------------------------------------- entropy graphs
+    // ZL_GRAPH_FSE / ZL_GRAPH_HUFFMAN on a serial stream
+    // (encode_entropy_binding.c:491-519, 601-753, 814-851).
+    int runEntropy(uint32_t sid, int huf)
+    {
+        EStream s = streams[sid];
+        if (s.type != T_SERIAL || s.width != 1) {
+            // width-1 numeric/struct inputs are converted at the top of the
+            // Huffman graph (doEntropyConversion, :765-812)
+            if (s.width != 1) return OZL_ERR_NODE_INVALID_INPUT;
+            ENode& nd = newNode(s.type == T_NUMERIC ? OZL_TR_CONV_NUM_TO_SERIAL_LE : OZL_TR_CONV_STRUCT_TO_SERIAL, sid);
+            nd.hdr[0] = s.type == T_NUMERIC ? 0 : 1;
+            nd.hdrLen = 1;
+            sid = newStream(T_SERIAL, 1, s.n, s.dev);
+            s = streams[sid];
+        }
+        const size_t kChunk = 1 << 15;
+        std::vector<uint32_t> chunks;
+        std::vector<uint64_t> offs;
+        if (s.n < 100000) {
+            chunks.push_back(sid);
+            offs.push_back(0);
+            offs.push_back(s.n);
+        } else {
+            size_t nb = (s.n + kChunk - 1) / kChunk;
+            ENode& nd = newNode(OZL_TR_SPLITN, sid);
+            nd.nbVO = (uint16_t)nb;
+            for (size_t c = 0; c < nb; c++) {
+                size_t a = c * kChunk, b = a + kChunk < s.n ? a + kChunk : s.n;
+                chunks.push_back(newStream(T_SERIAL, 1, b - a, s.dev + a));
+                offs.push_back(a);
+            }
+            offs.push_back(s.n);
+        }
+        uint32_t B = (uint32_t)chunks.size();
+        size_t maxBlock = s.n < kChunk ? s.n : (s.n < 100000 ? s.n : kChunk);
+        if (maxBlock == 0) maxBlock = 1;
+        size_t wsBytes = (size_t)B * (sizeof(ozl_eplan) + OZL_TAB_BYTES + OZL_AUX_BYTES + ozl_payload_cap(maxBlock)) + 256;
+        uint8_t* wsMem = alloc(wsBytes);
+        uint64_t* dOff = (uint64_t*)alloc(offs.size() * 8);
+        if (err) return err;
+        OZL_HIP_CHECK(hipMemcpyAsync(dOff, offs.data(), offs.size() * 8, hipMemcpyHostToDevice, st));
+        ozl_ews ws = ozl_ews_carve(wsMem, B, maxBlock);
+        int rc = ozl_entropy_encode_chunks(huf, s.dev, dOff, B, maxBlock, ws, st);
+        if (rc) return rc;
+        std::vector<ozl_eplan> plans(B);
+        OZL_HIP_CHECK(hipMemcpyAsync(plans.data(), ws.plans, (size_t)B * sizeof(ozl_eplan), hipMemcpyDeviceToHost, st));
+        OZL_HIP_CHECK(hipStreamSynchronize(st));
+        for (uint32_t c = 0; c < B; c++)
+            if (plans[c].status) return (int)plans[c].status;
+        // graph function: the FSE / Huffman nodes, chunk order
+        std::vector<uint32_t> side(B), bits(B);
+        for (uint32_t c = 0; c < B; c++) {
+            const ozl_eplan& p = plans[c];
+            uint32_t csid = chunks[c];
+            if (p.bk != OZL_BK_FSE && p.bk != OZL_BK_HUF) continue;  // routed to a successor graph
+            ENode& nd = newNode(p.bk == OZL_BK_FSE ? OZL_TR_FSE_V2 : OZL_TR_HUFFMAN_V2, csid);
+            int nbBytes = (ozl_next_pow2((uint64_t)p.n + 1) + 7) / 8;
+            nd.hdr[0] = p.nbStates;
+            for (int k = 0; k < nbBytes; k++) nd.hdr[1 + k] = (uint8_t)(p.n >> (8 * k));
+            nd.hdrLen = (uint8_t)(1 + nbBytes);
+            uint8_t* aux = ws.aux + (size_t)c * OZL_AUX_BYTES;
+            uint8_t* pay = ws.payload + (size_t)c * ws.payloadCap;
+            if (p.bk == OZL_BK_FSE) side[c] = newStream(T_NUMERIC, 2, p.maxSym + 1u, aux);
+            else side[c] = newStream(T_NUMERIC, 1, p.maxSym + 1u, aux);
+            bits[c] = newStream(T_SERIAL, 1, p.payload, pay);
+        }
+        // successors: chunks routed to a graph, then the node outputs
+        for (uint32_t c = 0; c < B; c++) {
+            const ozl_eplan& p = plans[c];
+            uint32_t csid = chunks[c];
+            bool tiny = streams[csid].n < (size_t)minStreamSize;
+            if (!tiny && (p.bk == OZL_BK_FSE || p.bk == OZL_BK_HUF)) continue;
+            uint8_t* aux = ws.aux + (size_t)c * OZL_AUX_BYTES;
+            uint8_t* pay = ws.payload + (size_t)c * ws.payloadCap;
+            if (tiny || p.bk == OZL_BK_STORE) { streams[csid].store = true; continue; }
+            switch (p.bk) {
+            case OZL_BK_CONSTANT: {
+                ENode& nd = newNode(OZL_TR_CONSTANT_SERIAL, csid);
+                nd.hdrLen = (uint8_t)ozl_varint(p.n, nd.hdr);
+                streams[newStream(T_SERIAL, 1, 1, aux)].store = true;
+                break;
+            }
+            case OZL_BK_FLATPACK: {
+                newNode(OZL_TR_FLATPACK, csid);
+                streams[newStream(T_SERIAL, 1, p.aux, aux)].store = true;
+                streams[newStream(T_SERIAL, 1, p.payload, pay)].store = true;
+                break;
+            }
+            case OZL_BK_BITPACK: {
+                ENode& nd = newNode(OZL_TR_BITPACK_SERIAL, csid);
+                size_t maxElts = (size_t)p.payload * 8 / p.aux;
+                nd.hdr[0] = (uint8_t)(p.aux - 1);
+                nd.hdrLen = 1;
+                if (maxElts > p.n) { nd.hdr[1] = (uint8_t)(maxElts - p.n); nd.hdrLen = 2; }
+                streams[newStream(T_SERIAL, 1, p.payload, pay)].store = true;
+                break;
+            }
+            default: return OZL_ERR_LOGIC;
+            }
+        }
+        for (uint32_t c = 0; c < B; c++) {
+            const ozl_eplan& p = plans[c];
+            uint32_t csid = chunks[c];
+            if (streams[csid].n < (size_t)minStreamSize) continue;
+            if (p.bk != OZL_BK_FSE && p.bk != OZL_BK_HUF) continue;
+            uint8_t* aux = ws.aux + (size_t)c * OZL_AUX_BYTES;
+            if (p.bk == OZL_BK_FSE) {
+                if (streams[side[c]].bytes() < (size_t)minStreamSize) {
+                    int rc = storeStream(side[c]);
+                    if (rc) return rc;
+                } else {
+                    newNode(OZL_TR_FSE_NCOUNT, side[c]);
+                    streams[newStream(T_SERIAL, 1, p.aux, aux)].store = true;
+                }
+            } else {
+                uint32_t m = p.maxSym + 1u;
+                if (m < (uint32_t)minStreamSize) {
+                    int rc = storeStream(side[c]);
+                    if (rc) return rc;
+                } else {
+                    ENode& cv = newNode(OZL_TR_CONV_NUM_TO_SERIAL_LE, side[c]);
+                    cv.hdr[0] = 0;
+                    cv.hdrLen = 1;
+                    uint32_t w = newStream(T_SERIAL, 1, m, aux);
+                    switch (p.wbk) {
+                    case OZL_BK_STORE: streams[w].store = true; break;
+                    case OZL_BK_CONSTANT: {
+                        ENode& k = newNode(OZL_TR_CONSTANT_SERIAL, w);
+                        k.hdrLen = (uint8_t)ozl_varint(m, k.hdr);
+                        streams[newStream(T_SERIAL, 1, 1, aux + OZL_AUX_W_PAYLOAD)].store = true;
+                        break;
+                    }
+                    case OZL_BK_FLATPACK:
+                        newNode(OZL_TR_FLATPACK, w);
+                        streams[newStream(T_SERIAL, 1, p.wAux, aux + OZL_AUX_W_SIDE)].store = true;
+                        streams[newStream(T_SERIAL, 1, p.wPayload, aux + OZL_AUX_W_PAYLOAD)].store = true;
+                        break;
+                    case OZL_BK_BITPACK: {
+                        ENode& k = newNode(OZL_TR_BITPACK_SERIAL, w);
+                        size_t maxElts = (size_t)p.wPayload * 8 / p.wAux8;
+                        k.hdr[0] = (uint8_t)(p.wAux8 - 1);
+                        k.hdrLen = 1;
+                        if (maxElts > m) { k.hdr[1] = (uint8_t)(maxElts - m); k.hdrLen = 2; }
+                        streams[newStream(T_SERIAL, 1, p.wPayload, aux + OZL_AUX_W_PAYLOAD)].store = true;
+                        break;
+                    }
+                    case OZL_BK_FSE: {
+                        ENode& k = newNode(OZL_TR_FSE_V2, w);
+                        int nbBytes = (ozl_next_pow2((uint64_t)m + 1) + 7) / 8;
+                        k.hdr[0] = 2;
+                        for (int q = 0; q < nbBytes; q++) k.hdr[1 + q] = (uint8_t)(m >> (8 * q));
+                        k.hdrLen = (uint8_t)(1 + nbBytes);
+                        uint32_t nc = newStream(T_NUMERIC, 2, p.wMaxSym + 1u, aux + OZL_AUX_W_SIDE);
+                        uint32_t wb = newStream(T_SERIAL, 1, p.wPayload, aux + OZL_AUX_W_PAYLOAD);
+                        if (streams[nc].bytes() < (size_t)minStreamSize) {
+                            int rc = storeStream(nc);
+                            if (rc) return rc;
+                        } else {
+                            newNode(OZL_TR_FSE_NCOUNT, nc);
+                            streams[newStream(T_SERIAL, 1, p.wAux, aux + OZL_AUX_W_SIDE)].store = true;
+                        }
+                        streams[wb].store = true;
+                        break;
+                    }
+                    default: return OZL_ERR_LOGIC;
+                    }
+                }
+            }
+            streams[bits[c]].store = true;
+        }
+        return 0;
+    }
+
+    // ---------------------------------------------------------------- graph walk
+    int runSuccessor(unsigned gid, uint32_t sid)
+    {
+        if (err) return err;
+        if (gid != GID_SERIAL_STORE && streams[sid].bytes() < (size_t)minStreamSize) return storeStream(sid);
+        return runSupervised(gid, sid);
+    }
+
+    int runSupervised(unsigned gid, uint32_t sid)
+    {
+        int mask = graphMask(gid);
+        if (!mask) return OZL_ERR_GRAPH_INVALID;
+        if (!(streams[sid].type & mask)) {
+            uint32_t o;
+            int rc = runConversion(sid, mask, &o);
+            if (rc) return rc;
+            sid = o;
+        }
+        switch (gid) {
+        case GID_SERIAL_STORE:
+        case GID_STORE: return storeStream(sid);
+        case GID_FSE: return runEntropy(sid, 0);
+        case GID_HUF: return runEntropy(sid, 1);
+        case GID_BITPACK: return runBitpack(sid);
+        default: break;
+        }
+        const GDef* g = custom(gid);
+        if (!g) return OZL_ERR_GRAPH_INVALID;
+        switch (g->kind) {
+        case GK_STATIC:
+        case GK_TRANSPOSE: {
+            std::vector<uint32_t> outs;
+            int rc = runNode(g->kind == GK_TRANSPOSE ? NID_TRANSPOSE_SPLIT : g->node, sid, outs);
+            if (rc) return rc;
+            if (g->succ.empty()) return OZL_ERR_GRAPH_INVALID;
+            for (size_t i = 0; i < outs.size(); i++) {
+                unsigned d = g->succ[i < g->succ.size() ? i : g->succ.size() - 1];
+                rc = runSuccessor(d, outs[i]);
+                if (rc) return rc;
+            }
+            return 0;
+        }
+        case GK_FUNCTION: return runFunction(g, sid);
+        case GK_BRUTE: return runBrute(g, sid);
+        default: return OZL_ERR_GRAPH_INVALID;
+        }
+    }
+
+    int runFunction(const GDef* g, uint32_t sid)
+    {
+        ZL_Graph_s graph;
+        graph.s = this;
+        graph.edges.emplace_back(new ZL_Edge_s{ &graph, sid, 0, false, false, { this, sid } });
+        ZL_Edge* in = graph.edges[0].get();
+        ZL_Report r = g->fn(&graph, &in, 1);
+        if (err) return err;
+        if (ZL_isError(r)) return (int)r._code;
+        // successors in edge order (cctx.c:656-700)
+        std::vector<std::pair<unsigned, uint32_t>> todo;
+        for (auto& e : graph.edges) {
+            if (e->consumed) continue;
+            if (!e->destSet) return OZL_ERR_SUCCESSOR_INVALID;
+            todo.push_back({ e->dest, e->sid });
+        }
+        for (auto& t : todo) {
+            int rc = runSuccessor(t.first, t.second);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+
+    // Exhaustive trial (selector_brute_force.c:9-50): every candidate compresses
+    // the input in a derived context with checksums off (cctx.c:1799-1865); the
+    // smallest frame wins, STORE when none beats the raw size.
+    int runBrute(const GDef* g, uint32_t sid);
+};
+
+// ---------------------------------------------------------------- edge API for function graphs
+extern "C" {
+
+const ZL_Input* ZL_Edge_getData(const ZL_Edge* e) { return &e->input; }
+size_t ZL_Input_numElts(const ZL_Input* in) { return in->s->streams[in->sid].n; }
+size_t ZL_Input_eltWidth(const ZL_Input* in) { return (size_t)in->s->streams[in->sid].width; }
+size_t ZL_Input_contentSize(const ZL_Input* in) { return in->s->streams[in->sid].bytes(); }
+ZL_Type ZL_Input_type(const ZL_Input* in) { return (ZL_Type)in->s->streams[in->sid].type; }
+
+ZL_Report ZL_Edge_setDestination(ZL_Edge* e, ZL_GraphID dst)
+{
+    if (e->consumed || e->destSet) return rep_err(ZL_ErrorCode_successor_invalid);
+    e->dest = dst.gid;
+    e->destSet = true;
+    return rep_ok(0);
+}
+
+ZL_Result_ZL_EdgeList ZL_Edge_runNode(ZL_Edge* e, ZL_NodeID node)
+{
+    ZL_Result_ZL_EdgeList res;
+    memset(&res, 0, sizeof(res));
+    ZL_Graph_s* g = e->g;
+    Session* s = g->s;
+    if (e->consumed || e->destSet) { res._code = ZL_ErrorCode_successor_invalid; return res; }
+    std::vector<uint32_t> outs;
+    int rc = s->runNode(node.nid, e->sid, outs);
+    if (rc) { s->err = rc; res._code = (ZL_ErrorCode)rc; return res; }
+    e->consumed = true;
+    g->lists.emplace_back(new std::vector<ZL_Edge*>());
+    std::vector<ZL_Edge*>& L = *g->lists.back();
+    for (uint32_t o : outs) {
+        g->edges.emplace_back(new ZL_Edge_s{ g, o, 0, false, false, { s, o } });
+        L.push_back(g->edges.back().get());
+    }
+    res._value._code = ZL_ErrorCode_no_error;
+    res._value._value.edges = L.data();
+    res._value._value.nbEdges = L.size();
+    return res;
+}
+
+ZL_Result_ZL_EdgeList ZL_Edge_runTransposeSplit(ZL_Edge* e, const ZL_Graph* graph)
+{
+    (void)graph;
+    return ZL_Edge_runNode(e, ZL_NodeID{ NID_TRANSPOSE_SPLIT });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- contexts
+struct ZL_CCtx_s {
+    const ZL_Compressor_s* comp = nullptr;
+    int contentChecksum = ZL_TernaryParam_auto;
+    int compressedChecksum = ZL_TernaryParam_auto;
+    int formatVersion = 0;
+    DevArena arena;
+    hipStream_t st = nullptr;
+    ~ZL_CCtx_s()
+    {
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+namespace {
+
+struct FrameParams {
+    int contentChecksum, compressedChecksum, formatVersion, minStreamSize;
+};
+
+FrameParams resolve(const ZL_CCtx_s* c)
+{
+    FrameParams f;
+    auto pick = [](int cctx, int comp) { return cctx != ZL_TernaryParam_auto ? cctx : comp; };
+    f.contentChecksum = pick(c->contentChecksum, c->comp->contentChecksum) != ZL_TernaryParam_disable;
+    f.compressedChecksum = pick(c->compressedChecksum, c->comp->compressedChecksum) != ZL_TernaryParam_disable;
+    f.formatVersion = c->formatVersion ? c->formatVersion : c->comp->formatVersion;
+    f.minStreamSize = c->comp->minStreamSize;
+    return f;
+}
+
+// Serialises the session's final graph; returns the frame header + chunk
+// header bytes and the stored streams (device) in frame order.
+int build_headers(Session& S, int inType, size_t inBytes, const FrameParams& fp, std::vector<uint8_t>& head,
+                  std::vector<const uint8_t*>& srcs, std::vector<uint64_t>& sizes)
+{
+    std::vector<ozl_fnode> fn(S.nodes.size());
+    for (size_t k = 0; k < S.nodes.size(); k++) {
+        const ENode& e = S.nodes[k];
+        fn[k].trid = e.trid;
+        fn[k].hdrLen = e.hdrLen;
+        fn[k].nbVO = e.nbVO;
+        fn[k].in = e.in;
+        fn[k].firstOut = e.firstOut;
+        memcpy(fn[k].hdr, e.hdr, sizeof(e.hdr));
+    }
+    std::vector<ozl_fstream> fs(S.streams.size());
+    for (size_t k = 0; k < S.streams.size(); k++) {
+        if (S.streams[k].bytes() > 0xFFFFFFFFull) return OZL_ERR_SRCSIZE_TOOSMALL;
+        fs[k].size = (uint32_t)S.streams[k].bytes();
+        fs[k].store = S.streams[k].store;
+        fs[k].src = 0;
+        fs[k].srcOff = 0;
+    }
+    if (S.nodes.size() >= 20000 || S.streams.size() + 1 >= 110000) return OZL_ERR_FORMAT_VERSION_UNSUPPORTED;
+    std::vector<int> order(S.streams.size());
+    size_t bound = 64 + S.nodes.size() * 24 + S.streams.size() * 10;
+    head.assign(bound, 0);
+    size_t fh = ozl_write_frame_header(head.data(), inType, inBytes, fp.contentChecksum, fp.compressedChecksum);
+    int nbStored = 0;
+    size_t ch = ozl_write_chunk_header(head.data() + fh, fn.data(), (int)fn.size(), fs.data(), (int)fs.size(),
+                                       order.data(), &nbStored);
+    head.resize(fh + ch);
+    srcs.clear();
+    sizes.clear();
+    for (int k = 0; k < nbStored; k++) {
+        srcs.push_back(S.streams[order[k]].dev);
+        sizes.push_back(S.streams[order[k]].bytes());
+    }
+    return 0;
+}
+
+}  // namespace
+
+int Session::runBrute(const GDef* g, uint32_t sid)
+{
+    size_t bestSize = streams[sid].bytes();
+    int best = -1;
+    for (size_t i = 0; i < g->succ.size(); i++) {
+        Session t;
+        t.comp = comp;
+        t.arena = arena;
+        t.st = st;
+        t.minStreamSize = minStreamSize;
+        EStream in = streams[sid];
+        in.store = false;
+        t.streams.push_back(in);
+        int rc = t.runSuccessor(g->succ[i], 0);
+        if (rc) continue;
+        FrameParams fp = { 0, 0, OZL_FORMAT_VERSION, minStreamSize };
+        std::vector<uint8_t> head;
+        std::vector<const uint8_t*> srcs;
+        std::vector<uint64_t> sizes;
+        if (build_headers(t, in.type == T_SERIAL ? OZL_T_SERIAL : in.type == T_STRUCT ? OZL_T_STRUCT : OZL_T_NUMERIC,
+                          in.bytes(), fp, head, srcs, sizes))
+            continue;
+        size_t total = head.size() + 1;
+        for (uint64_t s : sizes) total += s;
+        if (total < bestSize) { bestSize = total; best = (int)i; }
+    }
+    return runSuccessor(best < 0 ? (unsigned)GID_STORE : g->succ[best], sid);
+}
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* ZL_ErrorCode_toString(ZL_ErrorCode code)
+{
+    switch (code) {
+    case ZL_ErrorCode_no_error: return "No error detected";
+    case ZL_ErrorCode_GENERIC: return "Error (generic)";
+    case ZL_ErrorCode_srcSize_tooSmall: return "Source size too small";
+    case ZL_ErrorCode_srcSize_tooLarge: return "Source size too large";
+    case ZL_ErrorCode_dstCapacity_tooSmall: return "Destination capacity too small";
+    case ZL_ErrorCode_header_unknown: return "Unknown header";
+    case ZL_ErrorCode_corruption: return "Corruption detected";
+    case ZL_ErrorCode_compressedChecksumWrong: return "Compressed checksum mismatch";
+    case ZL_ErrorCode_contentChecksumWrong: return "Content checksum mismatch";
+    case ZL_ErrorCode_parameter_invalid: return "Parameter is invalid";
+    case ZL_ErrorCode_graph_invalid: return "Graph is invalid";
+    case ZL_ErrorCode_successor_invalid: return "Successor is invalid";
+    case ZL_ErrorCode_inputType_unsupported: return "Input type not supported";
+    case ZL_ErrorCode_node_invalid_input: return "Node input is invalid";
+    case ZL_ErrorCode_formatVersion_unsupported: return "Format version unsupported";
+    case ZL_ErrorCode_formatVersion_notSet: return "Format version not set";
+    case ZL_ErrorCode_allocation: return "Allocation error";
+    case ZL_ErrorCode_internalBuffer_tooSmall: return "Internal buffer too small";
+    default: return "Unspecified error code";
+    }
+}
+
+size_t ZL_compressBound(size_t n) { return n * 2 + 512 + 8; }
+
+ZL_Compressor* ZL_Compressor_create(void) { return new (std::nothrow) ZL_Compressor_s(); }
+void ZL_Compressor_free(ZL_Compressor* c) { delete c; }
+
+ZL_Report ZL_Compressor_setParameter(ZL_Compressor* c, ZL_CParam p, int v)
+{
+    if (!c) return rep_err(ZL_ErrorCode_parameter_invalid);
+    switch (p) {
+    case ZL_CParam_formatVersion: c->formatVersion = v; break;
+    case ZL_CParam_contentChecksum: c->contentChecksum = v; break;
+    case ZL_CParam_compressedChecksum: c->compressedChecksum = v; break;
+    case ZL_CParam_minStreamSize: c->minStreamSize = v; break;
+    case ZL_CParam_compressionLevel: break;
+    default: return rep_err(ZL_ErrorCode_parameter_invalid);
+    }
+    return rep_ok(0);
+}
+
+static ZL_GraphID add_graph(ZL_Compressor* c, GDef g)
+{
+    c->graphs.push_back(std::move(g));
+    return ZL_GraphID{ (unsigned)(GID_CUSTOM_BASE + c->graphs.size() - 1) };
+}
+
+static int node_input_mask(unsigned nid)
+{
+    switch (nid) {
+    case NID_DELTA: case NID_ZIGZAG: case NID_CONV_NUM_TO_STRUCT: case NID_CONV_NUM_TO_SERIAL: return T_NUMERIC;
+    case NID_TRANSPOSE_SPLIT: case NID_CONV_STRUCT_TO_SERIAL: return T_STRUCT;
+    default: return 0;
+    }
+}
+
+ZL_GraphID ZL_Compressor_registerStaticGraph_fromNode(ZL_Compressor* c, ZL_NodeID node, const ZL_GraphID* dsts,
+                                                      size_t nb)
+{
+    if (!c || !node_input_mask(node.nid) || nb == 0) return ZL_GRAPH_ILLEGAL;
+    GDef g{ GK_STATIC, node.nid, {}, nullptr, node_input_mask(node.nid) };
+    for (size_t i = 0; i < nb; i++) g.succ.push_back(dsts[i].gid);
+    return add_graph(c, g);
+}
+
+ZL_GraphID ZL_Compressor_registerStaticGraph_fromNode1o(ZL_Compressor* c, ZL_NodeID node, ZL_GraphID dst)
+{
+    return ZL_Compressor_registerStaticGraph_fromNode(c, node, &dst, 1);
+}
+
+// encode_transpose_binding.c:311-348: for format versions >= 11 the selector
+// always resolves to transpose_split followed by `successor` on every plane.
+ZL_GraphID ZL_Compressor_registerTransposeSplitGraph(ZL_Compressor* c, ZL_GraphID successor)
+{
+    if (!c) return ZL_GRAPH_ILLEGAL;
+    return add_graph(c, GDef{ GK_TRANSPOSE, NID_TRANSPOSE_SPLIT, { successor.gid }, nullptr, T_STRUCT });
+}
+
+ZL_GraphID ZL_Compressor_registerBruteForceSelectorGraph(ZL_Compressor* c, const ZL_GraphID* succ, size_t nb)
+{
+    if (!c || nb == 0) return ZL_GRAPH_ILLEGAL;
+    GDef g{ GK_BRUTE, 0, {}, nullptr, T_SERIAL | T_STRUCT | T_NUMERIC };
+    for (size_t i = 0; i < nb; i++) g.succ.push_back(succ[i].gid);
+    return add_graph(c, g);
+}
+
+ZL_GraphID ZL_Compressor_registerFunctionGraph(ZL_Compressor* c, const ZL_FunctionGraphDesc* d)
+{
+    if (!c || !d || !d->graph_f || d->nbInputs != 1 || !d->inputTypeMasks) return ZL_GRAPH_ILLEGAL;
+    return add_graph(c, GDef{ GK_FUNCTION, 0, {}, d->graph_f, (int)d->inputTypeMasks[0] });
+}
+
+ZL_Report ZL_Compressor_selectStartingGraphID(ZL_Compressor* c, ZL_GraphID g)
+{
+    if (!c || g.gid == 0) return rep_err(ZL_ErrorCode_graph_invalid);
+    c->start = g.gid;
+    return rep_ok(0);
+}
+
+ZL_CCtx* ZL_CCtx_create(void) { return new (std::nothrow) ZL_CCtx_s(); }
+void ZL_CCtx_free(ZL_CCtx* c) { delete c; }
+
+ZL_Report ZL_CCtx_refCompressor(ZL_CCtx* c, const ZL_Compressor* comp)
+{
+    if (!c || !comp) return rep_err(ZL_ErrorCode_parameter_invalid);
+    c->comp = comp;
+    return rep_ok(0);
+}
+
+ZL_Report ZL_CCtx_setParameter(ZL_CCtx* c, ZL_CParam p, int v)
+{
+    if (!c) return rep_err(ZL_ErrorCode_parameter_invalid);
+    switch (p) {
+    case ZL_CParam_formatVersion: c->formatVersion = v; break;
+    case ZL_CParam_contentChecksum: c->contentChecksum = v; break;
+    case ZL_CParam_compressedChecksum: c->compressedChecksum = v; break;
+    default: return rep_err(ZL_ErrorCode_parameter_invalid);
+    }
+    return rep_ok(0);
+}
+
+ZL_TypedRef* ZL_TypedRef_createSerial(const void* src, size_t n) { return new ZL_TypedRef_s{ src, T_SERIAL, 1, n }; }
+ZL_TypedRef* ZL_TypedRef_createStruct(const void* src, size_t w, size_t n) { return new ZL_TypedRef_s{ src, T_STRUCT, w, n }; }
+ZL_TypedRef* ZL_TypedRef_createNumeric(const void* src, size_t w, size_t n) { return new ZL_TypedRef_s{ src, T_NUMERIC, w, n }; }
+void ZL_TypedRef_free(ZL_TypedRef* t) { delete t; }
+
+ZL_Report ZL_CCtx_compressTypedRef(ZL_CCtx* c, void* dst, size_t dstCapacity, const ZL_TypedRef* in)
+{
+    if (!c || !in || !c->comp) return rep_err(ZL_ErrorCode_parameter_invalid);
+    FrameParams fp = resolve(c);
+    if (fp.formatVersion == 0) return rep_err(ZL_ErrorCode_formatVersion_notSet);
+    if (fp.formatVersion != (int)OZL_FORMAT_VERSION) return rep_err(ZL_ErrorCode_formatVersion_unsupported);
+    if (in->type == T_NUMERIC && in->width != 1 && in->width != 2 && in->width != 4 && in->width != 8)
+        return rep_err(ZL_ErrorCode_parameter_invalid);
+    if (!c->st && hipStreamCreate(&c->st) != hipSuccess) return rep_err(ZL_ErrorCode_allocation);
+    c->arena.reset();
+    size_t inBytes = in->n * in->width;
+    Session S;
+    S.comp = c->comp;
+    S.arena = &c->arena;
+    S.st = c->st;
+    S.minStreamSize = fp.minStreamSize;
+    uint8_t* dIn = S.alloc(inBytes + 16);
+    if (S.err) return rep_err(S.err);
+    if (inBytes && hipMemcpyAsync(dIn, in->ptr, inBytes, hipMemcpyHostToDevice, c->st) != hipSuccess)
+        return rep_err(ZL_ErrorCode_GENERIC);
+    S.newStream(in->type, (int)in->width, in->n, dIn);
+    int rc = S.runSuccessor(c->comp->start ? c->comp->start : GID_STORE, 0);
+    if (!rc) rc = S.err;
+    if (rc) return rep_err(rc);
+    std::vector<uint8_t> head;
+    std::vector<const uint8_t*> srcs;
+    std::vector<uint64_t> sizes;
+    rc = build_headers(S, frame_type(in->type), inBytes, fp, head, srcs, sizes);
+    if (rc) return rep_err(rc);
+    size_t total = head.size();
+    for (uint64_t s : sizes) total += s;
+    size_t fh = ozl_write_frame_header(head.data(), frame_type(in->type), inBytes, fp.contentChecksum,
+                                       fp.compressedChecksum);
+    size_t frameSize = total + (fp.contentChecksum ? 4 : 0) + (fp.compressedChecksum ? 4 : 0) + 1;
+    if (frameSize > dstCapacity) return rep_err(ZL_ErrorCode_dstCapacity_tooSmall);
+    // assemble on the device: header, stored streams, checksums
+    uint8_t* dFrame = S.alloc(frameSize + 16);
+    int n = (int)srcs.size();
+    std::vector<uint64_t> dstOffs(n);
+    size_t pos = head.size();
+    for (int k = 0; k < n; k++) { dstOffs[k] = pos; pos += sizes[k]; }
+    uint8_t* meta = S.alloc((size_t)n * 24 + 64);
+    size_t hs = ozl_xxh3_scratch_size(frameSize > inBytes ? frameSize : inBytes);
+    uint8_t* hscratch = S.alloc(hs);
+    uint64_t* dHash = (uint64_t*)S.alloc(16);
+    if (S.err) return rep_err(S.err);
+    if (hipMemcpyAsync(dFrame, head.data(), head.size(), hipMemcpyHostToDevice, c->st) != hipSuccess)
+        return rep_err(ZL_ErrorCode_GENERIC);
+    if (n) {
+        std::vector<uint64_t> m((size_t)n * 3);
+        for (int k = 0; k < n; k++) {
+            m[k] = (uint64_t)(uintptr_t)srcs[k];
+            m[n + k] = sizes[k];
+            m[2 * n + k] = dstOffs[k];
+        }
+        if (hipMemcpyAsync(meta, m.data(), m.size() * 8, hipMemcpyHostToDevice, c->st) != hipSuccess)
+            return rep_err(ZL_ErrorCode_GENERIC);
+        size_t mx = 0;
+        for (uint64_t z : sizes) mx = z > mx ? z : mx;
+        rc = ozl_gather_sized((const uint8_t* const*)meta, (const uint64_t*)meta + n, (const uint64_t*)meta + 2 * n, n,
+                              mx, dFrame, c->st);
+        if (rc) return rep_err(rc);
+    }
+    size_t p = pos;
+    if (fp.contentChecksum) {
+        rc = ozl_xxh3_device(dIn, inBytes, dHash, hscratch, hs, c->st);
+        if (rc) return rep_err(rc);
+        if (hipMemcpyAsync(dFrame + p, dHash, 4, hipMemcpyDeviceToDevice, c->st) != hipSuccess)
+            return rep_err(ZL_ErrorCode_GENERIC);
+        p += 4;
+    }
+    if (fp.compressedChecksum) {
+        rc = ozl_xxh3_device(dFrame + fh, p - fh, dHash + 1, hscratch, hs, c->st);
+        if (rc) return rep_err(rc);
+        if (hipMemcpyAsync(dFrame + p, dHash + 1, 4, hipMemcpyDeviceToDevice, c->st) != hipSuccess)
+            return rep_err(ZL_ErrorCode_GENERIC);
+        p += 4;
+    }
+    if (hipMemsetAsync(dFrame + p, 0, 1, c->st) != hipSuccess) return rep_err(ZL_ErrorCode_GENERIC);
+    p += 1;
+    if (hipMemcpyAsync(dst, dFrame, p, hipMemcpyDeviceToHost, c->st) != hipSuccess) return rep_err(ZL_ErrorCode_GENERIC);
+    if (hipStreamSynchronize(c->st) != hipSuccess) return rep_err(ZL_ErrorCode_GENERIC);
+    return rep_ok(p);
+}
+
+ZL_Report ZL_CCtx_compress(ZL_CCtx* c, void* dst, size_t cap, const void* src, size_t n)
+{
+    ZL_TypedRef_s t{ src, T_SERIAL, 1, n };
+    return ZL_CCtx_compressTypedRef(c, dst, cap, &t);
+}
+
+}  // extern "C"

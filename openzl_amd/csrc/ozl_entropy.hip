@@ -19,17 +19,7 @@
 #include "ozl_xxh3.h"
 #include "ozl_frame.h"
 #include "ozl_profile.h"
-
-#define OZL_TAB_BYTES 10240  // per block: FSE stateTable u16[4096] + tt u32[256]; HUF len[256] + val u16[256]
-
-struct ozl_ews {
-    ozl_eplan* plans;
-    uint8_t* tables;
-    uint8_t* aux;
-    uint8_t* payload;
-    uint64_t* frameOff;  // B+1
-    size_t payloadCap;
-};
+#include "ozl_entropy_api.h"
 
 namespace {
 
@@ -580,8 +570,7 @@ extern "C" {
 // maxBlock bytes.
 size_t ozl_entropy_compress_workspace(uint32_t B, size_t maxBlock)
 {
-    size_t cap = ((maxBlock + maxBlock / 128 + 256) + 15) & ~(size_t)15;
-    return (size_t)B * (sizeof(ozl_eplan) + OZL_TAB_BYTES + OZL_AUX_BYTES + cap) + (size_t)(B + 1) * 8 + 4096;
+    return (size_t)B * (sizeof(ozl_eplan) + OZL_TAB_BYTES + OZL_AUX_BYTES + ozl_payload_cap(maxBlock)) + 4096;
 }
 
 // Upper bound of one frame for a block of n bytes.
@@ -604,21 +593,12 @@ int ozl_entropy_compress_batch(int hufMode, const void* src, const uint64_t* off
     if (wsSize < ozl_entropy_compress_workspace(B, maxBlock)) return OZL_ERR_INTERNAL_BUFFER;
     if (dstCapacity < (size_t)B * ozl_entropy_frame_bound(maxBlock)) return OZL_ERR_DSTCAPACITY_TOOSMALL;
     hipStream_t st = (hipStream_t)stream;
-    uint8_t* w = (uint8_t*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
-    ozl_ews ws;
-    ws.payloadCap = ((maxBlock + maxBlock / 128 + 256) + 15) & ~(size_t)15;
-    ws.plans = (ozl_eplan*)w; w += (size_t)B * sizeof(ozl_eplan);
-    ws.tables = w; w += (size_t)B * OZL_TAB_BYTES;
-    ws.aux = w; w += (size_t)B * OZL_AUX_BYTES;
-    ws.payload = w; w += (size_t)B * ws.payloadCap;
+    ozl_ews ws = ozl_ews_carve(workspace, B, maxBlock);
     ws.frameOff = frameOff;
     const uint8_t* s = (const uint8_t*)src;
-    if (hufMode) OZL_LAUNCH("k_plan", k_plan<1>, dim3(B), dim3(kT), 0, st, s, off, B, ws);
-    else OZL_LAUNCH("k_plan", k_plan<0>, dim3(B), dim3(kT), 0, st, s, off, B, ws);
+    int rc = ozl_entropy_encode_chunks(hufMode, s, off, B, maxBlock, ws, st);
+    if (rc) return rc;
     OZL_LAUNCH("k_content_hash", k_content_hash, dim3(B), dim3(64), 0, st, s, off, B, ws);
-    if (hufMode) OZL_LAUNCH("k_huf_encode", k_huf_encode, dim3(B), dim3(kT), 0, st, s, off, B, ws);
-    else OZL_LAUNCH("k_fse_encode", k_fse_encode, dim3((B + kFseGroups - 1) / kFseGroups), dim3(64), 0, st, s, off, B, ws);
-    OZL_LAUNCH("k_pack_encode", k_pack_encode, dim3(B), dim3(kT), 0, st, s, off, B, ws);
     OZL_LAUNCH("k_frame_size", k_frame_size, dim3((B + 63) / 64), dim3(64), 0, st, B, ws);
     OZL_LAUNCH("k_scan_u64", k_scan_u64, dim3(1), dim3(kT), 0, st, frameOff, B);
     OZL_LAUNCH("k_frame_write", k_frame_write, dim3(B), dim3(kT), 0, st, s, off, B, ws, (uint8_t*)dst);
@@ -628,3 +608,17 @@ int ozl_entropy_compress_batch(int hufMode, const void* src, const uint64_t* off
 }
 
 }  // extern "C"
+
+int ozl_entropy_encode_chunks(int hufMode, const uint8_t* s, const uint64_t* off, uint32_t B, size_t maxBlock,
+                              const ozl_ews& ws, hipStream_t st)
+{
+    if (B == 0) return OZL_OK;
+    if (maxBlock >= 100000) return OZL_ERR_PARAMETER_INVALID;
+    if (hufMode) OZL_LAUNCH("k_plan", k_plan<1>, dim3(B), dim3(kT), 0, st, s, off, B, ws);
+    else OZL_LAUNCH("k_plan", k_plan<0>, dim3(B), dim3(kT), 0, st, s, off, B, ws);
+    if (hufMode) OZL_LAUNCH("k_huf_encode", k_huf_encode, dim3(B), dim3(kT), 0, st, s, off, B, ws);
+    else OZL_LAUNCH("k_fse_encode", k_fse_encode, dim3((B + kFseGroups - 1) / kFseGroups), dim3(64), 0, st, s, off, B, ws);
+    OZL_LAUNCH("k_pack_encode", k_pack_encode, dim3(B), dim3(kT), 0, st, s, off, B, ws);
+    OZL_HIP_CHECK(hipGetLastError());
+    return OZL_OK;
+}

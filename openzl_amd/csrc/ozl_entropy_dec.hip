@@ -32,7 +32,9 @@ struct ozl_dplan {
     uint32_t chunkOff;         // start of the chunk inside the frame
     uint32_t csumOff;          // position of the content checksum
     uint32_t status;
-    uint32_t pad[5];
+    uint32_t pad0b;
+    uint64_t srcPtr;           // main payload (absolute device address)
+    uint64_t outPtr;           // decoded output (absolute device address)
 };
 
 struct ozl_dws {
@@ -90,10 +92,6 @@ __device__ int decode_weights(const uint8_t* fr, const ozl_fnode* nodes, int nn,
                               const uint32_t* soff, uint32_t m, DLds& L)
 {
     // stream ids follow ozl_entropy_layout: 1 weights, 3 serial weights, 4.. nested outputs
-    if (nn == 1) {  // weights stored raw (fewer than minStreamSize bytes)
-        for (uint32_t i = 0; i < m; i++) L.w[i] = fr[soff[1] + i];
-        return 0;
-    }
     if (nn == 2) {  // conversion then STORE
         for (uint32_t i = 0; i < m; i++) L.w[i] = fr[soff[3] + i];
         return 0;
@@ -139,15 +137,15 @@ __device__ int decode_weights(const uint8_t* fr, const ozl_fnode* nodes, int nn,
     if (t == OZL_TR_FSE_V2) {
         unsigned ms, tl;
         int16_t wn[256];
-        if (nn == 4) {  // NCount encoded
+        if (nn == 4 && nodes[3].trid == OZL_TR_FSE_NCOUNT) {  // NCount encoded
             if (ozl_fse_read_ncount(wn, &ms, &tl, fr + soff[6], st[6].size) < 0) return -1;
-        } else {        // raw int16 counts
-            uint32_t cnt = st[4].size / 2;
+        } else {        // raw int16 counts, stored after a num->serial conversion
+            uint32_t cnt = st[6].size / 2;
             if (cnt == 0) return -1;
             uint32_t sum = 0;
             for (uint32_t s = 0; s < 256; s++) wn[s] = 0;
             for (uint32_t s = 0; s < cnt; s++) {
-                wn[s] = (int16_t)(fr[soff[4] + 2 * s] | (fr[soff[4] + 2 * s + 1] << 8));
+                wn[s] = (int16_t)(fr[soff[6] + 2 * s] | (fr[soff[6] + 2 * s + 1] << 8));
                 sum += wn[s] == -1 ? 1 : (uint32_t)wn[s];
             }
             if (sum == 0 || (sum & (sum - 1))) return -1;
@@ -162,7 +160,8 @@ __device__ int decode_weights(const uint8_t* fr, const ozl_fnode* nodes, int nn,
 }
 
 __global__ void __launch_bounds__(kT) k_dparse(const uint8_t* __restrict__ frames, const uint64_t* __restrict__ foff,
-                                               uint32_t B, const uint64_t* __restrict__ doff, ozl_dws ws)
+                                               uint32_t B, uint8_t* __restrict__ dst, const uint64_t* __restrict__ doff,
+                                               ozl_dws ws)
 {
     __shared__ DLds L;
     __shared__ uint16_t lut[4096];
@@ -235,7 +234,7 @@ __global__ void __launch_bounds__(kT) k_dparse(const uint8_t* __restrict__ frame
             else if (t0 == OZL_TR_CONSTANT_SERIAL && nn == 1) p.bk = OZL_BK_CONSTANT;
             else if (t0 == OZL_TR_FLATPACK && nn == 1) p.bk = OZL_BK_FLATPACK;
             else if (t0 == OZL_TR_BITPACK_SERIAL && nn == 1) p.bk = OZL_BK_BITPACK;
-            else if (t0 == OZL_TR_FSE_V2 && nn <= 2) p.bk = OZL_BK_FSE;
+            else if (t0 == OZL_TR_FSE_V2 && nn == 2) p.bk = OZL_BK_FSE;
             else if (t0 == OZL_TR_HUFFMAN_V2) p.bk = OZL_BK_HUF;
             else ok = false;
         }
@@ -273,18 +272,20 @@ __global__ void __launch_bounds__(kT) k_dparse(const uint8_t* __restrict__ frame
             case OZL_BK_BITPACK: p.aux = (uint16_t)((nodes[0].hdr[0] & 63) + 1); p.payload = storedSize[0]; break;
             case OZL_BK_FSE:
                 p.nbStates = nodes[0].hdr[0];
-                if (nn == 2) { p.aux = (uint16_t)storedSize[0]; p.payload = storedSize[1]; p.maxSym = 4; }
-                else { p.payload = storedSize[0]; p.maxSym = (uint8_t)(storedSize[1] / 2 - 1); }
+                // stored list: [ncount or raw counts, bits]
+                p.payload = storedSize[1];
+                if (nodes[1].trid == OZL_TR_FSE_NCOUNT) { p.aux = (uint16_t)storedSize[0]; p.maxSym = 4; }
+                else p.maxSym = (uint8_t)(storedSize[0] / 2 - 1);
                 break;
             case OZL_BK_HUF: {
                 p.nbStates = nodes[0].hdr[0];
                 // maxSym from the weights stream size (stored raw, or the conversion's stream)
                 // stored list: [bits, weights] when the weights are raw, else
                 // [nested streams..., bits]
-                p.payload = nn == 1 ? storedSize[0] : storedSize[nbStored - 1];
-                if (nn == 1) p.maxSym = (uint8_t)(storedSize[1] - 1);
-                else p.maxSym = 255;  // fixed below from the nested header
-                if (nn >= 2) {
+                p.payload = storedSize[nbStored - 1];
+                p.maxSym = 255;  // fixed below from the nested header
+                if (nn < 2 || nodes[1].trid != OZL_TR_CONV_NUM_TO_SERIAL_LE) ok = false;
+                if (ok) {
                     uint8_t t2 = nn > 2 ? nodes[2].trid : 0;
                     if (nn == 2) { p.wbk = OZL_BK_STORE; p.maxSym = (uint8_t)(storedSize[0] - 1); }
                     else if (t2 == OZL_TR_CONSTANT_SERIAL) {
@@ -298,8 +299,10 @@ __global__ void __launch_bounds__(kT) k_dparse(const uint8_t* __restrict__ frame
                         p.wbk = OZL_BK_BITPACK; p.wAux8 = (uint8_t)((nodes[2].hdr[0] & 63) + 1); p.wPayload = (uint16_t)storedSize[0];
                     } else if (t2 == OZL_TR_FSE_V2) {
                         p.wbk = OZL_BK_FSE;
-                        if (nn == 4) { p.wAux = (uint16_t)storedSize[0]; p.wPayload = (uint16_t)storedSize[1]; p.wMaxSym = 4; }
-                        else { p.wPayload = (uint16_t)storedSize[0]; p.wMaxSym = (uint8_t)(storedSize[1] / 2 - 1); }
+                        // stored list: [ncount or raw counts, nested bits, bits]
+                        p.wPayload = (uint16_t)storedSize[1];
+                        if (nn == 4 && nodes[3].trid == OZL_TR_FSE_NCOUNT) { p.wAux = (uint16_t)storedSize[0]; p.wMaxSym = 4; }
+                        else p.wMaxSym = (uint8_t)(storedSize[0] / 2 - 1);
                     } else ok = false;
                     if (t2 == OZL_TR_FLATPACK || t2 == OZL_TR_BITPACK_SERIAL || t2 == OZL_TR_FSE_V2) {
                         uint64_t m = 0;  // from the nested node header or sizes
@@ -343,16 +346,17 @@ __global__ void __launch_bounds__(kT) k_dparse(const uint8_t* __restrict__ frame
                 if (plen == 0 || dataPos + plen > flen || pk[plen - 1] == 0 || nb == 0) ok = false;
                 else p.maxSym = (uint8_t)(((uint64_t)(plen - 1) * 8 + ozl_hb32(pk[plen - 1])) / nb - 1);
             }
-            if (p.bk == OZL_BK_FSE && nn == 2 && dataPos + storedSize[0] > flen) ok = false;
-            if (p.bk == OZL_BK_HUF && p.wbk == OZL_BK_FSE && nn == 4 && dataPos + storedSize[0] > flen) ok = false;
-            if (ok && p.bk == OZL_BK_FSE && nn == 2) {
+            bool mainNc = p.bk == OZL_BK_FSE && nodes[1].trid == OZL_TR_FSE_NCOUNT;
+            bool nestedNc = p.bk == OZL_BK_HUF && p.wbk == OZL_BK_FSE && nn == 4 && nodes[3].trid == OZL_TR_FSE_NCOUNT;
+            if ((mainNc || nestedNc) && dataPos + storedSize[0] > flen) ok = false;
+            if (ok && mainNc) {
                 // maxSym from the NCount itself
                 unsigned ms, tl;
                 long used = ozl_fse_read_ncount(L.norm, &ms, &tl, fr + dataPos, storedSize[0]);
                 if (used < 0) ok = false;
                 else p.maxSym = (uint8_t)ms;
             }
-            if (ok && p.bk == OZL_BK_HUF && p.wbk == OZL_BK_FSE && nn == 4) {
+            if (ok && nestedNc) {
                 unsigned ms, tl;
                 long used = ozl_fse_read_ncount(L.norm, &ms, &tl, fr + dataPos, storedSize[0]);
                 if (used < 0) ok = false;
@@ -396,13 +400,13 @@ __global__ void __launch_bounds__(kT) k_dparse(const uint8_t* __restrict__ frame
             case OZL_BK_FSE: {
                 d.payOff = soff[2]; d.payLen = st[2].size; d.nbStates = p.nbStates;
                 unsigned ms, tl;
-                if (nn == 2) {
+                if (nodes[1].trid == OZL_TR_FSE_NCOUNT) {
                     if (ozl_fse_read_ncount(L.norm, &ms, &tl, fr + soff[3], st[3].size) < 0) { d.status = OZL_ERR_CORRUPTION; break; }
-                } else {
-                    uint32_t cnt = st[1].size / 2, sum = 0;
+                } else {  // raw int16 counts after a num->serial conversion
+                    uint32_t cnt = st[3].size / 2, sum = 0;
                     for (int s = 0; s < 256; s++) L.norm[s] = 0;
                     for (uint32_t s = 0; s < cnt; s++) {
-                        L.norm[s] = (int16_t)(fr[soff[1] + 2 * s] | (fr[soff[1] + 2 * s + 1] << 8));
+                        L.norm[s] = (int16_t)(fr[soff[3] + 2 * s] | (fr[soff[3] + 2 * s + 1] << 8));
                         sum += L.norm[s] == -1 ? 1 : (uint32_t)L.norm[s];
                     }
                     if (cnt == 0 || sum == 0 || (sum & (sum - 1))) { d.status = OZL_ERR_CORRUPTION; break; }
@@ -429,8 +433,10 @@ __global__ void __launch_bounds__(kT) k_dparse(const uint8_t* __restrict__ frame
             default: break;
             }
         }
+        d.srcPtr = (uint64_t)(uintptr_t)(fr + d.payOff);
+        d.outPtr = (uint64_t)(uintptr_t)(dst + doff[b]);
         ws.plans[b] = d;
-        if (doff && d.status == 0 && doff[b + 1] - doff[b] != d.n) ws.plans[b].status = OZL_ERR_DSTCAPACITY_TOOSMALL;
+        if (d.status == 0 && doff[b + 1] - doff[b] != d.n) ws.plans[b].status = OZL_ERR_DSTCAPACITY_TOOSMALL;
     }
     __syncthreads();
     if (sFill[1]) {  // Huffman lookup: entry = symbol | length << 8, indexed by the next tl bits
@@ -463,9 +469,7 @@ __device__ inline uint32_t read_bits_back(const uint8_t* s, long end, uint32_t n
 
 constexpr int kDecGroups = 16;
 
-__global__ void __launch_bounds__(64) k_fse_decode(const uint8_t* __restrict__ frames, const uint64_t* __restrict__ foff,
-                                                   uint32_t B, uint8_t* __restrict__ dst, const uint64_t* __restrict__ doff,
-                                                   ozl_dws ws)
+__global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
 {
     __shared__ uint32_t sDT[kDecGroups][2048];
     const int lane = threadIdx.x & 63, g = lane >> 2, q = lane & 3;
@@ -483,8 +487,8 @@ __global__ void __launch_bounds__(64) k_fse_decode(const uint8_t* __restrict__ f
     __syncthreads();
     if (!active) return;
     const uint32_t* DT = sDT[g];
-    const uint8_t* s = frames + foff[b] + d.payOff;
-    uint8_t* out = dst + doff[b];
+    const uint8_t* s = (const uint8_t*)(uintptr_t)d.srcPtr;
+    uint8_t* out = (uint8_t*)(uintptr_t)d.outPtr;
     const uint32_t S = d.nbStates, n = d.n, tl = d.tl;
     const int base = lane & ~3;
     if (d.payLen == 0 || s[d.payLen - 1] == 0) { if (q == 0) ws.plans[b].status = OZL_ERR_CORRUPTION; return; }
@@ -513,9 +517,7 @@ __global__ void __launch_bounds__(64) k_fse_decode(const uint8_t* __restrict__ f
     if (q == 0 && pos != 0) ws.plans[b].status = OZL_ERR_CORRUPTION;
 }
 
-__global__ void __launch_bounds__(64) k_huf_decode(const uint8_t* __restrict__ frames, const uint64_t* __restrict__ foff,
-                                                   uint32_t B, uint8_t* __restrict__ dst, const uint64_t* __restrict__ doff,
-                                                   ozl_dws ws)
+__global__ void __launch_bounds__(64) k_huf_decode(uint32_t B, ozl_dws ws)
 {
     __shared__ uint16_t sL[kDecGroups][2048];
     const int lane = threadIdx.x & 63, g = lane >> 2, q = lane & 3;
@@ -532,8 +534,8 @@ __global__ void __launch_bounds__(64) k_huf_decode(const uint8_t* __restrict__ f
     }
     __syncthreads();
     if (!active) return;
-    const uint8_t* s = frames + foff[b] + d.payOff;
-    uint8_t* out = dst + doff[b];
+    const uint8_t* s = (const uint8_t*)(uintptr_t)d.srcPtr;
+    uint8_t* out = (uint8_t*)(uintptr_t)d.outPtr;
     const uint32_t n = d.n, tl = d.tl;
     const int x4 = d.nbStates;
     if (!x4 && q > 0) return;
@@ -624,6 +626,93 @@ __global__ void __launch_bounds__(64) k_dverify(const uint8_t* __restrict__ fram
     if (threadIdx.x == 0) status[b] = st;
 }
 
+// ---------------------------------------------------------------- items of the graph decoder
+// One fse_v2 / huffman_v2 decoder of a generic frame: its side stream
+// (normalized counts as int16, or Huffman weights as u8, both already
+// regenerated in device memory), its bit stream and its output.
+}  // namespace
+
+struct ozl_ditem {
+    uint64_t side;
+    uint64_t bits;
+    uint64_t out;
+    uint32_t sideN;    // number of side elements
+    uint32_t bitsLen;
+    uint32_t n;        // symbols to decode
+    uint8_t huf;       // 0 FSE, 1 Huffman
+    uint8_t nbStates;  // FSE states, or the Huffman 4-stream flag
+    uint8_t pad[2];
+};
+
+namespace {
+
+__global__ void __launch_bounds__(64) k_prep_items(const ozl_ditem* __restrict__ items, uint32_t B, ozl_dws ws)
+{
+    __shared__ uint8_t cells[4096];
+    __shared__ int16_t norm[256];
+    __shared__ uint8_t w[256];
+    __shared__ uint8_t len[256];
+    __shared__ uint16_t val[256];
+    __shared__ int sTl;
+    uint32_t b = blockIdx.x;
+    if (b >= B) return;
+    ozl_ditem it = items[b];
+    uint8_t* tab = ws.tables + (size_t)b * OZL_DTAB_BYTES;
+    if (threadIdx.x == 0) {
+        ozl_dplan d;
+        uint32_t* dw = reinterpret_cast<uint32_t*>(&d);
+        for (size_t k = 0; k < sizeof(d) / 4; k++) dw[k] = 0;
+        d.n = it.n;
+        d.payLen = it.bitsLen;
+        d.nbStates = it.nbStates;
+        d.srcPtr = it.bits;
+        d.outPtr = it.out;
+        d.bk = it.huf ? OZL_BK_HUF : OZL_BK_FSE;
+        sTl = 0;
+        const uint8_t* side = (const uint8_t*)(uintptr_t)it.side;
+        if (!it.huf) {
+            uint32_t sum = 0;
+            for (int s = 0; s < 256; s++) norm[s] = 0;
+            for (uint32_t s = 0; s < it.sideN && s < 256; s++) {
+                norm[s] = (int16_t)(side[2 * s] | (side[2 * s + 1] << 8));
+                sum += norm[s] == -1 ? 1 : (uint32_t)(uint16_t)norm[s];
+            }
+            unsigned tl = sum ? (unsigned)ozl_hb32(sum) : 0;
+            if (it.sideN == 0 || it.sideN > 256 || sum == 0 || (sum & (sum - 1)) || tl < OZL_FSE_MIN_TABLELOG ||
+                tl > 11 || (it.nbStates != 2 && it.nbStates != 4))
+                d.status = OZL_ERR_CORRUPTION;
+            else {
+                d.tl = (uint8_t)tl;
+                if (ozl_fse_build_dtable(reinterpret_cast<uint32_t*>(tab), cells, norm, it.sideN - 1, tl))
+                    d.status = OZL_ERR_CORRUPTION;
+            }
+        } else {
+            for (uint32_t s = 0; s < it.sideN && s < 256; s++) w[s] = side[s];
+            int tl = it.sideN <= 256 ? ozl_huf_from_weights(len, val, w, it.sideN) : -1;
+            if (tl < 1 || tl > 11 || it.nbStates > 1 || (it.nbStates == 1 && it.n < 12)) d.status = OZL_ERR_CORRUPTION;
+            else { d.tl = (uint8_t)tl; sTl = tl; }
+        }
+        ws.plans[b] = d;
+    }
+    __syncthreads();
+    int tl = sTl;
+    if (tl) {
+        uint16_t* lut = reinterpret_cast<uint16_t*>(tab);
+        for (int s = threadIdx.x; s < 256; s += 64) {
+            uint32_t l = len[s];
+            if (!l || s >= (int)it.sideN) continue;
+            uint32_t first = (uint32_t)val[s] << (tl - l), cnt = 1u << (tl - l);
+            for (uint32_t k = 0; k < cnt; k++) lut[first + k] = (uint16_t)(s | (l << 8));
+        }
+    }
+}
+
+__global__ void k_item_status(uint32_t B, ozl_dws ws, int* status)
+{
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < B) status[b] = (int)ws.plans[b].status;
+}
+
 }  // namespace
 
 extern "C" {
@@ -653,10 +742,10 @@ int ozl_entropy_decompress_batch(const void* frames, const uint64_t* foff, uint3
     ws.tables = w;
     const uint8_t* f = (const uint8_t*)frames;
     uint8_t* o = (uint8_t*)dst;
-    OZL_LAUNCH("k_dparse", k_dparse, dim3(B), dim3(kT), 0, st, f, foff, B, doff, ws);
+    OZL_LAUNCH("k_dparse", k_dparse, dim3(B), dim3(kT), 0, st, f, foff, B, o, doff, ws);
     unsigned gq = (B + kDecGroups - 1) / kDecGroups;
-    OZL_LAUNCH("k_fse_decode", k_fse_decode, dim3(gq), dim3(64), 0, st, f, foff, B, o, doff, ws);
-    OZL_LAUNCH("k_huf_decode", k_huf_decode, dim3(gq), dim3(64), 0, st, f, foff, B, o, doff, ws);
+    OZL_LAUNCH("k_fse_decode", k_fse_decode, dim3(gq), dim3(64), 0, st, B, ws);
+    OZL_LAUNCH("k_huf_decode", k_huf_decode, dim3(gq), dim3(64), 0, st, B, ws);
     OZL_LAUNCH("k_pack_decode", k_pack_decode, dim3(B), dim3(kT), 0, st, f, foff, B, o, doff, ws);
     OZL_LAUNCH("k_dverify", k_dverify, dim3(B), dim3(64), 0, st, f, foff, B, (const uint8_t*)o, doff, ws, status);
     OZL_HIP_CHECK(hipGetLastError());
@@ -664,3 +753,25 @@ int ozl_entropy_decompress_batch(const void* frames, const uint64_t* foff, uint3
 }
 
 }  // extern "C"
+
+size_t ozl_entropy_items_workspace(uint32_t B) { return ozl_entropy_decompress_workspace(B); }
+
+// Decodes B fse_v2 / huffman_v2 items (device array) of a generic frame;
+// status (device, B ints) receives 0 or an error code per item.
+int ozl_entropy_decode_items(const ozl_ditem* items, uint32_t B, int* status, void* workspace, size_t wsSize,
+                             hipStream_t st)
+{
+    if (B == 0) return OZL_OK;
+    if (wsSize < ozl_entropy_decompress_workspace(B)) return OZL_ERR_INTERNAL_BUFFER;
+    uint8_t* w = (uint8_t*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
+    ozl_dws ws;
+    ws.plans = (ozl_dplan*)w; w += (size_t)B * sizeof(ozl_dplan);
+    ws.tables = w;
+    OZL_LAUNCH("k_prep_items", k_prep_items, dim3(B), dim3(64), 0, st, items, B, ws);
+    unsigned gq = (B + kDecGroups - 1) / kDecGroups;
+    OZL_LAUNCH("k_fse_decode", k_fse_decode, dim3(gq), dim3(64), 0, st, B, ws);
+    OZL_LAUNCH("k_huf_decode", k_huf_decode, dim3(gq), dim3(64), 0, st, B, ws);
+    OZL_LAUNCH("k_item_status", k_item_status, dim3((B + 63) / 64), dim3(64), 0, st, B, ws, status);
+    OZL_HIP_CHECK(hipGetLastError());
+    return OZL_OK;
+}

@@ -227,7 +227,11 @@ OZL_HD int ozl_entropy_layout(const ozl_eplan* p, ozl_fnode* nodes, ozl_fstream*
         int bitsId = ns;
         st[ns++] = { p->payload, 1, OZL_SRC_PAYLOAD, 0 };
         if (st[ncountId].size < OZL_MIN_STREAM_SIZE) {
-            st[ncountId].store = 1;  // raw int16 counts, kept in aux at 0
+            // stored through STORE1 -> serial_store: num->serial conversion first
+            ozl_fnode* c = node(OZL_TR_CONV_NUM_TO_SERIAL_LE, (uint32_t)ncountId, 1, 0);
+            c->hdr[0] = 1;  // ZL_nextPow2(eltWidth = 2)
+            c->hdrLen = 1;
+            st[ns++] = { st[ncountId].size, 1, OZL_SRC_AUX, 0 };  // raw int16 counts, in aux at 0
         } else {
             node(OZL_TR_FSE_NCOUNT, (uint32_t)ncountId, 1, 0);
             st[ns++] = { p->aux, 1, OZL_SRC_AUX, 0 };
@@ -242,15 +246,17 @@ OZL_HD int ozl_entropy_layout(const ozl_eplan* p, ozl_fnode* nodes, ozl_fstream*
         int wId = ns;
         st[ns++] = { m, 0, OZL_SRC_AUX, 0 };                  // weights (numeric u8)
         st[ns++] = { p->payload, 1, OZL_SRC_PAYLOAD, 0 };     // bits
-        if (m < OZL_MIN_STREAM_SIZE) {
-            st[wId].store = 1;
-            break;
-        }
+        // both the minStreamSize store and the FSE graph take the weights
+        // through a num->serial conversion
         ozl_fnode* c = node(OZL_TR_CONV_NUM_TO_SERIAL_LE, (uint32_t)wId, 1, 0);
         c->hdr[0] = 0;  // ZL_nextPow2(eltWidth = 1)
         c->hdrLen = 1;
         int sId = ns;
         st[ns++] = { m, 0, OZL_SRC_AUX, 0 };
+        if (m < OZL_MIN_STREAM_SIZE) {
+            st[sId].store = 1;
+            break;
+        }
         switch (p->wbk) {
         case OZL_BK_STORE: st[sId].store = 1; break;
         case OZL_BK_CONSTANT: {
@@ -280,7 +286,10 @@ OZL_HD int ozl_entropy_layout(const ozl_eplan* p, ozl_fnode* nodes, ozl_fstream*
             st[ns++] = { 2u * (p->wMaxSym + 1u), 0, OZL_SRC_AUX, OZL_AUX_W_SIDE };
             st[ns++] = { p->wPayload, 1, OZL_SRC_AUX, OZL_AUX_W_PAYLOAD };
             if (st[nc].size < OZL_MIN_STREAM_SIZE) {
-                st[nc].store = 1;
+                ozl_fnode* cv = node(OZL_TR_CONV_NUM_TO_SERIAL_LE, (uint32_t)nc, 1, 0);
+                cv->hdr[0] = 1;
+                cv->hdrLen = 1;
+                st[ns++] = { st[nc].size, 1, OZL_SRC_AUX, OZL_AUX_W_SIDE };
             } else {
                 node(OZL_TR_FSE_NCOUNT, (uint32_t)nc, 1, 0);
                 st[ns++] = { p->wAux, 1, OZL_SRC_AUX, OZL_AUX_W_SIDE };

@@ -18,6 +18,7 @@ enum {
     OZL_ERR_PARAMETER_INVALID = 21,
     OZL_ERR_GRAPH_INVALID = 30,
     OZL_ERR_INPUT_TYPE_UNSUPPORTED = 43,
+    OZL_ERR_SUCCESSOR_INVALID = 40,
     OZL_ERR_NODE_INVALID_INPUT = 55,
     OZL_ERR_FORMAT_VERSION_UNSUPPORTED = 60,
     OZL_ERR_ALLOCATION = 70,
