@@ -41,65 +41,7 @@ SEED = 0x5EED0002
 
 
 # ---------------------------------------------------------------- data
-def zipf_table(seed, s=1.1, card=256):
-    """uint32 inverse-CDF thresholds and the rank -> symbol permutation."""
-    p = 1.0 / np.arange(1, card + 1, dtype=np.float64) ** s
-    cdf = np.cumsum(p / p.sum())
-    thr = np.minimum(np.floor(cdf * 4294967296.0), 4294967295).astype(np.uint64)
-    thr[-1] = 4294967296
-    rng = np.random.default_rng(seed)
-    perm = rng.permutation(card).astype(np.uint8)
-    return thr, perm
-
-
-def splitmix_np(idx, seed):
-    with np.errstate(over="ignore"):
-        z = idx.astype(np.uint64) + np.uint64((seed * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF)
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        z = z ^ (z >> np.uint64(31))
-    return z
-
-
-def gen_blocks_np(first_block, n_blocks, seed=SEED):
-    thr, perm = zipf_table(seed)
-    idx = np.arange(first_block * BLOCK, (first_block + n_blocks) * BLOCK, dtype=np.uint64)
-    u = splitmix_np(idx, seed) >> np.uint64(32)
-    rank = np.searchsorted(thr, u, side="right")
-    return perm[rank]
-
-
-def gen_blocks_torch(torch, n_blocks, seed=SEED, device="cuda"):
-    """Same values as gen_blocks_np, generated on the GPU in slices."""
-    thr, perm = zipf_table(seed)
-    thr_t = torch.from_numpy(thr.astype(np.int64)).to(device)
-    perm_t = torch.from_numpy(perm).to(device)
-    total = n_blocks * BLOCK
-    out = torch.empty(total, dtype=torch.uint8, device=device)
-    M64 = (1 << 64) - 1
-
-    def s64(v):
-        v &= M64
-        return v - (1 << 64) if v >= (1 << 63) else v
-
-    add = s64(seed * 0x9E3779B97F4A7C15)
-    m1 = s64(0xBF58476D1CE4E5B9)
-    m2 = s64(0x94D049BB133111EB)
-
-    def lsr(x, k):  # logical shift right on int64
-        return (x >> k) & ((1 << (64 - k)) - 1)
-
-    step = 1 << 26
-    for a in range(0, total, step):
-        b = min(total, a + step)
-        z = torch.arange(a, b, dtype=torch.int64, device=device) + add
-        z = (z ^ lsr(z, 30)) * m1
-        z = (z ^ lsr(z, 27)) * m2
-        z = z ^ lsr(z, 31)
-        u = lsr(z, 32)
-        rank = torch.searchsorted(thr_t, u, right=True)
-        out[a:b] = perm_t[rank]
-    return out
+from openzl_amd.datagen import zipf_blocks_np as gen_blocks_np, zipf_blocks_torch as gen_blocks_torch  # noqa: E402
 
 
 # ---------------------------------------------------------------- CPU baseline

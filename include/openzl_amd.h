@@ -9,7 +9,7 @@
  *    the graphs of the hot path (ZL_NODE_DELTA_INT, ZL_NODE_ZIGZAG,
  *    ZL_NODE_TRANSPOSE_SPLIT, ZL_GRAPH_BITPACK, ZL_GRAPH_FSE, ZL_GRAPH_HUFFMAN,
  *    ZL_GRAPH_STORE, ZL_GRAPH_CONSTANT, ZL_GRAPH_FLATPACK, function graphs and
- *    the brute-force selector). A program written against include/openzl/*.h
+ *    the brute-force selector). A program written against the include/openzl/ headers
  *    for these graphs links against this library instead; the codec work runs
  *    on the GPU and frames are byte-identical to the reference's. Each
  *    declaration cites the reference declaration it replaces.

@@ -14,10 +14,18 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define OZL_HD __host__ __device__ __forceinline__
-#define OZL_CONST_TABLE static __device__ __constant__
 #else
 #define OZL_HD static inline
-#define OZL_CONST_TABLE static const
+#endif
+
+// Constant tables exist twice under hipcc: a __constant__ copy for device
+// code and a plain copy for host code (the host-side shadow of a __constant__
+// variable does not hold its initializer). OZL_TAB(name) picks the copy of
+// the side being compiled.
+#if defined(__HIPCC__) && defined(__HIP_DEVICE_COMPILE__)
+#define OZL_TAB(name) name##_dev
+#else
+#define OZL_TAB(name) name##_host
 #endif
 
 // ------------------------------------------------------------------ limits
@@ -50,9 +58,14 @@ OZL_HD size_t ozl_varint_size(uint64_t v)
 
 // floor(log2(1 + i/256) * 2^16) -- src/openzl/shared/data_stats.c:15-44.
 // Generated from the closed form with exact integer arithmetic.
-OZL_CONST_TABLE uint16_t ozl_log2_tab[256] = {
+static const uint16_t ozl_log2_tab_host[256] = {
 #include "ozl_log2_table.inc"
 };
+#if defined(__HIPCC__)
+static __device__ __constant__ uint16_t ozl_log2_tab_dev[256] = {
+#include "ozl_log2_table.inc"
+};
+#endif
 
 // ------------------------------------------------------------------ stats
 // One symbol's term of ZL_calculateEntropy (data_stats.c:46-73): the int64
@@ -66,7 +79,7 @@ OZL_HD int64_t ozl_entropy_term(uint32_t count, int64_t norm)
     int shift = 55 - clz;
     li >>= shift;
     li &= 255;
-    return -c * ((int64_t)ozl_log2_tab[li] - (int64_t)clz * 65536);
+    return -c * ((int64_t)OZL_TAB(ozl_log2_tab)[li] - (int64_t)clz * 65536);
 }
 OZL_HD double ozl_entropy_finish(int64_t sum, size_t total)
 {

@@ -19,23 +19,9 @@
 #include "ozl_core.h"
 #include "ozl_xxh3.h"
 #include "ozl_frame.h"
+#include "ozl_dparse.h"
 #include "ozl_profile.h"
 
-#define OZL_DTAB_BYTES 16384  // FSE: u32[4096]; Huffman: u16[4096] lookup
-
-struct ozl_dplan {
-    uint32_t n;
-    uint8_t bk, tl, nbStates, pad0;
-    uint32_t payOff, payLen;   // main payload inside the frame
-    uint32_t auxOff, auxLen;   // flatpack alphabet / constant byte inside the frame
-    uint32_t aux;              // bitpack / flatpack nbBits
-    uint32_t chunkOff;         // start of the chunk inside the frame
-    uint32_t csumOff;          // position of the content checksum
-    uint32_t status;
-    uint32_t pad0b;
-    uint64_t srcPtr;           // main payload (absolute device address)
-    uint64_t outPtr;           // decoded output (absolute device address)
-};
 
 struct ozl_dws {
     ozl_dplan* plans;
@@ -46,124 +32,11 @@ namespace {
 
 constexpr int kT = 256;
 
-struct Reader {
-    const uint8_t* p;
-    size_t pos, end;
-    bool bad;
-    __device__ uint32_t u8() { if (pos >= end) { bad = true; return 0; } return p[pos++]; }
-    __device__ uint64_t varint()
-    {
-        uint64_t v = 0;
-        for (int s = 0; s < 64; s += 7) {
-            uint32_t x = u8();
-            v |= (uint64_t)(x & 127) << s;
-            if (x < 128) return v;
-        }
-        bad = true;
-        return 0;
-    }
-    // LSB-first unpack of cnt fields of nb bits
-    __device__ void unpack(uint32_t* out, int cnt, int nb)
-    {
-        size_t nbytes = ((size_t)cnt * nb + 7) / 8;
-        if (pos + nbytes > end) { bad = true; return; }
-        size_t bit = 0;
-        for (int i = 0; i < cnt; i++) {
-            uint32_t v = 0;
-            for (int b = 0; b < nb; b++, bit++) v |= (uint32_t)((p[pos + (bit >> 3)] >> (bit & 7)) & 1) << b;
-            out[i] = v;
-        }
-        pos += nbytes;
-    }
-};
-
-struct DLds {
-    int16_t norm[256];
-    uint8_t cells[4096];
-    uint8_t w[256];
-    uint8_t wlen[256];
-    uint16_t wval[256];
-    uint32_t wdt[4096];
-    uint8_t expect[512];
-};
-
-// Decodes the Huffman weights stream described by the recognised layout.
-__device__ int decode_weights(const uint8_t* fr, const ozl_fnode* nodes, int nn, const ozl_fstream* st,
-                              const uint32_t* soff, uint32_t m, DLds& L)
-{
-    // stream ids follow ozl_entropy_layout: 1 weights, 3 serial weights, 4.. nested outputs
-    if (nn == 2) {  // conversion then STORE
-        for (uint32_t i = 0; i < m; i++) L.w[i] = fr[soff[3] + i];
-        return 0;
-    }
-    uint8_t t = nodes[2].trid;
-    if (t == OZL_TR_CONSTANT_SERIAL) {
-        for (uint32_t i = 0; i < m; i++) L.w[i] = fr[soff[4]];
-        return 0;
-    }
-    if (t == OZL_TR_FLATPACK) {
-        uint32_t card = st[4].size, plen = st[5].size;
-        const uint8_t* alpha = fr + soff[4];
-        const uint8_t* pk = fr + soff[5];
-        if (card == 0 || plen == 0 || pk[plen - 1] == 0) return -1;
-        uint32_t nb = card <= 1 ? card : 1 + (uint32_t)ozl_hb32(card - 1);
-        uint64_t totalBits = (uint64_t)(plen - 1) * 8 + ozl_hb32(pk[plen - 1]);
-        if (nb == 0 || totalBits / nb != m) return -1;
-        for (uint32_t i = 0; i < m; i++) {
-            uint32_t v = 0;
-            for (uint32_t b = 0; b < nb; b++) {
-                uint64_t bit = (uint64_t)i * nb + b;
-                v |= (uint32_t)((pk[bit >> 3] >> (bit & 7)) & 1) << b;
-            }
-            if (v >= card) return -1;
-            L.w[i] = alpha[v];
-        }
-        return 0;
-    }
-    if (t == OZL_TR_BITPACK_SERIAL) {
-        uint32_t nb = (nodes[2].hdr[0] & 63) + 1;
-        const uint8_t* pk = fr + soff[4];
-        if (nb > 8) return -1;
-        for (uint32_t i = 0; i < m; i++) {
-            uint32_t v = 0;
-            for (uint32_t b = 0; b < nb; b++) {
-                uint64_t bit = (uint64_t)i * nb + b;
-                v |= (uint32_t)((pk[bit >> 3] >> (bit & 7)) & 1) << b;
-            }
-            L.w[i] = (uint8_t)v;
-        }
-        return 0;
-    }
-    if (t == OZL_TR_FSE_V2) {
-        unsigned ms, tl;
-        int16_t wn[256];
-        if (nn == 4 && nodes[3].trid == OZL_TR_FSE_NCOUNT) {  // NCount encoded
-            if (ozl_fse_read_ncount(wn, &ms, &tl, fr + soff[6], st[6].size) < 0) return -1;
-        } else {        // raw int16 counts, stored after a num->serial conversion
-            uint32_t cnt = st[6].size / 2;
-            if (cnt == 0) return -1;
-            uint32_t sum = 0;
-            for (uint32_t s = 0; s < 256; s++) wn[s] = 0;
-            for (uint32_t s = 0; s < cnt; s++) {
-                wn[s] = (int16_t)(fr[soff[6] + 2 * s] | (fr[soff[6] + 2 * s + 1] << 8));
-                sum += wn[s] == -1 ? 1 : (uint32_t)wn[s];
-            }
-            if (sum == 0 || (sum & (sum - 1))) return -1;
-            ms = cnt - 1;
-            tl = (unsigned)ozl_hb32(sum);
-        }
-        if (tl < OZL_FSE_MIN_TABLELOG || tl > OZL_FSE_MAX_TABLELOG) return -1;
-        if (ozl_fse_build_dtable(L.wdt, L.cells, wn, ms, tl)) return -1;
-        return ozl_fse_decode_seq(L.w, m, fr + soff[5], st[5].size, L.wdt, tl, nodes[2].hdr[0]);
-    }
-    return -1;
-}
-
 __global__ void __launch_bounds__(kT) k_dparse(const uint8_t* __restrict__ frames, const uint64_t* __restrict__ foff,
                                                uint32_t B, uint8_t* __restrict__ dst, const uint64_t* __restrict__ doff,
                                                ozl_dws ws)
 {
-    __shared__ DLds L;
+    __shared__ ozl_dlds L;
     __shared__ uint16_t lut[4096];
     __shared__ int sFill[3];  // tl, filled flag
     uint32_t b = blockIdx.x;
@@ -172,267 +45,11 @@ __global__ void __launch_bounds__(kT) k_dparse(const uint8_t* __restrict__ frame
     size_t flen = foff[b + 1] - foff[b];
     uint8_t* tab = ws.tables + (size_t)b * OZL_DTAB_BYTES;
     if (threadIdx.x == 0) {
-        sFill[0] = 0;
-        sFill[1] = 0;
         ozl_dplan d;
-        uint32_t* dw = reinterpret_cast<uint32_t*>(&d);
-        for (size_t k = 0; k < sizeof(d) / 4; k++) dw[k] = 0;
-        d.status = OZL_ERR_CORRUPTION;
-        Reader r = { fr, 0, flen, false };
-        uint32_t magic = 0;
-        for (int k = 0; k < 4; k++) magic |= r.u8() << (8 * k);
-        uint32_t flags = r.u8();
-        uint32_t tok = r.u8();
-        uint64_t n1 = r.varint();
-        bool ok = !r.bad && magic == OZL_MAGIC_BASE + OZL_FORMAT_VERSION && flags == 3 && tok == 1 && n1 >= 1;
-        uint8_t fh[24];
-        size_t fhl = 0;
-        if (ok) {
-            fhl = ozl_write_frame_header(fh, OZL_T_SERIAL, n1 - 1, 1, 1);
-            for (size_t k = 0; k + 1 < fhl; k++) ok &= fr[k] == fh[k];
-            ok &= r.u8() == fh[fhl - 1];  // header checksum
-        }
-        d.n = (uint32_t)(n1 - 1);
-        d.chunkOff = (uint32_t)r.pos;
-        ozl_fnode nodes[8];
-        ozl_fstream st[12];
-        uint32_t soff[12];
-        int nn = 0, nbStored = 0;
-        if (ok) {
-            uint64_t nbDec = r.varint();
-            uint64_t nbSt = r.varint();
-            ok = !r.bad && nbDec >= 1 && nbDec - 1 <= 4 && nbSt >= 1 && nbSt <= 4;
-            if (ok) {
-                nn = (int)nbDec - 1;
-                nbStored = (int)nbSt;
-                uint32_t f[8], ids[8];
-                r.unpack(f, nn, 1);
-                r.unpack(ids, nn, 6);
-                // node k (creation order) is decoder nn-1-k
-                for (int k = 0; k < nn; k++) {
-                    nodes[k].trid = (uint8_t)ids[nn - 1 - k];
-                    nodes[k].hdrLen = 0;
-                    nodes[k].nbVO = 0;
-                }
-                uint32_t hf[8];
-                r.unpack(hf, nn, 1);
-                for (int k = nn - 1; k >= 0; k--)
-                    if (hf[nn - 1 - k]) nodes[k].hdrLen = (uint8_t)(r.varint() + 1);
-                ok = !r.bad;
-            }
-        }
-        // Recognise the layout from the creation-order transform ids and rebuild
-        // the plan; the layout function then yields the expected stream list.
-        ozl_eplan p;
-        uint32_t* pw = reinterpret_cast<uint32_t*>(&p);
-        for (size_t k = 0; k < sizeof(p) / 4; k++) pw[k] = 0;
-        p.n = d.n;
-        int ns = 0;
-        if (ok) {
-            uint8_t t0 = nn > 0 ? nodes[0].trid : 0;
-            if (nn == 0) p.bk = OZL_BK_STORE;
-            else if (t0 == OZL_TR_CONSTANT_SERIAL && nn == 1) p.bk = OZL_BK_CONSTANT;
-            else if (t0 == OZL_TR_FLATPACK && nn == 1) p.bk = OZL_BK_FLATPACK;
-            else if (t0 == OZL_TR_BITPACK_SERIAL && nn == 1) p.bk = OZL_BK_BITPACK;
-            else if (t0 == OZL_TR_FSE_V2 && nn == 2) p.bk = OZL_BK_FSE;
-            else if (t0 == OZL_TR_HUFFMAN_V2) p.bk = OZL_BK_HUF;
-            else ok = false;
-        }
-        // The stored sizes come after the VO/input flags and distances; parse
-        // the rest of the header generically first.
-        uint32_t storedSize[4] = { 0, 0, 0, 0 };
-        if (ok) {
-            uint32_t vf[8];
-            r.unpack(vf, nn, 1);
-            for (int k = 0; k < nn; k++) ok &= vf[k] == 0;
-            uint32_t inf[8];
-            r.unpack(inf, nn, 1);
-            for (int k = 0; k < nn; k++) ok &= inf[k] == 0;
-            uint32_t dist[8];
-            r.unpack(dist, nn, ozl_next_pow2((uint64_t)nn + (uint64_t)nbStored));
-            for (int k = 0; k < nbStored; k++) storedSize[k] = (uint32_t)r.varint();
-            ok &= !r.bad;
-        }
-        size_t hsPos = r.pos;  // header stream
-        uint32_t hsLen = 0;
-        if (ok) {
-            for (int k = 0; k < nn; k++) hsLen += nodes[k].hdrLen;
-            // node headers, decode order (reverse creation)
-            size_t q = hsPos;
-            for (int k = nn - 1; k >= 0; k--) {
-                if (nodes[k].hdrLen > 12 || q + nodes[k].hdrLen > flen) { ok = false; break; }
-                for (int i = 0; i < nodes[k].hdrLen; i++) nodes[k].hdr[i] = fr[q + i];
-                q += nodes[k].hdrLen;
-            }
-        }
-        // fill the plan fields the layout depends on
-        if (ok) {
-            switch (p.bk) {
-            case OZL_BK_FLATPACK: p.aux = (uint16_t)storedSize[1]; p.payload = storedSize[0]; break;
-            case OZL_BK_BITPACK: p.aux = (uint16_t)((nodes[0].hdr[0] & 63) + 1); p.payload = storedSize[0]; break;
-            case OZL_BK_FSE:
-                p.nbStates = nodes[0].hdr[0];
-                // stored list: [ncount or raw counts, bits]
-                p.payload = storedSize[1];
-                if (nodes[1].trid == OZL_TR_FSE_NCOUNT) { p.aux = (uint16_t)storedSize[0]; p.maxSym = 4; }
-                else p.maxSym = (uint8_t)(storedSize[0] / 2 - 1);
-                break;
-            case OZL_BK_HUF: {
-                p.nbStates = nodes[0].hdr[0];
-                // maxSym from the weights stream size (stored raw, or the conversion's stream)
-                // stored list: [bits, weights] when the weights are raw, else
-                // [nested streams..., bits]
-                p.payload = storedSize[nbStored - 1];
-                p.maxSym = 255;  // fixed below from the nested header
-                if (nn < 2 || nodes[1].trid != OZL_TR_CONV_NUM_TO_SERIAL_LE) ok = false;
-                if (ok) {
-                    uint8_t t2 = nn > 2 ? nodes[2].trid : 0;
-                    if (nn == 2) { p.wbk = OZL_BK_STORE; p.maxSym = (uint8_t)(storedSize[0] - 1); }
-                    else if (t2 == OZL_TR_CONSTANT_SERIAL) {
-                        p.wbk = OZL_BK_CONSTANT;
-                        uint64_t m = 0;
-                        for (int i = 0, s = 0; i < nodes[2].hdrLen; i++, s += 7) m |= (uint64_t)(nodes[2].hdr[i] & 127) << s;
-                        p.maxSym = (uint8_t)(m - 1);
-                    } else if (t2 == OZL_TR_FLATPACK) {
-                        p.wbk = OZL_BK_FLATPACK; p.wPayload = (uint16_t)storedSize[0]; p.wAux = (uint16_t)storedSize[1];
-                    } else if (t2 == OZL_TR_BITPACK_SERIAL) {
-                        p.wbk = OZL_BK_BITPACK; p.wAux8 = (uint8_t)((nodes[2].hdr[0] & 63) + 1); p.wPayload = (uint16_t)storedSize[0];
-                    } else if (t2 == OZL_TR_FSE_V2) {
-                        p.wbk = OZL_BK_FSE;
-                        // stored list: [ncount or raw counts, nested bits, bits]
-                        p.wPayload = (uint16_t)storedSize[1];
-                        if (nn == 4 && nodes[3].trid == OZL_TR_FSE_NCOUNT) { p.wAux = (uint16_t)storedSize[0]; p.wMaxSym = 4; }
-                        else p.wMaxSym = (uint8_t)(storedSize[0] / 2 - 1);
-                    } else ok = false;
-                    if (t2 == OZL_TR_FLATPACK || t2 == OZL_TR_BITPACK_SERIAL || t2 == OZL_TR_FSE_V2) {
-                        uint64_t m = 0;  // from the nested node header or sizes
-                        if (t2 == OZL_TR_FSE_V2) {
-                            for (int i = 1; i < nodes[2].hdrLen; i++) m |= (uint64_t)nodes[2].hdr[i] << (8 * (i - 1));
-                        } else if (t2 == OZL_TR_BITPACK_SERIAL) {
-                            uint32_t nb = p.wAux8;
-                            uint32_t extra = nodes[2].hdrLen == 2 ? nodes[2].hdr[1] : 0;
-                            m = (uint64_t)p.wPayload * 8 / nb - extra;
-                        } else {
-                            uint32_t card = p.wAux, plen = p.wPayload;
-                            uint32_t nb = card <= 1 ? card : 1 + (uint32_t)ozl_hb32(card - 1);
-                            const uint8_t* pk = nullptr;
-                            (void)pk;
-                            // terminating bit position gives the count; needs the stream
-                            m = 0;
-                            if (nb) {
-                                // payload position is known after layout; computed below
-                                m = (uint64_t)-1;
-                            }
-                            (void)plen;
-                        }
-                        if (m != (uint64_t)-1) p.maxSym = (uint8_t)(m - 1);
-                    }
-                }
-                break;
-            }
-            default: break;
-            }
-        }
-        // stored stream positions: header stream first, then stored streams in list order
-        size_t dataPos = hsPos + hsLen;
-        if (ok) {
-            // flatpack weights: recover m from the terminating bit now that positions are known
-            if (p.bk == OZL_BK_HUF && p.wbk == OZL_BK_FLATPACK) {
-                // stored list for this shape: [packed, alphabet, bits]
-                uint32_t plen = storedSize[0];
-                const uint8_t* pk = fr + dataPos;
-                uint32_t card = p.wAux;
-                uint32_t nb = card <= 1 ? card : 1 + (uint32_t)ozl_hb32(card - 1);
-                if (plen == 0 || dataPos + plen > flen || pk[plen - 1] == 0 || nb == 0) ok = false;
-                else p.maxSym = (uint8_t)(((uint64_t)(plen - 1) * 8 + ozl_hb32(pk[plen - 1])) / nb - 1);
-            }
-            bool mainNc = p.bk == OZL_BK_FSE && nodes[1].trid == OZL_TR_FSE_NCOUNT;
-            bool nestedNc = p.bk == OZL_BK_HUF && p.wbk == OZL_BK_FSE && nn == 4 && nodes[3].trid == OZL_TR_FSE_NCOUNT;
-            if ((mainNc || nestedNc) && dataPos + storedSize[0] > flen) ok = false;
-            if (ok && mainNc) {
-                // maxSym from the NCount itself
-                unsigned ms, tl;
-                long used = ozl_fse_read_ncount(L.norm, &ms, &tl, fr + dataPos, storedSize[0]);
-                if (used < 0) ok = false;
-                else p.maxSym = (uint8_t)ms;
-            }
-            if (ok && nestedNc) {
-                unsigned ms, tl;
-                long used = ozl_fse_read_ncount(L.norm, &ms, &tl, fr + dataPos, storedSize[0]);
-                if (used < 0) ok = false;
-                else p.wMaxSym = (uint8_t)ms;
-            }
-        }
-        int order[12], nbSt2 = 0;
-        ozl_fnode en[8];
-        if (ok) {
-            int enn = ozl_entropy_layout(&p, en, st, &ns);
-            ok = enn == nn;
-            for (int k = 0; ok && k < nn; k++) ok = en[k].trid == nodes[k].trid;
-            if (ok) {
-                size_t chl = ozl_write_chunk_header(L.expect, en, enn, st, ns, order, &nbSt2);
-                ok = nbSt2 == nbStored && d.chunkOff + chl <= flen;
-                for (size_t k = 0; ok && k < chl; k++) ok = L.expect[k] == fr[d.chunkOff + k];
-                if (ok) {
-                    size_t q = d.chunkOff + chl;
-                    for (int k = 0; k < 12; k++) soff[k] = 0;
-                    for (int k = 0; k < nbStored; k++) { soff[order[k]] = (uint32_t)q; q += st[order[k]].size; }
-                    d.csumOff = (uint32_t)q;
-                    ok = q + 8 + 1 == flen && fr[flen - 1] == 0;
-                }
-            }
-        }
-        // tables
-        if (ok) {
-            d.bk = p.bk;
-            d.status = 0;
-            switch (p.bk) {
-            case OZL_BK_STORE: d.payOff = soff[0]; d.payLen = d.n; break;
-            case OZL_BK_CONSTANT: d.auxOff = soff[1]; d.auxLen = 1; break;
-            case OZL_BK_FLATPACK:
-                d.auxOff = soff[1]; d.auxLen = st[1].size; d.payOff = soff[2]; d.payLen = st[2].size;
-                if (d.auxLen < 2 || d.auxLen > 256) d.status = OZL_ERR_CORRUPTION;
-                break;
-            case OZL_BK_BITPACK:
-                d.aux = p.aux; d.payOff = soff[1]; d.payLen = st[1].size;
-                if (d.aux < 1 || d.aux > 8 || d.payLen != ((uint64_t)d.n * d.aux + 7) / 8) d.status = OZL_ERR_CORRUPTION;
-                break;
-            case OZL_BK_FSE: {
-                d.payOff = soff[2]; d.payLen = st[2].size; d.nbStates = p.nbStates;
-                unsigned ms, tl;
-                if (nodes[1].trid == OZL_TR_FSE_NCOUNT) {
-                    if (ozl_fse_read_ncount(L.norm, &ms, &tl, fr + soff[3], st[3].size) < 0) { d.status = OZL_ERR_CORRUPTION; break; }
-                } else {  // raw int16 counts after a num->serial conversion
-                    uint32_t cnt = st[3].size / 2, sum = 0;
-                    for (int s = 0; s < 256; s++) L.norm[s] = 0;
-                    for (uint32_t s = 0; s < cnt; s++) {
-                        L.norm[s] = (int16_t)(fr[soff[3] + 2 * s] | (fr[soff[3] + 2 * s + 1] << 8));
-                        sum += L.norm[s] == -1 ? 1 : (uint32_t)L.norm[s];
-                    }
-                    if (cnt == 0 || sum == 0 || (sum & (sum - 1))) { d.status = OZL_ERR_CORRUPTION; break; }
-                    ms = cnt - 1;
-                    tl = (unsigned)ozl_hb32(sum);
-                }
-                if (tl < OZL_FSE_MIN_TABLELOG || tl > 11 || (d.nbStates != 2 && d.nbStates != 4)) { d.status = OZL_ERR_CORRUPTION; break; }
-                d.tl = (uint8_t)tl;
-                if (ozl_fse_build_dtable(reinterpret_cast<uint32_t*>(tab), L.cells, L.norm, ms, tl)) d.status = OZL_ERR_CORRUPTION;
-                break;
-            }
-            case OZL_BK_HUF: {
-                d.payOff = soff[2]; d.payLen = st[2].size; d.nbStates = p.nbStates;
-                if (d.nbStates > 1 || (d.nbStates == 1 && d.n < 12)) { d.status = OZL_ERR_CORRUPTION; break; }
-                uint32_t m = p.maxSym + 1u;
-                if (decode_weights(fr, nodes, nn, st, soff, m, L)) { d.status = OZL_ERR_CORRUPTION; break; }
-                int tl = ozl_huf_from_weights(L.wlen, L.wval, L.w, m);
-                if (tl < 1 || tl > 11) { d.status = OZL_ERR_CORRUPTION; break; }
-                d.tl = (uint8_t)tl;
-                sFill[0] = tl;
-                sFill[1] = 1;
-                break;
-            }
-            default: break;
-            }
-        }
+        int hufTl;
+        ozl_dparse_frame(fr, flen, tab, L, d, &hufTl);
+        sFill[0] = hufTl;
+        sFill[1] = hufTl > 0;
         d.srcPtr = (uint64_t)(uintptr_t)(fr + d.payOff);
         d.outPtr = (uint64_t)(uintptr_t)(dst + doff[b]);
         ws.plans[b] = d;
@@ -502,7 +119,9 @@ __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
         uint32_t p = r * S + q;
         bool mine = q < (int)S && p < n;
         uint32_t e = mine ? DT[state] : 0;
-        uint32_t nb = (mine && p + S < n) ? (e >> 8) & 255 : 0;
+        // every symbol but the last S moves its state, also with 0 bits to read
+        bool next = mine && p + S < n;
+        uint32_t nb = next ? (e >> 8) & 255 : 0;
         // read order inside the round = lane order
         uint32_t n0 = __shfl(nb, base + 0, 64), n1 = __shfl(nb, base + 1, 64);
         uint32_t n2 = __shfl(nb, base + 2, 64);
@@ -510,7 +129,7 @@ __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
         uint32_t all = __shfl(before + nb, base + (int)S - 1, 64);
         if (mine) {
             out[p] = (uint8_t)e;
-            if (nb) state = (e >> 16) + read_bits_back(s, pos - (long)before, nb);
+            if (next) state = (e >> 16) + read_bits_back(s, pos - (long)before, nb);
         }
         pos -= (long)all;
     }

@@ -6,6 +6,7 @@ reference frames computed here and frames computed on the GPU box compare.
 import numpy as np
 
 SEEDS = {1: 0x5EED0001, 2: 0x5EED0002, 3: 0x5EED0003, 4: 0x5EED0004, 5: 0x5EED0005}
+BLOCK = 65536
 
 
 def timestamps_u32(n=4194304, seed=SEEDS[1]):
@@ -48,3 +49,92 @@ def bf16_normal(n=536870912, seed=SEEDS[4], sigma=0.02):
         rnd = ((f >> 16) & 1) + 0x7FFF
         out[a:b] = ((f + rnd) >> 16).astype(np.uint16)
     return out
+
+
+def zipf_bytes(n, s=1.1, card=256, seed=1):
+    """n u8 symbols, Zipf(s) over `card` ranks mapped to random byte values."""
+    rng = np.random.default_rng(seed)
+    p = 1.0 / np.arange(1, card + 1) ** s
+    p /= p.sum()
+    perm = rng.permutation(256)[:card]
+    return perm[rng.choice(card, n, p=p)].astype(np.uint8)
+
+
+def walk_column(n, width, b, seed=1):
+    """One column of the batched-columns configuration with fixed length, width and b."""
+    rng = np.random.default_rng(seed)
+    steps = np.rint(rng.normal(0.0, float(2 ** b), n)).astype(np.int64)
+    walk = np.cumsum(steps)
+    dt = np.uint16 if width == 2 else np.uint32
+    return (walk & ((1 << (8 * width)) - 1)).astype(dt)
+
+
+def uniform_ints(n, hi, dtype, seed=1):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, hi, n).astype(dtype)
+
+
+def zipf_table(seed, s=1.1, card=256):
+    """uint32 inverse-CDF thresholds and the rank -> symbol permutation."""
+    p = 1.0 / np.arange(1, card + 1, dtype=np.float64) ** s
+    cdf = np.cumsum(p / p.sum())
+    thr = np.minimum(np.floor(cdf * 4294967296.0), 4294967295).astype(np.uint64)
+    thr[-1] = 4294967296
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(card).astype(np.uint8)
+    return thr, perm
+
+
+def splitmix_np(idx, seed):
+    with np.errstate(over="ignore"):
+        z = idx.astype(np.uint64) + np.uint64((seed * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def zipf_blocks_np(first_block, n_blocks, seed=SEEDS[2]):
+    """Blocks [first_block, first_block + n_blocks) of the entropy-stress input:
+    u8 symbols, Zipf(s=1.1) over 256 ranks permuted by the seed, one splitmix64
+    draw per byte (counter-based, so any block range is generated alone)."""
+    thr, perm = zipf_table(seed)
+    idx = np.arange(first_block * BLOCK, (first_block + n_blocks) * BLOCK, dtype=np.uint64)
+    u = splitmix_np(idx, seed) >> np.uint64(32)
+    rank = np.searchsorted(thr, u, side="right")
+    return perm[rank]
+
+
+def zipf_blocks_torch(torch, n_blocks, seed=SEEDS[2], device="cuda"):
+    """Same values as zipf_blocks_np(0, n_blocks), generated on the GPU in slices."""
+    thr, perm = zipf_table(seed)
+    thr_t = torch.from_numpy(thr.astype(np.int64)).to(device)
+    perm_t = torch.from_numpy(perm).to(device)
+    total = n_blocks * BLOCK
+    out = torch.empty(total, dtype=torch.uint8, device=device)
+    M64 = (1 << 64) - 1
+
+    def s64(v):
+        v &= M64
+        return v - (1 << 64) if v >= (1 << 63) else v
+
+    add = s64(seed * 0x9E3779B97F4A7C15)
+    m1 = s64(0xBF58476D1CE4E5B9)
+    m2 = s64(0x94D049BB133111EB)
+
+    def lsr(x, k):  # logical shift right on int64
+        return (x >> k) & ((1 << (64 - k)) - 1)
+
+    step = 1 << 26
+    for a in range(0, total, step):
+        b = min(total, a + step)
+        z = torch.arange(a, b, dtype=torch.int64, device=device) + add
+        z = (z ^ lsr(z, 30)) * m1
+        z = (z ^ lsr(z, 27)) * m2
+        z = z ^ lsr(z, 31)
+        u = lsr(z, 32)
+        rank = torch.searchsorted(thr_t, u, right=True)
+        out[a:b] = perm_t[rank]
+    return out
+
+

@@ -2,6 +2,8 @@
 // that the HIP kernels run is compiled for the host so its table builders and
 // decisions can be checked against the oracle without a GPU.
 #include "ozl_core.h"
+#include <string.h>
+#include "ozl_dparse.h"
 
 extern "C" {
 int cu_decide(const uint8_t* src, size_t n, int hufMode)
@@ -61,4 +63,18 @@ size_t cu_flatpack(uint8_t* alphabet, uint8_t* packed, size_t* packedSize, const
     for (size_t i = 0; i < n; i++) count[src[i]]++;
     return ozl_flatpack_encode_seq(alphabet, packed, packedSize, src, n, count);
 }
+
+// Frame recognition of the batched entropy decoder (ozl_dparse.h): returns
+// the plan's status and fills `plan` (the ozl_dplan bytes), the FSE decoding
+// table, or the Huffman code lengths and maximum length.
+int cu_dparse(const uint8_t* fr, size_t flen, void* plan, uint8_t* tab, uint8_t* hufLen, int* hufTl)
+{
+    static ozl_dlds L;
+    ozl_dplan d;
+    ozl_dparse_frame(fr, flen, tab, L, d, hufTl);
+    memcpy(plan, &d, sizeof(d));
+    memcpy(hufLen, L.wlen, 256);
+    return (int)d.status;
+}
+size_t cu_dplan_size(void) { return sizeof(ozl_dplan); }
 }
