@@ -160,6 +160,7 @@ typedef struct {
     char localParams[48];             /* layout placeholder, unused */
     char opaque[24];                  /* layout placeholder, unused */
 } ZL_FunctionGraphDesc;
+/* include/openzl/zl_graph_api.h:114 */
 ZL_GraphID ZL_Compressor_registerFunctionGraph(ZL_Compressor* compressor, const ZL_FunctionGraphDesc* desc);
 typedef struct {
     ZL_Edge** edges;
@@ -172,11 +173,15 @@ typedef union {
         ZL_EdgeList _value;
     } _value;
 } ZL_Result_ZL_EdgeList;
-/* include/openzl/zl_graph_api.h: runtime edge operations */
+/* include/openzl/zl_graph_api.h:254 */
 ZL_Result_ZL_EdgeList ZL_Edge_runNode(ZL_Edge* input, ZL_NodeID node);
+/* include/openzl/codecs/zl_transpose.h:56 */
 ZL_Result_ZL_EdgeList ZL_Edge_runTransposeSplit(ZL_Edge* edge, const ZL_Graph* graph);
+/* include/openzl/zl_graph_api.h:283 */
 ZL_Report ZL_Edge_setDestination(ZL_Edge* edge, ZL_GraphID destination);
+/* include/openzl/zl_graph_api.h:156 */
 const ZL_Input* ZL_Edge_getData(const ZL_Edge* edge);
+/* include/openzl/zl_input.h:54-99 (inline there, exported functions here) */
 size_t ZL_Input_numElts(const ZL_Input* input);
 size_t ZL_Input_eltWidth(const ZL_Input* input);
 size_t ZL_Input_contentSize(const ZL_Input* input);
@@ -185,10 +190,12 @@ ZL_Type ZL_Input_type(const ZL_Input* input);
 /* ------------------------------------------------------------ compression */
 /* include/openzl/zl_compress.h:50 */
 size_t ZL_compressBound(size_t totalSrcSize);
-/* include/openzl/zl_compress.h / zl_compressor.h:597 */
+/* include/openzl/zl_compress.h:60-61 */
 ZL_CCtx* ZL_CCtx_create(void);
 void ZL_CCtx_free(ZL_CCtx* cctx);
+/* include/openzl/zl_compressor.h:597 */
 ZL_Report ZL_CCtx_refCompressor(ZL_CCtx* cctx, const ZL_Compressor* compressor);
+/* include/openzl/zl_compress.h:152 */
 ZL_Report ZL_CCtx_setParameter(ZL_CCtx* cctx, ZL_CParam param, int value);
 /* include/openzl/zl_compress.h:313-349 */
 ZL_TypedRef* ZL_TypedRef_createSerial(const void* src, size_t srcSize);
@@ -197,7 +204,7 @@ ZL_TypedRef* ZL_TypedRef_createNumeric(const void* start, size_t numWidth, size_
 void ZL_TypedRef_free(ZL_TypedRef* tref);
 /* include/openzl/zl_compress.h:274 */
 ZL_Report ZL_CCtx_compressTypedRef(ZL_CCtx* cctx, void* dst, size_t dstCapacity, const ZL_TypedRef* input);
-/* include/openzl/zl_compress.h: serial convenience */
+/* include/openzl/zl_compress.h:194 */
 ZL_Report ZL_CCtx_compress(ZL_CCtx* cctx, void* dst, size_t dstCapacity, const void* src, size_t srcSize);
 
 /* ------------------------------------------------------------ decompression */
@@ -210,8 +217,9 @@ typedef struct {
 } ZL_OutputInfo;
 /* include/openzl/zl_decompress.h:32 */
 ZL_Report ZL_decompress(void* dst, size_t dstCapacity, const void* src, size_t srcSize);
-/* include/openzl/zl_decompress.h: ZL_getDecompressedSize */
+/* include/openzl/zl_decompress.h:47 */
 ZL_Report ZL_getDecompressedSize(const void* compressed, size_t cSize);
+/* include/openzl/zl_decompress.h:80, 87 */
 ZL_DCtx* ZL_DCtx_create(void);
 void ZL_DCtx_free(ZL_DCtx* dctx);
 /* include/openzl/zl_decompress.h:388 */
@@ -236,12 +244,21 @@ int ozl_bitpack_encode(const void* src, size_t n, int width, int nbBits, void* d
 int ozl_bitpack_decode(const void* src, size_t n, int width, int nbBits, void* dst, void* stream);
 
 /* Batched ZL_GRAPH_FSE (hufMode 0) / ZL_GRAPH_HUFFMAN (hufMode 1) frames, one
- * per block of < 100000 bytes. */
+ * per block of < 100000 bytes: the same frame as ZL_CCtx_compress
+ * (include/openzl/zl_compress.h:194) of each block with a compressor whose
+ * starting graph is ZL_GRAPH_FSE / ZL_GRAPH_HUFFMAN (src/openzl/codecs/
+ * entropy/encode_entropy_binding.c:814-851), format version 23, both
+ * checksums on. All pointers are device pointers; offsets has nbBlocks+1
+ * entries; frame b lands at dst + frameOffsets[b]; status[b] is 0 or the
+ * block's error code. */
 size_t ozl_entropy_compress_workspace(uint32_t nbBlocks, size_t maxBlock);
 size_t ozl_entropy_frame_bound(size_t blockSize);
 int ozl_entropy_compress_batch(int hufMode, const void* src, const uint64_t* offsets, uint32_t nbBlocks,
                                size_t maxBlock, void* dst, size_t dstCapacity, uint64_t* frameOffsets,
                                int* status, void* workspace, size_t workspaceSize, void* stream);
+/* ZL_decompress (include/openzl/zl_decompress.h:32) of many such frames in
+ * one call; frames of any other layout are reported as corruption in status
+ * and are left to ZL_DCtx_decompressTyped. */
 size_t ozl_entropy_decompress_workspace(uint32_t nbBlocks);
 int ozl_entropy_decompress_batch(const void* frames, const uint64_t* frameOffsets, uint32_t nbFrames, void* dst,
                                  const uint64_t* dstOffsets, int* status, void* workspace, size_t workspaceSize,
