@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1024, help="distinct blocks in the CPU baseline sample (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=8.0, help="wall-clock budget of the CPU baseline")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run 3 device-to-device copies of 1 GiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus})",
@@ -118,6 +120,15 @@ def main():
     from openzl_amd import batch, _lib
 
     L = _lib.lib()
+    if args.copy_probe:
+        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("[bench] copy probe: 3 x 1 GiB device copies", file=sys.stderr, flush=True)
+        return
     B = args.blocks
     dev = "cuda"
     print(f"[bench] generating {B} x 64 KiB Zipf(1.1) blocks", file=sys.stderr, flush=True)
@@ -127,14 +138,29 @@ def main():
     dec = {m: batch.EntropyBatchDecompressor(B) for m in (batch.FSE, batch.HUFFMAN)}
     out = {m: torch.empty(B * BLOCK, dtype=torch.uint8, device=dev) for m in (batch.FSE, batch.HUFFMAN)}
     torch.cuda.synchronize()
+    tag = {batch.FSE: "fse", batch.HUFFMAN: "huf"}
+    L.ozl_profile_tag.argtypes = [ctypes.c_char_p]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    phase_ms = {"encode": 0.0, "decode": 0.0}
 
-    def step():
+    def step(timed=False):
         frames = {}
+        if timed:
+            ev[0].record()
         for m in (batch.FSE, batch.HUFFMAN):
+            L.ozl_profile_tag(f"{tag[m]}_enc/".encode())
             frames[m] = enc[m].run(m, src, off)
+        if timed:
+            ev[1].record()
         for m in (batch.FSE, batch.HUFFMAN):
+            L.ozl_profile_tag(f"{tag[m]}_dec/".encode())
             dst, foff, _ = frames[m]
             dec[m].run(dst, foff, out[m], off)
+        if timed:
+            ev[2].record()
+            ev[2].synchronize()
+            phase_ms["encode"] += ev[0].elapsed_time(ev[1])
+            phase_ms["decode"] += ev[1].elapsed_time(ev[2])
         return frames
 
     print("[bench] correctness pass", file=sys.stderr, flush=True)
@@ -172,24 +198,51 @@ def main():
     U = B * BLOCK
     value = 2 * U / (ms_step / 1000) / 1e9
 
-    # payload bytes per coding (frames minus per-frame headers ~ same order; use frame bytes)
+    # encode / decode split, measured in separate steps with events between the phases
+    for _ in range(args.steps):
+        step(timed=True)
+    enc_ms = phase_ms["encode"] / args.steps
+    dec_ms = phase_ms["decode"] / args.steps
+
+    # roofline of the dominant kernel: algorithmic bytes per launch = U + C of
+    # the launch's coding (every launch processes the whole batch of blocks)
     roofline = None
     if prof:
         dom = max(prof.items(), key=lambda kv: kv[1][1])
         name, (cnt, tot_ms) = dom
         avg_ms = tot_ms / cnt
-        per_launch = {
-            "k_fse_encode": U + csize[batch.FSE], "k_fse_decode": U + csize[batch.FSE],
-            "k_huf_encode": U + csize[batch.HUFFMAN], "k_huf_decode": U + csize[batch.HUFFMAN],
-            "k_plan": U, "k_content_hash": U, "k_dverify": U + (csize[0] + csize[1]) // 2,
-            "k_frame_write": (csize[0] + csize[1]) // 2, "k_frame_checksum": (csize[0] + csize[1]) // 2,
-            "k_pack_decode": 0, "k_pack_encode": 0, "k_dparse": 0,
-        }.get(name, U)
+        coding = batch.FSE if name.startswith("fse") else batch.HUFFMAN
+        per_launch = U + csize[coding]
         achieved = per_launch / (avg_ms / 1000) / 1e9
+        # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+        # summary of this workload (copy-calibrated), when it covers the kernel
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "entropy_stress_1GiB_hbm_traffic.json")
+        if B == N_BLOCKS and os.path.exists(tpath):
+            with open(tpath) as f:
+                traffic = json.load(f)["kernels"].get(name, {}).get("traffic_bytes")
         roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": per_launch,
                     "kernel_ms_per_step": {k: round(v[1] / args.steps, 4) for k, v in sorted(prof.items())}}
+
+    # host-buffer variant (not `value`): pinned H2D / D2H bandwidth, no overlap
+    pin = torch.empty(1 << 28, dtype=torch.uint8, pin_memory=True)
+    dbuf = torch.empty(1 << 28, dtype=torch.uint8, device=dev)
+    t_a = time.perf_counter()
+    for _ in range(4):
+        dbuf.copy_(pin, non_blocking=True)
+    torch.cuda.synchronize()
+    h2d = 4 * (1 << 28) / (time.perf_counter() - t_a)
+    t_a = time.perf_counter()
+    for _ in range(4):
+        pin.copy_(dbuf, non_blocking=True)
+    torch.cuda.synchronize()
+    d2h = 4 * (1 << 28) / (time.perf_counter() - t_a)
+    C_tot = csize[batch.FSE] + csize[batch.HUFFMAN]
+    xfer_s = (2 * U + C_tot) / h2d + (2 * U + C_tot) / d2h
+    pcie_rate = 2 * U / (ms_step / 1000 + xfer_s) / 1e9
+    del pin, dbuf
 
     cpu = None
     if args.cpu_sample > 0:
@@ -217,7 +270,12 @@ def main():
             "blocks": B, "block_bytes": BLOCK, "input_bytes_per_coding": U,
             "frame_bytes_fse": csize[batch.FSE], "frame_bytes_huffman": csize[batch.HUFFMAN],
             "ratio_fse": round(U / csize[batch.FSE], 4), "ratio_huffman": round(U / csize[batch.HUFFMAN], 4),
-            "symbols_per_s": round(2 * U / (ms_step / 1000), 1),
+            "symbols_per_s_both_codings": round(2 * U / (ms_step / 1000), 1),
+            "encode_GBps": round(2 * U / (enc_ms / 1000) / 1e9, 3),
+            "decode_GBps": round(2 * U / (dec_ms / 1000) / 1e9, 3),
+            "encode_ms_per_step": round(enc_ms, 3), "decode_ms_per_step": round(dec_ms, 3),
+            "host_buffer_GBps_no_overlap": round(pcie_rate, 3),
+            "pcie_h2d_GBps": round(h2d / 1e9, 2), "pcie_d2h_GBps": round(d2h / 1e9, 2),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

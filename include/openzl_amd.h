@@ -264,8 +264,11 @@ int ozl_entropy_decompress_batch(const void* frames, const uint64_t* frameOffset
                                  const uint64_t* dstOffsets, int* status, void* workspace, size_t workspaceSize,
                                  void* stream);
 
-/* Per-kernel timing (HIP events around every launch while enabled). */
+/* Per-kernel timing (HIP events around every launch while enabled; no
+ * reference counterpart). ozl_profile_end writes "<tag><kernel> <launches>
+ * <total ms>" lines; ozl_profile_tag sets the tag of later launches. */
 void ozl_profile_begin(void);
+void ozl_profile_tag(const char* tag);
 size_t ozl_profile_end(char* out, size_t capacity);
 
 #ifdef __cplusplus

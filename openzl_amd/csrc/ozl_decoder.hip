@@ -103,11 +103,12 @@ ZL_Report rep_ok(size_t v)
     r._value._value = v;
     return r;
 }
+// A failed HIP runtime call has no reference code: it surfaces as GENERIC.
 ZL_Report rep_err(int code)
 {
     ZL_Report r;
     memset(&r, 0, sizeof(r));
-    r._code = (ZL_ErrorCode)code;
+    r._code = (ZL_ErrorCode)(code == OZL_ERR_HIP ? ZL_ErrorCode_GENERIC : code);
     return r;
 }
 

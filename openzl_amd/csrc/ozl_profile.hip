@@ -4,15 +4,17 @@
 #include <mutex>
 #include <stdio.h>
 #include <string.h>
+#include <string>
 #include <vector>
 
 namespace {
 struct Rec {
-    const char* name;
+    std::string name;
     hipEvent_t a, b;
 };
 std::mutex g_mu;
 bool g_on = false;
+std::string g_tag;
 std::vector<Rec> g_recs;
 std::vector<hipEvent_t> g_pool;
 
@@ -33,7 +35,7 @@ ozl_prof_token ozl_prof_start(const char* name, hipStream_t st)
 {
     std::lock_guard<std::mutex> g(g_mu);
     if (!g_on) return { -1 };
-    Rec r{ name, take(), take() };
+    Rec r{ g_tag + name, take(), take() };
     if (!r.a || !r.b) return { -1 };
     (void)hipEventRecord(r.a, st);
     g_recs.push_back(r);
@@ -56,13 +58,21 @@ void ozl_profile_begin(void)
     g_on = true;
 }
 
+// Prefixes the names of the launches recorded from now on (e.g. "fse/"), so
+// that launches of one kernel in different calls are told apart.
+void ozl_profile_tag(const char* tag)
+{
+    std::lock_guard<std::mutex> g(g_mu);
+    g_tag = tag ? tag : "";
+}
+
 // Stops recording, waits for the events, and writes one line per kernel name:
 // "<name> <launches> <total_ms>\n". Returns the bytes written (or needed).
 size_t ozl_profile_end(char* out, size_t cap)
 {
     std::lock_guard<std::mutex> g(g_mu);
     g_on = false;
-    std::vector<const char*> names;
+    std::vector<std::string> names;
     std::vector<int> counts;
     std::vector<double> ms;
     for (Rec& r : g_recs) {
@@ -71,7 +81,7 @@ size_t ozl_profile_end(char* out, size_t cap)
         (void)hipEventElapsedTime(&t, r.a, r.b);
         size_t k = 0;
         for (; k < names.size(); k++)
-            if (strcmp(names[k], r.name) == 0) break;
+            if (names[k] == r.name) break;
         if (k == names.size()) { names.push_back(r.name); counts.push_back(0); ms.push_back(0); }
         counts[k]++;
         ms[k] += t;
@@ -82,7 +92,7 @@ size_t ozl_profile_end(char* out, size_t cap)
     size_t used = 0;
     for (size_t k = 0; k < names.size(); k++) {
         char line[256];
-        int len = snprintf(line, sizeof(line), "%s %d %.6f\n", names[k], counts[k], ms[k]);
+        int len = snprintf(line, sizeof(line), "%s %d %.6f\n", names[k].c_str(), counts[k], ms[k]);
         if (out && used + (size_t)len < cap) memcpy(out + used, line, (size_t)len);
         used += (size_t)len;
     }
