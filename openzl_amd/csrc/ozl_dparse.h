@@ -7,7 +7,7 @@
 // ozl_entropy_layout / ozl_write_chunk_header and compared byte for byte
 // (any difference -> the frame is reported unsupported and is left to the
 // generic decoder), and the NCount / Huffman weights (with their nested
-// graphs) are decoded into an FSE decoding table or Huffman code lengths.
+// graphs) are decoded into normalized counts or Huffman code lengths.
 // The device runs it on one lane per frame; the CPU unit build runs the same
 // code against reference frames.
 #pragma once
@@ -24,7 +24,7 @@
 #define OZL_HDM inline
 #endif
 
-#define OZL_DTAB_BYTES 16384  // FSE: u32[4096]; Huffman: u16[4096] lookup
+
 
 struct ozl_dplan {
     uint32_t n;
@@ -40,8 +40,18 @@ struct ozl_dplan {
     uint64_t outPtr;           // decoded output (absolute device address)
 };
 
+// A frame seen through a copy of its first `pn` bytes (in LDS on the device):
+// reads inside the copy use it, the rest go to the frame itself.
+struct ozl_fview {
+    const uint8_t* g;  // the frame
+    const uint8_t* p;  // copy of its first pn bytes
+    size_t pn;
+    OZL_HDM uint8_t operator[](size_t i) const { return i < pn ? p[i] : g[i]; }
+    OZL_HDM const uint8_t* at(size_t off, size_t len) const { return off + len <= pn ? p + off : g + off; }
+};
+
 struct ozl_reader {
-    const uint8_t* p;
+    ozl_fview p;
     size_t pos, end;
     bool bad;
     OZL_HDM uint32_t u8() { if (pos >= end) { bad = true; return 0; } return p[pos++]; }
@@ -71,18 +81,23 @@ struct ozl_reader {
     }
 };
 
+// The nested FSE graph of the Huffman weights codes at most 256 symbols of
+// value <= 12: its tableLog never exceeds 6 (fse_compress.c:379-396), so its
+// decoding table is bounded here at 2^8 cells.
+#define OZL_WEIGHTS_MAX_TABLELOG 8
+
 struct ozl_dlds {
     int16_t norm[256];
-    uint8_t cells[4096];
+    uint8_t cells[1 << OZL_WEIGHTS_MAX_TABLELOG];
     uint8_t w[256];
     uint8_t wlen[256];
     uint16_t wval[256];
-    uint32_t wdt[4096];
+    uint32_t wdt[1 << OZL_WEIGHTS_MAX_TABLELOG];
     uint8_t expect[512];
 };
 
 // Decodes the Huffman weights stream described by the recognised layout.
-OZL_HDN int ozl_decode_weights(const uint8_t* fr, const ozl_fnode* nodes, int nn, const ozl_fstream* st,
+OZL_HDN int ozl_decode_weights(const ozl_fview& fr, const ozl_fnode* nodes, int nn, const ozl_fstream* st,
                                 const uint32_t* soff, uint32_t m, ozl_dlds& L)
 {
     // stream ids follow ozl_entropy_layout: 1 weights, 3 serial weights, 4.. nested outputs
@@ -97,8 +112,8 @@ OZL_HDN int ozl_decode_weights(const uint8_t* fr, const ozl_fnode* nodes, int nn
     }
     if (t == OZL_TR_FLATPACK) {
         uint32_t card = st[4].size, plen = st[5].size;
-        const uint8_t* alpha = fr + soff[4];
-        const uint8_t* pk = fr + soff[5];
+        const uint8_t* alpha = fr.at(soff[4], st[4].size);
+        const uint8_t* pk = fr.at(soff[5], st[5].size);
         if (card == 0 || plen == 0 || pk[plen - 1] == 0) return -1;
         uint32_t nb = card <= 1 ? card : 1 + (uint32_t)ozl_hb32(card - 1);
         uint64_t totalBits = (uint64_t)(plen - 1) * 8 + ozl_hb32(pk[plen - 1]);
@@ -116,7 +131,7 @@ OZL_HDN int ozl_decode_weights(const uint8_t* fr, const ozl_fnode* nodes, int nn
     }
     if (t == OZL_TR_BITPACK_SERIAL) {
         uint32_t nb = (nodes[2].hdr[0] & 63) + 1;
-        const uint8_t* pk = fr + soff[4];
+        const uint8_t* pk = fr.at(soff[4], st[4].size);
         if (nb > 8) return -1;
         for (uint32_t i = 0; i < m; i++) {
             uint32_t v = 0;
@@ -132,7 +147,7 @@ OZL_HDN int ozl_decode_weights(const uint8_t* fr, const ozl_fnode* nodes, int nn
         unsigned ms, tl;
         int16_t wn[256];
         if (nn == 4 && nodes[3].trid == OZL_TR_FSE_NCOUNT) {  // NCount encoded
-            if (ozl_fse_read_ncount(wn, &ms, &tl, fr + soff[6], st[6].size) < 0) return -1;
+            if (ozl_fse_read_ncount(wn, &ms, &tl, fr.at(soff[6], st[6].size), st[6].size) < 0) return -1;
         } else {        // raw int16 counts, stored after a num->serial conversion
             uint32_t cnt = st[6].size / 2;
             if (cnt == 0) return -1;
@@ -147,20 +162,20 @@ OZL_HDN int ozl_decode_weights(const uint8_t* fr, const ozl_fnode* nodes, int nn
             tl = (unsigned)ozl_hb32(sum);
         }
         if (tl < OZL_FSE_MIN_TABLELOG || tl > OZL_FSE_MAX_TABLELOG) return -1;
+        if (tl > OZL_WEIGHTS_MAX_TABLELOG) return -1;
         if (ozl_fse_build_dtable(L.wdt, L.cells, wn, ms, tl)) return -1;
-        return ozl_fse_decode_seq(L.w, m, fr + soff[5], st[5].size, L.wdt, tl, nodes[2].hdr[0]);
+        return ozl_fse_decode_seq(L.w, m, fr.at(soff[5], st[5].size), st[5].size, L.wdt, tl, nodes[2].hdr[0]);
     }
     return -1;
 }
 
-// Parses frame `fr` (flen bytes). On success d.status is 0 and, for FSE, the
-// decoding table is in `tab`; for Huffman, L.wlen / L.wval hold the code and
-// *hufTl its maximum length (0 otherwise).
-OZL_HDN void ozl_dparse_frame(const uint8_t* fr, size_t flen, uint8_t* tab, ozl_dlds& L, ozl_dplan& d, int* hufTl)
+// Parses frame `fr` (flen bytes). On success d.status is 0; for FSE, L.norm
+// holds the normalized counts (d.aux = maxSymbol, d.tl = tableLog); for
+// Huffman, L.wlen holds the code lengths (d.tl = maximum length). The tables
+// themselves are built by the decoding kernels.
+OZL_HDN void ozl_dparse_frame(const ozl_fview& fr, size_t flen, ozl_dlds& L, ozl_dplan& d)
 {
-    *hufTl = 0;
-    uint32_t* dw = reinterpret_cast<uint32_t*>(&d);
-    for (size_t k = 0; k < sizeof(d) / 4; k++) dw[k] = 0;
+    d = ozl_dplan{};
     d.status = OZL_ERR_CORRUPTION;
     ozl_reader r = { fr, 0, flen, false };
     uint32_t magic = 0;
@@ -207,9 +222,7 @@ OZL_HDN void ozl_dparse_frame(const uint8_t* fr, size_t flen, uint8_t* tab, ozl_
     }
     // Recognise the layout from the creation-order transform ids and rebuild
     // the plan; the layout function then yields the expected stream list.
-    ozl_eplan p;
-    uint32_t* pw = reinterpret_cast<uint32_t*>(&p);
-    for (size_t k = 0; k < sizeof(p) / 4; k++) pw[k] = 0;
+    ozl_eplan p = {};
     p.n = d.n;
     int ns = 0;
     if (ok) {
@@ -324,7 +337,7 @@ OZL_HDN void ozl_dparse_frame(const uint8_t* fr, size_t flen, uint8_t* tab, ozl_
         if (p.bk == OZL_BK_HUF && p.wbk == OZL_BK_FLATPACK) {
             // stored list for this shape: [packed, alphabet, bits]
             uint32_t plen = storedSize[0];
-            const uint8_t* pk = fr + dataPos;
+            const uint8_t* pk = fr.at(dataPos, plen);
             uint32_t card = p.wAux;
             uint32_t nb = card <= 1 ? card : 1 + (uint32_t)ozl_hb32(card - 1);
             if (plen == 0 || dataPos + plen > flen || pk[plen - 1] == 0 || nb == 0) ok = false;
@@ -336,13 +349,13 @@ OZL_HDN void ozl_dparse_frame(const uint8_t* fr, size_t flen, uint8_t* tab, ozl_
         if (ok && mainNc) {
             // maxSym from the NCount itself
             unsigned ms, tl;
-            long used = ozl_fse_read_ncount(L.norm, &ms, &tl, fr + dataPos, storedSize[0]);
+            long used = ozl_fse_read_ncount(L.norm, &ms, &tl, fr.at(dataPos, storedSize[0]), storedSize[0]);
             if (used < 0) ok = false;
             else p.maxSym = (uint8_t)ms;
         }
         if (ok && nestedNc) {
             unsigned ms, tl;
-            long used = ozl_fse_read_ncount(L.norm, &ms, &tl, fr + dataPos, storedSize[0]);
+            long used = ozl_fse_read_ncount(L.norm, &ms, &tl, fr.at(dataPos, storedSize[0]), storedSize[0]);
             if (used < 0) ok = false;
             else p.wMaxSym = (uint8_t)ms;
         }
@@ -385,21 +398,22 @@ OZL_HDN void ozl_dparse_frame(const uint8_t* fr, size_t flen, uint8_t* tab, ozl_
             d.payOff = soff[2]; d.payLen = st[2].size; d.nbStates = p.nbStates;
             unsigned ms, tl;
             if (nodes[1].trid == OZL_TR_FSE_NCOUNT) {
-                if (ozl_fse_read_ncount(L.norm, &ms, &tl, fr + soff[3], st[3].size) < 0) { d.status = OZL_ERR_CORRUPTION; break; }
+                if (ozl_fse_read_ncount(L.norm, &ms, &tl, fr.at(soff[3], st[3].size), st[3].size) < 0) { d.status = OZL_ERR_CORRUPTION; break; }
             } else {  // raw int16 counts after a num->serial conversion
                 uint32_t cnt = st[3].size / 2, sum = 0;
                 for (int s = 0; s < 256; s++) L.norm[s] = 0;
                 for (uint32_t s = 0; s < cnt; s++) {
                     L.norm[s] = (int16_t)(fr[soff[3] + 2 * s] | (fr[soff[3] + 2 * s + 1] << 8));
+                    if (L.norm[s] < -1) sum = 3;  // not a normalized count: rejected below
                     sum += L.norm[s] == -1 ? 1 : (uint32_t)L.norm[s];
                 }
-                if (cnt == 0 || sum == 0 || (sum & (sum - 1))) { d.status = OZL_ERR_CORRUPTION; break; }
+                if (cnt == 0 || cnt > 256 || sum == 0 || (sum & (sum - 1))) { d.status = OZL_ERR_CORRUPTION; break; }
                 ms = cnt - 1;
                 tl = (unsigned)ozl_hb32(sum);
             }
             if (tl < OZL_FSE_MIN_TABLELOG || tl > 11 || (d.nbStates != 2 && d.nbStates != 4)) { d.status = OZL_ERR_CORRUPTION; break; }
             d.tl = (uint8_t)tl;
-            if (ozl_fse_build_dtable(reinterpret_cast<uint32_t*>(tab), L.cells, L.norm, ms, tl)) d.status = OZL_ERR_CORRUPTION;
+            d.aux = ms;
             break;
         }
         case OZL_BK_HUF: {
@@ -410,7 +424,6 @@ OZL_HDN void ozl_dparse_frame(const uint8_t* fr, size_t flen, uint8_t* tab, ozl_
             int tl = ozl_huf_from_weights(L.wlen, L.wval, L.w, m);
             if (tl < 1 || tl > 11) { d.status = OZL_ERR_CORRUPTION; break; }
             d.tl = (uint8_t)tl;
-            *hufTl = tl;
             break;
         }
         default: break;

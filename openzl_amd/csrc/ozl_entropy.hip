@@ -20,6 +20,8 @@
 #include "ozl_frame.h"
 #include "ozl_profile.h"
 #include "ozl_entropy_api.h"
+#include "ozl_dparse.h"
+#include "ozl_wave.h"
 
 namespace {
 
@@ -55,12 +57,19 @@ struct PlanLds {
     uint32_t h4[4 * 256];
     uint32_t count[256];
     int16_t norm[256];
-    uint8_t cells[4096];
     ozl_hnode hn[2 * 256 + 2];
     uint8_t wts[256];
     uint32_t wcount[256];
-    uint16_t wst[4096];
+    uint8_t cells[1 << OZL_WEIGHTS_MAX_TABLELOG];   // nested weights graph
+    uint16_t wst[1 << OZL_WEIGHTS_MAX_TABLELOG];
     uint32_t wtt[256];
+    ozl_wave::FseBuild fb;                          // main FSE table build
+    uint16_t st[2048];
+    uint32_t tt[256];
+    uint8_t hlen[256];
+    uint16_t hval[256];
+    uint8_t aux[OZL_AUX_BYTES];
+    ozl_eplan plan;
 };
 
 // Nested FSE graph on the Huffman weights (entropyCompressChunk in EBM_fse
@@ -95,6 +104,8 @@ __device__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux)
         unsigned tl = ozl_fse_optimal_tablelog(OZL_FSE_DEFAULT_TABLELOG, m, wms, 2);
         p.wTl = (uint8_t)tl;
         int16_t wn[256];
+        // at most 256 weights of value <= 12: tableLog <= 5 (fse_compress.c:379-396)
+        if (tl > OZL_WEIGHTS_MAX_TABLELOG) { p.status = OZL_ERR_GENERIC; return; }
         if (ozl_fse_normalize(wn, tl, L.wcount, m, wms) != 0) { p.status = OZL_ERR_GENERIC; return; }
         if (ozl_fse_build_ctable(L.wst, L.wtt, L.cells, wn, wms, tl)) { p.status = OZL_ERR_GENERIC; return; }
         size_t bits = ozl_fse_encode_seq(pay, OZL_AUX_BYTES - OZL_AUX_W_PAYLOAD, L.wts, m, L.wst, L.wtt, tl, 2);
@@ -112,6 +123,10 @@ __device__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux)
     }
 }
 
+// One workgroup per block: histogram (all lanes), decision / normalisation /
+// Huffman code / nested weights graph (lane 0), FSE encoding table (wave 0,
+// ozl_wave::wave_fse_ctable); the tables and the auxiliary streams are built
+// in LDS and leave with coalesced stores.
 template <int HUF>
 __global__ void __launch_bounds__(kT) k_plan(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
                                              uint32_t B, ozl_ews ws)
@@ -122,80 +137,104 @@ __global__ void __launch_bounds__(kT) k_plan(const uint8_t* __restrict__ src, co
     const uint8_t* in = src + off[b];
     uint32_t n = (uint32_t)(off[b + 1] - off[b]);
     block_hist(in, n, L.h4, L.count);
-    if (threadIdx.x != 0) return;
-    ozl_eplan p;
-    uint32_t* pw = reinterpret_cast<uint32_t*>(&p);
-    for (size_t k = 0; k < sizeof(p) / 4; k++) pw[k] = 0;
-    p.n = n;
-    uint8_t* aux = ws.aux + (size_t)b * OZL_AUX_BYTES;
+    if (threadIdx.x == 0) {
+        ozl_eplan p = {};
+        p.n = n;
+        uint8_t* aux = L.aux;
+        unsigned maxSym, card;
+        int bk = ozl_decide_from_hist(L.count, n, HUF, &maxSym, &card);
+        if (n < OZL_MIN_STREAM_SIZE) bk = OZL_BK_STORE;
+        p.bk = (uint8_t)bk;
+        p.maxSym = (uint8_t)maxSym;
+        p.card = (uint16_t)card;
+        switch (bk) {
+        case OZL_BK_CONSTANT: aux[0] = (uint8_t)maxSym; p.aux = (uint16_t)maxSym; break;
+        case OZL_BK_FLATPACK: {
+            unsigned c = 0;
+            for (int s = 0; s < 256; s++)
+                if (L.count[s]) aux[c++] = (uint8_t)s;
+            p.aux = (uint16_t)c;
+            unsigned nb = c <= 1 ? c : 1 + (unsigned)ozl_hb32(c - 1);
+            p.payload = 1 + nb * n / 8;
+            break;
+        }
+        case OZL_BK_BITPACK: {
+            int nb = 1 + (maxSym ? ozl_hb32(maxSym) : 0);
+            p.aux = (uint16_t)nb;
+            p.payload = (uint32_t)(((size_t)n * nb + 7) / 8);
+            break;
+        }
+        case OZL_BK_FSE: {
+            unsigned tl = ozl_fse_optimal_tablelog(OZL_FSE_DEFAULT_TABLELOG, n, maxSym, 2);
+            p.tl = (uint8_t)tl;
+            p.nbStates = n < 1000 ? 2 : 4;
+            if (tl > 11 || ozl_fse_normalize(L.norm, tl, L.count, n, maxSym) != 0) { p.status = OZL_ERR_GENERIC; break; }
+            if (2u * (maxSym + 1u) < OZL_MIN_STREAM_SIZE) {
+                for (unsigned s = 0; s <= maxSym; s++) { aux[2 * s] = (uint8_t)L.norm[s]; aux[2 * s + 1] = (uint8_t)((uint16_t)L.norm[s] >> 8); }
+                p.aux = (uint16_t)(2 * (maxSym + 1));
+            } else {
+                p.aux = (uint16_t)ozl_fse_write_ncount(aux, L.norm, maxSym, tl);
+            }
+            break;
+        }
+        case OZL_BK_HUF: {
+            unsigned tl = ozl_fse_optimal_tablelog(OZL_HUF_DEFAULT_TABLELOG, n, maxSym, 1);
+            int mx = ozl_huf_build(L.hlen, L.hval, L.count, maxSym, tl, L.hn);
+            if (mx < 0) { p.status = OZL_ERR_GENERIC; break; }
+            p.tl = (uint8_t)mx;
+            p.nbStates = n > 1000 ? 1 : 0;
+            for (unsigned s = 0; s <= maxSym; s++) {
+                uint8_t w = L.hlen[s] ? (uint8_t)(mx + 1 - L.hlen[s]) : 0;
+                L.wts[s] = w;
+                aux[s] = w;
+            }
+            plan_weights(p, L, aux);
+            break;
+        }
+        default: break;
+        }
+        L.plan = p;
+    }
+    __syncthreads();
+    const ozl_eplan& p = L.plan;
     uint8_t* tab = ws.tables + (size_t)b * OZL_TAB_BYTES;
-    unsigned maxSym, card;
-    int bk = ozl_decide_from_hist(L.count, n, HUF, &maxSym, &card);
-    if (n < OZL_MIN_STREAM_SIZE) bk = OZL_BK_STORE;
-    p.bk = (uint8_t)bk;
-    p.maxSym = (uint8_t)maxSym;
-    p.card = (uint16_t)card;
-    switch (bk) {
-    case OZL_BK_CONSTANT: aux[0] = (uint8_t)maxSym; p.aux = (uint16_t)maxSym; break;
-    case OZL_BK_FLATPACK: {
-        unsigned c = 0;
-        for (int s = 0; s < 256; s++)
-            if (L.count[s]) aux[c++] = (uint8_t)s;
-        p.aux = (uint16_t)c;
-        unsigned nb = c <= 1 ? c : 1 + (unsigned)ozl_hb32(c - 1);
-        p.payload = 1 + nb * n / 8;
-        break;
-    }
-    case OZL_BK_BITPACK: {
-        int nb = 1 + (maxSym ? ozl_hb32(maxSym) : 0);
-        p.aux = (uint16_t)nb;
-        p.payload = (uint32_t)(((size_t)n * nb + 7) / 8);
-        break;
-    }
-    case OZL_BK_FSE: {
-        unsigned tl = ozl_fse_optimal_tablelog(OZL_FSE_DEFAULT_TABLELOG, n, maxSym, 2);
-        p.tl = (uint8_t)tl;
-        p.nbStates = n < 1000 ? 2 : 4;
-        if (ozl_fse_normalize(L.norm, tl, L.count, n, maxSym) != 0) { p.status = OZL_ERR_GENERIC; break; }
-        uint16_t* st = reinterpret_cast<uint16_t*>(tab);
+    if (p.bk == OZL_BK_FSE && p.status == 0) {
+        if (threadIdx.x < 64) ozl_wave::wave_fse_ctable(L.st, L.tt, L.norm, p.maxSym, p.tl, L.fb);
+        __syncthreads();
+        unsigned T = 1u << p.tl;
+        uint32_t* st32 = reinterpret_cast<uint32_t*>(tab);
+        const uint32_t* ls32 = reinterpret_cast<const uint32_t*>(L.st);
+        for (unsigned i = threadIdx.x; i < T / 2; i += kT) st32[i] = ls32[i];
         uint32_t* tt = reinterpret_cast<uint32_t*>(tab + 8192);
-        if (ozl_fse_build_ctable(st, tt, L.cells, L.norm, maxSym, tl)) { p.status = OZL_ERR_GENERIC; break; }
-        if (2u * (maxSym + 1u) < OZL_MIN_STREAM_SIZE) {
-            for (unsigned s = 0; s <= maxSym; s++) { aux[2 * s] = (uint8_t)L.norm[s]; aux[2 * s + 1] = (uint8_t)((uint16_t)L.norm[s] >> 8); }
-            p.aux = (uint16_t)(2 * (maxSym + 1));
-        } else {
-            p.aux = (uint16_t)ozl_fse_write_ncount(aux, L.norm, maxSym, tl);
-        }
-        break;
-    }
-    case OZL_BK_HUF: {
-        unsigned tl = ozl_fse_optimal_tablelog(OZL_HUF_DEFAULT_TABLELOG, n, maxSym, 1);
-        uint8_t* len = tab;
+        for (unsigned s = threadIdx.x; s <= p.maxSym; s += kT) tt[s] = L.tt[s];
+    } else if (p.bk == OZL_BK_HUF && p.status == 0) {
         uint16_t* val = reinterpret_cast<uint16_t*>(tab + 256);
-        int mx = ozl_huf_build(len, val, L.count, maxSym, tl, L.hn);
-        if (mx < 0) { p.status = OZL_ERR_GENERIC; break; }
-        p.tl = (uint8_t)mx;
-        p.nbStates = n > 1000 ? 1 : 0;
-        for (unsigned s = 0; s <= maxSym; s++) {
-            uint8_t w = len[s] ? (uint8_t)(mx + 1 - len[s]) : 0;
-            L.wts[s] = w;
-            aux[s] = w;
+        for (unsigned s = threadIdx.x; s < 256; s += kT) {
+            tab[s] = s <= p.maxSym ? L.hlen[s] : 0;
+            val[s] = s <= p.maxSym ? L.hval[s] : 0;
         }
-        plan_weights(p, L, aux);
-        break;
     }
-    default: break;
-    }
-    ws.plans[b] = p;
+    uint8_t* aux = ws.aux + (size_t)b * OZL_AUX_BYTES;
+    for (unsigned i = threadIdx.x; i < OZL_AUX_BYTES / 4; i += kT)
+        reinterpret_cast<uint32_t*>(aux)[i] = reinterpret_cast<const uint32_t*>(L.aux)[i];
+    if (threadIdx.x == 0) ws.plans[b] = p;
 }
 
 // ------------------------------------------------------------------ FSE encode
 // 16 blocks per 64-lane workgroup; lane q of a 4-lane group owns encoder state
 // q (fse_compress.c:607-660). Emission e (e = 0.. n-S-1) encodes position
 // p = n-S-1-e with state (e - n) mod S; a round gives each lane one emission,
-// the group combines the four (value, nbBits) pairs in emission order and lane
-// 0 appends them to a 32-bit-word writer.
+// the group gathers the four (value, nbBits) pairs in emission order (DPP) and
+// lane 0 appends them to a 32-bit-word writer. Each lane's input symbols are
+// loaded 8 rounds ahead of use (an unrolled register queue), so the global
+// load latency stays off the state chain.
 constexpr int kFseGroups = 16;
+constexpr int kFsePipe = 8;
+
+__device__ inline uint32_t sel4(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t i)
+{
+    return i == 0 ? a0 : i == 1 ? a1 : i == 2 ? a2 : a3;
+}
 
 __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
                                                    uint32_t B, ozl_ews ws)
@@ -205,12 +244,12 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
     const int lane = threadIdx.x & 63, g = lane >> 2, q = lane & 3;
     const uint32_t b = blockIdx.x * kFseGroups + g;
     bool active = b < B;
-    ozl_eplan p;
+    ozl_eplan p = {};
     if (active) {
         p = ws.plans[b];
         active = p.bk == OZL_BK_FSE && p.status == 0 && p.tl <= 11;
     }
-    const uint8_t* tab = ws.tables + (size_t)b * OZL_TAB_BYTES;
+    const uint8_t* tab = ws.tables + (size_t)(active ? b : 0) * OZL_TAB_BYTES;
     if (active) {
         unsigned ts = 1u << p.tl;
         const uint16_t* st = reinterpret_cast<const uint16_t*>(tab);
@@ -219,14 +258,13 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
         for (unsigned i = q; i <= p.maxSym; i += 4) sTT[g][i] = tt[i];
     }
     __syncthreads();
-    if (!active) return;
-    const uint8_t* in = src + off[b];
-    const uint32_t n = p.n, S = p.nbStates, tl = p.tl;
+    const uint8_t* in = src + (active ? off[b] : 0);
+    const uint32_t n = p.n, S = active ? p.nbStates : 4u, tl = p.tl;
     const uint16_t* ST = sST[g];
     const uint32_t* TT = sTT[g];
-    uint32_t* outw = reinterpret_cast<uint32_t*>(ws.payload + (size_t)b * ws.payloadCap);
+    uint32_t* outw = reinterpret_cast<uint32_t*>(ws.payload + (size_t)(active ? b : 0) * ws.payloadCap);
     const uint32_t capWords = (uint32_t)(ws.payloadCap / 4) - 2;
-    const bool mine = q < (int)S;
+    const bool mine = active && (uint32_t)q < S;
     // init: the one position among the last S whose state index is q
     uint32_t state = 0;
     if (mine) {
@@ -241,50 +279,77 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
         uint32_t v = (nbOut << 16) - dnb;
         state = ST[(int)(v >> nbOut) + ozl_tt_dfs(t)];
     }
-    const uint32_t nEmit = n - S;
-    const uint32_t kq = (uint32_t)(q + n) % S;  // this lane's slot inside a round
-    const int base = lane & ~3;
-    // for each slot, which lane owns it
-    int slotLane[4];
-    for (uint32_t k = 0; k < 4; k++) slotLane[k] = base + (int)((k + S - (n % S)) % S);
+    const uint32_t nEmit = active ? n - S : 0;
+    const uint32_t kq = active ? (uint32_t)(q + n) % S : 0u;  // this lane's slot inside a round
+    const uint32_t rot = active ? (S - (n % S)) % S : 0u;      // lane of slot k is (k + rot) % S
+    const uint32_t sl0 = rot % S, sl1 = (1 + rot) % S, sl2 = (2 + rot) % S, sl3 = (3 + rot) % S;
+    // rounds of the group with the most emissions set the loop length
+    uint32_t myRounds = active ? (nEmit + S - 1) / S : 0u;
+    uint32_t rounds = myRounds;
+    for (int d = 32; d >= 1; d >>= 1) {
+        uint32_t o = __shfl_xor(rounds, d, 64);
+        rounds = o > rounds ? o : rounds;
+    }
+    // every lane loads every round (invalid positions read a harmless byte),
+    // so the compiler sees a fixed number of loads in flight
+    const uint8_t* inSafe = active ? in : ws.payload;
+    uint32_t qbuf[kFsePipe];
+#pragma unroll
+    for (int k = 0; k < kFsePipe; k++) {
+        uint32_t e = (uint32_t)k * S + kq;
+        qbuf[k] = inSafe[(mine && e < nEmit) ? nEmit - 1 - e : 0u];
+    }
     uint64_t acc = 0;
     uint32_t accBits = 0, wpos = 0;
-    const uint32_t rounds = (nEmit + S - 1) / S;
-    for (uint32_t r = 0; r < rounds; r++) {
-        uint32_t e = r * S + kq;
-        uint32_t nb = 0, val = 0;
-        if (mine && e < nEmit) {
-            uint32_t t = TT[in[nEmit - 1 - e]];
-            nb = (state + ozl_tt_dnb(t)) >> 16;
-            val = state & ((1u << nb) - 1);
-            state = ST[(int)(state >> nb) + ozl_tt_dfs(t)];
-        }
-        // gather in emission order: two halves of at most 24 bits
-        uint32_t v0 = __shfl(val, slotLane[0], 64), n0 = __shfl(nb, slotLane[0], 64);
-        uint32_t v1 = __shfl(val, slotLane[1], 64), n1 = __shfl(nb, slotLane[1], 64);
-        uint32_t v2 = __shfl(val, slotLane[2], 64), n2 = __shfl(nb, slotLane[2], 64);
-        uint32_t v3 = __shfl(val, slotLane[3], 64), n3 = __shfl(nb, slotLane[3], 64);
-        if (S == 2) { n2 = n3 = 0; v2 = v3 = 0; }
-        acc |= (uint64_t)(v0 | (v1 << n0)) << accBits;
-        accBits += n0 + n1;
-        if (accBits >= 32) {
-            if (q == 0 && wpos < capWords) outw[wpos] = (uint32_t)acc;
-            wpos++;
-            acc >>= 32;
-            accBits -= 32;
-        }
-        acc |= (uint64_t)(v2 | (v3 << n2)) << accBits;
-        accBits += n2 + n3;
-        if (accBits >= 32) {
-            if (q == 0 && wpos < capWords) outw[wpos] = (uint32_t)acc;
-            wpos++;
-            acc >>= 32;
-            accBits -= 32;
+    for (uint32_t r0 = 0; r0 < rounds; r0 += kFsePipe) {
+#pragma unroll
+        for (int k = 0; k < kFsePipe; k++) {
+            const uint32_t r = r0 + (uint32_t)k;
+            const uint32_t e = r * S + kq;
+            const bool doit = mine && e < nEmit;
+            const uint32_t sym = qbuf[k] & 255u;
+            const uint32_t eNext = e + kFsePipe * S;
+            qbuf[k] = inSafe[(mine && eNext < nEmit) ? nEmit - 1 - eNext : 0u];
+            const uint32_t t = TT[sym];
+            const uint32_t nb = doit ? ((state + ozl_tt_dnb(t)) >> 16) & 31u : 0u;
+            const uint32_t packed = doit ? (state & ((1u << nb) - 1)) | (nb << 16) : 0u;
+            const int idx = doit ? (int)(state >> nb) + ozl_tt_dfs(t) : 0;
+            const uint32_t ns = ST[idx];
+            state = doit ? ns : state;
+            const uint32_t c0 = ozl_wave::quad_lane<0>(packed), c1 = ozl_wave::quad_lane<1>(packed);
+            const uint32_t c2 = ozl_wave::quad_lane<2>(packed), c3 = ozl_wave::quad_lane<3>(packed);
+            if (r < myRounds) {
+                uint32_t x0 = sel4(c0, c1, c2, c3, sl0), x1 = sel4(c0, c1, c2, c3, sl1);
+                uint32_t n0 = x0 >> 16, n1 = x1 >> 16;
+                acc |= (uint64_t)((x0 & 0xFFFF) | ((x1 & 0xFFFF) << n0)) << accBits;
+                accBits += n0 + n1;
+                if (accBits >= 32) {
+                    if (q == 0 && wpos < capWords) outw[wpos] = (uint32_t)acc;
+                    wpos++;
+                    acc >>= 32;
+                    accBits -= 32;
+                }
+                if (S == 4) {
+                    uint32_t x2 = sel4(c0, c1, c2, c3, sl2), x3 = sel4(c0, c1, c2, c3, sl3);
+                    uint32_t n2 = x2 >> 16, n3 = x3 >> 16;
+                    acc |= (uint64_t)((x2 & 0xFFFF) | ((x3 & 0xFFFF) << n2)) << accBits;
+                    accBits += n2 + n3;
+                    if (accBits >= 32) {
+                        if (q == 0 && wpos < capWords) outw[wpos] = (uint32_t)acc;
+                        wpos++;
+                        acc >>= 32;
+                        accBits -= 32;
+                    }
+                }
+            }
         }
     }
     // flush the states in index order, then the end mark
+    const uint32_t f0 = ozl_wave::quad_lane<0>(state), f1 = ozl_wave::quad_lane<1>(state);
+    const uint32_t f2 = ozl_wave::quad_lane<2>(state), f3 = ozl_wave::quad_lane<3>(state);
+    if (!active) return;
     for (uint32_t i = 0; i < S; i++) {
-        uint32_t sv = __shfl(state, base + (int)i, 64) & ((1u << tl) - 1);
+        uint32_t sv = sel4(f0, f1, f2, f3, i) & ((1u << tl) - 1);
         acc |= (uint64_t)sv << accBits;
         accBits += tl;
         if (accBits >= 32) {

@@ -1,0 +1,229 @@
+// ozl_wave.h -- wave-level (64-lane) building blocks of the entropy kernels:
+// lane masks and key matching with ballots, quad broadcasts (DPP), and the
+// parallel construction of FSE encoding / decoding tables and Huffman lookup
+// tables in LDS. Device only; every function is called by all 64 lanes of one
+// wave in uniform control flow (the other waves of its workgroup may idle).
+#pragma once
+#include "ozl_core.h"
+
+namespace ozl_wave {
+
+__device__ inline uint64_t lanes_below() { return (1ull << (threadIdx.x & 63)) - 1; }
+
+// Orders LDS accesses among the lanes of one wave (a wave's LDS operations
+// execute in order; this keeps the compiler from moving them across).
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// lanes among `act` whose 8-bit key equals this lane's
+__device__ inline uint64_t match8(uint32_t key, bool act)
+{
+    uint64_t m = __ballot(act);
+#pragma unroll
+    for (int bit = 0; bit < 8; bit++) {
+        uint64_t on = __ballot(act && ((key >> bit) & 1));
+        m &= ((key >> bit) & 1) ? on : ~on;
+    }
+    return m;
+}
+
+// value of lane K of this lane's quad (DPP quad_perm; every lane must run it)
+template <int K>
+__device__ inline uint32_t quad_lane(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, K * 0x55, 0xF, 0xF, false);
+}
+
+// ---------------------------------------------------------------- FSE decoding table
+struct FseBuild {
+    int16_t norm[256];
+    uint16_t cum[256];   // exclusive prefix of the positive counts
+    uint16_t next[256];  // per symbol: running rank base (decoder) or state slot (encoder)
+    uint8_t cell[2048];
+};
+
+// Symbol spread (fse_compress.c:93-155, fse_decompress.c:93-160) with one
+// wave: low-probability symbols take the top cells in symbol order; the k-th
+// occurrence of the other symbols (in symbol order) goes to the k-th position
+// of the sequence j*step mod T that is not above the low-probability cells.
+// Leaves the counts in S.norm (zero above maxSym) and, per symbol, the
+// exclusive prefix of the counts in S.next (-1 counting as 1) when
+// `prefixAll`, else the count itself (-1 as 1).
+static __device__ void wave_fse_spread(const int16_t* normSrc, unsigned maxSym, unsigned tl, FseBuild& S, bool prefixAll)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanes_below();
+    const unsigned T = 1u << tl, mask = T - 1, step = (T >> 1) + (T >> 3) + 3;
+    for (int s = lane; s < 256; s += 64) S.norm[s] = s <= (int)maxSym ? normSrc[s] : (int16_t)0;
+    wave_sync();
+    unsigned low = 0, carry = 0, carryAll = 0;
+    for (int c = 0; c < 4; c++) {
+        int s = c * 64 + lane;
+        int v = S.norm[s];
+        bool lp = v == -1;
+        uint64_t m = __ballot(lp);
+        if (lp) S.cell[T - 1 - (low + (unsigned)__popcll(m & lt))] = (uint8_t)s;
+        low += (unsigned)__popcll(m);
+        uint32_t cnt = v > 0 ? (uint32_t)v : 0u, cntAll = lp ? 1u : cnt;
+        uint32_t x = cnt, y = cntAll;
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t xs = __shfl_up(x, d, 64), ys = __shfl_up(y, d, 64);
+            if (lane >= d) { x += xs; y += ys; }
+        }
+        S.cum[s] = (uint16_t)(carry + x - cnt);
+        S.next[s] = (uint16_t)(prefixAll ? carryAll + y - cntAll : cntAll);
+        carry += __shfl(x, 63, 64);
+        carryAll += __shfl(y, 63, 64);
+    }
+    wave_sync();
+    const unsigned high = T - 1 - low;
+    unsigned run = 0;
+    for (unsigned j0 = 0; j0 < T; j0 += 64) {
+        unsigned j = j0 + lane;
+        unsigned pos = (j * step) & mask;
+        bool v = j < T && pos <= high;
+        uint64_t m = __ballot(v);
+        if (v) {
+            unsigned k = run + (unsigned)__popcll(m & lt);
+            unsigned s = 0;
+            for (unsigned half = 128; half; half >>= 1)
+                if (S.cum[s + half] <= k) s += half;
+            S.cell[pos] = (uint8_t)s;
+        }
+        run += (unsigned)__popcll(m);
+    }
+    wave_sync();
+}
+
+// FSE_buildDTable (fse_decompress.c:73-176): every cell's next state is its
+// rank among the cells of its symbol (in cell order) plus the symbol's count.
+// Entry: symbol | nbBits << 8 | newStateBase << 16.
+static __device__ void wave_fse_dtable(uint32_t* dt, const int16_t* normSrc, unsigned maxSym, unsigned tl, FseBuild& S)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanes_below();
+    const unsigned T = 1u << tl;
+    wave_fse_spread(normSrc, maxSym, tl, S, false);
+    for (unsigned u0 = 0; u0 < T; u0 += 64) {
+        unsigned u = u0 + lane;
+        bool act = u < T;
+        uint32_t s = act ? S.cell[u] : 0u;
+        uint64_t m = match8(s, act);
+        uint32_t base = act ? S.next[s] : 0u;
+        if (act) {
+            uint32_t ns = base + (uint32_t)__popcll(m & lt);
+            uint32_t nb = tl - (uint32_t)ozl_hb32(ns);
+            dt[u] = s | (nb << 8) | (((ns << nb) - T) << 16);
+        }
+        wave_sync();
+        if (act && (m & lt) == 0) S.next[s] = (uint16_t)(base + (uint32_t)__popcll(m));
+        wave_sync();
+    }
+}
+
+// FSE_buildCTable_wksp (fse_compress.c:69-215): the state table lists, per
+// symbol in symbol order, T + cell for its cells in cell order; the symbol
+// transforms are packed by ozl_tt_pack. st has T entries, tt maxSym+1.
+static __device__ void wave_fse_ctable(uint16_t* st, uint32_t* tt, const int16_t* normSrc, unsigned maxSym, unsigned tl,
+                                       FseBuild& S)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanes_below();
+    const unsigned T = 1u << tl;
+    wave_fse_spread(normSrc, maxSym, tl, S, true);
+    for (unsigned s = lane; s <= maxSym; s += 64) {
+        int c = S.norm[s];
+        int cumul = S.next[s];
+        if (c == 0) tt[s] = ozl_tt_pack(tl + 1, T, -1);
+        else if (c == -1 || c == 1) tt[s] = ozl_tt_pack(tl, T, cumul - 1);
+        else {
+            unsigned mb = tl - (unsigned)ozl_hb32((uint32_t)c - 1);
+            tt[s] = ozl_tt_pack(mb, (unsigned)c << mb, cumul - c);
+        }
+    }
+    wave_sync();
+    for (unsigned u0 = 0; u0 < T; u0 += 64) {
+        unsigned u = u0 + lane;
+        bool act = u < T;
+        uint32_t s = act ? S.cell[u] : 0u;
+        uint64_t m = match8(s, act);
+        uint32_t base = act ? S.next[s] : 0u;
+        if (act) st[base + (uint32_t)__popcll(m & lt)] = (uint16_t)(T + u);
+        wave_sync();
+        if (act && (m & lt) == 0) S.next[s] = (uint16_t)(base + (uint32_t)__popcll(m));
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------- Huffman lookup table
+struct HufBuild {
+    uint8_t len[256];
+    uint8_t list[256];    // symbols by code length, symbol order within a length
+    uint16_t start[16];   // first code value of each length
+    uint16_t off[16];     // first list index of each length
+    uint16_t cnt[16];
+};
+
+// Canonical codes of ozl_huf_from_weights (HUF_readDTableX1, huf_decompress.c:
+// 342-430): lengths tl..1 take consecutive value ranges from 0 upward, symbol
+// order within a length. Entry e of the lookup table (the next tl stream bits)
+// holds symbol | length << 8 of the code it starts with.
+static __device__ void wave_huf_lut(uint16_t* lut, const uint8_t* lenG, unsigned tl, HufBuild& S)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanes_below();
+    uint32_t run[13];
+#pragma unroll
+    for (int r = 0; r < 13; r++) run[r] = 0;
+    uint8_t myLen[4];
+    uint32_t myRank[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        int s = c * 64 + lane;
+        uint32_t l = lenG[s];
+        if (l > tl) l = 0;
+        myLen[c] = (uint8_t)l;
+        S.len[s] = (uint8_t)l;
+        uint32_t rk = 0;
+#pragma unroll
+        for (int r = 1; r <= 12; r++) {
+            uint64_t m = __ballot(l == (uint32_t)r);
+            if (l == (uint32_t)r) rk = run[r] + (uint32_t)__popcll(m & lt);
+            run[r] += (uint32_t)__popcll(m);
+        }
+        myRank[c] = rk;
+    }
+    if (lane == 0) {
+        uint32_t mn = 0, o = 0;
+        for (int r = (int)tl; r > 0; r--) {
+            S.start[r] = (uint16_t)mn;
+            mn = (mn + run[r]) >> 1;
+        }
+        for (int r = 1; r <= 12; r++) {
+            S.off[r] = (uint16_t)o;
+            S.cnt[r] = (uint16_t)run[r];
+            o += run[r];
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+        if (myLen[c]) S.list[S.off[myLen[c]] + myRank[c]] = (uint8_t)(c * 64 + lane);
+    wave_sync();
+    const unsigned T = 1u << tl;
+    for (unsigned e = lane; e < T; e += 64) {
+        uint32_t ent = 0;
+        for (unsigned r = 1; r <= tl; r++) {
+            uint32_t b0 = (uint32_t)S.start[r] << (tl - r), b1 = ((uint32_t)S.start[r] + S.cnt[r]) << (tl - r);
+            if (e >= b0 && e < b1) ent = S.list[S.off[r] + ((e - b0) >> (tl - r))] | (r << 8);
+        }
+        lut[e] = (uint16_t)ent;
+    }
+    wave_sync();
+}
+
+}  // namespace ozl_wave

@@ -65,15 +65,17 @@ size_t cu_flatpack(uint8_t* alphabet, uint8_t* packed, size_t* packedSize, const
 }
 
 // Frame recognition of the batched entropy decoder (ozl_dparse.h): returns
-// the plan's status and fills `plan` (the ozl_dplan bytes), the FSE decoding
-// table, or the Huffman code lengths and maximum length.
-int cu_dparse(const uint8_t* fr, size_t flen, void* plan, uint8_t* tab, uint8_t* hufLen, int* hufTl)
+// the plan's status and fills `plan` (the ozl_dplan bytes) and `side` (FSE
+// normalized counts int16[256] or Huffman code lengths u8[256]).
+int cu_dparse(const uint8_t* fr, size_t flen, void* plan, uint8_t* side)
 {
     static ozl_dlds L;
     ozl_dplan d;
-    ozl_dparse_frame(fr, flen, tab, L, d, hufTl);
+    ozl_fview v = { fr, fr, flen };
+    ozl_dparse_frame(v, flen, L, d);
     memcpy(plan, &d, sizeof(d));
-    memcpy(hufLen, L.wlen, 256);
+    if (d.status == 0 && d.bk == OZL_BK_FSE) memcpy(side, L.norm, 512);
+    if (d.status == 0 && d.bk == OZL_BK_HUF) memcpy(side, L.wlen, 256);
     return (int)d.status;
 }
 size_t cu_dplan_size(void) { return sizeof(ozl_dplan); }

@@ -138,3 +138,43 @@ def test_flatpack(cu):
             c2 = O.orc_flatpack_encode(ptr(a2), ptr(p2), ctypes.byref(s2), ptr(x), n)
             assert c1 == c2 and s1.value == s2.value
             assert np.array_equal(a1[:c1], a2[:c2]) and np.array_equal(p1[: s1.value], p2[: s2.value])
+
+
+def test_dparse_recognises_golden_frames(cu):
+    """The batched decoder's frame recognition (ozl_dparse.h, CPU build) on the
+    reference's FSE / Huffman frames below 100000 bytes: every frame is
+    accepted, and the normalized counts / code lengths it leaves decode the
+    frame's payload back to the input with the sequential decoders."""
+    import golden_cases as GC
+    P, SZ, U = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint
+    cu.cu_dparse.restype = ctypes.c_int
+    cu.cu_dparse.argtypes = [P, SZ, P, P]
+    cu.cu_dplan_size.restype = SZ
+    import struct
+    seen = {"fse": 0, "huf": 0}
+    for e in GC.small_entries():
+        if e["kind"] not in (1, 2) or e["input_bytes"] >= 100000:
+            continue
+        x = GC.make_input(e["gen"], e["kwargs"])
+        fr = np.frombuffer(GC.small_frame(e["name"]), np.uint8)
+        plan = np.zeros(cu.cu_dplan_size(), np.uint8)
+        side = np.zeros(512, np.uint8)
+        st = cu.cu_dparse(ptr(fr), fr.size, ptr(plan), ptr(side))
+        assert st == 0, e["name"]
+        n, bk, tl, nbs, _ = struct.unpack_from("<IBBBB", plan, 0)
+        payOff, payLen, _, _, aux = struct.unpack_from("<5I", plan, 8)
+        assert n == x.size
+        pay = fr[payOff:payOff + payLen].copy()
+        if bk == 5:  # FSE
+            norm = side.view(np.int16).copy()
+            out = np.zeros(max(n, 1), np.uint8)
+            assert cu.cu_fse_decode(ptr(out), n, ptr(pay), payLen, ptr(norm), aux, tl, nbs) == 0
+            assert out[:n].tobytes() == x.tobytes(), e["name"]
+            seen["fse"] += 1
+        elif bk == 4:  # Huffman: code lengths are those of the frame's weights
+            lens = side[:256]
+            assert lens.max() == tl and (lens[np.unique(x)] > 0).all(), e["name"]
+            kraft = sum(2.0 ** (tl - l) for l in lens if l)
+            assert kraft == 2 ** tl
+            seen["huf"] += 1
+    assert seen["fse"] >= 5 and seen["huf"] >= 5
