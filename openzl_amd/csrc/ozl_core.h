@@ -490,6 +490,16 @@ struct ozl_hnode {
     uint8_t nbBits;
 };
 
+// Working memory of ozl_huf_build: the tree and the small arrays the build
+// indexes at run time (kept together so that a kernel can place them in LDS).
+struct ozl_huf_scratch {
+    ozl_hnode tbl[2 * 256 + 2];
+    uint16_t base[192], curr[192];
+    int stk[32];
+    uint32_t rankLast[14];
+    uint16_t perRank[13], valPerRank[13];
+};
+
 OZL_HD uint32_t ozl_huf_bucket(uint32_t c) { return c < 166 ? c : (uint32_t)ozl_hb32(c) + 158; }
 
 // HUF_insertionSort (huf_compress.c:507-520): descending, stable.
@@ -510,9 +520,8 @@ OZL_HD void ozl_huf_insertion(ozl_hnode* a, int lo, int hi)
 // loop, recursing on the smaller side. Ranges handled by different calls are
 // disjoint, so running them in another order gives the same array; small
 // calls are therefore run at once and only large ones are stacked.
-OZL_HD void ozl_huf_sort_range(ozl_hnode* a, int lo0, int hi0)
+OZL_HD void ozl_huf_sort_range(ozl_hnode* a, int lo0, int hi0, int* stk)
 {
-    int stk[32];
     int sp = 0;
     stk[sp++] = lo0;
     stk[sp++] = hi0;
@@ -538,14 +547,16 @@ OZL_HD void ozl_huf_sort_range(ozl_hnode* a, int lo0, int hi0)
 
 // HUF_buildCTable_wksp (huf_compress.c:731-764): sort (:596-641), tree
 // (:657-704), height limit (:352-500), canonical codes (:706-729).
-// nodes: scratch of 2*256+2 entries. Returns the max code length or -1.
+// Returns the max code length or -1.
 OZL_HD int ozl_huf_build(uint8_t* nbBits, uint16_t* values, const uint32_t* count, unsigned maxSym,
-                         unsigned maxNbBits, ozl_hnode* tbl)
+                         unsigned maxNbBits, ozl_huf_scratch* W)
 {
+    ozl_hnode* tbl = W->tbl;
     ozl_hnode* node = tbl + 1;
     for (int k = 0; k < 2 * 256 + 2; k++) { tbl[k].count = 0; tbl[k].parent = 0; tbl[k].byte = 0; tbl[k].nbBits = 0; }
     {
-        uint16_t base[192], curr[192];
+        uint16_t* base = W->base;
+        uint16_t* curr = W->curr;
         for (int r = 0; r < 192; r++) base[r] = 0;
         for (unsigned s = 0; s <= maxSym; s++) base[ozl_huf_bucket(count[s])]++;
         for (int r = 191; r > 0; r--) base[r - 1] = (uint16_t)(base[r - 1] + base[r]);
@@ -558,7 +569,7 @@ OZL_HD int ozl_huf_build(uint8_t* nbBits, uint16_t* values, const uint32_t* coun
         }
         for (int r = 166; r < 191; r++) {
             int sz = curr[r] - base[r];
-            if (sz > 1) ozl_huf_sort_range(node + base[r], 0, sz - 1);
+            if (sz > 1) ozl_huf_sort_range(node + base[r], 0, sz - 1, W->stk);
         }
     }
     const int startNode = 256;
@@ -595,7 +606,7 @@ OZL_HD int ozl_huf_build(uint8_t* nbBits, uint16_t* values, const uint32_t* coun
         while (node[n].nbBits == target) --n;
         totalCost >>= (largest - target);
         const uint32_t none = 0xF0F0F0F0u;
-        uint32_t rankLast[14];
+        uint32_t* rankLast = W->rankLast;
         for (int k = 0; k < 14; k++) rankLast[k] = none;
         {
             unsigned cur = target;
@@ -639,7 +650,8 @@ OZL_HD int ozl_huf_build(uint8_t* nbBits, uint16_t* values, const uint32_t* coun
         largest = target;
     }
     if (largest > 12) return -1;
-    uint16_t perRank[13], valPerRank[13];
+    uint16_t* perRank = W->perRank;
+    uint16_t* valPerRank = W->valPerRank;
     for (int r = 0; r < 13; r++) { perRank[r] = 0; valPerRank[r] = 0; }
     for (int k = 0; k <= nonNull; k++) perRank[node[k].nbBits]++;
     {

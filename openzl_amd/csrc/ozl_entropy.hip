@@ -57,12 +57,13 @@ struct PlanLds {
     uint32_t h4[4 * 256];
     uint32_t count[256];
     int16_t norm[256];
-    ozl_hnode hn[2 * 256 + 2];
+    ozl_huf_scratch hs;
     uint8_t wts[256];
     uint32_t wcount[256];
     uint8_t cells[1 << OZL_WEIGHTS_MAX_TABLELOG];   // nested weights graph
     uint16_t wst[1 << OZL_WEIGHTS_MAX_TABLELOG];
     uint32_t wtt[256];
+    int16_t wn[256];
     ozl_wave::FseBuild fb;                          // main FSE table build
     uint16_t st[2048];
     uint32_t tt[256];
@@ -103,7 +104,7 @@ __device__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux)
     case OZL_BK_FSE: {
         unsigned tl = ozl_fse_optimal_tablelog(OZL_FSE_DEFAULT_TABLELOG, m, wms, 2);
         p.wTl = (uint8_t)tl;
-        int16_t wn[256];
+        int16_t* wn = L.wn;
         // at most 256 weights of value <= 12: tableLog <= 5 (fse_compress.c:379-396)
         if (tl > OZL_WEIGHTS_MAX_TABLELOG) { p.status = OZL_ERR_GENERIC; return; }
         if (ozl_fse_normalize(wn, tl, L.wcount, m, wms) != 0) { p.status = OZL_ERR_GENERIC; return; }
@@ -179,7 +180,7 @@ __global__ void __launch_bounds__(kT) k_plan(const uint8_t* __restrict__ src, co
         }
         case OZL_BK_HUF: {
             unsigned tl = ozl_fse_optimal_tablelog(OZL_HUF_DEFAULT_TABLELOG, n, maxSym, 1);
-            int mx = ozl_huf_build(L.hlen, L.hval, L.count, maxSym, tl, L.hn);
+            int mx = ozl_huf_build(L.hlen, L.hval, L.count, maxSym, tl, &L.hs);
             if (mx < 0) { p.status = OZL_ERR_GENERIC; break; }
             p.tl = (uint8_t)mx;
             p.nbStates = n > 1000 ? 1 : 0;

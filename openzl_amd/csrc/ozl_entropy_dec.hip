@@ -7,22 +7,23 @@
 //                  LDS, lane 0 parses it (ozl_dparse.h) and leaves the
 //                  normalized counts or the Huffman code lengths of the frame
 //                  in the workspace
-//   k_fse_decode   4 frames per 64-lane wave: the wave builds each frame's
-//                  decoding table in LDS (fse_decompress.c:73-176), then one
-//                  lane per interleaved state decodes (:255-338); the lanes of
-//                  a frame share one bit window, their read offsets come from
-//                  a quad prefix sum (DPP)
-//   k_huf_decode   4 frames per wave: the wave builds each frame's lookup
+//   k_fse_decode   6 frames per one-wave workgroup: the wave builds each
+//                  frame's decoding table in LDS (fse_decompress.c:73-176),
+//                  then a quad of lanes per frame decodes it, one lane per
+//                  interleaved state (:255-338); the lanes of a frame share
+//                  one bit position, their read offsets come from a quad
+//                  prefix sum (DPP)
+//   k_huf_decode   16 frames per wave: the wave builds each frame's lookup
 //                  table from the code lengths, then one lane per Huffman
 //                  stream decodes (huf_decompress.c:587-787)
 //   k_pack_decode  store / constant / flatpack / bitpack
 //   k_dverify      one wavefront per frame: content and compressed checksums
 //
-// Bit streams are read backwards through a 64-bit window per lane. Its bytes
-// come from a ring in LDS that every lane of the wave refills once per phase
-// of 32 rounds with aligned 16-byte loads issued one phase ahead of use, so
-// no global-memory latency sits on a decoding lane's dependency chain.
-// Decoded bytes leave in 4-byte stores.
+// Bit streams are read backwards through 64-bit windows over a ring in LDS
+// per stream, which the stream's lanes refill from HBM with aligned 16-byte
+// loads issued one staging step ahead of use, so no global-memory latency
+// sits on a decoding lane's dependency chain. Decoded bytes leave in 4-byte
+// stores.
 #include "ozl_internal.h"
 #include "ozl_core.h"
 #include "ozl_xxh3.h"
@@ -78,256 +79,413 @@ __global__ void __launch_bounds__(kParseT) k_dparse(const uint8_t* __restrict__ 
         for (int s = threadIdx.x; s < 256; s += kParseT) side[s] = L.wlen[s];
 }
 
-// ---------------------------------------------------------------- LDS rings of stream bytes
-// A ring of R bytes holds stream bytes by absolute address (slot = address
-// mod R). read8 returns the 8 bytes [A-8, A) as a little-endian u64.
-template <int R>
-__device__ inline uint64_t ring_read8(const uint32_t* ring, uint64_t A)
+__device__ inline uint64_t floor16(uint64_t a) { return a & ~15ull; }
+
+// ---------------------------------------------------------------- self-staged backward bit readers
+// Each decoding lane owns one stream and a ring of R bytes in LDS holding the
+// stream by absolute address (slot = address mod R). Every P rounds the lane
+// stores the aligned 16-byte blocks it loaded P rounds earlier and issues the
+// loads of the next blocks below, keeping LOOK bytes staged below its window;
+// bytes below the stream start read as zero. The window is 64 bits,
+// MSB-aligned, refilled 32 bits at a time from 4-aligned addresses.
+// global-memory accesses through address-space-1 pointers (global_load /
+// global_store rather than flat, which the compiler must order against LDS)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const u32x4 gc_u32x4;
+typedef __attribute__((address_space(1))) const uint32_t gc_uint32;
+__device__ inline uint4 gload16(uint64_t a)
 {
-    constexpr uint32_t W = R / 4 - 1;
-    uint64_t a0 = A - 8;
-    uint32_t w = (uint32_t)(a0 >> 2);
-    uint32_t d0 = ring[w & W], d1 = ring[(w + 1) & W], d2 = ring[(w + 2) & W];
-    uint32_t sh = (uint32_t)(a0 & 3) * 8;
-    uint64_t lo = (uint64_t)d0 | ((uint64_t)d1 << 32);
-    return sh ? (lo >> sh) | ((uint64_t)d2 << (64 - sh)) : lo;
+    u32x4 v = *(gc_u32x4*)(uintptr_t)a;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ inline uint32_t gload4(uint64_t a) { return *(gc_uint32*)(uintptr_t)a; }
+
+template <int R>
+__device__ inline uint32_t ring_dword(const uint32_t* ring, uint64_t a)
+{
+    return ring[(uint32_t)(a >> 2) & (R / 4 - 1)];
 }
 
-// Stages the 16-byte aligned block at absolute address `a` (loaded earlier into
-// `v`) into a ring; bytes below `lo` (the stream start) read as zero.
+struct Stager {
+    uint64_t lo;     // stream start
+    uint64_t aLo;    // lowest staged address (16-aligned)
+    uint4 pend[3];
+    uint64_t pa[3];
+    uint32_t np;
+};
+
+// initial fill of [max(floor16(top - look), floor16(lo - 8)), top), by this lane alone
+// The dwords of [a0, ceil4(top)) are loaded 8 at a time (the last one may
+// cover up to 3 bytes past the stream, which the window never reads; the
+// frame holds at least its end byte after every stream).
 template <int R>
-__device__ inline void ring_store16(uint32_t* ring, uint64_t a, uint4 v, uint64_t lo)
+__device__ inline void stage_init(uint32_t* ring, Stager& g, uint64_t lo, uint64_t top, uint64_t look)
 {
     constexpr uint32_t W = R / 4 - 1;
-    uint32_t d[4] = { v.x, v.y, v.z, v.w };
-    uint32_t w = (uint32_t)(a >> 2);
+    uint64_t a0 = floor16(top - look), f = floor16(lo - 8);
+    if (a0 < f) a0 = f;
+    uint64_t a1 = (top + 3) & ~3ull;
+    for (uint64_t a = a0; a < a1; a += 32) {
+        uint32_t d[8];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        uint64_t da = a + 4 * k;
-        uint32_t x = d[k];
-        if (da + 4 <= lo) x = 0;
-        else if (da < lo) x &= ~0u << (8 * (uint32_t)(lo - da));
-        ring[(w + k) & W] = x;
+        for (int j = 0; j < 8; j++) {
+            uint64_t da = a + 4 * j;
+            d[j] = (da < a1 && da + 4 > lo) ? gload4(da) : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            uint64_t da = a + 4 * j;
+            uint32_t x = d[j];
+            if (da < lo) x = da + 4 <= lo ? 0u : x & (~0u << (8 * (uint32_t)(lo - da)));
+            if (da < a1) ring[(uint32_t)(da >> 2) & W] = x;
+        }
     }
+    g.lo = lo;
+    g.aLo = a0;
+    g.np = 0;
 }
 
-// Loads the aligned block at `a` unless it lies wholly below the stream start.
-__device__ inline uint4 load_block(uint64_t a, uint64_t lo)
+template <int R, int MAXB>
+__device__ inline void stage_step(uint32_t* ring, Stager& g, uint64_t winLo, uint64_t look)
 {
-    if (a + 16 <= lo) return make_uint4(0, 0, 0, 0);
-    return *reinterpret_cast<const uint4*>((uintptr_t)a);
+    constexpr uint32_t W = R / 4 - 1;
+#pragma unroll
+    for (int k = 0; k < MAXB; k++) {
+        if ((uint32_t)k >= g.np) break;
+        uint32_t d[4] = { g.pend[k].x, g.pend[k].y, g.pend[k].z, g.pend[k].w };
+        uint32_t w = (uint32_t)(g.pa[k] >> 2);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            uint64_t da = g.pa[k] + 4 * j;
+            uint32_t x = d[j];
+            if (da + 4 <= g.lo) x = 0;
+            else if (da < g.lo) x &= ~0u << (8 * (uint32_t)(g.lo - da));
+            ring[(w + j) & W] = x;
+        }
+    }
+    uint64_t want = floor16(winLo - look), f = floor16(g.lo - 8);
+    if (want < f) want = f;
+    uint32_t np = 0;
+#pragma unroll
+    for (int k = 0; k < MAXB; k++) {
+        uint64_t a = g.aLo - 16 * (uint64_t)(k + 1);
+        if (a >= want && a < g.aLo) {
+            g.pa[k] = a;
+            g.pend[k] = a + 16 <= g.lo ? make_uint4(0, 0, 0, 0) : gload16(a);
+            np = k + 1;
+        }
+    }
+    g.np = np;
+    g.aLo -= 16 * (uint64_t)np;
 }
 
-// Backward bit window: `acc` holds the `valid` stream bits just below bit
-// position 8*lb + valid, most recent first (MSB-aligned); lb is the byte
-// offset of the window's low end inside the stream (may go below 0, where the
-// stream reads as zeros).
-struct BitWin {
+struct Win {
     uint64_t acc;
     int valid;
-    int lb;
+    int lb;  // byte offset of the window bottom in the stream (4-aligned address after init)
 };
 
-// Refill from the 8 bytes below the window (`v`, read before the window last
-// moved); applied only when `need` (select-based, no branch).
-__device__ inline void win_refill_with(BitWin& w, uint64_t v, bool need)
+// Window start: the top 4..7 bytes of the stream (so that the bottom lands on
+// a 4-aligned address), minus the bits above and including the end mark.
+// Returns false when the last byte is zero.
+template <int R>
+__device__ inline bool win_init(Win& w, const uint32_t* ring, uint64_t s, uint32_t len)
 {
-    int k = (64 - w.valid) >> 3;
-    if (w.lb <= 0) v = 0;
-    uint64_t keep = k >= 8 ? ~0ull : (k <= 0 ? 0ull : ~((1ull << (64 - 8 * k)) - 1));
-    uint64_t top = v & keep;
-    uint64_t acc = w.acc | (w.valid >= 64 ? 0ull : (w.valid ? (top >> w.valid) : top));
-    w.acc = need ? acc : w.acc;
-    w.valid = need ? w.valid + 8 * k : w.valid;
-    w.lb = need ? w.lb - k : w.lb;
+    uint64_t top = s + len;
+    uint64_t a0 = (top - 8) & ~3ull;
+    uint32_t d0 = ring_dword<R>(ring, a0), d1 = ring_dword<R>(ring, a0 + 4), d2 = ring_dword<R>(ring, a0 + 8);
+    uint32_t sh = (uint32_t)((top - 8) & 3) * 8;
+    uint64_t lo64 = (uint64_t)d0 | ((uint64_t)d1 << 32);
+    uint64_t v = sh ? (lo64 >> sh) | ((uint64_t)d2 << (64 - sh)) : lo64;  // bytes [top-8, top)
+    uint32_t take = 4 + (uint32_t)(top & 3);
+    w.acc = v & ~((1ull << (64 - 8 * take)) - 1);
+    w.valid = (int)(8 * take);
+    w.lb = (int)len - (int)take;
+    uint32_t topByte = (uint32_t)(w.acc >> 56);
+    if (len == 0 || topByte == 0) return false;
+    int skip = 8 - ozl_hb32(topByte);
+    w.acc <<= skip;
+    w.valid -= skip;
+    return true;
 }
 
-template <int R>
-__device__ inline void win_refill(BitWin& w, const uint32_t* ring, uint64_t s)
+// adds the 32 bits below the window when it holds <= 32 (the dword `d` was read
+// at the window bottom before it moved)
+__device__ inline void win_refill32(Win& w, uint32_t d)
 {
-    int k = (64 - w.valid) >> 3;
-    uint64_t v = w.lb > 0 ? ring_read8<R>(ring, s + (uint64_t)w.lb) : 0;
-    uint64_t keep = k >= 8 ? ~0ull : (k <= 0 ? 0ull : ~((1ull << (64 - 8 * k)) - 1));
-    uint64_t top = v & keep;
-    w.acc |= w.valid >= 64 ? 0ull : (w.valid ? (top >> w.valid) : top);
-    w.valid += 8 * k;
-    w.lb -= k;
+    bool need = w.valid <= 32;
+    uint32_t x = w.lb > 0 ? d : 0u;
+    uint64_t add = need ? ((uint64_t)x << (32 - w.valid)) : 0ull;
+    w.acc |= add;
+    w.valid += need ? 32 : 0;
+    w.lb -= need ? 4 : 0;
 }
+
+__device__ inline uint32_t win_hi(const Win& w) { return (uint32_t)(w.acc >> 32); }
 
 // ---------------------------------------------------------------- FSE decoding
-constexpr int kFF = 4;          // frames per workgroup, 4 lanes each
-constexpr int kFRing = 512;     // staged stream bytes per frame
-constexpr int kFLook = 384;     // bytes kept staged below a window
-constexpr int kPhase = 32;      // rounds between staging steps
+// A quad of lanes decodes one frame of 4 interleaved states
+// (fse_decompress.c:255-338), lane q holding state q: per round every lane
+// looks its state up, the quad adds up its 4 bit counts (each lane's count
+// placed in byte q, summed over the quad with two DPP exchanges, then a
+// multiply by 0x01010101 turns the bytes into prefix sums), and lane q takes
+// its bits below those of the states before it from a 64-bit window that every
+// lane reads from the frame's LDS ring at the quad's bit position. After 4
+// rounds the quad's 16 symbols are transposed across its lanes (two DPP byte
+// permutes) into one dword per lane. Frames of 2 states are decoded by lane 0
+// of their quad, state after state. 6 frames per one-wave workgroup: tables
+// (16-bit entries and the symbol of every cell) and rings take 40 KB, so 4
+// workgroups (one per SIMD) share a CU.
+constexpr int kFQ = 6;
+constexpr int kFRing = 256;
+constexpr int kFRingW = kFRing / 4;
+constexpr int kFLook = 128;     // one-lane frames
+constexpr int kFPhase = 8;      // one-lane frames: rounds between staging steps (<= 44 bytes consumed)
+constexpr int kQLook = 192;     // quads: bytes kept staged below the window
+constexpr int kQPhase = 16;     // quads: rounds between staging steps (<= 88 bytes consumed)
 
 struct FseDecLds {
-    uint32_t dt[kFF][2048];
-    uint32_t ring[kFF][kFRing / 4];
-    FseBuild bs;
-    ozl_dplan plan[kFF];
-    uint64_t aLo[kFF];    // lowest staged address (16-aligned)
-    uint64_t winLo[kFF];  // window bottom at the start of the phase
+    uint16_t dt[kFQ][2048];
+    uint8_t sym[kFQ][2048];
+    uint32_t ring[kFQ][kFRingW + 4];  // dwords kFRingW, kFRingW+1 mirror 0, 1
+    FseBuildHdr bs;
+    ozl_dplan plan[kFQ];
 };
 
-__device__ inline uint64_t floor16(uint64_t a) { return a & ~15ull; }
+// A frame's quad stages its stream into the ring 128 bytes per step, lane q
+// loading blocks q and q+4 of the 16-byte blocks below the staged range; each
+// step first stores the blocks the previous step loaded (kQPhase rounds
+// earlier, so their latency is hidden). The initial fill covers the top
+// kQLook bytes. Blocks may hold bytes below the stream start (down to 8 bytes
+// below it, which the frame header precedes) and above its end: the window
+// reads them, but a valid stream never takes their bits.
+struct QStage {
+    uint64_t aLo;     // lowest address loaded or requested (16-aligned)
+    uint64_t f;       // floor16(stream start - 8)
+    uint4 p0, p1;     // blocks loaded by the last step
+    uint64_t a0, a1;  // their addresses (0: none)
+};
+
+typedef __attribute__((address_space(1))) uint32_t g_uint32;
+typedef __attribute__((address_space(1))) uint8_t g_uint8;
+
+__device__ inline void qring_put(uint32_t* ring, uint64_t a, uint4 v)
+{
+    const uint32_t w = (uint32_t)(a >> 2) & (kFRingW - 1);
+    *reinterpret_cast<uint4*>(ring + w) = v;
+    if (w == 0) *reinterpret_cast<uint2*>(ring + kFRingW) = make_uint2(v.x, v.y);
+}
+
+__device__ inline void qstage_init(uint32_t* ring, QStage& g, uint64_t lo, uint64_t top, uint32_t q)
+{
+    g.f = floor16(lo - 8);
+    uint64_t a0 = floor16(top - kQLook);
+    if (a0 < g.f) a0 = g.f;
+    for (uint64_t a = a0 + 16 * q; a < top; a += 64) qring_put(ring, a, gload16(a));
+    g.aLo = a0;
+    g.a0 = g.a1 = 0;
+}
+
+__device__ inline void qstage_step(uint32_t* ring, QStage& g, uint64_t winLo, uint32_t q)
+{
+    if (g.a0) qring_put(ring, g.a0, g.p0);
+    if (g.a1) qring_put(ring, g.a1, g.p1);
+    uint64_t want = floor16(winLo - kQLook);
+    if (want < g.f) want = g.f;
+    uint64_t nb = g.aLo > want ? (g.aLo - want) >> 4 : 0;
+    if (nb > 8) nb = 8;
+    const uint64_t x0 = g.aLo - 16 * (uint64_t)(q + 1), x1 = x0 - 64;
+    g.p0 = gload16(q < nb ? x0 : g.f);
+    g.p1 = gload16(q + 4 < nb ? x1 : g.f);
+    g.a0 = q < nb ? x0 : 0;
+    g.a1 = q + 4 < nb ? x1 : 0;
+    g.aLo -= 16 * nb;
+}
+
+// the 64 stream bits below absolute bit address Q (mod 2^32)
+__device__ inline uint64_t qwindow(const uint32_t* ring, uint32_t Q)
+{
+    const uint32_t j = ((Q >> 5) - 2) & (kFRingW - 1);
+    const uint32_t d0 = ring[j], d1 = ring[j + 1], d2 = ring[j + 2];
+    return ((uint64_t)__builtin_amdgcn_alignbit(d2, d1, Q) << 32) | __builtin_amdgcn_alignbit(d1, d0, Q);
+}
+
+// Frames of any state count, one lane: per round one lookup per state, then
+// the states take their bits from the window in state order; the window is
+// refilled after each pair of states so that it always holds the <= 22 bits a
+// pair reads.
+__device__ bool fse_decode_lane(const ozl_dplan& P, const uint16_t* DT, const uint8_t* SY, uint32_t* ring)
+{
+    const uint32_t S = P.nbStates, n = P.n, tl = P.tl;
+    const uint64_t s = P.srcPtr;
+    uint8_t* out = reinterpret_cast<uint8_t*>((uintptr_t)P.outPtr);
+    Stager g;
+    stage_init<kFRing>(ring, g, s, s + P.payLen, kFLook);
+    Win w;
+    bool ok = win_init<kFRing>(w, ring, s, P.payLen);
+    uint32_t st[4] = { 0, 0, 0, 0 };
+    if (ok) {
+        win_refill32(w, ring_dword<kFRing>(ring, s + (uint64_t)(int64_t)w.lb - 4));
+        for (uint32_t j = 0; j < S; j++) {
+            st[j] = win_hi(w) >> (32 - tl);
+            w.acc <<= tl;
+            w.valid -= (int)tl;
+            if (j == 1) win_refill32(w, ring_dword<kFRing>(ring, s + (uint64_t)(int64_t)w.lb - 4));
+        }
+        win_refill32(w, ring_dword<kFRing>(ring, s + (uint64_t)(int64_t)w.lb - 4));
+    }
+    const uint32_t R = ok ? (n + S - 1) / S : 0u;
+    for (uint32_t r = 0; r < R; r++) {
+        if ((r & (kFPhase - 1)) == 0) stage_step<kFRing, 3>(ring, g, (uint64_t)((int64_t)s + w.lb), kFLook);
+        for (uint32_t j = 0; j < S; j++) {
+            const uint32_t p = r * S + j;
+            const uint32_t d = DT[st[j]];
+            if (p < n) out[p] = SY[st[j]];
+            // a state updates unless its symbol is among the last S
+            const uint32_t nb = p + S < n ? d & 15u : 0u;
+            const uint32_t bits = __builtin_amdgcn_ubfe(win_hi(w), 32 - nb, nb);
+            w.acc <<= nb;
+            w.valid -= (int)nb;
+            if (p + S < n) st[j] = (d >> 4) + bits;
+            if (j & 1) win_refill32(w, ring_dword<kFRing>(ring, s + (uint64_t)(int64_t)w.lb - 4));
+        }
+        if (S & 1) win_refill32(w, ring_dword<kFRing>(ring, s + (uint64_t)(int64_t)w.lb - 4));
+    }
+    long pos = 8L * w.lb + w.valid;
+    return ok && pos == 0;
+}
+
+// one round of a 4-state frame's quad: returns this lane's symbol; TAIL
+// rounds keep the states of the last 4 symbols (fse_decompress.c:287-336)
+template <bool TAIL>
+__device__ inline uint32_t fse_quad_round(const uint16_t* DT, const uint8_t* SY, const uint32_t* ring, uint32_t& st,
+                                          uint64_t& W, uint32_t& Q, uint32_t sq, uint32_t p, uint32_t n)
+{
+    const uint32_t e = DT[st], y = SY[st];
+    const bool upd = !TAIL || p + 4 < n;
+    const uint32_t nb = upd ? e & 15u : 0u;
+    uint32_t v = nb << sq;
+    v += quad_xor1(v);
+    v += quad_xor2(v);
+    const uint32_t M = v * 0x01010101u;           // byte k: bits of states 0..k
+    const uint32_t incl = __builtin_amdgcn_ubfe(M, sq, 8);
+    const uint32_t bits = __builtin_amdgcn_ubfe((uint32_t)(W >> ((64 - incl) & 63)), 0, nb);
+    st = upd ? (e >> 4) + bits : st;
+    Q -= M >> 24;
+    W = qwindow(ring, Q);
+    return y;
+}
 
 __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
 {
     __shared__ FseDecLds L;
     const int lane = threadIdx.x;
-    const uint32_t b0 = blockIdx.x * kFF;
-    {
-        int f = lane >> 4, k = lane & 15;
+    const uint32_t b0 = blockIdx.x * kFQ;
+    for (int i = lane; i < kFQ * 16; i += 64) {
+        int f = i >> 4, k = i & 15;
         reinterpret_cast<uint32_t*>(&L.plan[f])[k] =
             b0 + f < B ? reinterpret_cast<const uint32_t*>(&ws.plans[b0 + f])[k] : 0u;
     }
-    __syncthreads();
+    wave_sync();
     uint32_t factMask = 0;
-    uint32_t R = 0;
-#pragma unroll
-    for (int f = 0; f < kFF; f++) {
+    for (int f = 0; f < kFQ; f++) {
         const ozl_dplan& P = L.plan[f];
-        if (b0 + f < B && P.status == 0 && P.bk == OZL_BK_FSE) {
-            factMask |= 1u << f;
-            uint32_t rf = (P.n + P.nbStates - 1) / P.nbStates;
-            R = rf > R ? rf : R;
-        }
+        if (b0 + f < B && P.status == 0 && P.bk == OZL_BK_FSE) factMask |= 1u << f;
     }
-    if (R == 0) return;
-#pragma unroll
-    for (int f = 0; f < kFF; f++) {
+    if (!factMask) return;
+    for (int f = 0; f < kFQ; f++) {
         if (!((factMask >> f) & 1)) continue;
-        const ozl_dplan& P = L.plan[f];
-        wave_fse_dtable(L.dt[f], reinterpret_cast<const int16_t*>(ws.side + (size_t)(b0 + f) * OZL_DSIDE_BYTES), P.aux,
-                        P.tl, L.bs);
-        uint64_t s = P.srcPtr, top = s + P.payLen;
-        uint64_t a0 = floor16(top - kFLook), lo = floor16(s - 8);
-        if (a0 < lo) a0 = lo;
-        uint8_t* rb = reinterpret_cast<uint8_t*>(L.ring[f]);
-        for (uint64_t a = a0 + lane; a < top; a += 64) rb[a & (kFRing - 1)] = a >= s ? *reinterpret_cast<const uint8_t*>((uintptr_t)a) : 0;
-        if (lane == 0) L.aLo[f] = a0;
+        wave_fse_dtable16(L.dt[f], L.sym[f], reinterpret_cast<const int16_t*>(ws.side + (size_t)(b0 + f) * OZL_DSIDE_BYTES),
+                          L.plan[f].aux, L.plan[f].tl, L.bs);
     }
-    __syncthreads();
-
-    // decoding lanes: frame fd, state q
-    const int fd = lane < 4 * kFF ? lane >> 2 : 0, q = lane & 3;
-    const bool dl = lane < 4 * kFF && ((factMask >> fd) & 1);
-    const ozl_dplan& P = L.plan[fd];
-    const uint32_t S = dl ? P.nbStates : 4u, n = P.n, tl = P.tl;
-    const uint32_t* DT = L.dt[fd];
-    const uint32_t* ring = L.ring[fd];
-    const uint64_t s = P.srcPtr;
+    wave_sync();
+    const int f = lane >> 2;
+    const uint32_t q = lane & 3;
+    if (f >= kFQ || !((factMask >> f) & 1)) return;
+    const ozl_dplan P = L.plan[f];
+    const uint16_t* DT = L.dt[f];
+    const uint8_t* SY = L.sym[f];
+    uint32_t* ring = L.ring[f];
+    if (P.nbStates != 4) {
+        if (q == 0 && !fse_decode_lane(P, DT, SY, ring)) ws.plans[b0 + f].status = OZL_ERR_CORRUPTION;
+        return;
+    }
+    const uint32_t n = P.n, tl = P.tl, sq = 8 * q;
+    const uint64_t s = P.srcPtr, top = s + P.payLen;
     uint8_t* out = reinterpret_cast<uint8_t*>((uintptr_t)P.outPtr);
-    const bool wide = dl && S == 4 && (P.outPtr & 3) == 0;
-    BitWin w = { 0, 0, (int)P.payLen };
-    uint32_t state = 0;
-    bool ok = dl;
-    if (dl) {
-        win_refill<kFRing>(w, ring, s);
-        uint32_t topByte = (uint32_t)(w.acc >> 56);
-        if (topByte == 0) ok = false;
-        else {
-            int skip = 8 - ozl_hb32(topByte);
-            w.acc <<= skip;
-            w.valid -= skip;
-            if ((uint32_t)q < S) state = (uint32_t)((w.acc << (q * tl)) >> (64 - tl));
-            w.acc <<= S * tl;
-            w.valid -= (int)(S * tl);
-            win_refill<kFRing>(w, ring, s);
+    const bool wide = (P.outPtr & 3) == 0;
+    // byte transpose of the quad's 4x4 symbols: pairs of lanes swap bytes,
+    // then pairs of pairs swap halves
+    const uint32_t selA = (q & 1) ? 0x03070105u : 0x06020400u;
+    const uint32_t selB = (q & 2) ? 0x03020706u : 0x05040100u;
+    QStage g;
+    qstage_init(ring, g, s, top, q);
+    wave_sync();
+    // end mark: the highest set bit of the last byte
+    uint64_t Pb = 8 * top;
+    uint64_t W = qwindow(ring, (uint32_t)Pb);
+    const uint32_t topByte = (uint32_t)(W >> 56);
+    bool ok = P.payLen != 0 && topByte != 0;
+    Pb -= 8 - (uint32_t)ozl_hb32(topByte | 1);
+    W = qwindow(ring, (uint32_t)Pb);
+    // initial states, state 0 in the highest tl bits
+    uint32_t st = __builtin_amdgcn_ubfe((uint32_t)(W >> (64 - (q + 1) * tl)), 0, tl);
+    Pb -= 4 * tl;
+    uint32_t Q = (uint32_t)Pb;
+    W = qwindow(ring, Q);
+    const uint32_t G = ok && n > 19 ? (n - 20) / 16 + 1 : 0u;
+    for (uint32_t gi = 0; gi < G; gi++) {
+        if ((gi & (kQPhase / 4 - 1)) == 0) {
+            qstage_step(ring, g, (Pb >> 3) - 12, q);
+            wave_sync();
+        }
+        const uint32_t Q0 = Q;
+        uint32_t acc = fse_quad_round<false>(DT, SY, ring, st, W, Q, sq, 0, n);
+        acc |= fse_quad_round<false>(DT, SY, ring, st, W, Q, sq, 0, n) << 8;
+        acc |= fse_quad_round<false>(DT, SY, ring, st, W, Q, sq, 0, n) << 16;
+        acc |= fse_quad_round<false>(DT, SY, ring, st, W, Q, sq, 0, n) << 24;
+        Pb -= Q0 - Q;
+        const uint32_t c = __builtin_amdgcn_perm(quad_xor1(acc), acc, selA);
+        const uint32_t d = __builtin_amdgcn_perm(quad_xor2(c), c, selB);
+        uint8_t* o = out + 16 * (size_t)gi + 4 * q;
+        if (wide) {
+            *(g_uint32*)(uintptr_t)o = d;
+        } else {
+            g_uint8* ob = (g_uint8*)(uintptr_t)o;
+            ob[0] = (uint8_t)d;
+            ob[1] = (uint8_t)(d >> 8);
+            ob[2] = (uint8_t)(d >> 16);
+            ob[3] = (uint8_t)(d >> 24);
         }
     }
-    bool pendOn = false;
-    uint64_t pa = 0;
-    int pf = 0;
-    uint4 pend = make_uint4(0, 0, 0, 0);
-    uint32_t wacc = 0;
-    const uint32_t R4 = (R + 3) & ~3u;
-    for (uint32_t r = 0; r < R4; r++) {
-        if (r % kPhase == 0) {
-            if (dl && q == 0) L.winLo[fd] = (uint64_t)((int64_t)s + w.lb);
-            __syncthreads();
-            if (pendOn) ring_store16<kFRing>(L.ring[pf], pa, pend, L.plan[pf].srcPtr);
-            const int sf = lane >> 4, si = lane & 15;
-            pendOn = false;
-            uint64_t newLo = 0;
-            const bool sa = (factMask >> sf) & 1;
-            if (sa) {
-                uint64_t sb = L.plan[sf].srcPtr, cur = L.aLo[sf];
-                uint64_t floorLo = floor16(sb - 8), want = floor16(L.winLo[sf] - kFLook);
-                if (want < floorLo) want = floorLo;
-                uint64_t a = cur - 16 * (uint64_t)(si + 1);
-                if (a >= want && a < cur) {
-                    pendOn = true;
-                    pa = a;
-                    pf = sf;
-                    pend = load_block(a, sb);
-                }
-                uint64_t lim = cur - 16 * 16;
-                newLo = want < cur ? (want > lim ? want : lim) : cur;
-            }
-            __syncthreads();
-            if (sa && si == 0) L.aLo[sf] = newLo;
+    const uint32_t R = ok ? (n + 3) / 4 : 0u;
+    for (uint32_t r = 4 * G; r < R; r++) {
+        if ((r & (kQPhase - 1)) == 0) {
+            qstage_step(ring, g, (Pb >> 3) - 12, q);
+            wave_sync();
         }
-        // the table entry and the bytes below the window are read together;
-        // the window holds >= 44 bits at the top of every round
-        uint32_t p = r * S + (uint32_t)q;
-        bool mine = ok && (uint32_t)q < S && p < n;
-        uint32_t e = DT[state];
-        uint64_t below = ring_read8<kFRing>(ring, s + (uint64_t)(int64_t)w.lb);
-        e = mine ? e : 0u;
-        bool upd = mine && p + S < n;
-        uint32_t nb = upd ? ((e >> 8) & 255u) : 0u;
-        uint32_t n0 = quad_lane<0>(nb), n1 = quad_lane<1>(nb), n2 = quad_lane<2>(nb), n3 = quad_lane<3>(nb);
-        uint32_t before = (q > 0 ? n0 : 0u) + (q > 1 ? n1 : 0u) + (q > 2 ? n2 : 0u);
-        uint32_t all = n0 + n1 + n2 + n3;
-        uint64_t hi = w.acc << before;
-        uint32_t bits = nb ? (uint32_t)(hi >> (64 - nb)) : 0u;
-        state = upd ? (e >> 16) + bits : state;
-        w.acc <<= all;
-        w.valid -= (int)all;
-        win_refill_with(w, below, ok && w.valid < 44);
-        if (wide) wacc |= (e & 255u) << (8 * (r & 3));
-        else if (mine) out[p] = (uint8_t)e;
-        if ((r & 3) == 3) {
-            uint32_t t0 = quad_lane<0>(wacc), t1 = quad_lane<1>(wacc), t2 = quad_lane<2>(wacc), t3 = quad_lane<3>(wacc);
-            if (wide && ok) {
-                uint32_t r0 = r - 3, sh = 8 * (uint32_t)q;
-                if ((r0 + 4) * 4 <= n) {
-                    uint32_t tw = ((t0 >> sh) & 255u) | (((t1 >> sh) & 255u) << 8) | (((t2 >> sh) & 255u) << 16) |
-                                  (((t3 >> sh) & 255u) << 24);
-                    *reinterpret_cast<uint32_t*>(out + (r0 + (uint32_t)q) * 4) = tw;
-                } else {
-                    for (uint32_t j = 0; j < 4; j++) {
-                        uint32_t p2 = (r0 + j) * 4 + (uint32_t)q;
-                        if (p2 < n) out[p2] = (uint8_t)(wacc >> (8 * j));
-                    }
-                }
-            }
-            wacc = 0;
-        }
+        const uint32_t p = 4 * r + q, Q0 = Q;
+        const uint32_t y = fse_quad_round<true>(DT, SY, ring, st, W, Q, sq, p, n);
+        Pb -= Q0 - Q;
+        if (p < n) *(g_uint8*)(uintptr_t)(out + p) = (uint8_t)y;
     }
-    if (dl && q == 0) {
-        long pos = 8L * w.lb + w.valid;
-        if (!ok || pos != 0) ws.plans[b0 + fd].status = OZL_ERR_CORRUPTION;
-    }
+    if (q == 0 && (!ok || Pb != 8 * s)) ws.plans[b0 + f].status = OZL_ERR_CORRUPTION;
 }
 
 // ---------------------------------------------------------------- Huffman decoding
-constexpr int kHF = 4;          // frames per workgroup, one lane per stream
-constexpr int kHS = 4 * kHF;    // streams per workgroup
-constexpr int kHRing = 256;     // staged stream bytes per stream
-constexpr int kHLook = 160;
+// One lane per Huffman stream, 16 frames (64 streams) per workgroup; each lane
+// peeks tl bits, looks the code up in its frame's table and refills 32 bits
+// when the window holds <= 32 (huf_decompress.c:587-787).
+constexpr int kHF = 16;
+constexpr int kHRing = 128;
+constexpr int kHLook = 64;
+constexpr int kHPhase = 16;     // rounds between staging steps (<= 22 bytes consumed)
 
 struct HufDecLds {
     uint16_t lut[kHF][2048];
-    uint32_t ring[kHS][kHRing / 4];
+    uint32_t ring[4 * kHF][kHRing / 4];
     HufBuild hb;
     ozl_dplan plan[kHF];
-    uint64_t sLo[kHS];    // stream start
-    uint64_t sTop[kHS];   // stream end
-    uint64_t aLo[kHS];
-    uint64_t winLo[kHS];
-    uint32_t sAct[kHS];
 };
 
 __global__ void __launch_bounds__(64) k_huf_decode(uint32_t B, ozl_dws ws)
@@ -335,160 +493,86 @@ __global__ void __launch_bounds__(64) k_huf_decode(uint32_t B, ozl_dws ws)
     __shared__ HufDecLds L;
     const int lane = threadIdx.x;
     const uint32_t b0 = blockIdx.x * kHF;
-    {
-        int f = lane >> 4, k = lane & 15;
+    for (int i = lane; i < kHF * 16; i += 64) {
+        int f = i >> 4, k = i & 15;
         reinterpret_cast<uint32_t*>(&L.plan[f])[k] =
             b0 + f < B ? reinterpret_cast<const uint32_t*>(&ws.plans[b0 + f])[k] : 0u;
     }
-    __syncthreads();
+    wave_sync();
     uint32_t factMask = 0;
-#pragma unroll
     for (int f = 0; f < kHF; f++) {
         const ozl_dplan& P = L.plan[f];
         if (b0 + f < B && P.status == 0 && P.bk == OZL_BK_HUF && P.tl >= 1 && P.tl <= 11) factMask |= 1u << f;
     }
     if (!factMask) return;
-    // streams: lane k < kHS is stream k & 3 of frame k >> 2
-    uint32_t cnt = 0, outOff = 0;
-    bool sok = false;
-    if (lane < kHS) {
-        int f = lane >> 2, k = lane & 3;
-        const ozl_dplan& P = L.plan[f];
-        uint64_t s = P.srcPtr;
-        uint32_t n = P.n, payLen = P.payLen;
-        uint64_t lo = 0, top = 0;
-        if ((factMask >> f) & 1) {
-            if (P.nbStates) {
-                uint32_t seg = (n + 3) / 4;
-                const uint8_t* jt = reinterpret_cast<const uint8_t*>((uintptr_t)s);
-                uint32_t sz[3] = { 0, 0, 0 };
-                if (payLen >= 6)
-                    for (int j = 0; j < 3; j++) sz[j] = (uint32_t)jt[2 * j] | ((uint32_t)jt[2 * j + 1] << 8);
-                uint32_t sum = sz[0] + sz[1] + sz[2];
-                if (payLen >= 6 && 6 + sum <= payLen) {
-                    uint32_t start = 6;
-                    for (int j = 0; j < k; j++) start += sz[j];
-                    uint32_t len = k < 3 ? sz[k] : payLen - 6 - sum;
-                    lo = s + start;
-                    top = lo + len;
-                    cnt = k < 3 ? seg : n - 3 * seg;
-                    outOff = k * seg;
-                    sok = true;
-                } else if (k == 0) {
-                    ws.plans[b0 + f].status = OZL_ERR_CORRUPTION;
-                }
-            } else if (k == 0) {
-                lo = s;
-                top = s + payLen;
-                cnt = n;
-                sok = true;
-            }
-        }
-        L.sLo[lane] = lo;
-        L.sTop[lane] = top;
-        L.sAct[lane] = sok;
-    }
-    __syncthreads();
-#pragma unroll
     for (int f = 0; f < kHF; f++)
-        if ((factMask >> f) & 1)
-            wave_huf_lut(L.lut[f], ws.side + (size_t)(b0 + f) * OZL_DSIDE_BYTES, L.plan[f].tl, L.hb);
-    for (int k = 0; k < kHS; k++) {
-        if (!L.sAct[k]) continue;
-        uint64_t s = L.sLo[k], top = L.sTop[k];
-        uint64_t a0 = floor16(top - kHLook), lo = floor16(s - 8);
-        if (a0 < lo) a0 = lo;
-        uint8_t* rb = reinterpret_cast<uint8_t*>(L.ring[k]);
-        for (uint64_t a = a0 + lane; a < top; a += 64) rb[a & (kHRing - 1)] = a >= s ? *reinterpret_cast<const uint8_t*>((uintptr_t)a) : 0;
-        if (lane == 0) L.aLo[k] = a0;
-    }
-    __syncthreads();
-
-    const int sid = lane < kHS ? lane : 0;
-    const int fd = sid >> 2;
-    const bool dl = lane < kHS && sok;
-    const uint32_t tl = L.plan[fd].tl;
-    const uint16_t* LUT = L.lut[fd];
-    const uint32_t* ring = L.ring[sid];
-    const uint64_t s = L.sLo[sid];
-    uint8_t* out = reinterpret_cast<uint8_t*>((uintptr_t)L.plan[fd].outPtr) + outOff;
-    const bool wide = dl && (((uintptr_t)out) & 3) == 0;
-    BitWin w = { 0, 0, (int)(L.sTop[sid] - s) };
-    bool ok = dl;
-    if (dl) {
-        win_refill<kHRing>(w, ring, s);
-        uint32_t topByte = (uint32_t)(w.acc >> 56);
-        if (topByte == 0) ok = false;
-        else {
-            int skip = 8 - ozl_hb32(topByte);
-            w.acc <<= skip;
-            w.valid -= skip;
+        if ((factMask >> f) & 1) wave_huf_lut(L.lut[f], ws.side + (size_t)(b0 + f) * OZL_DSIDE_BYTES, L.plan[f].tl, L.hb);
+    wave_sync();
+    const int f = lane >> 2, k = lane & 3;
+    if (!((factMask >> f) & 1)) return;
+    const ozl_dplan P = L.plan[f];
+    const uint32_t n = P.n, payLen = P.payLen, tl = P.tl;
+    const uint64_t s0 = P.srcPtr;
+    uint64_t lo = 0;
+    uint32_t len = 0, cnt = 0, outOff = 0;
+    if (P.nbStates) {
+        uint32_t seg = (n + 3) / 4;
+        const uint8_t* jt = reinterpret_cast<const uint8_t*>((uintptr_t)s0);
+        uint32_t sz0 = 0, sz1 = 0, sz2 = 0;
+        if (payLen >= 6) {
+            sz0 = (uint32_t)jt[0] | ((uint32_t)jt[1] << 8);
+            sz1 = (uint32_t)jt[2] | ((uint32_t)jt[3] << 8);
+            sz2 = (uint32_t)jt[4] | ((uint32_t)jt[5] << 8);
         }
+        if (payLen < 6 || 6 + sz0 + sz1 + sz2 > payLen) {
+            if (k == 0) ws.plans[b0 + f].status = OZL_ERR_CORRUPTION;
+            return;
+        }
+        uint32_t start = 6 + (k > 0 ? sz0 : 0) + (k > 1 ? sz1 : 0) + (k > 2 ? sz2 : 0);
+        len = k == 0 ? sz0 : k == 1 ? sz1 : k == 2 ? sz2 : payLen - 6 - sz0 - sz1 - sz2;
+        lo = s0 + start;
+        cnt = k < 3 ? seg : n - 3 * seg;
+        outOff = k * seg;
+    } else {
+        if (k != 0) return;
+        lo = s0;
+        len = payLen;
+        cnt = n;
     }
-    uint32_t R = 0;
-    for (int k = 0; k < kHS; k++) {
-        // the longest stream of the workgroup sets the loop length
-        uint32_t c = __shfl(cnt, k, 64);
-        R = c > R ? c : R;
-    }
-    bool pendOn = false;
-    uint64_t pa = 0;
-    int pk = 0;
-    uint4 pend = make_uint4(0, 0, 0, 0);
+    const uint16_t* LUT = L.lut[f];
+    uint32_t* ring = L.ring[lane];
+    uint8_t* out = reinterpret_cast<uint8_t*>((uintptr_t)P.outPtr) + outOff;
+    const bool wide = (((uintptr_t)out) & 3) == 0;
+    Stager g;
+    stage_init<kHRing>(ring, g, lo, lo + len, kHLook);
+    Win w;
+    bool ok = win_init<kHRing>(w, ring, lo, len);
+    if (ok) win_refill32(w, ring_dword<kHRing>(ring, lo + (uint64_t)(int64_t)w.lb - 4));
+    const uint32_t R = ok ? cnt : 0u;
     uint32_t wacc = 0;
-    const uint32_t R4 = (R + 3) & ~3u;
-    for (uint32_t r = 0; r < R4; r++) {
-        if (r % kPhase == 0) {
-            if (dl) L.winLo[sid] = (uint64_t)((int64_t)s + w.lb);
-            __syncthreads();
-            if (pendOn) ring_store16<kHRing>(L.ring[pk], pa, pend, L.sLo[pk]);
-            const int st = lane >> 2, si = lane & 3;
-            pendOn = false;
-            uint64_t newLo = 0;
-            bool sa = L.sAct[st] != 0;
-            if (sa) {
-                uint64_t sb = L.sLo[st], cur = L.aLo[st];
-                uint64_t floorLo = floor16(sb - 8), want = floor16(L.winLo[st] - kHLook);
-                if (want < floorLo) want = floorLo;
-                uint64_t a = cur - 16 * (uint64_t)(si + 1);
-                if (a >= want && a < cur) {
-                    pendOn = true;
-                    pa = a;
-                    pk = st;
-                    pend = load_block(a, sb);
-                }
-                uint64_t lim = cur - 16 * 4;
-                newLo = want < cur ? (want > lim ? want : lim) : cur;
-            }
-            __syncthreads();
-            if (sa && si == 0) L.aLo[st] = newLo;
-        }
-        // lookup and the bytes below the window are read together; the
-        // window holds >= tl bits at the top of every round
-        bool mine = ok && r < cnt;
-        uint32_t e = LUT[(uint32_t)(w.acc >> (64 - tl))];
-        uint64_t below = ring_read8<kHRing>(ring, s + (uint64_t)(int64_t)w.lb);
-        uint32_t l = mine ? e >> 8 : 0u;
+    for (uint32_t r = 0; r < R; r++) {
+        if ((r & (kHPhase - 1)) == 0) stage_step<kHRing, 2>(ring, g, (uint64_t)((int64_t)lo + w.lb), kHLook);
+        uint32_t e = LUT[win_hi(w) >> (32 - tl)];
+        uint32_t d = ring_dword<kHRing>(ring, (uint64_t)((int64_t)lo + w.lb) - 4);
+        uint32_t l = e >> 8;
         w.acc <<= l;
         w.valid -= (int)l;
-        win_refill_with(w, below, ok && w.valid < (int)tl);
-        if (wide) wacc |= (e & 255u) << (8 * (r & 3));
-        else if (mine) out[r] = (uint8_t)e;
-        if (wide && (r & 3) == 3) {
-            uint32_t r0 = r - 3;
-            if (r0 + 4 <= cnt) *reinterpret_cast<uint32_t*>(out + r0) = wacc;
-            else
-                for (uint32_t j = 0; j < 4; j++)
-                    if (r0 + j < cnt) out[r0 + j] = (uint8_t)(wacc >> (8 * j));
-            wacc = 0;
+        win_refill32(w, d);
+        if (wide) {
+            wacc |= (e & 255u) << (8 * (r & 3));
+            if ((r & 3) == 3) {
+                *reinterpret_cast<uint32_t*>(out + r - 3) = wacc;
+                wacc = 0;
+            }
+        } else {
+            out[r] = (uint8_t)e;
         }
     }
-    if (lane < kHS && L.sAct[lane]) {
-        long pos = 8L * w.lb + w.valid;
-        if (!ok || pos != 0) ws.plans[b0 + fd].status = OZL_ERR_CORRUPTION;
-    }
+    if (wide)
+        for (uint32_t j = R & ~3u; j < R; j++) out[j] = (uint8_t)(wacc >> (8 * (j & 3)));
+    long pos = 8L * w.lb + w.valid;
+    if (!ok || pos != 0) ws.plans[b0 + f].status = OZL_ERR_CORRUPTION;
 }
-
 
 __global__ void __launch_bounds__(kT) k_pack_decode(const uint8_t* __restrict__ frames, const uint64_t* __restrict__ foff,
                                                     uint32_t B, uint8_t* __restrict__ dst, const uint64_t* __restrict__ doff,
@@ -664,7 +748,7 @@ int ozl_entropy_decompress_batch(const void* frames, const uint64_t* foff, uint3
     const uint8_t* f = (const uint8_t*)frames;
     uint8_t* o = (uint8_t*)dst;
     OZL_LAUNCH("k_dparse", k_dparse, dim3(B), dim3(kParseT), 0, st, f, foff, B, o, doff, ws);
-    OZL_LAUNCH("k_fse_decode", k_fse_decode, dim3((B + kFF - 1) / kFF), dim3(64), 0, st, B, ws);
+    OZL_LAUNCH("k_fse_decode", k_fse_decode, dim3((B + kFQ - 1) / kFQ), dim3(64), 0, st, B, ws);
     OZL_LAUNCH("k_huf_decode", k_huf_decode, dim3((B + kHF - 1) / kHF), dim3(64), 0, st, B, ws);
     OZL_LAUNCH("k_pack_decode", k_pack_decode, dim3(B), dim3(kT), 0, st, f, foff, B, o, doff, ws);
     OZL_LAUNCH("k_dverify", k_dverify, dim3(B), dim3(64), 0, st, f, foff, B, (const uint8_t*)o, doff, ws, status);
@@ -688,7 +772,7 @@ int ozl_entropy_decode_items(const ozl_ditem* items, uint32_t B, int* status, vo
     ws.plans = (ozl_dplan*)w; w += (size_t)B * sizeof(ozl_dplan);
     ws.side = w;
     OZL_LAUNCH("k_prep_items", k_prep_items, dim3(B), dim3(64), 0, st, items, B, ws);
-    OZL_LAUNCH("k_fse_decode", k_fse_decode, dim3((B + kFF - 1) / kFF), dim3(64), 0, st, B, ws);
+    OZL_LAUNCH("k_fse_decode", k_fse_decode, dim3((B + kFQ - 1) / kFQ), dim3(64), 0, st, B, ws);
     OZL_LAUNCH("k_huf_decode", k_huf_decode, dim3((B + kHF - 1) / kHF), dim3(64), 0, st, B, ws);
     OZL_LAUNCH("k_item_status", k_item_status, dim3((B + 63) / 64), dim3(64), 0, st, B, ws, status);
     OZL_HIP_CHECK(hipGetLastError());

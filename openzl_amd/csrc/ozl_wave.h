@@ -38,22 +38,30 @@ __device__ inline uint32_t quad_lane(uint32_t v)
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, K * 0x55, 0xF, 0xF, false);
 }
 
+// lanes of this lane's quad exchanged pairwise (DPP quad_perm [1,0,3,2]) and
+// in pairs of pairs ([2,3,0,1])
+__device__ inline uint32_t quad_xor1(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false); }
+__device__ inline uint32_t quad_xor2(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false); }
+
 // ---------------------------------------------------------------- FSE decoding table
-struct FseBuild {
+struct FseBuildHdr {
     int16_t norm[256];
     uint16_t cum[256];   // exclusive prefix of the positive counts
     uint16_t next[256];  // per symbol: running rank base (decoder) or state slot (encoder)
-    uint8_t cell[2048];
+};
+struct FseBuild : FseBuildHdr {
+    uint8_t cell[2048];  // spread, when the caller keeps none of its own
 };
 
 // Symbol spread (fse_compress.c:93-155, fse_decompress.c:93-160) with one
 // wave: low-probability symbols take the top cells in symbol order; the k-th
 // occurrence of the other symbols (in symbol order) goes to the k-th position
 // of the sequence j*step mod T that is not above the low-probability cells.
-// Leaves the counts in S.norm (zero above maxSym) and, per symbol, the
-// exclusive prefix of the counts in S.next (-1 counting as 1) when
-// `prefixAll`, else the count itself (-1 as 1).
-static __device__ void wave_fse_spread(const int16_t* normSrc, unsigned maxSym, unsigned tl, FseBuild& S, bool prefixAll)
+// Writes the symbol of every cell to `cell`, leaves the counts in S.norm
+// (zero above maxSym) and, per symbol, the exclusive prefix of the counts in
+// S.next (-1 counting as 1) when `prefixAll`, else the count itself (-1 as 1).
+static __device__ void wave_fse_spread(uint8_t* cell, const int16_t* normSrc, unsigned maxSym, unsigned tl, FseBuildHdr& S,
+                                       bool prefixAll)
 {
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanes_below();
@@ -66,7 +74,7 @@ static __device__ void wave_fse_spread(const int16_t* normSrc, unsigned maxSym, 
         int v = S.norm[s];
         bool lp = v == -1;
         uint64_t m = __ballot(lp);
-        if (lp) S.cell[T - 1 - (low + (unsigned)__popcll(m & lt))] = (uint8_t)s;
+        if (lp) cell[T - 1 - (low + (unsigned)__popcll(m & lt))] = (uint8_t)s;
         low += (unsigned)__popcll(m);
         uint32_t cnt = v > 0 ? (uint32_t)v : 0u, cntAll = lp ? 1u : cnt;
         uint32_t x = cnt, y = cntAll;
@@ -92,11 +100,37 @@ static __device__ void wave_fse_spread(const int16_t* normSrc, unsigned maxSym, 
             unsigned s = 0;
             for (unsigned half = 128; half; half >>= 1)
                 if (S.cum[s + half] <= k) s += half;
-            S.cell[pos] = (uint8_t)s;
+            cell[pos] = (uint8_t)s;
         }
         run += (unsigned)__popcll(m);
     }
     wave_sync();
+}
+
+// Decoding table in two arrays: sym[u] the cell's symbol, dt[u] = nbBits |
+// newStateBase << 4 (both fit 16 bits for tableLog <= 11).
+static __device__ void wave_fse_dtable16(uint16_t* dt, uint8_t* sym, const int16_t* normSrc, unsigned maxSym, unsigned tl,
+                                         FseBuildHdr& S)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanes_below();
+    const unsigned T = 1u << tl;
+    wave_fse_spread(sym, normSrc, maxSym, tl, S, false);
+    for (unsigned u0 = 0; u0 < T; u0 += 64) {
+        unsigned u = u0 + lane;
+        bool act = u < T;
+        uint32_t s = act ? sym[u] : 0u;
+        uint64_t m = match8(s, act);
+        uint32_t base = act ? S.next[s] : 0u;
+        if (act) {
+            uint32_t ns = base + (uint32_t)__popcll(m & lt);
+            uint32_t nb = tl - (uint32_t)ozl_hb32(ns);
+            dt[u] = (uint16_t)(nb | (((ns << nb) - T) << 4));
+        }
+        wave_sync();
+        if (act && (m & lt) == 0) S.next[s] = (uint16_t)(base + (uint32_t)__popcll(m));
+        wave_sync();
+    }
 }
 
 // FSE_buildDTable (fse_decompress.c:73-176): every cell's next state is its
@@ -107,7 +141,7 @@ static __device__ void wave_fse_dtable(uint32_t* dt, const int16_t* normSrc, uns
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanes_below();
     const unsigned T = 1u << tl;
-    wave_fse_spread(normSrc, maxSym, tl, S, false);
+    wave_fse_spread(S.cell, normSrc, maxSym, tl, S, false);
     for (unsigned u0 = 0; u0 < T; u0 += 64) {
         unsigned u = u0 + lane;
         bool act = u < T;
@@ -134,7 +168,7 @@ static __device__ void wave_fse_ctable(uint16_t* st, uint32_t* tt, const int16_t
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanes_below();
     const unsigned T = 1u << tl;
-    wave_fse_spread(normSrc, maxSym, tl, S, true);
+    wave_fse_spread(S.cell, normSrc, maxSym, tl, S, true);
     for (unsigned s = lane; s <= maxSym; s += 64) {
         int c = S.norm[s];
         int cumul = S.next[s];

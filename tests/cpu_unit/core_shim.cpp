@@ -50,8 +50,8 @@ long cu_fse_read_ncount(int16_t* norm, unsigned* ms, unsigned* tl, const uint8_t
 }
 int cu_huf_build(uint8_t* nb, uint16_t* val, const uint32_t* count, unsigned maxSym, unsigned maxNbBits)
 {
-    ozl_hnode tbl[2 * 256 + 2];
-    return ozl_huf_build(nb, val, count, maxSym, maxNbBits, tbl);
+    static ozl_huf_scratch w;
+    return ozl_huf_build(nb, val, count, maxSym, maxNbBits, &w);
 }
 int cu_huf_from_weights(uint8_t* len, uint16_t* val, const uint8_t* w, unsigned nbW)
 {
