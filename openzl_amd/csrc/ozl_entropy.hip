@@ -6,9 +6,9 @@
 //                   decides the backend (:601-753) and builds the FSE tables or
 //                   Huffman code, the NCount, and the nested graph of the
 //                   Huffman weights (ozl_core.h)
-//   k_fse_encode    one 4-lane group per block, one lane per interleaved state;
-//                   the 16 groups of a wavefront share nothing but the issue
-//                   slot, their tables sit in LDS
+//   k_fse_encode    one 4-lane group per block, one lane per interleaved state,
+//                   8 blocks per wavefront; a round's bit offsets come from a
+//                   quad prefix sum (DPP), tables sit in LDS
 //   k_huf_encode    one workgroup per block: code lengths, a block-wide scan of
 //                   bit offsets in reverse symbol order, bits OR-ed into LDS
 //   k_pack_encode   flatpack / bitpack payloads, one output byte per lane
@@ -222,15 +222,27 @@ __global__ void __launch_bounds__(kT) k_plan(const uint8_t* __restrict__ src, co
 }
 
 // ------------------------------------------------------------------ FSE encode
-// 16 blocks per 64-lane workgroup; lane q of a 4-lane group owns encoder state
-// q (fse_compress.c:607-660). Emission e (e = 0.. n-S-1) encodes position
-// p = n-S-1-e with state (e - n) mod S; a round gives each lane one emission,
-// the group gathers the four (value, nbBits) pairs in emission order (DPP) and
-// lane 0 appends them to a 32-bit-word writer. Each lane's input symbols are
+// 8 blocks per one-wave workgroup, a quad of lanes per block; lane q owns
+// encoder state q (fse_compress.c:607-660). Emission e (e = 0.. n-S-1)
+// encodes position p = n-S-1-e with state (e - n) mod S; a round gives each
+// lane one emission, in slot (q + n) mod S of the round. The quad turns its
+// bit counts into offsets in slot order (byte-packed counts summed with two
+// DPP exchanges, prefix sums by a multiply with 0x01010101), ORs the shifted
+// values into one 64-bit round word, and appends that to a 32-bit word writer
+// whose completed words lanes 0 and 1 store. Each lane's input symbols are
 // loaded 8 rounds ahead of use (an unrolled register queue), so the global
-// load latency stays off the state chain.
-constexpr int kFseGroups = 16;
+// load latency stays off the state chain. Tables take 5 KB per block (40 KB
+// per workgroup, 4 workgroups per CU).
+constexpr int kFseGroups = 8;
 constexpr int kFsePipe = 8;
+
+struct FseEncLds {
+    uint16_t st[kFseGroups][2048];
+    uint32_t tt[kFseGroups][256];
+};
+
+typedef __attribute__((address_space(1))) const uint8_t gc_uint8;
+typedef __attribute__((address_space(1))) uint32_t g_uint32;
 
 __device__ inline uint32_t sel4(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t i)
 {
@@ -240,33 +252,36 @@ __device__ inline uint32_t sel4(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t 
 __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
                                                    uint32_t B, ozl_ews ws)
 {
-    __shared__ uint16_t sST[kFseGroups][2048];
-    __shared__ uint32_t sTT[kFseGroups][256];
+    __shared__ FseEncLds L;
     const int lane = threadIdx.x & 63, g = lane >> 2, q = lane & 3;
     const uint32_t b = blockIdx.x * kFseGroups + g;
-    bool active = b < B;
+    bool active = g < kFseGroups && b < B;
     ozl_eplan p = {};
     if (active) {
         p = ws.plans[b];
-        active = p.bk == OZL_BK_FSE && p.status == 0 && p.tl <= 11;
+        active = p.bk == OZL_BK_FSE && p.status == 0 && p.tl <= 11 && p.n > p.nbStates;
     }
-    const uint8_t* tab = ws.tables + (size_t)(active ? b : 0) * OZL_TAB_BYTES;
-    if (active) {
-        unsigned ts = 1u << p.tl;
-        const uint16_t* st = reinterpret_cast<const uint16_t*>(tab);
-        const uint32_t* tt = reinterpret_cast<const uint32_t*>(tab + 8192);
-        for (unsigned i = q; i < ts; i += 4) sST[g][i] = st[i];
-        for (unsigned i = q; i <= p.maxSym; i += 4) sTT[g][i] = tt[i];
+    // the wave copies each block's state table and symbol transforms to LDS
+    for (int f = 0; f < kFseGroups; f++) {
+        const int actf = __shfl((int)active, 4 * f, 64);
+        if (!actf) continue;
+        const uint32_t tlf = (uint32_t)__shfl((int)p.tl, 4 * f, 64), msf = (uint32_t)__shfl((int)p.maxSym, 4 * f, 64);
+        const uint4* tab = reinterpret_cast<const uint4*>(ws.tables + (size_t)(blockIdx.x * kFseGroups + f) * OZL_TAB_BYTES);
+        const uint32_t nS = (1u << tlf) / 8, nT = (msf + 4) / 4;
+        for (uint32_t i = lane; i < nS; i += 64) reinterpret_cast<uint4*>(L.st[f])[i] = tab[i];
+        for (uint32_t i = lane; i < nT; i += 64) reinterpret_cast<uint4*>(L.tt[f])[i] = tab[512 + i];
     }
-    __syncthreads();
-    const uint8_t* in = src + (active ? off[b] : 0);
-    const uint32_t n = p.n, S = active ? p.nbStates : 4u, tl = p.tl;
-    const uint16_t* ST = sST[g];
-    const uint32_t* TT = sTT[g];
-    uint32_t* outw = reinterpret_cast<uint32_t*>(ws.payload + (size_t)(active ? b : 0) * ws.payloadCap);
+    ozl_wave::wave_sync();
+    if (!active) return;
+    gc_uint8* in = (gc_uint8*)(src + off[b]);
+    const uint32_t n = p.n, S = p.nbStates, tl = p.tl;
+    const uint16_t* ST = L.st[g] - 4096;   // indexed by (state >> nb) + (tt >> 18)
+    const uint32_t* TT = L.tt[g];
+    g_uint32* outw = (g_uint32*)(ws.payload + (size_t)b * ws.payloadCap);
     const uint32_t capWords = (uint32_t)(ws.payloadCap / 4) - 2;
-    const bool mine = active && (uint32_t)q < S;
+    const bool mine = (uint32_t)q < S;
     // init: the one position among the last S whose state index is q
+    // (FSE_initCState2)
     uint32_t state = 0;
     if (mine) {
         uint32_t pinit = 0;
@@ -278,83 +293,65 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
         uint32_t dnb = ozl_tt_dnb(t);
         uint32_t nbOut = (dnb + (1u << 15)) >> 16;
         uint32_t v = (nbOut << 16) - dnb;
-        state = ST[(int)(v >> nbOut) + ozl_tt_dfs(t)];
+        state = ST[(int)(v >> nbOut) + (int)(t >> 18)];
     }
-    const uint32_t nEmit = active ? n - S : 0;
-    const uint32_t kq = active ? (uint32_t)(q + n) % S : 0u;  // this lane's slot inside a round
-    const uint32_t rot = active ? (S - (n % S)) % S : 0u;      // lane of slot k is (k + rot) % S
-    const uint32_t sl0 = rot % S, sl1 = (1 + rot) % S, sl2 = (2 + rot) % S, sl3 = (3 + rot) % S;
-    // rounds of the group with the most emissions set the loop length
-    uint32_t myRounds = active ? (nEmit + S - 1) / S : 0u;
-    uint32_t rounds = myRounds;
-    for (int d = 32; d >= 1; d >>= 1) {
-        uint32_t o = __shfl_xor(rounds, d, 64);
-        rounds = o > rounds ? o : rounds;
-    }
-    // every lane loads every round (invalid positions read a harmless byte),
-    // so the compiler sees a fixed number of loads in flight
-    const uint8_t* inSafe = active ? in : ws.payload;
+    const uint32_t nEmit = n - S;
+    const uint32_t kq = mine ? (uint32_t)(q + n) % S : (uint32_t)q;  // this lane's slot in a round
+    const uint32_t sq = 8 * kq;
+    const uint32_t rounds = (nEmit + S - 1) / S;
     uint32_t qbuf[kFsePipe];
 #pragma unroll
     for (int k = 0; k < kFsePipe; k++) {
         uint32_t e = (uint32_t)k * S + kq;
-        qbuf[k] = inSafe[(mine && e < nEmit) ? nEmit - 1 - e : 0u];
+        qbuf[k] = in[(mine && e < nEmit) ? nEmit - 1 - e : 0u];
     }
-    uint64_t acc = 0;
-    uint32_t accBits = 0, wpos = 0;
+    uint32_t a0 = 0, ab = 0, wpos = 0;  // pending bits (ab < 32) and words written
     for (uint32_t r0 = 0; r0 < rounds; r0 += kFsePipe) {
 #pragma unroll
         for (int k = 0; k < kFsePipe; k++) {
-            const uint32_t r = r0 + (uint32_t)k;
-            const uint32_t e = r * S + kq;
+            const uint32_t e = (r0 + (uint32_t)k) * S + kq;
             const bool doit = mine && e < nEmit;
-            const uint32_t sym = qbuf[k] & 255u;
+            const uint32_t sym = qbuf[k];
             const uint32_t eNext = e + kFsePipe * S;
-            qbuf[k] = inSafe[(mine && eNext < nEmit) ? nEmit - 1 - eNext : 0u];
+            qbuf[k] = in[(mine && eNext < nEmit) ? nEmit - 1 - eNext : 0u];
             const uint32_t t = TT[sym];
-            const uint32_t nb = doit ? ((state + ozl_tt_dnb(t)) >> 16) & 31u : 0u;
-            const uint32_t packed = doit ? (state & ((1u << nb) - 1)) | (nb << 16) : 0u;
-            const int idx = doit ? (int)(state >> nb) + ozl_tt_dfs(t) : 0;
-            const uint32_t ns = ST[idx];
+            // nbBitsOut = (state + deltaNbBits) >> 16 = maxBits - (state < minStatePlus)
+            const uint32_t nb = doit ? (t & 15u) - (state < ((t >> 4) & 0x3FFFu) ? 1u : 0u) : 0u;
+            const uint32_t val = __builtin_amdgcn_ubfe(state, 0, nb);
+            const uint32_t ns = ST[doit ? (state >> nb) + (t >> 18) : 4096u];
             state = doit ? ns : state;
-            const uint32_t c0 = ozl_wave::quad_lane<0>(packed), c1 = ozl_wave::quad_lane<1>(packed);
-            const uint32_t c2 = ozl_wave::quad_lane<2>(packed), c3 = ozl_wave::quad_lane<3>(packed);
-            if (r < myRounds) {
-                uint32_t x0 = sel4(c0, c1, c2, c3, sl0), x1 = sel4(c0, c1, c2, c3, sl1);
-                uint32_t n0 = x0 >> 16, n1 = x1 >> 16;
-                acc |= (uint64_t)((x0 & 0xFFFF) | ((x1 & 0xFFFF) << n0)) << accBits;
-                accBits += n0 + n1;
-                if (accBits >= 32) {
-                    if (q == 0 && wpos < capWords) outw[wpos] = (uint32_t)acc;
-                    wpos++;
-                    acc >>= 32;
-                    accBits -= 32;
-                }
-                if (S == 4) {
-                    uint32_t x2 = sel4(c0, c1, c2, c3, sl2), x3 = sel4(c0, c1, c2, c3, sl3);
-                    uint32_t n2 = x2 >> 16, n3 = x3 >> 16;
-                    acc |= (uint64_t)((x2 & 0xFFFF) | ((x3 & 0xFFFF) << n2)) << accBits;
-                    accBits += n2 + n3;
-                    if (accBits >= 32) {
-                        if (q == 0 && wpos < capWords) outw[wpos] = (uint32_t)acc;
-                        wpos++;
-                        acc >>= 32;
-                        accBits -= 32;
-                    }
-                }
-            }
+            uint32_t v = nb << sq;
+            v += ozl_wave::quad_xor1(v);
+            v += ozl_wave::quad_xor2(v);
+            const uint32_t M = v * 0x01010101u;  // byte k: bits of slots 0..k
+            const uint32_t excl = __builtin_amdgcn_ubfe(M, sq, 8) - nb;
+            const uint64_t R = (uint64_t)val << excl;
+            uint32_t rl = (uint32_t)R, rh = (uint32_t)(R >> 32);
+            rl |= ozl_wave::quad_xor1(rl);
+            rh |= ozl_wave::quad_xor1(rh);
+            rl |= ozl_wave::quad_xor2(rl);
+            rh |= ozl_wave::quad_xor2(rh);
+            const uint64_t T = (((uint64_t)rh << 32) | rl) << ab;
+            const uint32_t w0 = a0 | (uint32_t)T, w1 = (uint32_t)(T >> 32), w2 = __builtin_amdgcn_ubfe(rh, 32 - ab, ab);
+            const uint32_t nAcc = ab + (M >> 24), ne = nAcc >> 5;
+            if ((uint32_t)q < ne && wpos + q < capWords) outw[wpos + q] = q == 0 ? w0 : w1;
+            a0 = ne == 0 ? w0 : ne == 1 ? w1 : w2;
+            ab = nAcc & 31;
+            wpos += ne;
         }
     }
     // flush the states in index order, then the end mark
     const uint32_t f0 = ozl_wave::quad_lane<0>(state), f1 = ozl_wave::quad_lane<1>(state);
     const uint32_t f2 = ozl_wave::quad_lane<2>(state), f3 = ozl_wave::quad_lane<3>(state);
-    if (!active) return;
+    if (q != 0) return;
+    uint64_t acc = a0;
+    uint32_t accBits = ab;
     for (uint32_t i = 0; i < S; i++) {
         uint32_t sv = sel4(f0, f1, f2, f3, i) & ((1u << tl) - 1);
         acc |= (uint64_t)sv << accBits;
         accBits += tl;
         if (accBits >= 32) {
-            if (q == 0 && wpos < capWords) outw[wpos] = (uint32_t)acc;
+            if (wpos < capWords) outw[wpos] = (uint32_t)acc;
             wpos++;
             acc >>= 32;
             accBits -= 32;
@@ -363,15 +360,13 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
     acc |= 1ull << accBits;
     accBits += 1;
     uint32_t tailBytes = (accBits + 7) / 8;
-    if (q == 0) {
-        if (wpos >= capWords) {
-            ws.plans[b].status = OZL_ERR_INTERNAL_BUFFER;
-        } else {
-            uint8_t* ob = reinterpret_cast<uint8_t*>(outw + wpos);
-            for (uint32_t k = 0; k < tailBytes; k++) ob[k] = (uint8_t)(acc >> (8 * k));
-        }
-        ws.plans[b].payload = wpos * 4 + tailBytes;
+    if (wpos >= capWords) {
+        ws.plans[b].status = OZL_ERR_INTERNAL_BUFFER;
+    } else {
+        uint8_t* ob = reinterpret_cast<uint8_t*>(ws.payload + (size_t)b * ws.payloadCap + 4 * (size_t)wpos);
+        for (uint32_t k = 0; k < tailBytes; k++) ob[k] = (uint8_t)(acc >> (8 * k));
     }
+    ws.plans[b].payload = wpos * 4 + tailBytes;
 }
 
 // ------------------------------------------------------------------ Huffman encode
