@@ -13,7 +13,7 @@
 //                  interleaved state (:255-338); the lanes of a frame share
 //                  one bit position, their read offsets come from a quad
 //                  prefix sum (DPP)
-//   k_huf_decode   16 frames per wave: the wave builds each frame's lookup
+//   k_huf_decode   8 frames per wave: the wave builds each frame's lookup
 //                  table from the code lengths, then one lane per Huffman
 //                  stream decodes (huf_decompress.c:587-787)
 //   k_pack_decode  store / constant / flatpack / bitpack
@@ -99,6 +99,8 @@ __device__ inline uint4 gload16(uint64_t a)
     return make_uint4(v.x, v.y, v.z, v.w);
 }
 __device__ inline uint32_t gload4(uint64_t a) { return *(gc_uint32*)(uintptr_t)a; }
+typedef __attribute__((address_space(1))) uint32_t g_uint32;
+typedef __attribute__((address_space(1))) uint8_t g_uint8;
 
 template <int R>
 __device__ inline uint32_t ring_dword(const uint32_t* ring, uint64_t a)
@@ -266,8 +268,6 @@ struct QStage {
     uint64_t a0, a1;  // their addresses (0: none)
 };
 
-typedef __attribute__((address_space(1))) uint32_t g_uint32;
-typedef __attribute__((address_space(1))) uint8_t g_uint8;
 
 __device__ inline void qring_put(uint32_t* ring, uint64_t a, uint4 v)
 {
@@ -286,10 +286,16 @@ __device__ inline void qstage_init(uint32_t* ring, QStage& g, uint64_t lo, uint6
     g.a0 = g.a1 = 0;
 }
 
-__device__ inline void qstage_step(uint32_t* ring, QStage& g, uint64_t winLo, uint32_t q)
+// a step is qstage_put (waits for the previous step's loads) then
+// qstage_load; stores issued between the two do not delay the next put
+__device__ inline void qstage_put(uint32_t* ring, QStage& g)
 {
     if (g.a0) qring_put(ring, g.a0, g.p0);
     if (g.a1) qring_put(ring, g.a1, g.p1);
+}
+
+__device__ inline void qstage_load(QStage& g, uint64_t winLo, uint32_t q)
+{
     uint64_t want = floor16(winLo - kQLook);
     if (want < g.f) want = g.f;
     uint64_t nb = g.aLo > want ? (g.aLo - want) >> 4 : 0;
@@ -300,6 +306,12 @@ __device__ inline void qstage_step(uint32_t* ring, QStage& g, uint64_t winLo, ui
     g.a0 = q < nb ? x0 : 0;
     g.a1 = q + 4 < nb ? x1 : 0;
     g.aLo -= 16 * nb;
+}
+
+__device__ inline void qstage_step(uint32_t* ring, QStage& g, uint64_t winLo, uint32_t q)
+{
+    qstage_put(ring, g);
+    qstage_load(g, winLo, q);
 }
 
 // the 64 stream bits below absolute bit address Q (mod 2^32)
@@ -353,6 +365,29 @@ __device__ bool fse_decode_lane(const ozl_dplan& P, const uint16_t* DT, const ui
     }
     long pos = 8L * w.lb + w.valid;
     return ok && pos == 0;
+}
+
+// a dword of decoded bytes to global memory (byte stores when the
+// destination is not 4-aligned); four of them `stride` bytes apart
+__device__ inline void put_dword(uint64_t a, uint32_t d, bool wide)
+{
+    if (wide) {
+        *(g_uint32*)(uintptr_t)a = d;
+    } else {
+        g_uint8* ob = (g_uint8*)(uintptr_t)a;
+        ob[0] = (uint8_t)d;
+        ob[1] = (uint8_t)(d >> 8);
+        ob[2] = (uint8_t)(d >> 16);
+        ob[3] = (uint8_t)(d >> 24);
+    }
+}
+__device__ inline void put4_dwords(uint64_t a, uint32_t stride, uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3,
+                                   bool wide)
+{
+    put_dword(a, d0, wide);
+    put_dword(a + stride, d1, wide);
+    put_dword(a + 2 * stride, d2, wide);
+    put_dword(a + 3 * stride, d3, wide);
 }
 
 // one round of a 4-state frame's quad: returns this lane's symbol; TAIL
@@ -433,30 +468,44 @@ __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
     Pb -= 4 * tl;
     uint32_t Q = (uint32_t)Pb;
     W = qwindow(ring, Q);
-    const uint32_t G = ok && n > 19 ? (n - 20) / 16 + 1 : 0u;
-    for (uint32_t gi = 0; gi < G; gi++) {
-        if ((gi & (kQPhase / 4 - 1)) == 0) {
-            qstage_step(ring, g, (Pb >> 3) - 12, q);
-            wave_sync();
-        }
-        const uint32_t Q0 = Q;
+    // a group is 4 rounds (16 symbols, one dword per lane after the
+    // transpose); the groups whose states all update run 4 per staging step,
+    // and their dwords are stored at the next step between its ring stores
+    // and its loads, so that waiting for a step's loads never waits for
+    // output stores (both count in vmcnt, which drains in order)
+    auto group = [&]() -> uint32_t {
         uint32_t acc = fse_quad_round<false>(DT, SY, ring, st, W, Q, sq, 0, n);
         acc |= fse_quad_round<false>(DT, SY, ring, st, W, Q, sq, 0, n) << 8;
         acc |= fse_quad_round<false>(DT, SY, ring, st, W, Q, sq, 0, n) << 16;
         acc |= fse_quad_round<false>(DT, SY, ring, st, W, Q, sq, 0, n) << 24;
-        Pb -= Q0 - Q;
         const uint32_t c = __builtin_amdgcn_perm(quad_xor1(acc), acc, selA);
-        const uint32_t d = __builtin_amdgcn_perm(quad_xor2(c), c, selB);
-        uint8_t* o = out + 16 * (size_t)gi + 4 * q;
-        if (wide) {
-            *(g_uint32*)(uintptr_t)o = d;
-        } else {
-            g_uint8* ob = (g_uint8*)(uintptr_t)o;
-            ob[0] = (uint8_t)d;
-            ob[1] = (uint8_t)(d >> 8);
-            ob[2] = (uint8_t)(d >> 16);
-            ob[3] = (uint8_t)(d >> 24);
+        return __builtin_amdgcn_perm(quad_xor2(c), c, selB);
+    };
+    const uint64_t oq = P.outPtr + 4 * q;
+    const uint32_t G = ok && n > 19 ? (n - 20) / 16 + 1 : 0u, G4 = G & ~3u;
+    uint32_t o0 = 0, o1 = 0, o2 = 0, o3 = 0;
+    for (uint32_t gi = 0; gi < G4; gi += 4) {
+        qstage_put(ring, g);
+        if (gi) put4_dwords(oq + 16 * (uint64_t)(gi - 4), 16, o0, o1, o2, o3, wide);
+        qstage_load(g, (Pb >> 3) - 12, q);
+        wave_sync();
+        const uint32_t Q0 = Q;
+        o0 = group();
+        o1 = group();
+        o2 = group();
+        o3 = group();
+        Pb -= Q0 - Q;
+    }
+    if (G4) put4_dwords(oq + 16 * (uint64_t)(G4 - 4), 16, o0, o1, o2, o3, wide);
+    for (uint32_t gi = G4; gi < G; gi++) {
+        if ((gi & 3) == 0) {
+            qstage_step(ring, g, (Pb >> 3) - 12, q);
+            wave_sync();
         }
+        const uint32_t Q0 = Q;
+        const uint32_t d = group();
+        Pb -= Q0 - Q;
+        put_dword(oq + 16 * (uint64_t)gi, d, wide);
     }
     const uint32_t R = ok ? (n + 3) / 4 : 0u;
     for (uint32_t r = 4 * G; r < R; r++) {
@@ -473,10 +522,16 @@ __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
 }
 
 // ---------------------------------------------------------------- Huffman decoding
-// One lane per Huffman stream, 16 frames (64 streams) per workgroup; each lane
-// peeks tl bits, looks the code up in its frame's table and refills 32 bits
-// when the window holds <= 32 (huf_decompress.c:587-787).
-constexpr int kHF = 16;
+// One lane per Huffman stream, 8 frames (32 streams) per one-wave workgroup,
+// so that 4 workgroups (one per SIMD) share a CU's LDS; each lane peeks tl
+// bits, looks the code up in its frame's table and refills 32 bits when the
+// window holds <= 32 (huf_decompress.c:587-787). The lane stages its stream
+// into its ring itself: every kHPhase rounds it stores the two 16-byte blocks
+// it loaded kHPhase rounds earlier and loads the next two. Blocks may hold
+// bytes below the stream start (down to 8 bytes below it) that the window
+// peeks but a valid stream never consumes: every table entry whose top bits
+// are a code holds that code whatever the bits below.
+constexpr int kHF = 8;
 constexpr int kHRing = 128;
 constexpr int kHLook = 64;
 constexpr int kHPhase = 16;     // rounds between staging steps (<= 22 bytes consumed)
@@ -487,6 +542,66 @@ struct HufDecLds {
     HufBuild hb;
     ozl_dplan plan[kHF];
 };
+
+struct LStage {
+    uint64_t aLo;     // lowest address loaded or requested (16-aligned)
+    uint64_t f;       // floor16(stream start - 8)
+    uint4 p0, p1;     // blocks loaded by the last step
+    uint64_t a0, a1;  // their addresses (0: none)
+};
+
+template <int R>
+__device__ inline void lring_put(uint32_t* ring, uint64_t a, uint4 v)
+{
+    *reinterpret_cast<uint4*>(ring + ((uint32_t)(a >> 2) & (R / 4 - 1))) = v;
+}
+
+// initial fill of [max(floor16(top - LOOK), floor16(lo - 8)), top)
+template <int R, int LOOK>
+__device__ inline void lstage_init(uint32_t* ring, LStage& g, uint64_t lo, uint64_t top)
+{
+    g.f = floor16(lo - 8);
+    uint64_t a0 = floor16(top - LOOK);
+    if (a0 < g.f) a0 = g.f;
+    constexpr int NB = LOOK / 16 + 1;
+    uint4 v[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) v[k] = gload16(a0 + 16 * k < top ? a0 + 16 * k : a0);
+#pragma unroll
+    for (int k = 0; k < NB; k++)
+        if (a0 + 16 * k < top) lring_put<R>(ring, a0 + 16 * k, v[k]);
+    g.aLo = a0;
+    g.a0 = g.a1 = 0;
+}
+
+template <int R>
+__device__ inline void lstage_put(uint32_t* ring, LStage& g)
+{
+    if (g.a0) lring_put<R>(ring, g.a0, g.p0);
+    if (g.a1) lring_put<R>(ring, g.a1, g.p1);
+}
+
+template <int LOOK>
+__device__ inline void lstage_load(LStage& g, uint64_t winLo)
+{
+    uint64_t want = floor16(winLo - LOOK);
+    if (want < g.f) want = g.f;
+    uint64_t nb = g.aLo > want ? (g.aLo - want) >> 4 : 0;
+    if (nb > 2) nb = 2;
+    const uint64_t x0 = g.aLo - 16, x1 = g.aLo - 32;
+    g.p0 = gload16(nb > 0 ? x0 : g.f);
+    g.p1 = gload16(nb > 1 ? x1 : g.f);
+    g.a0 = nb > 0 ? x0 : 0;
+    g.a1 = nb > 1 ? x1 : 0;
+    g.aLo -= 16 * nb;
+}
+
+template <int R, int LOOK>
+__device__ inline void lstage_step(uint32_t* ring, LStage& g, uint64_t winLo)
+{
+    lstage_put<R>(ring, g);
+    lstage_load<LOOK>(g, winLo);
+}
 
 __global__ void __launch_bounds__(64) k_huf_decode(uint32_t B, ozl_dws ws)
 {
@@ -509,7 +624,7 @@ __global__ void __launch_bounds__(64) k_huf_decode(uint32_t B, ozl_dws ws)
         if ((factMask >> f) & 1) wave_huf_lut(L.lut[f], ws.side + (size_t)(b0 + f) * OZL_DSIDE_BYTES, L.plan[f].tl, L.hb);
     wave_sync();
     const int f = lane >> 2, k = lane & 3;
-    if (!((factMask >> f) & 1)) return;
+    if (f >= kHF || !((factMask >> f) & 1)) return;
     const ozl_dplan P = L.plan[f];
     const uint32_t n = P.n, payLen = P.payLen, tl = P.tl;
     const uint64_t s0 = P.srcPtr;
@@ -541,35 +656,51 @@ __global__ void __launch_bounds__(64) k_huf_decode(uint32_t B, ozl_dws ws)
     }
     const uint16_t* LUT = L.lut[f];
     uint32_t* ring = L.ring[lane];
-    uint8_t* out = reinterpret_cast<uint8_t*>((uintptr_t)P.outPtr) + outOff;
-    const bool wide = (((uintptr_t)out) & 3) == 0;
-    Stager g;
-    stage_init<kHRing>(ring, g, lo, lo + len, kHLook);
+    const uint64_t outA = P.outPtr + outOff;
+    const bool wide = (outA & 3) == 0;
+    LStage g;
+    lstage_init<kHRing, kHLook>(ring, g, lo, lo + len);
+    wave_sync();
     Win w;
     bool ok = win_init<kHRing>(w, ring, lo, len);
     if (ok) win_refill32(w, ring_dword<kHRing>(ring, lo + (uint64_t)(int64_t)w.lb - 4));
-    const uint32_t R = ok ? cnt : 0u;
-    uint32_t wacc = 0;
-    for (uint32_t r = 0; r < R; r++) {
-        if ((r & (kHPhase - 1)) == 0) stage_step<kHRing, 2>(ring, g, (uint64_t)((int64_t)lo + w.lb), kHLook);
-        uint32_t e = LUT[win_hi(w) >> (32 - tl)];
-        uint32_t d = ring_dword<kHRing>(ring, (uint64_t)((int64_t)lo + w.lb) - 4);
-        uint32_t l = e >> 8;
+    const uint32_t R = ok ? cnt : 0u, R16 = R & ~15u;
+    auto sym = [&]() -> uint32_t {
+        const uint32_t e = LUT[win_hi(w) >> (32 - tl)];
+        const uint32_t d = ring_dword<kHRing>(ring, (uint64_t)((int64_t)lo + w.lb) - 4);
+        const uint32_t l = e >> 8;
         w.acc <<= l;
         w.valid -= (int)l;
         win_refill32(w, d);
-        if (wide) {
-            wacc |= (e & 255u) << (8 * (r & 3));
-            if ((r & 3) == 3) {
-                *reinterpret_cast<uint32_t*>(out + r - 3) = wacc;
-                wacc = 0;
-            }
-        } else {
-            out[r] = (uint8_t)e;
+        return e & 255u;
+    };
+    // 16 rounds per staging step; their 16 bytes are stored at the next step
+    // between its ring stores and its loads, so that waiting for a step's
+    // loads never waits for output stores (both count in vmcnt, which drains
+    // in order)
+    uint32_t o[4] = { 0, 0, 0, 0 };
+    for (uint32_t r = 0; r < R16; r += 16) {
+        lstage_put<kHRing>(ring, g);
+        if (r) put4_dwords(outA + r - 16, 4, o[0], o[1], o[2], o[3], wide);
+        lstage_load<kHLook>(g, (uint64_t)((int64_t)lo + w.lb) - 4);
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            uint32_t x = sym();
+            x |= sym() << 8;
+            x |= sym() << 16;
+            x |= sym() << 24;
+            o[j] = x;
         }
     }
-    if (wide)
-        for (uint32_t j = R & ~3u; j < R; j++) out[j] = (uint8_t)(wacc >> (8 * (j & 3)));
+    if (R16) put4_dwords(outA + R16 - 16, 4, o[0], o[1], o[2], o[3], wide);
+    for (uint32_t r = R16; r < R; r++) {
+        if ((r & (kHPhase - 1)) == 0) {
+            lstage_step<kHRing, kHLook>(ring, g, (uint64_t)((int64_t)lo + w.lb) - 4);
+            wave_sync();
+        }
+        *(g_uint8*)(uintptr_t)(outA + r) = (uint8_t)sym();
+    }
     long pos = 8L * w.lb + w.valid;
     if (!ok || pos != 0) ws.plans[b0 + f].status = OZL_ERR_CORRUPTION;
 }
