@@ -226,15 +226,16 @@ __global__ void __launch_bounds__(kT) k_plan(const uint8_t* __restrict__ src, co
 // encoder state q (fse_compress.c:607-660). Emission e (e = 0.. n-S-1)
 // encodes position p = n-S-1-e with state (e - n) mod S; a round gives each
 // lane one emission, in slot (q + n) mod S of the round. The quad turns its
-// bit counts into offsets in slot order (byte-packed counts summed with two
-// DPP exchanges, prefix sums by a multiply with 0x01010101), ORs the shifted
+// bit counts into offsets in slot order (each lane's count copied into bytes
+// slot..3 of a word, the words summed with two DPP exchanges, so that byte k
+// holds the bits of slots 0..k), ORs the shifted
 // values into one 64-bit round word, and appends that to a 32-bit word writer
 // whose completed words lanes 0 and 1 store. Each lane's input symbols are
-// loaded 8 rounds ahead of use (an unrolled register queue), so the global
+// loaded 16 rounds ahead of use (an unrolled register queue), so the global
 // load latency stays off the state chain. Tables take 5 KB per block (40 KB
 // per workgroup, 4 workgroups per CU).
 constexpr int kFseGroups = 8;
-constexpr int kFsePipe = 8;
+constexpr int kFsePipe = 16;
 
 struct FseEncLds {
     uint16_t st[kFseGroups][2048];
@@ -305,42 +306,63 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
         uint32_t e = (uint32_t)k * S + kq;
         qbuf[k] = in[(mine && e < nEmit) ? nEmit - 1 - e : 0u];
     }
+    uint32_t tn = TT[qbuf[0]];          // transform of the next round's symbol
     uint32_t a0 = 0, ab = 0, wpos = 0;  // pending bits (ab < 32) and words written
-    for (uint32_t r0 = 0; r0 < rounds; r0 += kFsePipe) {
+    // one emission per lane: the state update, then the quad appends the
+    // round's bits in slot order
+    auto emit = [&](uint32_t t, bool doit) {
+        // nbBitsOut = (state + deltaNbBits) >> 16 = maxBits - (state < minStatePlus)
+        const uint32_t nb = doit ? (t & 15u) - (state < ((t >> 4) & 0x3FFFu) ? 1u : 0u) : 0u;
+        const uint32_t val = __builtin_amdgcn_ubfe(state, 0, nb);
+        const uint32_t ns = ST[doit ? (state >> nb) + (t >> 18) : 4096u];
+        state = doit ? ns : state;
+        // nb in bytes kq..3 (24-bit multiply, full rate), summed over the quad
+        uint32_t v = (__umul24(nb, 0x10101u) | (nb << 24)) << sq;
+        v += ozl_wave::quad_xor1(v);
+        const uint32_t M = v + ozl_wave::quad_xor2(v);  // byte k: bits of slots 0..k
+        const uint32_t excl = __builtin_amdgcn_ubfe(M, sq, 8) - nb;
+        const uint64_t R = (uint64_t)val << excl;
+        uint32_t rl = (uint32_t)R, rh = (uint32_t)(R >> 32);
+        rl |= ozl_wave::quad_xor1(rl);
+        rh |= ozl_wave::quad_xor1(rh);
+        rl |= ozl_wave::quad_xor2(rl);
+        rh |= ozl_wave::quad_xor2(rh);
+        const uint64_t T = (((uint64_t)rh << 32) | rl) << ab;
+        const uint32_t w0 = a0 | (uint32_t)T, w1 = (uint32_t)(T >> 32), w2 = __builtin_amdgcn_ubfe(rh, 32 - ab, ab);
+        const uint32_t nAcc = ab + (M >> 24), ne = nAcc >> 5;
+        if ((uint32_t)q < ne && wpos + q < capWords) outw[wpos + q] = q == 0 ? w0 : w1;
+        a0 = ne == 0 ? w0 : ne == 1 ? w1 : w2;
+        ab = nAcc & 31;
+        wpos += ne;
+    };
+    uint32_t r0 = 0;
+    if (S == 4) {
+        // rounds whose 4 emissions all exist, while the queue's next 16 do too:
+        // no per-lane conditions, queue loads at immediate offsets
+        const uint32_t RF = nEmit / 4;
+        for (; r0 + 2 * kFsePipe <= RF; r0 += kFsePipe) {
+            gc_uint8* pb = in + (nEmit - 1 - 4 * (r0 + kFsePipe) - kq);
+#pragma unroll
+            for (int k = 0; k < kFsePipe; k++) {
+                const uint32_t t = tn;
+                tn = TT[qbuf[(k + 1) % kFsePipe]];
+                qbuf[k] = pb[-4 * k];
+                emit(t, true);
+            }
+        }
+    }
+    for (; r0 < rounds; r0 += kFsePipe) {
 #pragma unroll
         for (int k = 0; k < kFsePipe; k++) {
             const uint32_t e = (r0 + (uint32_t)k) * S + kq;
-            const bool doit = mine && e < nEmit;
-            const uint32_t sym = qbuf[k];
+            const uint32_t t = tn;
+            tn = TT[qbuf[(k + 1) % kFsePipe]];
             const uint32_t eNext = e + kFsePipe * S;
             qbuf[k] = in[(mine && eNext < nEmit) ? nEmit - 1 - eNext : 0u];
-            const uint32_t t = TT[sym];
-            // nbBitsOut = (state + deltaNbBits) >> 16 = maxBits - (state < minStatePlus)
-            const uint32_t nb = doit ? (t & 15u) - (state < ((t >> 4) & 0x3FFFu) ? 1u : 0u) : 0u;
-            const uint32_t val = __builtin_amdgcn_ubfe(state, 0, nb);
-            const uint32_t ns = ST[doit ? (state >> nb) + (t >> 18) : 4096u];
-            state = doit ? ns : state;
-            uint32_t v = nb << sq;
-            v += ozl_wave::quad_xor1(v);
-            v += ozl_wave::quad_xor2(v);
-            const uint32_t M = v * 0x01010101u;  // byte k: bits of slots 0..k
-            const uint32_t excl = __builtin_amdgcn_ubfe(M, sq, 8) - nb;
-            const uint64_t R = (uint64_t)val << excl;
-            uint32_t rl = (uint32_t)R, rh = (uint32_t)(R >> 32);
-            rl |= ozl_wave::quad_xor1(rl);
-            rh |= ozl_wave::quad_xor1(rh);
-            rl |= ozl_wave::quad_xor2(rl);
-            rh |= ozl_wave::quad_xor2(rh);
-            const uint64_t T = (((uint64_t)rh << 32) | rl) << ab;
-            const uint32_t w0 = a0 | (uint32_t)T, w1 = (uint32_t)(T >> 32), w2 = __builtin_amdgcn_ubfe(rh, 32 - ab, ab);
-            const uint32_t nAcc = ab + (M >> 24), ne = nAcc >> 5;
-            if ((uint32_t)q < ne && wpos + q < capWords) outw[wpos + q] = q == 0 ? w0 : w1;
-            a0 = ne == 0 ? w0 : ne == 1 ? w1 : w2;
-            ab = nAcc & 31;
-            wpos += ne;
+            emit(t, mine && e < nEmit);
         }
     }
-    // flush the states in index order, then the end mark
+    // the states in index order, then the end mark
     const uint32_t f0 = ozl_wave::quad_lane<0>(state), f1 = ozl_wave::quad_lane<1>(state);
     const uint32_t f2 = ozl_wave::quad_lane<2>(state), f3 = ozl_wave::quad_lane<3>(state);
     if (q != 0) return;

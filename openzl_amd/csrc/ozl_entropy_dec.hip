@@ -229,8 +229,8 @@ __device__ inline uint32_t win_hi(const Win& w) { return (uint32_t)(w.acc >> 32)
 // A quad of lanes decodes one frame of 4 interleaved states
 // (fse_decompress.c:255-338), lane q holding state q: per round every lane
 // looks its state up, the quad adds up its 4 bit counts (each lane's count
-// placed in byte q, summed over the quad with two DPP exchanges, then a
-// multiply by 0x01010101 turns the bytes into prefix sums), and lane q takes
+// copied into bytes q..3 of a word, the words summed over the quad with two
+// DPP exchanges, so that byte k holds the bits of states 0..k), and lane q takes
 // its bits below those of the states before it from a 64-bit window that every
 // lane reads from the frame's LDS ring at the quad's bit position. After 4
 // rounds the quad's 16 symbols are transposed across its lanes (two DPP byte
@@ -399,10 +399,10 @@ __device__ inline uint32_t fse_quad_round(const uint16_t* DT, const uint8_t* SY,
     const uint32_t e = DT[st], y = SY[st];
     const bool upd = !TAIL || p + 4 < n;
     const uint32_t nb = upd ? e & 15u : 0u;
-    uint32_t v = nb << sq;
+    // nb in bytes q..3 (24-bit multiply, full rate), summed over the quad
+    uint32_t v = (__umul24(nb, 0x10101u) | (nb << 24)) << sq;
     v += quad_xor1(v);
-    v += quad_xor2(v);
-    const uint32_t M = v * 0x01010101u;           // byte k: bits of states 0..k
+    const uint32_t M = v + quad_xor2(v);          // byte k: bits of states 0..k
     const uint32_t incl = __builtin_amdgcn_ubfe(M, sq, 8);
     const uint32_t bits = __builtin_amdgcn_ubfe((uint32_t)(W >> ((64 - incl) & 63)), 0, nb);
     st = upd ? (e >> 4) + bits : st;
