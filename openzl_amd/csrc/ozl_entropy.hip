@@ -2,10 +2,11 @@
 //
 // Pipeline for B blocks (each < 100000 bytes, so the graphs do not split it,
 // src/openzl/codecs/entropy/encode_entropy_binding.c:491-519):
-//   k_plan          one workgroup per block: LDS histogram, then one lane
-//                   decides the backend (:601-753) and builds the FSE tables or
-//                   Huffman code, the NCount, and the nested graph of the
-//                   Huffman weights (ozl_core.h)
+//   k_hist          one workgroup per block: LDS histogram
+//   k_plan          one wave per block: one lane decides the backend
+//                   (:601-753) and builds the FSE tables or Huffman code, the
+//                   NCount, and the nested graph of the Huffman weights
+//                   (ozl_core.h); the wave builds the FSE encoding table
 //   k_fse_encode    one 4-lane group per block, one lane per interleaved state,
 //                   8 blocks per wavefront; a round's bit offsets come from a
 //                   quad prefix sum (DPP), tables sit in LDS
@@ -54,24 +55,27 @@ __device__ inline void block_hist(const uint8_t* src, uint32_t n, uint32_t* h4 /
 }
 
 struct PlanLds {
-    uint32_t h4[4 * 256];
     uint32_t count[256];
     int16_t norm[256];
-    ozl_huf_scratch hs;
+    union {
+        ozl_huf_scratch hs;                         // Huffman code
+        ozl_wave::FseBuild fb;                      // main FSE table build
+    };
     uint8_t wts[256];
     uint32_t wcount[256];
     uint8_t cells[1 << OZL_WEIGHTS_MAX_TABLELOG];   // nested weights graph
     uint16_t wst[1 << OZL_WEIGHTS_MAX_TABLELOG];
     uint32_t wtt[256];
     int16_t wn[256];
-    ozl_wave::FseBuild fb;                          // main FSE table build
-    uint16_t st[2048];
-    uint32_t tt[256];
     uint8_t hlen[256];
     uint16_t hval[256];
     uint8_t aux[OZL_AUX_BYTES];
     ozl_eplan plan;
 };
+
+// The table area of a block holds its histogram (u32[256]) at this offset
+// between k_hist and k_plan; the FSE tables end below it.
+#define OZL_TAB_COUNT_OFF 9216
 
 // Nested FSE graph on the Huffman weights (entropyCompressChunk in EBM_fse
 // mode on the num->serial converted weights stream).
@@ -124,20 +128,38 @@ __device__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux)
     }
 }
 
-// One workgroup per block: histogram (all lanes), decision / normalisation /
-// Huffman code / nested weights graph (lane 0), FSE encoding table (wave 0,
-// ozl_wave::wave_fse_ctable); the tables and the auxiliary streams are built
-// in LDS and leave with coalesced stores.
+// Histogram of every block (one 256-thread workgroup per block, LDS
+// atomics on 4 sub-histograms), parked in the block's table area.
+__global__ void __launch_bounds__(kT) k_hist(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off, uint32_t B,
+                                             ozl_ews ws)
+{
+    __shared__ uint32_t h4[4 * 256];
+    __shared__ uint32_t count[256];
+    uint32_t b = blockIdx.x;
+    if (b >= B) return;
+    block_hist(src + off[b], (uint32_t)(off[b + 1] - off[b]), h4, count);
+    uint32_t* cg = reinterpret_cast<uint32_t*>(ws.tables + (size_t)b * OZL_TAB_BYTES + OZL_TAB_COUNT_OFF);
+    for (int s = threadIdx.x; s < 256; s += kT) cg[s] = count[s];
+}
+
+// One wave per block: decision / normalisation / Huffman code / nested
+// weights graph (lane 0, ozl_core.h), FSE encoding table (the wave,
+// ozl_wave::wave_fse_ctable, straight into the block's table area). Small LDS
+// (12 KB) lets a CU run a dozen of these serial builds at once.
 template <int HUF>
-__global__ void __launch_bounds__(kT) k_plan(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
+__global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
                                              uint32_t B, ozl_ews ws)
 {
     __shared__ PlanLds L;
     uint32_t b = blockIdx.x;
     if (b >= B) return;
-    const uint8_t* in = src + off[b];
     uint32_t n = (uint32_t)(off[b + 1] - off[b]);
-    block_hist(in, n, L.h4, L.count);
+    uint8_t* tab = ws.tables + (size_t)b * OZL_TAB_BYTES;
+    {
+        const uint32_t* cg = reinterpret_cast<const uint32_t*>(tab + OZL_TAB_COUNT_OFF);
+        for (int s = threadIdx.x; s < 256; s += 64) L.count[s] = cg[s];
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         ozl_eplan p = {};
         p.n = n;
@@ -198,25 +220,18 @@ __global__ void __launch_bounds__(kT) k_plan(const uint8_t* __restrict__ src, co
     }
     __syncthreads();
     const ozl_eplan& p = L.plan;
-    uint8_t* tab = ws.tables + (size_t)b * OZL_TAB_BYTES;
     if (p.bk == OZL_BK_FSE && p.status == 0) {
-        if (threadIdx.x < 64) ozl_wave::wave_fse_ctable(L.st, L.tt, L.norm, p.maxSym, p.tl, L.fb);
-        __syncthreads();
-        unsigned T = 1u << p.tl;
-        uint32_t* st32 = reinterpret_cast<uint32_t*>(tab);
-        const uint32_t* ls32 = reinterpret_cast<const uint32_t*>(L.st);
-        for (unsigned i = threadIdx.x; i < T / 2; i += kT) st32[i] = ls32[i];
-        uint32_t* tt = reinterpret_cast<uint32_t*>(tab + 8192);
-        for (unsigned s = threadIdx.x; s <= p.maxSym; s += kT) tt[s] = L.tt[s];
+        ozl_wave::wave_fse_ctable(reinterpret_cast<uint16_t*>(tab), reinterpret_cast<uint32_t*>(tab + 8192), L.norm,
+                                  p.maxSym, p.tl, L.fb);
     } else if (p.bk == OZL_BK_HUF && p.status == 0) {
         uint16_t* val = reinterpret_cast<uint16_t*>(tab + 256);
-        for (unsigned s = threadIdx.x; s < 256; s += kT) {
+        for (unsigned s = threadIdx.x; s < 256; s += 64) {
             tab[s] = s <= p.maxSym ? L.hlen[s] : 0;
             val[s] = s <= p.maxSym ? L.hval[s] : 0;
         }
     }
     uint8_t* aux = ws.aux + (size_t)b * OZL_AUX_BYTES;
-    for (unsigned i = threadIdx.x; i < OZL_AUX_BYTES / 4; i += kT)
+    for (unsigned i = threadIdx.x; i < OZL_AUX_BYTES / 4; i += 64)
         reinterpret_cast<uint32_t*>(aux)[i] = reinterpret_cast<const uint32_t*>(L.aux)[i];
     if (threadIdx.x == 0) ws.plans[b] = p;
 }
@@ -697,8 +712,9 @@ int ozl_entropy_encode_chunks(int hufMode, const uint8_t* s, const uint64_t* off
 {
     if (B == 0) return OZL_OK;
     if (maxBlock >= 100000) return OZL_ERR_PARAMETER_INVALID;
-    if (hufMode) OZL_LAUNCH("k_plan", k_plan<1>, dim3(B), dim3(kT), 0, st, s, off, B, ws);
-    else OZL_LAUNCH("k_plan", k_plan<0>, dim3(B), dim3(kT), 0, st, s, off, B, ws);
+    OZL_LAUNCH("k_hist", k_hist, dim3(B), dim3(kT), 0, st, s, off, B, ws);
+    if (hufMode) OZL_LAUNCH("k_plan", k_plan<1>, dim3(B), dim3(64), 0, st, s, off, B, ws);
+    else OZL_LAUNCH("k_plan", k_plan<0>, dim3(B), dim3(64), 0, st, s, off, B, ws);
     if (hufMode) OZL_LAUNCH("k_huf_encode", k_huf_encode, dim3(B), dim3(kT), 0, st, s, off, B, ws);
     else OZL_LAUNCH("k_fse_encode", k_fse_encode, dim3((B + kFseGroups - 1) / kFseGroups), dim3(64), 0, st, s, off, B, ws);
     OZL_LAUNCH("k_pack_encode", k_pack_encode, dim3(B), dim3(kT), 0, st, s, off, B, ws);
