@@ -408,9 +408,13 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
 
 // ------------------------------------------------------------------ Huffman encode
 // One workgroup per block (huf_compress.c:1139-1196 for 4 streams, :1022-1086
-// per stream). Per stream: lane t takes a run of symbols in reverse order,
-// sums their code lengths, a block scan gives its starting bit, and it ORs its
-// codes into an LDS bit buffer; lane 0 then knows the stream size.
+// per stream). Per stream the symbols are coded from the last to the first:
+// lane t takes the t-th run of `per` symbols from the end (per a multiple of
+// 4, read as dwords), sums their code lengths, a block scan gives its
+// starting bit, then it packs its codes in a 64-bit register and writes each
+// completed word to an LDS bit buffer (an atomic OR only for its first and
+// last words, which it shares with its neighbours); the buffer leaves in
+// dwords.
 constexpr int kHufMaxSeg = 25008;  // (100000+3)/4 symbols per stream
 
 template <typename T>
@@ -438,20 +442,16 @@ __device__ T block_scan_excl(T v, T* lds, T& total)
 __global__ void __launch_bounds__(kT) k_huf_encode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
                                                    uint32_t B, ozl_ews ws)
 {
-    __shared__ uint8_t sLen[256];
-    __shared__ uint16_t sVal[256];
-    __shared__ uint32_t sBits[(kHufMaxSeg * 12) / 32 + 64];
+    __shared__ uint32_t sCode[256];  // code value | length << 16
+    __shared__ uint32_t sBits[(kHufMaxSeg * 11) / 32 + 64];
     __shared__ uint32_t sScan[4];
-    __shared__ uint32_t sOut;
     uint32_t b = blockIdx.x;
     if (b >= B) return;
     ozl_eplan p = ws.plans[b];
     if (p.bk != OZL_BK_HUF || p.status != 0) return;
     const uint8_t* tab = ws.tables + (size_t)b * OZL_TAB_BYTES;
-    for (int s = threadIdx.x; s < 256; s += kT) {
-        sLen[s] = s <= p.maxSym ? tab[s] : 0;
-        sVal[s] = s <= p.maxSym ? reinterpret_cast<const uint16_t*>(tab + 256)[s] : 0;
-    }
+    for (int s = threadIdx.x; s < 256; s += kT)
+        sCode[s] = s <= p.maxSym ? (uint32_t)reinterpret_cast<const uint16_t*>(tab + 256)[s] | ((uint32_t)tab[s] << 16) : 0u;
     const uint8_t* in = src + off[b];
     uint8_t* out = ws.payload + (size_t)b * ws.payloadCap;
     const uint32_t n = p.n;
@@ -461,38 +461,88 @@ __global__ void __launch_bounds__(kT) k_huf_encode(const uint8_t* __restrict__ s
     uint32_t opos = x4 ? 6 : 0;
     __syncthreads();
     for (int k = 0; k < nbSeg; k++) {
-        uint32_t s0 = k * seg;
-        uint32_t len = (k < nbSeg - 1) ? seg : n - s0;
-        // lane t handles reverse indices [t*per, t*per+per): symbol s0+len-1-r
-        uint32_t per = (len + kT - 1) / kT;
-        uint32_t r0 = threadIdx.x * per, r1 = r0 + per < len ? r0 + per : len;
+        const uint32_t s0 = k * seg;
+        const uint32_t len = (k < nbSeg - 1) ? seg : n - s0;
+        const uint32_t per = ((len + kT - 1) / kT + 3) & ~3u;
+        // this lane's symbols: positions [lo, hi) of the stream, coded downward
+        const uint32_t hiR = threadIdx.x * per, loR = hiR + per;
+        const uint32_t hi = hiR < len ? len - hiR : 0u, lo = loR < len ? len - loR : 0u;
+        const uint8_t* base = in + s0;
         uint32_t mine = 0;
-        for (uint32_t r = r0; r < r1; r++) mine += sLen[in[s0 + len - 1 - r]];
+        for (uint32_t e = hi; e > lo; e -= 4) {
+            if (e >= lo + 4) {
+                const uint32_t d = *reinterpret_cast<const uint32_t*>(base + e - 4);
+                mine += (sCode[d >> 24] >> 16) + (sCode[(d >> 16) & 255] >> 16) + (sCode[(d >> 8) & 255] >> 16) +
+                        (sCode[d & 255] >> 16);
+            } else {
+                for (uint32_t q = e; q > lo; q--) mine += sCode[base[q - 1]] >> 16;
+                break;
+            }
+        }
         uint32_t total;
-        uint32_t bit = block_scan_excl<uint32_t>(mine, sScan, total);
-        uint32_t words = (total + 1 + 31) / 32;
+        const uint32_t bit = block_scan_excl<uint32_t>(mine, sScan, total);
+        const uint32_t words = (total + 1 + 31) / 32;
         for (uint32_t w = threadIdx.x; w < words; w += kT) sBits[w] = 0;
         __syncthreads();
-        for (uint32_t r = r0; r < r1; r++) {
-            uint8_t sy = in[s0 + len - 1 - r];
-            uint32_t l = sLen[sy], v = sVal[sy];
-            // code value bits go LSB-first at [bit, bit+l)
-            uint32_t w = bit >> 5, sh = bit & 31;
-            atomicOr(&sBits[w], v << sh);
-            if (sh + l > 32) atomicOr(&sBits[w + 1], v >> (32 - sh));
-            bit += l;
+        if (mine) {
+            const uint32_t w0 = bit >> 5;
+            uint64_t acc = 0;
+            uint32_t nacc = bit & 31, wpos = w0;
+            auto put = [&](uint32_t c) {
+                acc |= (uint64_t)(c & 0xFFFFu) << nacc;
+                nacc += c >> 16;
+                if (nacc >= 32) {
+                    if (wpos == w0) atomicOr(&sBits[wpos], (uint32_t)acc);
+                    else sBits[wpos] = (uint32_t)acc;
+                    wpos++;
+                    acc >>= 32;
+                    nacc -= 32;
+                }
+            };
+            for (uint32_t e = hi; e > lo; e -= 4) {
+                if (e >= lo + 4) {
+                    const uint32_t d = *reinterpret_cast<const uint32_t*>(base + e - 4);
+                    put(sCode[d >> 24]);
+                    put(sCode[(d >> 16) & 255]);
+                    put(sCode[(d >> 8) & 255]);
+                    put(sCode[d & 255]);
+                } else {
+                    for (uint32_t q = e; q > lo; q--) put(sCode[base[q - 1]]);
+                    break;
+                }
+            }
+            if (nacc) atomicOr(&sBits[wpos], (uint32_t)acc);
         }
-        if (threadIdx.x == 0) {
-            uint32_t w = total >> 5, sh = total & 31;
-            atomicOr(&sBits[w], 1u << sh);  // end mark
-        }
+        if (threadIdx.x == 0) atomicOr(&sBits[total >> 5], 1u << (total & 31));  // end mark
         __syncthreads();
-        uint32_t bytes = (total + 1 + 7) / 8;
+        const uint32_t bytes = (total + 1 + 7) / 8;
         if (opos + bytes > ws.payloadCap) {  // cannot happen for codes of <= 11 bits chosen by the estimate
             if (threadIdx.x == 0) ws.plans[b].status = OZL_ERR_INTERNAL_BUFFER;
             return;
         }
-        for (uint32_t i = threadIdx.x; i < bytes; i += kT) out[opos + i] = (uint8_t)(sBits[i >> 2] >> (8 * (i & 3)));
+        // out + opos need not be 4-aligned: lane i writes the i-th dword of
+        // the aligned span, assembled from the two buffer words it covers
+        uint8_t* dst = out + opos;
+        const uint32_t mis = (uint32_t)((uintptr_t)dst & 3);
+        uint32_t* dw = reinterpret_cast<uint32_t*>(dst - mis);
+        const uint32_t nw = (mis + bytes + 3) / 4;
+        for (uint32_t i = threadIdx.x; i < nw; i += kT) {
+            // dword i covers stream bytes [4i - mis, 4i - mis + 4)
+            const int sb = 4 * (int)i - (int)mis;
+            uint32_t v;
+            if (mis == 0) {
+                v = sBits[i];
+            } else {
+                const uint32_t lo32 = i > 0 ? sBits[i - 1] : 0u, hi32 = sBits[i];
+                v = __builtin_amdgcn_alignbit(hi32, lo32, 32 - 8 * mis);
+            }
+            if (sb >= 0 && sb + 4 <= (int)bytes) {
+                dw[i] = v;
+            } else {
+                for (int q = 0; q < 4; q++)
+                    if (sb + q >= 0 && sb + q < (int)bytes) dst[sb + q] = (uint8_t)(v >> (8 * q));
+            }
+        }
         if (threadIdx.x == 0 && x4 && k < 3) {
             out[2 * k] = (uint8_t)bytes;
             out[2 * k + 1] = (uint8_t)(bytes >> 8);
@@ -501,7 +551,6 @@ __global__ void __launch_bounds__(kT) k_huf_encode(const uint8_t* __restrict__ s
         __syncthreads();
     }
     if (threadIdx.x == 0) ws.plans[b].payload = opos;
-    (void)sOut;
 }
 
 // ------------------------------------------------------------------ flatpack / bitpack payloads
