@@ -205,9 +205,15 @@ OZL_HD int ozl_fse_normalize_m2(int16_t* norm, unsigned tl, const uint32_t* coun
 }
 
 // FSE_normalizeCount with useLowProbCount=1, fse_compress.c:487-549
+// rtbTable of FSE_normalizeCount (selects, not an indexed local array)
+OZL_HD uint32_t ozl_fse_rtb(int p)
+{
+    return p == 1 ? 473195u : p == 2 ? 504333u : p == 3 ? 520860u : p == 4 ? 550000u
+         : p == 5 ? 700000u : p == 6 ? 750000u : p == 7 ? 830000u : 0u;
+}
+
 OZL_HD int ozl_fse_normalize(int16_t* norm, unsigned tl, const uint32_t* count, size_t total, unsigned maxSym)
 {
-    const uint32_t rtb[8] = { 0, 473195, 504333, 520860, 550000, 700000, 750000, 830000 };
     if (tl < OZL_FSE_MIN_TABLELOG || tl > OZL_FSE_MAX_TABLELOG) return -1;
     if (tl < ozl_fse_min_tablelog(total, maxSym)) return -1;
     uint64_t scale = 62 - tl;
@@ -224,7 +230,7 @@ OZL_HD int ozl_fse_normalize(int16_t* norm, unsigned tl, const uint32_t* count, 
         if (c <= lowThreshold) { norm[s] = -1; still--; continue; }
         int16_t p = (int16_t)((c * step) >> scale);
         if (p < 8) {
-            uint64_t restToBeat = vStep * rtb[p];
+            uint64_t restToBeat = vStep * ozl_fse_rtb(p);
             p = (int16_t)(p + ((c * step) - ((uint64_t)p << scale) > restToBeat));
         }
         if (p > largestP) { largestP = p; largest = s; }
@@ -266,12 +272,11 @@ OZL_HD uint32_t ozl_tt_dnb(uint32_t p) { return ((p & 15) << 16) - ((p >> 4) & 0
 OZL_HD int ozl_tt_dfs(uint32_t p) { return (int)(p >> 18) - 4096; }
 
 // FSE_buildCTable_wksp, fse_compress.c:69-215. stateTable[1<<tl], tt[maxSym+1],
-// cellSym is scratch of 1<<tl bytes.
+// cellSym is scratch of 1<<tl bytes, cumul of maxSym+2 entries.
 OZL_HD int ozl_fse_build_ctable(uint16_t* stateTable, uint32_t* tt, uint8_t* cellSym, const int16_t* norm,
-                                unsigned maxSym, unsigned tl)
+                                unsigned maxSym, unsigned tl, uint16_t* cumul)
 {
     unsigned ts = 1u << tl;
-    uint16_t cumul[257];
     if (ozl_fse_spread(cellSym, norm, maxSym, tl)) return -1;
     cumul[0] = 0;
     for (unsigned s = 0; s <= maxSym; s++) cumul[s + 1] = (uint16_t)(cumul[s] + (norm[s] == -1 ? 1 : norm[s]));
@@ -290,10 +295,11 @@ OZL_HD int ozl_fse_build_ctable(uint16_t* stateTable, uint32_t* tt, uint8_t* cel
 
 // Decoding table entry: symbol | nbBits << 8 | newState << 16
 // (fse_decompress.c:160-172).
-OZL_HD int ozl_fse_build_dtable(uint32_t* dt, uint8_t* cellSym, const int16_t* norm, unsigned maxSym, unsigned tl)
+// `next` is workspace for maxSym+1 entries.
+OZL_HD int ozl_fse_build_dtable(uint32_t* dt, uint8_t* cellSym, const int16_t* norm, unsigned maxSym, unsigned tl,
+                                uint16_t* next)
 {
     unsigned ts = 1u << tl;
-    uint16_t next[256];
     if (ozl_fse_spread(cellSym, norm, maxSym, tl)) return -1;
     for (unsigned s = 0; s <= maxSym; s++) next[s] = (uint16_t)(norm[s] == -1 ? 1 : norm[s]);
     for (unsigned u = 0; u < ts; u++) {
@@ -360,10 +366,21 @@ OZL_HD size_t ozl_fse_write_ncount(uint8_t* dst, const int16_t* norm, unsigned m
 // FSE_readNCount). Returns bytes used, or -1.
 OZL_HD long ozl_fse_read_ncount(int16_t* norm, unsigned* maxSymOut, unsigned* tlOut, const uint8_t* src, size_t size)
 {
-    size_t pos = 0, total = size * 8;
+    // LSB-first fields through a bit buffer refilled a byte at a time (zero
+    // bits past the end)
+    size_t pos = 0, total = size * 8, next = 0;
+    uint64_t buf = 0;
+    int have = 0;
     auto rd = [&](int nb) -> uint32_t {
-        uint32_t v = 0;
-        for (int b = 0; b < nb; b++, pos++) v |= (uint32_t)(pos < total ? ((src[pos >> 3] >> (pos & 7)) & 1) : 0) << b;
+        while (have < nb) {
+            buf |= (uint64_t)(next < size ? src[next] : 0) << have;
+            next++;
+            have += 8;
+        }
+        uint32_t v = (uint32_t)(buf & ((1ull << nb) - 1));
+        buf >>= nb;
+        have -= nb;
+        pos += (size_t)nb;
         return v;
     };
     unsigned tl = rd(4) + OZL_FSE_MIN_TABLELOG;
@@ -410,18 +427,29 @@ OZL_HD long ozl_fse_read_ncount(int16_t* norm, unsigned* maxSymOut, unsigned* tl
 OZL_HD size_t ozl_fse_encode_seq(uint8_t* dst, size_t cap, const uint8_t* src, size_t n, const uint16_t* stateTable,
                                  const uint32_t* tt, unsigned tl, unsigned nbStates)
 {
-    if (n <= nbStates || cap <= 8) return 0;
-    uint32_t st[4];
+    if (n <= nbStates || nbStates < 1 || nbStates > 4 || cap <= 8) return 0;
+    // state (s + j) % nbStates lives in x[j]; every step uses x0 and rotates
+    // (no dynamically indexed local array)
+    uint32_t x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+    auto rot = [&]() {
+        uint32_t t = x0;
+        if (nbStates == 4) { x0 = x1; x1 = x2; x2 = x3; x3 = t; }
+        else if (nbStates == 3) { x0 = x1; x1 = x2; x2 = t; }
+        else if (nbStates == 2) { x0 = x1; x1 = t; }
+    };
     uint64_t acc = 0;
     unsigned accBits = 0;
     size_t op = 0;
-    size_t s = nbStates - (n % nbStates), p = n;
+    size_t p = n;
+    // state indices start at nbStates - (n % nbStates): rotate into place
+    for (size_t k = 0; k < (nbStates - (n % nbStates)) % nbStates; k++) rot();
     for (unsigned i = 0; i < nbStates; i++) {
         uint32_t t = tt[src[--p]];
         uint32_t dnb = ozl_tt_dnb(t);
         uint32_t nbOut = (dnb + (1u << 15)) >> 16;
         uint32_t v = (nbOut << 16) - dnb;
-        st[(s++) % nbStates] = stateTable[(int)(v >> nbOut) + ozl_tt_dfs(t)];
+        x0 = stateTable[(int)(v >> nbOut) + ozl_tt_dfs(t)];
+        rot();
     }
     auto put = [&](uint32_t v, unsigned nb) {
         acc |= (uint64_t)(v & ((1u << nb) - 1)) << accBits;
@@ -435,12 +463,15 @@ OZL_HD size_t ozl_fse_encode_seq(uint8_t* dst, size_t cap, const uint8_t* src, s
     };
     while (p > 0) {
         uint32_t t = tt[src[--p]];
-        uint32_t* x = &st[(s++) % nbStates];
-        uint32_t nbOut = (*x + ozl_tt_dnb(t)) >> 16;
-        put(*x, nbOut);
-        *x = stateTable[(int)(*x >> nbOut) + ozl_tt_dfs(t)];
+        uint32_t nbOut = (x0 + ozl_tt_dnb(t)) >> 16;
+        put(x0, nbOut);
+        x0 = stateTable[(int)(x0 >> nbOut) + ozl_tt_dfs(t)];
+        rot();
     }
-    for (unsigned i = 0; i < nbStates; i++) put(st[(s++) % nbStates], tl);
+    for (unsigned i = 0; i < nbStates; i++) {
+        put(x0, tl);
+        rot();
+    }
     put(1, 1);
     if (accBits) {
         if (op < cap) dst[op] = (uint8_t)acc;
@@ -670,7 +701,8 @@ OZL_HD int ozl_huf_build(uint8_t* nbBits, uint16_t* values, const uint32_t* coun
 
 // Canonical code lengths/values from weights (the decoder side of
 // decode_entropy_binding.c:205-284). Returns tableLog or -1.
-OZL_HD int ozl_huf_from_weights(uint8_t* len, uint16_t* val, const uint8_t* w, unsigned nbW)
+// `rk` is workspace for 28 entries.
+OZL_HD int ozl_huf_from_weights(uint8_t* len, uint16_t* val, const uint8_t* w, unsigned nbW, uint16_t* rk)
 {
     uint32_t total = 0;
     for (unsigned s = 0; s < nbW; s++) {
@@ -680,7 +712,7 @@ OZL_HD int ozl_huf_from_weights(uint8_t* len, uint16_t* val, const uint8_t* w, u
     if (total == 0 || (total & (total - 1))) return -1;
     unsigned tl = (unsigned)ozl_hb32(total);
     if (tl > 12) return -1;
-    uint16_t perRank[14], start[14];
+    uint16_t *perRank = rk, *start = rk + 14;
     for (int r = 0; r < 14; r++) perRank[r] = start[r] = 0;
     for (unsigned s = 0; s < 256; s++) len[s] = 0;
     for (unsigned s = 0; s < nbW; s++)
@@ -693,10 +725,10 @@ OZL_HD int ozl_huf_from_weights(uint8_t* len, uint16_t* val, const uint8_t* w, u
 
 // ------------------------------------------------------------------ small packers
 // flatpack (encode_flatpack_kernel.c:8-33, 93-130); returns alphabet size.
+// `map` is workspace of 256 entries.
 OZL_HD size_t ozl_flatpack_encode_seq(uint8_t* alphabet, uint8_t* packed, size_t* packedSize, const uint8_t* src,
-                                      size_t n, const uint32_t* count)
+                                      size_t n, const uint32_t* count, uint8_t* map)
 {
-    uint8_t map[256];
     size_t card = 0;
     for (int s = 0; s < 256; s++)
         if (count[s]) { map[s] = (uint8_t)card; alphabet[card++] = (uint8_t)s; }

@@ -86,7 +86,23 @@ struct ozl_reader {
 // decoding table is bounded here at 2^8 cells.
 #define OZL_WEIGHTS_MAX_TABLELOG 8
 
+// Per-frame parsing state kept with the tables (in LDS on the device, where
+// dynamically indexed locals would live in scratch memory).
+struct ozl_dscratch {
+    uint8_t fh[24];
+    ozl_fnode nodes[8];
+    ozl_fnode en[8];
+    ozl_fstream st[12];
+    uint32_t soff[12];
+    int order[12];
+    uint32_t f[8], ids[8], hf[8], vf[8], inf[8], dist[8];
+    uint32_t storedSize[4];
+    uint16_t next[256];  // table build workspace
+    uint16_t rk[28];
+};
+
 struct ozl_dlds {
+    ozl_dscratch x;
     int16_t norm[256];
     uint8_t cells[1 << OZL_WEIGHTS_MAX_TABLELOG];
     uint8_t w[256];
@@ -145,7 +161,7 @@ OZL_HDN int ozl_decode_weights(const ozl_fview& fr, const ozl_fnode* nodes, int 
     }
     if (t == OZL_TR_FSE_V2) {
         unsigned ms, tl;
-        int16_t wn[256];
+        int16_t* wn = L.norm;  // free in Huffman frames
         if (nn == 4 && nodes[3].trid == OZL_TR_FSE_NCOUNT) {  // NCount encoded
             if (ozl_fse_read_ncount(wn, &ms, &tl, fr.at(soff[6], st[6].size), st[6].size) < 0) return -1;
         } else {        // raw int16 counts, stored after a num->serial conversion
@@ -163,7 +179,7 @@ OZL_HDN int ozl_decode_weights(const ozl_fview& fr, const ozl_fnode* nodes, int 
         }
         if (tl < OZL_FSE_MIN_TABLELOG || tl > OZL_FSE_MAX_TABLELOG) return -1;
         if (tl > OZL_WEIGHTS_MAX_TABLELOG) return -1;
-        if (ozl_fse_build_dtable(L.wdt, L.cells, wn, ms, tl)) return -1;
+        if (ozl_fse_build_dtable(L.wdt, L.cells, wn, ms, tl, L.x.next)) return -1;
         return ozl_fse_decode_seq(L.w, m, fr.at(soff[5], st[5].size), st[5].size, L.wdt, tl, nodes[2].hdr[0]);
     }
     return -1;
@@ -184,7 +200,7 @@ OZL_HDN void ozl_dparse_frame(const ozl_fview& fr, size_t flen, ozl_dlds& L, ozl
     uint32_t tok = r.u8();
     uint64_t n1 = r.varint();
     bool ok = !r.bad && magic == OZL_MAGIC_BASE + OZL_FORMAT_VERSION && flags == 3 && tok == 1 && n1 >= 1;
-    uint8_t fh[24];
+    uint8_t* fh = L.x.fh;
     size_t fhl = 0;
     if (ok) {
         fhl = ozl_write_frame_header(fh, OZL_T_SERIAL, n1 - 1, 1, 1);
@@ -193,9 +209,9 @@ OZL_HDN void ozl_dparse_frame(const ozl_fview& fr, size_t flen, ozl_dlds& L, ozl
     }
     d.n = (uint32_t)(n1 - 1);
     d.chunkOff = (uint32_t)r.pos;
-    ozl_fnode nodes[8];
-    ozl_fstream st[12];
-    uint32_t soff[12];
+    ozl_fnode* nodes = L.x.nodes;
+    ozl_fstream* st = L.x.st;
+    uint32_t* soff = L.x.soff;
     int nn = 0, nbStored = 0;
     if (ok) {
         uint64_t nbDec = r.varint();
@@ -204,7 +220,8 @@ OZL_HDN void ozl_dparse_frame(const ozl_fview& fr, size_t flen, ozl_dlds& L, ozl
         if (ok) {
             nn = (int)nbDec - 1;
             nbStored = (int)nbSt;
-            uint32_t f[8], ids[8];
+            uint32_t* f = L.x.f;
+            uint32_t* ids = L.x.ids;
             r.unpack(f, nn, 1);
             r.unpack(ids, nn, 6);
             // node k (creation order) is decoder nn-1-k
@@ -213,7 +230,7 @@ OZL_HDN void ozl_dparse_frame(const ozl_fview& fr, size_t flen, ozl_dlds& L, ozl
                 nodes[k].hdrLen = 0;
                 nodes[k].nbVO = 0;
             }
-            uint32_t hf[8];
+            uint32_t* hf = L.x.hf;
             r.unpack(hf, nn, 1);
             for (int k = nn - 1; k >= 0; k--)
                 if (hf[nn - 1 - k]) nodes[k].hdrLen = (uint8_t)(r.varint() + 1);
@@ -237,15 +254,16 @@ OZL_HDN void ozl_dparse_frame(const ozl_fview& fr, size_t flen, ozl_dlds& L, ozl
     }
     // The stored sizes come after the VO/input flags and distances; parse
     // the rest of the header generically first.
-    uint32_t storedSize[4] = { 0, 0, 0, 0 };
+    uint32_t* storedSize = L.x.storedSize;
+    for (int k = 0; k < 4; k++) storedSize[k] = 0;
     if (ok) {
-        uint32_t vf[8];
+        uint32_t* vf = L.x.vf;
         r.unpack(vf, nn, 1);
         for (int k = 0; k < nn; k++) ok &= vf[k] == 0;
-        uint32_t inf[8];
+        uint32_t* inf = L.x.inf;
         r.unpack(inf, nn, 1);
         for (int k = 0; k < nn; k++) ok &= inf[k] == 0;
-        uint32_t dist[8];
+        uint32_t* dist = L.x.dist;
         r.unpack(dist, nn, ozl_next_pow2((uint64_t)nn + (uint64_t)nbStored));
         for (int k = 0; k < nbStored; k++) storedSize[k] = (uint32_t)r.varint();
         ok &= !r.bad;
@@ -360,8 +378,9 @@ OZL_HDN void ozl_dparse_frame(const ozl_fview& fr, size_t flen, ozl_dlds& L, ozl
             else p.wMaxSym = (uint8_t)ms;
         }
     }
-    int order[12], nbSt2 = 0;
-    ozl_fnode en[8];
+    int* order = L.x.order;
+    int nbSt2 = 0;
+    ozl_fnode* en = L.x.en;
     if (ok) {
         int enn = ozl_entropy_layout(&p, en, st, &ns);
         ok = enn == nn;
@@ -421,7 +440,7 @@ OZL_HDN void ozl_dparse_frame(const ozl_fview& fr, size_t flen, ozl_dlds& L, ozl
             if (d.nbStates > 1 || (d.nbStates == 1 && d.n < 12)) { d.status = OZL_ERR_CORRUPTION; break; }
             uint32_t m = p.maxSym + 1u;
             if (ozl_decode_weights(fr, nodes, nn, st, soff, m, L)) { d.status = OZL_ERR_CORRUPTION; break; }
-            int tl = ozl_huf_from_weights(L.wlen, L.wval, L.w, m);
+            int tl = ozl_huf_from_weights(L.wlen, L.wval, L.w, m, L.x.rk);
             if (tl < 1 || tl > 11) { d.status = OZL_ERR_CORRUPTION; break; }
             d.tl = (uint8_t)tl;
             break;

@@ -70,6 +70,8 @@ struct PlanLds {
     uint8_t hlen[256];
     uint16_t hval[256];
     uint8_t aux[OZL_AUX_BYTES];
+    uint16_t cumul[257];  // nested table build workspace
+    uint8_t fmap[256];
     ozl_eplan plan;
 };
 
@@ -95,7 +97,7 @@ __device__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux)
     case OZL_BK_CONSTANT: pay[0] = L.wts[0]; p.wPayload = 1; break;
     case OZL_BK_FLATPACK: {
         size_t ps;
-        p.wAux = (uint16_t)ozl_flatpack_encode_seq(side, pay, &ps, L.wts, m, L.wcount);
+        p.wAux = (uint16_t)ozl_flatpack_encode_seq(side, pay, &ps, L.wts, m, L.wcount, L.fmap);
         p.wPayload = (uint16_t)ps;
         break;
     }
@@ -112,7 +114,7 @@ __device__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux)
         // at most 256 weights of value <= 12: tableLog <= 5 (fse_compress.c:379-396)
         if (tl > OZL_WEIGHTS_MAX_TABLELOG) { p.status = OZL_ERR_GENERIC; return; }
         if (ozl_fse_normalize(wn, tl, L.wcount, m, wms) != 0) { p.status = OZL_ERR_GENERIC; return; }
-        if (ozl_fse_build_ctable(L.wst, L.wtt, L.cells, wn, wms, tl)) { p.status = OZL_ERR_GENERIC; return; }
+        if (ozl_fse_build_ctable(L.wst, L.wtt, L.cells, wn, wms, tl, L.cumul)) { p.status = OZL_ERR_GENERIC; return; }
         size_t bits = ozl_fse_encode_seq(pay, OZL_AUX_BYTES - OZL_AUX_W_PAYLOAD, L.wts, m, L.wst, L.wtt, tl, 2);
         if (bits == 0) { p.status = OZL_ERR_GENERIC; return; }
         p.wPayload = (uint16_t)bits;

@@ -792,6 +792,7 @@ __global__ void __launch_bounds__(64) k_prep_items(const ozl_ditem* __restrict__
     __shared__ uint8_t w[256];
     __shared__ uint8_t len[256];
     __shared__ uint16_t val[256];
+    __shared__ uint16_t rk[28];
     __shared__ int sBk;
     uint32_t b = blockIdx.x;
     if (b >= B) return;
@@ -828,7 +829,7 @@ __global__ void __launch_bounds__(64) k_prep_items(const ozl_ditem* __restrict__
             }
         } else {
             for (uint32_t s = 0; s < it.sideN && s < 256; s++) w[s] = side[s];
-            int tl = it.sideN <= 256 ? ozl_huf_from_weights(len, val, w, it.sideN) : -1;
+            int tl = it.sideN <= 256 ? ozl_huf_from_weights(len, val, w, it.sideN, rk) : -1;
             if (tl < 1 || tl > 11 || it.nbStates > 1 || (it.nbStates == 1 && it.n < 12)) d.status = OZL_ERR_CORRUPTION;
             else d.tl = (uint8_t)tl;
         }

@@ -25,7 +25,8 @@ unsigned cu_fse_optimal_tablelog(unsigned maxTL, size_t n, unsigned maxSym, unsi
 int cu_fse_build_ctable(uint16_t* st, uint32_t* tt, const int16_t* norm, unsigned maxSym, unsigned tl)
 {
     uint8_t cells[4096];
-    return ozl_fse_build_ctable(st, tt, cells, norm, maxSym, tl);
+    uint16_t cumul[257];
+    return ozl_fse_build_ctable(st, tt, cells, norm, maxSym, tl, cumul);
 }
 size_t cu_fse_encode(uint8_t* dst, size_t cap, const uint8_t* src, size_t n, const uint16_t* st, const uint32_t* tt,
                      unsigned tl, unsigned nbStates)
@@ -37,7 +38,8 @@ int cu_fse_decode(uint8_t* dst, size_t n, const uint8_t* src, size_t size, const
 {
     static uint32_t dt[4096];
     uint8_t cells[4096];
-    if (ozl_fse_build_dtable(dt, cells, norm, maxSym, tl)) return -2;
+    uint16_t next[256];
+    if (ozl_fse_build_dtable(dt, cells, norm, maxSym, tl, next)) return -2;
     return ozl_fse_decode_seq(dst, n, src, size, dt, tl, nbStates);
 }
 size_t cu_fse_write_ncount(uint8_t* dst, const int16_t* norm, unsigned maxSym, unsigned tl)
@@ -55,13 +57,15 @@ int cu_huf_build(uint8_t* nb, uint16_t* val, const uint32_t* count, unsigned max
 }
 int cu_huf_from_weights(uint8_t* len, uint16_t* val, const uint8_t* w, unsigned nbW)
 {
-    return ozl_huf_from_weights(len, val, w, nbW);
+    uint16_t rk[28];
+    return ozl_huf_from_weights(len, val, w, nbW, rk);
 }
 size_t cu_flatpack(uint8_t* alphabet, uint8_t* packed, size_t* packedSize, const uint8_t* src, size_t n)
 {
     uint32_t count[256] = { 0 };
     for (size_t i = 0; i < n; i++) count[src[i]]++;
-    return ozl_flatpack_encode_seq(alphabet, packed, packedSize, src, n, count);
+    uint8_t map[256];
+    return ozl_flatpack_encode_seq(alphabet, packed, packedSize, src, n, count, map);
 }
 
 // Frame recognition of the batched entropy decoder (ozl_dparse.h): returns
