@@ -65,12 +65,12 @@ struct PlanLds {
     uint32_t wcount[256];
     uint8_t cells[1 << OZL_WEIGHTS_MAX_TABLELOG];   // nested weights graph
     uint16_t wst[1 << OZL_WEIGHTS_MAX_TABLELOG];
-    uint32_t wtt[256];
-    int16_t wn[256];
+    uint32_t wtt[16];       // weights are <= 12: 13 symbols at most
+    int16_t wn[16];
     uint8_t hlen[256];
     uint16_t hval[256];
     uint8_t aux[OZL_AUX_BYTES];
-    uint16_t cumul[257];  // nested table build workspace
+    uint16_t cumul[18];   // nested table build workspace (13 weight symbols)
     uint8_t fmap[256];
     ozl_eplan plan;
 };
@@ -112,7 +112,7 @@ __device__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux)
         p.wTl = (uint8_t)tl;
         int16_t* wn = L.wn;
         // at most 256 weights of value <= 12: tableLog <= 5 (fse_compress.c:379-396)
-        if (tl > OZL_WEIGHTS_MAX_TABLELOG) { p.status = OZL_ERR_GENERIC; return; }
+        if (tl > OZL_WEIGHTS_MAX_TABLELOG || wms > 12) { p.status = OZL_ERR_GENERIC; return; }
         if (ozl_fse_normalize(wn, tl, L.wcount, m, wms) != 0) { p.status = OZL_ERR_GENERIC; return; }
         if (ozl_fse_build_ctable(L.wst, L.wtt, L.cells, wn, wms, tl, L.cumul)) { p.status = OZL_ERR_GENERIC; return; }
         size_t bits = ozl_fse_encode_seq(pay, OZL_AUX_BYTES - OZL_AUX_W_PAYLOAD, L.wts, m, L.wst, L.wtt, tl, 2);
