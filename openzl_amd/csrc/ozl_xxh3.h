@@ -4,9 +4,9 @@
 // 1537-1580, encode_frameheader.c:624-630). Written from the published XXH3
 // algorithm; the 192-byte default secret is the specification's table.
 //   ozl_xxh3_seq   one lane, any length (headers, small streams)
-//   ozl_xxh3_wave  one 64-lane wavefront: the sums over the 16 stripes of every
-//                  1 KiB block are independent, so lanes compute them in
-//                  parallel; only the per-block scramble chain is sequential.
+//   ozl_xxh3_wave  one 64-lane wavefront: each 1 KiB block is one coalesced
+//                  wave load; its 16 stripe sums are reduced across lanes and
+//                  only the per-block scramble chain is sequential.
 #pragma once
 #include "ozl_core.h"
 
@@ -162,35 +162,67 @@ OZL_HD uint64_t ozl_xxh3_seq(const uint8_t* in, size_t len)
 
 #if defined(__HIPCC__)
 // One wavefront hashes `len` bytes at `in`; every lane returns the hash.
-// Lane l handles blocks b = l, l+64, ...: it sums the 16 stripe terms of each
-// of its blocks for all 8 accumulators; then the 8 lanes < 8 run the scramble
-// chain over the blocks in order, reading the block sums from `lds`
-// (capacity: 8 * min(nbBlocks, chainCap) u64), in rounds of chainCap blocks.
+// Each 1 KiB block is one coalesced wave load: lane l takes bytes
+// [16l, 16l+16), i.e. words 2j, 2j+1 (j = l mod 4) of stripe l/4, and forms
+// their terms for accumulators 2j and 2j+1; the 16 stripes are summed across
+// the lanes with equal j (DPP row rotations, then two xor shuffles), and
+// every lane keeps the accumulator pair of its j through the per-block
+// scramble chain. The tail stripes run on lanes 0..7 as in ozl_xxh3_seq.
+// `lds` and `chainCap` are unused (kept for the callers' signature).
+template <int CTRL>
+__device__ inline uint64_t ozl_dpp64(uint64_t v)
+{
+    uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xF, 0xF, false);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, false);
+    return ((uint64_t)hi << 32) | lo;
+}
 __device__ inline uint64_t ozl_xxh3_wave(const uint8_t* in, size_t len, uint64_t* lds, int chainCap)
 {
+    (void)lds;
+    (void)chainCap;
     const int lane = threadIdx.x & 63;
     if (len <= 240) return ozl_xxh3_short(in, len);
-    uint64_t acc = 0;  // accumulator lane (lane < 8)
-    const uint64_t init[8] = { OZL_P32_3, OZL_P64_1, OZL_P64_2, OZL_P64_3, OZL_P64_4, OZL_P32_2, OZL_P64_5, OZL_P32_1 };
-    if (lane < 8) acc = init[lane];
-    size_t nbBlocks = (len - 1) / 1024;
-    for (size_t b0 = 0; b0 < nbBlocks; b0 += chainCap) {
-        size_t nb = nbBlocks - b0 < (size_t)chainCap ? nbBlocks - b0 : (size_t)chainCap;
-        for (size_t bb = lane; bb < nb; bb += 64) {
-            const uint8_t* blk = in + (b0 + bb) * 1024;
-            uint64_t sum[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
-            for (int s = 0; s < 16; s++)
-                for (int i = 0; i < 8; i++) sum[i] += ozl_stripe_term(blk + s * 64, s * 8, i);
-            for (int i = 0; i < 8; i++) lds[bb * 8 + i] = sum[i];
+    const int j = lane & 3, st = lane >> 2, i0 = 2 * j;
+    const uint64_t k0 = ozl_sec64(8 * st + 8 * i0), k1 = ozl_sec64(8 * st + 8 * i0 + 8);
+    const uint64_t c0 = ozl_sec64(192 - 64 + 8 * i0), c1 = ozl_sec64(192 - 64 + 8 * i0 + 8);
+    uint64_t A = j == 0 ? OZL_P32_3 : j == 1 ? OZL_P64_2 : j == 2 ? OZL_P64_4 : OZL_P64_5;
+    uint64_t B = j == 0 ? OZL_P64_1 : j == 1 ? OZL_P64_3 : j == 2 ? OZL_P32_2 : OZL_P32_1;
+    const size_t nbBlocks = (len - 1) / 1024;
+    const bool al = ((uintptr_t)in & 15) == 0;
+    for (size_t b = 0; b < nbBlocks; b++) {
+        const uint8_t* p = in + b * 1024 + 16 * lane;
+        uint32_t d0, d1, d2, d3;
+        if (al) {
+            const uint4 q = *reinterpret_cast<const uint4*>(p);
+            d0 = q.x; d1 = q.y; d2 = q.z; d3 = q.w;
+        } else {
+            const uint32_t* pw = reinterpret_cast<const uint32_t*>(p);
+            d0 = pw[0]; d1 = pw[1]; d2 = pw[2]; d3 = pw[3];
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (lane < 8)
-            for (size_t bb = 0; bb < nb; bb++) acc = ozl_scramble(acc + lds[bb * 8 + lane], lane);
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const uint64_t v0 = d0 | ((uint64_t)d1 << 32), v1 = d2 | ((uint64_t)d3 << 32);
+        const uint64_t x0 = v0 ^ k0, x1 = v1 ^ k1;
+        uint64_t sa = v1 + (uint64_t)(uint32_t)x0 * (x0 >> 32);
+        uint64_t sb = v0 + (uint64_t)(uint32_t)x1 * (x1 >> 32);
+        sa += ozl_dpp64<0x124>(sa);  // row_ror:4
+        sb += ozl_dpp64<0x124>(sb);
+        sa += ozl_dpp64<0x128>(sa);  // row_ror:8
+        sb += ozl_dpp64<0x128>(sb);
+        sa += __shfl_xor(sa, 16, 64);
+        sb += __shfl_xor(sb, 16, 64);
+        sa += __shfl_xor(sa, 32, 64);
+        sb += __shfl_xor(sb, 32, 64);
+        A += sa;
+        B += sb;
+        A ^= A >> 47; A ^= c0; A *= OZL_P32_1;
+        B ^= B >> 47; B ^= c1; B *= OZL_P32_1;
     }
-    // tail stripes and the last stripe: lanes < 8 each finish their accumulator
+    // accumulator i to lane i, then the tail stripes and the last stripe
+    uint64_t acc = 0;
+    for (int q = 0; q < 4; q++) {
+        const uint64_t a = __shfl(A, q, 64), bq = __shfl(B, q, 64);
+        if (lane == 2 * q) acc = a;
+        if (lane == 2 * q + 1) acc = bq;
+    }
     if (lane < 8) {
         size_t nbStripes = ((len - 1) - 1024 * nbBlocks) / 64;
         for (size_t s = 0; s < nbStripes; s++) acc += ozl_stripe_term(in + nbBlocks * 1024 + s * 64, (int)s * 8, lane);
