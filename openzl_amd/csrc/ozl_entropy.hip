@@ -28,16 +28,21 @@ namespace {
 
 constexpr int kT = 256;
 
-__device__ inline void block_hist(const uint8_t* src, uint32_t n, uint32_t* h4 /*4*256*/, uint32_t* count)
+// 16 copies of the histogram, bin-major (copy c of bin s at 16s + c), so that
+// the lanes of a wave that hit the same frequent symbol spread over 16 banks
+// and 16 addresses instead of serializing on one.
+constexpr int kHistCopies = 16;
+
+__device__ inline void block_hist(const uint8_t* src, uint32_t n, uint32_t* hc /*16*256*/, uint32_t* count)
 {
-    int tid = threadIdx.x, wave = tid >> 6;
-    for (int i = tid; i < 4 * 256; i += kT) h4[i] = 0;
+    int tid = threadIdx.x;
+    for (int i = tid; i < kHistCopies * 256; i += kT) hc[i] = 0;
     __syncthreads();
-    uint32_t* h = h4 + wave * 256;
+    uint32_t* h = hc + (tid & (kHistCopies - 1));
     // aligned 16-byte body, byte head and tail
     uint32_t head = (uint32_t)((16 - ((uintptr_t)src & 15)) & 15);
     if (head > n) head = n;
-    for (uint32_t i = tid; i < head; i += kT) atomicAdd(&h[src[i]], 1u);
+    for (uint32_t i = tid; i < head; i += kT) atomicAdd(&h[kHistCopies * src[i]], 1u);
     uint32_t nv = (n - head) / 16;
     const uint4* v = reinterpret_cast<const uint4*>(src + head);
     for (uint32_t i = tid; i < nv; i += kT) {
@@ -46,11 +51,17 @@ __device__ inline void block_hist(const uint8_t* src, uint32_t n, uint32_t* h4 /
 #pragma unroll
         for (int k = 0; k < 4; k++)
 #pragma unroll
-            for (int b = 0; b < 4; b++) atomicAdd(&h[(w[k] >> (8 * b)) & 255], 1u);
+            for (int b = 0; b < 4; b++) atomicAdd(&h[kHistCopies * ((w[k] >> (8 * b)) & 255)], 1u);
     }
-    for (uint32_t i = head + nv * 16 + tid; i < n; i += kT) atomicAdd(&h[src[i]], 1u);
+    for (uint32_t i = head + nv * 16 + tid; i < n; i += kT) atomicAdd(&h[kHistCopies * src[i]], 1u);
     __syncthreads();
-    for (int s = tid; s < 256; s += kT) count[s] = h4[s] + h4[256 + s] + h4[512 + s] + h4[768 + s];
+    for (int s = tid; s < 256; s += kT) {
+        const uint4* r = reinterpret_cast<const uint4*>(hc + kHistCopies * s);
+        uint32_t t = 0;
+#pragma unroll
+        for (int c = 0; c < kHistCopies / 4; c++) t += r[c].x + r[c].y + r[c].z + r[c].w;
+        count[s] = t;
+    }
     __syncthreads();
 }
 
@@ -131,15 +142,15 @@ __device__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux)
 }
 
 // Histogram of every block (one 256-thread workgroup per block, LDS
-// atomics on 4 sub-histograms), parked in the block's table area.
+// atomics on 16 interleaved copies), parked in the block's table area.
 __global__ void __launch_bounds__(kT) k_hist(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off, uint32_t B,
                                              ozl_ews ws)
 {
-    __shared__ uint32_t h4[4 * 256];
+    __shared__ __attribute__((aligned(16))) uint32_t hc[kHistCopies * 256];
     __shared__ uint32_t count[256];
     uint32_t b = blockIdx.x;
     if (b >= B) return;
-    block_hist(src + off[b], (uint32_t)(off[b + 1] - off[b]), h4, count);
+    block_hist(src + off[b], (uint32_t)(off[b + 1] - off[b]), hc, count);
     uint32_t* cg = reinterpret_cast<uint32_t*>(ws.tables + (size_t)b * OZL_TAB_BYTES + OZL_TAB_COUNT_OFF);
     for (int s = threadIdx.x; s < 256; s += kT) cg[s] = count[s];
 }
