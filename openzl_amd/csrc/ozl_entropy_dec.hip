@@ -674,27 +674,49 @@ __global__ void __launch_bounds__(64) k_huf_decode(uint32_t B, ozl_dws ws)
         win_refill32(w, d);
         return e & 255u;
     };
-    // 16 rounds per staging step; their 16 bytes are stored at the next step
-    // between its ring stores and its loads, so that waiting for a step's
-    // loads never waits for output stores (both count in vmcnt, which drains
-    // in order)
-    uint32_t o[4] = { 0, 0, 0, 0 };
-    for (uint32_t r = 0; r < R16; r += 16) {
-        lstage_put<kHRing>(ring, g);
-        if (r) put4_dwords(outA + r - 16, 4, o[0], o[1], o[2], o[3], wide);
-        lstage_load<kHLook>(g, (uint64_t)((int64_t)lo + w.lb) - 4);
-        wave_sync();
+    // 16 rounds per staging step, 4 steps per iteration; the iteration's 64
+    // bytes are stored at the next iteration's first step, between its ring
+    // stores and its loads, so that waiting for a step's loads never waits
+    // for output stores (both count in vmcnt, which drains in order), and
+    // whole 64-byte pieces reach L2 (16-byte pieces of 4 streams per frame
+    // left partly written lines that were written back several times)
+    const uint32_t R64 = R & ~63u;
+    uint4 ob[4] = { make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0) };
+    auto put64 = [&](uint64_t a) {
+        if ((a & 15) == 0) {
+            typedef __attribute__((address_space(1))) u32x4 g_u32x4;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            uint32_t x = sym();
-            x |= sym() << 8;
-            x |= sym() << 16;
-            x |= sym() << 24;
-            o[j] = x;
+            for (int k = 0; k < 4; k++) {
+                u32x4 v;
+                v.x = ob[k].x; v.y = ob[k].y; v.z = ob[k].z; v.w = ob[k].w;
+                *(g_u32x4*)(uintptr_t)(a + 16 * k) = v;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) put4_dwords(a + 16 * k, 4, ob[k].x, ob[k].y, ob[k].z, ob[k].w, wide);
+        }
+    };
+    for (uint32_t r = 0; r < R64; r += 64) {
+#pragma unroll
+        for (int q4 = 0; q4 < 4; q4++) {
+            lstage_put<kHRing>(ring, g);
+            if (q4 == 0 && r) put64(outA + r - 64);
+            lstage_load<kHLook>(g, (uint64_t)((int64_t)lo + w.lb) - 4);
+            wave_sync();
+            uint32_t x[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                uint32_t v = sym();
+                v |= sym() << 8;
+                v |= sym() << 16;
+                v |= sym() << 24;
+                x[j] = v;
+            }
+            ob[q4] = make_uint4(x[0], x[1], x[2], x[3]);
         }
     }
-    if (R16) put4_dwords(outA + R16 - 16, 4, o[0], o[1], o[2], o[3], wide);
-    for (uint32_t r = R16; r < R; r++) {
+    if (R64) put64(outA + R64 - 64);
+    for (uint32_t r = R64; r < R; r++) {
         if ((r & (kHPhase - 1)) == 0) {
             lstage_step<kHRing, kHLook>(ring, g, (uint64_t)((int64_t)lo + w.lb) - 4);
             wave_sync();
