@@ -45,6 +45,17 @@ def lib():
         "ozl_entropy_compress_batch": (I, [I, P, P, ctypes.c_uint32, SZ, P, SZ, P, P, P, SZ, P]),
         "ozl_entropy_decompress_workspace": (SZ, [ctypes.c_uint32]),
         "ozl_entropy_decompress_batch": (I, [P, P, ctypes.c_uint32, P, P, P, P, SZ, P]),
+        "ozl_columns_workspace": (SZ, [ctypes.c_uint32, SZ, ctypes.c_uint32]),
+        "ozl_columns_frame_bound": (SZ, [SZ]),
+        "ozl_columns_compress_batch": (I, [P, P, P, ctypes.c_uint32, P, SZ, P, P, P, SZ, ctypes.c_uint32,
+                                           ctypes.c_uint32, P]),
+        "ozl_columns_decompress_batch": (I, [P, P, ctypes.c_uint32, P, P, P, P, SZ, ctypes.c_uint32,
+                                             ctypes.c_uint32, P]),
+        "ozl_columns_autotune": (I, [P, P, P, ctypes.c_uint32, ctypes.c_uint64, P, SZ, P, P, P, SZ, P, P, P]),
+        "ozl_columns_graph_create": (P, [I, P, P, P, ctypes.c_uint32, P, SZ, P, P, P, SZ, ctypes.c_uint32,
+                                         ctypes.c_uint32]),
+        "ozl_graph_launch": (I, [P, P]),
+        "ozl_graph_destroy": (None, [P]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)

@@ -1,0 +1,909 @@
+// Batched numeric columns on gfx950: ZL_NODE_DELTA_INT -> ZL_NODE_ZIGZAG ->
+// ZL_GRAPH_BITPACK, one frame per column, thousands of columns of uneven
+// length per call.
+//
+// The graph per column follows the engine path (and the reference):
+// delta_int (encode_delta_binding.c:11-45, header = first element),
+// zigzag (encode_zigzag_binding.c:8-40), then the bitpack selector to
+// bitpack_int (encode_bitpack_binding.c:92-147: nbBits = 1 + highbit(max),
+// header byte0 = log2(width) << 6 | (nbBits - 1), byte1 = padding count when
+// the packed bytes could hold more elements) and a serial store. The frame
+// holds the three nodes, one stored stream, the content checksum (XXH3 of the
+// column) and the compressed checksum (XXH3 of the chunk and the content
+// checksum), as the engine writes it (ozl_frame.h).
+//
+// Work is cut into items of `tile` deltas (a multiple of 32, capped at 32 KiB
+// of input per item); each pass over items is one persistent launch whose
+// workgroups take items from a device counter until none is left:
+//   encode  k_col_setup -> k_col_scan_items -> k_col_max (persistent: max of
+//           the zigzag deltas per column) -> k_col_plan (nbBits, headers,
+//           content hash) -> k_col_scan_frames -> k_col_pack (persistent:
+//           bits straight into the frames) -> k_col_finish (checksums)
+//   decode  k_cold_parse -> k_cold_scan_items -> k_cold_sum (persistent: tile
+//           sums of the deltas) -> k_cold_carry (per-column scan of the tile
+//           sums) -> k_cold_apply (persistent: unpack, prefix sum, write) ->
+//           k_cold_verify (checksums)
+// Every launch clears what it needs with kernels, never with memsets, so a
+// call can be captured in a hipGraph and replayed.
+#include "ozl_internal.h"
+#include "ozl_core.h"
+#include "ozl_xxh3.h"
+#include "ozl_frame.h"
+#include "ozl_profile.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+namespace {
+
+constexpr int kCT = 256;          // threads per workgroup
+constexpr int kHdrSlot = 96;      // per-column header bytes (frame + chunk header)
+constexpr uint32_t kTileBytesMax = 32768;
+
+struct ColPlan {                  // 64 bytes, one per column
+    uint64_t src;                 // column address (encode: input, decode: packed stream)
+    uint64_t m;                   // deltas (elements - 1)
+    uint64_t packed;              // packed stream bytes
+    unsigned long long maxZ;      // encode: largest zigzag delta; decode: first element
+    uint32_t itemBase;            // first work item of the column
+    uint32_t nItems;
+    uint32_t tile;                // deltas per item
+    uint32_t status;
+    uint32_t hashes[2];           // content, compressed checksum (decode: stored values)
+    uint8_t w, nb, fhLen, hdrLen; // width, bits, frame header bytes, frame + chunk header bytes
+    uint8_t kind;                 // layout, see below
+    uint8_t pad[3];
+};
+// Layouts (cctx.c:1127-1140: a successor graph's input below minStreamSize, 10
+// bytes, is stored after a num->serial conversion, header [log2 width]):
+//   kBitpack  n*w >= 10 and (n-1)*w >= 10: delta, zigzag, bitpack_int
+//   kTinyIn   n*w < 10: the input itself is converted and stored
+//   kTinyDz   (n-1)*w < 10 <= n*w: delta, then the deltas stored (the zigzag
+//             node starts the successor graph, so it is not run either)
+enum { kBitpack = 0, kTinyIn = 1, kTinyDz = 2 };
+constexpr uint64_t kMinStream = 10;
+static_assert(sizeof(ColPlan) == 64, "ColPlan layout");
+
+struct ColCtr {
+    uint32_t queue[4];
+    uint32_t totalItems;
+    uint32_t error;
+    uint32_t pad[10];
+};
+
+struct ColWs {
+    ColPlan* plans;
+    uint8_t* hdr;                 // kHdrSlot bytes per column
+    uint32_t* itemCol;            // column of every item; the decoder's tile sums follow
+    ColCtr* ctr;
+};
+
+// ws layout: [ctr 256 B][plans K*64][hdr K*96][itemCol items*4][tileSum items*8]
+// (tileSum is placed by k_col_scan_items once the item count is known)
+static ColWs carve(void* w, uint32_t K)
+{
+    uint8_t* p = (uint8_t*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
+    ColWs ws;
+    ws.ctr = (ColCtr*)p; p += 256;
+    ws.plans = (ColPlan*)p; p += (size_t)K * sizeof(ColPlan);
+    ws.hdr = p; p += ((size_t)K * kHdrSlot + 255) & ~(size_t)255;
+    ws.itemCol = (uint32_t*)p;
+    return ws;
+}
+
+__device__ inline uint64_t ld_elem(const uint8_t* p, int w)
+{
+    switch (w) {
+    case 1: return *p;
+    case 2: return *reinterpret_cast<const uint16_t*>(p);
+    case 4: return *reinterpret_cast<const uint32_t*>(p);
+    default: return *reinterpret_cast<const uint64_t*>(p);
+    }
+}
+
+__device__ inline uint64_t wmask(int w) { return w == 8 ? ~0ull : (1ull << (8 * w)) - 1; }
+
+// zigzag of the width-w difference b - a (encode_zigzag_kernel.c:8-59)
+__device__ inline uint64_t zz_delta(uint64_t a, uint64_t b, int w)
+{
+    const uint64_t d = (b - a) & wmask(w);
+    const uint64_t sign = (d >> (8 * w - 1)) & 1;
+    return ((d << 1) ^ (0 - sign)) & wmask(w);
+}
+
+__device__ inline uint64_t unzz(uint64_t z, int w) { return ((z >> 1) ^ (0 - (z & 1))) & wmask(w); }
+
+// workgroup-wide item fetch from a device counter; returns the item or ~0u
+__device__ inline uint32_t next_item(uint32_t* q, uint32_t total, uint32_t* sItem)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) *sItem = atomicAdd(q, 1u);
+    __syncthreads();
+    const uint32_t it = *sItem;
+    return it < total ? it : ~0u;
+}
+
+// tile sums follow the item-to-column map (8-byte aligned)
+__device__ inline uint64_t* tile_sums(const ColWs& ws, uint32_t total)
+{
+    return reinterpret_cast<uint64_t*>(((uintptr_t)(ws.itemCol + total) + 7) & ~(uintptr_t)7);
+}
+
+template <typename T>
+__device__ T block_sum(T v, T* lds)
+{
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    const int wave = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[wave] = v;
+    __syncthreads();
+    T t = 0;
+    for (int k = 0; k < kCT / 64; k++) t += lds[k];
+    return t;
+}
+
+// exclusive scan of the workgroup's values (returns the exclusive prefix)
+template <typename T>
+__device__ T block_excl(T v, T* lds, T& total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const T y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    __syncthreads();
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    T off = 0, all = 0;
+    for (int k = 0; k < kCT / 64; k++) {
+        const T s = lds[k];
+        if (k < wave) off += s;
+        all += s;
+    }
+    total = all;
+    return off + x - v;
+}
+
+// ====================================================================== encode
+__global__ void __launch_bounds__(kCT) k_col_setup(const uint8_t* __restrict__ src, const uint64_t* __restrict__ colOff,
+                                                  const uint8_t* __restrict__ widths, uint32_t K, uint32_t tile, ColWs ws)
+{
+    const uint32_t c = blockIdx.x * kCT + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x < 16) reinterpret_cast<uint32_t*>(ws.ctr)[threadIdx.x] = 0;
+    if (c >= K) return;
+    ColPlan p = {};
+    const uint64_t a = colOff[2 * c], b = a + colOff[2 * c + 1];
+    const int w = widths[c];
+    p.w = (uint8_t)w;
+    p.src = (uint64_t)(uintptr_t)(src + a);
+    const bool wok = w == 1 || w == 2 || w == 4 || w == 8;
+    if (!wok || b <= a || (b - a) % (uint64_t)(wok ? w : 1) || ((p.src) & (uint64_t)(wok ? w - 1 : 0))) {
+        p.status = OZL_ERR_NODE_INVALID_INPUT;
+    } else {
+        const uint64_t n = (b - a) / (uint64_t)w;
+        p.m = n - 1;
+        if (n * (uint64_t)w < kMinStream) p.kind = kTinyIn;
+        else if ((n - 1) * (uint64_t)w < kMinStream) p.kind = kTinyDz;
+        else {
+            uint32_t t = tile < kTileBytesMax / (uint32_t)w ? tile : kTileBytesMax / (uint32_t)w;
+            p.tile = t;
+            p.nItems = (uint32_t)((p.m + t - 1) / t);
+        }
+    }
+    ws.plans[c] = p;
+}
+
+// item bases: exclusive scan of nItems (one workgroup). `sumsEnd` is the end
+// address of the workspace, with bit 63 set when the items need no tile sums
+// (encode); when the item map (and the sums) would pass it, no item runs and
+// every column reports OZL_ERR_PARAMETER_INVALID.
+__global__ void __launch_bounds__(kCT) k_col_scan_items(uint32_t K, ColWs ws, uint64_t sumsEnd)
+{
+    __shared__ uint32_t lds[kCT / 64];
+    const uint32_t per = (K + kCT - 1) / kCT, b0 = threadIdx.x * per, b1 = b0 + per < K ? b0 + per : K;
+    uint32_t s = 0;
+    for (uint32_t i = b0; i < b1; i++) s += ws.plans[i].nItems;
+    uint32_t total;
+    uint32_t off = block_excl<uint32_t>(s, lds, total);
+    for (uint32_t i = b0; i < b1; i++) {
+        ws.plans[i].itemBase = off;
+        off += ws.plans[i].nItems;
+    }
+    if (threadIdx.x == 0) {
+        const bool fits = (uint64_t)(uintptr_t)(sumsEnd >> 63 ? ws.itemCol + total : (uint32_t*)(tile_sums(ws, total) + total)) <=
+                          (sumsEnd & ~(1ull << 63));
+        ws.ctr->totalItems = fits ? total : 0u;
+        ws.ctr->error = fits ? 0u : 1u;
+    }
+}
+
+// item-to-column map: one workgroup per column writes its items
+__global__ void __launch_bounds__(64) k_col_map(uint32_t K, ColWs ws)
+{
+    const uint32_t c = blockIdx.x;
+    if (c >= K || ws.ctr->error) return;
+    const ColPlan& p = ws.plans[c];
+    for (uint32_t i = threadIdx.x; i < p.nItems; i += 64) ws.itemCol[p.itemBase + i] = c;
+}
+
+// largest zigzag delta per column; items stream their deltas with coalesced
+// element loads (the second load of each pair hits L1)
+__global__ void __launch_bounds__(kCT) k_col_max(uint32_t K, ColWs ws)
+{
+    __shared__ uint32_t sItem, sCol;
+    const uint32_t total = ws.ctr->totalItems;
+    for (;;) {
+        const uint32_t it = next_item(&ws.ctr->queue[0], total, &sItem);
+        if (it == ~0u) break;
+        if (threadIdx.x == 0) sCol = ws.itemCol[it];
+        __syncthreads();
+        const ColPlan& P = ws.plans[sCol];
+        const int w = P.w;
+        const uint64_t d0 = (uint64_t)(it - P.itemBase) * P.tile, d1 = d0 + P.tile < P.m ? d0 + P.tile : P.m;
+        const uint8_t* x = reinterpret_cast<const uint8_t*>((uintptr_t)P.src);
+        unsigned long long mx = 0;
+        for (uint64_t d = d0 + threadIdx.x; d < d1; d += kCT) {
+            const uint64_t z = zz_delta(ld_elem(x + d * w, w), ld_elem(x + (d + 1) * w, w), w);
+            mx = z > mx ? z : mx;
+        }
+        for (int s = 32; s >= 1; s >>= 1) {
+            const unsigned long long o = __shfl_xor(mx, s, 64);
+            mx = o > mx ? o : mx;
+        }
+        if ((threadIdx.x & 63) == 0 && mx) atomicMax(&ws.plans[sCol].maxZ, mx);
+    }
+}
+
+// the chunk of each layout (ozl_frame.h); `stored` = bytes of the stored
+// stream, `first` = the delta header value. Returns the chunk header bytes.
+__device__ inline size_t col_chunk_header(uint8_t* out, int kind, int w, int nb, uint64_t m, uint64_t stored,
+                                          uint64_t first)
+{
+    const int lg = w == 1 ? 0 : w == 2 ? 1 : w == 4 ? 2 : 3;
+    ozl_fnode nodes[3];
+    ozl_fstream st[4];
+    for (int k = 0; k < 4; k++) st[k] = ozl_fstream{ 0, 0, OZL_SRC_NONE, 0 };
+    int nn;
+    if (kind == kTinyIn) {
+        nodes[0] = ozl_fnode{ OZL_TR_CONV_NUM_TO_SERIAL_LE, 1, 0, 0, 1, {} };
+        nodes[0].hdr[0] = (uint8_t)lg;
+        nn = 1;
+    } else {
+        nodes[0] = ozl_fnode{ OZL_TR_DELTA_INT, (uint8_t)w, 0, 0, 1, {} };
+        for (int k = 0; k < w; k++) nodes[0].hdr[k] = (uint8_t)(first >> (8 * k));
+        if (kind == kTinyDz) {
+            nodes[1] = ozl_fnode{ OZL_TR_CONV_NUM_TO_SERIAL_LE, 1, 0, 1, 2, {} };
+            nodes[1].hdr[0] = (uint8_t)lg;
+            nn = 2;
+        } else {
+            nodes[1] = ozl_fnode{ OZL_TR_ZIGZAG, 0, 0, 1, 2, {} };
+            nodes[2] = ozl_fnode{ OZL_TR_BITPACK_INT, 1, 0, 2, 3, {} };
+            nodes[2].hdr[0] = (uint8_t)((lg << 6) | (nb - 1));
+            const uint64_t maxElts = stored * 8 / (uint64_t)nb;
+            if (maxElts > m) {
+                nodes[2].hdr[1] = (uint8_t)(maxElts - m);
+                nodes[2].hdrLen = 2;
+            }
+            nn = 3;
+        }
+    }
+    st[nn].size = (uint32_t)stored;
+    st[nn].store = 1;
+    int order[4], nbStored = 0;
+    return ozl_write_chunk_header(out, nodes, nn, st, nn + 1, order, &nbStored);
+}
+
+// per column (one wave): bits, packed size, frame and chunk headers, content
+// checksum; frame sizes go to frameOff[c] for the scan. Tiny layouts also put
+// their stored bytes (<= 9) after the header in the column's header slot.
+__global__ void __launch_bounds__(64) k_col_plan(uint32_t K, ColWs ws, uint64_t* __restrict__ frameOff)
+{
+    const uint32_t c = blockIdx.x;
+    if (c >= K) return;
+    ColPlan p = ws.plans[c];
+    if (p.status) {
+        if (threadIdx.x == 0) frameOff[c] = 0;
+        return;
+    }
+    const int w = p.w;
+    const uint8_t* x = reinterpret_cast<const uint8_t*>((uintptr_t)p.src);
+    const uint64_t n = p.m + 1;
+    const uint64_t h = ozl_xxh3_wave(x, n * (uint64_t)w, nullptr, 0);
+    if (threadIdx.x != 0) return;
+    uint8_t* hb = ws.hdr + (size_t)c * kHdrSlot;
+    const size_t fh = ozl_write_frame_header(hb, OZL_T_NUMERIC, n * (uint64_t)w, 1, 1);
+    const uint64_t first = ld_elem(x, w);
+    int nb = 0;
+    uint64_t stored;
+    if (p.kind == kTinyIn) {
+        stored = n * (uint64_t)w;
+    } else if (p.kind == kTinyDz) {
+        stored = p.m * (uint64_t)w;
+    } else {
+        nb = 1 + (p.maxZ ? ozl_hb64(p.maxZ) : 0);
+        stored = (p.m * (uint64_t)nb + 7) / 8;
+    }
+    const size_t ch = col_chunk_header(hb + fh, p.kind, w, nb, p.m, stored, first);
+    if (p.kind == kTinyIn) {
+        for (uint64_t k = 0; k < stored; k++) hb[fh + ch + k] = x[k];
+    } else if (p.kind == kTinyDz) {
+        for (uint64_t d = 0; d < p.m; d++) {
+            const uint64_t dv = (ld_elem(x + (d + 1) * w, w) - ld_elem(x + d * w, w)) & wmask(w);
+            for (int k = 0; k < w; k++) hb[fh + ch + d * w + k] = (uint8_t)(dv >> (8 * k));
+        }
+    }
+    p.nb = (uint8_t)nb;
+    p.packed = stored;
+    p.fhLen = (uint8_t)fh;
+    p.hdrLen = (uint8_t)(fh + ch);
+    p.hashes[0] = (uint32_t)h;
+    ws.plans[c] = p;
+    frameOff[c] = fh + ch + stored + 9;
+}
+
+// frame offsets: exclusive scan of the sizes in frameOff[0..K) -> frameOff[0..K]
+__global__ void __launch_bounds__(kCT) k_col_scan_frames(uint32_t K, uint64_t* __restrict__ frameOff, size_t dstCap,
+                                                        ColWs ws)
+{
+    __shared__ uint64_t lds[kCT / 64];
+    const uint32_t per = (K + kCT - 1) / kCT, b0 = threadIdx.x * per, b1 = b0 + per < K ? b0 + per : K;
+    uint64_t s = 0;
+    for (uint32_t i = b0; i < b1; i++) s += frameOff[i];
+    uint64_t total;
+    uint64_t off = block_excl<uint64_t>(s, lds, total);
+    for (uint32_t i = b0; i < b1; i++) {
+        const uint64_t v = frameOff[i];
+        frameOff[i] = off;
+        if (off + v > dstCap && ws.plans[i].status == 0) ws.plans[i].status = OZL_ERR_DSTCAPACITY_TOOSMALL;
+        off += v;
+    }
+    if (threadIdx.x == 0) frameOff[K] = total;
+}
+
+// Packs 32 consecutive deltas per lane: their nb*32 bits are nb whole words
+// of the stream (items start at multiples of 32 deltas), stored as dwords at
+// the stream's byte position; a partial last group ends in byte stores.
+__global__ void __launch_bounds__(kCT) k_col_pack(uint32_t K, ColWs ws, uint8_t* __restrict__ dst,
+                                                 const uint64_t* __restrict__ frameOff)
+{
+    __shared__ uint32_t sItem, sCol;
+    const uint32_t total = ws.ctr->totalItems;
+    for (;;) {
+        const uint32_t it = next_item(&ws.ctr->queue[1], total, &sItem);
+        if (it == ~0u) break;
+        if (threadIdx.x == 0) sCol = ws.itemCol[it];
+        __syncthreads();
+        const uint32_t c = sCol;
+        const ColPlan& P = ws.plans[c];
+        if (P.status) continue;
+        const int w = P.w, nb = P.nb;
+        const uint32_t t = it - P.itemBase;
+        const uint64_t d0 = (uint64_t)t * P.tile, d1 = d0 + P.tile < P.m ? d0 + P.tile : P.m;
+        uint8_t* fr = dst + frameOff[c];
+        if (t == 0 && threadIdx.x < P.hdrLen) fr[threadIdx.x] = ws.hdr[(size_t)c * kHdrSlot + threadIdx.x];
+        uint8_t* stream = fr + P.hdrLen;
+        const uint8_t* x = reinterpret_cast<const uint8_t*>((uintptr_t)P.src);
+        for (uint64_t g0 = d0 + 32 * (uint64_t)threadIdx.x; g0 < d1; g0 += 32 * (uint64_t)kCT) {
+            const uint32_t cnt = d1 - g0 < 32 ? (uint32_t)(d1 - g0) : 32u;
+            uint8_t* o = stream + g0 * (uint64_t)nb / 8;  // byte position of bit g0*nb (a multiple of 32 bits)
+            uint64_t acc = 0;
+            uint32_t bits = 0, wi = 0;
+            uint64_t prev = ld_elem(x + g0 * w, w);
+            for (uint32_t j = 0; j < cnt; j++) {
+                const uint64_t cur = ld_elem(x + (g0 + j + 1) * w, w);
+                const uint64_t z = zz_delta(prev, cur, w);
+                prev = cur;
+                // append nb bits (nb <= 64): low part then high part
+                const uint32_t n0 = nb < 32 ? (uint32_t)nb : 32u;
+                acc |= (z & ((1ull << n0) - 1)) << bits;
+                bits += n0;
+                if (bits >= 32) {
+                    *reinterpret_cast<uint32_t*>(o + 4 * wi) = (uint32_t)acc;
+                    wi++;
+                    acc >>= 32;
+                    bits -= 32;
+                }
+                if (nb > 32) {
+                    acc |= (z >> 32) << bits;
+                    bits += (uint32_t)nb - 32;
+                    if (bits >= 32) {
+                        *reinterpret_cast<uint32_t*>(o + 4 * wi) = (uint32_t)acc;
+                        wi++;
+                        acc >>= 32;
+                        bits -= 32;
+                    }
+                }
+            }
+            for (uint32_t k = 0; 8 * k < bits; k++) o[4 * wi + k] = (uint8_t)(acc >> (8 * k));
+        }
+    }
+}
+
+// content checksum, compressed checksum (chunk + content checksum), end byte
+__global__ void __launch_bounds__(64) k_col_finish(uint32_t K, ColWs ws, uint8_t* __restrict__ dst,
+                                                  const uint64_t* __restrict__ frameOff, int* __restrict__ status)
+{
+    const uint32_t c = blockIdx.x;
+    if (c >= K) return;
+    const ColPlan p = ws.plans[c];
+    if (p.status) {
+        if (threadIdx.x == 0) status[c] = (int)p.status;
+        return;
+    }
+    uint8_t* fr = dst + frameOff[c];
+    const uint64_t e = p.hdrLen + p.packed;
+    if (p.kind != kBitpack && threadIdx.x < e) fr[threadIdx.x] = ws.hdr[(size_t)c * kHdrSlot + threadIdx.x];
+    if (threadIdx.x < 4) fr[e + threadIdx.x] = (uint8_t)(p.hashes[0] >> (8 * threadIdx.x));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    const uint64_t h = ozl_xxh3_wave(fr + p.fhLen, e + 4 - p.fhLen, nullptr, 0);
+    if (threadIdx.x < 4) fr[e + 4 + threadIdx.x] = (uint8_t)(h >> (8 * threadIdx.x));
+    if (threadIdx.x == 0) {
+        fr[e + 8] = 0;
+        status[c] = 0;
+    }
+}
+
+// ====================================================================== decode
+// Frame check: the width and the layout follow from the frame's input size
+// and the candidate width; the bytes before the stored stream must be exactly
+// what the encoder writes for them (col_chunk_header), with nbBits tried
+// until the rebuilt header matches. Tiny layouts are decoded here.
+__global__ void __launch_bounds__(64) k_cold_parse(const uint8_t* __restrict__ frames, const uint64_t* __restrict__ foff,
+                                                  uint32_t K, uint8_t* __restrict__ dst, const uint64_t* __restrict__ doff,
+                                                  uint32_t tile, ColWs ws)
+{
+    const uint32_t c = blockIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x < 16) reinterpret_cast<uint32_t*>(ws.ctr)[threadIdx.x] = 0;
+    if (c >= K || threadIdx.x != 0) return;
+    ColPlan p = {};
+    p.status = OZL_ERR_CORRUPTION;
+    const uint8_t* fr = frames + foff[c];
+    const uint64_t flen = foff[c + 1] - foff[c];
+    const uint64_t outBytes = doff[2 * c + 1];
+    uint8_t* out = dst + doff[2 * c];
+    ws.plans[c] = p;
+    if (flen < 16) return;
+    uint64_t sz = 0;
+    int q = 6;
+    for (int sh = 0; q < 16; q++, sh += 7) {
+        sz |= (uint64_t)(fr[q] & 127) << sh;
+        if (!(fr[q] & 128)) break;
+    }
+    if (q >= 16 || sz == 0) return;
+    const uint64_t bytes = sz - 1;
+    if (bytes != outBytes) { ws.plans[c].status = OZL_ERR_DSTCAPACITY_TOOSMALL; return; }
+    uint8_t hb[kHdrSlot];
+    for (int lg = 0; lg < 4; lg++) {
+        const int w = 1 << lg;
+        if (bytes == 0 || bytes % (uint64_t)w) continue;
+        if (((uintptr_t)out) & (uintptr_t)(w - 1)) continue;
+        const uint64_t n = bytes / (uint64_t)w, m = n - 1;
+        const size_t fh = ozl_write_frame_header(hb, OZL_T_NUMERIC, bytes, 1, 1);
+        if (fh + 9 > flen) continue;
+        bool same = true;
+        for (size_t k = 0; k < fh; k++) same &= hb[k] == fr[k];
+        if (!same) continue;
+        const int kind = bytes < kMinStream ? kTinyIn : m * (uint64_t)w < kMinStream ? kTinyDz : kBitpack;
+        const int nbLo = kind == kBitpack ? 1 : 0, nbHi = kind == kBitpack ? 8 * w : 0;
+        for (int nb = nbLo; nb <= nbHi; nb++) {
+            const uint64_t stored = kind == kTinyIn ? bytes : kind == kTinyDz ? m * (uint64_t)w
+                                                                               : (m * (uint64_t)nb + 7) / 8;
+            // length with a placeholder first value, then the real one from the frame
+            const size_t ch = col_chunk_header(hb + fh, kind, w, nb, m, stored, 0);
+            if (fh + ch + stored + 9 != flen || fh + ch > kHdrSlot) continue;
+            uint64_t first = 0;
+            if (kind != kTinyIn)
+                for (int k = 0; k < w; k++) first |= (uint64_t)fr[fh + ch - w + k] << (8 * k);
+            col_chunk_header(hb + fh, kind, w, nb, m, stored, first);
+            bool eq = fr[flen - 1] == 0;
+            for (size_t k = fh; k < fh + ch; k++) eq &= hb[k] == fr[k];
+            if (!eq) continue;
+            const uint8_t* body = fr + fh + ch;
+            p.status = 0;
+            p.kind = (uint8_t)kind;
+            p.w = (uint8_t)w;
+            p.nb = (uint8_t)nb;
+            p.m = m;
+            p.packed = stored;
+            p.maxZ = first;
+            p.fhLen = (uint8_t)fh;
+            p.hdrLen = (uint8_t)(fh + ch);
+            p.src = (uint64_t)(uintptr_t)body;
+            const uint8_t* cs = body + stored;
+            p.hashes[0] = (uint32_t)cs[0] | ((uint32_t)cs[1] << 8) | ((uint32_t)cs[2] << 16) | ((uint32_t)cs[3] << 24);
+            p.hashes[1] = (uint32_t)cs[4] | ((uint32_t)cs[5] << 8) | ((uint32_t)cs[6] << 16) | ((uint32_t)cs[7] << 24);
+            if (kind == kTinyIn) {
+                for (uint64_t k = 0; k < bytes; k++) out[k] = body[k];
+            } else if (kind == kTinyDz) {
+                uint64_t v = first;
+                for (int k = 0; k < w; k++) out[k] = (uint8_t)(v >> (8 * k));
+                for (uint64_t d = 0; d < m; d++) {
+                    uint64_t dv = 0;
+                    for (int k = 0; k < w; k++) dv |= (uint64_t)body[d * w + k] << (8 * k);
+                    v = (v + dv) & wmask(w);
+                    for (int k = 0; k < w; k++) out[(d + 1) * w + k] = (uint8_t)(v >> (8 * k));
+                }
+            } else {
+                const uint32_t t = tile < kTileBytesMax / (uint32_t)w ? tile : kTileBytesMax / (uint32_t)w;
+                p.tile = t;
+                p.nItems = (uint32_t)((m + t - 1) / t);
+            }
+            ws.plans[c] = p;
+            return;
+        }
+    }
+}
+
+// unpacks the deltas of group [g0, g0+cnt) (g0 a multiple of 32: the group
+// starts on a word of the stream) into zigzag values, calling f(j, z)
+template <typename F>
+__device__ inline void unpack_group(const uint8_t* stream, uint64_t g0, uint32_t cnt, int nb, F f)
+{
+    const uint8_t* s = stream + g0 * (uint64_t)nb / 8;
+    const uint32_t nbytes = (cnt * (uint32_t)nb + 7) / 8;
+    uint64_t acc = 0;
+    uint32_t bits = 0, rd = 0;  // bytes consumed
+    auto refill = [&]() {
+        if (rd + 4 <= nbytes) {
+            acc |= (uint64_t)*reinterpret_cast<const uint32_t*>(s + rd) << bits;
+            rd += 4;
+            bits += 32;
+        } else {
+            while (rd < nbytes && bits <= 56) {
+                acc |= (uint64_t)s[rd] << bits;
+                rd++;
+                bits += 8;
+            }
+        }
+    };
+    for (uint32_t j = 0; j < cnt; j++) {
+        uint64_t z;
+        if (nb <= 32) {
+            while (bits < (uint32_t)nb) refill();
+            z = acc & ((1ull << nb) - 1);
+            acc >>= nb;
+            bits -= (uint32_t)nb;
+        } else {
+            while (bits < 32) refill();
+            z = acc & 0xFFFFFFFFull;
+            acc >>= 32;
+            bits -= 32;
+            const uint32_t hi = (uint32_t)nb - 32;
+            while (bits < hi) refill();
+            z |= (acc & ((1ull << hi) - 1)) << 32;
+            acc >>= hi;
+            bits -= hi;
+        }
+        f(j, z);
+    }
+}
+
+// tile sums of the deltas (mod 2^64; reduced mod 2^(8w) when used)
+__global__ void __launch_bounds__(kCT) k_cold_sum(uint32_t K, ColWs ws)
+{
+    __shared__ uint32_t sItem, sCol;
+    __shared__ uint64_t lds[kCT / 64];
+    const uint32_t total = ws.ctr->totalItems;
+    for (;;) {
+        const uint32_t it = next_item(&ws.ctr->queue[0], total, &sItem);
+        if (it == ~0u) break;
+        if (threadIdx.x == 0) sCol = ws.itemCol[it];
+        __syncthreads();
+        const ColPlan& P = ws.plans[sCol];
+        const int w = P.w, nb = P.nb;
+        const uint64_t d0 = (uint64_t)(it - P.itemBase) * P.tile, d1 = d0 + P.tile < P.m ? d0 + P.tile : P.m;
+        const uint8_t* stream = reinterpret_cast<const uint8_t*>((uintptr_t)P.src);
+        uint64_t s = 0;
+        for (uint64_t g0 = d0 + 32 * (uint64_t)threadIdx.x; g0 < d1; g0 += 32 * (uint64_t)kCT) {
+            const uint32_t cnt = d1 - g0 < 32 ? (uint32_t)(d1 - g0) : 32u;
+            unpack_group(stream, g0, cnt, nb, [&](uint32_t, uint64_t z) { s += unzz(z, w); });
+        }
+        s = block_sum<uint64_t>(s, lds);
+        if (threadIdx.x == 0) tile_sums(ws, total)[it] = s;
+    }
+}
+
+// per column: exclusive scan of its tile sums seeded with the first element
+__global__ void __launch_bounds__(64) k_cold_carry(uint32_t K, ColWs ws)
+{
+    const uint32_t c = blockIdx.x;
+    if (c >= K) return;
+    const ColPlan p = ws.plans[c];
+    if (p.status || ws.ctr->error) return;
+    const int lane = threadIdx.x;
+    uint64_t* sums = tile_sums(ws, ws.ctr->totalItems);
+    uint64_t carry = p.maxZ;
+    for (uint32_t t0 = 0; t0 < p.nItems; t0 += 64) {
+        const uint32_t t = t0 + lane;
+        const uint64_t v = t < p.nItems ? sums[p.itemBase + t] : 0;
+        uint64_t x = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (t < p.nItems) sums[p.itemBase + t] = carry + x - v;
+        carry += __shfl(x, 63, 64);
+    }
+}
+
+// unpack, prefix sum within the item, add the item's carry, write elements
+__global__ void __launch_bounds__(kCT) k_cold_apply(uint32_t K, ColWs ws, uint8_t* __restrict__ dst,
+                                                   const uint64_t* __restrict__ doff)
+{
+    __shared__ uint32_t sItem, sCol;
+    __shared__ uint64_t lds[kCT / 64];
+    const uint32_t total = ws.ctr->totalItems;
+    for (;;) {
+        const uint32_t it = next_item(&ws.ctr->queue[1], total, &sItem);
+        if (it == ~0u) break;
+        if (threadIdx.x == 0) sCol = ws.itemCol[it];
+        __syncthreads();
+        const uint32_t c = sCol;
+        const ColPlan& P = ws.plans[c];
+        const int w = P.w, nb = P.nb;
+        const uint32_t t = it - P.itemBase;
+        const uint64_t d0 = (uint64_t)t * P.tile, d1 = d0 + P.tile < P.m ? d0 + P.tile : P.m;
+        const uint8_t* stream = reinterpret_cast<const uint8_t*>((uintptr_t)P.src);
+        uint8_t* out = dst + doff[2 * c];
+        const uint64_t mask = wmask(w);
+        if (t == 0 && threadIdx.x == 0) {
+            for (int k = 0; k < w; k++) out[k] = (uint8_t)(P.maxZ >> (8 * k));
+        }
+        uint64_t base = tile_sums(ws, total)[it];
+        // rounds of kCT groups of 32 deltas
+        for (uint64_t r0 = d0; r0 < d1; r0 += 32 * (uint64_t)kCT) {
+            const uint64_t g0 = r0 + 32 * (uint64_t)threadIdx.x;
+            const uint32_t cnt = g0 < d1 ? (d1 - g0 < 32 ? (uint32_t)(d1 - g0) : 32u) : 0u;
+            uint64_t s = 0;
+            if (cnt) unpack_group(stream, g0, cnt, nb, [&](uint32_t, uint64_t z) { s += unzz(z, w); });
+            uint64_t tot;
+            uint64_t pre = block_excl<uint64_t>(s, lds, tot);
+            uint64_t v = base + pre;
+            if (cnt) {
+                uint8_t* o = out + (g0 + 1) * (uint64_t)w;
+                unpack_group(stream, g0, cnt, nb, [&](uint32_t j, uint64_t z) {
+                    v += unzz(z, w);
+                    const uint64_t e = v & mask;
+                    switch (w) {
+                    case 1: o[j] = (uint8_t)e; break;
+                    case 2: reinterpret_cast<uint16_t*>(o)[j] = (uint16_t)e; break;
+                    case 4: reinterpret_cast<uint32_t*>(o)[j] = (uint32_t)e; break;
+                    default: reinterpret_cast<uint64_t*>(o)[j] = e; break;
+                    }
+                });
+            }
+            base += tot;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) k_cold_verify(uint32_t K, ColWs ws, const uint8_t* __restrict__ frames,
+                                                   const uint64_t* __restrict__ foff, const uint8_t* __restrict__ dst,
+                                                   const uint64_t* __restrict__ doff, int* __restrict__ status)
+{
+    const uint32_t c = blockIdx.x;
+    if (c >= K) return;
+    const ColPlan p = ws.plans[c];
+    int st = ws.ctr->error ? (int)OZL_ERR_PARAMETER_INVALID : (int)p.status;
+    if (st == 0) {
+        const uint8_t* fr = frames + foff[c];
+        const uint64_t e = p.hdrLen + p.packed;
+        const uint64_t hc = ozl_xxh3_wave(fr + p.fhLen, e + 4 - p.fhLen, nullptr, 0);
+        const uint64_t hd = ozl_xxh3_wave(dst + doff[2 * c], (p.m + 1) * (uint64_t)p.w, nullptr, 0);
+        if ((uint32_t)hc != p.hashes[1]) st = OZL_ERR_COMPRESSED_CHECKSUM;
+        else if ((uint32_t)hd != p.hashes[0]) st = OZL_ERR_CONTENT_CHECKSUM;
+    }
+    if (threadIdx.x == 0) status[c] = st;
+}
+
+static int persistent_grid(uint32_t wgPerCu)
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 256 * 4;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+    }
+    return cus * (int)(wgPerCu ? wgPerCu : 4);
+}
+
+// ---------------------------------------------------------------- autotuner
+// (K, total bytes) -> best (tile, workgroups per CU), measured on the caller's
+// batch with HIP events over the whole encode call
+struct TuneKey {
+    uint32_t K;
+    uint64_t bytes;
+    bool operator<(const TuneKey& o) const { return std::tie(K, bytes) < std::tie(o.K, o.bytes); }
+};
+std::mutex g_tuneMu;
+std::map<TuneKey, std::pair<uint32_t, uint32_t>> g_tune;
+
+}  // namespace
+
+extern "C" {
+
+// Workspace bytes for K columns of `totalBytes` input bytes in all, cut into
+// items of `tile` deltas (64 B per column for the plan, 96 B of headers, 4 B
+// per item for the item map and 8 B for the decoder's tile sums).
+size_t ozl_columns_workspace(uint32_t K, size_t totalBytes, uint32_t tile)
+{
+    if (tile < 32) tile = 32;
+    const size_t items = totalBytes / tile + K + 1;  // deltas <= bytes; one partial item per column
+    return 512 + (size_t)K * sizeof(ColPlan) + (((size_t)K * kHdrSlot + 255) & ~(size_t)255) + items * 12 + 256;
+}
+
+// Upper bound of the frame of a column of `bytes` input bytes.
+size_t ozl_columns_frame_bound(size_t bytes) { return bytes + 128; }
+
+// Column c of a batch is colSpan[2c] (byte offset into src, a multiple of its
+// width) and colSpan[2c+1] (its bytes, a multiple of the width, > 0); on
+// decode, dstSpan gives the same pairs for the outputs. Frame c is
+// dst[frameOff[c], frameOff[c+1]).
+
+int ozl_columns_compress_batch(const void* src, const uint64_t* colOff, const uint8_t* widths, uint32_t K, void* dst,
+                               size_t dstCap, uint64_t* frameOff, int* status, void* workspace, size_t wsBytes,
+                               uint32_t tile, uint32_t wgPerCu, void* stream)
+{
+    if (K == 0) return OZL_OK;
+    if (!src || !colOff || !widths || !dst || !frameOff || !status || !workspace) return OZL_ERR_PARAMETER_INVALID;
+    if (tile < 32 || (tile & 31) || wsBytes < 512 + (size_t)K * (sizeof(ColPlan) + kHdrSlot) + 256)
+        return OZL_ERR_PARAMETER_INVALID;
+    hipStream_t st = (hipStream_t)stream;
+    ColWs ws = carve(workspace, K);
+    const uint8_t* s = (const uint8_t*)src;
+    const int G = persistent_grid(wgPerCu);
+    OZL_LAUNCH("k_col_setup", k_col_setup, dim3((K + kCT - 1) / kCT), dim3(kCT), 0, st, s, colOff, widths, K, tile, ws);
+    OZL_LAUNCH("k_col_scan_items", k_col_scan_items, dim3(1), dim3(kCT), 0, st, K, ws,
+               ((uint64_t)(uintptr_t)workspace + wsBytes) | (1ull << 63));
+    OZL_LAUNCH("k_col_map", k_col_map, dim3(K), dim3(64), 0, st, K, ws);
+    OZL_LAUNCH("k_col_max", k_col_max, dim3(G), dim3(kCT), 0, st, K, ws);
+    OZL_LAUNCH("k_col_plan", k_col_plan, dim3(K), dim3(64), 0, st, K, ws, frameOff);
+    OZL_LAUNCH("k_col_scan_frames", k_col_scan_frames, dim3(1), dim3(kCT), 0, st, K, frameOff, dstCap, ws);
+    OZL_LAUNCH("k_col_pack", k_col_pack, dim3(G), dim3(kCT), 0, st, K, ws, (uint8_t*)dst, (const uint64_t*)frameOff);
+    OZL_LAUNCH("k_col_finish", k_col_finish, dim3(K), dim3(64), 0, st, K, ws, (uint8_t*)dst, (const uint64_t*)frameOff,
+               status);
+    OZL_HIP_CHECK(hipGetLastError());
+    return OZL_OK;
+}
+
+int ozl_columns_decompress_batch(const void* frames, const uint64_t* frameOff, uint32_t K, void* dst,
+                                 const uint64_t* dstOff, int* status, void* workspace, size_t wsBytes, uint32_t tile,
+                                 uint32_t wgPerCu, void* stream)
+{
+    if (K == 0) return OZL_OK;
+    if (!frames || !frameOff || !dst || !dstOff || !status || !workspace) return OZL_ERR_PARAMETER_INVALID;
+    if (tile < 32 || (tile & 31) || wsBytes < 512 + (size_t)K * (sizeof(ColPlan) + kHdrSlot) + 256)
+        return OZL_ERR_PARAMETER_INVALID;
+    hipStream_t st = (hipStream_t)stream;
+    ColWs ws = carve(workspace, K);
+    const int G = persistent_grid(wgPerCu);
+    const uint8_t* f = (const uint8_t*)frames;
+    uint8_t* d = (uint8_t*)dst;
+    OZL_LAUNCH("k_cold_parse", k_cold_parse, dim3(K), dim3(64), 0, st, f, frameOff, K, d, dstOff, tile, ws);
+    const uint64_t wsEnd = (uint64_t)(uintptr_t)workspace + wsBytes;
+    OZL_LAUNCH("k_col_scan_items", k_col_scan_items, dim3(1), dim3(kCT), 0, st, K, ws, wsEnd);
+    OZL_LAUNCH("k_col_map", k_col_map, dim3(K), dim3(64), 0, st, K, ws);
+    OZL_LAUNCH("k_cold_sum", k_cold_sum, dim3(G), dim3(kCT), 0, st, K, ws);
+    OZL_LAUNCH("k_cold_carry", k_cold_carry, dim3(K), dim3(64), 0, st, K, ws);
+    OZL_LAUNCH("k_cold_apply", k_cold_apply, dim3(G), dim3(kCT), 0, st, K, ws, d, dstOff);
+    OZL_LAUNCH("k_cold_verify", k_cold_verify, dim3(K), dim3(64), 0, st, K, ws, f, frameOff, (const uint8_t*)d, dstOff,
+               status);
+    OZL_HIP_CHECK(hipGetLastError());
+    return OZL_OK;
+}
+
+// Autotuner: times ozl_columns_compress_batch on this batch for every
+// (tile, workgroups per CU) candidate and returns the fastest; the choice is
+// cached per (K, total bytes) for the process. The workspace must be sized
+// for the smallest candidate tile (1024).
+int ozl_columns_autotune(const void* src, const uint64_t* colOff, const uint8_t* widths, uint32_t K, uint64_t totalBytes,
+                         void* dst, size_t dstCap, uint64_t* frameOff, int* status, void* workspace, size_t wsBytes,
+                         uint32_t* tileOut, uint32_t* wgOut, void* stream)
+{
+    if (!tileOut || !wgOut) return OZL_ERR_PARAMETER_INVALID;
+    {
+        std::lock_guard<std::mutex> g(g_tuneMu);
+        auto itc = g_tune.find(TuneKey{ K, totalBytes });
+        if (itc != g_tune.end()) {
+            *tileOut = itc->second.first;
+            *wgOut = itc->second.second;
+            return OZL_OK;
+        }
+    }
+    hipStream_t st = (hipStream_t)stream;
+    static const uint32_t tiles[] = { 1024, 2048, 4096, 8192, 16384 };
+    static const uint32_t wgs[] = { 2, 4, 8 };
+    hipEvent_t a, b;
+    OZL_HIP_CHECK(hipEventCreate(&a));
+    OZL_HIP_CHECK(hipEventCreate(&b));
+    float best = 1e30f;
+    uint32_t bt = 4096, bw = 4;
+    int rc = OZL_OK;
+    for (uint32_t t : tiles) {
+        if (ozl_columns_workspace(K, totalBytes, t) > wsBytes) continue;
+        for (uint32_t wg : wgs) {
+            // one warm call, then the timed one
+            rc = ozl_columns_compress_batch(src, colOff, widths, K, dst, dstCap, frameOff, status, workspace, wsBytes, t,
+                                            wg, stream);
+            if (rc) break;
+            if (hipEventRecord(a, st) != hipSuccess) { rc = OZL_ERR_HIP; break; }
+            rc = ozl_columns_compress_batch(src, colOff, widths, K, dst, dstCap, frameOff, status, workspace, wsBytes, t,
+                                            wg, stream);
+            if (rc) break;
+            if (hipEventRecord(b, st) != hipSuccess || hipEventSynchronize(b) != hipSuccess) { rc = OZL_ERR_HIP; break; }
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, a, b) != hipSuccess) { rc = OZL_ERR_HIP; break; }
+            if (ms < best) { best = ms; bt = t; bw = wg; }
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> g(g_tuneMu);
+        g_tune[TuneKey{ K, totalBytes }] = { bt, bw };
+    }
+    *tileOut = bt;
+    *wgOut = bw;
+    return OZL_OK;
+}
+
+// hipGraph of one batch call: captures ozl_columns_compress_batch (dir 0) or
+// ozl_columns_decompress_batch (dir 1) with these pointers on a private
+// stream; ozl_graph_launch replays it on `stream`.
+struct ozl_graph {
+    hipGraph_t g;
+    hipGraphExec_t x;
+};
+
+void* ozl_columns_graph_create(int dir, const void* a0, const uint64_t* a1, const uint8_t* widths, uint32_t K, void* a3,
+                               size_t dstCap, uint64_t* a4, int* status, void* workspace, size_t wsBytes, uint32_t tile,
+                               uint32_t wgPerCu)
+{
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    ozl_graph* G = new ozl_graph{};
+    int rc = OZL_OK;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = OZL_ERR_HIP;
+    if (!rc) {
+        if (dir == 0)
+            rc = ozl_columns_compress_batch(a0, a1, widths, K, a3, dstCap, a4, status, workspace, wsBytes, tile, wgPerCu,
+                                            cs);
+        else
+            rc = ozl_columns_decompress_batch(a0, a1, K, a3, a4, status, workspace, wsBytes, tile, wgPerCu, cs);
+        if (hipStreamEndCapture(cs, &G->g) != hipSuccess) rc = OZL_ERR_HIP;
+    }
+    if (!rc && hipGraphInstantiate(&G->x, G->g, nullptr, nullptr, 0) != hipSuccess) rc = OZL_ERR_HIP;
+    (void)hipStreamDestroy(cs);
+    if (rc) {
+        delete G;
+        return nullptr;
+    }
+    return G;
+}
+
+int ozl_graph_launch(void* graph, void* stream)
+{
+    if (!graph) return OZL_ERR_PARAMETER_INVALID;
+    OZL_HIP_CHECK(hipGraphLaunch(((ozl_graph*)graph)->x, (hipStream_t)stream));
+    return OZL_OK;
+}
+
+void ozl_graph_destroy(void* graph)
+{
+    if (!graph) return;
+    ozl_graph* G = (ozl_graph*)graph;
+    (void)hipGraphExecDestroy(G->x);
+    (void)hipGraphDestroy(G->g);
+    delete G;
+}
+
+}  // extern "C"

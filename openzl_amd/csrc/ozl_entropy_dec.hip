@@ -664,7 +664,7 @@ __global__ void __launch_bounds__(64) k_huf_decode(uint32_t B, ozl_dws ws)
     Win w;
     bool ok = win_init<kHRing>(w, ring, lo, len);
     if (ok) win_refill32(w, ring_dword<kHRing>(ring, lo + (uint64_t)(int64_t)w.lb - 4));
-    const uint32_t R = ok ? cnt : 0u, R16 = R & ~15u;
+    const uint32_t R = ok ? cnt : 0u;
     auto sym = [&]() -> uint32_t {
         const uint32_t e = LUT[win_hi(w) >> (32 - tl)];
         const uint32_t d = ring_dword<kHRing>(ring, (uint64_t)((int64_t)lo + w.lb) - 4);
