@@ -15,14 +15,18 @@
 // Work is cut into items of `tile` deltas (a multiple of 32, capped at 32 KiB
 // of input per item); each pass over items is one persistent launch whose
 // workgroups take items from a device counter until none is left:
-//   encode  k_col_setup -> k_col_scan_items -> k_col_max (persistent: max of
-//           the zigzag deltas per column) -> k_col_plan (nbBits, headers,
-//           content hash) -> k_col_scan_frames -> k_col_pack (persistent:
-//           bits straight into the frames) -> k_col_finish (checksums)
-//   decode  k_cold_parse -> k_cold_scan_items -> k_cold_sum (persistent: tile
+//   encode  k_col_setup -> k_col_scan_items -> k_col_map -> hash pass over the
+//           columns (content checksums) -> k_col_max (largest zigzag delta per
+//           column) -> k_col_plan (nbBits, headers) -> k_col_scan_frames ->
+//           k_col_pack (bits straight into the frames) -> k_col_finish1 ->
+//           hash pass over the chunks (compressed checksums) -> k_col_finish2
+//   decode  k_cold_parse -> k_col_scan_items -> k_col_map -> k_cold_sum (tile
 //           sums of the deltas) -> k_cold_carry (per-column scan of the tile
-//           sums) -> k_cold_apply (persistent: unpack, prefix sum, write) ->
-//           k_cold_verify (checksums)
+//           sums) -> k_cold_apply (unpack, prefix sum, write) -> hash pass over
+//           the columns and the chunks -> k_cold_verify
+// A hash pass computes the XXH3 of many regions with many waves per region:
+// every 1 KiB block's stripe sums are independent, only the scramble chain
+// over a region's blocks is sequential (ozl_xxh3.h).
 // Every launch clears what it needs with kernels, never with memsets, so a
 // call can be captured in a hipGraph and replayed.
 #include "ozl_internal.h"
@@ -43,7 +47,8 @@ namespace {
 
 constexpr int kCT = 256;          // threads per workgroup
 constexpr int kHdrSlot = 96;      // per-column header bytes (frame + chunk header)
-constexpr uint32_t kTileBytesMax = 32768;
+constexpr uint32_t kTileMax = 4096;          // deltas per item (LDS: 8 bytes per delta)
+constexpr int kPer = (int)kTileMax / kCT;     // deltas per thread and item
 
 struct ColPlan {                  // 64 bytes, one per column
     uint64_t src;                 // column address (encode: input, decode: packed stream)
@@ -70,30 +75,61 @@ constexpr uint64_t kMinStream = 10;
 static_assert(sizeof(ColPlan) == 64, "ColPlan layout");
 
 struct ColCtr {
-    uint32_t queue[4];
-    uint32_t totalItems;
-    uint32_t error;
-    uint32_t pad[10];
+    uint32_t queue[4];            // work counters: column items (2), hash items (2)
+    uint32_t totalItems;          // column items
+    uint32_t error;               // bit 0: column items do not fit, bit 1: hash blocks do not fit
+    uint32_t hTotalItems;         // hash items of the current hash pass
+    uint32_t hTotalBlocks;
+    uint32_t pad[8];
 };
+
+// one XXH3 input of a hash pass: its 1 KiB blocks are summed by many waves
+// (items of kHG blocks), then one wave per region runs the scramble chain
+struct HashReg {
+    uint64_t ptr, len;
+    uint32_t blockBase, nBlocks, itemBase, nItems;
+};
+constexpr uint32_t kHG = 16;
 
 struct ColWs {
+    ColCtr* ctr;
     ColPlan* plans;
     uint8_t* hdr;                 // kHdrSlot bytes per column
-    uint32_t* itemCol;            // column of every item; the decoder's tile sums follow
-    ColCtr* ctr;
+    HashReg* hreg;                // 2K regions
+    uint64_t* hout;               // 2K hashes
+    uint8_t* dyn;                 // item map, tile sums, hash item map, hash block sums (sized on the device)
+    uint64_t end;                 // workspace end address
 };
 
-// ws layout: [ctr 256 B][plans K*64][hdr K*96][itemCol items*4][tileSum items*8]
-// (tileSum is placed by k_col_scan_items once the item count is known)
-static ColWs carve(void* w, uint32_t K)
+// ws layout: [ctr 256 B][plans K*64][hdr K*96][hreg 2K*32][hout 2K*8][dynamic...]
+static ColWs carve(void* w, uint32_t K, size_t wsBytes)
 {
     uint8_t* p = (uint8_t*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
     ColWs ws;
     ws.ctr = (ColCtr*)p; p += 256;
     ws.plans = (ColPlan*)p; p += (size_t)K * sizeof(ColPlan);
     ws.hdr = p; p += ((size_t)K * kHdrSlot + 255) & ~(size_t)255;
-    ws.itemCol = (uint32_t*)p;
+    ws.hreg = (HashReg*)p; p += (size_t)2 * K * sizeof(HashReg);
+    ws.hout = (uint64_t*)p; p += (size_t)2 * K * 8;
+    ws.dyn = (uint8_t*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
+    ws.end = (uint64_t)(uintptr_t)w + wsBytes;
     return ws;
+}
+static size_t fixed_bytes(uint32_t K) { return 256 + 256 + (size_t)K * sizeof(ColPlan) + (size_t)K * kHdrSlot + 256 +
+                                              (size_t)2 * K * (sizeof(HashReg) + 8) + 256; }
+
+__device__ inline uint32_t* item_col(const ColWs& ws) { return reinterpret_cast<uint32_t*>(ws.dyn); }
+__device__ inline uint64_t* tile_sums(const ColWs& ws, uint32_t items)
+{
+    return reinterpret_cast<uint64_t*>(((uintptr_t)(ws.dyn + 4 * (size_t)items) + 7) & ~(uintptr_t)7);
+}
+__device__ inline uint32_t* hash_map(const ColWs& ws, uint32_t items)
+{
+    return reinterpret_cast<uint32_t*>(tile_sums(ws, items) + items);
+}
+__device__ inline uint64_t* hash_sums(const ColWs& ws, uint32_t items, uint32_t hItems)
+{
+    return reinterpret_cast<uint64_t*>(((uintptr_t)(hash_map(ws, items) + hItems) + 63) & ~(uintptr_t)63);
 }
 
 __device__ inline uint64_t ld_elem(const uint8_t* p, int w)
@@ -126,12 +162,6 @@ __device__ inline uint32_t next_item(uint32_t* q, uint32_t total, uint32_t* sIte
     __syncthreads();
     const uint32_t it = *sItem;
     return it < total ? it : ~0u;
-}
-
-// tile sums follow the item-to-column map (8-byte aligned)
-__device__ inline uint64_t* tile_sums(const ColWs& ws, uint32_t total)
-{
-    return reinterpret_cast<uint64_t*>(((uintptr_t)(ws.itemCol + total) + 7) & ~(uintptr_t)7);
 }
 
 template <typename T>
@@ -191,7 +221,7 @@ __global__ void __launch_bounds__(kCT) k_col_setup(const uint8_t* __restrict__ s
         if (n * (uint64_t)w < kMinStream) p.kind = kTinyIn;
         else if ((n - 1) * (uint64_t)w < kMinStream) p.kind = kTinyDz;
         else {
-            uint32_t t = tile < kTileBytesMax / (uint32_t)w ? tile : kTileBytesMax / (uint32_t)w;
+            const uint32_t t = tile < kTileMax ? tile : kTileMax;
             p.tile = t;
             p.nItems = (uint32_t)((p.m + t - 1) / t);
         }
@@ -199,11 +229,10 @@ __global__ void __launch_bounds__(kCT) k_col_setup(const uint8_t* __restrict__ s
     ws.plans[c] = p;
 }
 
-// item bases: exclusive scan of nItems (one workgroup). `sumsEnd` is the end
-// address of the workspace, with bit 63 set when the items need no tile sums
-// (encode); when the item map (and the sums) would pass it, no item runs and
-// every column reports OZL_ERR_PARAMETER_INVALID.
-__global__ void __launch_bounds__(kCT) k_col_scan_items(uint32_t K, ColWs ws, uint64_t sumsEnd)
+// item bases: exclusive scan of nItems (one workgroup). When the item map
+// and the tile sums would pass the workspace end, no item runs and every
+// column reports OZL_ERR_PARAMETER_INVALID.
+__global__ void __launch_bounds__(kCT) k_col_scan_items(uint32_t K, ColWs ws)
 {
     __shared__ uint32_t lds[kCT / 64];
     const uint32_t per = (K + kCT - 1) / kCT, b0 = threadIdx.x * per, b1 = b0 + per < K ? b0 + per : K;
@@ -216,10 +245,9 @@ __global__ void __launch_bounds__(kCT) k_col_scan_items(uint32_t K, ColWs ws, ui
         off += ws.plans[i].nItems;
     }
     if (threadIdx.x == 0) {
-        const bool fits = (uint64_t)(uintptr_t)(sumsEnd >> 63 ? ws.itemCol + total : (uint32_t*)(tile_sums(ws, total) + total)) <=
-                          (sumsEnd & ~(1ull << 63));
+        const bool fits = (uint64_t)(uintptr_t)(hash_map(ws, total)) <= ws.end;
         ws.ctr->totalItems = fits ? total : 0u;
-        ws.ctr->error = fits ? 0u : 1u;
+        if (!fits) ws.ctr->error |= 1u;
     }
 }
 
@@ -229,11 +257,188 @@ __global__ void __launch_bounds__(64) k_col_map(uint32_t K, ColWs ws)
     const uint32_t c = blockIdx.x;
     if (c >= K || ws.ctr->error) return;
     const ColPlan& p = ws.plans[c];
-    for (uint32_t i = threadIdx.x; i < p.nItems; i += 64) ws.itemCol[p.itemBase + i] = c;
+    for (uint32_t i = threadIdx.x; i < p.nItems; i += 64) item_col(ws)[p.itemBase + i] = c;
 }
 
-// largest zigzag delta per column; items stream their deltas with coalesced
-// element loads (the second load of each pair hits L1)
+// ---------------------------------------------------------------- hash passes
+// Every region's XXH3: k_hash_scan places blocks and items, k_hash_map maps
+// items to regions, k_hash_sums (persistent, one wave per workgroup) sums the
+// 16 stripes of each block (ozl_xxh3_block_sums), k_hash_chain runs each
+// region's scramble chain over its block sums, the tail stripes and the merge
+// (one wave per region) into hout.
+__device__ inline void hreg_set(HashReg& r, uint64_t ptr, uint64_t len)
+{
+    r.ptr = ptr;
+    r.len = len;
+    r.nBlocks = len > 240 ? (uint32_t)((len - 1) / 1024) : 0u;
+    r.nItems = (r.nBlocks + kHG - 1) / kHG;
+    r.blockBase = r.itemBase = 0;
+}
+
+__global__ void __launch_bounds__(kCT) k_hreg_content(uint32_t K, ColWs ws)
+{
+    const uint32_t c = blockIdx.x * kCT + threadIdx.x;
+    if (c >= K) return;
+    const ColPlan& p = ws.plans[c];
+    hreg_set(ws.hreg[c], p.src, p.status ? 0 : (p.m + 1) * (uint64_t)p.w);
+}
+
+// compressed checksum input: the chunk and the content checksum
+__global__ void __launch_bounds__(kCT) k_hreg_frames(uint32_t K, ColWs ws, const uint8_t* __restrict__ dst,
+                                                    const uint64_t* __restrict__ frameOff)
+{
+    const uint32_t c = blockIdx.x * kCT + threadIdx.x;
+    if (c >= K) return;
+    const ColPlan& p = ws.plans[c];
+    hreg_set(ws.hreg[c], (uint64_t)(uintptr_t)(dst + frameOff[c] + p.fhLen),
+             p.status ? 0 : p.hdrLen - p.fhLen + p.packed + 4);
+}
+
+// decode: region c = decoded column, region K + c = the frame's chunk and content checksum
+__global__ void __launch_bounds__(kCT) k_hreg_decode(uint32_t K, ColWs ws, const uint8_t* __restrict__ frames,
+                                                    const uint64_t* __restrict__ foff, const uint8_t* __restrict__ dst,
+                                                    const uint64_t* __restrict__ doff)
+{
+    const uint32_t c = blockIdx.x * kCT + threadIdx.x;
+    if (c >= K) return;
+    const ColPlan& p = ws.plans[c];
+    const bool ok = p.status == 0 && !ws.ctr->error;
+    hreg_set(ws.hreg[c], (uint64_t)(uintptr_t)(dst + doff[2 * c]), ok ? (p.m + 1) * (uint64_t)p.w : 0);
+    hreg_set(ws.hreg[K + c], (uint64_t)(uintptr_t)(frames + foff[c] + p.fhLen),
+             ok ? p.hdrLen - p.fhLen + p.packed + 4 : 0);
+}
+
+__global__ void __launch_bounds__(kCT) k_hash_scan(uint32_t R, ColWs ws)
+{
+    __shared__ uint32_t lds[kCT / 64];
+    const uint32_t per = (R + kCT - 1) / kCT, b0 = threadIdx.x * per, b1 = b0 + per < R ? b0 + per : R;
+    uint32_t sb = 0, si = 0;
+    for (uint32_t i = b0; i < b1; i++) { sb += ws.hreg[i].nBlocks; si += ws.hreg[i].nItems; }
+    uint32_t tb, ti;
+    uint32_t ob = block_excl<uint32_t>(sb, lds, tb);
+    __syncthreads();
+    uint32_t oi = block_excl<uint32_t>(si, lds, ti);
+    for (uint32_t i = b0; i < b1; i++) {
+        ws.hreg[i].blockBase = ob;
+        ws.hreg[i].itemBase = oi;
+        ob += ws.hreg[i].nBlocks;
+        oi += ws.hreg[i].nItems;
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t items = ws.ctr->totalItems;
+        const bool fits = (uint64_t)(uintptr_t)(hash_sums(ws, items, ti) + 8 * (size_t)tb) <= ws.end;
+        ws.ctr->hTotalItems = fits ? ti : 0u;
+        ws.ctr->hTotalBlocks = fits ? tb : 0u;
+        if (!fits) ws.ctr->error |= 2u;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_hash_map(uint32_t R, ColWs ws)
+{
+    const uint32_t r = blockIdx.x;
+    if (r >= R || (ws.ctr->error & 2u)) return;
+    const HashReg& h = ws.hreg[r];
+    uint32_t* map = hash_map(ws, ws.ctr->totalItems);
+    for (uint32_t i = threadIdx.x; i < h.nItems; i += 64) map[h.itemBase + i] = r;
+}
+
+__global__ void __launch_bounds__(64) k_hash_sums(ColWs ws, int q)
+{
+    const uint32_t total = ws.ctr->hTotalItems, items = ws.ctr->totalItems;
+    const uint32_t* map = hash_map(ws, items);
+    uint64_t* sums = hash_sums(ws, items, total);
+    const ozl_xxh3_lane L = ozl_xxh3_lane_init();
+    const int lane = threadIdx.x;
+    for (;;) {
+        uint32_t v = 0;
+        if (lane == 0) v = atomicAdd(&ws.ctr->queue[q], 1u);
+        const uint32_t it = (uint32_t)__shfl((int)v, 0, 64);
+        if (it >= total) break;
+        const HashReg h = ws.hreg[map[it]];
+        const uint32_t b0 = (it - h.itemBase) * kHG, b1 = b0 + kHG < h.nBlocks ? b0 + kHG : h.nBlocks;
+        const uint8_t* in = reinterpret_cast<const uint8_t*>((uintptr_t)h.ptr);
+        const bool al = (h.ptr & 15) == 0;
+        for (uint32_t b = b0; b < b1; b++) {
+            uint64_t sa, sb;
+            ozl_xxh3_block_sums(in + (size_t)b * 1024, al, L, sa, sb);
+            if (lane < 4) {
+                uint64_t* o = sums + 8 * ((size_t)h.blockBase + b) + 2 * L.j;
+                o[0] = sa;
+                o[1] = sb;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) k_hash_chain(uint32_t R, ColWs ws)
+{
+    const uint32_t r = blockIdx.x;
+    if (r >= R) return;
+    const HashReg h = ws.hreg[r];
+    const int lane = threadIdx.x;
+    const uint8_t* in = reinterpret_cast<const uint8_t*>((uintptr_t)h.ptr);
+    if (h.len == 0 || (ws.ctr->error & 2u)) {
+        if (lane == 0) ws.hout[r] = 0;
+        return;
+    }
+    uint64_t hv;
+    if (h.len <= 240) {
+        hv = ozl_xxh3_short(in, h.len);
+    } else {
+        const uint64_t* sums = hash_sums(ws, ws.ctr->totalItems, ws.ctr->hTotalItems) + 8 * (size_t)h.blockBase;
+        const int i = lane & 7;
+        uint64_t acc = ozl_xxh3_init_acc(i);
+        const uint64_t c = ozl_sec64(192 - 64 + 8 * i);
+        uint32_t b = 0;
+        for (; b + 8 <= h.nBlocks; b += 8) {
+            uint64_t v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = sums[8 * (size_t)(b + k) + i];
+#pragma unroll
+            for (int k = 0; k < 8; k++) acc = ozl_xxh3_scr(acc + v[k], c);
+        }
+        for (; b < h.nBlocks; b++) acc = ozl_xxh3_scr(acc + sums[8 * (size_t)b + i], c);
+        hv = ozl_xxh3_finish(in, h.len, h.nBlocks, acc);
+    }
+    if (lane == 0) ws.hout[r] = hv;
+}
+
+template <int W> struct ElemT;
+template <> struct ElemT<1> { using T = uint8_t; };
+template <> struct ElemT<2> { using T = uint16_t; };
+template <> struct ElemT<4> { using T = uint32_t; };
+template <> struct ElemT<8> { using T = uint64_t; };
+
+// zigzag deltas d0+tid+256k (k < kPer) of an item: every load of the item is
+// issued before the first use, so their latency overlaps
+template <int W>
+__device__ inline void item_zz(const uint8_t* x, uint64_t d0, uint64_t d1, uint64_t* z)
+{
+    using T = typename ElemT<W>::T;
+    const T* e = reinterpret_cast<const T*>(x);
+    T a[kPer], b[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const uint64_t d = d0 + threadIdx.x + (uint64_t)kCT * k;
+        const uint64_t dd = d < d1 ? d : d0;
+        a[k] = e[dd];
+        b[k] = e[dd + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) z[k] = zz_delta((uint64_t)a[k], (uint64_t)b[k], W);
+}
+
+__device__ inline void item_zz_w(int w, const uint8_t* x, uint64_t d0, uint64_t d1, uint64_t* z)
+{
+    switch (w) {
+    case 1: item_zz<1>(x, d0, d1, z); break;
+    case 2: item_zz<2>(x, d0, d1, z); break;
+    case 4: item_zz<4>(x, d0, d1, z); break;
+    default: item_zz<8>(x, d0, d1, z); break;
+    }
+}
+
+// largest zigzag delta per column
 __global__ void __launch_bounds__(kCT) k_col_max(uint32_t K, ColWs ws)
 {
     __shared__ uint32_t sItem, sCol;
@@ -241,17 +446,16 @@ __global__ void __launch_bounds__(kCT) k_col_max(uint32_t K, ColWs ws)
     for (;;) {
         const uint32_t it = next_item(&ws.ctr->queue[0], total, &sItem);
         if (it == ~0u) break;
-        if (threadIdx.x == 0) sCol = ws.itemCol[it];
+        if (threadIdx.x == 0) sCol = item_col(ws)[it];
         __syncthreads();
         const ColPlan& P = ws.plans[sCol];
-        const int w = P.w;
         const uint64_t d0 = (uint64_t)(it - P.itemBase) * P.tile, d1 = d0 + P.tile < P.m ? d0 + P.tile : P.m;
-        const uint8_t* x = reinterpret_cast<const uint8_t*>((uintptr_t)P.src);
+        uint64_t z[kPer];
+        item_zz_w(P.w, reinterpret_cast<const uint8_t*>((uintptr_t)P.src), d0, d1, z);
         unsigned long long mx = 0;
-        for (uint64_t d = d0 + threadIdx.x; d < d1; d += kCT) {
-            const uint64_t z = zz_delta(ld_elem(x + d * w, w), ld_elem(x + (d + 1) * w, w), w);
-            mx = z > mx ? z : mx;
-        }
+#pragma unroll
+        for (int k = 0; k < kPer; k++)
+            if (d0 + threadIdx.x + (uint64_t)kCT * k < d1) mx = z[k] > mx ? z[k] : mx;
         for (int s = 32; s >= 1; s >>= 1) {
             const unsigned long long o = __shfl_xor(mx, s, 64);
             mx = o > mx ? o : mx;
@@ -299,9 +503,9 @@ __device__ inline size_t col_chunk_header(uint8_t* out, int kind, int w, int nb,
     return ozl_write_chunk_header(out, nodes, nn, st, nn + 1, order, &nbStored);
 }
 
-// per column (one wave): bits, packed size, frame and chunk headers, content
-// checksum; frame sizes go to frameOff[c] for the scan. Tiny layouts also put
-// their stored bytes (<= 9) after the header in the column's header slot.
+// per column: bits, packed size, frame and chunk headers; frame sizes go to
+// frameOff[c] for the scan. Tiny layouts also put their stored bytes (<= 9)
+// after the header in the column's header slot.
 __global__ void __launch_bounds__(64) k_col_plan(uint32_t K, ColWs ws, uint64_t* __restrict__ frameOff)
 {
     const uint32_t c = blockIdx.x;
@@ -314,7 +518,7 @@ __global__ void __launch_bounds__(64) k_col_plan(uint32_t K, ColWs ws, uint64_t*
     const int w = p.w;
     const uint8_t* x = reinterpret_cast<const uint8_t*>((uintptr_t)p.src);
     const uint64_t n = p.m + 1;
-    const uint64_t h = ozl_xxh3_wave(x, n * (uint64_t)w, nullptr, 0);
+    const uint64_t h = ws.hout[c];  // content hash (hash pass over the columns)
     if (threadIdx.x != 0) return;
     uint8_t* hb = ws.hdr + (size_t)c * kHdrSlot;
     const size_t fh = ozl_write_frame_header(hb, OZL_T_NUMERIC, n * (uint64_t)w, 1, 1);
@@ -366,88 +570,94 @@ __global__ void __launch_bounds__(kCT) k_col_scan_frames(uint32_t K, uint64_t* _
     if (threadIdx.x == 0) frameOff[K] = total;
 }
 
-// Packs 32 consecutive deltas per lane: their nb*32 bits are nb whole words
-// of the stream (items start at multiples of 32 deltas), stored as dwords at
-// the stream's byte position; a partial last group ends in byte stores.
+// An item's zigzag deltas go to LDS (coalesced loads); then every thread
+// builds whole 32-bit words of the item's bits from the deltas that overlap
+// them (items start at multiples of 32 deltas, so on word boundaries of the
+// stream) and consecutive threads store consecutive words.
 __global__ void __launch_bounds__(kCT) k_col_pack(uint32_t K, ColWs ws, uint8_t* __restrict__ dst,
                                                  const uint64_t* __restrict__ frameOff)
 {
     __shared__ uint32_t sItem, sCol;
+    __shared__ uint64_t Z[kTileMax];
     const uint32_t total = ws.ctr->totalItems;
     for (;;) {
         const uint32_t it = next_item(&ws.ctr->queue[1], total, &sItem);
         if (it == ~0u) break;
-        if (threadIdx.x == 0) sCol = ws.itemCol[it];
+        if (threadIdx.x == 0) sCol = item_col(ws)[it];
         __syncthreads();
         const uint32_t c = sCol;
         const ColPlan& P = ws.plans[c];
         if (P.status) continue;
-        const int w = P.w, nb = P.nb;
+        const int nb = P.nb;
         const uint32_t t = it - P.itemBase;
         const uint64_t d0 = (uint64_t)t * P.tile, d1 = d0 + P.tile < P.m ? d0 + P.tile : P.m;
+        const uint32_t cnt = (uint32_t)(d1 - d0);
         uint8_t* fr = dst + frameOff[c];
         if (t == 0 && threadIdx.x < P.hdrLen) fr[threadIdx.x] = ws.hdr[(size_t)c * kHdrSlot + threadIdx.x];
-        uint8_t* stream = fr + P.hdrLen;
-        const uint8_t* x = reinterpret_cast<const uint8_t*>((uintptr_t)P.src);
-        for (uint64_t g0 = d0 + 32 * (uint64_t)threadIdx.x; g0 < d1; g0 += 32 * (uint64_t)kCT) {
-            const uint32_t cnt = d1 - g0 < 32 ? (uint32_t)(d1 - g0) : 32u;
-            uint8_t* o = stream + g0 * (uint64_t)nb / 8;  // byte position of bit g0*nb (a multiple of 32 bits)
-            uint64_t acc = 0;
-            uint32_t bits = 0, wi = 0;
-            uint64_t prev = ld_elem(x + g0 * w, w);
-            for (uint32_t j = 0; j < cnt; j++) {
-                const uint64_t cur = ld_elem(x + (g0 + j + 1) * w, w);
-                const uint64_t z = zz_delta(prev, cur, w);
-                prev = cur;
-                // append nb bits (nb <= 64): low part then high part
-                const uint32_t n0 = nb < 32 ? (uint32_t)nb : 32u;
-                acc |= (z & ((1ull << n0) - 1)) << bits;
-                bits += n0;
-                if (bits >= 32) {
-                    *reinterpret_cast<uint32_t*>(o + 4 * wi) = (uint32_t)acc;
-                    wi++;
-                    acc >>= 32;
-                    bits -= 32;
-                }
-                if (nb > 32) {
-                    acc |= (z >> 32) << bits;
-                    bits += (uint32_t)nb - 32;
-                    if (bits >= 32) {
-                        *reinterpret_cast<uint32_t*>(o + 4 * wi) = (uint32_t)acc;
-                        wi++;
-                        acc >>= 32;
-                        bits -= 32;
-                    }
-                }
+        {
+            uint64_t z[kPer];
+            item_zz_w(P.w, reinterpret_cast<const uint8_t*>((uintptr_t)P.src), d0, d1, z);
+#pragma unroll
+            for (int k = 0; k < kPer; k++) {
+                const uint32_t i = threadIdx.x + kCT * k;
+                if (i < cnt) Z[i] = z[k];
             }
-            for (uint32_t k = 0; 8 * k < bits; k++) o[4 * wi + k] = (uint8_t)(acc >> (8 * k));
+        }
+        __syncthreads();
+        uint8_t* o = fr + P.hdrLen + d0 * (uint64_t)nb / 8;
+        const uint64_t nbits = (uint64_t)cnt * nb;
+        const uint32_t nW = (uint32_t)(nbits / 32), rem = (uint32_t)(nbits % 32);
+        for (uint32_t wi = threadIdx.x; wi < nW + (rem ? 1u : 0u); wi += kCT) {
+            const uint64_t bit0 = 32 * (uint64_t)wi;
+            uint32_t e = (uint32_t)(bit0 / nb);
+            const uint64_t bend = bit0 + 32 < nbits ? bit0 + 32 : nbits;
+            uint64_t word = 0;
+            for (; (uint64_t)e * nb < bend; e++) {
+                const int64_t off = (int64_t)e * nb - (int64_t)bit0;
+                const uint64_t v = Z[e];
+                word |= off >= 0 ? v << off : v >> (-off);
+            }
+            if (wi < nW) {
+                *reinterpret_cast<uint32_t*>(o + 4 * (size_t)wi) = (uint32_t)word;
+            } else {
+                for (uint32_t k = 0; 8 * k < rem; k++) o[4 * (size_t)wi + k] = (uint8_t)(word >> (8 * k));
+            }
         }
     }
 }
 
-// content checksum, compressed checksum (chunk + content checksum), end byte
-__global__ void __launch_bounds__(64) k_col_finish(uint32_t K, ColWs ws, uint8_t* __restrict__ dst,
-                                                  const uint64_t* __restrict__ frameOff, int* __restrict__ status)
+// tiny frames' bytes and every frame's content checksum (before the hash
+// pass over the chunks)
+__global__ void __launch_bounds__(64) k_col_finish1(uint32_t K, ColWs ws, uint8_t* __restrict__ dst,
+                                                   const uint64_t* __restrict__ frameOff)
 {
     const uint32_t c = blockIdx.x;
     if (c >= K) return;
-    const ColPlan p = ws.plans[c];
-    if (p.status) {
-        if (threadIdx.x == 0) status[c] = (int)p.status;
-        return;
-    }
+    const ColPlan& p = ws.plans[c];
+    if (p.status) return;
     uint8_t* fr = dst + frameOff[c];
     const uint64_t e = p.hdrLen + p.packed;
     if (p.kind != kBitpack && threadIdx.x < e) fr[threadIdx.x] = ws.hdr[(size_t)c * kHdrSlot + threadIdx.x];
     if (threadIdx.x < 4) fr[e + threadIdx.x] = (uint8_t)(p.hashes[0] >> (8 * threadIdx.x));
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __syncthreads();
-    const uint64_t h = ozl_xxh3_wave(fr + p.fhLen, e + 4 - p.fhLen, nullptr, 0);
-    if (threadIdx.x < 4) fr[e + 4 + threadIdx.x] = (uint8_t)(h >> (8 * threadIdx.x));
-    if (threadIdx.x == 0) {
+}
+
+// compressed checksum (hash pass over chunk + content checksum), end byte, status
+__global__ void __launch_bounds__(kCT) k_col_finish2(uint32_t K, ColWs ws, uint8_t* __restrict__ dst,
+                                                    const uint64_t* __restrict__ frameOff, int* __restrict__ status)
+{
+    const uint32_t c = blockIdx.x * kCT + threadIdx.x;
+    if (c >= K) return;
+    const ColPlan& p = ws.plans[c];
+    int st = (int)p.status;
+    if (!st && ws.ctr->error) st = OZL_ERR_PARAMETER_INVALID;
+    if (!st) {
+        uint8_t* fr = dst + frameOff[c];
+        const uint64_t e = p.hdrLen + p.packed;
+        const uint64_t h = ws.hout[c];
+        for (int k = 0; k < 4; k++) fr[e + 4 + k] = (uint8_t)(h >> (8 * k));
         fr[e + 8] = 0;
-        status[c] = 0;
     }
+    status[c] = st;
 }
 
 // ====================================================================== decode
@@ -531,7 +741,7 @@ __global__ void __launch_bounds__(64) k_cold_parse(const uint8_t* __restrict__ f
                     for (int k = 0; k < w; k++) out[(d + 1) * w + k] = (uint8_t)(v >> (8 * k));
                 }
             } else {
-                const uint32_t t = tile < kTileBytesMax / (uint32_t)w ? tile : kTileBytesMax / (uint32_t)w;
+                const uint32_t t = tile < kTileMax ? tile : kTileMax;
                 p.tile = t;
                 p.nItems = (uint32_t)((m + t - 1) / t);
             }
@@ -541,48 +751,26 @@ __global__ void __launch_bounds__(64) k_cold_parse(const uint8_t* __restrict__ f
     }
 }
 
-// unpacks the deltas of group [g0, g0+cnt) (g0 a multiple of 32: the group
-// starts on a word of the stream) into zigzag values, calling f(j, z)
-template <typename F>
-__device__ inline void unpack_group(const uint8_t* stream, uint64_t g0, uint32_t cnt, int nb, F f)
+// The packed words of an item (the stream bytes from bit d0*nb, a word
+// boundary) go to LDS with coalesced loads; one word more than the item's
+// bits is read (a value may straddle into it), which the frame's 9 trailing
+// bytes always back.
+__device__ inline uint32_t load_item_words(const uint8_t* stream, uint64_t d0, uint32_t cnt, int nb, uint32_t* Wd)
 {
-    const uint8_t* s = stream + g0 * (uint64_t)nb / 8;
-    const uint32_t nbytes = (cnt * (uint32_t)nb + 7) / 8;
-    uint64_t acc = 0;
-    uint32_t bits = 0, rd = 0;  // bytes consumed
-    auto refill = [&]() {
-        if (rd + 4 <= nbytes) {
-            acc |= (uint64_t)*reinterpret_cast<const uint32_t*>(s + rd) << bits;
-            rd += 4;
-            bits += 32;
-        } else {
-            while (rd < nbytes && bits <= 56) {
-                acc |= (uint64_t)s[rd] << bits;
-                rd++;
-                bits += 8;
-            }
-        }
-    };
-    for (uint32_t j = 0; j < cnt; j++) {
-        uint64_t z;
-        if (nb <= 32) {
-            while (bits < (uint32_t)nb) refill();
-            z = acc & ((1ull << nb) - 1);
-            acc >>= nb;
-            bits -= (uint32_t)nb;
-        } else {
-            while (bits < 32) refill();
-            z = acc & 0xFFFFFFFFull;
-            acc >>= 32;
-            bits -= 32;
-            const uint32_t hi = (uint32_t)nb - 32;
-            while (bits < hi) refill();
-            z |= (acc & ((1ull << hi) - 1)) << 32;
-            acc >>= hi;
-            bits -= hi;
-        }
-        f(j, z);
-    }
+    const uint8_t* sp = stream + d0 * (uint64_t)nb / 8;
+    const uint32_t nW = (uint32_t)(((uint64_t)cnt * nb + 31) / 32) + 1;
+    for (uint32_t k = threadIdx.x; k < nW; k += kCT) Wd[k] = *reinterpret_cast<const uint32_t*>(sp + 4 * (size_t)k);
+    return nW;
+}
+
+// zigzag value of delta e of the item (bits [e*nb, e*nb+nb) of the words)
+__device__ inline uint64_t item_value(const uint32_t* Wd, uint32_t e, int nb)
+{
+    const uint64_t bit = (uint64_t)e * nb;
+    const uint32_t q = (uint32_t)(bit >> 5), sh = (uint32_t)(bit & 31);
+    uint64_t v = ((uint64_t)Wd[q] | ((uint64_t)Wd[q + 1] << 32)) >> sh;
+    if (sh && nb + (int)sh > 64) v |= (uint64_t)Wd[q + 2] << (64 - sh);
+    return nb == 64 ? v : v & ((1ull << nb) - 1);
 }
 
 // tile sums of the deltas (mod 2^64; reduced mod 2^(8w) when used)
@@ -590,20 +778,24 @@ __global__ void __launch_bounds__(kCT) k_cold_sum(uint32_t K, ColWs ws)
 {
     __shared__ uint32_t sItem, sCol;
     __shared__ uint64_t lds[kCT / 64];
+    __shared__ __attribute__((aligned(16))) uint32_t Wd[2 * kTileMax + 4];
     const uint32_t total = ws.ctr->totalItems;
     for (;;) {
         const uint32_t it = next_item(&ws.ctr->queue[0], total, &sItem);
         if (it == ~0u) break;
-        if (threadIdx.x == 0) sCol = ws.itemCol[it];
+        if (threadIdx.x == 0) sCol = item_col(ws)[it];
         __syncthreads();
         const ColPlan& P = ws.plans[sCol];
         const int w = P.w, nb = P.nb;
         const uint64_t d0 = (uint64_t)(it - P.itemBase) * P.tile, d1 = d0 + P.tile < P.m ? d0 + P.tile : P.m;
-        const uint8_t* stream = reinterpret_cast<const uint8_t*>((uintptr_t)P.src);
+        const uint32_t cnt = (uint32_t)(d1 - d0);
+        load_item_words(reinterpret_cast<const uint8_t*>((uintptr_t)P.src), d0, cnt, nb, Wd);
+        __syncthreads();
         uint64_t s = 0;
-        for (uint64_t g0 = d0 + 32 * (uint64_t)threadIdx.x; g0 < d1; g0 += 32 * (uint64_t)kCT) {
-            const uint32_t cnt = d1 - g0 < 32 ? (uint32_t)(d1 - g0) : 32u;
-            unpack_group(stream, g0, cnt, nb, [&](uint32_t, uint64_t z) { s += unzz(z, w); });
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const uint32_t e = threadIdx.x + kCT * k;
+            if (e < cnt) s += unzz(item_value(Wd, e, nb), w);
         }
         s = block_sum<uint64_t>(s, lds);
         if (threadIdx.x == 0) tile_sums(ws, total)[it] = s;
@@ -633,74 +825,79 @@ __global__ void __launch_bounds__(64) k_cold_carry(uint32_t K, ColWs ws)
     }
 }
 
-// unpack, prefix sum within the item, add the item's carry, write elements
+// unpack, prefix sum within the item (thread t owns deltas [16t, 16t+16)),
+// add the item's carry; the elements are staged in LDS and leave in
+// consecutive dwords
 __global__ void __launch_bounds__(kCT) k_cold_apply(uint32_t K, ColWs ws, uint8_t* __restrict__ dst,
                                                    const uint64_t* __restrict__ doff)
 {
     __shared__ uint32_t sItem, sCol;
     __shared__ uint64_t lds[kCT / 64];
+    __shared__ __attribute__((aligned(16))) uint32_t Wd[2 * kTileMax + 4];
     const uint32_t total = ws.ctr->totalItems;
     for (;;) {
         const uint32_t it = next_item(&ws.ctr->queue[1], total, &sItem);
         if (it == ~0u) break;
-        if (threadIdx.x == 0) sCol = ws.itemCol[it];
+        if (threadIdx.x == 0) sCol = item_col(ws)[it];
         __syncthreads();
         const uint32_t c = sCol;
         const ColPlan& P = ws.plans[c];
         const int w = P.w, nb = P.nb;
         const uint32_t t = it - P.itemBase;
         const uint64_t d0 = (uint64_t)t * P.tile, d1 = d0 + P.tile < P.m ? d0 + P.tile : P.m;
-        const uint8_t* stream = reinterpret_cast<const uint8_t*>((uintptr_t)P.src);
+        const uint32_t cnt = (uint32_t)(d1 - d0);
         uint8_t* out = dst + doff[2 * c];
         const uint64_t mask = wmask(w);
-        if (t == 0 && threadIdx.x == 0) {
+        if (t == 0 && threadIdx.x == 0)
             for (int k = 0; k < w; k++) out[k] = (uint8_t)(P.maxZ >> (8 * k));
+        load_item_words(reinterpret_cast<const uint8_t*>((uintptr_t)P.src), d0, cnt, nb, Wd);
+        __syncthreads();
+        uint64_t v[kPer];
+        uint64_t run = 0;
+        const uint32_t e0 = kPer * threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const uint32_t e = e0 + k;
+            run += e < cnt ? unzz(item_value(Wd, e, nb), w) : 0;
+            v[k] = run;
         }
-        uint64_t base = tile_sums(ws, total)[it];
-        // rounds of kCT groups of 32 deltas
-        for (uint64_t r0 = d0; r0 < d1; r0 += 32 * (uint64_t)kCT) {
-            const uint64_t g0 = r0 + 32 * (uint64_t)threadIdx.x;
-            const uint32_t cnt = g0 < d1 ? (d1 - g0 < 32 ? (uint32_t)(d1 - g0) : 32u) : 0u;
-            uint64_t s = 0;
-            if (cnt) unpack_group(stream, g0, cnt, nb, [&](uint32_t, uint64_t z) { s += unzz(z, w); });
-            uint64_t tot;
-            uint64_t pre = block_excl<uint64_t>(s, lds, tot);
-            uint64_t v = base + pre;
-            if (cnt) {
-                uint8_t* o = out + (g0 + 1) * (uint64_t)w;
-                unpack_group(stream, g0, cnt, nb, [&](uint32_t j, uint64_t z) {
-                    v += unzz(z, w);
-                    const uint64_t e = v & mask;
-                    switch (w) {
-                    case 1: o[j] = (uint8_t)e; break;
-                    case 2: reinterpret_cast<uint16_t*>(o)[j] = (uint16_t)e; break;
-                    case 4: reinterpret_cast<uint32_t*>(o)[j] = (uint32_t)e; break;
-                    default: reinterpret_cast<uint64_t*>(o)[j] = e; break;
-                    }
-                });
+        uint64_t tot;
+        const uint64_t pre = block_excl<uint64_t>(run, lds, tot) + tile_sums(ws, total)[it];
+        __syncthreads();  // every thread is done with the words
+        uint8_t* xb = reinterpret_cast<uint8_t*>(Wd);
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const uint32_t e = e0 + k;
+            if (e < cnt) {
+                const uint64_t x = (pre + v[k]) & mask;
+                switch (w) {
+                case 1: xb[e] = (uint8_t)x; break;
+                case 2: reinterpret_cast<uint16_t*>(xb)[e] = (uint16_t)x; break;
+                case 4: reinterpret_cast<uint32_t*>(xb)[e] = (uint32_t)x; break;
+                default: reinterpret_cast<uint64_t*>(xb)[e] = x; break;
+                }
             }
-            base += tot;
         }
+        __syncthreads();
+        uint8_t* o = out + (d0 + 1) * (uint64_t)w;
+        const uint32_t nbytes = cnt * (uint32_t)w, nd = nbytes / 4;
+        for (uint32_t k = threadIdx.x; k < nd; k += kCT)
+            *reinterpret_cast<uint32_t*>(o + 4 * (size_t)k) = reinterpret_cast<const uint32_t*>(xb)[k];
+        for (uint32_t k = 4 * nd + threadIdx.x; k < nbytes; k += kCT) o[k] = xb[k];
     }
 }
 
-__global__ void __launch_bounds__(64) k_cold_verify(uint32_t K, ColWs ws, const uint8_t* __restrict__ frames,
-                                                   const uint64_t* __restrict__ foff, const uint8_t* __restrict__ dst,
-                                                   const uint64_t* __restrict__ doff, int* __restrict__ status)
+__global__ void __launch_bounds__(kCT) k_cold_verify(uint32_t K, ColWs ws, int* __restrict__ status)
 {
-    const uint32_t c = blockIdx.x;
+    const uint32_t c = blockIdx.x * kCT + threadIdx.x;
     if (c >= K) return;
-    const ColPlan p = ws.plans[c];
+    const ColPlan& p = ws.plans[c];
     int st = ws.ctr->error ? (int)OZL_ERR_PARAMETER_INVALID : (int)p.status;
     if (st == 0) {
-        const uint8_t* fr = frames + foff[c];
-        const uint64_t e = p.hdrLen + p.packed;
-        const uint64_t hc = ozl_xxh3_wave(fr + p.fhLen, e + 4 - p.fhLen, nullptr, 0);
-        const uint64_t hd = ozl_xxh3_wave(dst + doff[2 * c], (p.m + 1) * (uint64_t)p.w, nullptr, 0);
-        if ((uint32_t)hc != p.hashes[1]) st = OZL_ERR_COMPRESSED_CHECKSUM;
-        else if ((uint32_t)hd != p.hashes[0]) st = OZL_ERR_CONTENT_CHECKSUM;
+        if ((uint32_t)ws.hout[K + c] != p.hashes[1]) st = OZL_ERR_COMPRESSED_CHECKSUM;
+        else if ((uint32_t)ws.hout[c] != p.hashes[0]) st = OZL_ERR_CONTENT_CHECKSUM;
     }
-    if (threadIdx.x == 0) status[c] = st;
+    status[c] = st;
 }
 
 static int persistent_grid(uint32_t wgPerCu)
@@ -712,6 +909,19 @@ static int persistent_grid(uint32_t wgPerCu)
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
     }
     return cus * (int)(wgPerCu ? wgPerCu : 4);
+}
+
+// one-wave workgroups of the hash sums: 16 per CU
+static int persistent_grid_hash() { return persistent_grid(4) * 4; }
+
+// one hash pass over the R regions the caller's kernel has set
+static int hash_pass(uint32_t R, const ColWs& ws, int q, hipStream_t st)
+{
+    OZL_LAUNCH("k_hash_scan", k_hash_scan, dim3(1), dim3(kCT), 0, st, R, ws);
+    OZL_LAUNCH("k_hash_map", k_hash_map, dim3(R), dim3(64), 0, st, R, ws);
+    OZL_LAUNCH("k_hash_sums", k_hash_sums, dim3(persistent_grid_hash()), dim3(64), 0, st, ws, q);
+    OZL_LAUNCH("k_hash_chain", k_hash_chain, dim3(R), dim3(64), 0, st, R, ws);
+    return OZL_OK;
 }
 
 // ---------------------------------------------------------------- autotuner
@@ -730,13 +940,17 @@ std::map<TuneKey, std::pair<uint32_t, uint32_t>> g_tune;
 extern "C" {
 
 // Workspace bytes for K columns of `totalBytes` input bytes in all, cut into
-// items of `tile` deltas (64 B per column for the plan, 96 B of headers, 4 B
-// per item for the item map and 8 B for the decoder's tile sums).
+// items of `tile` deltas: per column the plan, headers and hash regions; per
+// item its map entry and the decoder's tile sum; per hashed 1 KiB block its
+// 64 bytes of stripe sums.
 size_t ozl_columns_workspace(uint32_t K, size_t totalBytes, uint32_t tile)
 {
     if (tile < 32) tile = 32;
     const size_t items = totalBytes / tile + K + 1;  // deltas <= bytes; one partial item per column
-    return 512 + (size_t)K * sizeof(ColPlan) + (((size_t)K * kHdrSlot + 255) & ~(size_t)255) + items * 12 + 256;
+    // hashed bytes: columns and frames (frames <= columns + 128 B each), per pass
+    const size_t hblocks = (2 * totalBytes + (size_t)128 * K) / 1024 + 2 * (size_t)K + 1;
+    const size_t hitems = hblocks / kHG + 2 * (size_t)K + 1;
+    return fixed_bytes(K) + items * 12 + 64 + hitems * 4 + 64 + hblocks * 64 + 256;
 }
 
 // Upper bound of the frame of a column of `bytes` input bytes.
@@ -753,22 +967,28 @@ int ozl_columns_compress_batch(const void* src, const uint64_t* colOff, const ui
 {
     if (K == 0) return OZL_OK;
     if (!src || !colOff || !widths || !dst || !frameOff || !status || !workspace) return OZL_ERR_PARAMETER_INVALID;
-    if (tile < 32 || (tile & 31) || wsBytes < 512 + (size_t)K * (sizeof(ColPlan) + kHdrSlot) + 256)
+    if (tile < 32 || (tile & 31) || wsBytes < fixed_bytes(K) + 1024)
         return OZL_ERR_PARAMETER_INVALID;
     hipStream_t st = (hipStream_t)stream;
-    ColWs ws = carve(workspace, K);
+    ColWs ws = carve(workspace, K, wsBytes);
     const uint8_t* s = (const uint8_t*)src;
+    uint8_t* d = (uint8_t*)dst;
     const int G = persistent_grid(wgPerCu);
-    OZL_LAUNCH("k_col_setup", k_col_setup, dim3((K + kCT - 1) / kCT), dim3(kCT), 0, st, s, colOff, widths, K, tile, ws);
-    OZL_LAUNCH("k_col_scan_items", k_col_scan_items, dim3(1), dim3(kCT), 0, st, K, ws,
-               ((uint64_t)(uintptr_t)workspace + wsBytes) | (1ull << 63));
+    const dim3 gk((K + kCT - 1) / kCT);
+    OZL_LAUNCH("k_col_setup", k_col_setup, gk, dim3(kCT), 0, st, s, colOff, widths, K, tile, ws);
+    OZL_LAUNCH("k_col_scan_items", k_col_scan_items, dim3(1), dim3(kCT), 0, st, K, ws);
     OZL_LAUNCH("k_col_map", k_col_map, dim3(K), dim3(64), 0, st, K, ws);
+    OZL_LAUNCH("k_hreg_content", k_hreg_content, gk, dim3(kCT), 0, st, K, ws);
+    hash_pass(K, ws, 2, st);
     OZL_LAUNCH("k_col_max", k_col_max, dim3(G), dim3(kCT), 0, st, K, ws);
     OZL_LAUNCH("k_col_plan", k_col_plan, dim3(K), dim3(64), 0, st, K, ws, frameOff);
     OZL_LAUNCH("k_col_scan_frames", k_col_scan_frames, dim3(1), dim3(kCT), 0, st, K, frameOff, dstCap, ws);
-    OZL_LAUNCH("k_col_pack", k_col_pack, dim3(G), dim3(kCT), 0, st, K, ws, (uint8_t*)dst, (const uint64_t*)frameOff);
-    OZL_LAUNCH("k_col_finish", k_col_finish, dim3(K), dim3(64), 0, st, K, ws, (uint8_t*)dst, (const uint64_t*)frameOff,
-               status);
+    OZL_LAUNCH("k_col_pack", k_col_pack, dim3(G), dim3(kCT), 0, st, K, ws, d, (const uint64_t*)frameOff);
+    OZL_LAUNCH("k_col_finish1", k_col_finish1, dim3(K), dim3(64), 0, st, K, ws, d, (const uint64_t*)frameOff);
+    OZL_LAUNCH("k_hreg_frames", k_hreg_frames, gk, dim3(kCT), 0, st, K, ws, (const uint8_t*)d,
+               (const uint64_t*)frameOff);
+    hash_pass(K, ws, 3, st);
+    OZL_LAUNCH("k_col_finish2", k_col_finish2, gk, dim3(kCT), 0, st, K, ws, d, (const uint64_t*)frameOff, status);
     OZL_HIP_CHECK(hipGetLastError());
     return OZL_OK;
 }
@@ -779,22 +999,23 @@ int ozl_columns_decompress_batch(const void* frames, const uint64_t* frameOff, u
 {
     if (K == 0) return OZL_OK;
     if (!frames || !frameOff || !dst || !dstOff || !status || !workspace) return OZL_ERR_PARAMETER_INVALID;
-    if (tile < 32 || (tile & 31) || wsBytes < 512 + (size_t)K * (sizeof(ColPlan) + kHdrSlot) + 256)
+    if (tile < 32 || (tile & 31) || wsBytes < fixed_bytes(K) + 1024)
         return OZL_ERR_PARAMETER_INVALID;
     hipStream_t st = (hipStream_t)stream;
-    ColWs ws = carve(workspace, K);
+    ColWs ws = carve(workspace, K, wsBytes);
     const int G = persistent_grid(wgPerCu);
     const uint8_t* f = (const uint8_t*)frames;
     uint8_t* d = (uint8_t*)dst;
+    const dim3 gk((K + kCT - 1) / kCT);
     OZL_LAUNCH("k_cold_parse", k_cold_parse, dim3(K), dim3(64), 0, st, f, frameOff, K, d, dstOff, tile, ws);
-    const uint64_t wsEnd = (uint64_t)(uintptr_t)workspace + wsBytes;
-    OZL_LAUNCH("k_col_scan_items", k_col_scan_items, dim3(1), dim3(kCT), 0, st, K, ws, wsEnd);
+    OZL_LAUNCH("k_col_scan_items", k_col_scan_items, dim3(1), dim3(kCT), 0, st, K, ws);
     OZL_LAUNCH("k_col_map", k_col_map, dim3(K), dim3(64), 0, st, K, ws);
     OZL_LAUNCH("k_cold_sum", k_cold_sum, dim3(G), dim3(kCT), 0, st, K, ws);
     OZL_LAUNCH("k_cold_carry", k_cold_carry, dim3(K), dim3(64), 0, st, K, ws);
     OZL_LAUNCH("k_cold_apply", k_cold_apply, dim3(G), dim3(kCT), 0, st, K, ws, d, dstOff);
-    OZL_LAUNCH("k_cold_verify", k_cold_verify, dim3(K), dim3(64), 0, st, K, ws, f, frameOff, (const uint8_t*)d, dstOff,
-               status);
+    OZL_LAUNCH("k_hreg_decode", k_hreg_decode, gk, dim3(kCT), 0, st, K, ws, f, frameOff, (const uint8_t*)d, dstOff);
+    hash_pass(2 * K, ws, 2, st);
+    OZL_LAUNCH("k_cold_verify", k_cold_verify, gk, dim3(kCT), 0, st, K, ws, status);
     OZL_HIP_CHECK(hipGetLastError());
     return OZL_OK;
 }
@@ -818,7 +1039,7 @@ int ozl_columns_autotune(const void* src, const uint64_t* colOff, const uint8_t*
         }
     }
     hipStream_t st = (hipStream_t)stream;
-    static const uint32_t tiles[] = { 1024, 2048, 4096, 8192, 16384 };
+    static const uint32_t tiles[] = { 1024, 2048, 4096 };
     static const uint32_t wgs[] = { 2, 4, 8 };
     hipEvent_t a, b;
     OZL_HIP_CHECK(hipEventCreate(&a));

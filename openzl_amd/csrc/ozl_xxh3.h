@@ -176,53 +176,69 @@ __device__ inline uint64_t ozl_dpp64(uint64_t v)
     uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, false);
     return ((uint64_t)hi << 32) | lo;
 }
-__device__ inline uint64_t ozl_xxh3_wave(const uint8_t* in, size_t len, uint64_t* lds, int chainCap)
+// Lane constants of the block step: lane l holds words 2j, 2j+1 (j = l mod 4)
+// of stripe l/4 and the accumulator pair 2j, 2j+1.
+struct ozl_xxh3_lane {
+    uint64_t k0, k1;  // stripe secrets of the lane's two words
+    uint64_t c0, c1;  // scramble secrets of accumulators 2j, 2j+1
+    int j;
+};
+__device__ inline ozl_xxh3_lane ozl_xxh3_lane_init()
 {
-    (void)lds;
-    (void)chainCap;
     const int lane = threadIdx.x & 63;
-    if (len <= 240) return ozl_xxh3_short(in, len);
-    const int j = lane & 3, st = lane >> 2, i0 = 2 * j;
-    const uint64_t k0 = ozl_sec64(8 * st + 8 * i0), k1 = ozl_sec64(8 * st + 8 * i0 + 8);
-    const uint64_t c0 = ozl_sec64(192 - 64 + 8 * i0), c1 = ozl_sec64(192 - 64 + 8 * i0 + 8);
-    uint64_t A = j == 0 ? OZL_P32_3 : j == 1 ? OZL_P64_2 : j == 2 ? OZL_P64_4 : OZL_P64_5;
-    uint64_t B = j == 0 ? OZL_P64_1 : j == 1 ? OZL_P64_3 : j == 2 ? OZL_P32_2 : OZL_P32_1;
-    const size_t nbBlocks = (len - 1) / 1024;
-    const bool al = ((uintptr_t)in & 15) == 0;
-    for (size_t b = 0; b < nbBlocks; b++) {
-        const uint8_t* p = in + b * 1024 + 16 * lane;
-        uint32_t d0, d1, d2, d3;
-        if (al) {
-            const uint4 q = *reinterpret_cast<const uint4*>(p);
-            d0 = q.x; d1 = q.y; d2 = q.z; d3 = q.w;
-        } else {
-            const uint32_t* pw = reinterpret_cast<const uint32_t*>(p);
-            d0 = pw[0]; d1 = pw[1]; d2 = pw[2]; d3 = pw[3];
-        }
-        const uint64_t v0 = d0 | ((uint64_t)d1 << 32), v1 = d2 | ((uint64_t)d3 << 32);
-        const uint64_t x0 = v0 ^ k0, x1 = v1 ^ k1;
-        uint64_t sa = v1 + (uint64_t)(uint32_t)x0 * (x0 >> 32);
-        uint64_t sb = v0 + (uint64_t)(uint32_t)x1 * (x1 >> 32);
-        sa += ozl_dpp64<0x124>(sa);  // row_ror:4
-        sb += ozl_dpp64<0x124>(sb);
-        sa += ozl_dpp64<0x128>(sa);  // row_ror:8
-        sb += ozl_dpp64<0x128>(sb);
-        sa += __shfl_xor(sa, 16, 64);
-        sb += __shfl_xor(sb, 16, 64);
-        sa += __shfl_xor(sa, 32, 64);
-        sb += __shfl_xor(sb, 32, 64);
-        A += sa;
-        B += sb;
-        A ^= A >> 47; A ^= c0; A *= OZL_P32_1;
-        B ^= B >> 47; B ^= c1; B *= OZL_P32_1;
+    ozl_xxh3_lane L;
+    L.j = lane & 3;
+    const int st = lane >> 2, i0 = 2 * L.j;
+    L.k0 = ozl_sec64(8 * st + 8 * i0);
+    L.k1 = ozl_sec64(8 * st + 8 * i0 + 8);
+    L.c0 = ozl_sec64(192 - 64 + 8 * i0);
+    L.c1 = ozl_sec64(192 - 64 + 8 * i0 + 8);
+    return L;
+}
+// sums over the 16 stripes of the 1 KiB block at `blk` for accumulators 2j
+// and 2j+1, in every lane of the pair's group
+__device__ inline void ozl_xxh3_block_sums(const uint8_t* blk, bool aligned, const ozl_xxh3_lane& L, uint64_t& sa,
+                                           uint64_t& sb)
+{
+    const uint8_t* p = blk + 16 * (threadIdx.x & 63);
+    uint32_t d0, d1, d2, d3;
+    if (aligned) {
+        const uint4 q = *reinterpret_cast<const uint4*>(p);
+        d0 = q.x; d1 = q.y; d2 = q.z; d3 = q.w;
+    } else {
+        const uint32_t* pw = reinterpret_cast<const uint32_t*>(p);
+        d0 = pw[0]; d1 = pw[1]; d2 = pw[2]; d3 = pw[3];
     }
-    // accumulator i to lane i, then the tail stripes and the last stripe
-    uint64_t acc = 0;
-    for (int q = 0; q < 4; q++) {
-        const uint64_t a = __shfl(A, q, 64), bq = __shfl(B, q, 64);
-        if (lane == 2 * q) acc = a;
-        if (lane == 2 * q + 1) acc = bq;
-    }
+    const uint64_t v0 = d0 | ((uint64_t)d1 << 32), v1 = d2 | ((uint64_t)d3 << 32);
+    const uint64_t x0 = v0 ^ L.k0, x1 = v1 ^ L.k1;
+    sa = v1 + (uint64_t)(uint32_t)x0 * (x0 >> 32);
+    sb = v0 + (uint64_t)(uint32_t)x1 * (x1 >> 32);
+    sa += ozl_dpp64<0x124>(sa);  // row_ror:4
+    sb += ozl_dpp64<0x124>(sb);
+    sa += ozl_dpp64<0x128>(sa);  // row_ror:8
+    sb += ozl_dpp64<0x128>(sb);
+    sa += __shfl_xor(sa, 16, 64);
+    sb += __shfl_xor(sb, 16, 64);
+    sa += __shfl_xor(sa, 32, 64);
+    sb += __shfl_xor(sb, 32, 64);
+}
+__device__ inline uint64_t ozl_xxh3_scr(uint64_t a, uint64_t c)
+{
+    a ^= a >> 47;
+    a ^= c;
+    return a * OZL_P32_1;
+}
+// initial accumulator i
+__device__ inline uint64_t ozl_xxh3_init_acc(int i)
+{
+    return i == 0 ? OZL_P32_3 : i == 1 ? OZL_P64_1 : i == 2 ? OZL_P64_2 : i == 3 ? OZL_P64_3
+         : i == 4 ? OZL_P64_4 : i == 5 ? OZL_P32_2 : i == 6 ? OZL_P64_5 : OZL_P32_1;
+}
+// lanes 0..7 hold accumulator `lane` after the full blocks; adds the tail
+// stripes and the last stripe and returns the hash in every lane
+__device__ inline uint64_t ozl_xxh3_finish(const uint8_t* in, size_t len, size_t nbBlocks, uint64_t acc)
+{
+    const int lane = threadIdx.x & 63;
     if (lane < 8) {
         size_t nbStripes = ((len - 1) - 1024 * nbBlocks) / 64;
         for (size_t s = 0; s < nbStripes; s++) acc += ozl_stripe_term(in + nbBlocks * 1024 + s * 64, (int)s * 8, lane);
@@ -231,5 +247,31 @@ __device__ inline uint64_t ozl_xxh3_wave(const uint8_t* in, size_t len, uint64_t
     uint64_t a[8];
     for (int i = 0; i < 8; i++) a[i] = __shfl(acc, i, 64);
     return ozl_xxh3_merge(a, len);
+}
+
+__device__ inline uint64_t ozl_xxh3_wave(const uint8_t* in, size_t len, uint64_t* lds, int chainCap)
+{
+    (void)lds;
+    (void)chainCap;
+    const int lane = threadIdx.x & 63;
+    if (len <= 240) return ozl_xxh3_short(in, len);
+    const ozl_xxh3_lane L = ozl_xxh3_lane_init();
+    uint64_t A = ozl_xxh3_init_acc(2 * L.j), B = ozl_xxh3_init_acc(2 * L.j + 1);
+    const size_t nbBlocks = (len - 1) / 1024;
+    const bool al = ((uintptr_t)in & 15) == 0;
+    for (size_t b = 0; b < nbBlocks; b++) {
+        uint64_t sa, sb;
+        ozl_xxh3_block_sums(in + b * 1024, al, L, sa, sb);
+        A = ozl_xxh3_scr(A + sa, L.c0);
+        B = ozl_xxh3_scr(B + sb, L.c1);
+    }
+    // accumulator i to lane i, then the tail stripes and the last stripe
+    uint64_t acc = 0;
+    for (int q = 0; q < 4; q++) {
+        const uint64_t a = __shfl(A, q, 64), bq = __shfl(B, q, 64);
+        if (lane == 2 * q) acc = a;
+        if (lane == 2 * q + 1) acc = bq;
+    }
+    return ozl_xxh3_finish(in, len, nbBlocks, acc);
 }
 #endif
