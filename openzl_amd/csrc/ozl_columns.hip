@@ -16,8 +16,8 @@
 // of input per item); each pass over items is one persistent launch whose
 // workgroups take items from a device counter until none is left:
 //   encode  k_col_setup -> k_col_scan_items -> k_col_map -> hash pass over the
-//           columns (content checksums) -> k_col_max (largest zigzag delta per
-//           column) -> k_col_plan (nbBits, headers) -> k_col_scan_frames ->
+//           columns (content checksums, and the largest zigzag delta of each
+//           column from the same reads) -> k_col_plan (nbBits, headers) -> k_col_scan_frames ->
 //           k_col_pack (bits straight into the frames) -> k_col_finish1 ->
 //           hash pass over the chunks (compressed checksums) -> k_col_finish2
 //   decode  k_cold_parse -> k_col_scan_items -> k_col_map -> k_cold_sum (tile
@@ -88,6 +88,9 @@ struct ColCtr {
 struct HashReg {
     uint64_t ptr, len;
     uint32_t blockBase, nBlocks, itemBase, nItems;
+    uint32_t zw;    // content pass over a bitpacked column: its width (the pass also
+                    // takes the column's largest zigzag delta), else 0
+    uint32_t pad;
 };
 constexpr uint32_t kHG = 16;
 
@@ -266,10 +269,11 @@ __global__ void __launch_bounds__(64) k_col_map(uint32_t K, ColWs ws)
 // 16 stripes of each block (ozl_xxh3_block_sums), k_hash_chain runs each
 // region's scramble chain over its block sums, the tail stripes and the merge
 // (one wave per region) into hout.
-__device__ inline void hreg_set(HashReg& r, uint64_t ptr, uint64_t len)
+__device__ inline void hreg_set(HashReg& r, uint64_t ptr, uint64_t len, uint32_t zw = 0)
 {
     r.ptr = ptr;
     r.len = len;
+    r.zw = zw;
     r.nBlocks = len > 240 ? (uint32_t)((len - 1) / 1024) : 0u;
     r.nItems = (r.nBlocks + kHG - 1) / kHG;
     r.blockBase = r.itemBase = 0;
@@ -280,7 +284,7 @@ __global__ void __launch_bounds__(kCT) k_hreg_content(uint32_t K, ColWs ws)
     const uint32_t c = blockIdx.x * kCT + threadIdx.x;
     if (c >= K) return;
     const ColPlan& p = ws.plans[c];
-    hreg_set(ws.hreg[c], p.src, p.status ? 0 : (p.m + 1) * (uint64_t)p.w);
+    hreg_set(ws.hreg[c], p.src, p.status ? 0 : (p.m + 1) * (uint64_t)p.w, p.status || p.kind != kBitpack ? 0u : p.w);
 }
 
 // compressed checksum input: the chunk and the content checksum
@@ -358,6 +362,7 @@ __global__ void __launch_bounds__(64) k_hash_sums(ColWs ws, int q)
         const uint32_t b0 = (it - h.itemBase) * kHG, b1 = b0 + kHG < h.nBlocks ? b0 + kHG : h.nBlocks;
         const uint8_t* in = reinterpret_cast<const uint8_t*>((uintptr_t)h.ptr);
         const bool al = (h.ptr & 15) == 0;
+        unsigned long long mx = 0;
         for (uint32_t b = b0; b < b1; b++) {
             uint64_t sa, sb;
             ozl_xxh3_block_sums(in + (size_t)b * 1024, al, L, sa, sb);
@@ -366,6 +371,40 @@ __global__ void __launch_bounds__(64) k_hash_sums(ColWs ws, int q)
                 o[0] = sa;
                 o[1] = sb;
             }
+            if (h.zw) {
+                // deltas whose first element is in this lane's 16 bytes
+                const int w = (int)h.zw, nE = 16 / w;
+                const uint8_t* pc = in + (size_t)b * 1024 + 16 * lane;
+                const uint32_t* pw = reinterpret_cast<const uint32_t*>(pc);
+                const uint64_t v0 = pw[0] | ((uint64_t)pw[1] << 32), v1 = pw[2] | ((uint64_t)pw[3] << 32);
+                const uint64_t msk = wmask(w);
+                uint64_t nxt = (uint64_t)__shfl_down((unsigned long long)(v0 & msk), 1, 64);
+                const uint64_t nOff = (uint64_t)(b + 1) * 1024;
+                bool hasNext = true;
+                if (lane == 63) {
+                    hasNext = nOff < h.len;
+                    nxt = hasNext ? ld_elem(in + nOff, w) : 0;
+                }
+                uint64_t prev = v0 & msk;
+                for (int k = 1; k < nE; k++) {
+                    const int off = 8 * w * k;
+                    const uint64_t cur = (off < 64 ? v0 >> off : v1 >> (off - 64)) & msk;
+                    const uint64_t z = zz_delta(prev, cur, w);
+                    mx = z > mx ? z : mx;
+                    prev = cur;
+                }
+                if (hasNext) {
+                    const uint64_t z = zz_delta(prev, nxt, w);
+                    mx = z > mx ? z : mx;
+                }
+            }
+        }
+        if (h.zw) {
+            for (int d = 32; d >= 1; d >>= 1) {
+                const unsigned long long o = __shfl_xor(mx, d, 64);
+                mx = o > mx ? o : mx;
+            }
+            if (lane == 0 && mx) atomicMax(&ws.plans[map[it]].maxZ, mx);
         }
     }
 }
@@ -380,6 +419,21 @@ __global__ void __launch_bounds__(64) k_hash_chain(uint32_t R, ColWs ws)
     if (h.len == 0 || (ws.ctr->error & 2u)) {
         if (lane == 0) ws.hout[r] = 0;
         return;
+    }
+    if (h.zw) {
+        // deltas whose first element follows the summed blocks
+        const int w = (int)h.zw;
+        const uint64_t n = h.len / (uint64_t)w, i0 = (uint64_t)h.nBlocks * 1024 / (uint64_t)w;
+        unsigned long long mx = 0;
+        for (uint64_t i = i0 + lane; i + 1 < n; i += 64) {
+            const uint64_t z = zz_delta(ld_elem(in + i * w, w), ld_elem(in + (i + 1) * w, w), w);
+            mx = z > mx ? z : mx;
+        }
+        for (int d = 32; d >= 1; d >>= 1) {
+            const unsigned long long o = __shfl_xor(mx, d, 64);
+            mx = o > mx ? o : mx;
+        }
+        if (lane == 0 && mx) atomicMax(&ws.plans[r].maxZ, mx);
     }
     uint64_t hv;
     if (h.len <= 240) {
@@ -435,32 +489,6 @@ __device__ inline void item_zz_w(int w, const uint8_t* x, uint64_t d0, uint64_t 
     case 2: item_zz<2>(x, d0, d1, z); break;
     case 4: item_zz<4>(x, d0, d1, z); break;
     default: item_zz<8>(x, d0, d1, z); break;
-    }
-}
-
-// largest zigzag delta per column
-__global__ void __launch_bounds__(kCT) k_col_max(uint32_t K, ColWs ws)
-{
-    __shared__ uint32_t sItem, sCol;
-    const uint32_t total = ws.ctr->totalItems;
-    for (;;) {
-        const uint32_t it = next_item(&ws.ctr->queue[0], total, &sItem);
-        if (it == ~0u) break;
-        if (threadIdx.x == 0) sCol = item_col(ws)[it];
-        __syncthreads();
-        const ColPlan& P = ws.plans[sCol];
-        const uint64_t d0 = (uint64_t)(it - P.itemBase) * P.tile, d1 = d0 + P.tile < P.m ? d0 + P.tile : P.m;
-        uint64_t z[kPer];
-        item_zz_w(P.w, reinterpret_cast<const uint8_t*>((uintptr_t)P.src), d0, d1, z);
-        unsigned long long mx = 0;
-#pragma unroll
-        for (int k = 0; k < kPer; k++)
-            if (d0 + threadIdx.x + (uint64_t)kCT * k < d1) mx = z[k] > mx ? z[k] : mx;
-        for (int s = 32; s >= 1; s >>= 1) {
-            const unsigned long long o = __shfl_xor(mx, s, 64);
-            mx = o > mx ? o : mx;
-        }
-        if ((threadIdx.x & 63) == 0 && mx) atomicMax(&ws.plans[sCol].maxZ, mx);
     }
 }
 
@@ -980,7 +1008,6 @@ int ozl_columns_compress_batch(const void* src, const uint64_t* colOff, const ui
     OZL_LAUNCH("k_col_map", k_col_map, dim3(K), dim3(64), 0, st, K, ws);
     OZL_LAUNCH("k_hreg_content", k_hreg_content, gk, dim3(kCT), 0, st, K, ws);
     hash_pass(K, ws, 2, st);
-    OZL_LAUNCH("k_col_max", k_col_max, dim3(G), dim3(kCT), 0, st, K, ws);
     OZL_LAUNCH("k_col_plan", k_col_plan, dim3(K), dim3(64), 0, st, K, ws, frameOff);
     OZL_LAUNCH("k_col_scan_frames", k_col_scan_frames, dim3(1), dim3(kCT), 0, st, K, frameOff, dstCap, ws);
     OZL_LAUNCH("k_col_pack", k_col_pack, dim3(G), dim3(kCT), 0, st, K, ws, d, (const uint64_t*)frameOff);
