@@ -264,6 +264,75 @@ int ozl_entropy_decompress_batch(const void* frames, const uint64_t* frameOffset
                                  const uint64_t* dstOffsets, int* status, void* workspace, size_t workspaceSize,
                                  void* stream);
 
+/* Batched numeric columns, one frame per column, with the graph
+ * ZL_NODE_DELTA_INT -> ZL_NODE_ZIGZAG -> ZL_GRAPH_BITPACK: the frame
+ * ZL_CCtx_compressTypedRef (include/openzl/zl_compress.h:274) writes for a
+ * numeric ZL_TypedRef of each column with that graph
+ * (ZL_Compressor_registerStaticGraph_fromNode1o, zl_compressor.h), format
+ * version 23, both checksums on. Column c is
+ * src[colSpans[2c], colSpans[2c] + colSpans[2c+1]) of element width
+ * widths[c] (1, 2, 4 or 8; start aligned to the width). `tile` is the number
+ * of deltas per work item (multiple of 32, at most 4096), wgPerCu the
+ * persistent workgroups per CU (0 = 4). Device pointers; no host
+ * synchronisation and no memset, so a call can be captured in a hipGraph. */
+size_t ozl_columns_workspace(uint32_t nbColumns, size_t totalBytes, uint32_t tile);
+size_t ozl_columns_frame_bound(size_t columnBytes);
+int ozl_columns_compress_batch(const void* src, const uint64_t* colSpans, const uint8_t* widths, uint32_t nbColumns,
+                               void* dst, size_t dstCapacity, uint64_t* frameOffsets, int* status, void* workspace,
+                               size_t workspaceSize, uint32_t tile, uint32_t wgPerCu, void* stream);
+/* ZL_decompress (include/openzl/zl_decompress.h:32) of such frames into
+ * dst[dstSpans[2c], +dstSpans[2c+1]); status[c] 0 or the frame's error code
+ * (corruption, checksum, a frame of another layout). */
+int ozl_columns_decompress_batch(const void* frames, const uint64_t* frameOffsets, uint32_t nbColumns, void* dst,
+                                 const uint64_t* dstSpans, int* status, void* workspace, size_t workspaceSize,
+                                 uint32_t tile, uint32_t wgPerCu, void* stream);
+/* Times compress calls over tiles {1024, 2048, 4096} x workgroups per CU
+ * {2, 4, 8} on the caller's batch (HIP events) and returns the fastest; the
+ * result is cached per (nbColumns, totalBytes). No reference counterpart. */
+int ozl_columns_autotune(const void* src, const uint64_t* colSpans, const uint8_t* widths, uint32_t nbColumns,
+                         uint64_t totalBytes, void* dst, size_t dstCapacity, uint64_t* frameOffsets, int* status,
+                         void* workspace, size_t workspaceSize, uint32_t* tileOut, uint32_t* wgPerCuOut,
+                         void* stream);
+/* A hipGraph holding one compress (dir 0: a = src, aOff = colSpans, b = dst,
+ * bOff = frameOffsets) or decompress (dir 1: a = frames, aOff =
+ * frameOffsets, b = dst, bOff = dstSpans) call with fixed arguments;
+ * ozl_graph_launch replays it on `stream`. No reference counterpart. */
+void* ozl_columns_graph_create(int dir, const void* a, const uint64_t* aOff, const uint8_t* widths,
+                               uint32_t nbColumns, void* b, size_t bCap, uint64_t* bOff, int* status, void* workspace,
+                               size_t workspaceSize, uint32_t tile, uint32_t wgPerCu);
+int ozl_graph_launch(void* graph, void* stream);
+void ozl_graph_destroy(void* graph);
+
+/* Learned graph selection over eight candidate graphs for numeric chunks
+ * (< 100000 bytes each): 0 ZL_GRAPH_STORE, 1 ZL_GRAPH_FSE, 2 ZL_GRAPH_BITPACK,
+ * 3 DELTA_INT -> FSE, 4 DELTA_INT -> ZIGZAG -> BITPACK, 5 transpose_split ->
+ * FSE, 6 transpose_split -> HUFFMAN, 7 DELTA_INT -> transpose_split -> FSE.
+ * ozl_select_trial is the brute-force selector over them
+ * (src/openzl/compress/selectors/selector_brute_force.c:9-50,
+ * ZL_Compressor_registerBruteForceSelectorGraph :52-66): each candidate's
+ * frame size with checksums off (sizes, K x 8) and the selected candidate
+ * (choice). ozl_select_compress writes each chunk's frame with candidate
+ * choice[c] (format version 23, checksums on): the reference's frame for that
+ * graph, and the brute-force selector's frame when choice comes from the
+ * trial. ozl_select_features / ozl_select_mlp are the learned selector (the
+ * counterpart of the GBT selector, src/openzl/compress/selectors/ml/
+ * mlselector.c): 40 features per chunk and a 40-64-64-8 MLP in bf16 whose
+ * argmax is the choice. Device pointers; trial and compress synchronise the
+ * host once per call. */
+size_t ozl_select_workspace(uint32_t nbChunks, size_t maxChunk);
+size_t ozl_select_frame_bound(size_t chunkBytes);
+int ozl_select_trial(const void* src, const uint64_t* spans, const uint8_t* widths, uint32_t nbChunks,
+                     size_t maxChunk, uint32_t* sizes, uint8_t* choice, void* workspace, size_t workspaceSize,
+                     void* stream);
+int ozl_select_compress(const void* src, const uint64_t* spans, const uint8_t* widths, uint32_t nbChunks,
+                        size_t maxChunk, const uint8_t* choice, void* dst, size_t dstCapacity,
+                        uint64_t* frameOffsets, int* status, void* workspace, size_t workspaceSize, void* stream);
+size_t ozl_select_mlp_params_bytes(void);
+int ozl_select_features(const void* src, const uint64_t* spans, const uint8_t* widths, uint32_t nbChunks,
+                        float* features, void* stream);
+int ozl_select_mlp(const float* features, uint32_t nbChunks, const void* params, uint8_t* choice, float* logits,
+                   void* stream);
+
 /* Per-kernel timing (HIP events around every launch while enabled; no
  * reference counterpart). ozl_profile_end writes "<tag><kernel> <launches>
  * <total ms>" lines; ozl_profile_tag sets the tag of later launches. */

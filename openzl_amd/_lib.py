@@ -54,6 +54,13 @@ def lib():
         "ozl_columns_autotune": (I, [P, P, P, ctypes.c_uint32, ctypes.c_uint64, P, SZ, P, P, P, SZ, P, P, P]),
         "ozl_columns_graph_create": (P, [I, P, P, P, ctypes.c_uint32, P, SZ, P, P, P, SZ, ctypes.c_uint32,
                                          ctypes.c_uint32]),
+        "ozl_select_workspace": (SZ, [ctypes.c_uint32, SZ]),
+        "ozl_select_frame_bound": (SZ, [SZ]),
+        "ozl_select_trial": (I, [P, P, P, ctypes.c_uint32, SZ, P, P, P, SZ, P]),
+        "ozl_select_compress": (I, [P, P, P, ctypes.c_uint32, SZ, P, P, SZ, P, P, P, SZ, P]),
+        "ozl_select_mlp_params_bytes": (SZ, []),
+        "ozl_select_features": (I, [P, P, P, ctypes.c_uint32, P, P]),
+        "ozl_select_mlp": (I, [P, ctypes.c_uint32, P, P, P, P]),
         "ozl_graph_launch": (I, [P, P]),
         "ozl_graph_destroy": (None, [P]),
     }

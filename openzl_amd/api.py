@@ -34,7 +34,7 @@ G_STORE, G_FSE, G_HUFFMAN, G_BITPACK = GraphID(2), GraphID(3), GraphID(4), Graph
 N_DELTA, N_TRANSPOSE_SPLIT, N_ZIGZAG = NodeID(2), NodeID(3), NodeID(4)
 
 # graph kinds, same numbering as the reference harness (oracle/ref_harness.c)
-STORE, FSE, HUFFMAN, BITPACK, DELTA_TS_FSE, DELTA_ZZ_BP, TS_HUF_FSE, DELTA_FSE, TS_FSE, TS_HUF, ZZ_BP, DELTA_BP = range(12)
+STORE, FSE, HUFFMAN, BITPACK, DELTA_TS_FSE, DELTA_ZZ_BP, TS_HUF_FSE, DELTA_FSE, TS_FSE, TS_HUF, ZZ_BP, DELTA_BP, SELECT8 = range(13)
 
 _setup = False
 

@@ -135,15 +135,7 @@ __device__ inline uint64_t* hash_sums(const ColWs& ws, uint32_t items, uint32_t 
     return reinterpret_cast<uint64_t*>(((uintptr_t)(hash_map(ws, items) + hItems) + 63) & ~(uintptr_t)63);
 }
 
-__device__ inline uint64_t ld_elem(const uint8_t* p, int w)
-{
-    switch (w) {
-    case 1: return *p;
-    case 2: return *reinterpret_cast<const uint16_t*>(p);
-    case 4: return *reinterpret_cast<const uint32_t*>(p);
-    default: return *reinterpret_cast<const uint64_t*>(p);
-    }
-}
+__device__ inline uint64_t ld_elem(const uint8_t* p, int w) { return ozl_gld_w(p, w); }
 
 __device__ inline uint64_t wmask(int w) { return w == 8 ? ~0ull : (1ull << (8 * w)) - 1; }
 
@@ -363,31 +355,49 @@ __global__ void __launch_bounds__(64) k_hash_sums(ColWs ws, int q)
         const uint8_t* in = reinterpret_cast<const uint8_t*>((uintptr_t)h.ptr);
         const bool al = (h.ptr & 15) == 0;
         unsigned long long mx = 0;
-        for (uint32_t b = b0; b < b1; b++) {
+        // the item's blocks are read up front (addresses clamped into the
+        // item, so every load is issued unconditionally), then summed
+        uint64_t bw[kHG][2];
+#pragma unroll
+        for (uint32_t k = 0; k < kHG; k++) {
+            const uint32_t b = b0 + k < b1 ? b0 + k : b1 - 1;
+            ozl_xxh3_block_load(in + (size_t)b * 1024, al, bw[k][0], bw[k][1]);
+        }
+        const uint64_t msk = wmask(h.zw ? (int)h.zw : 1);
+        const int w = (int)h.zw;
+        // the element after the item's last block, for the last delta
+        uint64_t after = 0;
+        const uint64_t nOff = (uint64_t)b1 * 1024;
+        const bool hasAfter = w && nOff < h.len;
+        if (hasAfter && lane == 63) after = ld_elem(in + nOff, w);
+#pragma unroll
+        for (uint32_t k = 0; k < kHG; k++) {
+            if (b0 + k >= b1) break;
+            const uint32_t b = b0 + k;
             uint64_t sa, sb;
-            ozl_xxh3_block_sums(in + (size_t)b * 1024, al, L, sa, sb);
+            ozl_xxh3_sums_of(bw[k][0], bw[k][1], L, sa, sb);
             if (lane < 4) {
                 uint64_t* o = sums + 8 * ((size_t)h.blockBase + b) + 2 * L.j;
-                o[0] = sa;
-                o[1] = sb;
+                ozl_gst8(o, sa);
+                ozl_gst8(o + 1, sb);
             }
-            if (h.zw) {
+            if (w) {
                 // deltas whose first element is in this lane's 16 bytes
-                const int w = (int)h.zw, nE = 16 / w;
-                const uint8_t* pc = in + (size_t)b * 1024 + 16 * lane;
-                const uint32_t* pw = reinterpret_cast<const uint32_t*>(pc);
-                const uint64_t v0 = pw[0] | ((uint64_t)pw[1] << 32), v1 = pw[2] | ((uint64_t)pw[3] << 32);
-                const uint64_t msk = wmask(w);
+                const int nE = 16 / w;
+                const uint64_t v0 = bw[k][0], v1 = bw[k][1];
                 uint64_t nxt = (uint64_t)__shfl_down((unsigned long long)(v0 & msk), 1, 64);
-                const uint64_t nOff = (uint64_t)(b + 1) * 1024;
+                // first element of the next block (lane 0's first word)
+                const uint64_t nb0 =
+                    k + 1 < kHG ? (uint64_t)__shfl((unsigned long long)(bw[k + 1 < kHG ? k + 1 : k][0] & msk), 0, 64) : 0;
                 bool hasNext = true;
                 if (lane == 63) {
-                    hasNext = nOff < h.len;
-                    nxt = hasNext ? ld_elem(in + nOff, w) : 0;
+                    const bool last = b0 + k + 1 >= b1;
+                    hasNext = last ? hasAfter : true;
+                    nxt = last ? after : nb0;
                 }
                 uint64_t prev = v0 & msk;
-                for (int k = 1; k < nE; k++) {
-                    const int off = 8 * w * k;
+                for (int e = 1; e < nE; e++) {
+                    const int off = 8 * w * e;
                     const uint64_t cur = (off < 64 ? v0 >> off : v1 >> (off - 64)) & msk;
                     const uint64_t z = zz_delta(prev, cur, w);
                     mx = z > mx ? z : mx;
@@ -469,14 +479,13 @@ template <int W>
 __device__ inline void item_zz(const uint8_t* x, uint64_t d0, uint64_t d1, uint64_t* z)
 {
     using T = typename ElemT<W>::T;
-    const T* e = reinterpret_cast<const T*>(x);
     T a[kPer], b[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
         const uint64_t d = d0 + threadIdx.x + (uint64_t)kCT * k;
         const uint64_t dd = d < d1 ? d : d0;
-        a[k] = e[dd];
-        b[k] = e[dd + 1];
+        a[k] = (T)ozl_gld_w(x + dd * W, W);
+        b[k] = (T)ozl_gld_w(x + (dd + 1) * W, W);
     }
 #pragma unroll
     for (int k = 0; k < kPer; k++) z[k] = zz_delta((uint64_t)a[k], (uint64_t)b[k], W);
@@ -621,7 +630,7 @@ __global__ void __launch_bounds__(kCT) k_col_pack(uint32_t K, ColWs ws, uint8_t*
         const uint64_t d0 = (uint64_t)t * P.tile, d1 = d0 + P.tile < P.m ? d0 + P.tile : P.m;
         const uint32_t cnt = (uint32_t)(d1 - d0);
         uint8_t* fr = dst + frameOff[c];
-        if (t == 0 && threadIdx.x < P.hdrLen) fr[threadIdx.x] = ws.hdr[(size_t)c * kHdrSlot + threadIdx.x];
+        if (t == 0 && threadIdx.x < P.hdrLen) ozl_gst1(fr + threadIdx.x, ws.hdr[(size_t)c * kHdrSlot + threadIdx.x]);
         {
             uint64_t z[kPer];
             item_zz_w(P.w, reinterpret_cast<const uint8_t*>((uintptr_t)P.src), d0, d1, z);
@@ -646,9 +655,9 @@ __global__ void __launch_bounds__(kCT) k_col_pack(uint32_t K, ColWs ws, uint8_t*
                 word |= off >= 0 ? v << off : v >> (-off);
             }
             if (wi < nW) {
-                *reinterpret_cast<uint32_t*>(o + 4 * (size_t)wi) = (uint32_t)word;
+                ozl_gst4(o + 4 * (size_t)wi, (uint32_t)word);
             } else {
-                for (uint32_t k = 0; 8 * k < rem; k++) o[4 * (size_t)wi + k] = (uint8_t)(word >> (8 * k));
+                for (uint32_t k = 0; 8 * k < rem; k++) ozl_gst1(o + 4 * (size_t)wi + k, (uint32_t)(word >> (8 * k)));
             }
         }
     }

@@ -9,6 +9,7 @@
 //                  only the per-block scramble chain is sequential.
 #pragma once
 #include "ozl_core.h"
+#include "ozl_gmem.h"
 
 #define OZL_XXH3_SECRET_BYTES \
     0xb8, 0xfe, 0x6c, 0x39, 0x23, 0xa4, 0x4b, 0xbe, 0x7c, 0x01, 0x81, 0x2c, 0xf7, 0x21, 0xad, 0x1c, \
@@ -195,21 +196,24 @@ __device__ inline ozl_xxh3_lane ozl_xxh3_lane_init()
     L.c1 = ozl_sec64(192 - 64 + 8 * i0 + 8);
     return L;
 }
-// sums over the 16 stripes of the 1 KiB block at `blk` for accumulators 2j
-// and 2j+1, in every lane of the pair's group
-__device__ inline void ozl_xxh3_block_sums(const uint8_t* blk, bool aligned, const ozl_xxh3_lane& L, uint64_t& sa,
-                                           uint64_t& sb)
+// this lane's 16 bytes of the 1 KiB block at `blk`, as two words
+__device__ inline void ozl_xxh3_block_load(const uint8_t* blk, bool aligned, uint64_t& v0, uint64_t& v1)
 {
     const uint8_t* p = blk + 16 * (threadIdx.x & 63);
     uint32_t d0, d1, d2, d3;
     if (aligned) {
-        const uint4 q = *reinterpret_cast<const uint4*>(p);
+        const ozl_u32x4 q = ozl_gld16(p);
         d0 = q.x; d1 = q.y; d2 = q.z; d3 = q.w;
     } else {
-        const uint32_t* pw = reinterpret_cast<const uint32_t*>(p);
-        d0 = pw[0]; d1 = pw[1]; d2 = pw[2]; d3 = pw[3];
+        d0 = ozl_gld4(p); d1 = ozl_gld4(p + 4); d2 = ozl_gld4(p + 8); d3 = ozl_gld4(p + 12);
     }
-    const uint64_t v0 = d0 | ((uint64_t)d1 << 32), v1 = d2 | ((uint64_t)d3 << 32);
+    v0 = d0 | ((uint64_t)d1 << 32);
+    v1 = d2 | ((uint64_t)d3 << 32);
+}
+// sums over the 16 stripes of a 1 KiB block for accumulators 2j and 2j+1,
+// in every lane of the pair's group, from the words ozl_xxh3_block_load read
+__device__ inline void ozl_xxh3_sums_of(uint64_t v0, uint64_t v1, const ozl_xxh3_lane& L, uint64_t& sa, uint64_t& sb)
+{
     const uint64_t x0 = v0 ^ L.k0, x1 = v1 ^ L.k1;
     sa = v1 + (uint64_t)(uint32_t)x0 * (x0 >> 32);
     sb = v0 + (uint64_t)(uint32_t)x1 * (x1 >> 32);
@@ -221,6 +225,13 @@ __device__ inline void ozl_xxh3_block_sums(const uint8_t* blk, bool aligned, con
     sb += __shfl_xor(sb, 16, 64);
     sa += __shfl_xor(sa, 32, 64);
     sb += __shfl_xor(sb, 32, 64);
+}
+__device__ inline void ozl_xxh3_block_sums(const uint8_t* blk, bool aligned, const ozl_xxh3_lane& L, uint64_t& sa,
+                                           uint64_t& sb)
+{
+    uint64_t v0, v1;
+    ozl_xxh3_block_load(blk, aligned, v0, v1);
+    ozl_xxh3_sums_of(v0, v1, L, sa, sb);
 }
 __device__ inline uint64_t ozl_xxh3_scr(uint64_t a, uint64_t c)
 {

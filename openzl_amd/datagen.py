@@ -138,3 +138,55 @@ def zipf_blocks_torch(torch, n_blocks, seed=SEEDS[2], device="cuda"):
     return out
 
 
+
+
+# ---------------------------------------------------------------- learned-selector chunks
+SELECT_TRAIN_SEED = 0x7EA10001
+SELECT_CHUNK = 65536
+SELECT_GENERATORS = ("timestamps_u32", "zipf_u16", "bf16_normal", "uniform_u8", "const_runs_u32",
+                     "sawtooth_u16", "ids_u64", "walk_i32")
+SELECT_WIDTHS = (4, 2, 2, 1, 4, 2, 8, 4)
+
+
+def _select_chunk(g, rng, nbytes):
+    w = SELECT_WIDTHS[g]
+    n = nbytes // w
+    if g == 0:    # sorted uint32 timestamps, Poisson(1000) increments (mod 2**32)
+        x = np.cumsum(rng.poisson(1000, n).astype(np.uint64)) + np.uint64(rng.integers(1_600_000_000, 1_800_000_000))
+        return (x & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    if g == 1:    # Zipf(s=1.2) categoricals: 1,000 distinct uint16 codes
+        p = 1.0 / np.arange(1, 1001, dtype=np.float64) ** 1.2
+        codes = rng.choice(65536, 1000, replace=False).astype(np.uint16)
+        return codes[rng.choice(1000, n, p=p / p.sum())]
+    if g == 2:    # bf16 N(0, 1), round to nearest even
+        f = rng.standard_normal(n).astype(np.float32).view(np.uint32)
+        return ((f + ((f >> 16) & 1) + 0x7FFF) >> 16).astype(np.uint16)
+    if g == 3:    # uint8 uniform
+        return rng.integers(0, 256, n, dtype=np.uint8)
+    if g == 4:    # uint32 constant runs, geometric run length of mean 64
+        runs = rng.geometric(1.0 / 64, n)
+        k = int(np.searchsorted(np.cumsum(runs), n)) + 1
+        vals = rng.integers(0, 1 << 32, k, dtype=np.uint64).astype(np.uint32)
+        return np.repeat(vals, runs[:k])[:n]
+    if g == 5:    # uint16 sawtooth plus noise
+        period = int(rng.integers(16, 2049))
+        slope = int(rng.integers(1, 65))
+        base = int(rng.integers(0, 65536))
+        i = np.arange(n, dtype=np.int64)
+        v = base + (i % period) * slope + rng.integers(-4, 5, n)
+        return (v & 0xFFFF).astype(np.uint16)
+    if g == 6:    # uint64 IDs: fixed high 44 bits, 20 random low bits
+        hi = np.uint64(int(rng.integers(0, 1 << 44)) << 20)
+        return hi | rng.integers(0, 1 << 20, n, dtype=np.uint64)
+    # int32 Gaussian random walk, step round(N(0, 2**b)), b in 0..12
+    b = int(rng.integers(0, 13))
+    walk = np.cumsum(np.rint(rng.normal(0.0, float(2 ** b), n)).astype(np.int64))
+    return (walk & 0xFFFFFFFF).astype(np.uint32)
+
+
+def selector_chunks(n_chunks=8192, seed=SEEDS[5], nbytes=SELECT_CHUNK):
+    """Chunks of the learned-selector configuration: each from one of the eight
+    generators chosen uniformly; returns (generator ids, list of arrays)."""
+    rng = np.random.default_rng(seed)
+    gens = rng.integers(0, 8, n_chunks)
+    return gens, [_select_chunk(int(g), rng, nbytes) for g in gens]

@@ -39,6 +39,8 @@ enum {
     REFH_TS_HUF         = 9,  /* transpose_split -> HUFFMAN per plane            */
     REFH_ZZ_BP          = 10, /* ZIGZAG -> BITPACK                               */
     REFH_DELTA_BP       = 11, /* DELTA_INT -> BITPACK                            */
+    REFH_SELECT8        = 12, /* brute-force selector over the 8 candidates of   */
+                              /* the learned-selector batch (see buildGraph)     */
 };
 
 static ZL_Report splitHufFse(ZL_Graph* g, ZL_Edge* inputs[], size_t nbIns)
@@ -86,6 +88,18 @@ static ZL_GraphID buildGraph(ZL_Compressor* c, int kind)
         return ZL_Compressor_registerStaticGraph_fromNode1o(c, ZL_NODE_ZIGZAG, ZL_GRAPH_BITPACK);
     case REFH_DELTA_BP:
         return ZL_Compressor_registerStaticGraph_fromNode1o(c, ZL_NODE_DELTA_INT, ZL_GRAPH_BITPACK);
+    case REFH_SELECT8: {
+        /* candidate order: STORE, FSE, BITPACK, DELTA->FSE, DELTA->ZIGZAG->BITPACK,
+         * transpose_split->FSE, transpose_split->HUFFMAN, DELTA->transpose_split->FSE */
+        static const int kinds[8] = { REFH_STORE, REFH_FSE, REFH_BITPACK, REFH_DELTA_FSE, REFH_DELTA_ZZ_BP,
+                                      REFH_TS_FSE, REFH_TS_HUF, REFH_DELTA_TS_FSE };
+        ZL_GraphID cand[8];
+        for (int i = 0; i < 8; i++) {
+            cand[i] = buildGraph(c, kinds[i]);
+            if (!ZL_GraphID_isValid(cand[i])) return ZL_GRAPH_ILLEGAL;
+        }
+        return ZL_Compressor_registerBruteForceSelectorGraph(c, cand, 8);
+    }
     default: return ZL_GRAPH_ILLEGAL;
     }
 }

@@ -157,7 +157,7 @@ def ptr(a):
 
 
 # graph kinds understood by refh_compress (oracle/ref_harness.c)
-STORE, FSE, HUFFMAN, BITPACK, DELTA_TS_FSE, DELTA_ZZ_BP, TS_HUF_FSE, DELTA_FSE, TS_FSE, TS_HUF, ZZ_BP, DELTA_BP = range(12)
+STORE, FSE, HUFFMAN, BITPACK, DELTA_TS_FSE, DELTA_ZZ_BP, TS_HUF_FSE, DELTA_FSE, TS_FSE, TS_HUF, ZZ_BP, DELTA_BP, SELECT8 = range(13)
 SERIAL, STRUCT, NUMERIC = 0, 1, 2
 
 
