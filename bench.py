@@ -283,6 +283,147 @@ def bench_columns(args):
     print(json.dumps(rec))
 
 
+def cpu_baseline_select(chunks, threads, budget_s):
+    """The reference library compressing chunks with its brute-force selector
+    over the same eight candidate graphs, one context per thread, cycling over
+    the chunks for `budget_s` seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ozl_checkers as C
+    if C.refh() is None:
+        return None
+    h = C.refh()
+    t_start = time.perf_counter()
+
+    def worker(tid, res):
+        nbytes = 0
+        i = tid
+        while time.perf_counter() - t_start < budget_s:
+            c = chunks[i % len(chunks)]
+            cap = h.refh_compressBound(c.nbytes) + 4096
+            out = np.empty(cap, np.uint8)
+            r = h.refh_compress(C.SELECT8, C.NUMERIC, c.dtype.itemsize, c.ctypes.data, c.size, out.ctypes.data,
+                                cap, 1, 1)
+            assert r > 0
+            nbytes += c.nbytes
+            i += threads
+        res[tid] = nbytes
+
+    res = [0] * threads
+    ths = [threading.Thread(target=worker, args=(t, res)) for t in range(threads)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    wall = time.perf_counter() - t_start
+    total = sum(res)
+    return {"value": round(total / wall / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "reference",
+            "sample": f"chunks of the same workload cycled, each compressed by the reference's brute-force "
+                      f"selector over the eight candidate graphs, one context per thread; {total / 1e9:.3f} GB "
+                      f"in {wall:.1f} s wall"}
+
+
+def bench_select(args):
+    """Learned graph selection: 8,192 chunks of 64 KiB from eight generators;
+    the MLP's choice (features + MLP + compression with the chosen candidate)
+    against exhaustive trial (the brute-force selector over the eight
+    candidates, then compression with its choice)."""
+    import torch
+    from openzl_amd import select as S, _lib
+    from openzl_amd import datagen as G
+    from openzl_amd import selector_train as T
+    L = _lib.lib()
+    print("[bench] generating the 8,192 chunks", file=sys.stderr, flush=True)
+    gens, chunks = G.selector_chunks(8192, seed=G.SEEDS[5])
+    src, spans, widths, _ = S.pack_chunks(chunks)
+    U = sum(c.nbytes for c in chunks)
+    K = len(chunks)
+    sb = S.SelectorBatch(spans, widths, G.SELECT_CHUNK)
+    params = S.load_params()
+
+    def exhaustive():
+        _, ch = sb.trial(src)
+        return sb.compress(src, ch)
+
+    def learned():
+        ch, _ = sb.predict(src, params)
+        return sb.compress(src, ch)
+
+    sizes, ex_choice = sb.trial(src)
+    sizes = sizes.cpu().numpy().astype(np.int64)
+    ex_choice = ex_choice.cpu().numpy().copy()
+    dst, fo, st = exhaustive()
+    torch.cuda.synchronize()
+    assert int(st.abs().sum()) == 0
+    C_ex = int(fo[-1].item())
+    ml_choice = sb.predict(src, params)[0].cpu().numpy().copy()
+    dst, fo, st = learned()
+    torch.cuda.synchronize()
+    assert int(st.abs().sum()) == 0
+    C_ml = int(fo[-1].item())
+    raw = np.array([c.nbytes for c in chunks], np.int64)
+    loss = T.ratio_loss(sizes, raw, ml_choice)
+    agree = float(np.mean(ml_choice == ex_choice))
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1000 / args.steps
+
+    print("[bench] timing", file=sys.stderr, flush=True)
+    ms_ml = timed(learned)
+    ms_ex = timed(exhaustive)
+    prof = {}
+    if not args.no_profile:
+        L.ozl_profile_begin()
+        L.ozl_profile_tag(b"learned/")
+        learned()
+        L.ozl_profile_tag(b"exhaustive/")
+        exhaustive()
+        torch.cuda.synchronize()
+        bufp = ctypes.create_string_buffer(1 << 16)
+        L.ozl_profile_end.restype = ctypes.c_size_t
+        L.ozl_profile_end.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+        L.ozl_profile_end(bufp, len(bufp))
+        for line in bufp.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            prof[name] = round(float(ms), 4)
+        L.ozl_profile_tag(b"")
+    ml_gbs = (U + C_ml) / (ms_ml / 1000) / 1e9
+    roofline = {"bound": "hbm", "kernel": "whole learned path (features, MLP, compression with the choice)",
+                "achieved": round(ml_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ml_gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel_ms_per_call": prof}
+    cpu = None
+    if args.cpu_sample > 0:
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = max(1, min(threads, 16))
+        cpu = cpu_baseline_select(chunks[:512], threads, args.cpu_seconds)
+    rec = {
+        "metric": METRIC, "value": round(U / (ms_ml / 1000) / 1e9, 3), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_ml, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8/u16/u32/u64",
+        "data": "synthetic (8,192 x 64 KiB chunks, eight seeded generators chosen uniformly)",
+        "config": {
+            "workload": "learned graph selection: 8192 x 64 KiB numeric chunks, MLP 40-64-64-8 (bf16 MFMA) over "
+                        "eight candidate graphs, against exhaustive trial (brute-force selector); format v23, "
+                        "checksums on",
+            "chunks": K, "input_bytes": U, "frame_bytes_learned": C_ml, "frame_bytes_exhaustive": C_ex,
+            "ratio_learned": round(U / C_ml, 4), "ratio_exhaustive": round(U / C_ex, 4),
+            "ratio_loss_vs_exhaustive": round(loss, 7), "choice_agreement": round(agree, 4),
+            "learned_ms": round(ms_ml, 4), "exhaustive_ms": round(ms_ex, 4),
+            "learned_GBps": round(U / (ms_ml / 1000) / 1e9, 3), "exhaustive_GBps": round(U / (ms_ex / 1000) / 1e9, 3),
+            "learned_speedup_vs_exhaustive": round(ms_ex / ms_ml, 3),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec))
+
+
 # ---------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
@@ -295,9 +436,10 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 3 device-to-device copies of 1 GiB (HBM counter calibration) and exit")
-    ap.add_argument("--workload", choices=["entropy", "columns"], default="entropy",
+    ap.add_argument("--workload", choices=["entropy", "columns", "select"], default="entropy",
                     help="entropy: the 16,384-block entropy stress (the headline line); columns: the batched "
-                         "4,096-column delta/zigzag/bitpack workload")
+                         "4,096-column delta/zigzag/bitpack workload; select: learned graph selection over "
+                         "8,192 chunks against exhaustive trial")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus})",
@@ -306,6 +448,8 @@ def main():
 
     if args.workload == "columns":
         return bench_columns(args)
+    if args.workload == "select":
+        return bench_select(args)
     import torch
     from openzl_amd import batch, _lib
 
