@@ -18,7 +18,18 @@ __device__ inline uint32_t ozl_gld2(const void* p) { return *(const OZL_GLOBAL(u
 __device__ inline uint32_t ozl_gld1(const void* p) { return *(const OZL_GLOBAL(uint8_t)*)(uintptr_t)p; }
 __device__ inline void ozl_gst8(void* p, uint64_t v) { *(OZL_GLOBAL(uint64_t)*)(uintptr_t)p = v; }
 __device__ inline void ozl_gst4(void* p, uint32_t v) { *(OZL_GLOBAL(uint32_t)*)(uintptr_t)p = v; }
+__device__ inline void ozl_gst2(void* p, uint32_t v) { *(OZL_GLOBAL(uint16_t)*)(uintptr_t)p = (uint16_t)v; }
 __device__ inline void ozl_gst1(void* p, uint32_t v) { *(OZL_GLOBAL(uint8_t)*)(uintptr_t)p = (uint8_t)v; }
+// element of width w (1, 2, 4, 8) to p
+__device__ inline void ozl_gst_w(void* p, uint64_t v, int w)
+{
+    switch (w) {
+    case 1: ozl_gst1(p, (uint32_t)v); break;
+    case 2: ozl_gst2(p, (uint32_t)v); break;
+    case 4: ozl_gst4(p, (uint32_t)v); break;
+    default: ozl_gst8(p, v); break;
+    }
+}
 // element of width w (1, 2, 4, 8) at p
 __device__ inline uint64_t ozl_gld_w(const void* p, int w)
 {

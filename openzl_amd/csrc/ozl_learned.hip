@@ -30,21 +30,13 @@
 #include "ozl_internal.h"
 #include "ozl_core.h"
 #include "ozl_profile.h"
+#include "ozl_gmem.h"
 
 namespace {
 
 constexpr int kF = 40;
 constexpr int kT = 256;
 
-__device__ inline uint64_t ld_el(const uint8_t* p, int w)
-{
-    switch (w) {
-    case 1: return *p;
-    case 2: return *reinterpret_cast<const uint16_t*>(p);
-    case 4: return *reinterpret_cast<const uint32_t*>(p);
-    default: return *reinterpret_cast<const uint64_t*>(p);
-    }
-}
 __device__ inline int bits64(uint64_t v) { return v ? 64 - __builtin_clzll(v) : 0; }
 
 template <typename T>
@@ -76,68 +68,138 @@ __device__ T wg_max(T v, T* lds)
 // c * log2(c) for one histogram bin
 __device__ inline float clog(uint32_t c) { return c ? (float)c * __log2f((float)c) : 0.f; }
 
-// one workgroup per chunk
-__global__ void __launch_bounds__(kT) k_features(const uint8_t* __restrict__ src, const uint64_t* __restrict__ spans,
-                                                 const uint8_t* __restrict__ widths, uint32_t K,
-                                                 float* __restrict__ feats)
+template <int W> struct ElemT;
+template <> struct ElemT<1> { using T = uint8_t; };
+template <> struct ElemT<2> { using T = uint16_t; };
+template <> struct ElemT<4> { using T = uint32_t; };
+template <> struct ElemT<8> { using T = uint64_t; };
+
+// Byte-plane histograms: W planes x 256 bins x (32 / W) copies in 32 KB of
+// LDS, bin-major (the copy index is the low part of the address, so lanes of
+// one wave that hit the same bin mostly hit different banks). Thread t owns
+// the elements [16/W * v, 16/W * (v + 1)) of vectors v = t, t + 256, ...;
+// equal bytes that follow each other in a plane are added as one count.
+constexpr int kHistWords = 8192;
+
+template <int W>
+__device__ inline void plane_add(uint32_t* hc, int p, uint32_t b, uint32_t cnt)
 {
-    __shared__ uint32_t hx[8][256];
-    __shared__ uint32_t hd[8][256];
-    __shared__ float lf[kT / 64];
-    __shared__ uint32_t lu[kT / 64];
-    __shared__ unsigned long long lz[kT / 64];
-    const uint32_t c = blockIdx.x;
-    if (c >= K) return;
-    const int w = widths[c];
-    const uint64_t bytes = spans[2 * c + 1];
-    const uint8_t* x = src + spans[2 * c];
-    float* f = feats + (size_t)c * kF;
-    const bool ok = (w == 1 || w == 2 || w == 4 || w == 8) && bytes >= (uint64_t)w && bytes % w == 0;
-    if (!ok) {
-        if (threadIdx.x < kF) f[threadIdx.x] = 0.f;
-        return;
-    }
-    const uint32_t n = (uint32_t)(bytes / w), m = n - 1;
-    for (int i = threadIdx.x; i < 8 * 256; i += kT) {
-        (&hx[0][0])[i] = 0;
-        (&hd[0][0])[i] = 0;
-    }
-    __syncthreads();
-    uint32_t zeros = 0;
-    unsigned long long bz = 0;
-    uint32_t mz = 0, mx = 0;
-    const uint64_t msk = w == 8 ? ~0ull : ((1ull << (8 * w)) - 1);
-    for (uint32_t i = threadIdx.x; i < n; i += kT) {
-        const uint64_t v = ld_el(x + (size_t)i * w, w);
-        for (int k = 0; k < w; k++) atomicAdd(&hx[k][(v >> (8 * k)) & 255], 1u);
-        const uint32_t bv = (uint32_t)bits64(v);
-        mx = bv > mx ? bv : mx;
-        if (i < m) {
-            const uint64_t d = (ld_el(x + (size_t)(i + 1) * w, w) - v) & msk;
-            const int sb = 8 * w - 1;
-            const uint64_t s = (uint64_t)((int64_t)(d << (63 - sb)) >> 63);
-            const uint64_t z = ((d << 1) ^ s) & msk;
-            zeros += d == 0;
-            const uint32_t b = (uint32_t)bits64(z);
-            bz += b;
-            mz = b > mz ? b : mz;
-            for (int k = 0; k < w; k++) atomicAdd(&hd[k][(d >> (8 * k)) & 255], 1u);
+    constexpr int C = 32 / W;
+    atomicAdd(&hc[(p * 256 + (int)b) * C], cnt);
+}
+
+template <int W, bool DELTA>
+__device__ void plane_hist(const uint8_t* x, uint32_t n, uint32_t* h, uint32_t& zeros, unsigned long long& bz,
+                           uint32_t& mb)
+{
+    using T = typename ElemT<W>::T;
+    constexpr int C = 32 / W, E = 16 / W;
+    uint32_t* hc = h + (threadIdx.x & (C - 1));
+    const uint32_t cnt = DELTA ? n - 1 : n;   // values: elements or deltas
+    const uint32_t nv = (cnt + E - 1) / E;
+    for (uint32_t v = threadIdx.x; v < nv; v += kT) {
+        const uint32_t e0 = v * E;
+        T val[E];
+        int nvalid = 0;
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const uint32_t e = e0 + j;
+            const uint32_t ee = e < cnt ? e : e0;
+            const T a = (T)ozl_gld_w(x + (size_t)ee * W, W);
+            if (DELTA) {
+                const T b = (T)ozl_gld_w(x + (size_t)(ee + 1) * W, W);
+                val[j] = (T)(b - a);
+            } else {
+                val[j] = a;
+            }
+            nvalid += e < cnt;
+        }
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            if (j < nvalid) {
+                const uint64_t u = (uint64_t)val[j];
+                if (DELTA) {
+                    const uint64_t s = (uint64_t)((int64_t)(u << (64 - 8 * W)) >> 63);
+                    const uint64_t z = ((u << 1) ^ s) & (W == 8 ? ~0ull : ((1ull << (8 * W)) - 1));
+                    zeros += u == 0;
+                    const uint32_t b = (uint32_t)bits64(z);
+                    bz += b;
+                    mb = b > mb ? b : mb;
+                } else {
+                    const uint32_t b = (uint32_t)bits64(u);
+                    mb = b > mb ? b : mb;
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < W; p++) {
+            uint32_t cur = (uint32_t)((uint64_t)val[0] >> (8 * p)) & 255u, run = 1;
+#pragma unroll
+            for (int j = 1; j < E; j++) {
+                if (j < nvalid) {
+                    const uint32_t b = (uint32_t)((uint64_t)val[j] >> (8 * p)) & 255u;
+                    if (b == cur) {
+                        run++;
+                    } else {
+                        plane_add<W>(hc, p, cur, run);
+                        cur = b;
+                        run = 1;
+                    }
+                }
+            }
+            plane_add<W>(hc, p, cur, run);
         }
     }
-    __syncthreads();
-    // thread v owns byte value v
-    const int v = threadIdx.x;
-    uint32_t all = 0;
-    for (int k = 0; k < w; k++) all += hx[k][v];
-    float sAll = clog(all), sX = 0.f, sD = 0.f;
-    for (int k = 0; k < w; k++) {
-        sX += clog(hx[k][v]);
-        sD += clog(hd[k][v]);
+}
+
+// sum of c log2 c over the W planes of h for bin v (and the total count of v)
+template <int W>
+__device__ inline float plane_clog(const uint32_t* h, int v, uint32_t& all)
+{
+    constexpr int C = 32 / W;
+    float s = 0.f;
+    all = 0;
+    for (int p = 0; p < W; p++) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int c = 0; c < C; c++) t += h[(p * 256 + v) * C + c];
+        s += clog(t);
+        all += t;
     }
-    // 32-bin histogram: 8 consecutive values per bin
+    return s;
+}
+
+template <int W>
+__device__ void features_w(const uint8_t* x, uint32_t n, float* f, uint32_t* h, float* lf, uint32_t* lu,
+                           unsigned long long* lz)
+{
+    const uint32_t m = n - 1;
+    const uint64_t bytes = (uint64_t)n * W;
+    const int v = threadIdx.x;
+    uint32_t zeros = 0, mx = 0, mz = 0;
+    unsigned long long bz = 0;
+    // element planes
+    for (int i = threadIdx.x; i < kHistWords; i += kT) h[i] = 0;
+    __syncthreads();
+    plane_hist<W, false>(x, n, h, zeros, bz, mx);
+    __syncthreads();
+    uint32_t all;
+    float sX = plane_clog<W>(h, v, all);
+    float sAll = clog(all);
     uint32_t bin = all;
     for (int d = 1; d < 8; d <<= 1) bin += __shfl_xor(bin, d, 64);
     if ((v & 7) == 0) f[v >> 3] = (float)bin / (float)bytes;
+    __syncthreads();
+    // delta planes
+    float sD = 0.f;
+    if (m) {
+        for (int i = threadIdx.x; i < kHistWords; i += kT) h[i] = 0;
+        __syncthreads();
+        plane_hist<W, true>(x, n, h, zeros, bz, mz);
+        __syncthreads();
+        uint32_t dall;
+        sD = plane_clog<W>(h, v, dall);
+    }
     sAll = wg_sum<float>(sAll, lf);
     sX = wg_sum<float>(sX, lf);
     sD = wg_sum<float>(sD, lf);
@@ -149,14 +211,44 @@ __global__ void __launch_bounds__(kT) k_features(const uint8_t* __restrict__ src
         // entropy of N samples = log2 N - (sum c log2 c) / N
         const float B = (float)bytes, N = (float)n, M = (float)m;
         f[32] = (__log2f(B) - sAll / B) / 8.f;
-        f[33] = (float)ozl_next_pow2((uint64_t)w) / 3.f;
+        f[33] = (float)ozl_next_pow2((uint64_t)W) / 3.f;
         f[34] = m ? (float)zeros / M : 0.f;
         f[35] = m ? (float)bz / M / 64.f : 0.f;
         f[36] = (float)mz / 64.f;
         f[37] = (float)mx / 64.f;
-        // per plane: log2(count) - sum/count, averaged over the w planes
-        f[38] = m ? (__log2f(M) - sD / ((float)w * M)) / 8.f : 0.f;
-        f[39] = (__log2f(N) - sX / ((float)w * N)) / 8.f;
+        // per plane: log2(count) - sum/count, averaged over the W planes
+        f[38] = m ? (__log2f(M) - sD / ((float)W * M)) / 8.f : 0.f;
+        f[39] = (__log2f(N) - sX / ((float)W * N)) / 8.f;
+    }
+}
+
+// one workgroup per chunk
+__global__ void __launch_bounds__(kT) k_features(const uint8_t* __restrict__ src, const uint64_t* __restrict__ spans,
+                                                 const uint8_t* __restrict__ widths, uint32_t K,
+                                                 float* __restrict__ feats)
+{
+    __shared__ uint32_t h[kHistWords];
+    __shared__ float lf[kT / 64];
+    __shared__ uint32_t lu[kT / 64];
+    __shared__ unsigned long long lz[kT / 64];
+    const uint32_t c = blockIdx.x;
+    if (c >= K) return;
+    const int w = widths[c];
+    const uint64_t bytes = spans[2 * c + 1];
+    const uint8_t* x = src + spans[2 * c];
+    float* f = feats + (size_t)c * kF;
+    const bool ok = (w == 1 || w == 2 || w == 4 || w == 8) && bytes >= (uint64_t)w && bytes % w == 0 &&
+                    bytes < (1ull << 31);
+    if (!ok) {
+        if (threadIdx.x < kF) f[threadIdx.x] = 0.f;
+        return;
+    }
+    const uint32_t n = (uint32_t)(bytes / w);
+    switch (w) {
+    case 1: features_w<1>(x, n, f, h, lf, lu, lz); break;
+    case 2: features_w<2>(x, n, f, h, lf, lu, lz); break;
+    case 4: features_w<4>(x, n, f, h, lf, lu, lz); break;
+    default: features_w<8>(x, n, f, h, lf, lu, lz); break;
     }
 }
 

@@ -46,6 +46,7 @@
 #include "ozl_frame.h"
 #include "ozl_entropy_api.h"
 #include "ozl_profile.h"
+#include "ozl_gmem.h"
 
 #include <string.h>
 
@@ -82,6 +83,13 @@ struct SelCtr {
     uint32_t pad[5];
 };
 
+struct SelFrame {
+    uint32_t hlen, nbStored;
+    uint8_t hdr[1024];       // frame header + chunk header
+    uint64_t ptr[80];        // stored streams in frame order (at most the layout's stream count)
+    uint32_t size[80];
+};
+
 struct SelWs {
     SelCtr* ctr;
     SelChunk* ch;
@@ -91,7 +99,7 @@ struct SelWs {
     uint8_t* stg;
     uint8_t* pack;
     uint8_t* ews;            // entropy workspaces
-    uint64_t* hashes;        // content hash per chunk
+    struct SelFrame* fr;     // header bytes and stored streams of each chunk's frame (compress)
     size_t stgCap, packCap, ewsCap;
 };
 
@@ -199,29 +207,44 @@ __device__ T wg_excl(T v, T* lds, T& total)
 }
 
 // exclusive prefixes of the group bytes, group blocks and pack bytes, then
-// the group bases and every chunk's staging offsets and block numbers
+// the group bases and every chunk's staging offsets and block numbers. Thread
+// t sums a contiguous range of chunks, one workgroup scan of the seven
+// totals, then each thread writes its range's prefixes.
 __global__ void __launch_bounds__(kT) k_sel_scan(uint32_t K, SelWs ws)
 {
     __shared__ uint64_t l64[kT / 64];
     __shared__ uint32_t l32[kT / 64];
-    uint64_t cb[3] = { 0, 0, 0 }, cp = 0;
-    uint32_t cbl[3] = { 0, 0, 0 };
-    for (uint32_t base = 0; base < K; base += kT) {
-        const uint32_t c = base + threadIdx.x;
-        SelChunk* h = c < K ? &ws.ch[c] : nullptr;
+    const uint32_t per = (K + kT - 1) / kT;
+    const uint32_t c0 = threadIdx.x * per, c1 = c0 + per < K ? c0 + per : K;
+    uint64_t sb[3] = { 0, 0, 0 }, sp = 0;
+    uint32_t sbl[3] = { 0, 0, 0 };
+    for (uint32_t c = c0; c < c1; c++) {
+        const SelChunk& h = ws.ch[c];
+        for (int g = 0; g < 3; g++) { sb[g] += h.gBytes[g]; sbl[g] += h.gBlocks[g]; }
+        sp += h.packBytes;
+    }
+    uint64_t cb[3], cp;
+    uint32_t cbl[3];
+    uint64_t eb[3], ep;
+    uint32_t ebl[3];
+    for (int g = 0; g < 3; g++) {
+        eb[g] = wg_excl<uint64_t>(sb[g], l64, cb[g]);
+        ebl[g] = wg_excl<uint32_t>(sbl[g], l32, cbl[g]);
+    }
+    ep = wg_excl<uint64_t>(sp, l64, cp);
+    for (uint32_t c = c0; c < c1; c++) {
+        SelChunk& h = ws.ch[c];
         for (int g = 0; g < 3; g++) {
-            uint64_t t;
-            uint32_t tb;
-            const uint64_t x = wg_excl<uint64_t>(h ? h->gBytes[g] : 0, l64, t);
-            const uint32_t xb = wg_excl<uint32_t>(h ? h->gBlocks[g] : 0u, l32, tb);
-            if (h) { h->gBytes[g] = cb[g] + x; h->gBlocks[g] = cbl[g] + xb; }
-            cb[g] += t;
-            cbl[g] += tb;
+            const uint64_t b = h.gBytes[g];
+            const uint32_t bl = h.gBlocks[g];
+            h.gBytes[g] = eb[g];
+            h.gBlocks[g] = ebl[g];
+            eb[g] += b;
+            ebl[g] += bl;
         }
-        uint64_t tp;
-        const uint64_t xp = wg_excl<uint64_t>(h ? h->packBytes : 0, l64, tp);
-        if (h) h->packBytes = cp + xp;
-        cp += tp;
+        const uint64_t pb = h.packBytes;
+        h.packBytes = ep;
+        ep += pb;
     }
     __syncthreads();
     const uint64_t gBase[3] = { 0, cb[0], cb[0] + cb[1] };
@@ -257,6 +280,78 @@ __global__ void __launch_bounds__(kT) k_sel_scan(uint32_t K, SelWs ws)
 }
 
 // ------------------------------------------------------------------ staging
+// Thread t takes the quads of elements [4v, 4v + 4) for v = t, t + 256, ...:
+// raw copy and deltas as whole elements, each byte plane's 4 bytes as one
+// dword (consecutive threads, consecutive dwords); elements past the last
+// quad one at a time.
+template <int W>
+__device__ void prep_w(const uint8_t* x, uint32_t n, uint8_t* raw, uint8_t* dl, uint8_t* ts, uint8_t* dts,
+                       unsigned long long& mv, unsigned long long& mz)
+{
+    const uint32_t m = n - 1, nq = n / 4;
+    const uint64_t msk = wmask(W);
+    for (uint32_t v = threadIdx.x; v < nq; v += kT) {
+        const uint32_t i0 = 4 * v;
+        uint64_t e[5];
+#pragma unroll
+        for (int j = 0; j < 4; j++) e[j] = ozl_gld_w(x + (size_t)(i0 + j) * W, W);
+        e[4] = i0 + 4 < n ? ozl_gld_w(x + (size_t)(i0 + 4) * W, W) : e[3];
+        uint64_t d[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            mv = e[j] > mv ? e[j] : mv;
+            d[j] = (e[j + 1] - e[j]) & msk;
+            if (i0 + j < m) {
+                const uint64_t z = zz(d[j], W);
+                mz = z > mz ? z : mz;
+            }
+        }
+        if (raw)
+#pragma unroll
+            for (int j = 0; j < 4; j++) ozl_gst_w(raw + (size_t)(i0 + j) * W, e[j], W);
+        if (dl)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (i0 + j < m) ozl_gst_w(dl + (size_t)(i0 + j) * W, d[j], W);
+        if (ts)
+#pragma unroll
+            for (int p = 0; p < W; p++) {
+                uint32_t q = 0;
+#pragma unroll
+                for (int j = 0; j < 4; j++) q |= (uint32_t)((e[j] >> (8 * p)) & 255) << (8 * j);
+                ozl_gst4(ts + (size_t)p * n + i0, q);
+            }
+        if (dts)
+#pragma unroll
+            for (int p = 0; p < W; p++) {
+                uint32_t q = 0;
+#pragma unroll
+                for (int j = 0; j < 4; j++) q |= (uint32_t)((d[j] >> (8 * p)) & 255) << (8 * j);
+                if (i0 + 3 < m) {
+                    ozl_gst4(dts + (size_t)p * m + i0, q);
+                } else {
+                    for (int j = 0; j < 4; j++)
+                        if (i0 + j < m) ozl_gst1(dts + (size_t)p * m + i0 + j, q >> (8 * j));
+                }
+            }
+    }
+    for (uint32_t i = 4 * nq + threadIdx.x; i < n; i += kT) {
+        const uint64_t v = ozl_gld_w(x + (size_t)i * W, W);
+        mv = v > mv ? v : mv;
+        if (raw) ozl_gst_w(raw + (size_t)i * W, v, W);
+        if (ts)
+            for (int k = 0; k < W; k++) ozl_gst1(ts + (size_t)k * n + i, (uint32_t)(v >> (8 * k)));
+        if (i < m) {
+            const uint64_t d = (ozl_gld_w(x + (size_t)(i + 1) * W, W) - v) & msk;
+            const uint64_t z = zz(d, W);
+            mz = z > mz ? z : mz;
+            if (dl) ozl_gst_w(dl + (size_t)i * W, d, W);
+            if (dts)
+                for (int k = 0; k < W; k++) ozl_gst1(dts + (size_t)k * m + i, (uint32_t)(d >> (8 * k)));
+        }
+    }
+}
+
 // one workgroup per chunk; element i and its delta d[i] = x[i+1] - x[i]
 __global__ void __launch_bounds__(kT) k_sel_prep(uint32_t K, SelWs ws)
 {
@@ -270,29 +365,17 @@ __global__ void __launch_bounds__(kT) k_sel_prep(uint32_t K, SelWs ws)
     uint8_t* tiny = ws.tiny + (size_t)c * kTinySlot;
     if (threadIdx.x == 0) H.first = ld_el(x, (int)w);
     if (q.small) return;
-    const uint64_t msk = wmask((int)w);
     uint8_t* raw = q.raw ? ws.stg + H.stRaw : nullptr;
     uint8_t* dl = q.delta ? (q.deltaTiny ? tiny : ws.stg + H.stDelta) : nullptr;
     uint8_t* ts = (q.tsHuf || q.tsFse) ? (q.tsTiny ? tiny + 16 : ws.stg + H.stTs) : nullptr;
     uint8_t* dts = q.dts ? (q.dtsTiny ? tiny + 96 : ws.stg + H.stDts) : nullptr;
     const bool wantV = has(H.need, C_BITPACK), wantZ = has(H.need, C_DELTA_ZZ_BP) && !q.deltaTiny;
     unsigned long long mv = 0, mz = 0;
-    for (uint32_t i = threadIdx.x; i < n; i += kT) {
-        const uint64_t v = ld_el(x + (size_t)i * w, (int)w);
-        mv = v > mv ? v : mv;
-        if (raw)
-            for (uint32_t k = 0; k < w; k++) raw[(size_t)i * w + k] = (uint8_t)(v >> (8 * k));
-        if (ts)
-            for (uint32_t k = 0; k < w; k++) ts[(size_t)k * n + i] = (uint8_t)(v >> (8 * k));
-        if (i < m) {
-            const uint64_t d = (ld_el(x + (size_t)(i + 1) * w, (int)w) - v) & msk;
-            const uint64_t z = zz(d, (int)w);
-            mz = z > mz ? z : mz;
-            if (dl)
-                for (uint32_t k = 0; k < w; k++) dl[(size_t)i * w + k] = (uint8_t)(d >> (8 * k));
-            if (dts)
-                for (uint32_t k = 0; k < w; k++) dts[(size_t)k * m + i] = (uint8_t)(d >> (8 * k));
-        }
+    switch (w) {
+    case 1: prep_w<1>(x, n, raw, dl, ts, dts, mv, mz); break;
+    case 2: prep_w<2>(x, n, raw, dl, ts, dts, mv, mz); break;
+    case 4: prep_w<4>(x, n, raw, dl, ts, dts, mv, mz); break;
+    default: prep_w<8>(x, n, raw, dl, ts, dts, mv, mz); break;
     }
     // block offsets
     if (threadIdx.x < w) {
@@ -550,14 +633,22 @@ __global__ void __launch_bounds__(64) k_sel_frame_size(uint32_t K, LayoutCtx X)
     SelChunk& H = X.ws.ch[c];
     if (H.status || X.ws.ctr->error) { X.ws.fsz[c] = 0; return; }
     Layout L;
-    uint8_t hdr[1024];
     int order[kMaxS], nbStored = 0;
+    SelFrame& F = X.ws.fr[c];
     const int rc = build_layout(X, c, (int)H.choice, L);
     if (rc) { H.status = (uint32_t)rc; X.ws.fsz[c] = 0; return; }
-    const size_t hl = layout_header(L, H.n * H.w, true, hdr, order, &nbStored);
-    for (int k = 0; k < nbStored; k++)
-        if (!L.ptr[order[k]] && L.st[order[k]].size) { H.status = OZL_ERR_LOGIC; X.ws.fsz[c] = 0; return; }
-    X.ws.fsz[c] = hl + layout_stored(L, order, nbStored) + 4 + 4 + 1;
+    const size_t hl = layout_header(L, H.n * H.w, true, F.hdr, order, &nbStored);
+    uint64_t stored = 0;
+    for (int k = 0; k < nbStored; k++) {
+        const int s = order[k];
+        if (!L.ptr[s] && L.st[s].size) { H.status = OZL_ERR_LOGIC; X.ws.fsz[c] = 0; return; }
+        F.ptr[k] = (uint64_t)(uintptr_t)L.ptr[s];
+        F.size[k] = L.st[s].size;
+        stored += L.st[s].size;
+    }
+    F.hlen = (uint32_t)hl;
+    F.nbStored = (uint32_t)nbStored;
+    X.ws.fsz[c] = hl + stored + 4 + 4 + 1;
 }
 
 // frame offsets: exclusive scan of the frame sizes (one workgroup); frames
@@ -618,36 +709,27 @@ __global__ void __launch_bounds__(kT) k_sel_pack(uint32_t K, SelWs ws)
     }
 }
 
-// header and stored streams of chunk c's frame
-__global__ void __launch_bounds__(kT) k_sel_write(uint32_t K, LayoutCtx X, uint8_t* __restrict__ dst,
+// header and stored streams of chunk c's frame (recorded by k_sel_frame_size)
+__global__ void __launch_bounds__(kT) k_sel_write(uint32_t K, SelWs ws, uint8_t* __restrict__ dst,
                                                   const uint64_t* __restrict__ frameOff)
 {
-    __shared__ uint8_t hdr[1024];
-    __shared__ const uint8_t* sp[kMaxS];
-    __shared__ uint32_t sz[kMaxS];
-    __shared__ int sInfo[2];
     const uint32_t c = blockIdx.x;
-    if (c >= K || X.ws.ctr->error) return;
-    const SelChunk& H = X.ws.ch[c];
-    if (H.status) return;
-    if (threadIdx.x == 0) {
-        Layout L;
-        int order[kMaxS], nbStored = 0;
-        build_layout(X, c, (int)H.choice, L);
-        const size_t hl = layout_header(L, H.n * H.w, true, hdr, order, &nbStored);
-        for (int k = 0; k < nbStored; k++) { sp[k] = L.ptr[order[k]]; sz[k] = L.st[order[k]].size; }
-        sInfo[0] = (int)hl;
-        sInfo[1] = nbStored;
-    }
-    __syncthreads();
+    if (c >= K || ws.ctr->error) return;
+    if (ws.ch[c].status) return;
+    const SelFrame& F = ws.fr[c];
     uint8_t* out = dst + frameOff[c];
-    const int hl = sInfo[0];
-    for (int i = threadIdx.x; i < hl; i += kT) out[i] = hdr[i];
-    size_t pos = (size_t)hl;
-    for (int k = 0; k < sInfo[1]; k++) {
-        const uint8_t* from = sp[k];
-        for (uint32_t i = threadIdx.x; i < sz[k]; i += kT) out[pos + i] = from[i];
-        pos += sz[k];
+    const uint32_t hl = F.hlen;
+    for (uint32_t i = threadIdx.x; i < hl; i += kT) ozl_gst1(out + i, F.hdr[i]);
+    size_t pos = hl;
+    const uint32_t ns = F.nbStored;
+    for (uint32_t k = 0; k < ns; k++) {
+        const uint8_t* from = reinterpret_cast<const uint8_t*>((uintptr_t)F.ptr[k]);
+        const uint32_t sz = F.size[k];
+        uint8_t* to = out + pos;
+        const uint32_t nd = sz / 4;
+        for (uint32_t i = threadIdx.x; i < nd; i += kT) ozl_gst4(to + 4 * (size_t)i, ozl_gld4(from + 4 * (size_t)i));
+        for (uint32_t i = 4 * nd + threadIdx.x; i < sz; i += kT) ozl_gst1(to + i, ozl_gld1(from + i));
+        pos += sz;
     }
 }
 
@@ -711,7 +793,7 @@ Carved carve(void* base, uint32_t K, size_t maxChunk)
     w.ch = reinterpret_cast<SelChunk*>(p); p += a256((size_t)K * sizeof(SelChunk));
     w.off = reinterpret_cast<uint64_t*>(p); p += a256(((size_t)K * kMaxBlocksPerChunk + 1) * 8);
     w.fsz = reinterpret_cast<uint64_t*>(p); p += a256((size_t)K * 8);
-    w.hashes = reinterpret_cast<uint64_t*>(p); p += a256((size_t)K * 8);
+    w.fr = reinterpret_cast<SelFrame*>(p); p += a256((size_t)K * sizeof(SelFrame));
     w.tiny = p; p += a256((size_t)K * kTinySlot);
     w.stgCap = (size_t)K * 4 * maxChunk;
     w.stg = p; p += a256(w.stgCap + 64);
@@ -811,7 +893,7 @@ int ozl_select_compress(const void* src, const uint64_t* spans, const uint8_t* w
     OZL_LAUNCH("k_sel_frame_size", k_sel_frame_size, dim3((K + 63) / 64), dim3(64), 0, st, K, X);
     OZL_LAUNCH("k_sel_scan_frames", k_sel_scan_frames, dim3(1), dim3(kT), 0, st, K, cv.ws, frameOff, dstCap);
     OZL_LAUNCH("k_sel_pack", k_sel_pack, dim3(K), dim3(kT), 0, st, K, cv.ws);
-    OZL_LAUNCH("k_sel_write", k_sel_write, dim3(K), dim3(kT), 0, st, K, X, d, frameOff);
+    OZL_LAUNCH("k_sel_write", k_sel_write, dim3(K), dim3(kT), 0, st, K, cv.ws, d, frameOff);
     OZL_LAUNCH("k_sel_content_hash", k_sel_hash, dim3(K), dim3(64), 0, st, K, cv.ws, d, frameOff, 0, status);
     OZL_LAUNCH("k_sel_frame_hash", k_sel_hash, dim3(K), dim3(64), 0, st, K, cv.ws, d, frameOff, 1, status);
     OZL_HIP_CHECK(hipGetLastError());
