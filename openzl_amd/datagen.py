@@ -184,9 +184,12 @@ def _select_chunk(g, rng, nbytes):
     return (walk & 0xFFFFFFFF).astype(np.uint32)
 
 
-def selector_chunks(n_chunks=8192, seed=SEEDS[5], nbytes=SELECT_CHUNK):
+def selector_chunks(n_chunks=8192, seed=SEEDS[5], nbytes=SELECT_CHUNK, limit=None):
     """Chunks of the learned-selector configuration: each from one of the eight
-    generators chosen uniformly; returns (generator ids, list of arrays)."""
+    generators chosen uniformly; returns (generator ids, list of arrays).
+    `limit` generates only the first chunks of the same sequence."""
     rng = np.random.default_rng(seed)
     gens = rng.integers(0, 8, n_chunks)
+    if limit is not None:
+        gens = gens[:limit]
     return gens, [_select_chunk(int(g), rng, nbytes) for g in gens]

@@ -110,3 +110,23 @@ def test_committed_model_file():
     if not os.path.exists(path):
         pytest.skip("no trained selector model")
     assert os.path.getsize(path) == (64 * 64 * 2 + 16 * 64) * 2 + (64 + 64 + 16) * 4
+
+
+def test_training_labels_match_reference_sizes():
+    """The committed training sizes (GPU exhaustive trial, checksums off) of
+    the first training chunks equal the reference library's frame sizes."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ozl_checkers as C
+    path = os.path.join(ROOT, "openzl_amd", "data", "selector_train.npz")
+    if not os.path.exists(path):
+        pytest.skip("no training data")
+    if C.refh() is None:
+        pytest.skip("reference library not built")
+    d = np.load(path)
+    n = d["sizes"].shape[0]
+    gens, chunks = G.selector_chunks(n, seed=G.SELECT_TRAIN_SEED, limit=48)
+    assert np.array_equal(gens, d["gens"][:48])
+    kinds = [C.STORE, C.FSE, C.BITPACK, C.DELTA_FSE, C.DELTA_ZZ_BP, C.TS_FSE, C.TS_HUF, C.DELTA_TS_FSE]
+    for i, c in enumerate(chunks):
+        ref = [len(C.ref_compress(k, c, C.NUMERIC, c.dtype.itemsize, False, False)) for k in kinds]
+        assert list(d["sizes"][i]) == ref, i
