@@ -9,7 +9,9 @@
 //                  only the per-block scramble chain is sequential.
 #pragma once
 #include "ozl_core.h"
+#if defined(__HIPCC__)
 #include "ozl_gmem.h"
+#endif
 
 #define OZL_XXH3_SECRET_BYTES \
     0xb8, 0xfe, 0x6c, 0x39, 0x23, 0xa4, 0x4b, 0xbe, 0x7c, 0x01, 0x81, 0x2c, 0xf7, 0x21, 0xad, 0x1c, \
