@@ -150,13 +150,28 @@ __device__ inline uint64_t zz_delta(uint64_t a, uint64_t b, int w)
 __device__ inline uint64_t unzz(uint64_t z, int w) { return ((z >> 1) ^ (0 - (z & 1))) & wmask(w); }
 
 // workgroup-wide item fetch from a device counter; returns the item or ~0u
-__device__ inline uint32_t next_item(uint32_t* q, uint32_t total, uint32_t* sItem)
+// Work queue: a workgroup takes g consecutive items per atomic. One counter
+// serves thousands of workgroups and its atomics serialise, so each grab
+// takes about 1/16 of a workgroup's fair share (grab_size). next_item
+// returns the next item of the current range, taking a new range when it is
+// used up, or ~0u when the queue is empty; it starts with a barrier, so the
+// previous item's LDS is free.
+__device__ inline uint32_t grab_size(uint32_t total)
+{
+    const uint32_t g = total / (gridDim.x * 16u);
+    return g ? g : 1u;
+}
+__device__ inline uint32_t next_item(uint32_t* q, uint32_t total, uint32_t g, uint32_t* sItem, uint32_t& it,
+                                     uint32_t& end)
 {
     __syncthreads();
-    if (threadIdx.x == 0) *sItem = atomicAdd(q, 1u);
+    if (it + 1 < end) return ++it;
+    if (threadIdx.x == 0) *sItem = atomicAdd(q, g);
     __syncthreads();
-    const uint32_t it = *sItem;
-    return it < total ? it : ~0u;
+    it = *sItem;
+    if (it >= total) return ~0u;
+    end = it + g < total ? it + g : total;
+    return it;
 }
 
 template <typename T>
@@ -345,11 +360,18 @@ __global__ void __launch_bounds__(64) k_hash_sums(ColWs ws, int q)
     uint64_t* sums = hash_sums(ws, items, total);
     const ozl_xxh3_lane L = ozl_xxh3_lane_init();
     const int lane = threadIdx.x;
+    const uint32_t g = grab_size(total);
+    uint32_t it = 0, end = 0;
     for (;;) {
-        uint32_t v = 0;
-        if (lane == 0) v = atomicAdd(&ws.ctr->queue[q], 1u);
-        const uint32_t it = (uint32_t)__shfl((int)v, 0, 64);
-        if (it >= total) break;
+        if (it + 1 < end) {
+            it++;
+        } else {
+            uint32_t v = 0;
+            if (lane == 0) v = atomicAdd(&ws.ctr->queue[q], g);
+            it = (uint32_t)__shfl((int)v, 0, 64);
+            if (it >= total) break;
+            end = it + g < total ? it + g : total;
+        }
         const HashReg h = ws.hreg[map[it]];
         const uint32_t b0 = (it - h.itemBase) * kHG, b1 = b0 + kHG < h.nBlocks ? b0 + kHG : h.nBlocks;
         const uint8_t* in = reinterpret_cast<const uint8_t*>((uintptr_t)h.ptr);
@@ -617,8 +639,10 @@ __global__ void __launch_bounds__(kCT) k_col_pack(uint32_t K, ColWs ws, uint8_t*
     __shared__ uint32_t sItem, sCol;
     __shared__ uint64_t Z[kTileMax];
     const uint32_t total = ws.ctr->totalItems;
+    const uint32_t g = grab_size(total);
+    uint32_t cur = 0, end = 0;
     for (;;) {
-        const uint32_t it = next_item(&ws.ctr->queue[1], total, &sItem);
+        const uint32_t it = next_item(&ws.ctr->queue[1], total, g, &sItem, cur, end);
         if (it == ~0u) break;
         if (threadIdx.x == 0) sCol = item_col(ws)[it];
         __syncthreads();
@@ -817,8 +841,10 @@ __global__ void __launch_bounds__(kCT) k_cold_sum(uint32_t K, ColWs ws)
     __shared__ uint64_t lds[kCT / 64];
     __shared__ __attribute__((aligned(16))) uint32_t Wd[2 * kTileMax + 4];
     const uint32_t total = ws.ctr->totalItems;
+    const uint32_t g = grab_size(total);
+    uint32_t cur = 0, end = 0;
     for (;;) {
-        const uint32_t it = next_item(&ws.ctr->queue[0], total, &sItem);
+        const uint32_t it = next_item(&ws.ctr->queue[0], total, g, &sItem, cur, end);
         if (it == ~0u) break;
         if (threadIdx.x == 0) sCol = item_col(ws)[it];
         __syncthreads();
@@ -872,8 +898,10 @@ __global__ void __launch_bounds__(kCT) k_cold_apply(uint32_t K, ColWs ws, uint8_
     __shared__ uint64_t lds[kCT / 64];
     __shared__ __attribute__((aligned(16))) uint32_t Wd[2 * kTileMax + 4];
     const uint32_t total = ws.ctr->totalItems;
+    const uint32_t g = grab_size(total);
+    uint32_t cur = 0, end = 0;
     for (;;) {
-        const uint32_t it = next_item(&ws.ctr->queue[1], total, &sItem);
+        const uint32_t it = next_item(&ws.ctr->queue[1], total, g, &sItem, cur, end);
         if (it == ~0u) break;
         if (threadIdx.x == 0) sCol = item_col(ws)[it];
         __syncthreads();

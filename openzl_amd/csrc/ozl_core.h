@@ -90,7 +90,8 @@ OZL_HD double ozl_entropy(const uint32_t* count, unsigned maxValue, size_t total
     if (total == 0) return 0;
     int64_t norm = ((int64_t)1 << 62) / (int64_t)total;
     int64_t acc = 0;
-    for (unsigned s = 0; s <= maxValue; s++) acc += ozl_entropy_term(count[s], norm);
+    for (unsigned s = 0; s <= maxValue; s++)
+        if (count[s]) acc += ozl_entropy_term(count[s], norm);  // a zero count's term is 0
     return ozl_entropy_finish(acc, total);
 }
 
@@ -132,7 +133,15 @@ OZL_HD int ozl_decide_from_hist(const uint32_t* count, size_t n, int hufMode, un
     *cardOut = card;
     if (n <= 1) return OZL_BK_STORE;
     if (card == 1) return OZL_BK_CONSTANT;
-    return ozl_decide(ozl_entropy(count, 255, n), maxSym, card, n, hufMode);
+    return ozl_decide(ozl_entropy(count, maxSym, n), maxSym, card, n, hufMode);
+}
+// The same decision from the histogram's statistics: maxSym, cardinality and
+// the int64 sum of ozl_entropy_term over the symbols (computed in parallel).
+OZL_HD int ozl_decide_from_stats(int64_t termSum, size_t n, unsigned maxSym, unsigned card, int hufMode)
+{
+    if (n <= 1) return OZL_BK_STORE;
+    if (card == 1) return OZL_BK_CONSTANT;
+    return ozl_decide(ozl_entropy_finish(termSum, n), maxSym, card, n, hufMode);
 }
 
 // ------------------------------------------------------------------ FSE

@@ -173,12 +173,33 @@ __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, co
         for (int s = threadIdx.x; s < 256; s += 64) L.count[s] = cg[s];
     }
     __syncthreads();
+    // the histogram's statistics by the wave: four symbols per lane, exact
+    // integer reductions (the entropy terms' sum does not depend on order)
+    int64_t tsum = 0;
+    uint32_t ms = 0, cd = 0;
+    {
+        const int64_t norm = n ? ((int64_t)1 << 62) / (int64_t)n : 0;
+        for (int k = 0; k < 4; k++) {
+            const uint32_t s = threadIdx.x + 64 * k, c = L.count[s];
+            if (c) {
+                tsum += ozl_entropy_term(c, norm);
+                ms = s;
+                cd++;
+            }
+        }
+        for (int d = 32; d >= 1; d >>= 1) {
+            tsum += __shfl_xor(tsum, d, 64);
+            cd += __shfl_xor(cd, d, 64);
+            const uint32_t o = __shfl_xor(ms, d, 64);
+            ms = o > ms ? o : ms;
+        }
+    }
     if (threadIdx.x == 0) {
         ozl_eplan p = {};
         p.n = n;
         uint8_t* aux = L.aux;
-        unsigned maxSym, card;
-        int bk = ozl_decide_from_hist(L.count, n, HUF, &maxSym, &card);
+        const unsigned maxSym = ms, card = cd;
+        int bk = ozl_decide_from_stats(tsum, n, maxSym, card, HUF);
         if (n < OZL_MIN_STREAM_SIZE) bk = OZL_BK_STORE;
         p.bk = (uint8_t)bk;
         p.maxSym = (uint8_t)maxSym;

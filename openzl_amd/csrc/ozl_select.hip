@@ -37,9 +37,11 @@
 //                 k_sel_write -> k_sel_hash (content) -> k_sel_hash
 //                 (compressed) and the end byte
 // Staging groups keep both coders' block lists contiguous: planes only the
-// Huffman coder reads, then planes both coders read, then streams only the
-// FSE coder reads. The Huffman list is the first two groups, the FSE list the
-// last two.
+// Huffman coder reads, then planes both coders read, then whole streams only
+// the FSE coder reads (raw, deltas), then planes only the FSE coder reads.
+// The Huffman list is the first two groups, the FSE list the last three; the
+// longest FSE blocks (whole chunks) precede the planes, so the FSE coder's
+// first workgroups, dispatched first, hold the longest state chains.
 #include "ozl_internal.h"
 #include "ozl_core.h"
 #include "ozl_xxh3.h"
@@ -64,8 +66,8 @@ struct SelChunk {
     uint64_t src;
     unsigned long long maxV, maxZ;
     uint64_t first;
-    uint64_t gBytes[3];      // staged bytes per group, then the exclusive prefix
-    uint32_t gBlocks[3];     // entropy blocks per group, then the exclusive prefix
+    uint64_t gBytes[4];      // staged bytes per group, then the exclusive prefix
+    uint32_t gBlocks[4];     // entropy blocks per group, then the exclusive prefix
     uint32_t n, w, need, choice, status;
     uint64_t packBytes;      // then the pack area offset
     uint64_t stTs, stRaw, stDelta, stDts;   // staging offsets
@@ -75,9 +77,9 @@ struct SelChunk {
 };
 
 struct SelCtr {
-    uint64_t gBase[3];       // staging offset of each group
-    uint32_t bBase[3];       // first block of each group
-    uint32_t blocks[3];
+    uint64_t gBase[4];       // staging offset of each group
+    uint32_t bBase[4];       // first block of each group
+    uint32_t blocks[4];
     uint64_t stgBytes, packBytes;
     uint32_t error;
     uint32_t pad[5];
@@ -174,11 +176,11 @@ __global__ void __launch_bounds__(kT) k_sel_setup(const uint8_t* __restrict__ sr
         if (q.raw) { h.gBytes[2] += n * w; h.gBlocks[2] += 1; }
         if (q.delta && !q.deltaTiny) { h.gBytes[2] += m * w; h.gBlocks[2] += 1; }
         if ((q.tsHuf || q.tsFse) && !q.tsTiny) {
-            const int g = q.tsHuf && q.tsFse ? 1 : q.tsHuf ? 0 : 2;
+            const int g = q.tsHuf && q.tsFse ? 1 : q.tsHuf ? 0 : 3;
             h.gBytes[g] += n * w;
             h.gBlocks[g] += w;
         }
-        if (q.dts && !q.dtsTiny) { h.gBytes[2] += m * w; h.gBlocks[2] += w; }
+        if (q.dts && !q.dtsTiny) { h.gBytes[3] += m * w; h.gBlocks[3] += w; }
         if (q.pack) h.packBytes = n * w + 16;
     }
     ws.ch[c] = h;
@@ -216,25 +218,25 @@ __global__ void __launch_bounds__(kT) k_sel_scan(uint32_t K, SelWs ws)
     __shared__ uint32_t l32[kT / 64];
     const uint32_t per = (K + kT - 1) / kT;
     const uint32_t c0 = threadIdx.x * per, c1 = c0 + per < K ? c0 + per : K;
-    uint64_t sb[3] = { 0, 0, 0 }, sp = 0;
-    uint32_t sbl[3] = { 0, 0, 0 };
+    uint64_t sb[4] = { 0, 0, 0, 0 }, sp = 0;
+    uint32_t sbl[4] = { 0, 0, 0, 0 };
     for (uint32_t c = c0; c < c1; c++) {
         const SelChunk& h = ws.ch[c];
-        for (int g = 0; g < 3; g++) { sb[g] += h.gBytes[g]; sbl[g] += h.gBlocks[g]; }
+        for (int g = 0; g < 4; g++) { sb[g] += h.gBytes[g]; sbl[g] += h.gBlocks[g]; }
         sp += h.packBytes;
     }
-    uint64_t cb[3], cp;
-    uint32_t cbl[3];
-    uint64_t eb[3], ep;
-    uint32_t ebl[3];
-    for (int g = 0; g < 3; g++) {
+    uint64_t cb[4], cp;
+    uint32_t cbl[4];
+    uint64_t eb[4], ep;
+    uint32_t ebl[4];
+    for (int g = 0; g < 4; g++) {
         eb[g] = wg_excl<uint64_t>(sb[g], l64, cb[g]);
         ebl[g] = wg_excl<uint32_t>(sbl[g], l32, cbl[g]);
     }
     ep = wg_excl<uint64_t>(sp, l64, cp);
     for (uint32_t c = c0; c < c1; c++) {
         SelChunk& h = ws.ch[c];
-        for (int g = 0; g < 3; g++) {
+        for (int g = 0; g < 4; g++) {
             const uint64_t b = h.gBytes[g];
             const uint32_t bl = h.gBlocks[g];
             h.gBytes[g] = eb[g];
@@ -247,34 +249,36 @@ __global__ void __launch_bounds__(kT) k_sel_scan(uint32_t K, SelWs ws)
         ep += pb;
     }
     __syncthreads();
-    const uint64_t gBase[3] = { 0, cb[0], cb[0] + cb[1] };
-    const uint32_t bBase[3] = { 0, cbl[0], cbl[0] + cbl[1] };
+    const uint64_t gBase[4] = { 0, cb[0], cb[0] + cb[1], cb[0] + cb[1] + cb[2] };
+    const uint32_t bBase[4] = { 0, cbl[0], cbl[0] + cbl[1], cbl[0] + cbl[1] + cbl[2] };
     if (threadIdx.x == 0) {
         SelCtr* t = ws.ctr;
-        for (int g = 0; g < 3; g++) { t->gBase[g] = gBase[g]; t->bBase[g] = bBase[g]; t->blocks[g] = cbl[g]; }
-        t->stgBytes = cb[0] + cb[1] + cb[2];
+        for (int g = 0; g < 4; g++) { t->gBase[g] = gBase[g]; t->bBase[g] = bBase[g]; t->blocks[g] = cbl[g]; }
+        t->stgBytes = cb[0] + cb[1] + cb[2] + cb[3];
         t->packBytes = cp;
         if (t->stgBytes > ws.stgCap || cp > ws.packCap) t->error = 1;
-        ws.off[bBase[2] + cbl[2]] = t->stgBytes;
+        ws.off[bBase[3] + cbl[3]] = t->stgBytes;
     }
     for (uint32_t c = threadIdx.x; c < K; c += kT) {
         SelChunk& h = ws.ch[c];
         if (h.status) continue;
         const Needs q = needs_of(h.need, h.n, h.w);
         if (q.small) continue;
-        const uint64_t n = h.n, m = n - 1;
+        const uint64_t n = h.n;
         const uint32_t w = h.w;
         if ((q.tsHuf || q.tsFse) && !q.tsTiny) {
-            const int g = q.tsHuf && q.tsFse ? 1 : q.tsHuf ? 0 : 2;
+            const int g = q.tsHuf && q.tsFse ? 1 : q.tsHuf ? 0 : 3;
             h.stTs = gBase[g] + h.gBytes[g];
             h.bTs = bBase[g] + h.gBlocks[g];
         }
-        // group 3 order per chunk: FSE-only planes, raw, delta, delta planes
+        // group 2 per chunk: raw, delta; group 3: FSE-only planes, delta planes
         uint64_t s = gBase[2] + h.gBytes[2];
         uint32_t b = bBase[2] + h.gBlocks[2];
-        if (q.tsFse && !q.tsHuf && !q.tsTiny) { s += n * w; b += w; }
         if (q.raw) { h.stRaw = s; h.bRaw = b; s += n * w; b += 1; }
-        if (q.delta && !q.deltaTiny) { h.stDelta = s; h.bDelta = b; s += m * w; b += 1; }
+        if (q.delta && !q.deltaTiny) { h.stDelta = s; h.bDelta = b; }
+        s = gBase[3] + h.gBytes[3];
+        b = bBase[3] + h.gBlocks[3];
+        if (q.tsFse && !q.tsHuf && !q.tsTiny) { s += n * w; b += w; }
         if (q.dts && !q.dtsTiny) { h.stDts = s; h.bDts = b; }
     }
 }
@@ -818,7 +822,7 @@ int sel_front(const uint8_t* src, const uint64_t* spans, const uint8_t* widths, 
     if (ctr.error) return OZL_ERR_INTERNAL_BUFFER;
     OZL_LAUNCH("k_sel_prep", k_sel_prep, dim3(K), dim3(kT), 0, st, K, ws);
     const uint32_t hufB = ctr.blocks[0] + ctr.blocks[1];
-    const uint32_t fseB = ctr.blocks[1] + ctr.blocks[2];
+    const uint32_t fseB = ctr.blocks[1] + ctr.blocks[2] + ctr.blocks[3];
     if ((size_t)ews_bytes(hufB, maxChunk) + ews_bytes(fseB, maxChunk) > ws.ewsCap) return OZL_ERR_INTERNAL_BUFFER;
     X.ws = ws;
     X.e.huf = ozl_ews_carve(ws.ews, hufB ? hufB : 1, maxChunk);
