@@ -637,7 +637,7 @@ __global__ void __launch_bounds__(kCT) k_col_pack(uint32_t K, ColWs ws, uint8_t*
                                                  const uint64_t* __restrict__ frameOff)
 {
     __shared__ uint32_t sItem, sCol;
-    __shared__ uint64_t Z[kTileMax];
+    __shared__ uint64_t Z[kTileMax + kTileMax / 16 + 1];   // element e at e + e / 16
     const uint32_t total = ws.ctr->totalItems;
     const uint32_t g = grab_size(total);
     uint32_t cur = 0, end = 0;
@@ -661,7 +661,7 @@ __global__ void __launch_bounds__(kCT) k_col_pack(uint32_t K, ColWs ws, uint8_t*
 #pragma unroll
             for (int k = 0; k < kPer; k++) {
                 const uint32_t i = threadIdx.x + kCT * k;
-                if (i < cnt) Z[i] = z[k];
+                if (i < cnt) Z[i + (i >> 4)] = z[k];
             }
         }
         __syncthreads();
@@ -675,7 +675,7 @@ __global__ void __launch_bounds__(kCT) k_col_pack(uint32_t K, ColWs ws, uint8_t*
             uint64_t word = 0;
             for (; (uint64_t)e * nb < bend; e++) {
                 const int64_t off = (int64_t)e * nb - (int64_t)bit0;
-                const uint64_t v = Z[e];
+                const uint64_t v = Z[e + (e >> 4)];
                 word |= off >= 0 ? v << off : v >> (-off);
             }
             if (wi < nW) {
@@ -816,11 +816,16 @@ __global__ void __launch_bounds__(64) k_cold_parse(const uint8_t* __restrict__ f
 // boundary) go to LDS with coalesced loads; one word more than the item's
 // bits is read (a value may straddle into it), which the frame's 9 trailing
 // bytes always back.
+// LDS slot of word i: one pad word after every 32, so lanes that read runs
+// of words at a fixed stride (16 deltas per lane) fall into different banks
+__device__ inline uint32_t sk(uint32_t i) { return i + (i >> 5); }
+constexpr uint32_t kWdWords = 2 * kTileMax + 4 + (2 * kTileMax + 4) / 32 + 2;
+
 __device__ inline uint32_t load_item_words(const uint8_t* stream, uint64_t d0, uint32_t cnt, int nb, uint32_t* Wd)
 {
     const uint8_t* sp = stream + d0 * (uint64_t)nb / 8;
     const uint32_t nW = (uint32_t)(((uint64_t)cnt * nb + 31) / 32) + 1;
-    for (uint32_t k = threadIdx.x; k < nW; k += kCT) Wd[k] = *reinterpret_cast<const uint32_t*>(sp + 4 * (size_t)k);
+    for (uint32_t k = threadIdx.x; k < nW; k += kCT) Wd[sk(k)] = ozl_gld4(sp + 4 * (size_t)k);
     return nW;
 }
 
@@ -829,8 +834,8 @@ __device__ inline uint64_t item_value(const uint32_t* Wd, uint32_t e, int nb)
 {
     const uint64_t bit = (uint64_t)e * nb;
     const uint32_t q = (uint32_t)(bit >> 5), sh = (uint32_t)(bit & 31);
-    uint64_t v = ((uint64_t)Wd[q] | ((uint64_t)Wd[q + 1] << 32)) >> sh;
-    if (sh && nb + (int)sh > 64) v |= (uint64_t)Wd[q + 2] << (64 - sh);
+    uint64_t v = ((uint64_t)Wd[sk(q)] | ((uint64_t)Wd[sk(q + 1)] << 32)) >> sh;
+    if (sh && nb + (int)sh > 64) v |= (uint64_t)Wd[sk(q + 2)] << (64 - sh);
     return nb == 64 ? v : v & ((1ull << nb) - 1);
 }
 
@@ -839,7 +844,7 @@ __global__ void __launch_bounds__(kCT) k_cold_sum(uint32_t K, ColWs ws)
 {
     __shared__ uint32_t sItem, sCol;
     __shared__ uint64_t lds[kCT / 64];
-    __shared__ __attribute__((aligned(16))) uint32_t Wd[2 * kTileMax + 4];
+    __shared__ __attribute__((aligned(16))) uint32_t Wd[kWdWords];
     const uint32_t total = ws.ctr->totalItems;
     const uint32_t g = grab_size(total);
     uint32_t cur = 0, end = 0;
@@ -896,7 +901,7 @@ __global__ void __launch_bounds__(kCT) k_cold_apply(uint32_t K, ColWs ws, uint8_
 {
     __shared__ uint32_t sItem, sCol;
     __shared__ uint64_t lds[kCT / 64];
-    __shared__ __attribute__((aligned(16))) uint32_t Wd[2 * kTileMax + 4];
+    __shared__ __attribute__((aligned(16))) uint32_t Wd[kWdWords];
     const uint32_t total = ws.ctr->totalItems;
     const uint32_t g = grab_size(total);
     uint32_t cur = 0, end = 0;
@@ -929,26 +934,30 @@ __global__ void __launch_bounds__(kCT) k_cold_apply(uint32_t K, ColWs ws, uint8_
         uint64_t tot;
         const uint64_t pre = block_excl<uint64_t>(run, lds, tot) + tile_sums(ws, total)[it];
         __syncthreads();  // every thread is done with the words
+        // staged in skewed words (byte a at word sk(a / 4), byte a % 4)
         uint8_t* xb = reinterpret_cast<uint8_t*>(Wd);
 #pragma unroll
         for (int k = 0; k < kPer; k++) {
             const uint32_t e = e0 + k;
             if (e < cnt) {
                 const uint64_t x = (pre + v[k]) & mask;
+                const uint32_t a = e * (uint32_t)w;
                 switch (w) {
-                case 1: xb[e] = (uint8_t)x; break;
-                case 2: reinterpret_cast<uint16_t*>(xb)[e] = (uint16_t)x; break;
-                case 4: reinterpret_cast<uint32_t*>(xb)[e] = (uint32_t)x; break;
-                default: reinterpret_cast<uint64_t*>(xb)[e] = x; break;
+                case 1: xb[4 * sk(a >> 2) + (a & 3)] = (uint8_t)x; break;
+                case 2: reinterpret_cast<uint16_t*>(xb + 4 * sk(a >> 2))[(a >> 1) & 1] = (uint16_t)x; break;
+                case 4: Wd[sk(a >> 2)] = (uint32_t)x; break;
+                default:
+                    Wd[sk(a >> 2)] = (uint32_t)x;
+                    Wd[sk((a >> 2) + 1)] = (uint32_t)(x >> 32);
+                    break;
                 }
             }
         }
         __syncthreads();
         uint8_t* o = out + (d0 + 1) * (uint64_t)w;
         const uint32_t nbytes = cnt * (uint32_t)w, nd = nbytes / 4;
-        for (uint32_t k = threadIdx.x; k < nd; k += kCT)
-            *reinterpret_cast<uint32_t*>(o + 4 * (size_t)k) = reinterpret_cast<const uint32_t*>(xb)[k];
-        for (uint32_t k = 4 * nd + threadIdx.x; k < nbytes; k += kCT) o[k] = xb[k];
+        for (uint32_t k = threadIdx.x; k < nd; k += kCT) ozl_gst4(o + 4 * (size_t)k, Wd[sk(k)]);
+        for (uint32_t k = 4 * nd + threadIdx.x; k < nbytes; k += kCT) ozl_gst1(o + k, xb[4 * sk(k >> 2) + (k & 3)]);
     }
 }
 
