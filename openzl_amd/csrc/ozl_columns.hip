@@ -47,8 +47,13 @@ namespace {
 
 constexpr int kCT = 256;          // threads per workgroup
 constexpr int kHdrSlot = 96;      // per-column header bytes (frame + chunk header)
-constexpr uint32_t kTileMax = 4096;          // deltas per item (LDS: 8 bytes per delta)
-constexpr int kPer = (int)kTileMax / kCT;     // deltas per thread and item
+// Items hold at most 2048 deltas: 8 per thread keeps the item kernels under
+// 64 VGPRs and 17 KB of LDS, so eight 256-thread workgroups (the wave limit)
+// fit a CU; measured, that beats 4096-delta items at four workgroups per CU.
+// Narrow columns (width <= 4) run in 32-bit arithmetic, wide ones in 64.
+constexpr uint32_t kTileMax = 2048;
+constexpr uint32_t kTileWide = 2048;
+template <bool NARROW> constexpr int per_of() { return (int)(NARROW ? kTileMax : kTileWide) / kCT; }
 
 struct ColPlan {                  // 64 bytes, one per column
     uint64_t src;                 // column address (encode: input, decode: packed stream)
@@ -75,12 +80,13 @@ constexpr uint64_t kMinStream = 10;
 static_assert(sizeof(ColPlan) == 64, "ColPlan layout");
 
 struct ColCtr {
-    uint32_t queue[4];            // work counters: column items (2), hash items (2)
+    uint32_t queue[8];            // work counters: column items (narrow 0, 1; wide 4, 5), hash items (2, 3)
     uint32_t totalItems;          // column items
     uint32_t error;               // bit 0: column items do not fit, bit 1: hash blocks do not fit
     uint32_t hTotalItems;         // hash items of the current hash pass
     uint32_t hTotalBlocks;
-    uint32_t pad[8];
+    uint32_t nWide;               // columns of width 8 (the wide item kernels exit at once when 0)
+    uint32_t pad[3];
 };
 
 // one XXH3 input of a hash pass: its 1 KiB blocks are summed by many waves
@@ -231,7 +237,8 @@ __global__ void __launch_bounds__(kCT) k_col_setup(const uint8_t* __restrict__ s
         if (n * (uint64_t)w < kMinStream) p.kind = kTinyIn;
         else if ((n - 1) * (uint64_t)w < kMinStream) p.kind = kTinyDz;
         else {
-            const uint32_t t = tile < kTileMax ? tile : kTileMax;
+            const uint32_t tmax = w <= 4 ? kTileMax : kTileWide;
+            const uint32_t t = tile < tmax ? tile : tmax;
             p.tile = t;
             p.nItems = (uint32_t)((p.m + t - 1) / t);
         }
@@ -246,15 +253,20 @@ __global__ void __launch_bounds__(kCT) k_col_scan_items(uint32_t K, ColWs ws)
 {
     __shared__ uint32_t lds[kCT / 64];
     const uint32_t per = (K + kCT - 1) / kCT, b0 = threadIdx.x * per, b1 = b0 + per < K ? b0 + per : K;
-    uint32_t s = 0;
-    for (uint32_t i = b0; i < b1; i++) s += ws.plans[i].nItems;
-    uint32_t total;
+    uint32_t s = 0, wide = 0;
+    for (uint32_t i = b0; i < b1; i++) {
+        s += ws.plans[i].nItems;
+        wide += ws.plans[i].nItems && ws.plans[i].w > 4;
+    }
+    uint32_t total, nWide;
     uint32_t off = block_excl<uint32_t>(s, lds, total);
+    block_excl<uint32_t>(wide, lds, nWide);
     for (uint32_t i = b0; i < b1; i++) {
         ws.plans[i].itemBase = off;
         off += ws.plans[i].nItems;
     }
     if (threadIdx.x == 0) {
+        ws.ctr->nWide = nWide;
         const bool fits = (uint64_t)(uintptr_t)(hash_map(ws, total)) <= ws.end;
         ws.ctr->totalItems = fits ? total : 0u;
         if (!fits) ws.ctr->error |= 1u;
@@ -353,6 +365,41 @@ __global__ void __launch_bounds__(64) k_hash_map(uint32_t R, ColWs ws)
     for (uint32_t i = threadIdx.x; i < h.nItems; i += 64) map[h.itemBase + i] = r;
 }
 
+// low 8w bits of a 32-bit word (w <= 4)
+constexpr uint32_t mask32(int W) { return W >= 4 ? 0xFFFFFFFFu : (1u << (8 * (W & 3))) - 1u; }
+
+// largest zigzag delta among the deltas whose first element lies in a lane's
+// 16 bytes (v0, v1) of a column of width W; nxt is the element after them
+template <int W>
+__device__ inline unsigned long long lane_zz_max(uint64_t v0, uint64_t v1, uint64_t nxt, bool hasNext,
+                                                 unsigned long long mx)
+{
+    constexpr int nE = 16 / W;
+    if constexpr (W <= 4) {
+        constexpr uint32_t M = mask32(W);
+        const uint32_t dw[4] = { (uint32_t)v0, (uint32_t)(v0 >> 32), (uint32_t)v1, (uint32_t)(v1 >> 32) };
+        uint32_t prev = dw[0] & M, m32 = 0;
+#pragma unroll
+        for (int e = 1; e <= nE; e++) {
+            if (e == nE && !hasNext) break;
+            const uint32_t cur = e < nE ? (dw[(e * W) / 4] >> (8 * ((e * W) % 4))) & M : (uint32_t)nxt & M;
+            const uint32_t d = (cur - prev) & M;
+            const uint32_t z = ((d << 1) ^ (0u - (d >> (8 * W - 1)))) & M;
+            m32 = z > m32 ? z : m32;
+            prev = cur;
+        }
+        return (unsigned long long)m32 > mx ? (unsigned long long)m32 : mx;
+    } else {
+        uint64_t z = zz_delta(v0, v1, 8);
+        mx = z > mx ? z : mx;
+        if (hasNext) {
+            z = zz_delta(v1, nxt, 8);
+            mx = z > mx ? z : mx;
+        }
+        return mx;
+    }
+}
+
 __global__ void __launch_bounds__(64) k_hash_sums(ColWs ws, int q)
 {
     const uint32_t total = ws.ctr->hTotalItems, items = ws.ctr->totalItems;
@@ -405,7 +452,6 @@ __global__ void __launch_bounds__(64) k_hash_sums(ColWs ws, int q)
             }
             if (w) {
                 // deltas whose first element is in this lane's 16 bytes
-                const int nE = 16 / w;
                 const uint64_t v0 = bw[k][0], v1 = bw[k][1];
                 uint64_t nxt = (uint64_t)__shfl_down((unsigned long long)(v0 & msk), 1, 64);
                 // first element of the next block (lane 0's first word)
@@ -417,17 +463,11 @@ __global__ void __launch_bounds__(64) k_hash_sums(ColWs ws, int q)
                     hasNext = last ? hasAfter : true;
                     nxt = last ? after : nb0;
                 }
-                uint64_t prev = v0 & msk;
-                for (int e = 1; e < nE; e++) {
-                    const int off = 8 * w * e;
-                    const uint64_t cur = (off < 64 ? v0 >> off : v1 >> (off - 64)) & msk;
-                    const uint64_t z = zz_delta(prev, cur, w);
-                    mx = z > mx ? z : mx;
-                    prev = cur;
-                }
-                if (hasNext) {
-                    const uint64_t z = zz_delta(prev, nxt, w);
-                    mx = z > mx ? z : mx;
+                switch (w) {
+                case 1: mx = lane_zz_max<1>(v0, v1, nxt, hasNext, mx); break;
+                case 2: mx = lane_zz_max<2>(v0, v1, nxt, hasNext, mx); break;
+                case 4: mx = lane_zz_max<4>(v0, v1, nxt, hasNext, mx); break;
+                default: mx = lane_zz_max<8>(v0, v1, nxt, hasNext, mx); break;
                 }
             }
         }
@@ -497,29 +537,50 @@ template <> struct ElemT<8> { using T = uint64_t; };
 
 // zigzag deltas d0+tid+256k (k < kPer) of an item: every load of the item is
 // issued before the first use, so their latency overlaps
-template <int W>
+template <int W, int KP>
 __device__ inline void item_zz(const uint8_t* x, uint64_t d0, uint64_t d1, uint64_t* z)
 {
     using T = typename ElemT<W>::T;
-    T a[kPer], b[kPer];
+    T a[KP], b[KP];
 #pragma unroll
-    for (int k = 0; k < kPer; k++) {
+    for (int k = 0; k < KP; k++) {
         const uint64_t d = d0 + threadIdx.x + (uint64_t)kCT * k;
         const uint64_t dd = d < d1 ? d : d0;
         a[k] = (T)ozl_gld_w(x + dd * W, W);
         b[k] = (T)ozl_gld_w(x + (dd + 1) * W, W);
     }
 #pragma unroll
-    for (int k = 0; k < kPer; k++) z[k] = zz_delta((uint64_t)a[k], (uint64_t)b[k], W);
+    for (int k = 0; k < KP; k++) z[k] = zz_delta((uint64_t)a[k], (uint64_t)b[k], W);
 }
 
+// the same for narrow columns, in 32 bits
+template <int W, int KP>
+__device__ inline void item_zz32(const uint8_t* x, uint64_t d0, uint64_t d1, uint32_t* z)
+{
+    constexpr uint32_t M = mask32(W);
+    uint32_t a[KP], b[KP];
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+        const uint64_t d = d0 + threadIdx.x + (uint64_t)kCT * k;
+        const uint64_t dd = d < d1 ? d : d0;
+        a[k] = (uint32_t)ozl_gld_w(x + dd * W, W);
+        b[k] = (uint32_t)ozl_gld_w(x + (dd + 1) * W, W);
+    }
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+        const uint32_t d = (b[k] - a[k]) & M;
+        z[k] = ((d << 1) ^ (0u - (d >> ((8 * W - 1) & 31)))) & M;
+    }
+}
+
+template <int KP>
 __device__ inline void item_zz_w(int w, const uint8_t* x, uint64_t d0, uint64_t d1, uint64_t* z)
 {
     switch (w) {
-    case 1: item_zz<1>(x, d0, d1, z); break;
-    case 2: item_zz<2>(x, d0, d1, z); break;
-    case 4: item_zz<4>(x, d0, d1, z); break;
-    default: item_zz<8>(x, d0, d1, z); break;
+    case 1: item_zz<1, KP>(x, d0, d1, z); break;
+    case 2: item_zz<2, KP>(x, d0, d1, z); break;
+    case 4: item_zz<4, KP>(x, d0, d1, z); break;
+    default: item_zz<8, KP>(x, d0, d1, z); break;
     }
 }
 
@@ -633,41 +694,84 @@ __global__ void __launch_bounds__(kCT) k_col_scan_frames(uint32_t K, uint64_t* _
 // builds whole 32-bit words of the item's bits from the deltas that overlap
 // them (items start at multiples of 32 deltas, so on word boundaries of the
 // stream) and consecutive threads store consecutive words.
+// NARROW takes the items of columns of width <= 4, the wide variant the rest
+template <bool NARROW>
 __global__ void __launch_bounds__(kCT) k_col_pack(uint32_t K, ColWs ws, uint8_t* __restrict__ dst,
                                                  const uint64_t* __restrict__ frameOff)
 {
+    constexpr int KP = per_of<NARROW>();
     __shared__ uint32_t sItem, sCol;
-    __shared__ uint64_t Z[kTileMax + kTileMax / 16 + 1];   // element e at e + e / 16
+    // narrow: u32 element e at e + e / 32; wide: u64 element e at e + e / 16
+    __shared__ uint64_t Z[kTileWide + kTileWide / 16 + 2];
+    if (!NARROW && ws.ctr->nWide == 0) return;
     const uint32_t total = ws.ctr->totalItems;
     const uint32_t g = grab_size(total);
     uint32_t cur = 0, end = 0;
     for (;;) {
-        const uint32_t it = next_item(&ws.ctr->queue[1], total, g, &sItem, cur, end);
+        const uint32_t it = (uint32_t)__builtin_amdgcn_readfirstlane(
+            (int)next_item(&ws.ctr->queue[NARROW ? 1 : 5], total, g, &sItem, cur, end));
         if (it == ~0u) break;
         if (threadIdx.x == 0) sCol = item_col(ws)[it];
         __syncthreads();
-        const uint32_t c = sCol;
+        const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)sCol);
         const ColPlan& P = ws.plans[c];
-        if (P.status) continue;
+        if (P.status || (P.w <= 4) != NARROW) continue;
         const int nb = P.nb;
         const uint32_t t = it - P.itemBase;
         const uint64_t d0 = (uint64_t)t * P.tile, d1 = d0 + P.tile < P.m ? d0 + P.tile : P.m;
         const uint32_t cnt = (uint32_t)(d1 - d0);
         uint8_t* fr = dst + frameOff[c];
         if (t == 0 && threadIdx.x < P.hdrLen) ozl_gst1(fr + threadIdx.x, ws.hdr[(size_t)c * kHdrSlot + threadIdx.x]);
-        {
-            uint64_t z[kPer];
-            item_zz_w(P.w, reinterpret_cast<const uint8_t*>((uintptr_t)P.src), d0, d1, z);
+        uint8_t* o = fr + P.hdrLen + d0 * (uint64_t)nb / 8;
+        const uint64_t nbits = (uint64_t)cnt * nb;
+        const uint32_t nW = (uint32_t)(nbits / 32), rem = (uint32_t)(nbits % 32);
+        if constexpr (NARROW) {
+            // 32-bit deltas and words
+            uint32_t* Z32 = reinterpret_cast<uint32_t*>(Z);
+            {
+                uint32_t z[KP];
+                const uint8_t* x = reinterpret_cast<const uint8_t*>((uintptr_t)P.src);
+                switch (P.w) {
+                case 1: item_zz32<1, KP>(x, d0, d1, z); break;
+                case 2: item_zz32<2, KP>(x, d0, d1, z); break;
+                default: item_zz32<4, KP>(x, d0, d1, z); break;
+                }
 #pragma unroll
-            for (int k = 0; k < kPer; k++) {
+                for (int k = 0; k < KP; k++) {
+                    const uint32_t i = threadIdx.x + kCT * k;
+                    if (i < cnt) Z32[i + (i >> 5)] = z[k];
+                }
+            }
+            __syncthreads();
+            const uint32_t nb32 = (uint32_t)nb, nbits32 = (uint32_t)nbits;
+            for (uint32_t wi = threadIdx.x; wi < nW + (rem ? 1u : 0u); wi += kCT) {
+                const uint32_t bit0 = 32 * wi;
+                uint32_t e = bit0 / nb32;
+                const uint32_t bend = bit0 + 32 < nbits32 ? bit0 + 32 : nbits32;
+                uint32_t word = 0;
+                for (; e * nb32 < bend; e++) {
+                    const int off = (int)(e * nb32) - (int)bit0;
+                    const uint32_t v = Z32[e + (e >> 5)];
+                    word |= off >= 0 ? v << off : v >> (-off);
+                }
+                if (wi < nW) {
+                    ozl_gst4(o + 4 * (size_t)wi, word);
+                } else {
+                    for (uint32_t k = 0; 8 * k < rem; k++) ozl_gst1(o + 4 * (size_t)wi + k, word >> (8 * k));
+                }
+            }
+            continue;
+        }
+        {
+            uint64_t z[KP];
+            item_zz_w<KP>(P.w, reinterpret_cast<const uint8_t*>((uintptr_t)P.src), d0, d1, z);
+#pragma unroll
+            for (int k = 0; k < KP; k++) {
                 const uint32_t i = threadIdx.x + kCT * k;
                 if (i < cnt) Z[i + (i >> 4)] = z[k];
             }
         }
         __syncthreads();
-        uint8_t* o = fr + P.hdrLen + d0 * (uint64_t)nb / 8;
-        const uint64_t nbits = (uint64_t)cnt * nb;
-        const uint32_t nW = (uint32_t)(nbits / 32), rem = (uint32_t)(nbits % 32);
         for (uint32_t wi = threadIdx.x; wi < nW + (rem ? 1u : 0u); wi += kCT) {
             const uint64_t bit0 = 32 * (uint64_t)wi;
             uint32_t e = (uint32_t)(bit0 / nb);
@@ -802,7 +906,8 @@ __global__ void __launch_bounds__(64) k_cold_parse(const uint8_t* __restrict__ f
                     for (int k = 0; k < w; k++) out[(d + 1) * w + k] = (uint8_t)(v >> (8 * k));
                 }
             } else {
-                const uint32_t t = tile < kTileMax ? tile : kTileMax;
+                const uint32_t tmax = w <= 4 ? kTileMax : kTileWide;
+                const uint32_t t = tile < tmax ? tile : tmax;
                 p.tile = t;
                 p.nItems = (uint32_t)((m + t - 1) / t);
             }
@@ -819,7 +924,9 @@ __global__ void __launch_bounds__(64) k_cold_parse(const uint8_t* __restrict__ f
 // LDS slot of word i: one pad word after every 32, so lanes that read runs
 // of words at a fixed stride (16 deltas per lane) fall into different banks
 __device__ inline uint32_t sk(uint32_t i) { return i + (i >> 5); }
-constexpr uint32_t kWdWords = 2 * kTileMax + 4 + (2 * kTileMax + 4) / 32 + 2;
+// packed words of an item (<= 64 bits per delta), later its staged output
+// (<= 16 KiB), skewed
+constexpr uint32_t kWdWords = 2 * kTileWide + 4 + (2 * kTileWide + 4) / 32 + 2;
 
 __device__ inline uint32_t load_item_words(const uint8_t* stream, uint64_t d0, uint32_t cnt, int nb, uint32_t* Wd)
 {
@@ -839,31 +946,58 @@ __device__ inline uint64_t item_value(const uint32_t* Wd, uint32_t e, int nb)
     return nb == 64 ? v : v & ((1ull << nb) - 1);
 }
 
+// Narrow columns (width <= 4, so nbBits <= 32): value e is one funnel shift
+// of two skewed words, and sums and zigzag run in 32 bits (sums are only ever
+// used modulo 2^(8w)).
+__device__ inline uint32_t item_value32(const uint32_t* Wd, uint32_t e, int nb)
+{
+    const uint32_t bit = e * (uint32_t)nb;
+    const uint32_t q = bit >> 5, sh = bit & 31;
+    const uint32_t v = __builtin_amdgcn_alignbit(Wd[sk(q + 1)], Wd[sk(q)], sh);
+    return nb == 32 ? v : v & ((1u << nb) - 1u);
+}
+__device__ inline uint32_t unzz32(uint32_t z) { return (z >> 1) ^ (0u - (z & 1u)); }
+
 // tile sums of the deltas (mod 2^64; reduced mod 2^(8w) when used)
+template <bool NARROW>
 __global__ void __launch_bounds__(kCT) k_cold_sum(uint32_t K, ColWs ws)
 {
+    constexpr int KP = per_of<NARROW>();
     __shared__ uint32_t sItem, sCol;
     __shared__ uint64_t lds[kCT / 64];
     __shared__ __attribute__((aligned(16))) uint32_t Wd[kWdWords];
+    if (!NARROW && ws.ctr->nWide == 0) return;
     const uint32_t total = ws.ctr->totalItems;
     const uint32_t g = grab_size(total);
     uint32_t cur = 0, end = 0;
     for (;;) {
-        const uint32_t it = next_item(&ws.ctr->queue[0], total, g, &sItem, cur, end);
+        const uint32_t it = (uint32_t)__builtin_amdgcn_readfirstlane(
+            (int)next_item(&ws.ctr->queue[NARROW ? 0 : 4], total, g, &sItem, cur, end));
         if (it == ~0u) break;
         if (threadIdx.x == 0) sCol = item_col(ws)[it];
         __syncthreads();
-        const ColPlan& P = ws.plans[sCol];
+        const ColPlan& P = ws.plans[(uint32_t)__builtin_amdgcn_readfirstlane((int)sCol)];
+        if ((P.w <= 4) != NARROW) continue;
         const int w = P.w, nb = P.nb;
         const uint64_t d0 = (uint64_t)(it - P.itemBase) * P.tile, d1 = d0 + P.tile < P.m ? d0 + P.tile : P.m;
         const uint32_t cnt = (uint32_t)(d1 - d0);
         load_item_words(reinterpret_cast<const uint8_t*>((uintptr_t)P.src), d0, cnt, nb, Wd);
         __syncthreads();
         uint64_t s = 0;
+        if constexpr (NARROW) {
+            uint32_t s32 = 0;
 #pragma unroll
-        for (int k = 0; k < kPer; k++) {
-            const uint32_t e = threadIdx.x + kCT * k;
-            if (e < cnt) s += unzz(item_value(Wd, e, nb), w);
+            for (int k = 0; k < KP; k++) {
+                const uint32_t e = threadIdx.x + kCT * k;
+                if (e < cnt) s32 += unzz32(item_value32(Wd, e, nb));
+            }
+            s = s32;
+        } else {
+#pragma unroll
+            for (int k = 0; k < KP; k++) {
+                const uint32_t e = threadIdx.x + kCT * k;
+                if (e < cnt) s += unzz(item_value(Wd, e, nb), w);
+            }
         }
         s = block_sum<uint64_t>(s, lds);
         if (threadIdx.x == 0) tile_sums(ws, total)[it] = s;
@@ -896,22 +1030,27 @@ __global__ void __launch_bounds__(64) k_cold_carry(uint32_t K, ColWs ws)
 // unpack, prefix sum within the item (thread t owns deltas [16t, 16t+16)),
 // add the item's carry; the elements are staged in LDS and leave in
 // consecutive dwords
+template <bool NARROW>
 __global__ void __launch_bounds__(kCT) k_cold_apply(uint32_t K, ColWs ws, uint8_t* __restrict__ dst,
                                                    const uint64_t* __restrict__ doff)
 {
+    constexpr int KP = per_of<NARROW>();
     __shared__ uint32_t sItem, sCol;
     __shared__ uint64_t lds[kCT / 64];
     __shared__ __attribute__((aligned(16))) uint32_t Wd[kWdWords];
+    if (!NARROW && ws.ctr->nWide == 0) return;
     const uint32_t total = ws.ctr->totalItems;
     const uint32_t g = grab_size(total);
     uint32_t cur = 0, end = 0;
     for (;;) {
-        const uint32_t it = next_item(&ws.ctr->queue[1], total, g, &sItem, cur, end);
+        const uint32_t it = (uint32_t)__builtin_amdgcn_readfirstlane(
+            (int)next_item(&ws.ctr->queue[NARROW ? 1 : 5], total, g, &sItem, cur, end));
         if (it == ~0u) break;
         if (threadIdx.x == 0) sCol = item_col(ws)[it];
         __syncthreads();
-        const uint32_t c = sCol;
+        const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)sCol);
         const ColPlan& P = ws.plans[c];
+        if ((P.w <= 4) != NARROW) continue;
         const int w = P.w, nb = P.nb;
         const uint32_t t = it - P.itemBase;
         const uint64_t d0 = (uint64_t)t * P.tile, d1 = d0 + P.tile < P.m ? d0 + P.tile : P.m;
@@ -922,22 +1061,36 @@ __global__ void __launch_bounds__(kCT) k_cold_apply(uint32_t K, ColWs ws, uint8_
             for (int k = 0; k < w; k++) out[k] = (uint8_t)(P.maxZ >> (8 * k));
         load_item_words(reinterpret_cast<const uint8_t*>((uintptr_t)P.src), d0, cnt, nb, Wd);
         __syncthreads();
-        uint64_t v[kPer];
-        uint64_t run = 0;
-        const uint32_t e0 = kPer * threadIdx.x;
+        uint64_t v[KP];
+        uint64_t pre;
+        const uint32_t e0 = KP * threadIdx.x;
+        const uint64_t carry = tile_sums(ws, total)[it];
+        if constexpr (NARROW) {
+            uint32_t run = 0;
 #pragma unroll
-        for (int k = 0; k < kPer; k++) {
-            const uint32_t e = e0 + k;
-            run += e < cnt ? unzz(item_value(Wd, e, nb), w) : 0;
-            v[k] = run;
+            for (int k = 0; k < KP; k++) {
+                const uint32_t e = e0 + k;
+                run += e < cnt ? unzz32(item_value32(Wd, e, nb)) : 0u;
+                v[k] = run;
+            }
+            uint32_t tot;
+            pre = (uint32_t)(block_excl<uint32_t>(run, reinterpret_cast<uint32_t*>(lds), tot) + (uint32_t)carry);
+        } else {
+            uint64_t run = 0;
+#pragma unroll
+            for (int k = 0; k < KP; k++) {
+                const uint32_t e = e0 + k;
+                run += e < cnt ? unzz(item_value(Wd, e, nb), w) : 0;
+                v[k] = run;
+            }
+            uint64_t tot;
+            pre = block_excl<uint64_t>(run, lds, tot) + carry;
         }
-        uint64_t tot;
-        const uint64_t pre = block_excl<uint64_t>(run, lds, tot) + tile_sums(ws, total)[it];
         __syncthreads();  // every thread is done with the words
         // staged in skewed words (byte a at word sk(a / 4), byte a % 4)
         uint8_t* xb = reinterpret_cast<uint8_t*>(Wd);
 #pragma unroll
-        for (int k = 0; k < kPer; k++) {
+        for (int k = 0; k < KP; k++) {
             const uint32_t e = e0 + k;
             if (e < cnt) {
                 const uint64_t x = (pre + v[k]) & mask;
@@ -1056,7 +1209,8 @@ int ozl_columns_compress_batch(const void* src, const uint64_t* colOff, const ui
     hash_pass(K, ws, 2, st);
     OZL_LAUNCH("k_col_plan", k_col_plan, dim3(K), dim3(64), 0, st, K, ws, frameOff);
     OZL_LAUNCH("k_col_scan_frames", k_col_scan_frames, dim3(1), dim3(kCT), 0, st, K, frameOff, dstCap, ws);
-    OZL_LAUNCH("k_col_pack", k_col_pack, dim3(G), dim3(kCT), 0, st, K, ws, d, (const uint64_t*)frameOff);
+    OZL_LAUNCH("k_col_pack", k_col_pack<true>, dim3(G), dim3(kCT), 0, st, K, ws, d, (const uint64_t*)frameOff);
+    OZL_LAUNCH("k_col_pack_wide", k_col_pack<false>, dim3(G), dim3(kCT), 0, st, K, ws, d, (const uint64_t*)frameOff);
     OZL_LAUNCH("k_col_finish1", k_col_finish1, dim3(K), dim3(64), 0, st, K, ws, d, (const uint64_t*)frameOff);
     OZL_LAUNCH("k_hreg_frames", k_hreg_frames, gk, dim3(kCT), 0, st, K, ws, (const uint8_t*)d,
                (const uint64_t*)frameOff);
@@ -1083,9 +1237,11 @@ int ozl_columns_decompress_batch(const void* frames, const uint64_t* frameOff, u
     OZL_LAUNCH("k_cold_parse", k_cold_parse, dim3(K), dim3(64), 0, st, f, frameOff, K, d, dstOff, tile, ws);
     OZL_LAUNCH("k_col_scan_items", k_col_scan_items, dim3(1), dim3(kCT), 0, st, K, ws);
     OZL_LAUNCH("k_col_map", k_col_map, dim3(K), dim3(64), 0, st, K, ws);
-    OZL_LAUNCH("k_cold_sum", k_cold_sum, dim3(G), dim3(kCT), 0, st, K, ws);
+    OZL_LAUNCH("k_cold_sum", k_cold_sum<true>, dim3(G), dim3(kCT), 0, st, K, ws);
+    OZL_LAUNCH("k_cold_sum_wide", k_cold_sum<false>, dim3(G), dim3(kCT), 0, st, K, ws);
     OZL_LAUNCH("k_cold_carry", k_cold_carry, dim3(K), dim3(64), 0, st, K, ws);
-    OZL_LAUNCH("k_cold_apply", k_cold_apply, dim3(G), dim3(kCT), 0, st, K, ws, d, dstOff);
+    OZL_LAUNCH("k_cold_apply", k_cold_apply<true>, dim3(G), dim3(kCT), 0, st, K, ws, d, dstOff);
+    OZL_LAUNCH("k_cold_apply_wide", k_cold_apply<false>, dim3(G), dim3(kCT), 0, st, K, ws, d, dstOff);
     OZL_LAUNCH("k_hreg_decode", k_hreg_decode, gk, dim3(kCT), 0, st, K, ws, f, frameOff, (const uint8_t*)d, dstOff);
     hash_pass(2 * K, ws, 2, st);
     OZL_LAUNCH("k_cold_verify", k_cold_verify, gk, dim3(kCT), 0, st, K, ws, status);
