@@ -381,11 +381,10 @@ __device__ inline unsigned long long lane_zz_max(uint64_t v0, uint64_t v1, uint6
         uint32_t prev = dw[0] & M, m32 = 0;
 #pragma unroll
         for (int e = 1; e <= nE; e++) {
-            if (e == nE && !hasNext) break;
             const uint32_t cur = e < nE ? (dw[(e * W) / 4] >> (8 * ((e * W) % 4))) & M : (uint32_t)nxt & M;
             const uint32_t d = (cur - prev) & M;
             const uint32_t z = ((d << 1) ^ (0u - (d >> (8 * W - 1)))) & M;
-            m32 = z > m32 ? z : m32;
+            if (e < nE || hasNext) m32 = z > m32 ? z : m32;
             prev = cur;
         }
         return (unsigned long long)m32 > mx ? (unsigned long long)m32 : mx;
@@ -422,7 +421,6 @@ __global__ void __launch_bounds__(64) k_hash_sums(ColWs ws, int q)
         const HashReg h = ws.hreg[map[it]];
         const uint32_t b0 = (it - h.itemBase) * kHG, b1 = b0 + kHG < h.nBlocks ? b0 + kHG : h.nBlocks;
         const uint8_t* in = reinterpret_cast<const uint8_t*>((uintptr_t)h.ptr);
-        const bool al = (h.ptr & 15) == 0;
         unsigned long long mx = 0;
         // the item's blocks are read up front (addresses clamped into the
         // item, so every load is issued unconditionally), then summed
@@ -430,7 +428,7 @@ __global__ void __launch_bounds__(64) k_hash_sums(ColWs ws, int q)
 #pragma unroll
         for (uint32_t k = 0; k < kHG; k++) {
             const uint32_t b = b0 + k < b1 ? b0 + k : b1 - 1;
-            ozl_xxh3_block_load(in + (size_t)b * 1024, al, bw[k][0], bw[k][1]);
+            ozl_xxh3_block_load(in + (size_t)b * 1024, ozl_xxh3_mode(in, h.len, b), bw[k][0], bw[k][1]);
         }
         const uint64_t msk = wmask(h.zw ? (int)h.zw : 1);
         const int w = (int)h.zw;
@@ -441,7 +439,7 @@ __global__ void __launch_bounds__(64) k_hash_sums(ColWs ws, int q)
         if (hasAfter && lane == 63) after = ld_elem(in + nOff, w);
 #pragma unroll
         for (uint32_t k = 0; k < kHG; k++) {
-            if (b0 + k >= b1) break;
+            if (b0 + k >= b1) continue;
             const uint32_t b = b0 + k;
             uint64_t sa, sb;
             ozl_xxh3_sums_of(bw[k][0], bw[k][1], L, sa, sb);

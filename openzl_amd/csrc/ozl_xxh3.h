@@ -198,19 +198,50 @@ __device__ inline ozl_xxh3_lane ozl_xxh3_lane_init()
     L.c1 = ozl_sec64(192 - 64 + 8 * i0 + 8);
     return L;
 }
-// this lane's 16 bytes of the 1 KiB block at `blk`, as two words
-__device__ inline void ozl_xxh3_block_load(const uint8_t* blk, bool aligned, uint64_t& v0, uint64_t& v1)
+// this lane's 16 bytes of the 1 KiB block at `blk`, as two words. mode 1:
+// the block is 16-byte aligned (one 16-byte load); mode 2: unaligned, with at
+// least 16 readable bytes after the block (two aligned 16-byte loads, the
+// second one the next lane's first, and a funnel shift: whole cache lines per
+// instruction); mode 0: four dword loads.
+template <int Q>
+__device__ inline void ozl_xxh3_pick(const uint32_t* d, uint32_t bs, uint32_t& d0, uint32_t& d1, uint32_t& d2,
+                                     uint32_t& d3)
+{
+    d0 = __builtin_amdgcn_alignbit(d[Q + 1], d[Q], bs);
+    d1 = __builtin_amdgcn_alignbit(d[Q + 2], d[Q + 1], bs);
+    d2 = __builtin_amdgcn_alignbit(d[Q + 3], d[Q + 2], bs);
+    d3 = __builtin_amdgcn_alignbit(d[Q + 4], d[Q + 3], bs);
+}
+__device__ inline void ozl_xxh3_block_load(const uint8_t* blk, int mode, uint64_t& v0, uint64_t& v1)
 {
     const uint8_t* p = blk + 16 * (threadIdx.x & 63);
     uint32_t d0, d1, d2, d3;
-    if (aligned) {
+    if (mode == 1) {
         const ozl_u32x4 q = ozl_gld16(p);
         d0 = q.x; d1 = q.y; d2 = q.z; d3 = q.w;
+    } else if (mode == 2) {
+        const uintptr_t a = (uintptr_t)p;
+        const uint8_t* base = reinterpret_cast<const uint8_t*>(a & ~(uintptr_t)15);
+        const uint32_t sh = (uint32_t)(a & 15), bs = 8 * (sh & 3);
+        const ozl_u32x4 lo = ozl_gld16(base), hi = ozl_gld16(base + 16);
+        const uint32_t d[8] = { lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w };
+        switch (sh >> 2) {
+        case 0: ozl_xxh3_pick<0>(d, bs, d0, d1, d2, d3); break;
+        case 1: ozl_xxh3_pick<1>(d, bs, d0, d1, d2, d3); break;
+        case 2: ozl_xxh3_pick<2>(d, bs, d0, d1, d2, d3); break;
+        default: ozl_xxh3_pick<3>(d, bs, d0, d1, d2, d3); break;
+        }
     } else {
         d0 = ozl_gld4(p); d1 = ozl_gld4(p + 4); d2 = ozl_gld4(p + 8); d3 = ozl_gld4(p + 12);
     }
     v0 = d0 | ((uint64_t)d1 << 32);
     v1 = d2 | ((uint64_t)d3 << 32);
+}
+// load mode of block b of a region of len bytes at `in` (see ozl_xxh3_block_load)
+__device__ inline int ozl_xxh3_mode(const uint8_t* in, size_t len, size_t b)
+{
+    if (((uintptr_t)in & 15) == 0) return 1;
+    return (b + 1) * 1024 + 16 <= len ? 2 : 0;
 }
 // sums over the 16 stripes of a 1 KiB block for accumulators 2j and 2j+1,
 // in every lane of the pair's group, from the words ozl_xxh3_block_load read
@@ -228,11 +259,11 @@ __device__ inline void ozl_xxh3_sums_of(uint64_t v0, uint64_t v1, const ozl_xxh3
     sa += __shfl_xor(sa, 32, 64);
     sb += __shfl_xor(sb, 32, 64);
 }
-__device__ inline void ozl_xxh3_block_sums(const uint8_t* blk, bool aligned, const ozl_xxh3_lane& L, uint64_t& sa,
+__device__ inline void ozl_xxh3_block_sums(const uint8_t* blk, int mode, const ozl_xxh3_lane& L, uint64_t& sa,
                                            uint64_t& sb)
 {
     uint64_t v0, v1;
-    ozl_xxh3_block_load(blk, aligned, v0, v1);
+    ozl_xxh3_block_load(blk, mode, v0, v1);
     ozl_xxh3_sums_of(v0, v1, L, sa, sb);
 }
 __device__ inline uint64_t ozl_xxh3_scr(uint64_t a, uint64_t c)
@@ -271,10 +302,9 @@ __device__ inline uint64_t ozl_xxh3_wave(const uint8_t* in, size_t len, uint64_t
     const ozl_xxh3_lane L = ozl_xxh3_lane_init();
     uint64_t A = ozl_xxh3_init_acc(2 * L.j), B = ozl_xxh3_init_acc(2 * L.j + 1);
     const size_t nbBlocks = (len - 1) / 1024;
-    const bool al = ((uintptr_t)in & 15) == 0;
     for (size_t b = 0; b < nbBlocks; b++) {
         uint64_t sa, sb;
-        ozl_xxh3_block_sums(in + b * 1024, al, L, sa, sb);
+        ozl_xxh3_block_sums(in + b * 1024, ozl_xxh3_mode(in, len, b), L, sa, sb);
         A = ozl_xxh3_scr(A + sa, L.c0);
         B = ozl_xxh3_scr(B + sb, L.c1);
     }
