@@ -424,6 +424,99 @@ def bench_select(args):
     print(json.dumps(rec))
 
 
+def bench_engine(args):
+    """Single-frame workloads through the ZL_CCtx / ZL_DCtx engine (host
+    buffers in and out, so the numbers include the PCIe transfers):
+    timestamps: 4,194,304 sorted u32 (16 MiB, one frame), DELTA_INT ->
+    transpose_split -> ZL_GRAPH_FSE per plane; bf16: 536,870,912 bf16 N(0,
+    0.02) values (1 GiB) in 64 frames of 16 MiB, transpose_split -> plane 1
+    ZL_GRAPH_HUFFMAN, plane 0 ZL_GRAPH_FSE."""
+    import torch
+    from openzl_amd import api
+    from openzl_amd import datagen as G
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ozl_checkers as C
+    if args.workload == "timestamps":
+        x = G.timestamps_u32()
+        parts = [x]
+        kind, typ, width = api.DELTA_TS_FSE, api.NUMERIC, 4
+        workload = ("timestamps: 4,194,304 u32 (start 1.7e9, Poisson(1000) increments, mod 2^32), one frame, "
+                    "DELTA_INT -> transpose_split -> ZL_GRAPH_FSE per plane, format v23, checksums on")
+        dtype = "u32"
+    else:
+        w = G.bf16_normal()
+        seg = 8388608
+        parts = [w[a:a + seg] for a in range(0, w.size, seg)]
+        kind, typ, width = api.TS_HUF_FSE, api.STRUCT, 2
+        workload = ("bf16 weights: 536,870,912 bf16 N(0, 0.02) in 64 frames of 16 MiB, transpose_split -> plane 1 "
+                    "ZL_GRAPH_HUFFMAN, plane 0 ZL_GRAPH_FSE, format v23, checksums on")
+        dtype = "bf16"
+    U = sum(p.nbytes for p in parts)
+    comp = api.Compressor(kind)
+    dec = api.Decompressor()
+    frames = [comp.compress(p, typ, width) for p in parts]
+    Cb = sum(len(f) for f in frames)
+    for f, p in zip(frames, parts):
+        out, _, _ = dec.decompress(f, p.nbytes + 64)
+        assert out == p.tobytes()
+    for _ in range(args.warmup):
+        [comp.compress(p, typ, width) for p in parts]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for p in parts:
+            comp.compress(p, typ, width)
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        for f, p in zip(frames, parts):
+            dec.decompress(f, p.nbytes + 64)
+    t2 = time.perf_counter()
+    enc_ms = (t1 - t0) * 1000 / args.steps
+    dec_ms = (t2 - t1) * 1000 / args.steps
+    cpu = None
+    if args.cpu_sample > 0 and C.refh() is not None:
+        # the reference library on the same frames: one thread for the single
+        # timestamps frame, all host cores (one context per thread) for the bf16 frames
+        threads = 1 if len(parts) == 1 else max(1, min(16, len(os.sched_getaffinity(0))))
+        ctyp = {api.SERIAL: C.SERIAL, api.STRUCT: C.STRUCT, api.NUMERIC: C.NUMERIC}[typ]
+        res = [0] * threads
+        t_start = time.perf_counter()
+
+        def worker(tid):
+            i, nbytes = tid, 0
+            while time.perf_counter() - t_start < args.cpu_seconds:
+                p = parts[i % len(parts)]
+                f = C.ref_compress(kind, p, ctyp, width)
+                C.ref_decompress(f, p.nbytes + 64)
+                nbytes += p.nbytes
+                i += threads
+            res[tid] = nbytes
+        ths = [threading.Thread(target=worker, args=(t,)) for t in range(threads)]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        wall = time.perf_counter() - t_start
+        cpu = {"value": round(sum(res) / wall / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "reference",
+               "sample": f"the same frames cycled, compressed and decompressed by the reference library "
+                         f"({threads} thread(s)); {sum(res) / 1e9:.3f} GB in {wall:.1f} s wall"}
+    ms = enc_ms + dec_ms
+    both = 2 * (U + Cb) / (ms / 1000) / 1e9
+    rec = {
+        "metric": METRIC, "value": round(U / (ms / 1000) / 1e9, 3), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded generator)",
+        "config": {"workload": workload, "input_bytes": U, "frame_bytes": Cb, "ratio": round(U / Cb, 4),
+                   "frames": len(frames), "encode_ms": round(enc_ms, 3), "decode_ms": round(dec_ms, 3),
+                   "host_buffers": "yes: every call copies its input to the GPU and its output back (PCIe included)"},
+        "roofline": {"bound": "hbm", "kernel": "whole engine calls (host-orchestrated, PCIe included)",
+                     "achieved": round(both, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(both / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec))
+
+
 # ---------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
@@ -436,10 +529,11 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 3 device-to-device copies of 1 GiB (HBM counter calibration) and exit")
-    ap.add_argument("--workload", choices=["entropy", "columns", "select"], default="entropy",
+    ap.add_argument("--workload", choices=["entropy", "columns", "select", "timestamps", "bf16"], default="entropy",
                     help="entropy: the 16,384-block entropy stress (the headline line); columns: the batched "
                          "4,096-column delta/zigzag/bitpack workload; select: learned graph selection over "
-                         "8,192 chunks against exhaustive trial")
+                         "8,192 chunks against exhaustive trial; timestamps / bf16: the single-frame workloads "
+                         "through the ZL_CCtx engine")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus})",
@@ -450,6 +544,8 @@ def main():
         return bench_columns(args)
     if args.workload == "select":
         return bench_select(args)
+    if args.workload in ("timestamps", "bf16"):
+        return bench_engine(args)
     import torch
     from openzl_amd import batch, _lib
 

@@ -26,6 +26,8 @@ int ozl_transpose_join(const void* planes, size_t n, int width, void* dst, void*
 int ozl_bitpack_decode(const void* src, size_t n, int width, int nbBits, void* dst, void* stream);
 }
 int ozl_xxh3_device(const uint8_t* data, size_t len, uint64_t* dOut, void* scratch, size_t scratchSize, hipStream_t st);
+int ozl_gather_sized(const uint8_t* const* srcs, const uint64_t* sizes, const uint64_t* dstOffs, int count,
+                     size_t maxSize, uint8_t* dst, hipStream_t st);
 size_t ozl_xxh3_scratch_size(size_t len);
 
 struct ozl_ditem {
@@ -66,6 +68,21 @@ __global__ void k_ncount_items(const NcItem* __restrict__ items, uint32_t B, int
     uint8_t* o = (uint8_t*)(uintptr_t)it.out;
     for (unsigned s = 0; s <= ms; s++) { o[2 * s] = (uint8_t)norm[s]; o[2 * s + 1] = (uint8_t)((uint16_t)norm[s] >> 8); }
     counts[b] = (int)ms + 1;
+}
+
+// constant chunks of one batch: item y fills [pieces of 16 KiB] with its byte
+struct FillItem {
+    uint64_t dst, n, val;
+};
+__global__ void __launch_bounds__(kT) k_fill_items(const FillItem* __restrict__ items)
+{
+    const FillItem it = items[blockIdx.y];
+    const uint64_t a = (uint64_t)blockIdx.x * 16384;
+    if (a >= it.n) return;
+    const uint64_t b = a + 16384 < it.n ? a + 16384 : it.n;
+    uint8_t* d = reinterpret_cast<uint8_t*>((uintptr_t)it.dst);
+    const uint8_t x = *reinterpret_cast<const uint8_t*>((uintptr_t)it.val);
+    for (uint64_t i = a + threadIdx.x; i < b; i += kT) d[i] = x;
 }
 
 __global__ void __launch_bounds__(kT) k_fill(uint8_t* __restrict__ dst, size_t n, const uint8_t* __restrict__ v)
@@ -150,6 +167,7 @@ struct DStream {
     int width = 1;
     bool ready = false;
     bool pending = false;
+    bool ncount = false;   // still the NCount header (dev, bytes): the FSE item reads it
 };
 
 struct DNode {
@@ -218,6 +236,9 @@ struct Decoder {
     std::vector<uint32_t> itemStream;
     std::vector<NcItem> nc;
     std::vector<uint32_t> ncStream;
+    std::vector<std::pair<int*, uint32_t>> statuses;   // per launched item batch, checked at the end
+    std::vector<FillItem> fills;                       // constant chunks waiting for one launch
+    std::vector<uint32_t> fillStream;
     int err = 0;
 
     uint8_t* alloc(size_t n)
@@ -229,6 +250,26 @@ struct Decoder {
 
     int flush()
     {
+        if (!fills.empty()) {
+            const uint32_t B = (uint32_t)fills.size();
+            uint64_t mx = 0;
+            for (const FillItem& f : fills) mx = f.n > mx ? f.n : mx;
+            FillItem* dF = (FillItem*)alloc(B * sizeof(FillItem));
+            if (err) return err;
+            OZL_HIP_CHECK(hipMemcpyAsync(dF, host_keep(fills.data(), B * sizeof(FillItem)), B * sizeof(FillItem),
+                                         hipMemcpyHostToDevice, st));
+            for (uint32_t y0 = 0; y0 < B; y0 += 65535) {
+                const uint32_t cnt = B - y0 < 65535 ? B - y0 : 65535;
+                OZL_LAUNCH("k_fill_items", k_fill_items, dim3((unsigned)((mx + 16383) / 16384), cnt), dim3(kT), 0, st,
+                           dF + y0);
+            }
+            for (uint32_t k = 0; k < B; k++) {
+                S[fillStream[k]].pending = false;
+                S[fillStream[k]].ready = true;
+            }
+            fills.clear();
+            fillStream.clear();
+        }
         if (!nc.empty()) {
             uint32_t B = (uint32_t)nc.size();
             NcItem* dItems = (NcItem*)alloc(B * sizeof(NcItem));
@@ -255,25 +296,67 @@ struct Decoder {
             for (uint32_t k = 0; k < B; k++) {
                 (void)k;
             }
+            // launched in stream order without waiting: later nodes that read
+            // these outputs are enqueued after them; the statuses are read
+            // once, after the whole frame (check_statuses)
             ozl_ditem* dItems = (ozl_ditem*)alloc(B * sizeof(ozl_ditem));
             int* dStatus = (int*)alloc(B * 4);
             size_t wsz = ozl_entropy_items_workspace(B);
             uint8_t* ws = alloc(wsz);
+            uint8_t* hItems = host_keep(items.data(), B * sizeof(ozl_ditem));
             if (err) return err;
-            OZL_HIP_CHECK(hipMemcpyAsync(dItems, items.data(), B * sizeof(ozl_ditem), hipMemcpyHostToDevice, st));
+            OZL_HIP_CHECK(hipMemcpyAsync(dItems, hItems, B * sizeof(ozl_ditem), hipMemcpyHostToDevice, st));
             int rc = ozl_entropy_decode_items(dItems, B, dStatus, ws, wsz, st);
             if (rc) return rc;
-            std::vector<int> status(B);
-            OZL_HIP_CHECK(hipMemcpyAsync(status.data(), dStatus, B * 4, hipMemcpyDeviceToHost, st));
-            OZL_HIP_CHECK(hipStreamSynchronize(st));
+            statuses.push_back({ dStatus, B });
             for (uint32_t k = 0; k < B; k++) {
-                if (status[k]) return status[k];
                 S[itemStream[k]].pending = false;
                 S[itemStream[k]].ready = true;
             }
             items.clear();
             itemStream.clear();
         }
+        return 0;
+    }
+
+    // host bytes that asynchronous copies read: kept until the frame is done
+    std::vector<std::vector<uint8_t>> keep;
+    uint8_t* host_keep(const void* p, size_t n)
+    {
+        keep.emplace_back((const uint8_t*)p, (const uint8_t*)p + n);
+        return keep.back().data();
+    }
+
+    // copies of `count` device ranges into dst at the given offsets, one launch
+    int gather(const std::vector<const uint8_t*>& srcs, const std::vector<uint64_t>& sizes,
+               const std::vector<uint64_t>& offs, uint8_t* dst)
+    {
+        const size_t n = srcs.size();
+        if (n == 0) return 0;
+        std::vector<uint64_t> m(3 * n);
+        size_t mx = 0;
+        for (size_t k = 0; k < n; k++) {
+            m[k] = (uint64_t)(uintptr_t)srcs[k];
+            m[n + k] = sizes[k];
+            m[2 * n + k] = offs[k];
+            mx = sizes[k] > mx ? sizes[k] : mx;
+        }
+        uint64_t* dm = (uint64_t*)alloc(m.size() * 8);
+        if (err) return err;
+        OZL_HIP_CHECK(hipMemcpyAsync(dm, host_keep(m.data(), m.size() * 8), m.size() * 8, hipMemcpyHostToDevice, st));
+        return mx ? ozl_gather_sized((const uint8_t* const*)dm, dm + n, dm + 2 * n, (int)n, mx, dst, st) : 0;
+    }
+
+    // every launched item batch's status (one synchronisation, at the end)
+    int check_statuses()
+    {
+        for (auto& b : statuses) {
+            std::vector<int> st2(b.second);
+            OZL_HIP_CHECK(hipMemcpy(st2.data(), b.first, b.second * 4, hipMemcpyDeviceToHost));
+            for (int v : st2)
+                if (v) return v;
+        }
+        statuses.clear();
         return 0;
     }
 
@@ -294,15 +377,19 @@ struct Decoder {
         const uint32_t o0 = nd.firstOut;
         switch (nd.trid) {
         case OZL_TR_FSE_NCOUNT: {
+            // the normalized counts stay in their NCount form: the FSE item
+            // that takes them as its side stream reads the header on the
+            // device (no size has to come back to the host)
             int rc = need(nd);
             if (rc) return rc;
-            in.dev = alloc(512);
-            if (err) return err;
+            if (S[o0].bytes == 0 || S[o0].bytes > 0x7FFFFFFF) return OZL_ERR_CORRUPTION;
+            in.dev = S[o0].dev;
+            in.bytes = S[o0].bytes;
             in.type = T_NUMERIC_;
             in.width = 2;
-            in.pending = true;
-            nc.push_back({ (uint64_t)(uintptr_t)S[o0].dev, (uint64_t)(uintptr_t)in.dev, (uint32_t)S[o0].bytes, 0 });
-            ncStream.push_back(nd.in);
+            in.ncount = true;
+            in.ready = true;
+            in.pending = false;
             return 0;
         }
         case OZL_TR_FSE_V2:
@@ -324,7 +411,12 @@ struct Decoder {
             ozl_ditem it;
             memset(&it, 0, sizeof(it));
             it.side = (uint64_t)(uintptr_t)side.dev;
-            it.sideN = (uint32_t)(nd.trid == OZL_TR_FSE_V2 ? side.bytes / 2 : side.bytes);
+            if (side.ncount) {
+                if (nd.trid != OZL_TR_FSE_V2) return OZL_ERR_CORRUPTION;
+                it.sideN = 0x80000000u | (uint32_t)side.bytes;   // NCount header of sideN & 0x7FFFFFFF bytes
+            } else {
+                it.sideN = (uint32_t)(nd.trid == OZL_TR_FSE_V2 ? side.bytes / 2 : side.bytes);
+            }
             it.bits = (uint64_t)(uintptr_t)bits.dev;
             it.bitsLen = (uint32_t)bits.bytes;
             it.out = (uint64_t)(uintptr_t)in.dev;
@@ -352,10 +444,9 @@ struct Decoder {
             uint8_t* scratch = alloc(ozl_delta_decode_scratch_size(n, w));
             if (err) return err;
             if (n) {
-                OZL_HIP_CHECK(hipMemcpyAsync(first, nd.hdr.data(), w, hipMemcpyHostToDevice, st));
+                OZL_HIP_CHECK(hipMemcpyAsync(first, host_keep(nd.hdr.data(), w), w, hipMemcpyHostToDevice, st));
                 rc = ozl_delta_decode(first, dl.dev, n, w, in.dev, scratch, st);
                 if (rc) return rc;
-                OZL_HIP_CHECK(hipStreamSynchronize(st));  // header staging buffer
             }
             in.bytes = n * w;
             in.type = T_NUMERIC_;
@@ -383,8 +474,13 @@ struct Decoder {
             uint8_t* planes = alloc(n * W);
             in.dev = alloc(n * W);
             if (err) return err;
-            for (uint32_t p = 0; p < W; p++)
-                if (n) OZL_HIP_CHECK(hipMemcpyAsync(planes + (size_t)p * n, S[o0 + p].dev, n, hipMemcpyDeviceToDevice, st));
+            if (n) {
+                std::vector<const uint8_t*> srcs(W);
+                std::vector<uint64_t> sz(W, n), offs(W);
+                for (uint32_t p = 0; p < W; p++) { srcs[p] = S[o0 + p].dev; offs[p] = (uint64_t)p * n; }
+                rc = gather(srcs, sz, offs, planes);
+                if (rc) return rc;
+            }
             if (n) {
                 rc = ozl_transpose_join(planes, n, (int)W, in.dev, st);
                 if (rc) return rc;
@@ -474,10 +570,17 @@ struct Decoder {
             if (v.bytes != 1) return OZL_ERR_CORRUPTION;
             in.dev = alloc(n);
             if (err) return err;
-            if (n) OZL_LAUNCH("k_fill", k_fill, dim3(ozl_blocks(n, kT, 4096)), dim3(kT), 0, st, in.dev, (size_t)n, v.dev);
             in.bytes = n;
             in.type = T_SERIAL_;
             in.width = 1;
+            if (n) {
+                // batched with the other constant chunks (flush)
+                fills.push_back({ (uint64_t)(uintptr_t)in.dev, n, (uint64_t)(uintptr_t)v.dev });
+                fillStream.push_back(nd.in);
+                in.ready = false;
+                in.pending = true;
+                return 0;
+            }
             break;
         }
         case OZL_TR_SPLITN: {  // decode_splitN_binding.c:16 -- concatenation
@@ -485,12 +588,19 @@ struct Decoder {
             for (uint32_t k = 0; k < nd.nbVO; k++) total += S[o0 + k].bytes;
             in.dev = alloc(total);
             if (err) return err;
+            std::vector<const uint8_t*> srcs;
+            std::vector<uint64_t> sz, offs;
             size_t pos = 0;
             for (uint32_t k = 0; k < nd.nbVO; k++) {
-                if (S[o0 + k].bytes)
-                    OZL_HIP_CHECK(hipMemcpyAsync(in.dev + pos, S[o0 + k].dev, S[o0 + k].bytes, hipMemcpyDeviceToDevice, st));
+                if (S[o0 + k].bytes) {
+                    srcs.push_back(S[o0 + k].dev);
+                    sz.push_back(S[o0 + k].bytes);
+                    offs.push_back(pos);
+                }
                 pos += S[o0 + k].bytes;
             }
+            rc = gather(srcs, sz, offs, in.dev);
+            if (rc) return rc;
             in.bytes = total;
             in.type = T_SERIAL_;
             in.width = 1;
@@ -668,6 +778,8 @@ int decompress_impl(ZL_DCtx_s* dc, const uint8_t* src, size_t srcSize, void* dst
     OZL_HIP_CHECK(hipMemcpyAsync(hashes, dH, 16, hipMemcpyDeviceToHost, dc->st));
     if (out.bytes) OZL_HIP_CHECK(hipMemcpyAsync(dst, out.dev, out.bytes, hipMemcpyDeviceToHost, dc->st));
     OZL_HIP_CHECK(hipStreamSynchronize(dc->st));
+    rc = D.check_statuses();
+    if (rc) return rc;
     size_t cp = P.csumPos;
     if (P.flags & 2) {
         size_t at = cp + ((P.flags & 1) ? 4 : 0);

@@ -834,12 +834,24 @@ __global__ void __launch_bounds__(64) k_prep_items(const ozl_ditem* __restrict__
         const uint8_t* side = (const uint8_t*)(uintptr_t)it.side;
         if (!huf) {
             uint32_t sum = 0;
-            bool bad = it.sideN == 0 || it.sideN > 256;
+            bool bad = false;
+            uint32_t sideN = it.sideN;
             for (int s = 0; s < 256; s++) norm[s] = 0;
-            for (uint32_t s = 0; !bad && s < it.sideN; s++) {
-                norm[s] = (int16_t)(side[2 * s] | (side[2 * s + 1] << 8));
-                if (norm[s] < -1) bad = true;
-                sum += norm[s] == -1 ? 1 : (uint32_t)norm[s];
+            if (sideN & 0x80000000u) {
+                // the side stream is still an NCount header (fse_compress.c:246-357)
+                const uint32_t len = sideN & 0x7FFFFFFFu;
+                unsigned ms = 0, ntl = 0;
+                const long r = ozl_fse_read_ncount(norm, &ms, &ntl, side, len);
+                bad = r < 0 || (uint32_t)r != len;
+                sideN = bad ? 0u : ms + 1;
+                for (uint32_t s = 0; !bad && s < sideN; s++) sum += norm[s] == -1 ? 1 : (uint32_t)norm[s];
+            } else {
+                bad = sideN == 0 || sideN > 256;
+                for (uint32_t s = 0; !bad && s < sideN; s++) {
+                    norm[s] = (int16_t)(side[2 * s] | (side[2 * s + 1] << 8));
+                    if (norm[s] < -1) bad = true;
+                    sum += norm[s] == -1 ? 1 : (uint32_t)norm[s];
+                }
             }
             unsigned tl = sum ? (unsigned)ozl_hb32(sum) : 0;
             if (bad || sum == 0 || (sum & (sum - 1)) || tl < OZL_FSE_MIN_TABLELOG || tl > 11 ||
@@ -847,7 +859,7 @@ __global__ void __launch_bounds__(64) k_prep_items(const ozl_ditem* __restrict__
                 d.status = OZL_ERR_CORRUPTION;
             else {
                 d.tl = (uint8_t)tl;
-                d.aux = it.sideN - 1;
+                d.aux = sideN - 1;
             }
         } else {
             for (uint32_t s = 0; s < it.sideN && s < 256; s++) w[s] = side[s];

@@ -50,7 +50,36 @@ __global__ void __launch_bounds__(64) k_xxh3_finish(const uint8_t* __restrict__ 
     uint64_t acc = lane < 8 ? init[lane] : 0;
     size_t nbBlocks = (len - 1) / 1024;
     if (lane < 8) {
-        for (size_t b = 0; b < nbBlocks; b++) acc = ozl_scramble(acc + sums[b * 8 + lane], lane);
+        // the chain is serial: the block sums are read 32 blocks ahead of
+        // their use, and the lane's secret word is read once
+        const uint64_t sec = ozl_sec64(192 - 64 + 8 * lane);
+        constexpr int kAhead = 32;
+        uint64_t cur[kAhead], nxt[kAhead];
+        size_t b = 0;
+#pragma unroll
+        for (int k = 0; k < kAhead; k++) cur[k] = (size_t)k < nbBlocks ? sums[(size_t)k * 8 + lane] : 0;
+        for (; b + kAhead <= nbBlocks; b += kAhead) {
+#pragma unroll
+            for (int k = 0; k < kAhead; k++) {
+                const size_t q = b + kAhead + (size_t)k;
+                nxt[k] = q < nbBlocks ? sums[q * 8 + lane] : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kAhead; k++) {
+                uint64_t a2 = acc + cur[k];
+                a2 ^= a2 >> 47;
+                a2 ^= sec;
+                acc = a2 * OZL_P32_1;
+            }
+#pragma unroll
+            for (int k = 0; k < kAhead; k++) cur[k] = nxt[k];
+        }
+        for (; b < nbBlocks; b++) {
+            uint64_t a2 = acc + sums[b * 8 + lane];
+            a2 ^= a2 >> 47;
+            a2 ^= sec;
+            acc = a2 * OZL_P32_1;
+        }
         size_t nbStripes = ((len - 1) - 1024 * nbBlocks) / 64;
         for (size_t s = 0; s < nbStripes; s++) acc += ozl_stripe_term(in + nbBlocks * 1024 + s * 64, (int)s * 8, lane);
         acc += ozl_stripe_term(in + len - 64, 192 - 64 - 7, lane);
