@@ -49,27 +49,6 @@ namespace {
 
 constexpr int kT = 256;
 
-struct NcItem {
-    uint64_t src;
-    uint64_t out;   // int16[256]
-    uint32_t len;
-    uint32_t pad;
-};
-
-__global__ void k_ncount_items(const NcItem* __restrict__ items, uint32_t B, int* __restrict__ counts)
-{
-    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
-    NcItem it = items[b];
-    int16_t norm[256];
-    unsigned ms, tl;
-    long r = ozl_fse_read_ncount(norm, &ms, &tl, (const uint8_t*)(uintptr_t)it.src, it.len);
-    if (r < 0 || (size_t)r != it.len) { counts[b] = -1; return; }
-    uint8_t* o = (uint8_t*)(uintptr_t)it.out;
-    for (unsigned s = 0; s <= ms; s++) { o[2 * s] = (uint8_t)norm[s]; o[2 * s + 1] = (uint8_t)((uint16_t)norm[s] >> 8); }
-    counts[b] = (int)ms + 1;
-}
-
 // constant chunks of one batch: item y fills [pieces of 16 KiB] with its byte
 struct FillItem {
     uint64_t dst, n, val;
@@ -85,11 +64,6 @@ __global__ void __launch_bounds__(kT) k_fill_items(const FillItem* __restrict__ 
     for (uint64_t i = a + threadIdx.x; i < b; i += kT) d[i] = x;
 }
 
-__global__ void __launch_bounds__(kT) k_fill(uint8_t* __restrict__ dst, size_t n, const uint8_t* __restrict__ v)
-{
-    uint8_t x = *v;
-    for (size_t i = (size_t)blockIdx.x * kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) dst[i] = x;
-}
 
 // flatpack decode (decode_flatpack_kernel.c:359)
 __global__ void __launch_bounds__(kT) k_flatpack_decode(const uint8_t* __restrict__ alpha, uint32_t card,
@@ -233,12 +207,8 @@ struct Decoder {
     hipStream_t st;
     std::vector<DStream> S;
     std::vector<ozl_ditem> items;
-    std::vector<uint32_t> itemStream;
-    std::vector<NcItem> nc;
-    std::vector<uint32_t> ncStream;
     std::vector<std::pair<int*, uint32_t>> statuses;   // per launched item batch, checked at the end
     std::vector<FillItem> fills;                       // constant chunks waiting for one launch
-    std::vector<uint32_t> fillStream;
     int err = 0;
 
     uint8_t* alloc(size_t n)
@@ -263,32 +233,7 @@ struct Decoder {
                 OZL_LAUNCH("k_fill_items", k_fill_items, dim3((unsigned)((mx + 16383) / 16384), cnt), dim3(kT), 0, st,
                            dF + y0);
             }
-            for (uint32_t k = 0; k < B; k++) {
-                S[fillStream[k]].pending = false;
-                S[fillStream[k]].ready = true;
-            }
             fills.clear();
-            fillStream.clear();
-        }
-        if (!nc.empty()) {
-            uint32_t B = (uint32_t)nc.size();
-            NcItem* dItems = (NcItem*)alloc(B * sizeof(NcItem));
-            int* dCounts = (int*)alloc(B * 4);
-            if (err) return err;
-            OZL_HIP_CHECK(hipMemcpyAsync(dItems, nc.data(), B * sizeof(NcItem), hipMemcpyHostToDevice, st));
-            OZL_LAUNCH("k_ncount_items", k_ncount_items, dim3((B + 63) / 64), dim3(64), 0, st, dItems, B, dCounts);
-            std::vector<int> counts(B);
-            OZL_HIP_CHECK(hipMemcpyAsync(counts.data(), dCounts, B * 4, hipMemcpyDeviceToHost, st));
-            OZL_HIP_CHECK(hipStreamSynchronize(st));
-            for (uint32_t k = 0; k < B; k++) {
-                if (counts[k] <= 0) return OZL_ERR_CORRUPTION;
-                DStream& o = S[ncStream[k]];
-                o.bytes = (size_t)counts[k] * 2;
-                o.pending = false;
-                o.ready = true;
-            }
-            nc.clear();
-            ncStream.clear();
         }
         if (!items.empty()) {
             uint32_t B = (uint32_t)items.size();
@@ -309,13 +254,12 @@ struct Decoder {
             int rc = ozl_entropy_decode_items(dItems, B, dStatus, ws, wsz, st);
             if (rc) return rc;
             statuses.push_back({ dStatus, B });
-            for (uint32_t k = 0; k < B; k++) {
-                S[itemStream[k]].pending = false;
-                S[itemStream[k]].ready = true;
-            }
             items.clear();
-            itemStream.clear();
         }
+        // everything pending is now enqueued (conversions alias their
+        // producer's buffer, so they were pending too)
+        for (DStream& x : S)
+            if (x.pending) { x.pending = false; x.ready = true; }
         return 0;
     }
 
@@ -424,7 +368,32 @@ struct Decoder {
             it.huf = nd.trid == OZL_TR_HUFFMAN_V2;
             it.nbStates = nd.hdr[0];
             items.push_back(it);
-            itemStream.push_back(nd.in);
+            return 0;
+        }
+        case OZL_TR_CONV_NUM_TO_STRUCT_LE:
+        case OZL_TR_CONV_STRUCT_TO_SERIAL:
+        case OZL_TR_CONV_NUM_TO_SERIAL_LE: {
+            // byte-identical: the regenerated stream aliases the decoded one,
+            // pending or not (no wait for the batch that produces it)
+            const DStream& s = S[o0];
+            if (!s.ready && !s.pending) return OZL_ERR_CORRUPTION;
+            if (s.ncount) return OZL_ERR_CORRUPTION;
+            int type = T_NUMERIC_, width = s.width;
+            if (nd.trid == OZL_TR_CONV_STRUCT_TO_SERIAL) {
+                uint64_t w = 0;
+                for (size_t k = 0; k < nd.hdr.size() && k < 9; k++) w |= (uint64_t)(nd.hdr[k] & 127) << (7 * k);
+                if (w == 0 || s.bytes % w) return OZL_ERR_CORRUPTION;
+                type = T_STRUCT_;
+                width = (int)w;
+            } else if (nd.trid == OZL_TR_CONV_NUM_TO_SERIAL_LE) {
+                if (nd.hdr.size() != 1 || nd.hdr[0] > 3) return OZL_ERR_CORRUPTION;
+                const int w = 1 << nd.hdr[0];
+                if (s.bytes % w) return OZL_ERR_CORRUPTION;
+                width = w;
+            }
+            in = s;
+            in.type = type;
+            in.width = width;
             return 0;
         }
         default: break;
@@ -490,32 +459,6 @@ struct Decoder {
             in.width = (int)W;
             break;
         }
-        case OZL_TR_CONV_NUM_TO_STRUCT_LE: {
-            const DStream& s = S[o0];
-            in = s;
-            in.type = T_NUMERIC_;
-            break;
-        }
-        case OZL_TR_CONV_STRUCT_TO_SERIAL: {
-            const DStream& s = S[o0];
-            uint64_t w = 0;
-            for (size_t k = 0; k < nd.hdr.size() && k < 9; k++) w |= (uint64_t)(nd.hdr[k] & 127) << (7 * k);
-            if (w == 0 || s.bytes % w) return OZL_ERR_CORRUPTION;
-            in = s;
-            in.type = T_STRUCT_;
-            in.width = (int)w;
-            break;
-        }
-        case OZL_TR_CONV_NUM_TO_SERIAL_LE: {
-            const DStream& s = S[o0];
-            if (nd.hdr.size() != 1 || nd.hdr[0] > 3) return OZL_ERR_CORRUPTION;
-            int w = 1 << nd.hdr[0];
-            if (s.bytes % w) return OZL_ERR_CORRUPTION;
-            in = s;
-            in.type = T_NUMERIC_;
-            in.width = w;
-            break;
-        }
         case OZL_TR_BITPACK_SERIAL:
         case OZL_TR_BITPACK_INT: {  // decode_bitpack_binding.c:10-60
             const DStream& s = S[o0];
@@ -576,7 +519,6 @@ struct Decoder {
             if (n) {
                 // batched with the other constant chunks (flush)
                 fills.push_back({ (uint64_t)(uintptr_t)in.dev, n, (uint64_t)(uintptr_t)v.dev });
-                fillStream.push_back(nd.in);
                 in.ready = false;
                 in.pending = true;
                 return 0;
