@@ -296,7 +296,8 @@ struct Decoder {
     {
         for (auto& b : statuses) {
             std::vector<int> st2(b.second);
-            OZL_HIP_CHECK(hipMemcpy(st2.data(), b.first, b.second * 4, hipMemcpyDeviceToHost));
+            OZL_HIP_CHECK(hipMemcpyAsync(st2.data(), b.first, b.second * 4, hipMemcpyDeviceToHost, st));
+            OZL_HIP_CHECK(hipStreamSynchronize(st));
             for (int v : st2)
                 if (v) return v;
         }
@@ -659,7 +660,7 @@ int decompress_impl(ZL_DCtx_s* dc, const uint8_t* src, size_t srcSize, void* dst
     ParsedFrame P;
     int rc = parse_frame(src, srcSize, P);
     if (rc) return rc;
-    if (!dc->st && hipStreamCreate(&dc->st) != hipSuccess) return OZL_ERR_ALLOCATION;
+    if (!dc->st && hipStreamCreateWithFlags(&dc->st, hipStreamNonBlocking) != hipSuccess) return OZL_ERR_ALLOCATION;
     dc->reset();
     Decoder D;
     D.d = dc;

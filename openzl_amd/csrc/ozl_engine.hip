@@ -891,7 +891,8 @@ ZL_Report ZL_CCtx_compressTypedRef(ZL_CCtx* c, void* dst, size_t dstCapacity, co
     if (fp.formatVersion != (int)OZL_FORMAT_VERSION) return rep_err(ZL_ErrorCode_formatVersion_unsupported);
     if (in->type == T_NUMERIC && in->width != 1 && in->width != 2 && in->width != 4 && in->width != 8)
         return rep_err(ZL_ErrorCode_parameter_invalid);
-    if (!c->st && hipStreamCreate(&c->st) != hipSuccess) return rep_err(ZL_ErrorCode_allocation);
+    if (!c->st && hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
+        return rep_err(ZL_ErrorCode_allocation);
     c->arena.reset();
     size_t inBytes = in->n * in->width;
     Session S;
