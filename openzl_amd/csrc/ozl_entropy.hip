@@ -20,6 +20,7 @@
 #include "ozl_xxh3.h"
 #include "ozl_frame.h"
 #include "ozl_profile.h"
+#include "ozl_gmem.h"
 #include "ozl_entropy_api.h"
 #include "ozl_dparse.h"
 #include "ozl_wave.h"
@@ -711,7 +712,11 @@ __global__ void __launch_bounds__(kT) k_frame_write(const uint8_t* __restrict__ 
     for (int k = 0; k < sInfo[1]; k++) {
         const ozl_fstream s = st[order[k]];
         const uint8_t* from = s.src == OZL_SRC_INPUT ? in : s.src == OZL_SRC_PAYLOAD ? pay : aux + s.srcOff;
-        for (uint32_t i = threadIdx.x; i < s.size; i += kT) out[pos + i] = from[i];
+        // dword copies (unaligned both sides), then the tail bytes
+        uint8_t* to = out + pos;
+        const uint32_t nd = s.size / 4;
+        for (uint32_t i = threadIdx.x; i < nd; i += kT) ozl_gst4(to + 4 * (size_t)i, ozl_gld4(from + 4 * (size_t)i));
+        for (uint32_t i = 4 * nd + threadIdx.x; i < s.size; i += kT) ozl_gst1(to + i, ozl_gld1(from + i));
         pos += s.size;
     }
     if (threadIdx.x == 0) {
