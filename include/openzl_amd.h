@@ -206,6 +206,15 @@ void ZL_TypedRef_free(ZL_TypedRef* tref);
 ZL_Report ZL_CCtx_compressTypedRef(ZL_CCtx* cctx, void* dst, size_t dstCapacity, const ZL_TypedRef* input);
 /* include/openzl/zl_compress.h:194 */
 ZL_Report ZL_CCtx_compress(ZL_CCtx* cctx, void* dst, size_t dstCapacity, const void* src, size_t srcSize);
+/* nbInputs calls of ZL_CCtx_compressTypedRef (include/openzl/zl_compress.h:274)
+ * in one: frame i (the same bytes as that call writes for inputs[i]) lands in
+ * dst after frames 0..i-1, frameSizes[i] = its size; returns the total. The
+ * entropy stages of all frames run as one launch per coder. Inputs and dst
+ * may be host or device memory (device inputs are read in place); the
+ * single-frame calls accept device memory the same way. No counterpart in the
+ * reference: its callers loop over ZL_CCtx_compressTypedRef. */
+ZL_Report ozl_CCtx_compressBatch(ZL_CCtx* cctx, void* dst, size_t dstCapacity, size_t* frameSizes,
+                                 const ZL_TypedRef* const* inputs, size_t nbInputs);
 
 /* ------------------------------------------------------------ decompression */
 /* include/openzl/zl_decompress.h:355-366 */
@@ -225,6 +234,14 @@ void ZL_DCtx_free(ZL_DCtx* dctx);
 /* include/openzl/zl_decompress.h:388 */
 ZL_Report ZL_DCtx_decompressTyped(ZL_DCtx* dctx, ZL_OutputInfo* outputInfo, void* dst, size_t dstByteCapacity,
                                   const void* compressed, size_t cSize);
+/* nbFrames calls of ZL_DCtx_decompressTyped (include/openzl/zl_decompress.h:388)
+ * in one: frame i is src[sum of frameSizes[0..i), + frameSizes[i]); its
+ * content lands in dst after those of frames 0..i-1, outSizes[i] = its size;
+ * returns the total. The entropy stages of all frames run as shared launches.
+ * src and dst may be host or device memory. No counterpart in the reference:
+ * its callers loop over ZL_DCtx_decompressTyped. */
+ZL_Report ozl_DCtx_decompressBatch(ZL_DCtx* dctx, void* dst, size_t dstCapacity, size_t* outSizes, const void* src,
+                                   const size_t* frameSizes, size_t nbFrames);
 
 /* ------------------------------------------------------------ device batch entry points */
 /* Numeric transforms on device buffers (no reference counterpart: the

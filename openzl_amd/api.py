@@ -65,6 +65,8 @@ def _api():
         "ZL_DCtx_free": (None, [P]),
         "ZL_DCtx_decompressTyped": (ZLReport, [P, ctypes.POINTER(OutputInfo), P, SZ, P, SZ]),
         "ZL_ErrorCode_toString": (ctypes.c_char_p, [ctypes.c_int]),
+        "ozl_CCtx_compressBatch": (ZLReport, [P, P, SZ, ctypes.POINTER(SZ), ctypes.POINTER(P), SZ]),
+        "ozl_DCtx_decompressBatch": (ZLReport, [P, P, SZ, ctypes.POINTER(SZ), P, ctypes.POINTER(SZ), SZ]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -178,6 +180,42 @@ class Compressor:
             L.ZL_TypedRef_free(tr)
         return out[:size].tobytes()
 
+    def _typed_ref(self, ptr, nbytes, typ, width):
+        L = self.L
+        if typ == SERIAL:
+            return L.ZL_TypedRef_createSerial(ptr, nbytes)
+        if typ == STRUCT:
+            return L.ZL_TypedRef_createStruct(ptr, width, nbytes // width)
+        return L.ZL_TypedRef_createNumeric(ptr, width, nbytes // width)
+
+    def compress_ptrs(self, ptrs, nbytes, typ, width, dst, cap):
+        """ozl_CCtx_compressBatch over inputs at ptrs (host or device
+        addresses, nbytes[i] bytes each) into dst (host or device address,
+        cap bytes); returns the frame sizes."""
+        L = self.L
+        n = len(ptrs)
+        refs = [self._typed_ref(p, b, typ, width) for p, b in zip(ptrs, nbytes)]
+        sizes = (SZ * max(n, 1))()
+        try:
+            _chk(L.ozl_CCtx_compressBatch(self.cctx, dst, cap, sizes, (P * max(n, 1))(*refs), n), "compressBatch")
+        finally:
+            for r in refs:
+                L.ZL_TypedRef_free(r)
+        return [sizes[i] for i in range(n)]
+
+    def compress_batch(self, arrs, typ, width):
+        """One frame per host array, in one ozl_CCtx_compressBatch call."""
+        arrs = [np.ascontiguousarray(a) for a in arrs]
+        cap = sum(self.L.ZL_compressBound(a.nbytes) for a in arrs) + 64
+        out = np.empty(cap, np.uint8)
+        sizes = self.compress_ptrs([a.ctypes.data for a in arrs], [a.nbytes for a in arrs], typ, width,
+                                   out.ctypes.data, cap)
+        frames, pos = [], 0
+        for z in sizes:
+            frames.append(out[pos:pos + z].tobytes())
+            pos += z
+        return frames
+
     def __del__(self):
         try:
             self.L.ZL_CCtx_free(self.cctx)
@@ -200,6 +238,27 @@ class Decompressor:
         size = _chk(self.L.ZL_DCtx_decompressTyped(self.d, ctypes.byref(info), out.ctypes.data, out.size,
                                                    src.ctypes.data, src.size), "decompress")
         return out[:size].tobytes(), info.type, info.fixedWidth
+
+    def decompress_ptrs(self, src, frame_sizes, dst, cap):
+        """ozl_DCtx_decompressBatch of the frames back to back at src (host or
+        device address) into dst (host or device address, cap bytes); returns
+        the content sizes."""
+        n = len(frame_sizes)
+        fs = (SZ * max(n, 1))(*frame_sizes)
+        outs = (SZ * max(n, 1))()
+        _chk(self.L.ozl_DCtx_decompressBatch(self.d, dst, cap, outs, src, fs, n), "decompressBatch")
+        return [outs[i] for i in range(n)]
+
+    def decompress_batch(self, frames, capacity):
+        """Contents of host frames, decoded in one ozl_DCtx_decompressBatch call."""
+        src = np.frombuffer(b"".join(frames) or b"\0", np.uint8)
+        out = np.empty(max(capacity, 1), np.uint8)
+        sizes = self.decompress_ptrs(src.ctypes.data, [len(f) for f in frames], out.ctypes.data, out.size)
+        res, pos = [], 0
+        for z in sizes:
+            res.append(out[pos:pos + z].tobytes())
+            pos += z
+        return res
 
     def __del__(self):
         try:

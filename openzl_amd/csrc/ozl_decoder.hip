@@ -9,6 +9,7 @@
 // into one launch each until a later decoder needs their output.
 #include <string.h>
 
+#include <memory>
 #include <vector>
 
 #include "openzl_amd.h"
@@ -17,6 +18,7 @@
 #include "ozl_xxh3.h"
 #include "ozl_frame.h"
 #include "ozl_profile.h"
+#include "ozl_engine_util.h"
 
 extern "C" {
 int ozl_delta_decode(const void* first, const void* deltas, size_t n, int width, void* dst, void* scratch, void* stream);
@@ -25,10 +27,6 @@ int ozl_zigzag_decode(const void* src, size_t n, int width, void* dst, void* str
 int ozl_transpose_join(const void* planes, size_t n, int width, void* dst, void* stream);
 int ozl_bitpack_decode(const void* src, size_t n, int width, int nbBits, void* dst, void* stream);
 }
-int ozl_xxh3_device(const uint8_t* data, size_t len, uint64_t* dOut, void* scratch, size_t scratchSize, hipStream_t st);
-int ozl_gather_sized(const uint8_t* const* srcs, const uint64_t* sizes, const uint64_t* dstOffs, int count,
-                     size_t maxSize, uint8_t* dst, hipStream_t st);
-size_t ozl_xxh3_scratch_size(size_t len);
 
 struct ozl_ditem {
     uint64_t side;
@@ -202,13 +200,22 @@ struct ZL_DCtx_s {
 
 namespace {
 
-struct Decoder {
+// a decoder that needs a stream a queued launch has not produced yet
+constexpr int kBlocked = -1;
+
+struct Decoder;
+
+// Work queued by the decoders of one call (one frame or many): entropy items
+// and constant chunks go out in shared launches once every decoder waits for
+// one of their outputs; the launches' statuses are read once, at the end.
+struct DQueue {
     ZL_DCtx_s* d;
     hipStream_t st;
-    std::vector<DStream> S;
     std::vector<ozl_ditem> items;
-    std::vector<std::pair<int*, uint32_t>> statuses;   // per launched item batch, checked at the end
-    std::vector<FillItem> fills;                       // constant chunks waiting for one launch
+    std::vector<FillItem> fills;
+    std::vector<std::pair<int*, uint32_t>> statuses;
+    std::vector<std::vector<uint8_t>> keep;   // host bytes asynchronous copies read
+    std::vector<Decoder*> members;
     int err = 0;
 
     uint8_t* alloc(size_t n)
@@ -217,59 +224,43 @@ struct Decoder {
         if (!p && !err) err = OZL_ERR_ALLOCATION;
         return p;
     }
-
-    int flush()
-    {
-        if (!fills.empty()) {
-            const uint32_t B = (uint32_t)fills.size();
-            uint64_t mx = 0;
-            for (const FillItem& f : fills) mx = f.n > mx ? f.n : mx;
-            FillItem* dF = (FillItem*)alloc(B * sizeof(FillItem));
-            if (err) return err;
-            OZL_HIP_CHECK(hipMemcpyAsync(dF, host_keep(fills.data(), B * sizeof(FillItem)), B * sizeof(FillItem),
-                                         hipMemcpyHostToDevice, st));
-            for (uint32_t y0 = 0; y0 < B; y0 += 65535) {
-                const uint32_t cnt = B - y0 < 65535 ? B - y0 : 65535;
-                OZL_LAUNCH("k_fill_items", k_fill_items, dim3((unsigned)((mx + 16383) / 16384), cnt), dim3(kT), 0, st,
-                           dF + y0);
-            }
-            fills.clear();
-        }
-        if (!items.empty()) {
-            uint32_t B = (uint32_t)items.size();
-            // side streams may have been regenerated by the NCount batch
-            for (uint32_t k = 0; k < B; k++) {
-                (void)k;
-            }
-            // launched in stream order without waiting: later nodes that read
-            // these outputs are enqueued after them; the statuses are read
-            // once, after the whole frame (check_statuses)
-            ozl_ditem* dItems = (ozl_ditem*)alloc(B * sizeof(ozl_ditem));
-            int* dStatus = (int*)alloc(B * 4);
-            size_t wsz = ozl_entropy_items_workspace(B);
-            uint8_t* ws = alloc(wsz);
-            uint8_t* hItems = host_keep(items.data(), B * sizeof(ozl_ditem));
-            if (err) return err;
-            OZL_HIP_CHECK(hipMemcpyAsync(dItems, hItems, B * sizeof(ozl_ditem), hipMemcpyHostToDevice, st));
-            int rc = ozl_entropy_decode_items(dItems, B, dStatus, ws, wsz, st);
-            if (rc) return rc;
-            statuses.push_back({ dStatus, B });
-            items.clear();
-        }
-        // everything pending is now enqueued (conversions alias their
-        // producer's buffer, so they were pending too)
-        for (DStream& x : S)
-            if (x.pending) { x.pending = false; x.ready = true; }
-        return 0;
-    }
-
-    // host bytes that asynchronous copies read: kept until the frame is done
-    std::vector<std::vector<uint8_t>> keep;
     uint8_t* host_keep(const void* p, size_t n)
     {
         keep.emplace_back((const uint8_t*)p, (const uint8_t*)p + n);
         return keep.back().data();
     }
+    int flush();
+    // every launched item batch's status (one synchronisation)
+    int check_statuses()
+    {
+        std::vector<std::vector<int>> h(statuses.size());
+        for (size_t k = 0; k < statuses.size(); k++) {
+            h[k].resize(statuses[k].second);
+            OZL_HIP_CHECK(hipMemcpyAsync(h[k].data(), statuses[k].first, statuses[k].second * 4, hipMemcpyDeviceToHost, st));
+        }
+        OZL_HIP_CHECK(hipStreamSynchronize(st));
+        statuses.clear();
+        for (auto& v : h)
+            for (int x : v)
+                if (x) return x;
+        return 0;
+    }
+};
+
+struct Decoder {
+    ZL_DCtx_s* d;
+    DQueue* q;
+    hipStream_t st;
+    std::vector<DStream> S;
+    int err = 0;
+
+    uint8_t* alloc(size_t n)
+    {
+        uint8_t* p = d->alloc(n + 16);
+        if (!p && !err) err = OZL_ERR_ALLOCATION;
+        return p;
+    }
+    uint8_t* host_keep(const void* p, size_t n) { return q->host_keep(p, n); }
 
     // copies of `count` device ranges into dst at the given offsets, one launch
     int gather(const std::vector<const uint8_t*>& srcs, const std::vector<uint64_t>& sizes,
@@ -291,26 +282,12 @@ struct Decoder {
         return mx ? ozl_gather_sized((const uint8_t* const*)dm, dm + n, dm + 2 * n, (int)n, mx, dst, st) : 0;
     }
 
-    // every launched item batch's status (one synchronisation, at the end)
-    int check_statuses()
-    {
-        for (auto& b : statuses) {
-            std::vector<int> st2(b.second);
-            OZL_HIP_CHECK(hipMemcpyAsync(st2.data(), b.first, b.second * 4, hipMemcpyDeviceToHost, st));
-            OZL_HIP_CHECK(hipStreamSynchronize(st));
-            for (int v : st2)
-                if (v) return v;
-        }
-        statuses.clear();
-        return 0;
-    }
-
     // make sure the given streams hold their final content
     int need(const DNode& nd)
     {
         for (uint32_t k = 0; k < nd.nbOut; k++) {
             const DStream& s = S[nd.firstOut + k];
-            if (s.pending) return flush();
+            if (s.pending) return kBlocked;
             if (!s.ready) return OZL_ERR_CORRUPTION;
         }
         return 0;
@@ -368,7 +345,7 @@ struct Decoder {
             it.n = (uint32_t)n;
             it.huf = nd.trid == OZL_TR_HUFFMAN_V2;
             it.nbStates = nd.hdr[0];
-            items.push_back(it);
+            q->items.push_back(it);
             return 0;
         }
         case OZL_TR_CONV_NUM_TO_STRUCT_LE:
@@ -519,7 +496,7 @@ struct Decoder {
             in.width = 1;
             if (n) {
                 // batched with the other constant chunks (flush)
-                fills.push_back({ (uint64_t)(uintptr_t)in.dev, n, (uint64_t)(uintptr_t)v.dev });
+                q->fills.push_back({ (uint64_t)(uintptr_t)in.dev, n, (uint64_t)(uintptr_t)v.dev });
                 in.ready = false;
                 in.pending = true;
                 return 0;
@@ -559,6 +536,47 @@ struct Decoder {
     static constexpr int T_SERIAL_ = ZL_Type_serial, T_STRUCT_ = ZL_Type_struct, T_NUMERIC_ = ZL_Type_numeric;
 };
 
+int DQueue::flush()
+{
+    if (!fills.empty()) {
+        const uint32_t B = (uint32_t)fills.size();
+        uint64_t mx = 0;
+        for (const FillItem& f : fills) mx = f.n > mx ? f.n : mx;
+        FillItem* dF = (FillItem*)alloc(B * sizeof(FillItem));
+        if (err) return err;
+        OZL_HIP_CHECK(hipMemcpyAsync(dF, host_keep(fills.data(), B * sizeof(FillItem)), B * sizeof(FillItem),
+                                     hipMemcpyHostToDevice, st));
+        for (uint32_t y0 = 0; y0 < B; y0 += 65535) {
+            const uint32_t cnt = B - y0 < 65535 ? B - y0 : 65535;
+            OZL_LAUNCH("k_fill_items", k_fill_items, dim3((unsigned)((mx + 16383) / 16384), cnt), dim3(kT), 0, st,
+                       dF + y0);
+        }
+        fills.clear();
+    }
+    if (!items.empty()) {
+        // launched in stream order without waiting: later nodes that read
+        // these outputs are enqueued after them
+        const uint32_t B = (uint32_t)items.size();
+        ozl_ditem* dItems = (ozl_ditem*)alloc(B * sizeof(ozl_ditem));
+        int* dStatus = (int*)alloc(B * 4);
+        const size_t wsz = ozl_entropy_items_workspace(B);
+        uint8_t* ws = alloc(wsz);
+        uint8_t* hItems = host_keep(items.data(), B * sizeof(ozl_ditem));
+        if (err) return err;
+        OZL_HIP_CHECK(hipMemcpyAsync(dItems, hItems, B * sizeof(ozl_ditem), hipMemcpyHostToDevice, st));
+        int rc = ozl_entropy_decode_items(dItems, B, dStatus, ws, wsz, st);
+        if (rc) return rc;
+        statuses.push_back({ dStatus, B });
+        items.clear();
+    }
+    // everything pending is now enqueued (conversions alias their producer's
+    // buffer, so they were pending too)
+    for (Decoder* m : members)
+        for (DStream& x : m->S)
+            if (x.pending) { x.pending = false; x.ready = true; }
+    return 0;
+}
+
 struct ParsedFrame {
     int flags, type;
     uint64_t size;
@@ -569,9 +587,13 @@ struct ParsedFrame {
     uint32_t nbStreams;
 };
 
-int parse_frame(const uint8_t* f, size_t size, ParsedFrame& P)
+// Reads the headers of the frame f[0, size); only f[0, avail) is read (a
+// frame in device memory has its first avail bytes copied to the host). The
+// trailer (checksums, end byte) is checked by the caller.
+int parse_frame(const uint8_t* f, size_t size, size_t avail, ParsedFrame& P)
 {
-    HostReader r = { f, 0, size, false };
+    if (avail > size) avail = size;
+    HostReader r = { f, 0, avail, false };
     uint32_t magic = 0;
     for (int k = 0; k < 4; k++) magic |= r.u8() << (8 * k);
     if (r.bad || magic < OZL_MAGIC_BASE || magic > OZL_MAGIC_BASE + 64) return OZL_ERR_HEADER_UNKNOWN;
@@ -620,7 +642,7 @@ int parse_frame(const uint8_t* f, size_t size, ParsedFrame& P)
     size_t hp = r.pos;
     for (size_t i = 0; i < nn; i++) {
         DNode& nd = P.nodes[nn - 1 - i];
-        if (hp + hlen[nn - 1 - i] > size) return OZL_ERR_CORRUPTION;
+        if (hp + hlen[nn - 1 - i] > avail) return OZL_ERR_CORRUPTION;
         nd.hdr.assign(f + hp, f + hp + hlen[nn - 1 - i]);
         hp += hlen[nn - 1 - i];
     }
@@ -651,93 +673,215 @@ int parse_frame(const uint8_t* f, size_t size, ParsedFrame& P)
     for (size_t k = 0; k < nbStored; k++) pos += P.storedSize[k];
     P.csumPos = pos;
     P.end = pos + ((P.flags & 1) ? 4 : 0) + ((P.flags & 2) ? 4 : 0);
-    if (P.end + 1 > size || f[P.end] != 0) return OZL_ERR_CORRUPTION;
+    if (P.end + 1 > size) return OZL_ERR_CORRUPTION;
+    return 0;
+}
+
+// One frame of a decompression call.
+struct FrameJob {
+    const uint8_t* src;    // the frame (host or device memory)
+    size_t size;
+    bool dev;              // src is device memory
+    std::unique_ptr<uint8_t[]> hbuf;   // host copy of a device frame's headers
+    size_t avail = 0;
+    const uint8_t* host = nullptr;     // host view of the headers
+    ParsedFrame P;
+    uint8_t* dFrame = nullptr;
+    std::unique_ptr<Decoder> D;
+    int64_t next = 0;      // next node to decode (frame order: last node first)
+    uint8_t tail[9];       // checksums and end byte
+    uint64_t tailOff = 0;
+    int wantType = 0;
+};
+
+// Decompresses n frames (host or device memory) into dst (host or device
+// memory) back to back: the host reads every frame's headers; the decoders
+// of all frames run in lockstep, each until it needs a stream a queued
+// launch still has to produce, then the queue goes out as shared launches;
+// checksums are verified in batches; one synchronisation at the end.
+int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSizes, size_t n, void* dst, size_t cap,
+                      size_t* outSizes, ZL_OutputInfo* infos)
+{
+    if (!dc->st && hipStreamCreateWithFlags(&dc->st, hipStreamNonBlocking) != hipSuccess) return OZL_ERR_ALLOCATION;
+    hipStream_t st = dc->st;
+    dc->reset();
+    std::vector<FrameJob> J(n);
+    const bool devIn = n && ozl_is_device_ptr(src);
+    size_t off = 0;
+    for (size_t i = 0; i < n; i++) {
+        J[i].src = src + off;
+        J[i].size = frameSizes[i];
+        J[i].dev = devIn;
+        off += frameSizes[i];
+    }
+    // headers of device frames: the first 256 KiB of each come to the host,
+    // the whole frame when its headers are longer
+    if (devIn) {
+        for (FrameJob& j : J) {
+            j.hbuf.reset(new uint8_t[j.size ? j.size : 1]);
+            j.avail = j.size < ((size_t)256 << 10) ? j.size : ((size_t)256 << 10);
+            if (j.avail) OZL_HIP_CHECK(hipMemcpyAsync(j.hbuf.get(), j.src, j.avail, hipMemcpyDeviceToHost, st));
+            j.host = j.hbuf.get();
+        }
+        OZL_HIP_CHECK(hipStreamSynchronize(st));
+    } else {
+        for (FrameJob& j : J) {
+            j.host = j.src;
+            j.avail = j.size;
+        }
+    }
+    for (FrameJob& j : J) {
+        int rc = parse_frame(j.host, j.size, j.avail, j.P);
+        if (rc && j.avail < j.size) {
+            OZL_HIP_CHECK(hipMemcpy(j.hbuf.get(), j.src, j.size, hipMemcpyDeviceToHost));
+            j.avail = j.size;
+            rc = parse_frame(j.host, j.size, j.avail, j.P);
+        }
+        if (rc) return rc;
+    }
+    DQueue Q;
+    Q.d = dc;
+    Q.st = st;
+    for (FrameJob& j : J) {
+        const ParsedFrame& P = j.P;
+        j.D.reset(new Decoder());
+        Decoder& D = *j.D;
+        D.d = dc;
+        D.q = &Q;
+        D.st = st;
+        D.S.assign(P.nbStreams, DStream());
+        Q.members.push_back(&D);
+        if (j.dev) {
+            j.dFrame = const_cast<uint8_t*>(j.src);   // read in place
+        } else {
+            j.dFrame = D.alloc(j.size);
+            if (D.err) return D.err;
+            OZL_HIP_CHECK(hipMemcpyAsync(j.dFrame, j.src, j.size, hipMemcpyHostToDevice, st));
+        }
+        size_t pos = P.dataStart;
+        for (size_t k = 0; k < P.storedId.size(); k++) {
+            DStream& x = D.S[P.storedId[k]];
+            x.dev = j.dFrame + pos;
+            x.bytes = P.storedSize[k];
+            x.type = ZL_Type_serial;
+            x.width = 1;
+            x.ready = true;
+            pos += P.storedSize[k];
+        }
+        // stored numeric streams keep their type: side inputs of fse_v2
+        // (int16) and huffman_v2 (u8) when stored raw
+        for (auto& nd : P.nodes) {
+            if (nd.trid == OZL_TR_FSE_V2 && D.S[nd.firstOut].ready) { D.S[nd.firstOut].type = ZL_Type_numeric; D.S[nd.firstOut].width = 2; }
+            if (nd.trid == OZL_TR_HUFFMAN_V2 && D.S[nd.firstOut].ready) { D.S[nd.firstOut].type = ZL_Type_numeric; }
+        }
+        j.next = (int64_t)P.nodes.size() - 1;
+        // checksums and end byte
+        j.tailOff = P.csumPos;
+        const size_t tl = P.end + 1 - P.csumPos;
+        if (j.dev) OZL_HIP_CHECK(hipMemcpyAsync(j.tail, j.src + P.csumPos, tl, hipMemcpyDeviceToHost, st));
+        else memcpy(j.tail, j.src + P.csumPos, tl);
+    }
+    // the decoders in lockstep
+    for (;;) {
+        bool left = false;
+        for (FrameJob& j : J) {
+            while (j.next >= 0) {
+                int rc = j.D->decode(j.P.nodes[(size_t)j.next]);
+                if (rc == kBlocked) break;
+                if (rc) return rc;
+                if (j.D->err) return j.D->err;
+                j.next--;
+            }
+            if (j.next >= 0) left = true;
+        }
+        int rc = Q.flush();
+        if (!rc) rc = Q.err;
+        if (rc) return rc;
+        if (!left) break;
+    }
+    // outputs, checksum jobs
+    size_t total = 0;
+    std::vector<uint64_t> outOff(n);
+    std::vector<uint64_t> jobs;   // ozl_xjob triples: content hashes, then compressed hashes
+    std::vector<int> jobKind;     // per job: frame index * 2 + (0 content, 1 compressed)
+    size_t sumWords = 0, maxLen = 0;
+    for (size_t i = 0; i < n; i++) {
+        FrameJob& j = J[i];
+        const ParsedFrame& P = j.P;
+        DStream& out = j.D->S[0];
+        if (P.nodes.empty())  // stored input
+            out.type = P.type == OZL_T_SERIAL ? ZL_Type_serial : P.type == OZL_T_STRUCT ? ZL_Type_struct : ZL_Type_numeric;
+        if (out.bytes != P.size) return OZL_ERR_CORRUPTION;
+        j.wantType = P.type == OZL_T_SERIAL ? ZL_Type_serial : P.type == OZL_T_STRUCT ? ZL_Type_struct : ZL_Type_numeric;
+        // a stored input keeps the frame's type; a regenerated one must match
+        if (!P.nodes.empty() && out.type != j.wantType && j.wantType != ZL_Type_serial) return OZL_ERR_CORRUPTION;
+        outOff[i] = total;
+        total += out.bytes;
+        if (P.flags & 1) {
+            const uint8_t* p = out.dev ? out.dev : j.dFrame;
+            jobs.insert(jobs.end(), { (uint64_t)(uintptr_t)p, out.bytes, sumWords });
+            jobKind.push_back((int)(2 * i));
+            sumWords += ozl_xxh3_batch_sums(out.bytes);
+            maxLen = out.bytes > maxLen ? out.bytes : maxLen;
+        }
+        if (P.flags & 2) {
+            const size_t clen = P.csumPos + ((P.flags & 1) ? 4 : 0) - P.chunkStart;
+            jobs.insert(jobs.end(), { (uint64_t)(uintptr_t)(j.dFrame + P.chunkStart), clen, sumWords });
+            jobKind.push_back((int)(2 * i + 1));
+            sumWords += ozl_xxh3_batch_sums(clen);
+            maxLen = clen > maxLen ? clen : maxLen;
+        }
+    }
+    if (total > cap) return OZL_ERR_DSTCAPACITY_TOOSMALL;
+    const uint32_t nj = (uint32_t)jobKind.size();
+    std::vector<uint64_t> hashes(nj);
+    if (nj) {
+        uint64_t* dJobs = (uint64_t*)dc->alloc(jobs.size() * 8 + 16);
+        uint64_t* dSums = (uint64_t*)dc->alloc(sumWords * 8 + 16);
+        uint64_t* dH = (uint64_t*)dc->alloc((size_t)nj * 8 + 16);
+        if (!dJobs || !dSums || !dH) return OZL_ERR_ALLOCATION;
+        OZL_HIP_CHECK(hipMemcpyAsync(dJobs, jobs.data(), jobs.size() * 8, hipMemcpyHostToDevice, st));
+        int rc = ozl_xxh3_device_batch((const ozl_xjob*)dJobs, nj, maxLen, dSums, dH, st);
+        if (rc) return rc;
+        OZL_HIP_CHECK(hipMemcpyAsync(hashes.data(), dH, (size_t)nj * 8, hipMemcpyDeviceToHost, st));
+    }
+    const bool devOut = total && ozl_is_device_ptr(dst);
+    for (size_t i = 0; i < n; i++) {
+        const DStream& out = J[i].D->S[0];
+        if (!out.bytes) continue;
+        OZL_HIP_CHECK(hipMemcpyAsync((uint8_t*)dst + outOff[i], out.dev, out.bytes,
+                                     devOut ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+    }
+    int rc = Q.check_statuses();   // synchronises the stream
+    if (rc) return rc;
+    for (size_t i = 0; i < n; i++)
+        if (J[i].tail[J[i].P.end - J[i].P.csumPos] != 0) return OZL_ERR_CORRUPTION;
+    for (uint32_t k = 0; k < nj; k++) {
+        const FrameJob& j = J[(size_t)jobKind[k] / 2];
+        if (jobKind[k] & 1) {
+            const size_t at = (j.P.flags & 1) ? 4 : 0;
+            if ((uint32_t)hashes[k] != ozl_le32(j.tail + at)) return OZL_ERR_COMPRESSED_CHECKSUM;
+        } else if ((uint32_t)hashes[k] != ozl_le32(j.tail)) {
+            return OZL_ERR_CONTENT_CHECKSUM;
+        }
+    }
+    for (size_t i = 0; i < n; i++) {
+        const DStream& out = J[i].D->S[0];
+        outSizes[i] = out.bytes;
+        if (infos) {
+            infos[i].type = (ZL_Type)J[i].wantType;
+            infos[i].fixedWidth = (uint32_t)(J[i].wantType == ZL_Type_serial ? 1 : out.width);
+            infos[i].decompressedByteSize = out.bytes;
+            infos[i].numElts = out.bytes / infos[i].fixedWidth;
+        }
+    }
     return 0;
 }
 
 int decompress_impl(ZL_DCtx_s* dc, const uint8_t* src, size_t srcSize, void* dst, size_t cap, ZL_OutputInfo* info)
 {
-    ParsedFrame P;
-    int rc = parse_frame(src, srcSize, P);
-    if (rc) return rc;
-    if (!dc->st && hipStreamCreateWithFlags(&dc->st, hipStreamNonBlocking) != hipSuccess) return OZL_ERR_ALLOCATION;
-    dc->reset();
-    Decoder D;
-    D.d = dc;
-    D.st = dc->st;
-    D.S.assign(P.nbStreams, DStream());
-    uint8_t* dFrame = D.alloc(srcSize);
-    if (D.err) return D.err;
-    OZL_HIP_CHECK(hipMemcpyAsync(dFrame, src, srcSize, hipMemcpyHostToDevice, dc->st));
-    size_t pos = P.dataStart;
-    for (size_t k = 0; k < P.storedId.size(); k++) {
-        DStream& s = D.S[P.storedId[k]];
-        s.dev = dFrame + pos;
-        s.bytes = P.storedSize[k];
-        s.type = ZL_Type_serial;
-        s.width = 1;
-        s.ready = true;
-        pos += P.storedSize[k];
-    }
-    // stored numeric streams keep their type: side inputs of fse_v2 (int16)
-    // and huffman_v2 (u8) when stored raw
-    for (auto& nd : P.nodes) {
-        if (nd.trid == OZL_TR_FSE_V2 && D.S[nd.firstOut].ready) { D.S[nd.firstOut].type = ZL_Type_numeric; D.S[nd.firstOut].width = 2; }
-        if (nd.trid == OZL_TR_HUFFMAN_V2 && D.S[nd.firstOut].ready) { D.S[nd.firstOut].type = ZL_Type_numeric; }
-        if (nd.trid == OZL_TR_FSE_NCOUNT) { /* input regenerated as numeric */ }
-    }
-    for (int64_t k = (int64_t)P.nodes.size() - 1; k >= 0; k--) {
-        rc = D.decode(P.nodes[k]);
-        if (rc) return rc;
-    }
-    rc = D.flush();
-    if (rc) return rc;
-    DStream& out = D.S[0];
-    if (P.nodes.empty()) {  // stored input
-        out.type = P.type == OZL_T_SERIAL ? ZL_Type_serial : P.type == OZL_T_STRUCT ? ZL_Type_struct : ZL_Type_numeric;
-    }
-    if (out.bytes != P.size) return OZL_ERR_CORRUPTION;
-    int wantType = P.type == OZL_T_SERIAL ? ZL_Type_serial : P.type == OZL_T_STRUCT ? ZL_Type_struct : ZL_Type_numeric;
-    if (!P.nodes.empty() && out.type != wantType) {
-        // a stored input keeps the frame's type; a regenerated one must match
-        if (!(wantType == ZL_Type_serial)) return OZL_ERR_CORRUPTION;
-    }
-    if (out.bytes > cap) return OZL_ERR_DSTCAPACITY_TOOSMALL;
-    // checksums
-    uint64_t hashes[2] = { 0, 0 };
-    uint64_t* dH = (uint64_t*)D.alloc(16);
-    size_t hs = ozl_xxh3_scratch_size(out.bytes > srcSize ? out.bytes : srcSize);
-    uint8_t* scratch = D.alloc(hs);
-    if (D.err) return D.err;
-    if (P.flags & 1) {
-        rc = ozl_xxh3_device(out.dev ? out.dev : dFrame, out.bytes, dH, scratch, hs, dc->st);
-        if (rc) return rc;
-    }
-    if (P.flags & 2) {
-        size_t clen = P.csumPos + ((P.flags & 1) ? 4 : 0) - P.chunkStart;
-        rc = ozl_xxh3_device(dFrame + P.chunkStart, clen, dH + 1, scratch, hs, dc->st);
-        if (rc) return rc;
-    }
-    OZL_HIP_CHECK(hipMemcpyAsync(hashes, dH, 16, hipMemcpyDeviceToHost, dc->st));
-    if (out.bytes) OZL_HIP_CHECK(hipMemcpyAsync(dst, out.dev, out.bytes, hipMemcpyDeviceToHost, dc->st));
-    OZL_HIP_CHECK(hipStreamSynchronize(dc->st));
-    rc = D.check_statuses();
-    if (rc) return rc;
-    size_t cp = P.csumPos;
-    if (P.flags & 2) {
-        size_t at = cp + ((P.flags & 1) ? 4 : 0);
-        if ((uint32_t)hashes[1] != ozl_le32(src + at)) return OZL_ERR_COMPRESSED_CHECKSUM;
-    }
-    if (P.flags & 1) {
-        if ((uint32_t)hashes[0] != ozl_le32(src + cp)) return OZL_ERR_CONTENT_CHECKSUM;
-    }
-    if (info) {
-        info->type = (ZL_Type)wantType;
-        info->fixedWidth = (uint32_t)(wantType == ZL_Type_serial ? 1 : out.width);
-        info->decompressedByteSize = out.bytes;
-        info->numElts = out.bytes / info->fixedWidth;
-    }
-    return 0;
+    size_t outSize = 0;
+    return decompress_frames(dc, src, &srcSize, 1, dst, cap, &outSize, info);
 }
 
 }  // namespace
@@ -758,6 +902,18 @@ ZL_Report ZL_DCtx_decompressTyped(ZL_DCtx* dctx, ZL_OutputInfo* info, void* dst,
     return rep_ok(tmp.decompressedByteSize);
 }
 
+ZL_Report ozl_DCtx_decompressBatch(ZL_DCtx* dctx, void* dst, size_t dstCapacity, size_t* outSizes, const void* src,
+                                   const size_t* frameSizes, size_t nbFrames)
+{
+    if (!dctx || !outSizes || (nbFrames && (!src || !frameSizes))) return rep_err(ZL_ErrorCode_parameter_invalid);
+    if (nbFrames == 0) return rep_ok(0);
+    int rc = decompress_frames(dctx, (const uint8_t*)src, frameSizes, nbFrames, dst, dstCapacity, outSizes, nullptr);
+    if (rc) return rep_err(rc);
+    size_t total = 0;
+    for (size_t i = 0; i < nbFrames; i++) total += outSizes[i];
+    return rep_ok(total);
+}
+
 ZL_Report ZL_decompress(void* dst, size_t cap, const void* src, size_t srcSize)
 {
     ZL_DCtx_s d;
@@ -770,6 +926,12 @@ ZL_Report ZL_decompress(void* dst, size_t cap, const void* src, size_t srcSize)
 ZL_Report ZL_getDecompressedSize(const void* src, size_t srcSize)
 {
     const uint8_t* f = (const uint8_t*)src;
+    uint8_t head[16];
+    if (src && srcSize && ozl_is_device_ptr(src)) {   // a frame in device memory: its first bytes
+        srcSize = srcSize < sizeof(head) ? srcSize : sizeof(head);
+        if (hipMemcpy(head, src, srcSize, hipMemcpyDeviceToHost) != hipSuccess) return rep_err(ZL_ErrorCode_GENERIC);
+        f = head;
+    }
     HostReader r = { f, 0, srcSize, false };
     uint32_t magic = 0;
     for (int k = 0; k < 4; k++) magic |= r.u8() << (8 * k);

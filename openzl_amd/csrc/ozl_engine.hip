@@ -16,7 +16,9 @@
 // keeps only ids, sizes, node headers and the per-chunk entropy plans.
 #include <string.h>
 
+#include <algorithm>
 #include <memory>
+#include <numeric>
 #include <vector>
 
 #include "openzl_amd.h"
@@ -26,6 +28,7 @@
 #include "ozl_frame.h"
 #include "ozl_entropy_api.h"
 #include "ozl_profile.h"
+#include "ozl_engine_util.h"
 
 // numeric / serial device entry points (ozl_numeric.hip)
 extern "C" {
@@ -35,10 +38,6 @@ int ozl_transpose_split(const void* src, size_t n, int width, void* planes, void
 int ozl_max_async(const void* src, size_t n, int width, unsigned long long* dmax, void* stream);
 int ozl_bitpack_encode(const void* src, size_t n, int width, int nbBits, void* dst, void* stream);
 }
-int ozl_xxh3_device(const uint8_t* data, size_t len, uint64_t* dOut, void* scratch, size_t scratchSize, hipStream_t st);
-size_t ozl_xxh3_scratch_size(size_t len);
-int ozl_gather_sized(const uint8_t* const* srcs, const uint64_t* sizes, const uint64_t* dstOffs, int count,
-                     size_t maxSize, uint8_t* dst, hipStream_t st);
 
 namespace {
 
@@ -109,12 +108,16 @@ struct DevArena {
     }
 };
 
+// Streams and nodes carry a creation key: the frame lists them in the order
+// the reference engine creates them, and a deferred entropy call creates its
+// nodes and streams later, under keys just above the one it reserved.
 struct EStream {
     int type;
     int width;
     size_t n;
     uint8_t* dev;
     bool store;
+    uint64_t key;
     size_t bytes() const { return n * (size_t)width; }
 };
 
@@ -125,6 +128,7 @@ struct ENode {
     uint32_t in;
     uint32_t firstOut;
     uint8_t hdr[12];
+    uint64_t key;
 };
 
 }  // namespace
@@ -145,6 +149,30 @@ struct ZL_TypedRef_s {
 };
 
 struct Session;
+
+// A ZL_GRAPH_FSE / ZL_GRAPH_HUFFMAN call whose chunks are planned and encoded
+// with those of other calls (of the same frame or of other frames of a batch).
+struct EntReq {
+    Session* s;
+    int huf;
+    uint64_t mark;                 // key reserved for the nodes and streams it creates
+    std::vector<uint32_t> chunks;  // chunk streams, in order
+    uint32_t first;                // index of its first chunk in its coder's launch
+};
+
+// Every deferred call's chunks, one launch per coder (ozl_entropy_encode_chunks
+// over begin / end addresses), one read-back of the plans.
+struct EntBatch {
+    std::vector<EntReq> reqs;
+    std::vector<uint64_t> beg[2], end[2];
+    size_t maxBlock[2] = { 1, 1 };
+    ozl_ews ws[2];
+    std::vector<ozl_eplan> plans[2];
+    void add(EntReq&& r);
+    int run(DevArena* arena, hipStream_t st);
+    int expand();
+};
+
 struct ZL_Input_s {
     Session* s;
     uint32_t sid;
@@ -171,6 +199,11 @@ struct Session {
     std::vector<EStream> streams;
     std::vector<ENode> nodes;
     int err = 0;
+    EntBatch* batch = nullptr;   // entropy calls deferred to this batch (none: run at once)
+    uint64_t clock = 0, expBase = 0;
+    uint32_t expMinor = 0;
+
+    uint64_t nextKey() { return expBase ? expBase + (++expMinor) : (++clock) << 20; }
 
     uint8_t* alloc(size_t n)
     {
@@ -180,7 +213,7 @@ struct Session {
     }
     uint32_t newStream(int type, int width, size_t n, uint8_t* dev)
     {
-        streams.push_back({ type, width, n, dev, false });
+        streams.push_back({ type, width, n, dev, false, nextKey() });
         return (uint32_t)streams.size() - 1;
     }
     ENode& newNode(uint8_t trid, uint32_t in)
@@ -190,6 +223,7 @@ struct Session {
         nd.trid = trid;
         nd.in = in;
         nd.firstOut = (uint32_t)streams.size();
+        nd.key = nextKey();
         nodes.push_back(nd);
         return nodes.back();
     }
@@ -367,11 +401,8 @@ struct Session {
         }
         const size_t kChunk = 1 << 15;
         std::vector<uint32_t> chunks;
-        std::vector<uint64_t> offs;
         if (s.n < 100000) {
             chunks.push_back(sid);
-            offs.push_back(0);
-            offs.push_back(s.n);
         } else {
             size_t nb = (s.n + kChunk - 1) / kChunk;
             ENode& nd = newNode(OZL_TR_SPLITN, sid);
@@ -379,26 +410,42 @@ struct Session {
             for (size_t c = 0; c < nb; c++) {
                 size_t a = c * kChunk, b = a + kChunk < s.n ? a + kChunk : s.n;
                 chunks.push_back(newStream(T_SERIAL, 1, b - a, s.dev + a));
-                offs.push_back(a);
             }
-            offs.push_back(s.n);
         }
-        uint32_t B = (uint32_t)chunks.size();
-        size_t maxBlock = s.n < kChunk ? s.n : (s.n < 100000 ? s.n : kChunk);
-        if (maxBlock == 0) maxBlock = 1;
-        size_t wsBytes = (size_t)B * (sizeof(ozl_eplan) + OZL_TAB_BYTES + OZL_AUX_BYTES + ozl_payload_cap(maxBlock)) + 256;
-        uint8_t* wsMem = alloc(wsBytes);
-        uint64_t* dOff = (uint64_t*)alloc(offs.size() * 8);
-        if (err) return err;
-        OZL_HIP_CHECK(hipMemcpyAsync(dOff, offs.data(), offs.size() * 8, hipMemcpyHostToDevice, st));
-        ozl_ews ws = ozl_ews_carve(wsMem, B, maxBlock);
-        int rc = ozl_entropy_encode_chunks(huf, s.dev, dOff, B, maxBlock, ws, st);
-        if (rc) return rc;
-        std::vector<ozl_eplan> plans(B);
-        OZL_HIP_CHECK(hipMemcpyAsync(plans.data(), ws.plans, (size_t)B * sizeof(ozl_eplan), hipMemcpyDeviceToHost, st));
-        OZL_HIP_CHECK(hipStreamSynchronize(st));
-        for (uint32_t c = 0; c < B; c++)
-            if (plans[c].status) return (int)plans[c].status;
+        EntReq r;
+        r.s = this;
+        r.huf = huf;
+        r.chunks = std::move(chunks);
+        r.mark = nextKey();
+        r.first = 0;
+        if (batch) {
+            batch->add(std::move(r));
+            return 0;
+        }
+        EntBatch local;
+        local.add(std::move(r));
+        int rc = local.run(arena, st);
+        return rc ? rc : local.expand();
+    }
+
+    // The nodes and streams of a planned entropy call (plans and workspace
+    // areas of its chunks start at r.first), created under its reserved key.
+    int expandEntropy(const EntReq& r, const ozl_eplan* plans, const ozl_ews& ws)
+    {
+        expBase = r.mark;
+        expMinor = 0;
+        int rc = expandEntropyAt(r, plans, ws);
+        expBase = 0;
+        return rc;
+    }
+
+    int expandEntropyAt(const EntReq& r, const ozl_eplan* plans, const ozl_ews& ws0)
+    {
+        const std::vector<uint32_t>& chunks = r.chunks;
+        const uint32_t B = (uint32_t)chunks.size();
+        ozl_ews ws = ws0;
+        ws.aux += (size_t)r.first * OZL_AUX_BYTES;
+        ws.payload += (size_t)r.first * ws.payloadCap;
         // graph function: the FSE / Huffman nodes, chunk order
         std::vector<uint32_t> side(B), bits(B);
         for (uint32_t c = 0; c < B; c++) {
@@ -594,11 +641,91 @@ struct Session {
         return 0;
     }
 
+    // Puts streams and nodes in creation-key order (the order of an engine
+    // without deferred calls) and renumbers the nodes' stream references.
+    void finalize()
+    {
+        const size_t ns = streams.size();
+        std::vector<uint32_t> ord(ns);
+        std::iota(ord.begin(), ord.end(), 0u);
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](uint32_t x, uint32_t y) { return streams[x].key < streams[y].key; });
+        std::vector<uint32_t> remap(ns + 1);
+        std::vector<EStream> ss(ns);
+        for (size_t i = 0; i < ns; i++) {
+            remap[ord[i]] = (uint32_t)i;
+            ss[i] = streams[ord[i]];
+        }
+        remap[ns] = (uint32_t)ns;
+        for (ENode& nd : nodes) {
+            nd.in = remap[nd.in];
+            nd.firstOut = remap[nd.firstOut];
+        }
+        std::stable_sort(nodes.begin(), nodes.end(), [](const ENode& x, const ENode& y) { return x.key < y.key; });
+        streams.swap(ss);
+    }
+
     // Exhaustive trial (selector_brute_force.c:9-50): every candidate compresses
     // the input in a derived context with checksums off (cctx.c:1799-1865); the
     // smallest frame wins, STORE when none beats the raw size.
     int runBrute(const GDef* g, uint32_t sid);
 };
+
+void EntBatch::add(EntReq&& r)
+{
+    const int m = r.huf ? 1 : 0;
+    r.first = (uint32_t)beg[m].size();
+    for (uint32_t c : r.chunks) {
+        const EStream& x = r.s->streams[c];
+        beg[m].push_back((uint64_t)(uintptr_t)x.dev);
+        end[m].push_back((uint64_t)(uintptr_t)x.dev + x.n);
+        maxBlock[m] = x.n > maxBlock[m] ? x.n : maxBlock[m];
+    }
+    reqs.push_back(std::move(r));
+}
+
+int EntBatch::run(DevArena* arena, hipStream_t st)
+{
+    std::vector<uint64_t> offs[2];   // read by the launches: kept until the wait below
+    for (int m = 0; m < 2; m++) {
+        const uint32_t B = (uint32_t)beg[m].size();
+        if (!B) continue;
+        // chunk addresses relative to the lowest one
+        const uint64_t lo = *std::min_element(beg[m].begin(), beg[m].end());
+        std::vector<uint64_t>& o = offs[m];
+        o.resize(2 * (size_t)B);
+        for (uint32_t c = 0; c < B; c++) {
+            o[c] = beg[m][c] - lo;
+            o[B + c] = end[m][c] - lo;
+        }
+        const size_t wsBytes =
+            (size_t)B * (sizeof(ozl_eplan) + OZL_TAB_BYTES + OZL_AUX_BYTES + ozl_payload_cap(maxBlock[m])) + 256;
+        uint8_t* wsMem = arena->alloc(wsBytes);
+        uint64_t* dO = (uint64_t*)arena->alloc(o.size() * 8);
+        if (!wsMem || !dO) return OZL_ERR_ALLOCATION;
+        OZL_HIP_CHECK(hipMemcpyAsync(dO, o.data(), o.size() * 8, hipMemcpyHostToDevice, st));
+        ws[m] = ozl_ews_carve(wsMem, B, maxBlock[m]);
+        int rc = ozl_entropy_encode_chunks(m, (const uint8_t*)(uintptr_t)lo, dO, dO + B, B, maxBlock[m], ws[m], st);
+        if (rc) return rc;
+        plans[m].resize(B);
+        OZL_HIP_CHECK(hipMemcpyAsync(plans[m].data(), ws[m].plans, (size_t)B * sizeof(ozl_eplan), hipMemcpyDeviceToHost, st));
+    }
+    OZL_HIP_CHECK(hipStreamSynchronize(st));
+    for (int m = 0; m < 2; m++)
+        for (const ozl_eplan& p : plans[m])
+            if (p.status) return (int)p.status;
+    return 0;
+}
+
+int EntBatch::expand()
+{
+    for (const EntReq& r : reqs) {
+        const int m = r.huf ? 1 : 0;
+        int rc = r.s->expandEntropy(r, plans[m].data() + r.first, ws[m]);
+        if (rc) return rc;
+    }
+    return 0;
+}
 
 // ---------------------------------------------------------------- edge API for function graphs
 extern "C" {
@@ -737,6 +864,7 @@ int Session::runBrute(const GDef* g, uint32_t sid)
         t.streams.push_back(in);
         int rc = t.runSuccessor(g->succ[i], 0);
         if (rc) continue;
+        t.finalize();
         FrameParams fp = { 0, 0, OZL_FORMAT_VERSION, minStreamSize };
         std::vector<uint8_t> head;
         std::vector<const uint8_t*> srcs;
@@ -750,6 +878,165 @@ int Session::runBrute(const GDef* g, uint32_t sid)
     }
     return runSuccessor(best < 0 ? (unsigned)GID_STORE : g->succ[best], sid);
 }
+
+namespace {
+
+// Compresses n inputs (host or device memory) into n frames written back to
+// back at dst (host or device memory): each input's graph walk, with its
+// entropy calls deferred; one plan / encode launch per coder for all frames;
+// the frames' headers on the host; one gather of headers and stored streams;
+// batched checksums; one copy out. sizes[i] = size of frame i.
+int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void* dst, size_t cap, size_t* sizes)
+{
+    if (!c || !c->comp) return ZL_ErrorCode_parameter_invalid;
+    FrameParams fp = resolve(c);
+    if (fp.formatVersion == 0) return ZL_ErrorCode_formatVersion_notSet;
+    if (fp.formatVersion != (int)OZL_FORMAT_VERSION) return ZL_ErrorCode_formatVersion_unsupported;
+    for (size_t i = 0; i < n; i++) {
+        const ZL_TypedRef_s* in = ins[i];
+        if (!in) return ZL_ErrorCode_parameter_invalid;
+        if (in->type == T_NUMERIC && in->width != 1 && in->width != 2 && in->width != 4 && in->width != 8)
+            return ZL_ErrorCode_parameter_invalid;
+    }
+    if (!c->st && hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return ZL_ErrorCode_allocation;
+    hipStream_t st = c->st;
+    c->arena.reset();
+    EntBatch eb;
+    std::vector<std::unique_ptr<Session>> S(n);
+    std::vector<const uint8_t*> dIn(n);
+    std::vector<size_t> inBytes(n);
+    for (size_t i = 0; i < n; i++) {
+        const ZL_TypedRef_s* in = ins[i];
+        inBytes[i] = in->n * in->width;
+        S[i].reset(new Session());
+        Session& T = *S[i];
+        T.comp = c->comp;
+        T.arena = &c->arena;
+        T.st = st;
+        T.minStreamSize = fp.minStreamSize;
+        T.batch = &eb;
+        // device-resident inputs are read in place
+        if (inBytes[i] && ozl_is_device_ptr(in->ptr)) {
+            dIn[i] = (const uint8_t*)in->ptr;
+        } else {
+            uint8_t* d = T.alloc(inBytes[i] + 16);
+            if (T.err) return T.err;
+            if (inBytes[i]) OZL_HIP_CHECK(hipMemcpyAsync(d, in->ptr, inBytes[i], hipMemcpyHostToDevice, st));
+            dIn[i] = d;
+        }
+        T.newStream(in->type, (int)in->width, in->n, (uint8_t*)dIn[i]);
+        int rc = T.runSuccessor(c->comp->start ? c->comp->start : GID_STORE, 0);
+        if (!rc) rc = T.err;
+        if (rc) return rc;
+    }
+    int rc = eb.run(&c->arena, st);
+    if (!rc) rc = eb.expand();
+    if (rc) return rc;
+    // headers and stored streams of every frame
+    const int cc = fp.contentChecksum, kc = fp.compressedChecksum;
+    std::vector<uint8_t> heads;
+    std::vector<uint64_t> gSrc, gSize, gOff;   // gather items: headers, then stored streams
+    std::vector<uint64_t> fOff(n + 1), fh(n), body(n), headItem(n);
+    size_t total = 0;
+    for (size_t i = 0; i < n; i++) {
+        Session& T = *S[i];
+        T.finalize();
+        const int ft = frame_type(ins[i]->type);
+        std::vector<uint8_t> head;
+        std::vector<const uint8_t*> srcs;
+        std::vector<uint64_t> ssz;
+        rc = build_headers(T, ft, inBytes[i], fp, head, srcs, ssz);
+        if (rc) return rc;
+        fh[i] = ozl_write_frame_header(head.data(), ft, inBytes[i], cc, kc);
+        fOff[i] = total;
+        headItem[i] = gSrc.size();
+        gSrc.push_back(heads.size());   // rebased once the headers are on the device
+        gSize.push_back(head.size());
+        gOff.push_back(total);
+        size_t pos = total + head.size();
+        heads.insert(heads.end(), head.begin(), head.end());
+        for (size_t k = 0; k < srcs.size(); k++) {
+            if (ssz[k]) {
+                gSrc.push_back((uint64_t)(uintptr_t)srcs[k]);
+                gSize.push_back(ssz[k]);
+                gOff.push_back(pos);
+            }
+            pos += ssz[k];
+        }
+        body[i] = pos;   // checksums from here
+        sizes[i] = pos - total + (cc ? 4 : 0) + (kc ? 4 : 0) + 1;
+        total += sizes[i];
+    }
+    fOff[n] = total;
+    if (total > cap) return ZL_ErrorCode_dstCapacity_tooSmall;
+    const bool devOut = ozl_is_device_ptr(dst);
+    uint8_t* dF = devOut ? (uint8_t*)dst : c->arena.alloc(total + 16);
+    uint8_t* dHead = c->arena.alloc(heads.size() + 16);
+    // device metadata: gather arrays, checksum jobs and tail positions
+    const size_t G = gSrc.size();
+    std::vector<uint64_t> meta(3 * G + 6 * n + 2 * n);
+    size_t sumWords = 0, maxLen = 0;
+    uint64_t* mJobC = meta.data() + 3 * G;
+    uint64_t* mJobK = mJobC + 3 * n;
+    uint64_t* mAtC = mJobK + 3 * n;
+    uint64_t* mAtK = mAtC + n;
+    for (size_t i = 0; i < n; i++) {
+        mJobC[3 * i] = (uint64_t)(uintptr_t)dIn[i];
+        mJobC[3 * i + 1] = inBytes[i];
+        mJobC[3 * i + 2] = sumWords;
+        sumWords += ozl_xxh3_batch_sums(inBytes[i]);
+        maxLen = inBytes[i] > maxLen ? inBytes[i] : maxLen;
+        const uint64_t k0 = fOff[i] + fh[i], k1 = body[i] + (cc ? 4 : 0);
+        mJobK[3 * i] = 0;   // rebased below (frame buffer address)
+        mJobK[3 * i + 1] = k1 - k0;
+        mJobK[3 * i + 2] = sumWords;
+        sumWords += ozl_xxh3_batch_sums(k1 - k0);
+        maxLen = k1 - k0 > maxLen ? k1 - k0 : maxLen;
+        mAtC[i] = body[i];
+        mAtK[i] = body[i] + (cc ? 4 : 0);
+    }
+    uint64_t* dMeta = (uint64_t*)c->arena.alloc(meta.size() * 8 + 16);
+    uint64_t* dSums = (uint64_t*)c->arena.alloc(sumWords * 8 + 16);
+    uint64_t* dHash = (uint64_t*)c->arena.alloc(2 * n * 8 + 16);
+    if (!dF || !dHead || !dMeta || !dSums || !dHash) return OZL_ERR_ALLOCATION;
+    size_t maxItem = 0;
+    for (size_t g = 0; g < G; g++) {
+        meta[g] = gSrc[g];
+        meta[G + g] = gSize[g];
+        meta[2 * G + g] = gOff[g];
+        maxItem = gSize[g] > maxItem ? gSize[g] : maxItem;
+    }
+    // header items: offsets into `heads` become device addresses
+    for (size_t i = 0; i < n; i++) meta[headItem[i]] = (uint64_t)(uintptr_t)dHead + gSrc[headItem[i]];
+    for (size_t i = 0; i < n; i++) mJobK[3 * i] = (uint64_t)(uintptr_t)(dF + fOff[i] + fh[i]);
+    if (heads.size()) OZL_HIP_CHECK(hipMemcpyAsync(dHead, heads.data(), heads.size(), hipMemcpyHostToDevice, st));
+    OZL_HIP_CHECK(hipMemcpyAsync(dMeta, meta.data(), meta.size() * 8, hipMemcpyHostToDevice, st));
+    rc = ozl_gather_sized((const uint8_t* const*)dMeta, dMeta + G, dMeta + 2 * G, (int)G, maxItem, dF, st);
+    if (rc) return rc;
+    const ozl_xjob* dJobC = (const ozl_xjob*)(dMeta + 3 * G);
+    const ozl_xjob* dJobK = dJobC + n;
+    const uint64_t* dAtC = dMeta + 3 * G + 6 * n;
+    const uint64_t* dAtK = dAtC + n;
+    if (cc) {
+        rc = ozl_xxh3_device_batch(dJobC, (uint32_t)n, maxLen, dSums, dHash, st);
+        if (!rc) rc = ozl_put_tails(dF, dAtC, dHash, (uint32_t)n, !kc, st);
+        if (rc) return rc;
+    }
+    if (kc) {
+        rc = ozl_xxh3_device_batch(dJobK, (uint32_t)n, maxLen, dSums, dHash + n, st);
+        if (!rc) rc = ozl_put_tails(dF, dAtK, dHash + n, (uint32_t)n, 1, st);
+        if (rc) return rc;
+    }
+    if (!cc && !kc) {
+        rc = ozl_put_tails(dF, dAtC, nullptr, (uint32_t)n, 1, st);
+        if (rc) return rc;
+    }
+    if (!devOut && total) OZL_HIP_CHECK(hipMemcpyAsync(dst, dF, total, hipMemcpyDeviceToHost, st));
+    OZL_HIP_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+}  // namespace
 
 // ====================================================================== C ABI
 extern "C" {
@@ -885,88 +1172,21 @@ void ZL_TypedRef_free(ZL_TypedRef* t) { delete t; }
 
 ZL_Report ZL_CCtx_compressTypedRef(ZL_CCtx* c, void* dst, size_t dstCapacity, const ZL_TypedRef* in)
 {
-    if (!c || !in || !c->comp) return rep_err(ZL_ErrorCode_parameter_invalid);
-    FrameParams fp = resolve(c);
-    if (fp.formatVersion == 0) return rep_err(ZL_ErrorCode_formatVersion_notSet);
-    if (fp.formatVersion != (int)OZL_FORMAT_VERSION) return rep_err(ZL_ErrorCode_formatVersion_unsupported);
-    if (in->type == T_NUMERIC && in->width != 1 && in->width != 2 && in->width != 4 && in->width != 8)
-        return rep_err(ZL_ErrorCode_parameter_invalid);
-    if (!c->st && hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
-        return rep_err(ZL_ErrorCode_allocation);
-    c->arena.reset();
-    size_t inBytes = in->n * in->width;
-    Session S;
-    S.comp = c->comp;
-    S.arena = &c->arena;
-    S.st = c->st;
-    S.minStreamSize = fp.minStreamSize;
-    uint8_t* dIn = S.alloc(inBytes + 16);
-    if (S.err) return rep_err(S.err);
-    if (inBytes && hipMemcpyAsync(dIn, in->ptr, inBytes, hipMemcpyHostToDevice, c->st) != hipSuccess)
-        return rep_err(ZL_ErrorCode_GENERIC);
-    S.newStream(in->type, (int)in->width, in->n, dIn);
-    int rc = S.runSuccessor(c->comp->start ? c->comp->start : GID_STORE, 0);
-    if (!rc) rc = S.err;
+    size_t size = 0;
+    int rc = compress_frames(c, &in, 1, dst, dstCapacity, &size);
+    return rc ? rep_err(rc) : rep_ok(size);
+}
+
+ZL_Report ozl_CCtx_compressBatch(ZL_CCtx* c, void* dst, size_t dstCapacity, size_t* frameSizes,
+                                 const ZL_TypedRef* const* inputs, size_t nbInputs)
+{
+    if (!frameSizes || (!inputs && nbInputs)) return rep_err(ZL_ErrorCode_parameter_invalid);
+    if (nbInputs == 0) return rep_ok(0);
+    int rc = compress_frames(c, inputs, nbInputs, dst, dstCapacity, frameSizes);
     if (rc) return rep_err(rc);
-    std::vector<uint8_t> head;
-    std::vector<const uint8_t*> srcs;
-    std::vector<uint64_t> sizes;
-    rc = build_headers(S, frame_type(in->type), inBytes, fp, head, srcs, sizes);
-    if (rc) return rep_err(rc);
-    size_t total = head.size();
-    for (uint64_t s : sizes) total += s;
-    size_t fh = ozl_write_frame_header(head.data(), frame_type(in->type), inBytes, fp.contentChecksum,
-                                       fp.compressedChecksum);
-    size_t frameSize = total + (fp.contentChecksum ? 4 : 0) + (fp.compressedChecksum ? 4 : 0) + 1;
-    if (frameSize > dstCapacity) return rep_err(ZL_ErrorCode_dstCapacity_tooSmall);
-    // assemble on the device: header, stored streams, checksums
-    uint8_t* dFrame = S.alloc(frameSize + 16);
-    int n = (int)srcs.size();
-    std::vector<uint64_t> dstOffs(n);
-    size_t pos = head.size();
-    for (int k = 0; k < n; k++) { dstOffs[k] = pos; pos += sizes[k]; }
-    uint8_t* meta = S.alloc((size_t)n * 24 + 64);
-    size_t hs = ozl_xxh3_scratch_size(frameSize > inBytes ? frameSize : inBytes);
-    uint8_t* hscratch = S.alloc(hs);
-    uint64_t* dHash = (uint64_t*)S.alloc(16);
-    if (S.err) return rep_err(S.err);
-    if (hipMemcpyAsync(dFrame, head.data(), head.size(), hipMemcpyHostToDevice, c->st) != hipSuccess)
-        return rep_err(ZL_ErrorCode_GENERIC);
-    if (n) {
-        std::vector<uint64_t> m((size_t)n * 3);
-        for (int k = 0; k < n; k++) {
-            m[k] = (uint64_t)(uintptr_t)srcs[k];
-            m[n + k] = sizes[k];
-            m[2 * n + k] = dstOffs[k];
-        }
-        if (hipMemcpyAsync(meta, m.data(), m.size() * 8, hipMemcpyHostToDevice, c->st) != hipSuccess)
-            return rep_err(ZL_ErrorCode_GENERIC);
-        size_t mx = 0;
-        for (uint64_t z : sizes) mx = z > mx ? z : mx;
-        rc = ozl_gather_sized((const uint8_t* const*)meta, (const uint64_t*)meta + n, (const uint64_t*)meta + 2 * n, n,
-                              mx, dFrame, c->st);
-        if (rc) return rep_err(rc);
-    }
-    size_t p = pos;
-    if (fp.contentChecksum) {
-        rc = ozl_xxh3_device(dIn, inBytes, dHash, hscratch, hs, c->st);
-        if (rc) return rep_err(rc);
-        if (hipMemcpyAsync(dFrame + p, dHash, 4, hipMemcpyDeviceToDevice, c->st) != hipSuccess)
-            return rep_err(ZL_ErrorCode_GENERIC);
-        p += 4;
-    }
-    if (fp.compressedChecksum) {
-        rc = ozl_xxh3_device(dFrame + fh, p - fh, dHash + 1, hscratch, hs, c->st);
-        if (rc) return rep_err(rc);
-        if (hipMemcpyAsync(dFrame + p, dHash + 1, 4, hipMemcpyDeviceToDevice, c->st) != hipSuccess)
-            return rep_err(ZL_ErrorCode_GENERIC);
-        p += 4;
-    }
-    if (hipMemsetAsync(dFrame + p, 0, 1, c->st) != hipSuccess) return rep_err(ZL_ErrorCode_GENERIC);
-    p += 1;
-    if (hipMemcpyAsync(dst, dFrame, p, hipMemcpyDeviceToHost, c->st) != hipSuccess) return rep_err(ZL_ErrorCode_GENERIC);
-    if (hipStreamSynchronize(c->st) != hipSuccess) return rep_err(ZL_ErrorCode_GENERIC);
-    return rep_ok(p);
+    size_t total = 0;
+    for (size_t i = 0; i < nbInputs; i++) total += frameSizes[i];
+    return rep_ok(total);
 }
 
 ZL_Report ZL_CCtx_compress(ZL_CCtx* c, void* dst, size_t cap, const void* src, size_t n)

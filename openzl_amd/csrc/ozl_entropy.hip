@@ -144,14 +144,14 @@ __device__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux)
 
 // Histogram of every block (one 256-thread workgroup per block, LDS
 // atomics on 16 interleaved copies), parked in the block's table area.
-__global__ void __launch_bounds__(kT) k_hist(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off, uint32_t B,
-                                             ozl_ews ws)
+__global__ void __launch_bounds__(kT) k_hist(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
+                                             const uint64_t* __restrict__ offEnd, uint32_t B, ozl_ews ws)
 {
     __shared__ __attribute__((aligned(16))) uint32_t hc[kHistCopies * 256];
     __shared__ uint32_t count[256];
     uint32_t b = blockIdx.x;
     if (b >= B) return;
-    block_hist(src + off[b], (uint32_t)(off[b + 1] - off[b]), hc, count);
+    block_hist(src + off[b], (uint32_t)(offEnd[b] - off[b]), hc, count);
     uint32_t* cg = reinterpret_cast<uint32_t*>(ws.tables + (size_t)b * OZL_TAB_BYTES + OZL_TAB_COUNT_OFF);
     for (int s = threadIdx.x; s < 256; s += kT) cg[s] = count[s];
 }
@@ -162,12 +162,12 @@ __global__ void __launch_bounds__(kT) k_hist(const uint8_t* __restrict__ src, co
 // (12 KB) lets a CU run a dozen of these serial builds at once.
 template <int HUF>
 __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
-                                             uint32_t B, ozl_ews ws)
+                                             const uint64_t* __restrict__ offEnd, uint32_t B, ozl_ews ws)
 {
     __shared__ PlanLds L;
     uint32_t b = blockIdx.x;
     if (b >= B) return;
-    uint32_t n = (uint32_t)(off[b + 1] - off[b]);
+    uint32_t n = (uint32_t)(offEnd[b] - off[b]);
     uint8_t* tab = ws.tables + (size_t)b * OZL_TAB_BYTES;
     {
         const uint32_t* cg = reinterpret_cast<const uint32_t*>(tab + OZL_TAB_COUNT_OFF);
@@ -782,7 +782,7 @@ int ozl_entropy_compress_batch(int hufMode, const void* src, const uint64_t* off
     ozl_ews ws = ozl_ews_carve(workspace, B, maxBlock);
     ws.frameOff = frameOff;
     const uint8_t* s = (const uint8_t*)src;
-    int rc = ozl_entropy_encode_chunks(hufMode, s, off, B, maxBlock, ws, st);
+    int rc = ozl_entropy_encode_chunks(hufMode, s, off, off + 1, B, maxBlock, ws, st);
     if (rc) return rc;
     OZL_LAUNCH("k_content_hash", k_content_hash, dim3(B), dim3(64), 0, st, s, off, B, ws);
     OZL_LAUNCH("k_frame_size", k_frame_size, dim3((B + 63) / 64), dim3(64), 0, st, B, ws);
@@ -795,14 +795,14 @@ int ozl_entropy_compress_batch(int hufMode, const void* src, const uint64_t* off
 
 }  // extern "C"
 
-int ozl_entropy_encode_chunks(int hufMode, const uint8_t* s, const uint64_t* off, uint32_t B, size_t maxBlock,
-                              const ozl_ews& ws, hipStream_t st)
+int ozl_entropy_encode_chunks(int hufMode, const uint8_t* s, const uint64_t* off, const uint64_t* offEnd, uint32_t B,
+                              size_t maxBlock, const ozl_ews& ws, hipStream_t st)
 {
     if (B == 0) return OZL_OK;
     if (maxBlock >= 100000) return OZL_ERR_PARAMETER_INVALID;
-    OZL_LAUNCH("k_hist", k_hist, dim3(B), dim3(kT), 0, st, s, off, B, ws);
-    if (hufMode) OZL_LAUNCH("k_plan", k_plan<1>, dim3(B), dim3(64), 0, st, s, off, B, ws);
-    else OZL_LAUNCH("k_plan", k_plan<0>, dim3(B), dim3(64), 0, st, s, off, B, ws);
+    OZL_LAUNCH("k_hist", k_hist, dim3(B), dim3(kT), 0, st, s, off, offEnd, B, ws);
+    if (hufMode) OZL_LAUNCH("k_plan", k_plan<1>, dim3(B), dim3(64), 0, st, s, off, offEnd, B, ws);
+    else OZL_LAUNCH("k_plan", k_plan<0>, dim3(B), dim3(64), 0, st, s, off, offEnd, B, ws);
     if (hufMode) OZL_LAUNCH("k_huf_encode", k_huf_encode, dim3(B), dim3(kT), 0, st, s, off, B, ws);
     else OZL_LAUNCH("k_fse_encode", k_fse_encode, dim3((B + kFseGroups - 1) / kFseGroups), dim3(64), 0, st, s, off, B, ws);
     OZL_LAUNCH("k_pack_encode", k_pack_encode, dim3(B), dim3(kT), 0, st, s, off, B, ws);

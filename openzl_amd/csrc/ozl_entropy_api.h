@@ -31,8 +31,9 @@ static inline ozl_ews ozl_ews_carve(void* workspace, uint32_t B, size_t maxBlock
     return ws;
 }
 
-// Plans and encodes B chunks (chunk b = src[off[b], off[b+1]), each < 100000
-// bytes) for the FSE (hufMode 0) or Huffman (1) graph: plans, tables, NCount /
-// weights (with their nested graph) and payloads land in the workspace.
-int ozl_entropy_encode_chunks(int hufMode, const uint8_t* src, const uint64_t* off, uint32_t B, size_t maxBlock,
-                              const ozl_ews& ws, hipStream_t st);
+// Plans and encodes B chunks (chunk b = src[off[b], offEnd[b]), each < 100000
+// bytes; offEnd = off + 1 for back-to-back chunks) for the FSE (hufMode 0) or
+// Huffman (1) graph: plans, tables, NCount / weights (with their nested graph)
+// and payloads land in the workspace.
+int ozl_entropy_encode_chunks(int hufMode, const uint8_t* src, const uint64_t* off, const uint64_t* offEnd, uint32_t B,
+                              size_t maxBlock, const ozl_ews& ws, hipStream_t st);

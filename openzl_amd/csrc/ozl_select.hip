@@ -830,11 +830,11 @@ int sel_front(const uint8_t* src, const uint64_t* spans, const uint8_t* widths, 
     X.e.hufB = hufB;
     X.e.fseFirst = ctr.bBase[1];
     if (hufB) {
-        int rc = ozl_entropy_encode_chunks(1, ws.stg, ws.off, hufB, maxChunk, X.e.huf, st);
+        int rc = ozl_entropy_encode_chunks(1, ws.stg, ws.off, ws.off + 1, hufB, maxChunk, X.e.huf, st);
         if (rc) return rc;
     }
     if (fseB) {
-        int rc = ozl_entropy_encode_chunks(0, ws.stg, ws.off + ctr.bBase[1], fseB, maxChunk, X.e.fse, st);
+        int rc = ozl_entropy_encode_chunks(0, ws.stg, ws.off + ctr.bBase[1], ws.off + ctr.bBase[1] + 1, fseB, maxChunk, X.e.fse, st);
         if (rc) return rc;
     }
     return OZL_OK;

@@ -2,34 +2,55 @@
 // and XXH3-64 of long device buffers.
 #include "ozl_internal.h"
 #include "ozl_xxh3.h"
+#include "ozl_gmem.h"
 #include "ozl_profile.h"
+#include "ozl_engine_util.h"
 
 namespace {
 
 constexpr int kT = 256;
 constexpr size_t kGatherPiece = 1 << 16;
 
-// grid.y = stream, grid.x = 64 KiB piece of the stream
+// grid.y = stream, grid.x = 64 KiB piece of the stream. Destination dwords
+// are written whole; a source that is not dword-aligned with its destination
+// is read as two aligned dwords and a funnel shift (an aligned dword holding
+// a byte of the stream lies in the same page as that byte, so no read leaves
+// the stream's pages). The few bytes before the first and after the last
+// whole destination dword of a piece are copied one by one.
 __global__ void __launch_bounds__(kT) k_gather(const uint8_t* const* __restrict__ srcs, const uint64_t* __restrict__ sizes,
                                                const uint64_t* __restrict__ dstOffs, uint8_t* __restrict__ dst)
 {
-    int k = blockIdx.y;
-    size_t size = sizes[k];
-    size_t a = (size_t)blockIdx.x * kGatherPiece;
+    const int k = blockIdx.y;
+    const size_t size = sizes[k];
+    const size_t a = (size_t)blockIdx.x * kGatherPiece;
     if (a >= size) return;
-    size_t b = a + kGatherPiece < size ? a + kGatherPiece : size;
+    const size_t b = a + kGatherPiece < size ? a + kGatherPiece : size;
     const uint8_t* s = srcs[k];
     uint8_t* d = dst + dstOffs[k];
-    for (size_t i = a + threadIdx.x; i < b; i += kT) d[i] = s[i];
+    const size_t h = (4 - ((uintptr_t)d & 3)) & 3;     // first position whose destination is dword-aligned
+    const size_t lo = a + h < b ? a + h : b;
+    const size_t hi = lo + ((b - lo) & ~(size_t)3);
+    const size_t nw = (hi - lo) >> 2;
+    const uintptr_t sa = (uintptr_t)(s + lo);
+    const uint32_t sh = (uint32_t)(sa & 3) * 8;
+    const uint8_t* s4 = (const uint8_t*)(sa & ~(uintptr_t)3);
+    uint8_t* d4 = d + lo;
+    if (sh == 0) {
+        for (size_t j = threadIdx.x; j < nw; j += kT) ozl_gst4(d4 + 4 * j, ozl_gld4(s4 + 4 * j));
+    } else {
+        for (size_t j = threadIdx.x; j < nw; j += kT) {
+            const uint32_t w0 = ozl_gld4(s4 + 4 * j), w1 = ozl_gld4(s4 + 4 * j + 4);
+            ozl_gst4(d4 + 4 * j, __builtin_amdgcn_alignbit(w1, w0, sh));
+        }
+    }
+    if (threadIdx.x < lo - a) d[a + threadIdx.x] = s[a + threadIdx.x];
+    if (threadIdx.x < b - hi) d[hi + threadIdx.x] = s[hi + threadIdx.x];
 }
 
 // XXH3 of a long buffer: per 1 KiB block, the 8 accumulator sums of its 16
 // stripes (independent across blocks) ...
-__global__ void __launch_bounds__(kT) k_xxh3_blocks(const uint8_t* __restrict__ in, size_t nbBlocks,
-                                                    uint64_t* __restrict__ sums)
+__device__ inline void xxh3_block_sums(const uint8_t* in, size_t nbBlocks, uint64_t* sums, size_t blk, int i)
 {
-    size_t blk = (size_t)blockIdx.x * (kT / 8) + (threadIdx.x >> 3);
-    int i = threadIdx.x & 7;
     if (blk >= nbBlocks) return;
     const uint8_t* p = in + blk * 1024;
     uint64_t s = 0;
@@ -38,8 +59,7 @@ __global__ void __launch_bounds__(kT) k_xxh3_blocks(const uint8_t* __restrict__ 
 }
 
 // ... then one wavefront runs the scramble chain, the tail and the merge.
-__global__ void __launch_bounds__(64) k_xxh3_finish(const uint8_t* __restrict__ in, size_t len,
-                                                    const uint64_t* __restrict__ sums, uint64_t* __restrict__ out)
+__device__ inline void xxh3_finish_wave(const uint8_t* in, size_t len, const uint64_t* sums, uint64_t* out)
 {
     int lane = threadIdx.x;
     if (len <= 240) {
@@ -89,6 +109,50 @@ __global__ void __launch_bounds__(64) k_xxh3_finish(const uint8_t* __restrict__ 
     if (lane == 0) *out = ozl_xxh3_merge(a, len);
 }
 
+__global__ void __launch_bounds__(kT) k_xxh3_blocks(const uint8_t* __restrict__ in, size_t nbBlocks,
+                                                    uint64_t* __restrict__ sums)
+{
+    xxh3_block_sums(in, nbBlocks, sums, (size_t)blockIdx.x * (kT / 8) + (threadIdx.x >> 3), threadIdx.x & 7);
+}
+
+__global__ void __launch_bounds__(64) k_xxh3_finish(const uint8_t* __restrict__ in, size_t len,
+                                                    const uint64_t* __restrict__ sums, uint64_t* __restrict__ out)
+{
+    xxh3_finish_wave(in, len, sums, out);
+}
+
+// many buffers: grid.y = buffer (block sums), grid.x = buffer (chains)
+__global__ void __launch_bounds__(kT) k_xxh3_blocks_n(const ozl_xjob* __restrict__ jobs, uint64_t* __restrict__ sums)
+{
+    const ozl_xjob j = jobs[blockIdx.y];
+    const size_t nbBlocks = j.len > 240 ? (j.len - 1) / 1024 : 0;
+    xxh3_block_sums((const uint8_t*)(uintptr_t)j.ptr, nbBlocks, sums + j.sumOff,
+                    (size_t)blockIdx.x * (kT / 8) + (threadIdx.x >> 3), threadIdx.x & 7);
+}
+
+__global__ void __launch_bounds__(64) k_xxh3_finish_n(const ozl_xjob* __restrict__ jobs, const uint64_t* __restrict__ sums,
+                                                      uint64_t* __restrict__ out)
+{
+    const ozl_xjob j = jobs[blockIdx.x];
+    xxh3_finish_wave((const uint8_t*)(uintptr_t)j.ptr, j.len, sums + j.sumOff, out + blockIdx.x);
+}
+
+// frame tails: the low 4 bytes of hashes[k] (little endian) at dst + at[k]
+// (when hashes is given), then a zero end byte when endByte is set
+__global__ void __launch_bounds__(64) k_put_tails(uint8_t* __restrict__ dst, const uint64_t* __restrict__ at,
+                                                  const uint64_t* __restrict__ hashes, uint32_t count, int endByte)
+{
+    const uint32_t k = blockIdx.x * 64 + threadIdx.x;
+    if (k >= count) return;
+    uint8_t* p = dst + at[k];
+    if (hashes) {
+        const uint64_t h = hashes[k];
+        for (int q = 0; q < 4; q++) p[q] = (uint8_t)(h >> (8 * q));
+        p += 4;
+    }
+    if (endByte) p[0] = 0;
+}
+
 }  // namespace
 
 size_t ozl_xxh3_scratch_size(size_t len) { return ((len / 1024) + 1) * 64 + 256; }
@@ -103,6 +167,47 @@ int ozl_xxh3_device(const uint8_t* data, size_t len, uint64_t* dOut, void* scrat
     OZL_LAUNCH("k_xxh3_finish", k_xxh3_finish, dim3(1), dim3(64), 0, st, data, len, sums, dOut);
     OZL_HIP_CHECK(hipGetLastError());
     return OZL_OK;
+}
+
+size_t ozl_xxh3_batch_sums(size_t len) { return len > 240 ? ((len - 1) / 1024) * 8 : 0; }
+
+// XXH3-64 of `count` device buffers (jobs: device array; the block sums of
+// job k at sums + jobs[k].sumOff, ozl_xxh3_batch_sums(len) words) into
+// dOut[k], asynchronous.
+int ozl_xxh3_device_batch(const ozl_xjob* jobs, uint32_t count, size_t maxLen, uint64_t* sums, uint64_t* dOut,
+                          hipStream_t st)
+{
+    if (count == 0) return OZL_OK;
+    const size_t maxBlocks = maxLen > 240 ? (maxLen - 1) / 1024 : 0;
+    if (maxBlocks)
+        for (uint32_t y0 = 0; y0 < count; y0 += 65535) {
+            const uint32_t cnt = count - y0 < 65535 ? count - y0 : 65535;
+            OZL_LAUNCH("k_xxh3_blocks_n", k_xxh3_blocks_n, dim3((unsigned)((maxBlocks + 31) / 32), cnt), dim3(kT), 0, st,
+                       jobs + y0, sums);
+        }
+    OZL_LAUNCH("k_xxh3_finish_n", k_xxh3_finish_n, dim3(count), dim3(64), 0, st, jobs, sums, dOut);
+    OZL_HIP_CHECK(hipGetLastError());
+    return OZL_OK;
+}
+
+// Writes frame tails (see k_put_tails); at and hashes are device arrays.
+int ozl_put_tails(uint8_t* dst, const uint64_t* at, const uint64_t* hashes, uint32_t count, int endByte, hipStream_t st)
+{
+    if (count == 0) return OZL_OK;
+    OZL_LAUNCH("k_put_tails", k_put_tails, dim3((count + 63) / 64), dim3(64), 0, st, dst, at, hashes, count, endByte);
+    OZL_HIP_CHECK(hipGetLastError());
+    return OZL_OK;
+}
+
+// Whether p points into device memory (host memory, pinned or pageable, is not).
+bool ozl_is_device_ptr(const void* p)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();   // plain host memory: clear the error so no later check sees it
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice;
 }
 
 // Copies `count` device buffers srcs[k] (sizes[k] bytes, at most maxSize) to

@@ -8,6 +8,8 @@ graph (transpose_split, plane 0 FSE, plane 1 Huffman), and the other graphs of
 the selector candidates, at sizes that exercise 32 KiB chunking (>= 100000
 elements), minStreamSize auto-store and tiny / empty inputs.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -84,3 +86,73 @@ def test_gpu_decompressor_roundtrips(api):
         if out != arr.tobytes():
             bad.append((name, arr.nbytes))
     assert not bad, bad
+
+
+def batch_cases():
+    """Groups of inputs for one batch call each: (kind, type, width, arrays)."""
+    from openzl_amd import datagen as G
+    rng = np.random.default_rng(23)
+    ts = G.timestamps_u32(600000)
+    w = G.bf16_normal(1 << 20)
+    return [
+        ("cfg1", 4, 2, 4, [ts[:262144], ts[262144:262144 + 5], ts[:0], ts[300000:600000]]),
+        ("cfg4", 6, 1, 2, [w[:300000], w[300000:300001], w[300001:]]),
+        ("fse", 1, 0, 1, [zipf_bytes(rng, n, 1.1) for n in (10, 5000, 99999, 150000)]),
+        ("huf", 2, 0, 1, [zipf_bytes(rng, n, 1.2) for n in (150000, 0, 70000)]),
+        ("zzbp", 5, 2, 2, [c for c in G.random_walk_columns(n_cols=5, min_len=16, max_len=50000)
+                           if c.dtype.itemsize == 2] or [np.arange(100, dtype=np.uint16)]),
+    ]
+
+
+def test_batch_frames_match_reference(api):
+    """ozl_CCtx_compressBatch writes, per input, the reference's frame, and
+    ozl_DCtx_decompressBatch returns the inputs, for host buffers."""
+    A = api
+    dec = A.Decompressor()
+    for name, kind, typ, width, arrs in batch_cases():
+        print(f"batch {name} inputs={len(arrs)}", flush=True)
+        comp = A.Compressor(kind)
+        at = {0: A.SERIAL, 1: A.STRUCT, 2: A.NUMERIC}[typ]
+        frames = comp.compress_batch(arrs, at, width)
+        for a, f in zip(arrs, frames):
+            assert f == C.ref_compress(kind, a, TYPES[typ], width), (name, a.nbytes)
+        outs = dec.decompress_batch(frames, sum(a.nbytes for a in arrs) + 64)
+        assert outs == [a.tobytes() for a in arrs], name
+
+
+def test_batch_device_buffers(api):
+    """Inputs, frames and outputs in device memory: the same frames as from
+    host buffers, decoded back in place; single calls take device memory too."""
+    import torch
+    A = api
+    dec = A.Decompressor()
+    for name, kind, typ, width, arrs in batch_cases()[:3]:
+        print(f"device batch {name}", flush=True)
+        comp = A.Compressor(kind)
+        at = {0: A.SERIAL, 1: A.STRUCT, 2: A.NUMERIC}[typ]
+        want = comp.compress_batch(arrs, at, width)
+        dins = [torch.from_numpy(a.view(np.uint8).copy()).cuda() for a in arrs]
+        cap = sum(len(f) for f in want) + 4096
+        dfr = torch.zeros(cap, dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        sizes = comp.compress_ptrs([t.data_ptr() for t in dins], [a.nbytes for a in arrs], at, width,
+                                   dfr.data_ptr(), cap)
+        got = dfr[:sum(sizes)].cpu().numpy().tobytes()
+        assert sizes == [len(f) for f in want] and got == b"".join(want), name
+        total = sum(a.nbytes for a in arrs)
+        dout = torch.zeros(total + 64, dtype=torch.uint8, device="cuda")
+        outs = dec.decompress_ptrs(dfr.data_ptr(), sizes, dout.data_ptr(), total + 64)
+        assert outs == [a.nbytes for a in arrs], name
+        assert dout[:total].cpu().numpy().tobytes() == b"".join(a.tobytes() for a in arrs), name
+        # one frame through the single-frame calls, device in and out
+        L = comp.L
+        tr = comp._typed_ref(dins[-1].data_ptr(), arrs[-1].nbytes, at, width)
+        try:
+            z = A._chk(L.ZL_CCtx_compressTypedRef(comp.cctx, dfr.data_ptr(), cap, tr), "compress")
+        finally:
+            L.ZL_TypedRef_free(tr)
+        assert dfr[:z].cpu().numpy().tobytes() == want[-1], name
+        info = A.OutputInfo()
+        m = A._chk(dec.L.ZL_DCtx_decompressTyped(dec.d, ctypes.byref(info), dout.data_ptr(), total + 64,
+                                                 dfr.data_ptr(), z), "decompress")
+        assert dout[:m].cpu().numpy().tobytes() == arrs[-1].tobytes(), name
