@@ -25,6 +25,7 @@ int ozl_delta_decode(const void* first, const void* deltas, size_t n, int width,
 size_t ozl_delta_decode_scratch_size(size_t n, int width);
 int ozl_zigzag_decode(const void* src, size_t n, int width, void* dst, void* stream);
 int ozl_transpose_join(const void* planes, size_t n, int width, void* dst, void* stream);
+int ozl_transpose_join_planes(const void* const* planes, size_t n, int width, void* dst, void* stream);
 int ozl_bitpack_decode(const void* src, size_t n, int width, int nbBits, void* dst, void* stream);
 }
 
@@ -165,14 +166,21 @@ int nb_singletons(uint8_t trid)
 
 struct ZL_DCtx_s {
     hipStream_t st = nullptr;
+    hipStream_t st2 = nullptr;   // compressed checksums, alongside the decoding
+    hipEvent_t evIn = nullptr, evHash = nullptr;
     uint8_t* base = nullptr;
     size_t cap = 0, used = 0;
     std::vector<uint8_t*> old;
     ~ZL_DCtx_s()
     {
+        if (st2) (void)hipStreamSynchronize(st2);
+        if (st) (void)hipStreamSynchronize(st);
         if (base) (void)hipFree(base);
         for (uint8_t* p : old) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
+        if (st2) (void)hipStreamDestroy(st2);
+        if (evIn) (void)hipEventDestroy(evIn);
+        if (evHash) (void)hipEventDestroy(evHash);
     }
     void reset()
     {
@@ -216,6 +224,7 @@ struct DQueue {
     std::vector<std::pair<int*, uint32_t>> statuses;
     std::vector<std::vector<uint8_t>> keep;   // host bytes asynchronous copies read
     std::vector<Decoder*> members;
+    std::vector<uint64_t> gSrc, gSize, gDst;  // queued copies (device addresses)
     int err = 0;
 
     uint8_t* alloc(size_t n)
@@ -262,24 +271,17 @@ struct Decoder {
     }
     uint8_t* host_keep(const void* p, size_t n) { return q->host_keep(p, n); }
 
-    // copies of `count` device ranges into dst at the given offsets, one launch
-    int gather(const std::vector<const uint8_t*>& srcs, const std::vector<uint64_t>& sizes,
-               const std::vector<uint64_t>& offs, uint8_t* dst)
+    // copies of device ranges into dst at the given offsets, queued: they go
+    // out with the other queued work (one launch for all queued copies)
+    void gather(const std::vector<const uint8_t*>& srcs, const std::vector<uint64_t>& sizes,
+                const std::vector<uint64_t>& offs, uint8_t* dst)
     {
-        const size_t n = srcs.size();
-        if (n == 0) return 0;
-        std::vector<uint64_t> m(3 * n);
-        size_t mx = 0;
-        for (size_t k = 0; k < n; k++) {
-            m[k] = (uint64_t)(uintptr_t)srcs[k];
-            m[n + k] = sizes[k];
-            m[2 * n + k] = offs[k];
-            mx = sizes[k] > mx ? sizes[k] : mx;
+        for (size_t k = 0; k < srcs.size(); k++) {
+            if (!sizes[k]) continue;
+            q->gSrc.push_back((uint64_t)(uintptr_t)srcs[k]);
+            q->gSize.push_back(sizes[k]);
+            q->gDst.push_back((uint64_t)(uintptr_t)dst + offs[k]);
         }
-        uint64_t* dm = (uint64_t*)alloc(m.size() * 8);
-        if (err) return err;
-        OZL_HIP_CHECK(hipMemcpyAsync(dm, host_keep(m.data(), m.size() * 8), m.size() * 8, hipMemcpyHostToDevice, st));
-        return mx ? ozl_gather_sized((const uint8_t* const*)dm, dm + n, dm + 2 * n, (int)n, mx, dst, st) : 0;
     }
 
     // make sure the given streams hold their final content
@@ -418,18 +420,27 @@ struct Decoder {
             size_t n = S[o0].bytes;
             for (uint32_t p = 1; p < W; p++)
                 if (S[o0 + p].bytes != n) return OZL_ERR_CORRUPTION;  // decode_transpose_binding.c:54-57
-            uint8_t* planes = alloc(n * W);
             in.dev = alloc(n * W);
             if (err) return err;
-            if (n) {
-                std::vector<const uint8_t*> srcs(W);
-                std::vector<uint64_t> sz(W, n), offs(W);
-                for (uint32_t p = 0; p < W; p++) { srcs[p] = S[o0 + p].dev; offs[p] = (uint64_t)p * n; }
-                rc = gather(srcs, sz, offs, planes);
+            if (n && W <= 8) {   // straight from the planes
+                const void* pl[8];
+                for (uint32_t p = 0; p < W; p++) pl[p] = S[o0 + p].dev;
+                rc = ozl_transpose_join_planes(pl, n, (int)W, in.dev, st);
                 if (rc) return rc;
-            }
-            if (n) {
-                rc = ozl_transpose_join(planes, n, (int)W, in.dev, st);
+            } else if (n) {      // wider: planes copied together first
+                uint8_t* planes = alloc(n * W);
+                if (err) return err;
+                std::vector<uint64_t> m(3 * (size_t)W);
+                for (uint32_t p = 0; p < W; p++) {
+                    m[p] = (uint64_t)(uintptr_t)S[o0 + p].dev;
+                    m[W + p] = n;
+                    m[2 * W + p] = (uint64_t)p * n;
+                }
+                uint64_t* dm = (uint64_t*)alloc(m.size() * 8);
+                if (err) return err;
+                OZL_HIP_CHECK(hipMemcpyAsync(dm, host_keep(m.data(), m.size() * 8), m.size() * 8, hipMemcpyHostToDevice, st));
+                rc = ozl_gather_sized((const uint8_t* const*)dm, dm + W, dm + 2 * W, (int)W, n, planes, st);
+                if (!rc) rc = ozl_transpose_join(planes, n, (int)W, in.dev, st);
                 if (rc) return rc;
             }
             in.bytes = n * W;
@@ -519,12 +530,13 @@ struct Decoder {
                 }
                 pos += S[o0 + k].bytes;
             }
-            rc = gather(srcs, sz, offs, in.dev);
-            if (rc) return rc;
+            gather(srcs, sz, offs, in.dev);
             in.bytes = total;
             in.type = T_SERIAL_;
             in.width = 1;
-            break;
+            in.ready = false;
+            in.pending = true;   // the concatenation goes out with the queue
+            return 0;
         }
         default: return OZL_ERR_CORRUPTION;
         }
@@ -538,6 +550,28 @@ struct Decoder {
 
 int DQueue::flush()
 {
+    if (!gSrc.empty()) {
+        // destinations relative to the lowest one
+        const size_t G = gSrc.size();
+        uint64_t lo = gDst[0];
+        for (uint64_t x : gDst) lo = x < lo ? x : lo;
+        std::vector<uint64_t> m(3 * G);
+        size_t mx = 0;
+        for (size_t k = 0; k < G; k++) {
+            m[k] = gSrc[k];
+            m[G + k] = gSize[k];
+            m[2 * G + k] = gDst[k] - lo;
+            mx = gSize[k] > mx ? gSize[k] : mx;
+        }
+        uint64_t* dm = (uint64_t*)alloc(m.size() * 8);
+        if (err) return err;
+        OZL_HIP_CHECK(hipMemcpyAsync(dm, host_keep(m.data(), m.size() * 8), m.size() * 8, hipMemcpyHostToDevice, st));
+        int rc = ozl_gather_sized((const uint8_t* const*)dm, dm + G, dm + 2 * G, (int)G, mx, (uint8_t*)(uintptr_t)lo, st);
+        if (rc) return rc;
+        gSrc.clear();
+        gSize.clear();
+        gDst.clear();
+    }
     if (!fills.empty()) {
         const uint32_t B = (uint32_t)fills.size();
         uint64_t mx = 0;
@@ -703,8 +737,17 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
                       size_t* outSizes, ZL_OutputInfo* infos)
 {
     if (!dc->st && hipStreamCreateWithFlags(&dc->st, hipStreamNonBlocking) != hipSuccess) return OZL_ERR_ALLOCATION;
+    if (!dc->st2 && hipStreamCreateWithFlags(&dc->st2, hipStreamNonBlocking) != hipSuccess) return OZL_ERR_ALLOCATION;
+    if (!dc->evIn && hipEventCreateWithFlags(&dc->evIn, hipEventDisableTiming) != hipSuccess) return OZL_ERR_ALLOCATION;
+    if (!dc->evHash && hipEventCreateWithFlags(&dc->evHash, hipEventDisableTiming) != hipSuccess) return OZL_ERR_ALLOCATION;
     hipStream_t st = dc->st;
+    // a call that failed part-way may have left work running
+    OZL_HIP_CHECK(hipStreamSynchronize(dc->st2));
+    OZL_HIP_CHECK(hipStreamSynchronize(st));
     dc->reset();
+    // every return (errors included) leaves no work of this call running:
+    // the kernels read the caller's buffers
+    OzlStreamsDone done{ st, dc->st2 };
     std::vector<FrameJob> J(n);
     const bool devIn = n && ozl_is_device_ptr(src);
     size_t off = 0;
@@ -714,30 +757,51 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         J[i].dev = devIn;
         off += frameSizes[i];
     }
-    // headers of device frames: the first 256 KiB of each come to the host,
-    // the whole frame when its headers are longer
+    // headers of device frames: the first 64 KiB of each come to the host in
+    // one copy, the whole frame when its headers are longer
+    constexpr size_t kPre = (size_t)64 << 10;
+    std::vector<uint8_t> pre;
     if (devIn) {
-        for (FrameJob& j : J) {
-            j.hbuf.reset(new uint8_t[j.size ? j.size : 1]);
-            j.avail = j.size < ((size_t)256 << 10) ? j.size : ((size_t)256 << 10);
-            if (j.avail) OZL_HIP_CHECK(hipMemcpyAsync(j.hbuf.get(), j.src, j.avail, hipMemcpyDeviceToHost, st));
-            j.host = j.hbuf.get();
+        std::vector<uint64_t> m(3 * n);
+        for (size_t i = 0; i < n; i++) {
+            m[i] = (uint64_t)(uintptr_t)J[i].src;
+            m[n + i] = J[i].size < kPre ? J[i].size : kPre;
+            m[2 * n + i] = i * kPre;
         }
+        uint8_t* dPre = dc->alloc(n * kPre + 16);
+        uint64_t* dm = (uint64_t*)dc->alloc(m.size() * 8 + 16);
+        if (!dPre || !dm) return OZL_ERR_ALLOCATION;
+        pre.resize(n * kPre);
+        OZL_HIP_CHECK(hipMemcpyAsync(dm, m.data(), m.size() * 8, hipMemcpyHostToDevice, st));
+        int rc = ozl_gather_sized((const uint8_t* const*)dm, dm + n, dm + 2 * n, (int)n, kPre, dPre, st);
+        if (rc) return rc;
+        OZL_HIP_CHECK(hipMemcpyAsync(pre.data(), dPre, n * kPre, hipMemcpyDeviceToHost, st));
         OZL_HIP_CHECK(hipStreamSynchronize(st));
+        for (size_t i = 0; i < n; i++) {
+            J[i].host = pre.data() + i * kPre;
+            J[i].avail = m[n + i];
+        }
     } else {
         for (FrameJob& j : J) {
             j.host = j.src;
             j.avail = j.size;
         }
     }
-    for (FrameJob& j : J) {
-        int rc = parse_frame(j.host, j.size, j.avail, j.P);
-        if (rc && j.avail < j.size) {
-            OZL_HIP_CHECK(hipMemcpy(j.hbuf.get(), j.src, j.size, hipMemcpyDeviceToHost));
-            j.avail = j.size;
-            rc = parse_frame(j.host, j.size, j.avail, j.P);
+    {
+        std::vector<int> prc(n, 0);
+        ozl_parallel_for(n, [&](size_t i) { prc[i] = parse_frame(J[i].host, J[i].size, J[i].avail, J[i].P); });
+        for (size_t i = 0; i < n; i++) {
+            FrameJob& j = J[i];
+            int rc = prc[i];
+            if (rc && j.avail < j.size) {
+                j.hbuf.reset(new uint8_t[j.size]);
+                OZL_HIP_CHECK(hipMemcpy(j.hbuf.get(), j.src, j.size, hipMemcpyDeviceToHost));
+                j.host = j.hbuf.get();
+                j.avail = j.size;
+                rc = parse_frame(j.host, j.size, j.avail, j.P);
+            }
+            if (rc) return rc;
         }
-        if (rc) return rc;
     }
     DQueue Q;
     Q.d = dc;
@@ -775,11 +839,52 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
             if (nd.trid == OZL_TR_HUFFMAN_V2 && D.S[nd.firstOut].ready) { D.S[nd.firstOut].type = ZL_Type_numeric; }
         }
         j.next = (int64_t)P.nodes.size() - 1;
-        // checksums and end byte
+        // checksums and end byte (device frames: gathered below)
         j.tailOff = P.csumPos;
-        const size_t tl = P.end + 1 - P.csumPos;
-        if (j.dev) OZL_HIP_CHECK(hipMemcpyAsync(j.tail, j.src + P.csumPos, tl, hipMemcpyDeviceToHost, st));
-        else memcpy(j.tail, j.src + P.csumPos, tl);
+        if (!j.dev) memcpy(j.tail, j.src + P.csumPos, P.end + 1 - P.csumPos);
+    }
+    std::vector<uint8_t> tails;
+    uint8_t* dTails = nullptr;
+    if (devIn) {
+        std::vector<uint64_t> m(3 * n);
+        for (size_t i = 0; i < n; i++) {
+            m[i] = (uint64_t)(uintptr_t)(J[i].src + J[i].P.csumPos);
+            m[n + i] = J[i].P.end + 1 - J[i].P.csumPos;
+            m[2 * n + i] = i * 16;
+        }
+        dTails = dc->alloc(n * 16 + 16);
+        uint64_t* dm = (uint64_t*)dc->alloc(m.size() * 8 + 16);
+        if (!dTails || !dm) return OZL_ERR_ALLOCATION;
+        OZL_HIP_CHECK(hipMemcpyAsync(dm, Q.host_keep(m.data(), m.size() * 8), m.size() * 8, hipMemcpyHostToDevice, st));
+        int rc = ozl_gather_sized((const uint8_t* const*)dm, dm + n, dm + 2 * n, (int)n, 16, dTails, st);
+        if (rc) return rc;
+        tails.resize(n * 16);
+    }
+    // compressed checksums (they read only the frames) on the second stream
+    std::vector<uint64_t> kJobs;
+    std::vector<size_t> kIdx;
+    size_t kSum = 0, kMax = 0;
+    for (size_t i = 0; i < n; i++) {
+        const ParsedFrame& P = J[i].P;
+        if (!(P.flags & 2)) continue;
+        const size_t clen = P.csumPos + ((P.flags & 1) ? 4 : 0) - P.chunkStart;
+        kJobs.insert(kJobs.end(), { (uint64_t)(uintptr_t)(J[i].dFrame + P.chunkStart), clen, kSum });
+        kIdx.push_back(i);
+        kSum += ozl_xxh3_batch_sums(clen);
+        kMax = clen > kMax ? clen : kMax;
+    }
+    uint64_t* dHK = nullptr;
+    if (!kIdx.empty()) {
+        uint64_t* dJobs = (uint64_t*)dc->alloc(kJobs.size() * 8 + 16);
+        uint64_t* dSums = (uint64_t*)dc->alloc(kSum * 8 + 16);
+        dHK = (uint64_t*)dc->alloc(kIdx.size() * 8 + 16);
+        if (!dJobs || !dSums || !dHK) return OZL_ERR_ALLOCATION;
+        OZL_HIP_CHECK(hipMemcpyAsync(dJobs, kJobs.data(), kJobs.size() * 8, hipMemcpyHostToDevice, st));
+        OZL_HIP_CHECK(hipEventRecord(dc->evIn, st));
+        OZL_HIP_CHECK(hipStreamWaitEvent(dc->st2, dc->evIn, 0));
+        int rc = ozl_xxh3_device_batch((const ozl_xjob*)dJobs, (uint32_t)kIdx.size(), kMax, dSums, dHK, dc->st2);
+        if (rc) return rc;
+        OZL_HIP_CHECK(hipEventRecord(dc->evHash, dc->st2));
     }
     // the decoders in lockstep
     for (;;) {
@@ -799,12 +904,12 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         if (rc) return rc;
         if (!left) break;
     }
-    // outputs, checksum jobs
+    // outputs, content checksum jobs
     size_t total = 0;
     std::vector<uint64_t> outOff(n);
-    std::vector<uint64_t> jobs;   // ozl_xjob triples: content hashes, then compressed hashes
-    std::vector<int> jobKind;     // per job: frame index * 2 + (0 content, 1 compressed)
-    size_t sumWords = 0, maxLen = 0;
+    std::vector<uint64_t> cJobs;
+    std::vector<size_t> cIdx;
+    size_t cSum = 0, cMax = 0;
     for (size_t i = 0; i < n; i++) {
         FrameJob& j = J[i];
         const ParsedFrame& P = j.P;
@@ -819,31 +924,27 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         total += out.bytes;
         if (P.flags & 1) {
             const uint8_t* p = out.dev ? out.dev : j.dFrame;
-            jobs.insert(jobs.end(), { (uint64_t)(uintptr_t)p, out.bytes, sumWords });
-            jobKind.push_back((int)(2 * i));
-            sumWords += ozl_xxh3_batch_sums(out.bytes);
-            maxLen = out.bytes > maxLen ? out.bytes : maxLen;
-        }
-        if (P.flags & 2) {
-            const size_t clen = P.csumPos + ((P.flags & 1) ? 4 : 0) - P.chunkStart;
-            jobs.insert(jobs.end(), { (uint64_t)(uintptr_t)(j.dFrame + P.chunkStart), clen, sumWords });
-            jobKind.push_back((int)(2 * i + 1));
-            sumWords += ozl_xxh3_batch_sums(clen);
-            maxLen = clen > maxLen ? clen : maxLen;
+            cJobs.insert(cJobs.end(), { (uint64_t)(uintptr_t)p, out.bytes, cSum });
+            cIdx.push_back(i);
+            cSum += ozl_xxh3_batch_sums(out.bytes);
+            cMax = out.bytes > cMax ? out.bytes : cMax;
         }
     }
     if (total > cap) return OZL_ERR_DSTCAPACITY_TOOSMALL;
-    const uint32_t nj = (uint32_t)jobKind.size();
-    std::vector<uint64_t> hashes(nj);
-    if (nj) {
-        uint64_t* dJobs = (uint64_t*)dc->alloc(jobs.size() * 8 + 16);
-        uint64_t* dSums = (uint64_t*)dc->alloc(sumWords * 8 + 16);
-        uint64_t* dH = (uint64_t*)dc->alloc((size_t)nj * 8 + 16);
-        if (!dJobs || !dSums || !dH) return OZL_ERR_ALLOCATION;
-        OZL_HIP_CHECK(hipMemcpyAsync(dJobs, jobs.data(), jobs.size() * 8, hipMemcpyHostToDevice, st));
-        int rc = ozl_xxh3_device_batch((const ozl_xjob*)dJobs, nj, maxLen, dSums, dH, st);
+    std::vector<uint64_t> hC(cIdx.size()), hK(kIdx.size());
+    if (!cIdx.empty()) {
+        uint64_t* dJobs = (uint64_t*)dc->alloc(cJobs.size() * 8 + 16);
+        uint64_t* dSums = (uint64_t*)dc->alloc(cSum * 8 + 16);
+        uint64_t* dHC = (uint64_t*)dc->alloc(cIdx.size() * 8 + 16);
+        if (!dJobs || !dSums || !dHC) return OZL_ERR_ALLOCATION;
+        OZL_HIP_CHECK(hipMemcpyAsync(dJobs, cJobs.data(), cJobs.size() * 8, hipMemcpyHostToDevice, st));
+        int rc = ozl_xxh3_device_batch((const ozl_xjob*)dJobs, (uint32_t)cIdx.size(), cMax, dSums, dHC, st);
         if (rc) return rc;
-        OZL_HIP_CHECK(hipMemcpyAsync(hashes.data(), dH, (size_t)nj * 8, hipMemcpyDeviceToHost, st));
+        OZL_HIP_CHECK(hipMemcpyAsync(hC.data(), dHC, cIdx.size() * 8, hipMemcpyDeviceToHost, st));
+    }
+    if (!kIdx.empty()) {
+        OZL_HIP_CHECK(hipStreamWaitEvent(st, dc->evHash, 0));
+        OZL_HIP_CHECK(hipMemcpyAsync(hK.data(), dHK, kIdx.size() * 8, hipMemcpyDeviceToHost, st));
     }
     const bool devOut = total && ozl_is_device_ptr(dst);
     for (size_t i = 0; i < n; i++) {
@@ -852,19 +953,19 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         OZL_HIP_CHECK(hipMemcpyAsync((uint8_t*)dst + outOff[i], out.dev, out.bytes,
                                      devOut ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     }
+    if (devIn) OZL_HIP_CHECK(hipMemcpyAsync(tails.data(), dTails, n * 16, hipMemcpyDeviceToHost, st));
     int rc = Q.check_statuses();   // synchronises the stream
     if (rc) return rc;
+    if (devIn)
+        for (size_t i = 0; i < n; i++) memcpy(J[i].tail, tails.data() + i * 16, J[i].P.end + 1 - J[i].P.csumPos);
     for (size_t i = 0; i < n; i++)
         if (J[i].tail[J[i].P.end - J[i].P.csumPos] != 0) return OZL_ERR_CORRUPTION;
-    for (uint32_t k = 0; k < nj; k++) {
-        const FrameJob& j = J[(size_t)jobKind[k] / 2];
-        if (jobKind[k] & 1) {
-            const size_t at = (j.P.flags & 1) ? 4 : 0;
-            if ((uint32_t)hashes[k] != ozl_le32(j.tail + at)) return OZL_ERR_COMPRESSED_CHECKSUM;
-        } else if ((uint32_t)hashes[k] != ozl_le32(j.tail)) {
-            return OZL_ERR_CONTENT_CHECKSUM;
-        }
+    for (size_t k = 0; k < kIdx.size(); k++) {
+        const FrameJob& j = J[kIdx[k]];
+        if ((uint32_t)hK[k] != ozl_le32(j.tail + ((j.P.flags & 1) ? 4 : 0))) return OZL_ERR_COMPRESSED_CHECKSUM;
     }
+    for (size_t k = 0; k < cIdx.size(); k++)
+        if ((uint32_t)hC[k] != ozl_le32(J[cIdx[k]].tail)) return OZL_ERR_CONTENT_CHECKSUM;
     for (size_t i = 0; i < n; i++) {
         const DStream& out = J[i].D->S[0];
         outSizes[i] = out.bytes;

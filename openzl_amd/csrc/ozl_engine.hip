@@ -717,13 +717,25 @@ int EntBatch::run(DevArena* arena, hipStream_t st)
     return 0;
 }
 
+// Sessions expand their calls on parallel host threads (an expansion only
+// touches its own session's node and stream lists).
 int EntBatch::expand()
 {
-    for (const EntReq& r : reqs) {
-        const int m = r.huf ? 1 : 0;
-        int rc = r.s->expandEntropy(r, plans[m].data() + r.first, ws[m]);
+    std::vector<size_t> runs;   // first request of each session's run
+    for (size_t k = 0; k < reqs.size(); k++)
+        if (k == 0 || reqs[k].s != reqs[k - 1].s) runs.push_back(k);
+    runs.push_back(reqs.size());
+    const size_t R = runs.size() - 1;
+    std::vector<int> rcs(R, 0);
+    ozl_parallel_for(R, [&](size_t g) {
+        for (size_t k = runs[g]; k < runs[g + 1] && !rcs[g]; k++) {
+            const EntReq& r = reqs[k];
+            const int m = r.huf ? 1 : 0;
+            rcs[g] = r.s->expandEntropy(r, plans[m].data() + r.first, ws[m]);
+        }
+    });
+    for (int rc : rcs)
         if (rc) return rc;
-    }
     return 0;
 }
 
@@ -783,9 +795,15 @@ struct ZL_CCtx_s {
     int formatVersion = 0;
     DevArena arena;
     hipStream_t st = nullptr;
+    hipStream_t st2 = nullptr;              // content checksums, alongside the graph work
+    hipEvent_t evIn = nullptr, evHash = nullptr;
     ~ZL_CCtx_s()
     {
+        if (st2) (void)hipStreamSynchronize(st2);
         if (st) (void)hipStreamDestroy(st);
+        if (st2) (void)hipStreamDestroy(st2);
+        if (evIn) (void)hipEventDestroy(evIn);
+        if (evHash) (void)hipEventDestroy(evHash);
     }
 };
 
@@ -899,8 +917,18 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
             return ZL_ErrorCode_parameter_invalid;
     }
     if (!c->st && hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return ZL_ErrorCode_allocation;
+    if (!c->st2 && hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess) return ZL_ErrorCode_allocation;
+    if (!c->evIn && hipEventCreateWithFlags(&c->evIn, hipEventDisableTiming) != hipSuccess) return ZL_ErrorCode_allocation;
+    if (!c->evHash && hipEventCreateWithFlags(&c->evHash, hipEventDisableTiming) != hipSuccess)
+        return ZL_ErrorCode_allocation;
     hipStream_t st = c->st;
+    // a call that failed part-way may have left checksum work running
+    OZL_HIP_CHECK(hipStreamSynchronize(c->st2));
+    OZL_HIP_CHECK(hipStreamSynchronize(st));
     c->arena.reset();
+    // every return (errors included) leaves no work of this call running:
+    // the kernels read the caller's buffers
+    OzlStreamsDone done{ st, c->st2 };
     EntBatch eb;
     std::vector<std::unique_ptr<Session>> S(n);
     std::vector<const uint8_t*> dIn(n);
@@ -908,6 +936,41 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
     for (size_t i = 0; i < n; i++) {
         const ZL_TypedRef_s* in = ins[i];
         inBytes[i] = in->n * in->width;
+        // device-resident inputs are read in place
+        if (inBytes[i] && ozl_is_device_ptr(in->ptr)) {
+            dIn[i] = (const uint8_t*)in->ptr;
+        } else {
+            uint8_t* d = c->arena.alloc(inBytes[i] + 16);
+            if (!d) return OZL_ERR_ALLOCATION;
+            if (inBytes[i]) OZL_HIP_CHECK(hipMemcpyAsync(d, in->ptr, inBytes[i], hipMemcpyHostToDevice, st));
+            dIn[i] = d;
+        }
+    }
+    // content checksums (they read only the inputs) on the second stream
+    const int cc = fp.contentChecksum, kc = fp.compressedChecksum;
+    std::vector<uint64_t> jobC(3 * n);
+    size_t sumC = 0, maxC = 0;
+    for (size_t i = 0; i < n; i++) {
+        jobC[3 * i] = (uint64_t)(uintptr_t)dIn[i];
+        jobC[3 * i + 1] = inBytes[i];
+        jobC[3 * i + 2] = sumC;
+        sumC += ozl_xxh3_batch_sums(inBytes[i]);
+        maxC = inBytes[i] > maxC ? inBytes[i] : maxC;
+    }
+    uint64_t* dJobC = (uint64_t*)c->arena.alloc(jobC.size() * 8 + 16);
+    uint64_t* dSumC = (uint64_t*)c->arena.alloc(sumC * 8 + 16);
+    uint64_t* dHashC = (uint64_t*)c->arena.alloc(n * 8 + 16);
+    if (!dJobC || !dSumC || !dHashC) return OZL_ERR_ALLOCATION;
+    if (cc) {
+        OZL_HIP_CHECK(hipMemcpyAsync(dJobC, jobC.data(), jobC.size() * 8, hipMemcpyHostToDevice, st));
+        OZL_HIP_CHECK(hipEventRecord(c->evIn, st));
+        OZL_HIP_CHECK(hipStreamWaitEvent(c->st2, c->evIn, 0));
+        int rc = ozl_xxh3_device_batch((const ozl_xjob*)dJobC, (uint32_t)n, maxC, dSumC, dHashC, c->st2);
+        if (rc) return rc;
+        OZL_HIP_CHECK(hipEventRecord(c->evHash, c->st2));
+    }
+    for (size_t i = 0; i < n; i++) {
+        const ZL_TypedRef_s* in = ins[i];
         S[i].reset(new Session());
         Session& T = *S[i];
         T.comp = c->comp;
@@ -915,15 +978,6 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
         T.st = st;
         T.minStreamSize = fp.minStreamSize;
         T.batch = &eb;
-        // device-resident inputs are read in place
-        if (inBytes[i] && ozl_is_device_ptr(in->ptr)) {
-            dIn[i] = (const uint8_t*)in->ptr;
-        } else {
-            uint8_t* d = T.alloc(inBytes[i] + 16);
-            if (T.err) return T.err;
-            if (inBytes[i]) OZL_HIP_CHECK(hipMemcpyAsync(d, in->ptr, inBytes[i], hipMemcpyHostToDevice, st));
-            dIn[i] = d;
-        }
         T.newStream(in->type, (int)in->width, in->n, (uint8_t*)dIn[i]);
         int rc = T.runSuccessor(c->comp->start ? c->comp->start : GID_STORE, 0);
         if (!rc) rc = T.err;
@@ -933,21 +987,25 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
     if (!rc) rc = eb.expand();
     if (rc) return rc;
     // headers and stored streams of every frame
-    const int cc = fp.contentChecksum, kc = fp.compressedChecksum;
     std::vector<uint8_t> heads;
     std::vector<uint64_t> gSrc, gSize, gOff;   // gather items: headers, then stored streams
     std::vector<uint64_t> fOff(n + 1), fh(n), body(n), headItem(n);
     size_t total = 0;
-    for (size_t i = 0; i < n; i++) {
-        Session& T = *S[i];
-        T.finalize();
+    std::vector<std::vector<uint8_t>> hd(n);
+    std::vector<std::vector<const uint8_t*>> hs(n);
+    std::vector<std::vector<uint64_t>> hz(n);
+    std::vector<int> hrc(n, 0);
+    ozl_parallel_for(n, [&](size_t i) {
+        S[i]->finalize();
         const int ft = frame_type(ins[i]->type);
-        std::vector<uint8_t> head;
-        std::vector<const uint8_t*> srcs;
-        std::vector<uint64_t> ssz;
-        rc = build_headers(T, ft, inBytes[i], fp, head, srcs, ssz);
-        if (rc) return rc;
-        fh[i] = ozl_write_frame_header(head.data(), ft, inBytes[i], cc, kc);
+        hrc[i] = build_headers(*S[i], ft, inBytes[i], fp, hd[i], hs[i], hz[i]);
+        if (!hrc[i]) fh[i] = ozl_write_frame_header(hd[i].data(), ft, inBytes[i], cc, kc);
+    });
+    for (size_t i = 0; i < n; i++) {
+        if (hrc[i]) return hrc[i];
+        const std::vector<uint8_t>& head = hd[i];
+        const std::vector<const uint8_t*>& srcs = hs[i];
+        const std::vector<uint64_t>& ssz = hz[i];
         fOff[i] = total;
         headItem[i] = gSrc.size();
         gSrc.push_back(heads.size());   // rebased once the headers are on the device
@@ -974,18 +1032,12 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
     uint8_t* dHead = c->arena.alloc(heads.size() + 16);
     // device metadata: gather arrays, checksum jobs and tail positions
     const size_t G = gSrc.size();
-    std::vector<uint64_t> meta(3 * G + 6 * n + 2 * n);
+    std::vector<uint64_t> meta(3 * G + 3 * n + 2 * n);
     size_t sumWords = 0, maxLen = 0;
-    uint64_t* mJobC = meta.data() + 3 * G;
-    uint64_t* mJobK = mJobC + 3 * n;
+    uint64_t* mJobK = meta.data() + 3 * G;
     uint64_t* mAtC = mJobK + 3 * n;
     uint64_t* mAtK = mAtC + n;
     for (size_t i = 0; i < n; i++) {
-        mJobC[3 * i] = (uint64_t)(uintptr_t)dIn[i];
-        mJobC[3 * i + 1] = inBytes[i];
-        mJobC[3 * i + 2] = sumWords;
-        sumWords += ozl_xxh3_batch_sums(inBytes[i]);
-        maxLen = inBytes[i] > maxLen ? inBytes[i] : maxLen;
         const uint64_t k0 = fOff[i] + fh[i], k1 = body[i] + (cc ? 4 : 0);
         mJobK[3 * i] = 0;   // rebased below (frame buffer address)
         mJobK[3 * i + 1] = k1 - k0;
@@ -1013,13 +1065,12 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
     OZL_HIP_CHECK(hipMemcpyAsync(dMeta, meta.data(), meta.size() * 8, hipMemcpyHostToDevice, st));
     rc = ozl_gather_sized((const uint8_t* const*)dMeta, dMeta + G, dMeta + 2 * G, (int)G, maxItem, dF, st);
     if (rc) return rc;
-    const ozl_xjob* dJobC = (const ozl_xjob*)(dMeta + 3 * G);
-    const ozl_xjob* dJobK = dJobC + n;
-    const uint64_t* dAtC = dMeta + 3 * G + 6 * n;
+    const ozl_xjob* dJobK = (const ozl_xjob*)(dMeta + 3 * G);
+    const uint64_t* dAtC = dMeta + 3 * G + 3 * n;
     const uint64_t* dAtK = dAtC + n;
     if (cc) {
-        rc = ozl_xxh3_device_batch(dJobC, (uint32_t)n, maxLen, dSums, dHash, st);
-        if (!rc) rc = ozl_put_tails(dF, dAtC, dHash, (uint32_t)n, !kc, st);
+        OZL_HIP_CHECK(hipStreamWaitEvent(st, c->evHash, 0));
+        rc = ozl_put_tails(dF, dAtC, dHashC, (uint32_t)n, !kc, st);
         if (rc) return rc;
     }
     if (kc) {

@@ -19,3 +19,37 @@ int ozl_put_tails(uint8_t* dst, const uint64_t* at, const uint64_t* hashes, uint
 int ozl_gather_sized(const uint8_t* const* srcs, const uint64_t* sizes, const uint64_t* dstOffs, int count,
                      size_t maxSize, uint8_t* dst, hipStream_t st);
 bool ozl_is_device_ptr(const void* p);
+
+#include <algorithm>
+#include <atomic>
+#include <thread>
+#include <vector>
+
+// f(0) .. f(n-1) on up to eight host threads (pure host work: no HIP calls).
+template <class F>
+void ozl_parallel_for(size_t n, F f)
+{
+    const unsigned hc = std::thread::hardware_concurrency();
+    const size_t T = std::min<size_t>(n, std::min<size_t>(8, hc ? hc : 1));
+    if (T <= 1 || n < 4) {
+        for (size_t i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<size_t> next(0);
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < T; t++)
+        th.emplace_back([&]() {
+            for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+        });
+    for (auto& x : th) x.join();
+}
+
+// Waits for two streams when it goes out of scope.
+struct OzlStreamsDone {
+    hipStream_t a, b;
+    ~OzlStreamsDone()
+    {
+        if (a) (void)hipStreamSynchronize(a);
+        if (b) (void)hipStreamSynchronize(b);
+    }
+};

@@ -237,6 +237,44 @@ __global__ void __launch_bounds__(kThreads) k_transpose_join(const uint8_t* __re
         for (int p = 0; p < W; p++) dst[i * W + p] = planes[(size_t)p * n + i];
 }
 
+// the same from W planes at their own addresses (16-byte aligned)
+struct PlanePtrs {
+    const uint8_t* p[8];
+};
+template <int W>
+__global__ void __launch_bounds__(kThreads) k_transpose_join_ptrs(PlanePtrs pp, size_t n, uint8_t* __restrict__ dst)
+{
+    size_t nVec = n / 16;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nVec; v += stride) {
+        union { uint4 q[W]; uint8_t b[16 * W]; } out;
+#pragma unroll
+        for (int p = 0; p < W; p++) {
+            union { uint4 q; uint8_t b[16]; } in;
+            in.q = *reinterpret_cast<const uint4*>(pp.p[p] + v * 16);
+#pragma unroll
+            for (int e = 0; e < 16; e++) out.b[e * W + p] = in.b[e];
+        }
+#pragma unroll
+        for (int k = 0; k < W; k++) reinterpret_cast<uint4*>(dst + v * 16 * W)[k] = out.q[k];
+    }
+    for (size_t i = nVec * 16 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+#pragma unroll
+        for (int p = 0; p < W; p++) dst[i * W + p] = pp.p[p][i];
+}
+
+// planes at any address, any W <= 8: byte per thread
+__global__ void __launch_bounds__(kThreads) k_transpose_join_ptrs_any(PlanePtrs pp, size_t n, int w,
+                                                                      uint8_t* __restrict__ dst)
+{
+    size_t total = n * (size_t)w;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += stride) {
+        size_t i = j / w, p = j - i * w;
+        dst[j] = pp.p[p][i];
+    }
+}
+
 // generic element width (any W >= 1): byte per thread
 __global__ void __launch_bounds__(kThreads) k_transpose_split_any(const uint8_t* __restrict__ src, size_t n, int w,
                                                                   uint8_t* __restrict__ planes)
@@ -492,6 +530,37 @@ int ozl_transpose_join(const void* planes, size_t n, int width, void* dst, void*
     default:
         hipLaunchKernelGGL(k_transpose_join_any, dim3(ozl_blocks(n * width, kThreads, 8192)), dim3(kThreads), 0, st,
                            s, n, width, d);
+    }
+    OZL_HIP_CHECK(hipGetLastError());
+    return OZL_OK;
+}
+
+// transpose_split decode from `width` (at most 8) planes of n bytes each at
+// the device addresses planes[0..width) (a host array), into dst.
+int ozl_transpose_join_planes(const void* const* planes, size_t n, int width, void* dst, void* stream)
+{
+    if (width <= 0 || width > 8) return OZL_ERR_PARAMETER_INVALID;
+    if (n == 0) return OZL_OK;
+    hipStream_t st = (hipStream_t)stream;
+    PlanePtrs pp = {};
+    bool aligned = true;
+    for (int p = 0; p < width; p++) {
+        pp.p[p] = (const uint8_t*)planes[p];
+        aligned = aligned && ((uintptr_t)planes[p] & 15) == 0;
+    }
+    unsigned g = ozl_blocks(n / 16 + 1, kThreads, 8192);
+    uint8_t* d = (uint8_t*)dst;
+    if (width == 1) {
+        OZL_HIP_CHECK(hipMemcpyAsync(dst, planes[0], n, hipMemcpyDeviceToDevice, st));
+    } else if (aligned && width == 2) {
+        hipLaunchKernelGGL(k_transpose_join_ptrs<2>, dim3(g), dim3(kThreads), 0, st, pp, n, d);
+    } else if (aligned && width == 4) {
+        hipLaunchKernelGGL(k_transpose_join_ptrs<4>, dim3(g), dim3(kThreads), 0, st, pp, n, d);
+    } else if (aligned && width == 8) {
+        hipLaunchKernelGGL(k_transpose_join_ptrs<8>, dim3(g), dim3(kThreads), 0, st, pp, n, d);
+    } else {
+        hipLaunchKernelGGL(k_transpose_join_ptrs_any, dim3(ozl_blocks(n * width, kThreads, 8192)), dim3(kThreads), 0,
+                           st, pp, n, width, d);
     }
     OZL_HIP_CHECK(hipGetLastError());
     return OZL_OK;
