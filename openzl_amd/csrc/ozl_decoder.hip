@@ -9,6 +9,7 @@
 // into one launch each until a later decoder needs their output.
 #include <string.h>
 
+#include <algorithm>
 #include <memory>
 #include <vector>
 
@@ -141,6 +142,7 @@ struct DStream {
     bool ready = false;
     bool pending = false;
     bool ncount = false;   // still the NCount header (dev, bytes): the FSE item reads it
+    bool waiting = false;  // its decoder waits for a queued launch (this round)
 };
 
 struct DNode {
@@ -168,6 +170,7 @@ struct ZL_DCtx_s {
     hipStream_t st = nullptr;
     hipStream_t st2 = nullptr;   // compressed checksums, alongside the decoding
     hipEvent_t evIn = nullptr, evHash = nullptr;
+    OzlHostStage stage;          // pinned staging of the uploads
     uint8_t* base = nullptr;
     size_t cap = 0, used = 0;
     std::vector<uint8_t*> old;
@@ -222,7 +225,7 @@ struct DQueue {
     std::vector<ozl_ditem> items;
     std::vector<FillItem> fills;
     std::vector<std::pair<int*, uint32_t>> statuses;
-    std::vector<std::vector<uint8_t>> keep;   // host bytes asynchronous copies read
+    std::vector<std::vector<uint8_t>> keep;   // host bytes asynchronous copies read (when pinned staging fails)
     std::vector<Decoder*> members;
     std::vector<uint64_t> gSrc, gSize, gDst;  // queued copies (device addresses)
     int err = 0;
@@ -235,6 +238,8 @@ struct DQueue {
     }
     uint8_t* host_keep(const void* p, size_t n)
     {
+        uint8_t* h = d->stage.put(p, n);
+        if (h) return h;
         keep.emplace_back((const uint8_t*)p, (const uint8_t*)p + n);
         return keep.back().data();
     }
@@ -271,6 +276,16 @@ struct Decoder {
     }
     uint8_t* host_keep(const void* p, size_t n) { return q->host_keep(p, n); }
 
+    // the frame's content (stream 0) goes straight to the caller's device
+    // buffer when it has the size the frame header announces
+    uint8_t* rootDst = nullptr;
+    size_t rootSize = 0;
+    uint8_t* out_alloc(const DNode& nd, size_t n)
+    {
+        if (nd.in == 0 && rootDst && n == rootSize) return rootDst;
+        return alloc(n);
+    }
+
     // copies of device ranges into dst at the given offsets, queued: they go
     // out with the other queued work (one launch for all queued copies)
     void gather(const std::vector<const uint8_t*>& srcs, const std::vector<uint64_t>& sizes,
@@ -289,7 +304,7 @@ struct Decoder {
     {
         for (uint32_t k = 0; k < nd.nbOut; k++) {
             const DStream& s = S[nd.firstOut + k];
-            if (s.pending) return kBlocked;
+            if (s.pending || s.waiting) return kBlocked;
             if (!s.ready) return OZL_ERR_CORRUPTION;
         }
         return 0;
@@ -326,7 +341,7 @@ struct Decoder {
             if (n > 0xFFFFFFFFull) return OZL_ERR_CORRUPTION;
             const DStream& side = S[o0];
             const DStream& bits = S[o0 + 1];
-            in.dev = alloc(n);
+            in.dev = out_alloc(nd, n);
             if (err) return err;
             in.bytes = n;
             in.type = T_SERIAL_;
@@ -356,6 +371,7 @@ struct Decoder {
             // byte-identical: the regenerated stream aliases the decoded one,
             // pending or not (no wait for the batch that produces it)
             const DStream& s = S[o0];
+            if (s.waiting) return kBlocked;
             if (!s.ready && !s.pending) return OZL_ERR_CORRUPTION;
             if (s.ncount) return OZL_ERR_CORRUPTION;
             int type = T_NUMERIC_, width = s.width;
@@ -388,7 +404,7 @@ struct Decoder {
             size_t m = dl.bytes / w;
             size_t n = nd.hdr.empty() ? 0 : m + 1;
             if (!nd.hdr.empty() && nd.hdr.size() != (size_t)w) return OZL_ERR_CORRUPTION;
-            in.dev = alloc(n * w);
+            in.dev = out_alloc(nd, n * w);
             uint8_t* first = alloc(8);
             uint8_t* scratch = alloc(ozl_delta_decode_scratch_size(n, w));
             if (err) return err;
@@ -405,7 +421,7 @@ struct Decoder {
         case OZL_TR_ZIGZAG: {
             const DStream& z = S[o0];
             if (z.type != T_NUMERIC_) return OZL_ERR_CORRUPTION;
-            in.dev = alloc(z.bytes);
+            in.dev = out_alloc(nd, z.bytes);
             if (err) return err;
             rc = ozl_zigzag_decode(z.dev, z.bytes / z.width, z.width, in.dev, st);
             if (rc) return rc;
@@ -420,7 +436,7 @@ struct Decoder {
             size_t n = S[o0].bytes;
             for (uint32_t p = 1; p < W; p++)
                 if (S[o0 + p].bytes != n) return OZL_ERR_CORRUPTION;  // decode_transpose_binding.c:54-57
-            in.dev = alloc(n * W);
+            in.dev = out_alloc(nd, n * W);
             if (err) return err;
             if (n && W <= 8) {   // straight from the planes
                 const void* pl[8];
@@ -460,7 +476,7 @@ struct Decoder {
             if (extra > maxElts) return OZL_ERR_CORRUPTION;
             size_t n = maxElts - extra;
             if ((n * nb + 7) / 8 != s.bytes) return OZL_ERR_CORRUPTION;
-            in.dev = alloc(n * w);
+            in.dev = out_alloc(nd, n * w);
             if (err) return err;
             if (n) {
                 rc = ozl_bitpack_decode(s.dev, n, w, nb, in.dev, st);
@@ -485,7 +501,7 @@ struct Decoder {
                 if (last == 0 || nb == 0) return OZL_ERR_CORRUPTION;
                 n = ((pk.bytes - 1) * 8 + ozl_hb32(last)) / nb;
             }
-            in.dev = alloc(n);
+            in.dev = out_alloc(nd, n);
             if (err) return err;
             if (n)
                 OZL_LAUNCH("k_flatpack_decode", k_flatpack_decode, dim3(ozl_blocks(n, kT, 4096)), dim3(kT), 0, st, a.dev,
@@ -500,7 +516,7 @@ struct Decoder {
             uint64_t n = 0;
             for (size_t k = 0; k < nd.hdr.size() && k < 10; k++) n |= (uint64_t)(nd.hdr[k] & 127) << (7 * k);
             if (v.bytes != 1) return OZL_ERR_CORRUPTION;
-            in.dev = alloc(n);
+            in.dev = out_alloc(nd, n);
             if (err) return err;
             in.bytes = n;
             in.type = T_SERIAL_;
@@ -517,7 +533,7 @@ struct Decoder {
         case OZL_TR_SPLITN: {  // decode_splitN_binding.c:16 -- concatenation
             size_t total = 0;
             for (uint32_t k = 0; k < nd.nbVO; k++) total += S[o0 + k].bytes;
-            in.dev = alloc(total);
+            in.dev = out_alloc(nd, total);
             if (err) return err;
             std::vector<const uint8_t*> srcs;
             std::vector<uint64_t> sz, offs;
@@ -589,7 +605,12 @@ int DQueue::flush()
     }
     if (!items.empty()) {
         // launched in stream order without waiting: later nodes that read
-        // these outputs are enqueued after them
+        // these outputs are enqueued after them. Items of one coder sit
+        // together (each coder's kernel packs consecutive items into a
+        // workgroup), longest first, so the long state chains start first.
+        std::stable_sort(items.begin(), items.end(), [](const ozl_ditem& a, const ozl_ditem& b) {
+            return a.huf != b.huf ? a.huf < b.huf : a.n > b.n;
+        });
         const uint32_t B = (uint32_t)items.size();
         ozl_ditem* dItems = (ozl_ditem*)alloc(B * sizeof(ozl_ditem));
         int* dStatus = (int*)alloc(B * 4);
@@ -722,7 +743,7 @@ struct FrameJob {
     ParsedFrame P;
     uint8_t* dFrame = nullptr;
     std::unique_ptr<Decoder> D;
-    int64_t next = 0;      // next node to decode (frame order: last node first)
+    std::vector<uint32_t> todo;   // nodes still to decode, in decode order (last node first)
     uint8_t tail[9];       // checksums and end byte
     uint64_t tailOff = 0;
     int wantType = 0;
@@ -745,6 +766,7 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
     OZL_HIP_CHECK(hipStreamSynchronize(dc->st2));
     OZL_HIP_CHECK(hipStreamSynchronize(st));
     dc->reset();
+    dc->stage.reset();
     // every return (errors included) leaves no work of this call running:
     // the kernels read the caller's buffers
     OzlStreamsDone done{ st, dc->st2 };
@@ -772,7 +794,9 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         uint64_t* dm = (uint64_t*)dc->alloc(m.size() * 8 + 16);
         if (!dPre || !dm) return OZL_ERR_ALLOCATION;
         pre.resize(n * kPre);
-        OZL_HIP_CHECK(hipMemcpyAsync(dm, m.data(), m.size() * 8, hipMemcpyHostToDevice, st));
+        const uint8_t* hm = dc->stage.put(m.data(), m.size() * 8);
+        if (!hm) return OZL_ERR_ALLOCATION;
+        OZL_HIP_CHECK(hipMemcpyAsync(dm, hm, m.size() * 8, hipMemcpyHostToDevice, st));
         int rc = ozl_gather_sized((const uint8_t* const*)dm, dm + n, dm + 2 * n, (int)n, kPre, dPre, st);
         if (rc) return rc;
         OZL_HIP_CHECK(hipMemcpyAsync(pre.data(), dPre, n * kPre, hipMemcpyDeviceToHost, st));
@@ -838,7 +862,7 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
             if (nd.trid == OZL_TR_FSE_V2 && D.S[nd.firstOut].ready) { D.S[nd.firstOut].type = ZL_Type_numeric; D.S[nd.firstOut].width = 2; }
             if (nd.trid == OZL_TR_HUFFMAN_V2 && D.S[nd.firstOut].ready) { D.S[nd.firstOut].type = ZL_Type_numeric; }
         }
-        j.next = (int64_t)P.nodes.size() - 1;
+        for (size_t k = P.nodes.size(); k-- > 0;) j.todo.push_back((uint32_t)k);
         // checksums and end byte (device frames: gathered below)
         j.tailOff = P.csumPos;
         if (!j.dev) memcpy(j.tail, j.src + P.csumPos, P.end + 1 - P.csumPos);
@@ -879,25 +903,52 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         uint64_t* dSums = (uint64_t*)dc->alloc(kSum * 8 + 16);
         dHK = (uint64_t*)dc->alloc(kIdx.size() * 8 + 16);
         if (!dJobs || !dSums || !dHK) return OZL_ERR_ALLOCATION;
-        OZL_HIP_CHECK(hipMemcpyAsync(dJobs, kJobs.data(), kJobs.size() * 8, hipMemcpyHostToDevice, st));
+        OZL_HIP_CHECK(hipMemcpyAsync(dJobs, Q.host_keep(kJobs.data(), kJobs.size() * 8), kJobs.size() * 8,
+                                     hipMemcpyHostToDevice, st));
         OZL_HIP_CHECK(hipEventRecord(dc->evIn, st));
         OZL_HIP_CHECK(hipStreamWaitEvent(dc->st2, dc->evIn, 0));
         int rc = ozl_xxh3_device_batch((const ozl_xjob*)dJobs, (uint32_t)kIdx.size(), kMax, dSums, dHK, dc->st2);
         if (rc) return rc;
         OZL_HIP_CHECK(hipEventRecord(dc->evHash, dc->st2));
     }
-    // the decoders in lockstep
+    // device destination: a frame's content whose place there is 16-byte
+    // aligned (as the library's own buffers are) is written in place
+    {
+        size_t t = 0;
+        for (const FrameJob& j : J) t += j.P.size;
+        if (t && t <= cap && ozl_is_device_ptr(dst)) {
+            size_t o = 0;
+            for (FrameJob& j : J) {
+                if ((((uintptr_t)dst + o) & 15) == 0) {
+                    j.D->rootDst = (uint8_t*)dst + o;
+                    j.D->rootSize = j.P.size;
+                }
+                o += j.P.size;
+            }
+        }
+    }
+    // the decoders in lockstep: every node runs as soon as the streams it
+    // reads are ready or queued; a node that waits for a queued launch (or
+    // for such a node) is retried once the queue has gone out
     for (;;) {
         bool left = false;
         for (FrameJob& j : J) {
-            while (j.next >= 0) {
-                int rc = j.D->decode(j.P.nodes[(size_t)j.next]);
-                if (rc == kBlocked) break;
+            Decoder& D = *j.D;
+            std::vector<uint32_t> rest;
+            for (uint32_t k : j.todo) {
+                const DNode& nd = j.P.nodes[k];
+                int rc = D.decode(nd);
+                if (rc == kBlocked) {
+                    D.S[nd.in].waiting = true;
+                    rest.push_back(k);
+                    continue;
+                }
                 if (rc) return rc;
-                if (j.D->err) return j.D->err;
-                j.next--;
+                if (D.err) return D.err;
             }
-            if (j.next >= 0) left = true;
+            for (uint32_t k : rest) D.S[j.P.nodes[k].in].waiting = false;
+            j.todo.swap(rest);
+            if (!j.todo.empty()) left = true;
         }
         int rc = Q.flush();
         if (!rc) rc = Q.err;
@@ -937,7 +988,8 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         uint64_t* dSums = (uint64_t*)dc->alloc(cSum * 8 + 16);
         uint64_t* dHC = (uint64_t*)dc->alloc(cIdx.size() * 8 + 16);
         if (!dJobs || !dSums || !dHC) return OZL_ERR_ALLOCATION;
-        OZL_HIP_CHECK(hipMemcpyAsync(dJobs, cJobs.data(), cJobs.size() * 8, hipMemcpyHostToDevice, st));
+        OZL_HIP_CHECK(hipMemcpyAsync(dJobs, Q.host_keep(cJobs.data(), cJobs.size() * 8), cJobs.size() * 8,
+                                     hipMemcpyHostToDevice, st));
         int rc = ozl_xxh3_device_batch((const ozl_xjob*)dJobs, (uint32_t)cIdx.size(), cMax, dSums, dHC, st);
         if (rc) return rc;
         OZL_HIP_CHECK(hipMemcpyAsync(hC.data(), dHC, cIdx.size() * 8, hipMemcpyDeviceToHost, st));
@@ -949,7 +1001,7 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
     const bool devOut = total && ozl_is_device_ptr(dst);
     for (size_t i = 0; i < n; i++) {
         const DStream& out = J[i].D->S[0];
-        if (!out.bytes) continue;
+        if (!out.bytes || out.dev == (uint8_t*)dst + outOff[i]) continue;
         OZL_HIP_CHECK(hipMemcpyAsync((uint8_t*)dst + outOff[i], out.dev, out.bytes,
                                      devOut ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     }

@@ -169,7 +169,7 @@ struct EntBatch {
     ozl_ews ws[2];
     std::vector<ozl_eplan> plans[2];
     void add(EntReq&& r);
-    int run(DevArena* arena, hipStream_t st);
+    int run(DevArena* arena, OzlHostStage* stage, hipStream_t st, hipStream_t alt = nullptr, hipEvent_t fork = nullptr);
     int expand();
 };
 
@@ -424,7 +424,7 @@ struct Session {
         }
         EntBatch local;
         local.add(std::move(r));
-        int rc = local.run(arena, st);
+        int rc = local.run(arena, nullptr, st);
         return rc ? rc : local.expand();
     }
 
@@ -641,28 +641,76 @@ struct Session {
         return 0;
     }
 
+    // Order of the items by key: the creation order is a few ascending runs
+    // (the walk, then each deferred call's items), merged here.
+    template <class Item>
+    static std::vector<uint32_t> key_order(const std::vector<Item>& v)
+    {
+        const uint32_t n = (uint32_t)v.size();
+        std::vector<std::pair<uint32_t, uint32_t>> runs;
+        for (uint32_t i = 0; i < n;) {
+            uint32_t j = i + 1;
+            while (j < n && v[j].key > v[j - 1].key) j++;
+            runs.push_back({ i, j });
+            i = j;
+        }
+        std::vector<uint32_t> ord;
+        ord.reserve(n);
+        if (runs.size() <= 1) {
+            for (uint32_t i = 0; i < n; i++) ord.push_back(i);
+        } else if (runs.size() > 64) {
+            for (uint32_t i = 0; i < n; i++) ord.push_back(i);
+            std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return v[x].key < v[y].key; });
+        } else {
+            for (;;) {
+                int best = -1;
+                for (size_t r = 0; r < runs.size(); r++)
+                    if (runs[r].first < runs[r].second &&
+                        (best < 0 || v[runs[r].first].key < v[runs[(size_t)best].first].key))
+                        best = (int)r;
+                if (best < 0) break;
+                auto& rr = runs[(size_t)best];
+                // the whole stretch of this run below every other run's head
+                uint64_t lim = ~0ull;
+                for (size_t r = 0; r < runs.size(); r++)
+                    if ((int)r != best && runs[r].first < runs[r].second && v[runs[r].first].key < lim)
+                        lim = v[runs[r].first].key;
+                while (rr.first < rr.second && v[rr.first].key < lim) ord.push_back(rr.first++);
+            }
+        }
+        return ord;
+    }
+
     // Puts streams and nodes in creation-key order (the order of an engine
     // without deferred calls) and renumbers the nodes' stream references.
     void finalize()
     {
         const size_t ns = streams.size();
-        std::vector<uint32_t> ord(ns);
-        std::iota(ord.begin(), ord.end(), 0u);
-        std::stable_sort(ord.begin(), ord.end(),
-                         [&](uint32_t x, uint32_t y) { return streams[x].key < streams[y].key; });
-        std::vector<uint32_t> remap(ns + 1);
-        std::vector<EStream> ss(ns);
-        for (size_t i = 0; i < ns; i++) {
-            remap[ord[i]] = (uint32_t)i;
-            ss[i] = streams[ord[i]];
+        const std::vector<uint32_t> ord = key_order(streams);
+        bool ident = true;
+        for (size_t i = 0; i < ns && ident; i++) ident = ord[i] == i;
+        if (!ident) {
+            std::vector<uint32_t> remap(ns + 1);
+            std::vector<EStream> ss(ns);
+            for (size_t i = 0; i < ns; i++) {
+                remap[ord[i]] = (uint32_t)i;
+                ss[i] = streams[ord[i]];
+            }
+            remap[ns] = (uint32_t)ns;
+            for (ENode& nd : nodes) {
+                nd.in = remap[nd.in];
+                nd.firstOut = remap[nd.firstOut];
+            }
+            streams.swap(ss);
         }
-        remap[ns] = (uint32_t)ns;
-        for (ENode& nd : nodes) {
-            nd.in = remap[nd.in];
-            nd.firstOut = remap[nd.firstOut];
+        const std::vector<uint32_t> nord = key_order(nodes);
+        bool nident = true;
+        for (size_t i = 0; i < nord.size() && nident; i++) nident = nord[i] == i;
+        if (!nident) {
+            std::vector<ENode> nn(nodes.size());
+            for (size_t i = 0; i < nord.size(); i++) nn[i] = nodes[nord[i]];
+            nodes.swap(nn);
         }
-        std::stable_sort(nodes.begin(), nodes.end(), [](const ENode& x, const ENode& y) { return x.key < y.key; });
-        streams.swap(ss);
     }
 
     // Exhaustive trial (selector_brute_force.c:9-50): every candidate compresses
@@ -684,9 +732,17 @@ void EntBatch::add(EntReq&& r)
     reqs.push_back(std::move(r));
 }
 
-int EntBatch::run(DevArena* arena, hipStream_t st)
+// The FSE and the Huffman chunks go out on two streams (alt after the work
+// queued on st so far), so the two coders' launches overlap.
+int EntBatch::run(DevArena* arena, OzlHostStage* stage, hipStream_t st, hipStream_t alt, hipEvent_t fork)
 {
-    std::vector<uint64_t> offs[2];   // read by the launches: kept until the wait below
+    hipStream_t sm[2] = { st, st };
+    if (alt && fork && !beg[0].empty() && !beg[1].empty()) {
+        OZL_HIP_CHECK(hipEventRecord(fork, st));
+        OZL_HIP_CHECK(hipStreamWaitEvent(alt, fork, 0));
+        sm[1] = alt;
+    }
+    std::vector<uint64_t> offs[2];   // read by the launches: kept until the waits below
     for (int m = 0; m < 2; m++) {
         const uint32_t B = (uint32_t)beg[m].size();
         if (!B) continue;
@@ -703,14 +759,22 @@ int EntBatch::run(DevArena* arena, hipStream_t st)
         uint8_t* wsMem = arena->alloc(wsBytes);
         uint64_t* dO = (uint64_t*)arena->alloc(o.size() * 8);
         if (!wsMem || !dO) return OZL_ERR_ALLOCATION;
-        OZL_HIP_CHECK(hipMemcpyAsync(dO, o.data(), o.size() * 8, hipMemcpyHostToDevice, st));
+        const uint8_t* h = stage ? stage->put(o.data(), o.size() * 8) : nullptr;
+        OZL_HIP_CHECK(hipMemcpyAsync(dO, h ? (const void*)h : (const void*)o.data(), o.size() * 8, hipMemcpyHostToDevice,
+                                     sm[m]));
         ws[m] = ozl_ews_carve(wsMem, B, maxBlock[m]);
-        int rc = ozl_entropy_encode_chunks(m, (const uint8_t*)(uintptr_t)lo, dO, dO + B, B, maxBlock[m], ws[m], st);
+        int rc = ozl_entropy_encode_chunks(m, (const uint8_t*)(uintptr_t)lo, dO, dO + B, B, maxBlock[m], ws[m], sm[m]);
         if (rc) return rc;
+    }
+    for (int m = 0; m < 2; m++) {
+        const uint32_t B = (uint32_t)beg[m].size();
+        if (!B) continue;
         plans[m].resize(B);
-        OZL_HIP_CHECK(hipMemcpyAsync(plans[m].data(), ws[m].plans, (size_t)B * sizeof(ozl_eplan), hipMemcpyDeviceToHost, st));
+        OZL_HIP_CHECK(hipMemcpyAsync(plans[m].data(), ws[m].plans, (size_t)B * sizeof(ozl_eplan), hipMemcpyDeviceToHost,
+                                     sm[m]));
     }
     OZL_HIP_CHECK(hipStreamSynchronize(st));
+    if (sm[1] != st) OZL_HIP_CHECK(hipStreamSynchronize(sm[1]));
     for (int m = 0; m < 2; m++)
         for (const ozl_eplan& p : plans[m])
             if (p.status) return (int)p.status;
@@ -796,12 +860,17 @@ struct ZL_CCtx_s {
     DevArena arena;
     hipStream_t st = nullptr;
     hipStream_t st2 = nullptr;              // content checksums, alongside the graph work
-    hipEvent_t evIn = nullptr, evHash = nullptr;
+    hipStream_t st3 = nullptr;              // Huffman chunks, alongside the FSE chunks
+    hipEvent_t evIn = nullptr, evHash = nullptr, evFork = nullptr;
+    OzlHostStage stage;                     // pinned staging of the uploads
     ~ZL_CCtx_s()
     {
         if (st2) (void)hipStreamSynchronize(st2);
+        if (st3) (void)hipStreamSynchronize(st3);
         if (st) (void)hipStreamDestroy(st);
         if (st2) (void)hipStreamDestroy(st2);
+        if (st3) (void)hipStreamDestroy(st3);
+        if (evFork) (void)hipEventDestroy(evFork);
         if (evIn) (void)hipEventDestroy(evIn);
         if (evHash) (void)hipEventDestroy(evHash);
     }
@@ -918,17 +987,22 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
     }
     if (!c->st && hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return ZL_ErrorCode_allocation;
     if (!c->st2 && hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess) return ZL_ErrorCode_allocation;
+    if (!c->st3 && hipStreamCreateWithFlags(&c->st3, hipStreamNonBlocking) != hipSuccess) return ZL_ErrorCode_allocation;
     if (!c->evIn && hipEventCreateWithFlags(&c->evIn, hipEventDisableTiming) != hipSuccess) return ZL_ErrorCode_allocation;
+    if (!c->evFork && hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess)
+        return ZL_ErrorCode_allocation;
     if (!c->evHash && hipEventCreateWithFlags(&c->evHash, hipEventDisableTiming) != hipSuccess)
         return ZL_ErrorCode_allocation;
     hipStream_t st = c->st;
-    // a call that failed part-way may have left checksum work running
+    // a call that failed part-way may have left work running
     OZL_HIP_CHECK(hipStreamSynchronize(c->st2));
+    OZL_HIP_CHECK(hipStreamSynchronize(c->st3));
     OZL_HIP_CHECK(hipStreamSynchronize(st));
     c->arena.reset();
+    c->stage.reset();
     // every return (errors included) leaves no work of this call running:
     // the kernels read the caller's buffers
-    OzlStreamsDone done{ st, c->st2 };
+    OzlStreamsDone done{ st, c->st2, c->st3 };
     EntBatch eb;
     std::vector<std::unique_ptr<Session>> S(n);
     std::vector<const uint8_t*> dIn(n);
@@ -962,7 +1036,9 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
     uint64_t* dHashC = (uint64_t*)c->arena.alloc(n * 8 + 16);
     if (!dJobC || !dSumC || !dHashC) return OZL_ERR_ALLOCATION;
     if (cc) {
-        OZL_HIP_CHECK(hipMemcpyAsync(dJobC, jobC.data(), jobC.size() * 8, hipMemcpyHostToDevice, st));
+        const uint8_t* h = c->stage.put(jobC.data(), jobC.size() * 8);
+        if (!h) return OZL_ERR_ALLOCATION;
+        OZL_HIP_CHECK(hipMemcpyAsync(dJobC, h, jobC.size() * 8, hipMemcpyHostToDevice, st));
         OZL_HIP_CHECK(hipEventRecord(c->evIn, st));
         OZL_HIP_CHECK(hipStreamWaitEvent(c->st2, c->evIn, 0));
         int rc = ozl_xxh3_device_batch((const ozl_xjob*)dJobC, (uint32_t)n, maxC, dSumC, dHashC, c->st2);
@@ -983,7 +1059,7 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
         if (!rc) rc = T.err;
         if (rc) return rc;
     }
-    int rc = eb.run(&c->arena, st);
+    int rc = eb.run(&c->arena, &c->stage, st);
     if (!rc) rc = eb.expand();
     if (rc) return rc;
     // headers and stored streams of every frame
@@ -1061,8 +1137,11 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
     // header items: offsets into `heads` become device addresses
     for (size_t i = 0; i < n; i++) meta[headItem[i]] = (uint64_t)(uintptr_t)dHead + gSrc[headItem[i]];
     for (size_t i = 0; i < n; i++) mJobK[3 * i] = (uint64_t)(uintptr_t)(dF + fOff[i] + fh[i]);
-    if (heads.size()) OZL_HIP_CHECK(hipMemcpyAsync(dHead, heads.data(), heads.size(), hipMemcpyHostToDevice, st));
-    OZL_HIP_CHECK(hipMemcpyAsync(dMeta, meta.data(), meta.size() * 8, hipMemcpyHostToDevice, st));
+    const uint8_t* hHead = c->stage.put(heads.data(), heads.size());
+    const uint8_t* hMeta = c->stage.put(meta.data(), meta.size() * 8);
+    if (!hHead || !hMeta) return OZL_ERR_ALLOCATION;
+    if (heads.size()) OZL_HIP_CHECK(hipMemcpyAsync(dHead, hHead, heads.size(), hipMemcpyHostToDevice, st));
+    OZL_HIP_CHECK(hipMemcpyAsync(dMeta, hMeta, meta.size() * 8, hipMemcpyHostToDevice, st));
     rc = ozl_gather_sized((const uint8_t* const*)dMeta, dMeta + G, dMeta + 2 * G, (int)G, maxItem, dF, st);
     if (rc) return rc;
     const ozl_xjob* dJobK = (const ozl_xjob*)(dMeta + 3 * G);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
+#include <string.h>
 
 // one XXH3 job of a batch: ptr/len of the buffer, its block sums' offset (words)
 struct ozl_xjob {
@@ -44,12 +45,52 @@ void ozl_parallel_for(size_t n, F f)
     for (auto& x : th) x.join();
 }
 
-// Waits for two streams when it goes out of scope.
+// Waits for up to three streams when it goes out of scope.
 struct OzlStreamsDone {
-    hipStream_t a, b;
+    hipStream_t a, b, c = nullptr;
     ~OzlStreamsDone()
     {
         if (a) (void)hipStreamSynchronize(a);
         if (b) (void)hipStreamSynchronize(b);
+        if (c) (void)hipStreamSynchronize(c);
+    }
+};
+
+// Pinned host staging for the small host-to-device uploads of one call
+// (metadata, headers): hipMemcpyAsync from pageable memory waits for the
+// stream, from pinned memory it does not. Blocks are kept across calls;
+// reset() once the call's streams have drained.
+struct OzlHostStage {
+    std::vector<std::pair<uint8_t*, size_t>> blocks;
+    size_t cur = 0, used = 0;
+    ~OzlHostStage()
+    {
+        for (auto& b : blocks) (void)hipHostFree(b.first);
+    }
+    void reset()
+    {
+        cur = 0;
+        used = 0;
+    }
+    // a pinned copy of p[0, n) that stays valid until reset(); null on failure
+    uint8_t* put(const void* p, size_t n)
+    {
+        const size_t need = (n + 63) & ~(size_t)63;
+        while (cur < blocks.size() && used + need > blocks[cur].second) {
+            cur++;
+            used = 0;
+        }
+        if (cur == blocks.size()) {
+            size_t sz = blocks.empty() ? ((size_t)4 << 20) : blocks.back().second * 2;
+            if (sz < need) sz = need;
+            void* b = nullptr;
+            if (hipHostMalloc(&b, sz, hipHostMallocDefault) != hipSuccess) return nullptr;
+            blocks.push_back({ (uint8_t*)b, sz });
+            used = 0;
+        }
+        uint8_t* d = blocks[cur].first + used;
+        used += need;
+        if (n) memcpy(d, p, n);
+        return d;
     }
 };

@@ -548,6 +548,7 @@ int ozl_transpose_join_planes(const void* const* planes, size_t n, int width, vo
         pp.p[p] = (const uint8_t*)planes[p];
         aligned = aligned && ((uintptr_t)planes[p] & 15) == 0;
     }
+    aligned = aligned && ((uintptr_t)dst & 15) == 0;
     unsigned g = ozl_blocks(n / 16 + 1, kThreads, 8192);
     uint8_t* d = (uint8_t*)dst;
     if (width == 1) {
