@@ -58,55 +58,68 @@ __device__ inline void xxh3_block_sums(const uint8_t* in, size_t nbBlocks, uint6
     sums[blk * 8 + i] = s;
 }
 
-// ... then one wavefront runs the scramble chain, the tail and the merge.
-__device__ inline void xxh3_finish_wave(const uint8_t* in, size_t len, const uint64_t* sums, uint64_t* out)
+// ... then the scramble chain, the tail and the merge, with one wave per
+// accumulator word: every value is uniform
+// across the wave, so the scramble chain runs on the scalar unit (dependent
+// scalar instructions issue back to back, the 64-bit multiply takes two
+// 32-bit ones) and the block sums come through scalar loads, read eight
+// blocks ahead. Eight waves per buffer; wave 0 merges.
+__constant__ uint64_t kXxh3Init[8] = { OZL_P32_3, OZL_P64_1, OZL_P64_2, OZL_P64_3,
+                                       OZL_P64_4, OZL_P32_2, OZL_P64_5, OZL_P32_1 };
+
+__device__ inline void xxh3_finish_waves(const uint8_t* __restrict__ in, size_t len, const uint64_t* __restrict__ sums,
+                                         uint64_t* __restrict__ out, uint64_t* accs)
 {
-    int lane = threadIdx.x;
     if (len <= 240) {
-        if (lane == 0) *out = ozl_xxh3_short(in, len);
+        if (threadIdx.x == 0) *out = ozl_xxh3_short(in, len);
         return;
     }
-    const uint64_t init[8] = { OZL_P32_3, OZL_P64_1, OZL_P64_2, OZL_P64_3, OZL_P64_4, OZL_P32_2, OZL_P64_5, OZL_P32_1 };
-    uint64_t acc = lane < 8 ? init[lane] : 0;
-    size_t nbBlocks = (len - 1) / 1024;
-    if (lane < 8) {
-        // the chain is serial: the block sums are read 32 blocks ahead of
-        // their use, and the lane's secret word is read once
-        const uint64_t sec = ozl_sec64(192 - 64 + 8 * lane);
-        constexpr int kAhead = 32;
-        uint64_t cur[kAhead], nxt[kAhead];
-        size_t b = 0;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    uint64_t acc = kXxh3Init[w];
+    const size_t nbBlocks = (len - 1) / 1024;
+    const uint64_t sec = ozl_sec64(192 - 64 + 8 * w);
+    const uint64_t* sp = sums + w;
+    constexpr int kA = 8;
+    auto scr = [&](uint64_t v) {
+        uint64_t a2 = acc + v;
+        a2 ^= a2 >> 47;
+        a2 ^= sec;
+        acc = a2 * OZL_P32_1;
+    };
+    size_t b = 0;
+    if (nbBlocks >= (size_t)kA) {
+        // lane k of the wave holds the sum of block b + k. The sums come
+        // through vector loads, which complete in order: the next batch's
+        // load stays in flight while this batch is scrambled (a scalar load
+        // could only be waited for together with every other one).
+        // (every lane loads, lane k and k + 8j the same sum: no branch)
+        const size_t lane = threadIdx.x & (kA - 1);
+        uint64_t cur = sp[8 * lane];
+        for (; b + 2 * kA <= nbBlocks; b += kA) {
+            const uint64_t nxt = sp[8 * (b + kA + lane)];
 #pragma unroll
-        for (int k = 0; k < kAhead; k++) cur[k] = (size_t)k < nbBlocks ? sums[(size_t)k * 8 + lane] : 0;
-        for (; b + kAhead <= nbBlocks; b += kAhead) {
-#pragma unroll
-            for (int k = 0; k < kAhead; k++) {
-                const size_t q = b + kAhead + (size_t)k;
-                nxt[k] = q < nbBlocks ? sums[q * 8 + lane] : 0;
+            for (int k = 0; k < kA; k++) {
+                const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)cur, k);
+                const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(cur >> 32), k);
+                scr(((uint64_t)hi << 32) | lo);
             }
-#pragma unroll
-            for (int k = 0; k < kAhead; k++) {
-                uint64_t a2 = acc + cur[k];
-                a2 ^= a2 >> 47;
-                a2 ^= sec;
-                acc = a2 * OZL_P32_1;
-            }
-#pragma unroll
-            for (int k = 0; k < kAhead; k++) cur[k] = nxt[k];
+            cur = nxt;
         }
-        for (; b < nbBlocks; b++) {
-            uint64_t a2 = acc + sums[b * 8 + lane];
-            a2 ^= a2 >> 47;
-            a2 ^= sec;
-            acc = a2 * OZL_P32_1;
+#pragma unroll
+        for (int k = 0; k < kA; k++) {
+            const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)cur, k);
+            const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(cur >> 32), k);
+            scr(((uint64_t)hi << 32) | lo);
         }
-        size_t nbStripes = ((len - 1) - 1024 * nbBlocks) / 64;
-        for (size_t s = 0; s < nbStripes; s++) acc += ozl_stripe_term(in + nbBlocks * 1024 + s * 64, (int)s * 8, lane);
-        acc += ozl_stripe_term(in + len - 64, 192 - 64 - 7, lane);
+        b += kA;
     }
-    uint64_t a[8];
-    for (int i = 0; i < 8; i++) a[i] = __shfl(acc, i, 64);
-    if (lane == 0) *out = ozl_xxh3_merge(a, len);
+    for (; b < nbBlocks; b++) scr(sp[8 * b]);
+    const size_t nbStripes = ((len - 1) - 1024 * nbBlocks) / 64;
+    for (size_t s = 0; s < nbStripes; s++) acc += ozl_stripe_term(in + nbBlocks * 1024 + s * 64, (int)s * 8, w);
+    acc += ozl_stripe_term(in + len - 64, 192 - 64 - 7, w);
+    if ((threadIdx.x & 63) == 0) accs[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = ozl_xxh3_merge(accs, len);
 }
 
 __global__ void __launch_bounds__(kT) k_xxh3_blocks(const uint8_t* __restrict__ in, size_t nbBlocks,
@@ -115,10 +128,11 @@ __global__ void __launch_bounds__(kT) k_xxh3_blocks(const uint8_t* __restrict__ 
     xxh3_block_sums(in, nbBlocks, sums, (size_t)blockIdx.x * (kT / 8) + (threadIdx.x >> 3), threadIdx.x & 7);
 }
 
-__global__ void __launch_bounds__(64) k_xxh3_finish(const uint8_t* __restrict__ in, size_t len,
-                                                    const uint64_t* __restrict__ sums, uint64_t* __restrict__ out)
+__global__ void __launch_bounds__(512) k_xxh3_finish(const uint8_t* __restrict__ in, size_t len,
+                                                     const uint64_t* __restrict__ sums, uint64_t* __restrict__ out)
 {
-    xxh3_finish_wave(in, len, sums, out);
+    __shared__ uint64_t accs[8];
+    xxh3_finish_waves(in, len, sums, out, accs);
 }
 
 // many buffers: grid.y = buffer (block sums), grid.x = buffer (chains)
@@ -130,11 +144,12 @@ __global__ void __launch_bounds__(kT) k_xxh3_blocks_n(const ozl_xjob* __restrict
                     (size_t)blockIdx.x * (kT / 8) + (threadIdx.x >> 3), threadIdx.x & 7);
 }
 
-__global__ void __launch_bounds__(64) k_xxh3_finish_n(const ozl_xjob* __restrict__ jobs, const uint64_t* __restrict__ sums,
-                                                      uint64_t* __restrict__ out)
+__global__ void __launch_bounds__(512) k_xxh3_finish_n(const ozl_xjob* __restrict__ jobs,
+                                                       const uint64_t* __restrict__ sums, uint64_t* __restrict__ out)
 {
+    __shared__ uint64_t accs[8];
     const ozl_xjob j = jobs[blockIdx.x];
-    xxh3_finish_wave((const uint8_t*)(uintptr_t)j.ptr, j.len, sums + j.sumOff, out + blockIdx.x);
+    xxh3_finish_waves((const uint8_t*)(uintptr_t)j.ptr, j.len, sums + j.sumOff, out + blockIdx.x, accs);
 }
 
 // frame tails: the low 4 bytes of hashes[k] (little endian) at dst + at[k]
@@ -164,7 +179,7 @@ int ozl_xxh3_device(const uint8_t* data, size_t len, uint64_t* dOut, void* scrat
     if (scratchSize < ozl_xxh3_scratch_size(len)) return OZL_ERR_INTERNAL_BUFFER;
     uint64_t* sums = (uint64_t*)scratch;
     if (nbBlocks) OZL_LAUNCH("k_xxh3_blocks", k_xxh3_blocks, dim3((unsigned)((nbBlocks + 31) / 32)), dim3(kT), 0, st, data, nbBlocks, sums);
-    OZL_LAUNCH("k_xxh3_finish", k_xxh3_finish, dim3(1), dim3(64), 0, st, data, len, sums, dOut);
+    OZL_LAUNCH("k_xxh3_finish", k_xxh3_finish, dim3(1), dim3(512), 0, st, data, len, sums, dOut);
     OZL_HIP_CHECK(hipGetLastError());
     return OZL_OK;
 }
@@ -185,7 +200,7 @@ int ozl_xxh3_device_batch(const ozl_xjob* jobs, uint32_t count, size_t maxLen, u
             OZL_LAUNCH("k_xxh3_blocks_n", k_xxh3_blocks_n, dim3((unsigned)((maxBlocks + 31) / 32), cnt), dim3(kT), 0, st,
                        jobs + y0, sums);
         }
-    OZL_LAUNCH("k_xxh3_finish_n", k_xxh3_finish_n, dim3(count), dim3(64), 0, st, jobs, sums, dOut);
+    OZL_LAUNCH("k_xxh3_finish_n", k_xxh3_finish_n, dim3(count), dim3(512), 0, st, jobs, sums, dOut);
     OZL_HIP_CHECK(hipGetLastError());
     return OZL_OK;
 }
