@@ -377,6 +377,9 @@ def bench_select(args):
     print("[bench] timing", file=sys.stderr, flush=True)
     ms_ml = timed(learned)
     ms_ex = timed(exhaustive)
+    # the selection alone: features + MLP against the exhaustive trial's sizes and choice
+    ms_sel_ml = timed(lambda: sb.predict(src, params))
+    ms_sel_ex = timed(lambda: sb.trial(src))
     prof = {}
     if not args.no_profile:
         L.ozl_profile_begin()
@@ -417,6 +420,8 @@ def bench_select(args):
             "learned_ms": round(ms_ml, 4), "exhaustive_ms": round(ms_ex, 4),
             "learned_GBps": round(U / (ms_ml / 1000) / 1e9, 3), "exhaustive_GBps": round(U / (ms_ex / 1000) / 1e9, 3),
             "learned_speedup_vs_exhaustive": round(ms_ex / ms_ml, 3),
+            "selection_ms_learned": round(ms_sel_ml, 4), "selection_ms_exhaustive": round(ms_sel_ex, 4),
+            "selection_speedup": round(ms_sel_ex / ms_sel_ml, 3),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -45,6 +45,7 @@ typedef enum {
     ZL_ErrorCode_compressedChecksumWrong = 13,
     ZL_ErrorCode_contentChecksumWrong = 14,
     ZL_ErrorCode_parameter_invalid = 21,
+    ZL_ErrorCode_segmenter_inputNotConsumed = 26,
     ZL_ErrorCode_graph_invalid = 30,
     ZL_ErrorCode_graph_invalidNumInputs = 33,
     ZL_ErrorCode_successor_invalid = 40,
@@ -162,6 +163,53 @@ typedef struct {
 } ZL_FunctionGraphDesc;
 /* include/openzl/zl_graph_api.h:114 */
 ZL_GraphID ZL_Compressor_registerFunctionGraph(ZL_Compressor* compressor, const ZL_FunctionGraphDesc* desc);
+
+/* Segmenters: include/openzl/zl_segmenter.h:69-108, 127-330 (subset: one
+ * input; processChunk takes no runtime parameters). A segmenter cuts its
+ * input into chunks, each compressed by a graph into a chunk of the frame
+ * (format version 21+). */
+typedef struct {
+    const ZL_GraphID* graphids;
+    size_t nbGraphIDs;
+} ZL_GraphIDList;                     /* include/openzl/zl_common_types.h:50-53 */
+typedef struct {
+    void* ptr;
+    void* freeOpaquePtr;
+    void (*freeFn)(void* freeOpaquePtr, void* ptr);
+} ZL_OpaquePtr;                       /* include/openzl/zl_common_types.h:25-49 */
+typedef struct ZL_Segmenter_s ZL_Segmenter;
+typedef ZL_Report (*ZL_SegmenterFn)(ZL_Segmenter* sctx);
+typedef struct {
+    const char* name;
+    ZL_SegmenterFn segmenterFn;
+    const ZL_Type* inputTypeMasks;
+    size_t numInputs;
+    int lastInputIsVariable;          /* bool in the reference; 1 input only here */
+    const ZL_GraphID* customGraphs;
+    size_t numCustomGraphs;
+    char localParams[48];             /* layout placeholder, unused */
+    ZL_OpaquePtr opaque;
+} ZL_SegmenterDesc;
+/* include/openzl/zl_segmenter.h:97 */
+ZL_GraphID ZL_Compressor_registerSegmenter(ZL_Compressor* compressor, const ZL_SegmenterDesc* segDesc);
+/* include/openzl/zl_segmenter.h:127 */
+const void* ZL_Segmenter_getOpaquePtr(const ZL_Segmenter* segCtx);
+/* include/openzl/zl_segmenter.h:140 */
+int ZL_Segmenter_getCParam(const ZL_Segmenter* segCtx, ZL_CParam gparam);
+/* include/openzl/zl_segmenter.h:187 */
+ZL_GraphIDList ZL_Segmenter_getCustomGraphs(const ZL_Segmenter* segCtx);
+/* include/openzl/zl_segmenter.h:200 */
+size_t ZL_Segmenter_numInputs(const ZL_Segmenter* segCtx);
+/* include/openzl/zl_segmenter.h:222 (the remaining input; sizes and type only:
+ * its bytes are in device memory) */
+const ZL_Input* ZL_Segmenter_getInput(const ZL_Segmenter* segCtx, size_t inputID);
+/* include/openzl/zl_segmenter.h:253 */
+ZL_Report ZL_Segmenter_getNumElts(const ZL_Segmenter* segCtx, size_t numElts[], size_t numInputs);
+/* include/openzl/zl_segmenter.h:277 (host memory, freed when the call ends) */
+void* ZL_Segmenter_getScratchSpace(ZL_Segmenter* segCtx, size_t size);
+/* include/openzl/zl_segmenter.h:300-330; rGraphParams must be NULL */
+ZL_Report ZL_Segmenter_processChunk(ZL_Segmenter* segCtx, const size_t numElts[], size_t numInputs,
+                                    ZL_GraphID startingGraphID, const void* rGraphParams);
 typedef struct {
     ZL_Edge** edges;
     size_t nbEdges;

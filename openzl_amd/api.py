@@ -34,7 +34,8 @@ G_STORE, G_FSE, G_HUFFMAN, G_BITPACK = GraphID(2), GraphID(3), GraphID(4), Graph
 N_DELTA, N_TRANSPOSE_SPLIT, N_ZIGZAG = NodeID(2), NodeID(3), NodeID(4)
 
 # graph kinds, same numbering as the reference harness (oracle/ref_harness.c)
-STORE, FSE, HUFFMAN, BITPACK, DELTA_TS_FSE, DELTA_ZZ_BP, TS_HUF_FSE, DELTA_FSE, TS_FSE, TS_HUF, ZZ_BP, DELTA_BP, SELECT8 = range(13)
+STORE, FSE, HUFFMAN, BITPACK, DELTA_TS_FSE, DELTA_ZZ_BP, TS_HUF_FSE, DELTA_FSE, TS_FSE, TS_HUF, ZZ_BP, DELTA_BP, SELECT8, \
+    SEGMENTED = range(14)
 
 _setup = False
 
@@ -105,11 +106,17 @@ def _graphs_lib():
         g = ctypes.CDLL(path)
         g.ozl_graph_weights_split.restype = GraphID
         g.ozl_graph_weights_split.argtypes = [P]
+        g.ozl_graph_fixed_chunks.restype = GraphID
+        g.ozl_graph_fixed_chunks.argtypes = [P, GraphID, SZ]
         _graphs = g
     return _graphs
 
 
-def build_graph(L, c, kind):
+def build_graph(L, c, kind, segment=None):
+    """GraphID of a graph kind; SEGMENTED takes segment = (inner kind, chunk elements)."""
+    if kind == SEGMENTED:
+        inner, elts = segment
+        return _graphs_lib().ozl_graph_fixed_chunks(c, build_graph(L, c, inner), elts)
     if kind == STORE:
         return G_STORE
     if kind == FSE:
@@ -143,7 +150,7 @@ def build_graph(L, c, kind):
 class Compressor:
     """A ZL_Compressor + ZL_CCtx pair for one graph kind (format version 23)."""
 
-    def __init__(self, kind, candidates=None, content_checksum=True, compressed_checksum=True):
+    def __init__(self, kind, candidates=None, content_checksum=True, compressed_checksum=True, segment=None):
         L = _api()
         self.L = L
         self.c = L.ZL_Compressor_create()
@@ -156,7 +163,7 @@ class Compressor:
             gids = (GraphID * len(candidates))(*[build_graph(L, self.c, k) for k in candidates])
             g = L.ZL_Compressor_registerBruteForceSelectorGraph(self.c, gids, len(candidates))
         else:
-            g = build_graph(L, self.c, kind)
+            g = build_graph(L, self.c, kind, segment)
         _chk(L.ZL_Compressor_selectStartingGraphID(self.c, g), "selectStartingGraphID")
         self.cctx = L.ZL_CCtx_create()
         _chk(L.ZL_CCtx_refCompressor(self.cctx, self.c), "refCompressor")

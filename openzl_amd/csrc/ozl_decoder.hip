@@ -103,11 +103,16 @@ ZL_Report rep_err(int code)
     return r;
 }
 
+// Reads frame bytes [base, end) held at p (a window of the frame on the host).
 struct HostReader {
     const uint8_t* p;
-    size_t pos, end;
+    size_t base, pos, end;
     bool bad;
-    uint32_t u8() { if (pos >= end) { bad = true; return 0; } return p[pos++]; }
+    uint32_t u8()
+    {
+        if (pos < base || pos >= end) { bad = true; return 0; }
+        return p[pos++ - base];
+    }
     uint64_t varint()
     {
         uint64_t v = 0;
@@ -123,11 +128,12 @@ struct HostReader {
     {
         out.assign(cnt, 0);
         size_t nbytes = (cnt * (size_t)nb + 7) / 8;
-        if (pos + nbytes > end) { bad = true; return; }
+        if (pos < base || pos + nbytes > end) { bad = true; return; }
+        const uint8_t* q = p + (pos - base);
         size_t bit = 0;
         for (size_t i = 0; i < cnt; i++) {
             uint32_t v = 0;
-            for (int b = 0; b < nb; b++, bit++) v |= (uint32_t)((p[pos + (bit >> 3)] >> (bit & 7)) & 1) << b;
+            for (int b = 0; b < nb; b++, bit++) v |= (uint32_t)((q[bit >> 3] >> (bit & 7)) & 1) << b;
             out[i] = v;
         }
         pos += nbytes;
@@ -632,23 +638,31 @@ int DQueue::flush()
     return 0;
 }
 
-struct ParsedFrame {
-    int flags, type;
-    uint64_t size;
-    size_t chunkStart, dataStart, csumPos, end;
+// One chunk of a frame (format version 21+: a frame holds chunks, each with
+// its own graph, stored streams and checksums, then a zero byte;
+// decompress2.c:1442-1560, 1690-1720).
+struct ParsedChunk {
+    size_t start = 0, dataStart = 0, csumPos = 0, end = 0;   // end: after its checksums
     std::vector<DNode> nodes;
     std::vector<uint64_t> storedSize;
     std::vector<uint32_t> storedId;
-    uint32_t nbStreams;
+    uint32_t nbStreams = 0;
 };
 
-// Reads the headers of the frame f[0, size); only f[0, avail) is read (a
-// frame in device memory has its first avail bytes copied to the host). The
-// trailer (checksums, end byte) is checked by the caller.
-int parse_frame(const uint8_t* f, size_t size, size_t avail, ParsedFrame& P)
+struct ParsedFrame {
+    int flags = 0, type = 0;
+    uint64_t size = 0;
+    size_t headerEnd = 0;
+    std::vector<ParsedChunk> chunks;
+    size_t end = 0;   // position of the end-of-frame marker
+};
+
+// a read ran past the host window before the end of the frame
+constexpr int kNeedMore = -2;
+
+// Frame header (decode_frameheader.c); the window starts at the frame start.
+int parse_frame_header(HostReader& r, ParsedFrame& P)
 {
-    if (avail > size) avail = size;
-    HostReader r = { f, 0, avail, false };
     uint32_t magic = 0;
     for (int k = 0; k < 4; k++) magic |= r.u8() << (8 * k);
     if (r.bad || magic < OZL_MAGIC_BASE || magic > OZL_MAGIC_BASE + 64) return OZL_ERR_HEADER_UNKNOWN;
@@ -662,50 +676,63 @@ int parse_frame(const uint8_t* f, size_t size, size_t avail, ParsedFrame& P)
     if (r.bad || s1 == 0) return OZL_ERR_CORRUPTION;
     P.size = s1 - 1;
     if (P.flags & 2) {
-        uint8_t hc = (uint8_t)(ozl_xxh3_short(f, r.pos) & 255);
+        uint8_t hc = (uint8_t)(ozl_xxh3_short(r.p, r.pos) & 255);
         if (r.u8() != hc) return OZL_ERR_CORRUPTION;
     }
-    P.chunkStart = r.pos;
+    if (r.bad) return OZL_ERR_CORRUPTION;
+    P.headerEnd = r.pos;
+    return 0;
+}
+
+// The headers of the chunk at r.pos of a frame of frameSize bytes.
+int parse_chunk(HostReader& r, size_t frameSize, int flags, ParsedChunk& C)
+{
+    const bool windowed = r.end < frameSize;
+    auto fail = [&]() { return windowed ? kNeedMore : OZL_ERR_CORRUPTION; };
+    C.start = r.pos;
     uint64_t nbDec1 = r.varint();
-    if (r.bad || nbDec1 == 0) return OZL_ERR_CORRUPTION;
+    if (r.bad) return fail();
+    if (nbDec1 == 0) return OZL_ERR_CORRUPTION;
     size_t nn = (size_t)nbDec1 - 1;
     uint64_t nbStored = r.varint();
-    if (r.bad || nn > 20000 || nbStored > 110000) return OZL_ERR_CORRUPTION;
+    if (r.bad) return fail();
+    if (nn > 20000 || nbStored > 110000) return OZL_ERR_CORRUPTION;
     std::vector<uint32_t> trt, ids, hf, vf, inf, dist;
     r.unpack(trt, nn, 1);
     for (uint32_t t : trt)
         if (t) return ZL_ErrorCode_frameParameter_unsupported;  // custom transforms
     r.unpack(ids, nn, 6);
-    P.nodes.assign(nn, DNode());
-    for (size_t k = 0; k < nn; k++) P.nodes[k].trid = (uint8_t)ids[nn - 1 - k];
+    C.nodes.assign(nn, DNode());
+    for (size_t k = 0; k < nn; k++) C.nodes[k].trid = (uint8_t)ids[nn - 1 - k];
     r.unpack(hf, nn, 1);
     std::vector<size_t> hlen(nn, 0);
     for (size_t i = 0; i < nn; i++)
         if (hf[i]) hlen[nn - 1 - i] = (size_t)r.varint() + 1;
     r.unpack(vf, nn, 1);
-    for (size_t k = 0; k < nn; k++) P.nodes[k].nbVO = 0;
+    for (size_t k = 0; k < nn; k++) C.nodes[k].nbVO = 0;
     for (size_t i = 0; i < nn; i++)
-        if (vf[i]) P.nodes[nn - 1 - i].nbVO = (uint32_t)r.varint() + 1;
+        if (vf[i]) C.nodes[nn - 1 - i].nbVO = (uint32_t)r.varint() + 1;
     r.unpack(inf, nn, 1);
     for (uint32_t x : inf)
         if (x) return ZL_ErrorCode_frameParameter_unsupported;  // multi-input transforms
     r.unpack(dist, nn, ozl_next_pow2((uint64_t)nn + nbStored));
-    P.storedSize.resize(nbStored);
-    for (size_t k = 0; k < nbStored; k++) P.storedSize[k] = r.varint();
-    if (r.bad) return OZL_ERR_CORRUPTION;
+    C.storedSize.resize(nbStored);
+    for (size_t k = 0; k < nbStored; k++) C.storedSize[k] = r.varint();
+    if (r.bad) return fail();
     // node headers (decode order)
     size_t hp = r.pos;
     for (size_t i = 0; i < nn; i++) {
-        DNode& nd = P.nodes[nn - 1 - i];
-        if (hp + hlen[nn - 1 - i] > avail) return OZL_ERR_CORRUPTION;
-        nd.hdr.assign(f + hp, f + hp + hlen[nn - 1 - i]);
-        hp += hlen[nn - 1 - i];
+        DNode& nd = C.nodes[nn - 1 - i];
+        const size_t l = hlen[nn - 1 - i];
+        if (hp + l > r.end) return fail();
+        nd.hdr.assign(r.p + (hp - r.base), r.p + (hp - r.base) + l);
+        hp += l;
     }
-    P.dataStart = hp;
+    C.dataStart = hp;
     // stream numbering (creation order)
     uint32_t next = 1;
     for (size_t k = 0; k < nn; k++) {
-        DNode& nd = P.nodes[k];
+        DNode& nd = C.nodes[k];
         int s1n = nb_singletons(nd.trid);
         if (s1n < 0) return ZL_ErrorCode_frameParameter_unsupported;
         nd.firstOut = next;
@@ -715,44 +742,92 @@ int parse_frame(const uint8_t* f, size_t size, size_t avail, ParsedFrame& P)
         if (dd > nd.firstOut) return OZL_ERR_CORRUPTION;
         nd.in = nd.firstOut - dd;
     }
-    P.nbStreams = next;
+    C.nbStreams = next;
     std::vector<uint8_t> consumed(next, 0);
-    for (auto& nd : P.nodes) {
+    for (auto& nd : C.nodes) {
         if (consumed[nd.in]) return OZL_ERR_CORRUPTION;
         consumed[nd.in] = 1;
     }
-    for (int64_t s = (int64_t)next - 1; s >= 0; s--)
-        if (!consumed[s]) P.storedId.push_back((uint32_t)s);
-    if (P.storedId.size() != nbStored) return OZL_ERR_CORRUPTION;
-    size_t pos = P.dataStart;
-    for (size_t k = 0; k < nbStored; k++) pos += P.storedSize[k];
-    P.csumPos = pos;
-    P.end = pos + ((P.flags & 1) ? 4 : 0) + ((P.flags & 2) ? 4 : 0);
-    if (P.end + 1 > size) return OZL_ERR_CORRUPTION;
+    for (int64_t t = (int64_t)next - 1; t >= 0; t--)
+        if (!consumed[t]) C.storedId.push_back((uint32_t)t);
+    if (C.storedId.size() != nbStored) return OZL_ERR_CORRUPTION;
+    size_t pos = C.dataStart;
+    for (size_t k = 0; k < nbStored; k++) {
+        if (C.storedSize[k] > frameSize) return OZL_ERR_CORRUPTION;
+        pos += C.storedSize[k];
+    }
+    C.csumPos = pos;
+    C.end = pos + ((flags & 1) ? 4 : 0) + ((flags & 2) ? 4 : 0);
+    if (C.end + 1 > frameSize) return OZL_ERR_CORRUPTION;   // the end marker follows at the latest
     return 0;
 }
+
+// The chunks of one frame whose bytes [base, base + len) are on the host
+// (the whole frame, or a window of a device frame): parses the header when
+// pos is 0, then chunks until the end marker, or until the window runs out
+// (kNeedMore: the next window starts at pos; `longer` when a chunk's headers
+// did not fit a window that starts at the chunk).
+int parse_window(const uint8_t* w, size_t base, size_t len, size_t frameSize, ParsedFrame& P, size_t& pos,
+                 bool& done, bool& longer)
+{
+    longer = false;
+    HostReader r = { w, base, 0, base + len, false };
+    if (pos == 0) {
+        if (base != 0) return OZL_ERR_LOGIC;
+        r.pos = 0;
+        int rc = parse_frame_header(r, P);
+        if (rc) return (r.end < frameSize && r.bad) ? (longer = true, kNeedMore) : rc;
+        pos = P.headerEnd;
+    }
+    for (;;) {
+        if (pos >= base + len) return kNeedMore;
+        if (w[pos - base] == 0) {   // end of frame
+            P.end = pos;
+            done = true;
+            return 0;
+        }
+        ParsedChunk C;
+        r.pos = pos;
+        r.bad = false;
+        int rc = parse_chunk(r, frameSize, P.flags, C);
+        if (rc == kNeedMore) {
+            longer = pos == base;
+            return kNeedMore;
+        }
+        if (rc) return rc;
+        pos = C.end;
+        P.chunks.push_back(std::move(C));
+    }
+}
+
+// One chunk of a decompression call: its decoder and where its content goes.
+struct ChunkJob {
+    size_t frame;
+    const ParsedChunk* C;
+    std::unique_ptr<Decoder> D;
+    std::vector<uint32_t> todo;   // nodes still to decode, in decode order (last node first)
+    uint8_t tail[8];              // checksums
+    uint64_t outOff = 0;          // in the destination
+};
 
 // One frame of a decompression call.
 struct FrameJob {
     const uint8_t* src;    // the frame (host or device memory)
     size_t size;
     bool dev;              // src is device memory
-    std::unique_ptr<uint8_t[]> hbuf;   // host copy of a device frame's headers
-    size_t avail = 0;
-    const uint8_t* host = nullptr;     // host view of the headers
     ParsedFrame P;
     uint8_t* dFrame = nullptr;
-    std::unique_ptr<Decoder> D;
-    std::vector<uint32_t> todo;   // nodes still to decode, in decode order (last node first)
-    uint8_t tail[9];       // checksums and end byte
-    uint64_t tailOff = 0;
+    size_t pos = 0;        // parse position
+    bool done = false;
+    size_t win = 0;        // window length of a device frame
     int wantType = 0;
 };
 
 // Decompresses n frames (host or device memory) into dst (host or device
-// memory) back to back: the host reads every frame's headers; the decoders
-// of all frames run in lockstep, each until it needs a stream a queued
-// launch still has to produce, then the queue goes out as shared launches;
+// memory) back to back: the host reads every frame's headers (windows of
+// device frames come over in batched copies); the decoders of all chunks of
+// all frames run in lockstep, each until it needs a stream a queued launch
+// still has to produce, then the queue goes out as shared launches;
 // checksums are verified in batches; one synchronisation at the end.
 int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSizes, size_t n, void* dst, size_t cap,
                       size_t* outSizes, ZL_OutputInfo* infos)
@@ -779,121 +854,151 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         J[i].dev = devIn;
         off += frameSizes[i];
     }
-    // headers of device frames: the first 64 KiB of each come to the host in
-    // one copy, the whole frame when its headers are longer
-    constexpr size_t kPre = (size_t)64 << 10;
-    std::vector<uint8_t> pre;
-    if (devIn) {
-        std::vector<uint64_t> m(3 * n);
-        for (size_t i = 0; i < n; i++) {
-            m[i] = (uint64_t)(uintptr_t)J[i].src;
-            m[n + i] = J[i].size < kPre ? J[i].size : kPre;
-            m[2 * n + i] = i * kPre;
-        }
-        uint8_t* dPre = dc->alloc(n * kPre + 16);
-        uint64_t* dm = (uint64_t*)dc->alloc(m.size() * 8 + 16);
-        if (!dPre || !dm) return OZL_ERR_ALLOCATION;
-        pre.resize(n * kPre);
-        const uint8_t* hm = dc->stage.put(m.data(), m.size() * 8);
-        if (!hm) return OZL_ERR_ALLOCATION;
-        OZL_HIP_CHECK(hipMemcpyAsync(dm, hm, m.size() * 8, hipMemcpyHostToDevice, st));
-        int rc = ozl_gather_sized((const uint8_t* const*)dm, dm + n, dm + 2 * n, (int)n, kPre, dPre, st);
-        if (rc) return rc;
-        OZL_HIP_CHECK(hipMemcpyAsync(pre.data(), dPre, n * kPre, hipMemcpyDeviceToHost, st));
-        OZL_HIP_CHECK(hipStreamSynchronize(st));
-        for (size_t i = 0; i < n; i++) {
-            J[i].host = pre.data() + i * kPre;
-            J[i].avail = m[n + i];
-        }
-    } else {
-        for (FrameJob& j : J) {
-            j.host = j.src;
-            j.avail = j.size;
-        }
-    }
-    {
+    // headers of every chunk
+    if (!devIn) {
         std::vector<int> prc(n, 0);
-        ozl_parallel_for(n, [&](size_t i) { prc[i] = parse_frame(J[i].host, J[i].size, J[i].avail, J[i].P); });
-        for (size_t i = 0; i < n; i++) {
-            FrameJob& j = J[i];
-            int rc = prc[i];
-            if (rc && j.avail < j.size) {
-                j.hbuf.reset(new uint8_t[j.size]);
-                OZL_HIP_CHECK(hipMemcpy(j.hbuf.get(), j.src, j.size, hipMemcpyDeviceToHost));
-                j.host = j.hbuf.get();
-                j.avail = j.size;
-                rc = parse_frame(j.host, j.size, j.avail, j.P);
+        ozl_parallel_for(n, [&](size_t i) {
+            bool longer = false;
+            prc[i] = parse_window(J[i].src, 0, J[i].size, J[i].size, J[i].P, J[i].pos, J[i].done, longer);
+        });
+        for (int rc : prc)
+            if (rc) return rc == kNeedMore ? OZL_ERR_CORRUPTION : rc;
+    } else {
+        // windows of the device frames (64 KiB, longer where a chunk's
+        // headers need it), one gather and one copy per round
+        constexpr size_t kWin = (size_t)64 << 10;
+        for (FrameJob& j : J) j.win = j.size < kWin ? j.size : kWin;
+        for (;;) {
+            std::vector<size_t> L;
+            for (size_t i = 0; i < n; i++)
+                if (!J[i].done) L.push_back(i);
+            if (L.empty()) break;
+            std::vector<uint64_t> m(3 * L.size());
+            std::vector<size_t> wOff(L.size()), wLen(L.size());
+            size_t tot = 0, mx = 0;
+            for (size_t k = 0; k < L.size(); k++) {
+                FrameJob& j = J[L[k]];
+                if (j.pos >= j.size) return OZL_ERR_CORRUPTION;   // no end marker
+                const size_t len = j.size - j.pos < j.win ? j.size - j.pos : j.win;
+                m[k] = (uint64_t)(uintptr_t)(j.src + j.pos);
+                m[L.size() + k] = len;
+                m[2 * L.size() + k] = tot;
+                wOff[k] = tot;
+                wLen[k] = len;
+                tot += len;
+                mx = len > mx ? len : mx;
             }
+            uint8_t* dWin = dc->alloc(tot + 16);
+            uint64_t* dm = (uint64_t*)dc->alloc(m.size() * 8 + 16);
+            const uint8_t* hm = dc->stage.put(m.data(), m.size() * 8);
+            if (!dWin || !dm || !hm) return OZL_ERR_ALLOCATION;
+            OZL_HIP_CHECK(hipMemcpyAsync(dm, hm, m.size() * 8, hipMemcpyHostToDevice, st));
+            int rc = ozl_gather_sized((const uint8_t* const*)dm, dm + L.size(), dm + 2 * L.size(), (int)L.size(), mx,
+                                      dWin, st);
             if (rc) return rc;
+            std::vector<uint8_t> hw(tot + 1);
+            OZL_HIP_CHECK(hipMemcpyAsync(hw.data(), dWin, tot, hipMemcpyDeviceToHost, st));
+            OZL_HIP_CHECK(hipStreamSynchronize(st));
+            std::vector<int> prc(L.size(), 0);
+            std::vector<char> lg(L.size(), 0);
+            ozl_parallel_for(L.size(), [&](size_t k) {
+                FrameJob& j = J[L[k]];
+                bool longer = false;
+                prc[k] = parse_window(hw.data() + wOff[k], j.pos, wLen[k], j.size, j.P, j.pos, j.done, longer);
+                lg[k] = longer;
+            });
+            for (size_t k = 0; k < L.size(); k++) {
+                FrameJob& j = J[L[k]];
+                if (prc[k] == kNeedMore) {
+                    if (lg[k]) {
+                        if (wLen[k] >= j.size - j.pos) return OZL_ERR_CORRUPTION;
+                        j.win *= 4;
+                    }
+                    continue;
+                }
+                if (prc[k]) return prc[k];
+            }
         }
     }
+    // chunks: decoders, stored streams, checksum tails
+    std::vector<ChunkJob> U;
+    for (size_t i = 0; i < n; i++)
+        for (const ParsedChunk& C : J[i].P.chunks) {
+            ChunkJob u;
+            u.frame = i;
+            u.C = &C;
+            U.push_back(std::move(u));
+        }
     DQueue Q;
     Q.d = dc;
     Q.st = st;
     for (FrameJob& j : J) {
-        const ParsedFrame& P = j.P;
-        j.D.reset(new Decoder());
-        Decoder& D = *j.D;
-        D.d = dc;
-        D.q = &Q;
-        D.st = st;
-        D.S.assign(P.nbStreams, DStream());
-        Q.members.push_back(&D);
         if (j.dev) {
             j.dFrame = const_cast<uint8_t*>(j.src);   // read in place
         } else {
-            j.dFrame = D.alloc(j.size);
-            if (D.err) return D.err;
+            j.dFrame = dc->alloc(j.size + 16);
+            if (!j.dFrame) return OZL_ERR_ALLOCATION;
             OZL_HIP_CHECK(hipMemcpyAsync(j.dFrame, j.src, j.size, hipMemcpyHostToDevice, st));
         }
-        size_t pos = P.dataStart;
-        for (size_t k = 0; k < P.storedId.size(); k++) {
-            DStream& x = D.S[P.storedId[k]];
+    }
+    for (ChunkJob& u : U) {
+        const ParsedChunk& C = *u.C;
+        const FrameJob& j = J[u.frame];
+        u.D.reset(new Decoder());
+        Decoder& D = *u.D;
+        D.d = dc;
+        D.q = &Q;
+        D.st = st;
+        D.S.assign(C.nbStreams, DStream());
+        Q.members.push_back(&D);
+        size_t pos = C.dataStart;
+        for (size_t k = 0; k < C.storedId.size(); k++) {
+            DStream& x = D.S[C.storedId[k]];
             x.dev = j.dFrame + pos;
-            x.bytes = P.storedSize[k];
+            x.bytes = C.storedSize[k];
             x.type = ZL_Type_serial;
             x.width = 1;
             x.ready = true;
-            pos += P.storedSize[k];
+            pos += C.storedSize[k];
         }
         // stored numeric streams keep their type: side inputs of fse_v2
         // (int16) and huffman_v2 (u8) when stored raw
-        for (auto& nd : P.nodes) {
+        for (auto& nd : C.nodes) {
             if (nd.trid == OZL_TR_FSE_V2 && D.S[nd.firstOut].ready) { D.S[nd.firstOut].type = ZL_Type_numeric; D.S[nd.firstOut].width = 2; }
             if (nd.trid == OZL_TR_HUFFMAN_V2 && D.S[nd.firstOut].ready) { D.S[nd.firstOut].type = ZL_Type_numeric; }
         }
-        for (size_t k = P.nodes.size(); k-- > 0;) j.todo.push_back((uint32_t)k);
-        // checksums and end byte (device frames: gathered below)
-        j.tailOff = P.csumPos;
-        if (!j.dev) memcpy(j.tail, j.src + P.csumPos, P.end + 1 - P.csumPos);
+        for (size_t k = C.nodes.size(); k-- > 0;) u.todo.push_back((uint32_t)k);
+        if (!j.dev) memcpy(u.tail, j.src + C.csumPos, C.end - C.csumPos);
     }
     std::vector<uint8_t> tails;
     uint8_t* dTails = nullptr;
-    if (devIn) {
-        std::vector<uint64_t> m(3 * n);
-        for (size_t i = 0; i < n; i++) {
-            m[i] = (uint64_t)(uintptr_t)(J[i].src + J[i].P.csumPos);
-            m[n + i] = J[i].P.end + 1 - J[i].P.csumPos;
-            m[2 * n + i] = i * 16;
+    if (devIn && !U.empty()) {
+        const size_t nu = U.size();
+        std::vector<uint64_t> m(3 * nu);
+        for (size_t k = 0; k < nu; k++) {
+            m[k] = (uint64_t)(uintptr_t)(J[U[k].frame].src + U[k].C->csumPos);
+            m[nu + k] = U[k].C->end - U[k].C->csumPos;
+            m[2 * nu + k] = k * 8;
         }
-        dTails = dc->alloc(n * 16 + 16);
+        dTails = dc->alloc(nu * 8 + 16);
         uint64_t* dm = (uint64_t*)dc->alloc(m.size() * 8 + 16);
         if (!dTails || !dm) return OZL_ERR_ALLOCATION;
         OZL_HIP_CHECK(hipMemcpyAsync(dm, Q.host_keep(m.data(), m.size() * 8), m.size() * 8, hipMemcpyHostToDevice, st));
-        int rc = ozl_gather_sized((const uint8_t* const*)dm, dm + n, dm + 2 * n, (int)n, 16, dTails, st);
+        int rc = ozl_gather_sized((const uint8_t* const*)dm, dm + nu, dm + 2 * nu, (int)nu, 8, dTails, st);
         if (rc) return rc;
-        tails.resize(n * 16);
+        tails.resize(nu * 8);
     }
     // compressed checksums (they read only the frames) on the second stream
     std::vector<uint64_t> kJobs;
     std::vector<size_t> kIdx;
     size_t kSum = 0, kMax = 0;
-    for (size_t i = 0; i < n; i++) {
-        const ParsedFrame& P = J[i].P;
-        if (!(P.flags & 2)) continue;
-        const size_t clen = P.csumPos + ((P.flags & 1) ? 4 : 0) - P.chunkStart;
-        kJobs.insert(kJobs.end(), { (uint64_t)(uintptr_t)(J[i].dFrame + P.chunkStart), clen, kSum });
-        kIdx.push_back(i);
+    for (size_t k = 0; k < U.size(); k++) {
+        const FrameJob& j = J[U[k].frame];
+        const ParsedChunk& C = *U[k].C;
+        if (!(j.P.flags & 2)) continue;
+        const size_t clen = C.csumPos + ((j.P.flags & 1) ? 4 : 0) - C.start;
+        kJobs.insert(kJobs.end(), { (uint64_t)(uintptr_t)(j.dFrame + C.start), clen, kSum });
+        kIdx.push_back(k);
         kSum += ozl_xxh3_batch_sums(clen);
         kMax = clen > kMax ? clen : kMax;
     }
@@ -911,32 +1016,28 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         if (rc) return rc;
         OZL_HIP_CHECK(hipEventRecord(dc->evHash, dc->st2));
     }
-    // device destination: a frame's content whose place there is 16-byte
-    // aligned (as the library's own buffers are) is written in place
-    {
-        size_t t = 0;
-        for (const FrameJob& j : J) t += j.P.size;
-        if (t && t <= cap && ozl_is_device_ptr(dst)) {
-            size_t o = 0;
-            for (FrameJob& j : J) {
-                if ((((uintptr_t)dst + o) & 15) == 0) {
-                    j.D->rootDst = (uint8_t*)dst + o;
-                    j.D->rootSize = j.P.size;
-                }
-                o += j.P.size;
+    // output places: frames back to back; a frame of one chunk whose place
+    // in a device destination is 16-byte aligned (as the library's own
+    // buffers are) is decoded in place
+    std::vector<uint64_t> frameOut(n + 1, 0);
+    for (size_t i = 0; i < n; i++) frameOut[i + 1] = frameOut[i] + J[i].P.size;
+    const bool devOut = frameOut[n] && frameOut[n] <= cap && ozl_is_device_ptr(dst);
+    if (devOut)
+        for (ChunkJob& u : U)
+            if (J[u.frame].P.chunks.size() == 1 && (((uintptr_t)dst + frameOut[u.frame]) & 15) == 0) {
+                u.D->rootDst = (uint8_t*)dst + frameOut[u.frame];
+                u.D->rootSize = J[u.frame].P.size;
             }
-        }
-    }
     // the decoders in lockstep: every node runs as soon as the streams it
     // reads are ready or queued; a node that waits for a queued launch (or
     // for such a node) is retried once the queue has gone out
     for (;;) {
         bool left = false;
-        for (FrameJob& j : J) {
-            Decoder& D = *j.D;
+        for (ChunkJob& u : U) {
+            Decoder& D = *u.D;
             std::vector<uint32_t> rest;
-            for (uint32_t k : j.todo) {
-                const DNode& nd = j.P.nodes[k];
+            for (uint32_t k : u.todo) {
+                const DNode& nd = u.C->nodes[k];
                 int rc = D.decode(nd);
                 if (rc == kBlocked) {
                     D.S[nd.in].waiting = true;
@@ -946,9 +1047,9 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
                 if (rc) return rc;
                 if (D.err) return D.err;
             }
-            for (uint32_t k : rest) D.S[j.P.nodes[k].in].waiting = false;
-            j.todo.swap(rest);
-            if (!j.todo.empty()) left = true;
+            for (uint32_t k : rest) D.S[u.C->nodes[k].in].waiting = false;
+            u.todo.swap(rest);
+            if (!u.todo.empty()) left = true;
         }
         int rc = Q.flush();
         if (!rc) rc = Q.err;
@@ -956,32 +1057,36 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         if (!left) break;
     }
     // outputs, content checksum jobs
-    size_t total = 0;
-    std::vector<uint64_t> outOff(n);
+    std::vector<uint64_t> got(n, 0);
+    std::vector<int> width(n, 0);
     std::vector<uint64_t> cJobs;
     std::vector<size_t> cIdx;
     size_t cSum = 0, cMax = 0;
-    for (size_t i = 0; i < n; i++) {
-        FrameJob& j = J[i];
-        const ParsedFrame& P = j.P;
-        DStream& out = j.D->S[0];
-        if (P.nodes.empty())  // stored input
-            out.type = P.type == OZL_T_SERIAL ? ZL_Type_serial : P.type == OZL_T_STRUCT ? ZL_Type_struct : ZL_Type_numeric;
-        if (out.bytes != P.size) return OZL_ERR_CORRUPTION;
-        j.wantType = P.type == OZL_T_SERIAL ? ZL_Type_serial : P.type == OZL_T_STRUCT ? ZL_Type_struct : ZL_Type_numeric;
+    for (size_t i = 0; i < n; i++)
+        J[i].wantType = J[i].P.type == OZL_T_SERIAL ? ZL_Type_serial
+                        : J[i].P.type == OZL_T_STRUCT ? ZL_Type_struct : ZL_Type_numeric;
+    for (size_t k = 0; k < U.size(); k++) {
+        ChunkJob& u = U[k];
+        FrameJob& j = J[u.frame];
+        DStream& out = u.D->S[0];
+        if (u.C->nodes.empty()) out.type = j.wantType;   // stored input
         // a stored input keeps the frame's type; a regenerated one must match
-        if (!P.nodes.empty() && out.type != j.wantType && j.wantType != ZL_Type_serial) return OZL_ERR_CORRUPTION;
-        outOff[i] = total;
-        total += out.bytes;
-        if (P.flags & 1) {
+        if (!u.C->nodes.empty() && out.type != j.wantType && j.wantType != ZL_Type_serial) return OZL_ERR_CORRUPTION;
+        if (!width[u.frame]) width[u.frame] = out.width;
+        u.outOff = frameOut[u.frame] + got[u.frame];
+        got[u.frame] += out.bytes;
+        if (got[u.frame] > j.P.size) return OZL_ERR_CORRUPTION;
+        if (j.P.flags & 1) {
             const uint8_t* p = out.dev ? out.dev : j.dFrame;
             cJobs.insert(cJobs.end(), { (uint64_t)(uintptr_t)p, out.bytes, cSum });
-            cIdx.push_back(i);
+            cIdx.push_back(k);
             cSum += ozl_xxh3_batch_sums(out.bytes);
             cMax = out.bytes > cMax ? out.bytes : cMax;
         }
     }
-    if (total > cap) return OZL_ERR_DSTCAPACITY_TOOSMALL;
+    for (size_t i = 0; i < n; i++)
+        if (got[i] != J[i].P.size) return OZL_ERR_CORRUPTION;
+    if (frameOut[n] > cap) return OZL_ERR_DSTCAPACITY_TOOSMALL;
     std::vector<uint64_t> hC(cIdx.size()), hK(kIdx.size());
     if (!cIdx.empty()) {
         uint64_t* dJobs = (uint64_t*)dc->alloc(cJobs.size() * 8 + 16);
@@ -998,34 +1103,31 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         OZL_HIP_CHECK(hipStreamWaitEvent(st, dc->evHash, 0));
         OZL_HIP_CHECK(hipMemcpyAsync(hK.data(), dHK, kIdx.size() * 8, hipMemcpyDeviceToHost, st));
     }
-    const bool devOut = total && ozl_is_device_ptr(dst);
-    for (size_t i = 0; i < n; i++) {
-        const DStream& out = J[i].D->S[0];
-        if (!out.bytes || out.dev == (uint8_t*)dst + outOff[i]) continue;
-        OZL_HIP_CHECK(hipMemcpyAsync((uint8_t*)dst + outOff[i], out.dev, out.bytes,
-                                     devOut ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+    const bool dstDev = frameOut[n] && ozl_is_device_ptr(dst);
+    for (ChunkJob& u : U) {
+        const DStream& out = u.D->S[0];
+        if (!out.bytes || out.dev == (uint8_t*)dst + u.outOff) continue;
+        OZL_HIP_CHECK(hipMemcpyAsync((uint8_t*)dst + u.outOff, out.dev, out.bytes,
+                                     dstDev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     }
-    if (devIn) OZL_HIP_CHECK(hipMemcpyAsync(tails.data(), dTails, n * 16, hipMemcpyDeviceToHost, st));
+    if (devIn && !U.empty()) OZL_HIP_CHECK(hipMemcpyAsync(tails.data(), dTails, U.size() * 8, hipMemcpyDeviceToHost, st));
     int rc = Q.check_statuses();   // synchronises the stream
     if (rc) return rc;
     if (devIn)
-        for (size_t i = 0; i < n; i++) memcpy(J[i].tail, tails.data() + i * 16, J[i].P.end + 1 - J[i].P.csumPos);
-    for (size_t i = 0; i < n; i++)
-        if (J[i].tail[J[i].P.end - J[i].P.csumPos] != 0) return OZL_ERR_CORRUPTION;
+        for (size_t k = 0; k < U.size(); k++) memcpy(U[k].tail, tails.data() + k * 8, U[k].C->end - U[k].C->csumPos);
     for (size_t k = 0; k < kIdx.size(); k++) {
-        const FrameJob& j = J[kIdx[k]];
-        if ((uint32_t)hK[k] != ozl_le32(j.tail + ((j.P.flags & 1) ? 4 : 0))) return OZL_ERR_COMPRESSED_CHECKSUM;
+        const ChunkJob& u = U[kIdx[k]];
+        if ((uint32_t)hK[k] != ozl_le32(u.tail + ((J[u.frame].P.flags & 1) ? 4 : 0))) return OZL_ERR_COMPRESSED_CHECKSUM;
     }
     for (size_t k = 0; k < cIdx.size(); k++)
-        if ((uint32_t)hC[k] != ozl_le32(J[cIdx[k]].tail)) return OZL_ERR_CONTENT_CHECKSUM;
+        if ((uint32_t)hC[k] != ozl_le32(U[cIdx[k]].tail)) return OZL_ERR_CONTENT_CHECKSUM;
     for (size_t i = 0; i < n; i++) {
-        const DStream& out = J[i].D->S[0];
-        outSizes[i] = out.bytes;
+        outSizes[i] = J[i].P.size;
         if (infos) {
             infos[i].type = (ZL_Type)J[i].wantType;
-            infos[i].fixedWidth = (uint32_t)(J[i].wantType == ZL_Type_serial ? 1 : out.width);
-            infos[i].decompressedByteSize = out.bytes;
-            infos[i].numElts = out.bytes / infos[i].fixedWidth;
+            infos[i].fixedWidth = (uint32_t)(J[i].wantType == ZL_Type_serial || !width[i] ? 1 : width[i]);
+            infos[i].decompressedByteSize = J[i].P.size;
+            infos[i].numElts = J[i].P.size / infos[i].fixedWidth;
         }
     }
     return 0;

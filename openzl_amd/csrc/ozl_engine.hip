@@ -45,7 +45,7 @@ enum { NID_DELTA = 2, NID_TRANSPOSE_SPLIT = 3, NID_ZIGZAG = 4, NID_CONV_STRUCT_T
        NID_CONV_NUM_TO_STRUCT = 11, NID_CONV_NUM_TO_SERIAL = 12 };
 enum { GID_SERIAL_STORE = 1, GID_STORE = 2, GID_FSE = 3, GID_HUF = 4, GID_CONSTANT = 6, GID_BITPACK = 8,
        GID_FLATPACK = 9, GID_CUSTOM_BASE = 4096 };
-enum { GK_STATIC, GK_TRANSPOSE, GK_FUNCTION, GK_BRUTE };
+enum { GK_STATIC, GK_TRANSPOSE, GK_FUNCTION, GK_BRUTE, GK_SEGMENTER };
 enum { T_SERIAL = 1, T_STRUCT = 2, T_NUMERIC = 4 };
 
 ZL_Report rep_ok(size_t v)
@@ -69,9 +69,11 @@ int frame_type(int t) { return t == T_SERIAL ? OZL_T_SERIAL : t == T_STRUCT ? OZ
 struct GDef {
     int kind;
     unsigned node;
-    std::vector<unsigned> succ;
+    std::vector<unsigned> succ;   // successors; a segmenter's custom graphs
     ZL_FunctionGraphFn fn;
     int mask;
+    ZL_SegmenterFn segFn = nullptr;
+    const void* opaque = nullptr;
 };
 
 // grow-only device arena, reset per compression
@@ -134,6 +136,12 @@ struct ENode {
 }  // namespace
 
 struct ZL_Compressor_s {
+    std::vector<ZL_OpaquePtr> opaques;   // owned (segmenter descriptors), freed with the compressor
+    ~ZL_Compressor_s()
+    {
+        for (const ZL_OpaquePtr& o : opaques)
+            if (o.freeFn) o.freeFn(o.freeOpaquePtr, o.ptr);
+    }
     int formatVersion = 0;
     int contentChecksum = ZL_TernaryParam_auto;
     int compressedChecksum = ZL_TernaryParam_auto;
@@ -896,7 +904,7 @@ FrameParams resolve(const ZL_CCtx_s* c)
 // Serialises the session's final graph; returns the frame header + chunk
 // header bytes and the stored streams (device) in frame order.
 int build_headers(Session& S, int inType, size_t inBytes, const FrameParams& fp, std::vector<uint8_t>& head,
-                  std::vector<const uint8_t*>& srcs, std::vector<uint64_t>& sizes)
+                  std::vector<const uint8_t*>& srcs, std::vector<uint64_t>& sizes, bool frameHeader = true)
 {
     std::vector<ozl_fnode> fn(S.nodes.size());
     for (size_t k = 0; k < S.nodes.size(); k++) {
@@ -920,7 +928,9 @@ int build_headers(Session& S, int inType, size_t inBytes, const FrameParams& fp,
     std::vector<int> order(S.streams.size());
     size_t bound = 64 + S.nodes.size() * 24 + S.streams.size() * 10;
     head.assign(bound, 0);
-    size_t fh = ozl_write_frame_header(head.data(), inType, inBytes, fp.contentChecksum, fp.compressedChecksum);
+    size_t fh = frameHeader ? ozl_write_frame_header(head.data(), inType, inBytes, fp.contentChecksum,
+                                                     fp.compressedChecksum)
+                            : 0;
     int nbStored = 0;
     size_t ch = ozl_write_chunk_header(head.data() + fh, fn.data(), (int)fn.size(), fs.data(), (int)fs.size(),
                                        order.data(), &nbStored);
@@ -968,6 +978,40 @@ int Session::runBrute(const GDef* g, uint32_t sid)
 
 namespace {
 
+// One chunk of a frame: its graph walk and its input bytes (device).
+struct ChunkUnit {
+    size_t frame = 0;
+    std::unique_ptr<Session> S;
+    const uint8_t* dIn = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+// A segmenter's view of one input (segmenter.c): the remaining input, and
+// chunks cut from its front, each walked by a session that defers its
+// entropy calls to the batch of the whole call.
+struct ZL_Segmenter_s {
+    const GDef* g = nullptr;
+    const ZL_Compressor_s* comp = nullptr;
+    FrameParams fp;
+    DevArena* arena = nullptr;
+    hipStream_t st = nullptr;
+    EntBatch* eb = nullptr;
+    std::vector<ChunkUnit>* units = nullptr;
+    size_t frame = 0;
+    int type = 0, width = 1;
+    const uint8_t* dIn = nullptr;
+    size_t n = 0, consumed = 0;
+    int err = 0;
+    Session view;          // one stream: the remaining input
+    ZL_Input_s input;
+    std::vector<std::vector<uint8_t>> scratch;
+    std::vector<ZL_GraphID> custom;
+};
+
+namespace {
+
 // Compresses n inputs (host or device memory) into n frames written back to
 // back at dst (host or device memory): each input's graph walk, with its
 // entropy calls deferred; one plan / encode launch per coder for all frames;
@@ -1004,7 +1048,7 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
     // the kernels read the caller's buffers
     OzlStreamsDone done{ st, c->st2, c->st3 };
     EntBatch eb;
-    std::vector<std::unique_ptr<Session>> S(n);
+    const int cc = fp.contentChecksum, kc = fp.compressedChecksum;
     std::vector<const uint8_t*> dIn(n);
     std::vector<size_t> inBytes(n);
     for (size_t i = 0; i < n; i++) {
@@ -1020,113 +1064,160 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
             dIn[i] = d;
         }
     }
-    // content checksums (they read only the inputs) on the second stream
-    const int cc = fp.contentChecksum, kc = fp.compressedChecksum;
-    std::vector<uint64_t> jobC(3 * n);
-    size_t sumC = 0, maxC = 0;
-    for (size_t i = 0; i < n; i++) {
-        jobC[3 * i] = (uint64_t)(uintptr_t)dIn[i];
-        jobC[3 * i + 1] = inBytes[i];
-        jobC[3 * i + 2] = sumC;
-        sumC += ozl_xxh3_batch_sums(inBytes[i]);
-        maxC = inBytes[i] > maxC ? inBytes[i] : maxC;
-    }
-    uint64_t* dJobC = (uint64_t*)c->arena.alloc(jobC.size() * 8 + 16);
-    uint64_t* dSumC = (uint64_t*)c->arena.alloc(sumC * 8 + 16);
-    uint64_t* dHashC = (uint64_t*)c->arena.alloc(n * 8 + 16);
-    if (!dJobC || !dSumC || !dHashC) return OZL_ERR_ALLOCATION;
-    if (cc) {
-        const uint8_t* h = c->stage.put(jobC.data(), jobC.size() * 8);
-        if (!h) return OZL_ERR_ALLOCATION;
-        OZL_HIP_CHECK(hipMemcpyAsync(dJobC, h, jobC.size() * 8, hipMemcpyHostToDevice, st));
-        OZL_HIP_CHECK(hipEventRecord(c->evIn, st));
-        OZL_HIP_CHECK(hipStreamWaitEvent(c->st2, c->evIn, 0));
-        int rc = ozl_xxh3_device_batch((const ozl_xjob*)dJobC, (uint32_t)n, maxC, dSumC, dHashC, c->st2);
-        if (rc) return rc;
-        OZL_HIP_CHECK(hipEventRecord(c->evHash, c->st2));
-    }
+    // the graph walks: one chunk per input, or the chunks its segmenter cuts
+    std::vector<ChunkUnit> units;
+    const unsigned start = c->comp->start ? c->comp->start : GID_STORE;
+    const GDef* sg = start >= GID_CUSTOM_BASE && start - GID_CUSTOM_BASE < c->comp->graphs.size()
+                         ? &c->comp->graphs[start - GID_CUSTOM_BASE]
+                         : nullptr;
     for (size_t i = 0; i < n; i++) {
         const ZL_TypedRef_s* in = ins[i];
-        S[i].reset(new Session());
-        Session& T = *S[i];
+        if (sg && sg->kind == GK_SEGMENTER) {
+            // cctx.c:843-918: no conversion in front of a segmenter
+            if (!(in->type & sg->mask)) return ZL_ErrorCode_temporaryLibraryLimitation;
+            ZL_Segmenter_s seg;
+            seg.g = sg;
+            seg.comp = c->comp;
+            seg.fp = fp;
+            seg.arena = &c->arena;
+            seg.st = st;
+            seg.eb = &eb;
+            seg.units = &units;
+            seg.frame = i;
+            seg.type = in->type;
+            seg.width = (int)in->width;
+            seg.dIn = dIn[i];
+            seg.n = in->n;
+            ZL_Report r = sg->segFn(&seg);
+            if (ZL_isError(r)) return seg.err ? seg.err : (int)ZL_errorCode(r);
+            if (seg.consumed < seg.n) return ZL_ErrorCode_segmenter_inputNotConsumed;
+            continue;
+        }
+        ChunkUnit u;
+        u.frame = i;
+        u.dIn = dIn[i];
+        u.bytes = inBytes[i];
+        u.S.reset(new Session());
+        Session& T = *u.S;
         T.comp = c->comp;
         T.arena = &c->arena;
         T.st = st;
         T.minStreamSize = fp.minStreamSize;
         T.batch = &eb;
         T.newStream(in->type, (int)in->width, in->n, (uint8_t*)dIn[i]);
-        int rc = T.runSuccessor(c->comp->start ? c->comp->start : GID_STORE, 0);
+        int rc = T.runSuccessor(start, 0);
         if (!rc) rc = T.err;
         if (rc) return rc;
+        units.push_back(std::move(u));
+    }
+    const size_t nu = units.size();
+    // content checksums of the chunks (they read only the inputs) on the
+    // second stream, alongside the entropy stages
+    std::vector<uint64_t> jobC(3 * nu + 3);
+    size_t sumC = 0, maxC = 0;
+    for (size_t k = 0; k < nu; k++) {
+        jobC[3 * k] = (uint64_t)(uintptr_t)units[k].dIn;
+        jobC[3 * k + 1] = units[k].bytes;
+        jobC[3 * k + 2] = sumC;
+        sumC += ozl_xxh3_batch_sums(units[k].bytes);
+        maxC = units[k].bytes > maxC ? units[k].bytes : maxC;
+    }
+    uint64_t* dJobC = (uint64_t*)c->arena.alloc(jobC.size() * 8 + 16);
+    uint64_t* dSumC = (uint64_t*)c->arena.alloc(sumC * 8 + 16);
+    uint64_t* dHashC = (uint64_t*)c->arena.alloc(nu * 8 + 16);
+    if (!dJobC || !dSumC || !dHashC) return OZL_ERR_ALLOCATION;
+    if (cc && nu) {
+        const uint8_t* h = c->stage.put(jobC.data(), jobC.size() * 8);
+        if (!h) return OZL_ERR_ALLOCATION;
+        OZL_HIP_CHECK(hipMemcpyAsync(dJobC, h, jobC.size() * 8, hipMemcpyHostToDevice, st));
+        OZL_HIP_CHECK(hipEventRecord(c->evIn, st));
+        OZL_HIP_CHECK(hipStreamWaitEvent(c->st2, c->evIn, 0));
+        int rc = ozl_xxh3_device_batch((const ozl_xjob*)dJobC, (uint32_t)nu, maxC, dSumC, dHashC, c->st2);
+        if (rc) return rc;
+        OZL_HIP_CHECK(hipEventRecord(c->evHash, c->st2));
     }
     int rc = eb.run(&c->arena, &c->stage, st);
     if (!rc) rc = eb.expand();
     if (rc) return rc;
-    // headers and stored streams of every frame
-    std::vector<uint8_t> heads;
-    std::vector<uint64_t> gSrc, gSize, gOff;   // gather items: headers, then stored streams
-    std::vector<uint64_t> fOff(n + 1), fh(n), body(n), headItem(n);
-    size_t total = 0;
-    std::vector<std::vector<uint8_t>> hd(n);
-    std::vector<std::vector<const uint8_t*>> hs(n);
-    std::vector<std::vector<uint64_t>> hz(n);
-    std::vector<int> hrc(n, 0);
-    ozl_parallel_for(n, [&](size_t i) {
-        S[i]->finalize();
-        const int ft = frame_type(ins[i]->type);
-        hrc[i] = build_headers(*S[i], ft, inBytes[i], fp, hd[i], hs[i], hz[i]);
-        if (!hrc[i]) fh[i] = ozl_write_frame_header(hd[i].data(), ft, inBytes[i], cc, kc);
+    // chunk headers (host threads, one chunk each)
+    std::vector<std::vector<uint8_t>> hd(nu);
+    std::vector<std::vector<const uint8_t*>> hs(nu);
+    std::vector<std::vector<uint64_t>> hz(nu);
+    std::vector<int> hrc(nu, 0);
+    ozl_parallel_for(nu, [&](size_t k) {
+        ChunkUnit& u = units[k];
+        u.S->finalize();
+        hrc[k] = build_headers(*u.S, frame_type(ins[u.frame]->type), u.bytes, fp, hd[k], hs[k], hz[k], false);
     });
+    for (int e : hrc)
+        if (e) return e;
+    // layout: per frame, its header, its chunks (header, stored streams,
+    // checksums), the end marker
+    std::vector<uint8_t> heads;
+    std::vector<uint64_t> gSrc, gSize, gOff, headItems;   // gather items; header items take offsets into heads
+    std::vector<uint64_t> chunkStart(nu), body(nu), endAt(n);
+    size_t total = 0, k = 0;
     for (size_t i = 0; i < n; i++) {
-        if (hrc[i]) return hrc[i];
-        const std::vector<uint8_t>& head = hd[i];
-        const std::vector<const uint8_t*>& srcs = hs[i];
-        const std::vector<uint64_t>& ssz = hz[i];
-        fOff[i] = total;
-        headItem[i] = gSrc.size();
-        gSrc.push_back(heads.size());   // rebased once the headers are on the device
-        gSize.push_back(head.size());
-        gOff.push_back(total);
-        size_t pos = total + head.size();
-        heads.insert(heads.end(), head.begin(), head.end());
-        for (size_t k = 0; k < srcs.size(); k++) {
-            if (ssz[k]) {
-                gSrc.push_back((uint64_t)(uintptr_t)srcs[k]);
-                gSize.push_back(ssz[k]);
-                gOff.push_back(pos);
+        uint8_t fhb[64];
+        const size_t fhl = ozl_write_frame_header(fhb, frame_type(ins[i]->type), inBytes[i], cc, kc);
+        const size_t f0 = total;
+        size_t pos = total;
+        headItems.push_back(gSrc.size());
+        gSrc.push_back(heads.size());
+        gSize.push_back(fhl);
+        gOff.push_back(pos);
+        heads.insert(heads.end(), fhb, fhb + fhl);
+        pos += fhl;
+        for (; k < nu && units[k].frame == i; k++) {
+            chunkStart[k] = pos;
+            headItems.push_back(gSrc.size());
+            gSrc.push_back(heads.size());
+            gSize.push_back(hd[k].size());
+            gOff.push_back(pos);
+            heads.insert(heads.end(), hd[k].begin(), hd[k].end());
+            pos += hd[k].size();
+            for (size_t q = 0; q < hs[k].size(); q++) {
+                if (hz[k][q]) {
+                    gSrc.push_back((uint64_t)(uintptr_t)hs[k][q]);
+                    gSize.push_back(hz[k][q]);
+                    gOff.push_back(pos);
+                }
+                pos += hz[k][q];
             }
-            pos += ssz[k];
+            body[k] = pos;   // checksums from here
+            pos += (cc ? 4 : 0) + (kc ? 4 : 0);
         }
-        body[i] = pos;   // checksums from here
-        sizes[i] = pos - total + (cc ? 4 : 0) + (kc ? 4 : 0) + 1;
-        total += sizes[i];
+        endAt[i] = pos;
+        sizes[i] = pos + 1 - f0;
+        total = pos + 1;
     }
-    fOff[n] = total;
     if (total > cap) return ZL_ErrorCode_dstCapacity_tooSmall;
     const bool devOut = ozl_is_device_ptr(dst);
     uint8_t* dF = devOut ? (uint8_t*)dst : c->arena.alloc(total + 16);
     uint8_t* dHead = c->arena.alloc(heads.size() + 16);
-    // device metadata: gather arrays, checksum jobs and tail positions
+    // device metadata: gather arrays, compressed-checksum jobs, tail positions
     const size_t G = gSrc.size();
-    std::vector<uint64_t> meta(3 * G + 3 * n + 2 * n);
+    std::vector<uint64_t> meta(3 * G + 3 * nu + 2 * nu + n + 3);
     size_t sumWords = 0, maxLen = 0;
     uint64_t* mJobK = meta.data() + 3 * G;
-    uint64_t* mAtC = mJobK + 3 * n;
-    uint64_t* mAtK = mAtC + n;
-    for (size_t i = 0; i < n; i++) {
-        const uint64_t k0 = fOff[i] + fh[i], k1 = body[i] + (cc ? 4 : 0);
-        mJobK[3 * i] = 0;   // rebased below (frame buffer address)
-        mJobK[3 * i + 1] = k1 - k0;
-        mJobK[3 * i + 2] = sumWords;
+    uint64_t* mAtC = mJobK + 3 * nu;
+    uint64_t* mAtK = mAtC + nu;
+    uint64_t* mAtE = mAtK + nu;
+    for (size_t q = 0; q < nu; q++) {
+        const uint64_t k0 = chunkStart[q], k1 = body[q] + (cc ? 4 : 0);
+        mJobK[3 * q] = (uint64_t)(uintptr_t)(dF + k0);
+        mJobK[3 * q + 1] = k1 - k0;
+        mJobK[3 * q + 2] = sumWords;
         sumWords += ozl_xxh3_batch_sums(k1 - k0);
         maxLen = k1 - k0 > maxLen ? k1 - k0 : maxLen;
-        mAtC[i] = body[i];
-        mAtK[i] = body[i] + (cc ? 4 : 0);
+        mAtC[q] = body[q];
+        mAtK[q] = body[q] + (cc ? 4 : 0);
     }
+    for (size_t i = 0; i < n; i++) mAtE[i] = endAt[i];
     uint64_t* dMeta = (uint64_t*)c->arena.alloc(meta.size() * 8 + 16);
     uint64_t* dSums = (uint64_t*)c->arena.alloc(sumWords * 8 + 16);
-    uint64_t* dHash = (uint64_t*)c->arena.alloc(2 * n * 8 + 16);
-    if (!dF || !dHead || !dMeta || !dSums || !dHash) return OZL_ERR_ALLOCATION;
+    uint64_t* dHashK = (uint64_t*)c->arena.alloc(nu * 8 + 16);
+    if (!dF || !dHead || !dMeta || !dSums || !dHashK) return OZL_ERR_ALLOCATION;
     size_t maxItem = 0;
     for (size_t g = 0; g < G; g++) {
         meta[g] = gSrc[g];
@@ -1135,8 +1226,7 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
         maxItem = gSize[g] > maxItem ? gSize[g] : maxItem;
     }
     // header items: offsets into `heads` become device addresses
-    for (size_t i = 0; i < n; i++) meta[headItem[i]] = (uint64_t)(uintptr_t)dHead + gSrc[headItem[i]];
-    for (size_t i = 0; i < n; i++) mJobK[3 * i] = (uint64_t)(uintptr_t)(dF + fOff[i] + fh[i]);
+    for (uint64_t h : headItems) meta[h] = (uint64_t)(uintptr_t)dHead + gSrc[h];
     const uint8_t* hHead = c->stage.put(heads.data(), heads.size());
     const uint8_t* hMeta = c->stage.put(meta.data(), meta.size() * 8);
     if (!hHead || !hMeta) return OZL_ERR_ALLOCATION;
@@ -1145,22 +1235,21 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
     rc = ozl_gather_sized((const uint8_t* const*)dMeta, dMeta + G, dMeta + 2 * G, (int)G, maxItem, dF, st);
     if (rc) return rc;
     const ozl_xjob* dJobK = (const ozl_xjob*)(dMeta + 3 * G);
-    const uint64_t* dAtC = dMeta + 3 * G + 3 * n;
-    const uint64_t* dAtK = dAtC + n;
-    if (cc) {
+    const uint64_t* dAtC = dMeta + 3 * G + 3 * nu;
+    const uint64_t* dAtK = dAtC + nu;
+    const uint64_t* dAtE = dAtK + nu;
+    if (cc && nu) {
         OZL_HIP_CHECK(hipStreamWaitEvent(st, c->evHash, 0));
-        rc = ozl_put_tails(dF, dAtC, dHashC, (uint32_t)n, !kc, st);
+        rc = ozl_put_tails(dF, dAtC, dHashC, (uint32_t)nu, 0, st);
         if (rc) return rc;
     }
-    if (kc) {
-        rc = ozl_xxh3_device_batch(dJobK, (uint32_t)n, maxLen, dSums, dHash + n, st);
-        if (!rc) rc = ozl_put_tails(dF, dAtK, dHash + n, (uint32_t)n, 1, st);
+    if (kc && nu) {
+        rc = ozl_xxh3_device_batch(dJobK, (uint32_t)nu, maxLen, dSums, dHashK, st);
+        if (!rc) rc = ozl_put_tails(dF, dAtK, dHashK, (uint32_t)nu, 0, st);
         if (rc) return rc;
     }
-    if (!cc && !kc) {
-        rc = ozl_put_tails(dF, dAtC, nullptr, (uint32_t)n, 1, st);
-        if (rc) return rc;
-    }
+    rc = ozl_put_tails(dF, dAtE, nullptr, (uint32_t)n, 1, st);
+    if (rc) return rc;
     if (!devOut && total) OZL_HIP_CHECK(hipMemcpyAsync(dst, dF, total, hipMemcpyDeviceToHost, st));
     OZL_HIP_CHECK(hipStreamSynchronize(st));
     return 0;
@@ -1257,6 +1346,113 @@ ZL_GraphID ZL_Compressor_registerBruteForceSelectorGraph(ZL_Compressor* c, const
     if (!c || nb == 0) return ZL_GRAPH_ILLEGAL;
     GDef g{ GK_BRUTE, 0, {}, nullptr, T_SERIAL | T_STRUCT | T_NUMERIC };
     for (size_t i = 0; i < nb; i++) g.succ.push_back(succ[i].gid);
+    return add_graph(c, g);
+}
+
+const void* ZL_Segmenter_getOpaquePtr(const ZL_Segmenter* s) { return s ? s->g->opaque : nullptr; }
+
+int ZL_Segmenter_getCParam(const ZL_Segmenter* s, ZL_CParam p)
+{
+    if (!s) return 0;
+    switch (p) {
+    case ZL_CParam_formatVersion: return s->fp.formatVersion;
+    case ZL_CParam_contentChecksum: return s->fp.contentChecksum ? ZL_TernaryParam_enable : ZL_TernaryParam_disable;
+    case ZL_CParam_compressedChecksum: return s->fp.compressedChecksum ? ZL_TernaryParam_enable : ZL_TernaryParam_disable;
+    case ZL_CParam_minStreamSize: return s->fp.minStreamSize;
+    default: return 0;
+    }
+}
+
+ZL_GraphIDList ZL_Segmenter_getCustomGraphs(const ZL_Segmenter* s)
+{
+    ZL_GraphIDList l = { nullptr, 0 };
+    if (!s) return l;
+    ZL_Segmenter* m = const_cast<ZL_Segmenter*>(s);
+    if (m->custom.empty())
+        for (unsigned gid : s->g->succ) m->custom.push_back(ZL_GraphID{ gid });
+    l.graphids = m->custom.data();
+    l.nbGraphIDs = m->custom.size();
+    return l;
+}
+
+size_t ZL_Segmenter_numInputs(const ZL_Segmenter* s) { return s ? 1 : 0; }
+
+const ZL_Input* ZL_Segmenter_getInput(const ZL_Segmenter* s, size_t inputID)
+{
+    if (!s || inputID != 0) return nullptr;
+    ZL_Segmenter* m = const_cast<ZL_Segmenter*>(s);
+    m->view.streams.clear();
+    m->view.streams.push_back({ s->type, s->width, s->n - s->consumed,
+                                const_cast<uint8_t*>(s->dIn) + s->consumed * (size_t)s->width, false, 0 });
+    m->input.s = &m->view;
+    m->input.sid = 0;
+    return &m->input;
+}
+
+ZL_Report ZL_Segmenter_getNumElts(const ZL_Segmenter* s, size_t numElts[], size_t numInputs)
+{
+    if (!s || !numElts) return rep_err(ZL_ErrorCode_parameter_invalid);
+    if (numInputs != 1) return rep_err(ZL_ErrorCode_graph_invalidNumInputs);
+    numElts[0] = s->n - s->consumed;
+    return rep_ok(0);
+}
+
+void* ZL_Segmenter_getScratchSpace(ZL_Segmenter* s, size_t size)
+{
+    if (!s) return nullptr;
+    s->scratch.emplace_back(size ? size : 1);
+    return s->scratch.back().data();
+}
+
+// segmenter.c:262-337: the next numElts[0] elements become a chunk, walked
+// by startingGraphID (a segmenter cannot be a chunk's graph).
+ZL_Report ZL_Segmenter_processChunk(ZL_Segmenter* s, const size_t numElts[], size_t numInputs,
+                                    ZL_GraphID startingGraphID, const void* rGraphParams)
+{
+    if (!s || !numElts) return rep_err(ZL_ErrorCode_parameter_invalid);
+    if (numInputs != 1) return rep_err(ZL_ErrorCode_graph_invalidNumInputs);
+    if (rGraphParams) return rep_err(ZL_ErrorCode_temporaryLibraryLimitation);
+    if (numElts[0] > s->n - s->consumed) return rep_err(ZL_ErrorCode_parameter_invalid);
+    const unsigned gid = startingGraphID.gid;
+    if (gid >= GID_CUSTOM_BASE && gid - GID_CUSTOM_BASE < s->comp->graphs.size() &&
+        s->comp->graphs[gid - GID_CUSTOM_BASE].kind == GK_SEGMENTER)
+        return rep_err(ZL_ErrorCode_graph_invalid);
+    ChunkUnit u;
+    u.frame = s->frame;
+    u.dIn = s->dIn + s->consumed * (size_t)s->width;
+    u.bytes = numElts[0] * (size_t)s->width;
+    u.S.reset(new Session());
+    Session& T = *u.S;
+    T.comp = s->comp;
+    T.arena = s->arena;
+    T.st = s->st;
+    T.minStreamSize = s->fp.minStreamSize;
+    T.batch = s->eb;
+    T.newStream(s->type, s->width, numElts[0], const_cast<uint8_t*>(u.dIn));
+    s->consumed += numElts[0];
+    int rc = T.runSuccessor(gid, 0);
+    if (!rc) rc = T.err;
+    if (rc) {
+        s->err = rc;
+        return rep_err(rc);
+    }
+    s->units->push_back(std::move(u));
+    return rep_ok(0);
+}
+
+ZL_GraphID ZL_Compressor_registerSegmenter(ZL_Compressor* c, const ZL_SegmenterDesc* d)
+{
+    if (c && d && d->opaque.freeFn) c->opaques.push_back(d->opaque);   // owned even when registration fails
+    if (!c || !d || !d->segmenterFn || d->numInputs != 1 || !d->inputTypeMasks) return ZL_GraphID{ 0 };
+    if (d->numCustomGraphs && !d->customGraphs) return ZL_GraphID{ 0 };
+    GDef g;
+    g.kind = GK_SEGMENTER;
+    g.node = 0;
+    for (size_t i = 0; i < d->numCustomGraphs; i++) g.succ.push_back(d->customGraphs[i].gid);
+    g.fn = nullptr;
+    g.mask = (int)d->inputTypeMasks[0];
+    g.segFn = d->segmenterFn;
+    g.opaque = d->opaque.ptr;
     return add_graph(c, g);
 }
 

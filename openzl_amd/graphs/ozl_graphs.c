@@ -6,6 +6,8 @@
  * libozl_hip.so, so every codec they select runs on the GPU.
  */
 #include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
 
 #include "openzl_amd.h"
 
@@ -46,4 +48,59 @@ ZL_GraphID ozl_graph_weights_split(ZL_Compressor* compressor)
     d.inputTypeMasks = kStructMask;
     d.nbInputs = 1;
     return ZL_Compressor_registerFunctionGraph(compressor, &d);
+}
+
+/* Fixed-size segmenter (include/openzl/zl_segmenter.h): the input is cut
+ * into chunks of chunkElts elements (the last one shorter), each compressed
+ * by `inner` into its own chunk of the frame. The chunk size travels in the
+ * segmenter's opaque pointer, freed with the compressor. */
+static ZL_Report fixed_chunks_fn(ZL_Segmenter* sctx)
+{
+    const size_t per = *(const size_t*)ZL_Segmenter_getOpaquePtr(sctx);
+    ZL_GraphIDList gl = ZL_Segmenter_getCustomGraphs(sctx);
+    ZL_Report rep;
+    size_t left = 0;
+    if (gl.nbGraphIDs != 1 || per == 0) {
+        rep._code = ZL_ErrorCode_graph_invalid;
+        return rep;
+    }
+    rep = ZL_Segmenter_getNumElts(sctx, &left, 1);
+    if (rep._code != ZL_ErrorCode_no_error) return rep;
+    while (left > 0) {
+        size_t n = left < per ? left : per;
+        rep = ZL_Segmenter_processChunk(sctx, &n, 1, gl.graphids[0], NULL);
+        if (rep._code != ZL_ErrorCode_no_error) return rep;
+        left -= n;
+    }
+    rep._code = ZL_ErrorCode_no_error;
+    rep._value._value = 0;
+    return rep;
+}
+
+static void free_chunk_size(void* unused, void* p)
+{
+    (void)unused;
+    free(p);
+}
+
+static const ZL_Type kAnyMask[1] = { (ZL_Type)(ZL_Type_serial | ZL_Type_struct | ZL_Type_numeric) };
+
+ZL_GraphID ozl_graph_fixed_chunks(ZL_Compressor* compressor, ZL_GraphID inner, size_t chunkElts)
+{
+    ZL_SegmenterDesc d;
+    size_t* per = (size_t*)malloc(sizeof(size_t));
+    ZL_GraphID bad = { 0 };
+    if (!per) return bad;
+    *per = chunkElts;
+    memset(&d, 0, sizeof(d));
+    d.name = "fixed_chunks";
+    d.segmenterFn = fixed_chunks_fn;
+    d.inputTypeMasks = kAnyMask;
+    d.numInputs = 1;
+    d.customGraphs = &inner;
+    d.numCustomGraphs = 1;
+    d.opaque.ptr = per;
+    d.opaque.freeOpaquePtr = NULL;
+    d.opaque.freeFn = free_chunk_size;
+    return ZL_Compressor_registerSegmenter(compressor, &d);
 }

@@ -24,6 +24,7 @@
 #include "openzl/codecs/zl_zigzag.h"
 #include "openzl/openzl.h"
 #include "openzl/zl_graph_api.h"
+#include "openzl/zl_segmenter.h"
 
 /* Graph kinds (mirrored by openzl_amd.graphs on the Python side). */
 enum {
@@ -40,6 +41,9 @@ enum {
     REFH_ZZ_BP          = 10, /* ZIGZAG -> BITPACK                               */
     REFH_DELTA_BP       = 11, /* DELTA_INT -> BITPACK                            */
     REFH_SELECT8        = 12, /* brute-force selector over the 8 candidates of   */
+    REFH_SEGMENTED      = 13, /* segmenter (include/openzl/zl_segmenter.h): chunks of */
+                              /* refh_set_segmenter's element count, each through  */
+                              /* the graph of refh_set_segmenter's kind             */
                               /* the learned-selector batch (see buildGraph)     */
 };
 
@@ -51,6 +55,49 @@ static ZL_Report splitHufFse(ZL_Graph* g, ZL_Edge* inputs[], size_t nbIns)
     ZL_RET_R_IF_ERR(ZL_Edge_setDestination(planes.edges[0], ZL_GRAPH_FSE));
     ZL_RET_R_IF_ERR(ZL_Edge_setDestination(planes.edges[1], ZL_GRAPH_HUFFMAN));
     return ZL_returnSuccess();
+}
+
+/* segmenter: fixed chunk size, one graph for every chunk (its GraphID is the
+ * segmenter's only custom graph) */
+static int g_segKind = REFH_FSE;
+static size_t g_segElts = 1u << 16;
+
+void refh_set_segmenter(int kind, size_t chunkElts)
+{
+    g_segKind = kind;
+    g_segElts = chunkElts ? chunkElts : 1;
+}
+
+static ZL_Report fixedChunks(ZL_Segmenter* sctx)
+{
+    ZL_GraphIDList gl = ZL_Segmenter_getCustomGraphs(sctx);
+    if (gl.nbGraphIDs != 1) return ZL_REPORT_ERROR(GENERIC);
+    size_t left = 0;
+    ZL_RET_R_IF_ERR(ZL_Segmenter_getNumElts(sctx, &left, 1));
+    while (left > 0) {
+        size_t n = left < g_segElts ? left : g_segElts;
+        ZL_RET_R_IF_ERR(ZL_Segmenter_processChunk(sctx, &n, 1, gl.graphids[0], NULL));
+        left -= n;
+    }
+    return ZL_returnSuccess();
+}
+
+static ZL_GraphID buildGraph(ZL_Compressor* c, int kind);
+
+static ZL_GraphID buildSegmenter(ZL_Compressor* c)
+{
+    static const ZL_Type anyType = (ZL_Type)(ZL_Type_serial | ZL_Type_struct | ZL_Type_numeric);
+    ZL_GraphID inner = buildGraph(c, g_segKind);
+    if (!ZL_GraphID_isValid(inner)) return ZL_GRAPH_ILLEGAL;
+    ZL_SegmenterDesc d;
+    memset(&d, 0, sizeof(d));
+    d.name = "refh_fixed_chunks";
+    d.segmenterFn = fixedChunks;
+    d.inputTypeMasks = &anyType;
+    d.numInputs = 1;
+    d.customGraphs = &inner;
+    d.numCustomGraphs = 1;
+    return ZL_Compressor_registerSegmenter(c, &d);
 }
 
 static ZL_GraphID buildGraph(ZL_Compressor* c, int kind)
@@ -100,6 +147,7 @@ static ZL_GraphID buildGraph(ZL_Compressor* c, int kind)
         }
         return ZL_Compressor_registerBruteForceSelectorGraph(c, cand, 8);
     }
+    case REFH_SEGMENTED: return g_segKind == REFH_SEGMENTED ? ZL_GRAPH_ILLEGAL : buildSegmenter(c);
     default: return ZL_GRAPH_ILLEGAL;
     }
 }

@@ -157,7 +157,15 @@ def ptr(a):
 
 
 # graph kinds understood by refh_compress (oracle/ref_harness.c)
-STORE, FSE, HUFFMAN, BITPACK, DELTA_TS_FSE, DELTA_ZZ_BP, TS_HUF_FSE, DELTA_FSE, TS_FSE, TS_HUF, ZZ_BP, DELTA_BP, SELECT8 = range(13)
+STORE, FSE, HUFFMAN, BITPACK, DELTA_TS_FSE, DELTA_ZZ_BP, TS_HUF_FSE, DELTA_FSE, TS_FSE, TS_HUF, ZZ_BP, DELTA_BP, SELECT8, \
+    SEGMENTED = range(14)
+
+
+def ref_set_segmenter(kind, chunk_elts):
+    """The graph and chunk size (elements) of the reference harness's SEGMENTED kind."""
+    h = refh()
+    h.refh_set_segmenter.argtypes = [ctypes.c_int, ctypes.c_size_t]
+    h.refh_set_segmenter(kind, chunk_elts)
 SERIAL, STRUCT, NUMERIC = 0, 1, 2
 
 

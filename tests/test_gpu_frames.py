@@ -156,3 +156,47 @@ def test_batch_device_buffers(api):
         m = A._chk(dec.L.ZL_DCtx_decompressTyped(dec.d, ctypes.byref(info), dout.data_ptr(), total + 64,
                                                  dfr.data_ptr(), z), "decompress")
         assert dout[:m].cpu().numpy().tobytes() == arrs[-1].tobytes(), name
+
+
+def segmented_cases():
+    """(inner kind, type, width, chunk elements, array): frames of several chunks."""
+    from openzl_amd import datagen as G
+    rng = np.random.default_rng(29)
+    ts = G.timestamps_u32(300000)
+    w = G.bf16_normal(400000)
+    return [
+        (1, 0, 1, 50000, zipf_bytes(rng, 180000, 1.1)),       # FSE, 4 chunks (last shorter)
+        (2, 0, 1, 100000, zipf_bytes(rng, 100000, 1.2)),      # Huffman, exactly one chunk
+        (4, 2, 4, 120000, ts),                                # DELTA_TS_FSE on u32, 3 chunks
+        (6, 1, 2, 150000, w),                                 # transpose_split HUF/FSE on bf16
+        (1, 0, 1, 4096, rng.integers(0, 3, 10000).astype(np.uint8)),
+    ]
+
+
+def test_segmented_frames_match_reference(api):
+    """A segmenter (ZL_Compressor_registerSegmenter) cutting fixed-size
+    chunks: frames of several chunks equal the reference's, and the universal
+    decompressor reads them (single calls, batch calls, device buffers)."""
+    import torch
+    A = api
+    dec = A.Decompressor()
+    frames, arrs = [], []
+    for inner, typ, width, elts, arr in segmented_cases():
+        print(f"segmented inner={inner} elts={elts} bytes={arr.nbytes}", flush=True)
+        C.ref_set_segmenter(inner, elts)
+        ref = C.ref_compress(C.SEGMENTED, arr, TYPES[typ], width)
+        comp = A.Compressor(A.SEGMENTED, segment=(inner, elts))
+        got = comp.compress(arr, {0: A.SERIAL, 1: A.STRUCT, 2: A.NUMERIC}[typ], width)
+        assert got == ref, (inner, elts, arr.nbytes, len(got), len(ref))
+        out, _, _ = dec.decompress(ref, arr.nbytes + 64)
+        assert out == arr.tobytes(), (inner, elts)
+        frames.append(ref)
+        arrs.append(arr)
+    total = sum(a.nbytes for a in arrs)
+    outs = dec.decompress_batch(frames, total + 64)
+    assert outs == [a.tobytes() for a in arrs]
+    dfr = torch.from_numpy(np.frombuffer(b"".join(frames), np.uint8).copy()).cuda()
+    dout = torch.zeros(total + 64, dtype=torch.uint8, device="cuda")
+    sizes = dec.decompress_ptrs(dfr.data_ptr(), [len(f) for f in frames], dout.data_ptr(), total + 64)
+    assert sizes == [a.nbytes for a in arrs]
+    assert dout[:total].cpu().numpy().tobytes() == b"".join(a.tobytes() for a in arrs)
