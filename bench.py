@@ -444,6 +444,7 @@ def bench_engine(args):
     from openzl_amd import datagen as G
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import ozl_checkers as C
+    segment = None
     if args.workload == "timestamps":
         x = G.timestamps_u32()
         parts = [x]
@@ -451,7 +452,7 @@ def bench_engine(args):
         workload = ("timestamps: 4,194,304 u32 (start 1.7e9, Poisson(1000) increments, mod 2^32), one frame, "
                     "DELTA_INT -> transpose_split -> ZL_GRAPH_FSE per plane, format v23, checksums on")
         dtype = "u32"
-    else:
+    elif args.workload == "bf16":
         w = G.bf16_normal()
         seg = 8388608
         parts = [w[a:a + seg] for a in range(0, w.size, seg)]
@@ -459,13 +460,24 @@ def bench_engine(args):
         workload = ("bf16 weights: 536,870,912 bf16 N(0, 0.02) in 64 frames of 16 MiB, transpose_split -> plane 1 "
                     "ZL_GRAPH_HUFFMAN, plane 0 ZL_GRAPH_FSE, format v23, checksums on")
         dtype = "bf16"
+    else:
+        # one frame: a segmenter cuts 64 chunks of 16 MiB, each through the same graph
+        parts = [G.bf16_normal()]
+        kind, typ, width = api.SEGMENTED, api.STRUCT, 2
+        segment = (api.TS_HUF_FSE, 8388608)
+        workload = ("bf16 weights: 536,870,912 bf16 N(0, 0.02) as one frame of 64 chunks of 16 MiB (fixed-size "
+                    "segmenter), transpose_split -> plane 1 ZL_GRAPH_HUFFMAN, plane 0 ZL_GRAPH_FSE, format v23, "
+                    "checksums on")
+        dtype = "bf16"
     U = sum(p.nbytes for p in parts)
-    comp = api.Compressor(kind)
+    comp = api.Compressor(kind, segment=segment)
     dec = api.Decompressor()
     # host buffers (PCIe included): the reference frames' check and the side figure
     frames = comp.compress_batch(parts, typ, width)
     Cb = sum(len(f) for f in frames)
     if C.refh() is not None:
+        if segment:
+            C.ref_set_segmenter(*segment)
         assert frames[0] == C.ref_compress(kind, parts[0], {api.SERIAL: C.SERIAL, api.STRUCT: C.STRUCT,
                                                            api.NUMERIC: C.NUMERIC}[typ], width)
     outs = dec.decompress_batch(frames, U + 64)
@@ -561,11 +573,12 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 3 device-to-device copies of 1 GiB (HBM counter calibration) and exit")
-    ap.add_argument("--workload", choices=["entropy", "columns", "select", "timestamps", "bf16"], default="entropy",
+    ap.add_argument("--workload", choices=["entropy", "columns", "select", "timestamps", "bf16", "bf16seg"],
+                    default="entropy",
                     help="entropy: the 16,384-block entropy stress (the headline line); columns: the batched "
                          "4,096-column delta/zigzag/bitpack workload; select: learned graph selection over "
-                         "8,192 chunks against exhaustive trial; timestamps / bf16: the single-frame workloads "
-                         "through the ZL_CCtx engine")
+                         "8,192 chunks against exhaustive trial; timestamps / bf16 / bf16seg: workloads through the "
+                         "ZL_CCtx engine (bf16seg: the 1 GiB tensor as one segmented frame)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus})",
@@ -576,7 +589,7 @@ def main():
         return bench_columns(args)
     if args.workload == "select":
         return bench_select(args)
-    if args.workload in ("timestamps", "bf16"):
+    if args.workload in ("timestamps", "bf16", "bf16seg"):
         return bench_engine(args)
     import torch
     from openzl_amd import batch, _lib

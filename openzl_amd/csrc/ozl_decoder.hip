@@ -888,23 +888,33 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
                 tot += len;
                 mx = len > mx ? len : mx;
             }
-            uint8_t* dWin = dc->alloc(tot + 16);
-            uint64_t* dm = (uint64_t*)dc->alloc(m.size() * 8 + 16);
-            const uint8_t* hm = dc->stage.put(m.data(), m.size() * 8);
-            if (!dWin || !dm || !hm) return OZL_ERR_ALLOCATION;
-            OZL_HIP_CHECK(hipMemcpyAsync(dm, hm, m.size() * 8, hipMemcpyHostToDevice, st));
-            int rc = ozl_gather_sized((const uint8_t* const*)dm, dm + L.size(), dm + 2 * L.size(), (int)L.size(), mx,
-                                      dWin, st);
-            if (rc) return rc;
-            std::vector<uint8_t> hw(tot + 1);
-            OZL_HIP_CHECK(hipMemcpyAsync(hw.data(), dWin, tot, hipMemcpyDeviceToHost, st));
+            // the windows land in pinned memory: straight copies for a few
+            // frames (a frame of many chunks takes one round per chunk), a
+            // gather and one copy for many
+            uint8_t* hw = dc->stage.put(nullptr, tot + 1);
+            if (!hw) return OZL_ERR_ALLOCATION;
+            if (L.size() <= 4) {
+                for (size_t k = 0; k < L.size(); k++)
+                    OZL_HIP_CHECK(hipMemcpyAsync(hw + wOff[k], (const void*)(uintptr_t)m[k], wLen[k],
+                                                 hipMemcpyDeviceToHost, st));
+            } else {
+                uint8_t* dWin = dc->alloc(tot + 16);
+                uint64_t* dm = (uint64_t*)dc->alloc(m.size() * 8 + 16);
+                const uint8_t* hm = dc->stage.put(m.data(), m.size() * 8);
+                if (!dWin || !dm || !hm) return OZL_ERR_ALLOCATION;
+                OZL_HIP_CHECK(hipMemcpyAsync(dm, hm, m.size() * 8, hipMemcpyHostToDevice, st));
+                int rc = ozl_gather_sized((const uint8_t* const*)dm, dm + L.size(), dm + 2 * L.size(), (int)L.size(),
+                                          mx, dWin, st);
+                if (rc) return rc;
+                OZL_HIP_CHECK(hipMemcpyAsync(hw, dWin, tot, hipMemcpyDeviceToHost, st));
+            }
             OZL_HIP_CHECK(hipStreamSynchronize(st));
             std::vector<int> prc(L.size(), 0);
             std::vector<char> lg(L.size(), 0);
             ozl_parallel_for(L.size(), [&](size_t k) {
                 FrameJob& j = J[L[k]];
                 bool longer = false;
-                prc[k] = parse_window(hw.data() + wOff[k], j.pos, wLen[k], j.size, j.P, j.pos, j.done, longer);
+                prc[k] = parse_window(hw + wOff[k], j.pos, wLen[k], j.size, j.P, j.pos, j.done, longer);
                 lg[k] = longer;
             });
             for (size_t k = 0; k < L.size(); k++) {

@@ -72,7 +72,8 @@ struct OzlHostStage {
         cur = 0;
         used = 0;
     }
-    // a pinned copy of p[0, n) that stays valid until reset(); null on failure
+    // a pinned copy of p[0, n) (p null: n bytes, not written) that stays
+    // valid until reset(); null on failure
     uint8_t* put(const void* p, size_t n)
     {
         const size_t need = (n + 63) & ~(size_t)63;
@@ -90,7 +91,7 @@ struct OzlHostStage {
         }
         uint8_t* d = blocks[cur].first + used;
         used += need;
-        if (n) memcpy(d, p, n);
+        if (n && p) memcpy(d, p, n);
         return d;
     }
 };
