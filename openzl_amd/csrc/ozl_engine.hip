@@ -177,7 +177,7 @@ struct EntBatch {
     ozl_ews ws[2];
     std::vector<ozl_eplan> plans[2];
     void add(EntReq&& r);
-    int run(DevArena* arena, OzlHostStage* stage, hipStream_t st, hipStream_t alt = nullptr, hipEvent_t fork = nullptr);
+    int run(DevArena* arena, OzlHostStage* stage, hipStream_t st);
     int expand();
 };
 
@@ -740,16 +740,10 @@ void EntBatch::add(EntReq&& r)
     reqs.push_back(std::move(r));
 }
 
-// The FSE and the Huffman chunks go out on two streams (alt after the work
-// queued on st so far), so the two coders' launches overlap.
-int EntBatch::run(DevArena* arena, OzlHostStage* stage, hipStream_t st, hipStream_t alt, hipEvent_t fork)
+// Both coders' launches go out one after the other on st (run side by side
+// on two streams they slowed each other down), then one read-back.
+int EntBatch::run(DevArena* arena, OzlHostStage* stage, hipStream_t st)
 {
-    hipStream_t sm[2] = { st, st };
-    if (alt && fork && !beg[0].empty() && !beg[1].empty()) {
-        OZL_HIP_CHECK(hipEventRecord(fork, st));
-        OZL_HIP_CHECK(hipStreamWaitEvent(alt, fork, 0));
-        sm[1] = alt;
-    }
     std::vector<uint64_t> offs[2];   // read by the launches: kept until the waits below
     for (int m = 0; m < 2; m++) {
         const uint32_t B = (uint32_t)beg[m].size();
@@ -769,9 +763,9 @@ int EntBatch::run(DevArena* arena, OzlHostStage* stage, hipStream_t st, hipStrea
         if (!wsMem || !dO) return OZL_ERR_ALLOCATION;
         const uint8_t* h = stage ? stage->put(o.data(), o.size() * 8) : nullptr;
         OZL_HIP_CHECK(hipMemcpyAsync(dO, h ? (const void*)h : (const void*)o.data(), o.size() * 8, hipMemcpyHostToDevice,
-                                     sm[m]));
+                                     st));
         ws[m] = ozl_ews_carve(wsMem, B, maxBlock[m]);
-        int rc = ozl_entropy_encode_chunks(m, (const uint8_t*)(uintptr_t)lo, dO, dO + B, B, maxBlock[m], ws[m], sm[m]);
+        int rc = ozl_entropy_encode_chunks(m, (const uint8_t*)(uintptr_t)lo, dO, dO + B, B, maxBlock[m], ws[m], st);
         if (rc) return rc;
     }
     for (int m = 0; m < 2; m++) {
@@ -779,10 +773,9 @@ int EntBatch::run(DevArena* arena, OzlHostStage* stage, hipStream_t st, hipStrea
         if (!B) continue;
         plans[m].resize(B);
         OZL_HIP_CHECK(hipMemcpyAsync(plans[m].data(), ws[m].plans, (size_t)B * sizeof(ozl_eplan), hipMemcpyDeviceToHost,
-                                     sm[m]));
+                                     st));
     }
     OZL_HIP_CHECK(hipStreamSynchronize(st));
-    if (sm[1] != st) OZL_HIP_CHECK(hipStreamSynchronize(sm[1]));
     for (int m = 0; m < 2; m++)
         for (const ozl_eplan& p : plans[m])
             if (p.status) return (int)p.status;
@@ -868,17 +861,14 @@ struct ZL_CCtx_s {
     DevArena arena;
     hipStream_t st = nullptr;
     hipStream_t st2 = nullptr;              // content checksums, alongside the graph work
-    hipStream_t st3 = nullptr;              // Huffman chunks, alongside the FSE chunks
-    hipEvent_t evIn = nullptr, evHash = nullptr, evFork = nullptr;
+    hipEvent_t evIn = nullptr, evHash = nullptr;
     OzlHostStage stage;                     // pinned staging of the uploads
     ~ZL_CCtx_s()
     {
         if (st2) (void)hipStreamSynchronize(st2);
-        if (st3) (void)hipStreamSynchronize(st3);
         if (st) (void)hipStreamDestroy(st);
         if (st2) (void)hipStreamDestroy(st2);
-        if (st3) (void)hipStreamDestroy(st3);
-        if (evFork) (void)hipEventDestroy(evFork);
+
         if (evIn) (void)hipEventDestroy(evIn);
         if (evHash) (void)hipEventDestroy(evHash);
     }
@@ -1031,22 +1021,18 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
     }
     if (!c->st && hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return ZL_ErrorCode_allocation;
     if (!c->st2 && hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess) return ZL_ErrorCode_allocation;
-    if (!c->st3 && hipStreamCreateWithFlags(&c->st3, hipStreamNonBlocking) != hipSuccess) return ZL_ErrorCode_allocation;
     if (!c->evIn && hipEventCreateWithFlags(&c->evIn, hipEventDisableTiming) != hipSuccess) return ZL_ErrorCode_allocation;
-    if (!c->evFork && hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess)
-        return ZL_ErrorCode_allocation;
     if (!c->evHash && hipEventCreateWithFlags(&c->evHash, hipEventDisableTiming) != hipSuccess)
         return ZL_ErrorCode_allocation;
     hipStream_t st = c->st;
     // a call that failed part-way may have left work running
     OZL_HIP_CHECK(hipStreamSynchronize(c->st2));
-    OZL_HIP_CHECK(hipStreamSynchronize(c->st3));
     OZL_HIP_CHECK(hipStreamSynchronize(st));
     c->arena.reset();
     c->stage.reset();
     // every return (errors included) leaves no work of this call running:
     // the kernels read the caller's buffers
-    OzlStreamsDone done{ st, c->st2, c->st3 };
+    OzlStreamsDone done{ st, c->st2 };
     EntBatch eb;
     const int cc = fp.contentChecksum, kc = fp.compressedChecksum;
     std::vector<const uint8_t*> dIn(n);
