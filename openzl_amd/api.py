@@ -66,6 +66,7 @@ def _api():
         "ZL_DCtx_free": (None, [P]),
         "ZL_DCtx_decompressTyped": (ZLReport, [P, ctypes.POINTER(OutputInfo), P, SZ, P, SZ]),
         "ZL_ErrorCode_toString": (ctypes.c_char_p, [ctypes.c_int]),
+        "ZL_getDecompressedSize": (ZLReport, [P, SZ]),
         "ozl_CCtx_compressBatch": (ZLReport, [P, P, SZ, ctypes.POINTER(SZ), ctypes.POINTER(P), SZ]),
         "ozl_DCtx_decompressBatch": (ZLReport, [P, P, SZ, ctypes.POINTER(SZ), P, ctypes.POINTER(SZ), SZ]),
     }

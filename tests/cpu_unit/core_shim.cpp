@@ -84,3 +84,48 @@ int cu_dparse(const uint8_t* fr, size_t flen, void* plan, uint8_t* side)
 }
 size_t cu_dplan_size(void) { return sizeof(ozl_dplan); }
 }
+
+// Frame and chunk headers read through windows of `window` bytes (0: the
+// whole frame), as the universal decoder reads device frames: out receives
+// [nbChunks, headerEnd, endMarker, then per chunk start, dataStart, csumPos,
+// end, nbNodes, nbStored]; returns 0 or the error code.
+#include <algorithm>
+#include "ozl_hparse.h"
+extern "C" int cu_parse_frame(const uint8_t* f, size_t size, size_t window, uint64_t* out, size_t outCap)
+{
+    using namespace ozl_hparse;
+    ParsedFrame P;
+    size_t pos = 0, win = window ? window : size;
+    bool done = false;
+    std::vector<uint8_t> buf;
+    while (!done) {
+        if (pos >= size) return OZL_ERR_CORRUPTION;
+        const size_t len = std::min(win, size - pos), base = pos;
+        buf.assign(f + base, f + base + len);
+        bool longer = false;
+        int rc = parse_window(buf.data(), base, len, size, P, pos, done, longer);
+        if (rc == kNeedMore) {
+            if (len == size - base && window == 0) return OZL_ERR_CORRUPTION;
+            if (longer) {
+                if (len >= size - base) return OZL_ERR_CORRUPTION;
+                win *= 4;
+            }
+            continue;
+        }
+        if (rc) return rc;
+    }
+    size_t k = 0;
+    auto put = [&](uint64_t v) { if (k < outCap) out[k] = v; k++; };
+    put(P.chunks.size());
+    put(P.headerEnd);
+    put(P.end);
+    for (const ParsedChunk& C : P.chunks) {
+        put(C.start);
+        put(C.dataStart);
+        put(C.csumPos);
+        put(C.end);
+        put(C.nodes.size());
+        put(C.storedId.size());
+    }
+    return 0;
+}

@@ -195,6 +195,14 @@ def test_segmented_frames_match_reference(api):
     total = sum(a.nbytes for a in arrs)
     outs = dec.decompress_batch(frames, total + 64)
     assert outs == [a.tobytes() for a in arrs]
+    # ZL_getDecompressedSize reads the frame header (host and device frames)
+    L = A._api()
+    for f, a in zip(frames, arrs):
+        src = np.frombuffer(f, np.uint8)
+        assert A._chk(L.ZL_getDecompressedSize(src.ctypes.data, src.size), "size") == a.nbytes
+        d = torch.from_numpy(src.copy()).cuda()
+        torch.cuda.synchronize()
+        assert A._chk(L.ZL_getDecompressedSize(d.data_ptr(), src.size), "size") == a.nbytes
     dfr = torch.from_numpy(np.frombuffer(b"".join(frames), np.uint8).copy()).cuda()
     dout = torch.zeros(total + 64, dtype=torch.uint8, device="cuda")
     sizes = dec.decompress_ptrs(dfr.data_ptr(), [len(f) for f in frames], dout.data_ptr(), total + 64)
