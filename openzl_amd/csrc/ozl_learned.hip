@@ -97,22 +97,39 @@ __device__ void plane_hist(const uint8_t* x, uint32_t n, uint32_t* h, uint32_t& 
     uint32_t* hc = h + (threadIdx.x & (C - 1));
     const uint32_t cnt = DELTA ? n - 1 : n;   // values: elements or deltas
     const uint32_t nv = (cnt + E - 1) / E;
+    const bool al8 = ((uintptr_t)x & 7) == 0;
     for (uint32_t v = threadIdx.x; v < nv; v += kT) {
         const uint32_t e0 = v * E;
         T val[E];
         int nvalid = 0;
+        if (al8 && e0 + E <= cnt) {
+            // a whole vector: two aligned 8-byte loads (and, for deltas, the
+            // element after it), the elements taken out of the words
+            const uint64_t wa = ozl_gld8(x + (size_t)e0 * W), wb = ozl_gld8(x + (size_t)e0 * W + 8);
+            T el[E + 1];
 #pragma unroll
-        for (int j = 0; j < E; j++) {
-            const uint32_t e = e0 + j;
-            const uint32_t ee = e < cnt ? e : e0;
-            const T a = (T)ozl_gld_w(x + (size_t)ee * W, W);
-            if (DELTA) {
-                const T b = (T)ozl_gld_w(x + (size_t)(ee + 1) * W, W);
-                val[j] = (T)(b - a);
-            } else {
-                val[j] = a;
+            for (int j = 0; j < E; j++) {
+                const uint64_t word = j * W < 8 ? wa : wb;
+                el[j] = (T)(word >> ((j * W) % 8 * 8));
             }
-            nvalid += e < cnt;
+            if (DELTA) el[E] = (T)ozl_gld_w(x + (size_t)(e0 + E) * W, W);
+#pragma unroll
+            for (int j = 0; j < E; j++) val[j] = DELTA ? (T)(el[j + 1] - el[j]) : el[j];
+            nvalid = E;
+        } else {
+#pragma unroll
+            for (int j = 0; j < E; j++) {
+                const uint32_t e = e0 + j;
+                const uint32_t ee = e < cnt ? e : e0;
+                const T a = (T)ozl_gld_w(x + (size_t)ee * W, W);
+                if (DELTA) {
+                    const T b = (T)ozl_gld_w(x + (size_t)(ee + 1) * W, W);
+                    val[j] = (T)(b - a);
+                } else {
+                    val[j] = a;
+                }
+                nvalid += e < cnt;
+            }
         }
 #pragma unroll
         for (int j = 0; j < E; j++) {
