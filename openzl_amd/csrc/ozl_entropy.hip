@@ -93,7 +93,7 @@ struct PlanLds {
 
 // Nested FSE graph on the Huffman weights (entropyCompressChunk in EBM_fse
 // mode on the num->serial converted weights stream).
-__device__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux)
+__device__ __forceinline__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux)
 {
     uint32_t m = p.maxSym + 1u;
     if (m < OZL_MIN_STREAM_SIZE) { p.wbk = 255; return; }
@@ -196,7 +196,10 @@ __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, co
         }
     }
     if (threadIdx.x == 0) {
-        ozl_eplan p = {};
+        // the plan is built in place in LDS (a local copy handed to
+        // plan_weights by reference lived in scratch memory)
+        ozl_eplan& p = L.plan;
+        p = ozl_eplan{};
         p.n = n;
         uint8_t* aux = L.aux;
         const unsigned maxSym = ms, card = cd;
@@ -251,7 +254,6 @@ __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, co
         }
         default: break;
         }
-        L.plan = p;
     }
     __syncthreads();
     const ozl_eplan& p = L.plan;
