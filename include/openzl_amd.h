@@ -24,6 +24,7 @@
 #ifndef OPENZL_AMD_H
 #define OPENZL_AMD_H
 
+#include <stdbool.h>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -39,12 +40,16 @@ typedef enum {
     ZL_ErrorCode_srcSize_tooSmall = 3,
     ZL_ErrorCode_srcSize_tooLarge = 4,
     ZL_ErrorCode_dstCapacity_tooSmall = 5,
+    ZL_ErrorCode_userBuffers_invalidNum = 8,
     ZL_ErrorCode_header_unknown = 10,
     ZL_ErrorCode_frameParameter_unsupported = 11,
     ZL_ErrorCode_corruption = 12,
     ZL_ErrorCode_compressedChecksumWrong = 13,
     ZL_ErrorCode_contentChecksumWrong = 14,
+    ZL_ErrorCode_outputs_tooNumerous = 15,
+    ZL_ErrorCode_compressionParameter_invalid = 20,
     ZL_ErrorCode_parameter_invalid = 21,
+    ZL_ErrorCode_outputID_invalid = 22,
     ZL_ErrorCode_segmenter_inputNotConsumed = 26,
     ZL_ErrorCode_graph_invalid = 30,
     ZL_ErrorCode_graph_invalidNumInputs = 33,
@@ -105,16 +110,27 @@ typedef struct { unsigned gid; } ZL_GraphID;
 #define ZL_GRAPH_BITPACK ((ZL_GraphID){ 8 })
 #define ZL_GRAPH_FLATPACK ((ZL_GraphID){ 9 })
 
-/* include/openzl/zl_compress.h:64-131 (subset) */
+/* include/openzl/zl_compress.h:64-131 */
 typedef enum {
+    ZL_CParam_stickyParameters = 1,
     ZL_CParam_compressionLevel = 2,
+    ZL_CParam_decompressionLevel = 3,
     ZL_CParam_formatVersion = 4,
+    ZL_CParam_permissiveCompression = 5,
     ZL_CParam_compressedChecksum = 6,
     ZL_CParam_contentChecksum = 7,
     ZL_CParam_minStreamSize = 11,
 } ZL_CParam;
 typedef enum { ZL_TernaryParam_auto = 0, ZL_TernaryParam_enable = 1, ZL_TernaryParam_disable = 2 } ZL_TernaryParam;
+#define ZL_MIN_FORMAT_VERSION 8  /* include/openzl/zl_version.h:49 */
 #define ZL_MAX_FORMAT_VERSION 23 /* include/openzl/zl_version.h:57 */
+#define ZL_MINSTREAMSIZE_DEFAULT 10 /* include/openzl/zl_compress.h:137 */
+/* include/openzl/zl_decompress.h:91-127 */
+typedef enum {
+    ZL_DParam_stickyParameters = 1,
+    ZL_DParam_checkCompressedChecksum = 2,
+    ZL_DParam_checkContentChecksum = 3,
+} ZL_DParam;
 
 typedef struct ZL_Compressor_s ZL_Compressor;
 typedef struct ZL_CCtx_s ZL_CCtx;
@@ -184,7 +200,7 @@ typedef struct {
     ZL_SegmenterFn segmenterFn;
     const ZL_Type* inputTypeMasks;
     size_t numInputs;
-    int lastInputIsVariable;          /* bool in the reference; 1 input only here */
+    bool lastInputIsVariable;         /* 1 input only here */
     const ZL_GraphID* customGraphs;
     size_t numCustomGraphs;
     char localParams[48];             /* layout placeholder, unused */
@@ -243,8 +259,19 @@ ZL_CCtx* ZL_CCtx_create(void);
 void ZL_CCtx_free(ZL_CCtx* cctx);
 /* include/openzl/zl_compressor.h:597 */
 ZL_Report ZL_CCtx_refCompressor(ZL_CCtx* cctx, const ZL_Compressor* compressor);
-/* include/openzl/zl_compress.h:152 */
+/* include/openzl/zl_compress.h:152. Every ZL_CParam is accepted (others:
+ * compressionParameter_invalid; formatVersion must be 0 or 8..23). A CCtx
+ * value that is not 0 overrides the compressor's, which overrides the
+ * default (src/openzl/compress/cctx.c:350-364, gcparams.c:124-137). Unless
+ * ZL_CParam_stickyParameters is 1, every compression ends with
+ * ZL_CCtx_resetParameters (compress2.c:141-145), which also drops the
+ * compressor reference. minStreamSize 0 means 10; a negative value turns the
+ * automatic store off. */
 ZL_Report ZL_CCtx_setParameter(ZL_CCtx* cctx, ZL_CParam param, int value);
+/* include/openzl/zl_compress.h:161 */
+int ZL_CCtx_getParameter(const ZL_CCtx* cctx, ZL_CParam param);
+/* include/openzl/zl_compress.h:168 */
+ZL_Report ZL_CCtx_resetParameters(ZL_CCtx* cctx);
 /* include/openzl/zl_compress.h:313-349 */
 ZL_TypedRef* ZL_TypedRef_createSerial(const void* src, size_t srcSize);
 ZL_TypedRef* ZL_TypedRef_createStruct(const void* start, size_t structWidth, size_t structCount);
@@ -252,7 +279,14 @@ ZL_TypedRef* ZL_TypedRef_createNumeric(const void* start, size_t numWidth, size_
 void ZL_TypedRef_free(ZL_TypedRef* tref);
 /* include/openzl/zl_compress.h:274 */
 ZL_Report ZL_CCtx_compressTypedRef(ZL_CCtx* cctx, void* dst, size_t dstCapacity, const ZL_TypedRef* input);
-/* include/openzl/zl_compress.h:194 */
+/* include/openzl/zl_compress.h:295. One input writes the frame
+ * ZL_CCtx_compressTypedRef writes; frames of several inputs are outside the
+ * hot path and are rejected with temporaryLibraryLimitation. */
+ZL_Report ZL_CCtx_compressMultiTypedRef(ZL_CCtx* cctx, void* dst, size_t dstCapacity, const ZL_TypedRef* inputs[],
+                                        size_t nbInputs);
+/* include/openzl/zl_compress.h:194. Without a compressor the reference runs
+ * its generic LZ graph (compress2.c:371-380), which is not part of the hot
+ * path: that call returns graph_invalid here. */
 ZL_Report ZL_CCtx_compress(ZL_CCtx* cctx, void* dst, size_t dstCapacity, const void* src, size_t srcSize);
 /* nbInputs calls of ZL_CCtx_compressTypedRef (include/openzl/zl_compress.h:274)
  * in one: frame i (the same bytes as that call writes for inputs[i]) lands in
@@ -279,9 +313,53 @@ ZL_Report ZL_getDecompressedSize(const void* compressed, size_t cSize);
 /* include/openzl/zl_decompress.h:80, 87 */
 ZL_DCtx* ZL_DCtx_create(void);
 void ZL_DCtx_free(ZL_DCtx* dctx);
+/* include/openzl/zl_decompress.h:140, 149, 159. Parameters reset at the end
+ * of every decompression unless ZL_DParam_stickyParameters is 1
+ * (decompress2.c); a checksum is verified unless its parameter is
+ * ZL_TernaryParam_disable. Unknown parameters: parameter_invalid. */
+ZL_Report ZL_DCtx_setParameter(ZL_DCtx* dctx, ZL_DParam param, int value);
+int ZL_DCtx_getParameter(const ZL_DCtx* dctx, ZL_DParam param);
+ZL_Report ZL_DCtx_resetParameters(ZL_DCtx* dctx);
+/* include/openzl/zl_decompress.h:230 */
+ZL_Report ZL_DCtx_decompress(ZL_DCtx* dctx, void* dst, size_t dstCapacity, const void* compressed, size_t cSize);
 /* include/openzl/zl_decompress.h:388 */
 ZL_Report ZL_DCtx_decompressTyped(ZL_DCtx* dctx, ZL_OutputInfo* outputInfo, void* dst, size_t dstByteCapacity,
                                   const void* compressed, size_t cSize);
+
+/* Typed buffers: include/openzl/zl_decompress.h:396-613 (String type
+ * excluded: string outputs are not part of the hot path). An empty buffer
+ * (ZL_TypedBuffer_create) owns host memory sized by the frame; a wrapped one
+ * writes into the caller's buffer (host or device memory). */
+typedef struct ZL_TypedBuffer_s ZL_TypedBuffer;
+ZL_TypedBuffer* ZL_TypedBuffer_create(void);
+void ZL_TypedBuffer_free(ZL_TypedBuffer* tbuffer);
+ZL_TypedBuffer* ZL_TypedBuffer_createWrapSerial(void* buffer, size_t bufferCapacity);
+ZL_TypedBuffer* ZL_TypedBuffer_createWrapStruct(void* structBuffer, size_t structWidth, size_t numStructs);
+ZL_TypedBuffer* ZL_TypedBuffer_createWrapNumeric(void* numArray, size_t numWidth, size_t numCapacity);
+/* include/openzl/zl_decompress.h:505 */
+ZL_Report ZL_DCtx_decompressTBuffer(ZL_DCtx* dctx, ZL_TypedBuffer* output, const void* compressed, size_t cSize);
+/* include/openzl/zl_decompress.h:527 (frames of this library have one
+ * output: nbOutputs must be 1, else userBuffers_invalidNum) */
+ZL_Report ZL_DCtx_decompressMultiTBuffer(ZL_DCtx* dctx, ZL_TypedBuffer* outputs[], size_t nbOutputs,
+                                         const void* compressed, size_t cSize);
+/* include/openzl/zl_decompress.h:550-600 */
+ZL_Type ZL_TypedBuffer_type(const ZL_TypedBuffer* tbuffer);
+size_t ZL_TypedBuffer_byteSize(const ZL_TypedBuffer* tbuffer);
+const void* ZL_TypedBuffer_rPtr(const ZL_TypedBuffer* tbuffer);
+size_t ZL_TypedBuffer_numElts(const ZL_TypedBuffer* tbuffer);
+size_t ZL_TypedBuffer_eltWidth(const ZL_TypedBuffer* tbuffer);
+
+/* Frame queries: include/openzl/zl_decompress.h:245-342 (host memory). */
+ZL_Report ZL_getNumOutputs(const void* compressed, size_t cSize);
+ZL_Report ZL_getOutputType(ZL_Type* stPtr, const void* compressed, size_t cSize);
+typedef struct ZL_FrameInfo ZL_FrameInfo;
+ZL_FrameInfo* ZL_FrameInfo_create(const void* compressed, size_t cSize);
+void ZL_FrameInfo_free(ZL_FrameInfo* fi);
+ZL_Report ZL_FrameInfo_getFormatVersion(const ZL_FrameInfo* fi);
+ZL_Report ZL_FrameInfo_getNumOutputs(const ZL_FrameInfo* fi);
+ZL_Report ZL_FrameInfo_getOutputType(const ZL_FrameInfo* fi, int outputID);
+ZL_Report ZL_FrameInfo_getDecompressedSize(const ZL_FrameInfo* fi, int outputID);
+ZL_Report ZL_FrameInfo_getNumElts(const ZL_FrameInfo* fi, int outputID);
 /* nbFrames calls of ZL_DCtx_decompressTyped (include/openzl/zl_decompress.h:388)
  * in one: frame i is src[sum of frameSizes[0..i), + frameSizes[i]); its
  * content lands in dst after those of frames 0..i-1, outSizes[i] = its size;

@@ -30,6 +30,8 @@ class OutputInfo(ctypes.Structure):
 
 
 SERIAL, STRUCT, NUMERIC = 1, 2, 4
+# ZL_CParam values (include/openzl/zl_compress.h:64-131)
+CP_STICKY, CP_LEVEL, CP_DLEVEL, CP_FORMAT, CP_PERMISSIVE, CP_KSUM, CP_CSUM, CP_MINSTREAM = 1, 2, 3, 4, 5, 6, 7, 11
 G_STORE, G_FSE, G_HUFFMAN, G_BITPACK = GraphID(2), GraphID(3), GraphID(4), GraphID(8)
 N_DELTA, N_TRANSPOSE_SPLIT, N_ZIGZAG = NodeID(2), NodeID(3), NodeID(4)
 
@@ -56,6 +58,36 @@ def _api():
         "ZL_CCtx_create": (P, []),
         "ZL_CCtx_free": (None, [P]),
         "ZL_CCtx_refCompressor": (ZLReport, [P, P]),
+        "ZL_CCtx_setParameter": (ZLReport, [P, ctypes.c_int, ctypes.c_int]),
+        "ZL_CCtx_getParameter": (ctypes.c_int, [P, ctypes.c_int]),
+        "ZL_CCtx_resetParameters": (ZLReport, [P]),
+        "ZL_CCtx_compressMultiTypedRef": (ZLReport, [P, P, SZ, ctypes.POINTER(P), SZ]),
+        "ZL_DCtx_setParameter": (ZLReport, [P, ctypes.c_int, ctypes.c_int]),
+        "ZL_DCtx_getParameter": (ctypes.c_int, [P, ctypes.c_int]),
+        "ZL_DCtx_resetParameters": (ZLReport, [P]),
+        "ZL_DCtx_decompress": (ZLReport, [P, P, SZ, P, SZ]),
+        "ZL_decompress": (ZLReport, [P, SZ, P, SZ]),
+        "ZL_TypedBuffer_create": (P, []),
+        "ZL_TypedBuffer_free": (None, [P]),
+        "ZL_TypedBuffer_createWrapSerial": (P, [P, SZ]),
+        "ZL_TypedBuffer_createWrapStruct": (P, [P, SZ, SZ]),
+        "ZL_TypedBuffer_createWrapNumeric": (P, [P, SZ, SZ]),
+        "ZL_DCtx_decompressTBuffer": (ZLReport, [P, P, P, SZ]),
+        "ZL_DCtx_decompressMultiTBuffer": (ZLReport, [P, ctypes.POINTER(P), SZ, P, SZ]),
+        "ZL_TypedBuffer_type": (ctypes.c_int, [P]),
+        "ZL_TypedBuffer_byteSize": (SZ, [P]),
+        "ZL_TypedBuffer_rPtr": (P, [P]),
+        "ZL_TypedBuffer_numElts": (SZ, [P]),
+        "ZL_TypedBuffer_eltWidth": (SZ, [P]),
+        "ZL_getNumOutputs": (ZLReport, [P, SZ]),
+        "ZL_getOutputType": (ZLReport, [ctypes.POINTER(ctypes.c_int), P, SZ]),
+        "ZL_FrameInfo_create": (P, [P, SZ]),
+        "ZL_FrameInfo_free": (None, [P]),
+        "ZL_FrameInfo_getFormatVersion": (ZLReport, [P]),
+        "ZL_FrameInfo_getNumOutputs": (ZLReport, [P]),
+        "ZL_FrameInfo_getOutputType": (ZLReport, [P, ctypes.c_int]),
+        "ZL_FrameInfo_getDecompressedSize": (ZLReport, [P, ctypes.c_int]),
+        "ZL_FrameInfo_getNumElts": (ZLReport, [P, ctypes.c_int]),
         "ZL_TypedRef_createSerial": (P, [P, SZ]),
         "ZL_TypedRef_createStruct": (P, [P, SZ, SZ]),
         "ZL_TypedRef_createNumeric": (P, [P, SZ, SZ]),
@@ -151,11 +183,18 @@ def build_graph(L, c, kind, segment=None):
 class Compressor:
     """A ZL_Compressor + ZL_CCtx pair for one graph kind (format version 23)."""
 
-    def __init__(self, kind, candidates=None, content_checksum=True, compressed_checksum=True, segment=None):
+    PARAMS = {"minStreamSize": CP_MINSTREAM, "compressionLevel": CP_LEVEL, "contentChecksum": CP_CSUM,
+              "compressedChecksum": CP_KSUM}
+
+    def __init__(self, kind, candidates=None, content_checksum=True, compressed_checksum=True, segment=None,
+                 params=None):
+        """params: further compressor parameters by name (PARAMS)."""
         L = _api()
         self.L = L
         self.c = L.ZL_Compressor_create()
-        _chk(L.ZL_Compressor_setParameter(self.c, 4, 23), "formatVersion")
+        _chk(L.ZL_Compressor_setParameter(self.c, CP_FORMAT, 23), "formatVersion")
+        for k, v in (params or {}).items():
+            _chk(L.ZL_Compressor_setParameter(self.c, self.PARAMS[k], int(v)), k)
         if not content_checksum:
             _chk(L.ZL_Compressor_setParameter(self.c, 7, 2), "contentChecksum")
         if not compressed_checksum:
@@ -167,6 +206,9 @@ class Compressor:
             g = build_graph(L, self.c, kind, segment)
         _chk(L.ZL_Compressor_selectStartingGraphID(self.c, g), "selectStartingGraphID")
         self.cctx = L.ZL_CCtx_create()
+        # a context drops its parameters and its compressor after every call
+        # unless they are sticky (as the reference's does)
+        _chk(L.ZL_CCtx_setParameter(self.cctx, CP_STICKY, 1), "stickyParameters")
         _chk(L.ZL_CCtx_refCompressor(self.cctx, self.c), "refCompressor")
 
     def compress(self, arr, typ, width):

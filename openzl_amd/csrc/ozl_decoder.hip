@@ -119,7 +119,12 @@ struct DStream {
 
 }  // namespace
 
+// A decompression parameter's applied value: 0 stands for the default
+// (enable); only enable turns a check on (decompress2.c:1495, 1538).
+inline bool ozl_dparam_on(int v) { return (v ? v : ZL_TernaryParam_enable) == ZL_TernaryParam_enable; }
+
 struct ZL_DCtx_s {
+    int sticky = 0, checkCompressed = 0, checkContent = 0;   // requested parameters
     hipStream_t st = nullptr;
     hipStream_t st2 = nullptr;   // compressed checksums, alongside the decoding
     hipEvent_t evIn = nullptr, evHash = nullptr;
@@ -233,8 +238,17 @@ struct Decoder {
     // buffer when it has the size the frame header announces
     uint8_t* rootDst = nullptr;
     size_t rootSize = 0;
+    // No regenerated stream of a hot-path frame is larger than the frame's
+    // content plus the largest side stream (16-bit Huffman weights, 65536
+    // bytes): a header that announces more is corrupt, and is rejected
+    // before anything is allocated or filled.
+    size_t limit = 0;
     uint8_t* out_alloc(const DNode& nd, size_t n)
     {
+        if (n > limit) {
+            if (!err) err = OZL_ERR_CORRUPTION;
+            return nullptr;
+        }
         if (nd.in == 0 && rootDst && n == rootSize) return rootDst;
         return alloc(n);
     }
@@ -622,6 +636,9 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
     if (!dc->evIn && hipEventCreateWithFlags(&dc->evIn, hipEventDisableTiming) != hipSuccess) return OZL_ERR_ALLOCATION;
     if (!dc->evHash && hipEventCreateWithFlags(&dc->evHash, hipEventDisableTiming) != hipSuccess) return OZL_ERR_ALLOCATION;
     hipStream_t st = dc->st;
+    // checksums are verified when their parameter resolves to enable
+    // (decompress2.c:1492-1545, gdparams.c:8-45)
+    const bool vK = ozl_dparam_on(dc->checkCompressed), vC = ozl_dparam_on(dc->checkContent);
     // a call that failed part-way may have left work running
     OZL_HIP_CHECK(hipStreamSynchronize(dc->st2));
     OZL_HIP_CHECK(hipStreamSynchronize(st));
@@ -745,6 +762,7 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         D.q = &Q;
         D.st = st;
         D.S.assign(C.nbStreams, DStream());
+        D.limit = j.P.size + ((size_t)1 << 17);
         Q.members.push_back(&D);
         size_t pos = C.dataStart;
         for (size_t k = 0; k < C.storedId.size(); k++) {
@@ -790,7 +808,7 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
     for (size_t k = 0; k < U.size(); k++) {
         const FrameJob& j = J[U[k].frame];
         const ParsedChunk& C = *U[k].C;
-        if (!(j.P.flags & 2)) continue;
+        if (!(j.P.flags & 2) || !vK) continue;
         const size_t clen = C.csumPos + ((j.P.flags & 1) ? 4 : 0) - C.start;
         kJobs.insert(kJobs.end(), { (uint64_t)(uintptr_t)(j.dFrame + C.start), clen, kSum });
         kIdx.push_back(k);
@@ -871,7 +889,7 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         u.outOff = frameOut[u.frame] + got[u.frame];
         got[u.frame] += out.bytes;
         if (got[u.frame] > j.P.size) return OZL_ERR_CORRUPTION;
-        if (j.P.flags & 1) {
+        if ((j.P.flags & 1) && vC) {
             const uint8_t* p = out.dev ? out.dev : j.dFrame;
             cJobs.insert(cJobs.end(), { (uint64_t)(uintptr_t)p, out.bytes, cSum });
             cIdx.push_back(k);
@@ -934,12 +952,77 @@ int decompress_impl(ZL_DCtx_s* dc, const uint8_t* src, size_t srcSize, void* dst
     return decompress_frames(dc, src, &srcSize, 1, dst, cap, &outSize, info);
 }
 
+// Parameters requested on the context are dropped at the end of every
+// decompression unless they are sticky (decompress2.c:1764-1767).
+void dctx_end(ZL_DCtx_s* d)
+{
+    if (!d->sticky) d->checkCompressed = d->checkContent = 0;
+}
+
+// Frame header of a host frame: type, size, format version.
+int host_frame_info(const void* src, size_t srcSize, ParsedFrame& P)
+{
+    if (!src) return ZL_ErrorCode_parameter_invalid;
+    HostReader r = { (const uint8_t*)src, 0, 0, srcSize, false };
+    return parse_frame_header(r, P);
+}
+
+ZL_Type frame_ztype(int t) { return t == OZL_T_SERIAL ? ZL_Type_serial : t == OZL_T_STRUCT ? ZL_Type_struct : ZL_Type_numeric; }
+
 }  // namespace
+
+// An output buffer of ZL_DCtx_decompressTBuffer: either its own host memory,
+// sized from the frame header, or a caller's buffer of a declared type.
+struct ZL_TypedBuffer_s {
+    std::vector<uint8_t> own;
+    void* wrap = nullptr;
+    size_t cap = 0;
+    int wrapType = 0;      // 0: not wrapped
+    size_t wrapWidth = 0;
+    ZL_Type type = ZL_Type_serial;
+    size_t width = 1, bytes = 0;
+};
+
+struct ZL_FrameInfo {
+    ZL_Type type;
+    uint64_t size;
+};
 
 extern "C" {
 
 ZL_DCtx* ZL_DCtx_create(void) { return new (std::nothrow) ZL_DCtx_s(); }
 void ZL_DCtx_free(ZL_DCtx* d) { delete d; }
+
+// gdparams.c:15-33: unknown parameters are compressionParameter_invalid.
+ZL_Report ZL_DCtx_setParameter(ZL_DCtx* d, ZL_DParam p, int v)
+{
+    if (!d) return rep_err(ZL_ErrorCode_parameter_invalid);
+    switch (p) {
+    case ZL_DParam_stickyParameters: d->sticky = v != 0; break;
+    case ZL_DParam_checkCompressedChecksum: d->checkCompressed = v; break;
+    case ZL_DParam_checkContentChecksum: d->checkContent = v; break;
+    default: return rep_err(ZL_ErrorCode_compressionParameter_invalid);
+    }
+    return rep_ok(0);
+}
+
+int ZL_DCtx_getParameter(const ZL_DCtx* d, ZL_DParam p)
+{
+    if (!d) return 0;
+    switch (p) {
+    case ZL_DParam_stickyParameters: return d->sticky;
+    case ZL_DParam_checkCompressedChecksum: return d->checkCompressed;
+    case ZL_DParam_checkContentChecksum: return d->checkContent;
+    default: return 0;
+    }
+}
+
+ZL_Report ZL_DCtx_resetParameters(ZL_DCtx* d)
+{
+    if (!d) return rep_err(ZL_ErrorCode_parameter_invalid);
+    d->sticky = d->checkCompressed = d->checkContent = 0;
+    return rep_ok(0);
+}
 
 ZL_Report ZL_DCtx_decompressTyped(ZL_DCtx* dctx, ZL_OutputInfo* info, void* dst, size_t cap, const void* src,
                                   size_t srcSize)
@@ -947,9 +1030,19 @@ ZL_Report ZL_DCtx_decompressTyped(ZL_DCtx* dctx, ZL_OutputInfo* info, void* dst,
     if (!dctx || !src) return rep_err(ZL_ErrorCode_parameter_invalid);
     ZL_OutputInfo tmp;
     int rc = decompress_impl(dctx, (const uint8_t*)src, srcSize, dst, cap, &tmp);
+    dctx_end(dctx);
     if (rc) return rep_err(rc);
     if (info) *info = tmp;
     return rep_ok(tmp.decompressedByteSize);
+}
+
+// decompress2.c:1845-1866: only frames of serial content.
+ZL_Report ZL_DCtx_decompress(ZL_DCtx* dctx, void* dst, size_t cap, const void* src, size_t srcSize)
+{
+    ZL_OutputInfo info;
+    ZL_Report r = ZL_DCtx_decompressTyped(dctx, &info, dst, cap, src, srcSize);
+    if (!ZL_isError(r) && info.type != ZL_Type_serial) return rep_err(ZL_ErrorCode_GENERIC);
+    return r;
 }
 
 ZL_Report ozl_DCtx_decompressBatch(ZL_DCtx* dctx, void* dst, size_t dstCapacity, size_t* outSizes, const void* src,
@@ -958,20 +1051,101 @@ ZL_Report ozl_DCtx_decompressBatch(ZL_DCtx* dctx, void* dst, size_t dstCapacity,
     if (!dctx || !outSizes || (nbFrames && (!src || !frameSizes))) return rep_err(ZL_ErrorCode_parameter_invalid);
     if (nbFrames == 0) return rep_ok(0);
     int rc = decompress_frames(dctx, (const uint8_t*)src, frameSizes, nbFrames, dst, dstCapacity, outSizes, nullptr);
+    dctx_end(dctx);
     if (rc) return rep_err(rc);
     size_t total = 0;
     for (size_t i = 0; i < nbFrames; i++) total += outSizes[i];
     return rep_ok(total);
 }
 
+// decompress2.c:1868-1882
 ZL_Report ZL_decompress(void* dst, size_t cap, const void* src, size_t srcSize)
 {
     ZL_DCtx_s d;
-    ZL_OutputInfo info;
-    int rc = decompress_impl(&d, (const uint8_t*)src, srcSize, dst, cap, &info);
-    if (rc) return rep_err(rc);
-    return rep_ok(info.decompressedByteSize);
+    return ZL_DCtx_decompress(&d, dst, cap, src, srcSize);
 }
+
+ZL_TypedBuffer* ZL_TypedBuffer_create(void) { return new (std::nothrow) ZL_TypedBuffer_s(); }
+void ZL_TypedBuffer_free(ZL_TypedBuffer* t) { delete t; }
+
+static ZL_TypedBuffer* wrap_buffer(void* p, size_t cap, int type, size_t width)
+{
+    if (!p && cap) return nullptr;
+    ZL_TypedBuffer_s* t = new (std::nothrow) ZL_TypedBuffer_s();
+    if (!t) return nullptr;
+    t->wrap = p;
+    t->cap = cap;
+    t->wrapType = type;
+    t->wrapWidth = width;
+    return t;
+}
+ZL_TypedBuffer* ZL_TypedBuffer_createWrapSerial(void* b, size_t cap) { return wrap_buffer(b, cap, ZL_Type_serial, 1); }
+ZL_TypedBuffer* ZL_TypedBuffer_createWrapStruct(void* b, size_t w, size_t n)
+{
+    return w ? wrap_buffer(b, w * n, ZL_Type_struct, w) : nullptr;
+}
+ZL_TypedBuffer* ZL_TypedBuffer_createWrapNumeric(void* b, size_t w, size_t n)
+{
+    return (w == 1 || w == 2 || w == 4 || w == 8) ? wrap_buffer(b, w * n, ZL_Type_numeric, w) : nullptr;
+}
+
+// decompress2.c:1581-1700: a wrapped buffer must be large enough and, once
+// the frame's output type and width are known, of that type and width
+// (stream.c:146-160 reports a mismatch as corruption).
+ZL_Report ZL_DCtx_decompressTBuffer(ZL_DCtx* dctx, ZL_TypedBuffer* t, const void* src, size_t srcSize)
+{
+    if (!dctx || !t || !src) return rep_err(ZL_ErrorCode_parameter_invalid);
+    void* dst = t->wrap;
+    size_t cap = t->cap;
+    if (!t->wrapType) {
+        size_t sz = 0;
+        if (ozl_is_device_ptr(src)) {
+            ZL_Report r = ZL_getDecompressedSize(src, srcSize);
+            if (ZL_isError(r)) { dctx_end(dctx); return r; }
+            sz = ZL_validResult(r);
+        } else {
+            ParsedFrame P;
+            int rc = host_frame_info(src, srcSize, P);
+            if (rc) { dctx_end(dctx); return rep_err(rc); }
+            sz = P.size;
+        }
+        t->own.assign(sz ? sz : 1, 0);
+        dst = t->own.data();
+        cap = sz;
+    }
+    ZL_OutputInfo info;
+    ZL_Report r = ZL_DCtx_decompressTyped(dctx, &info, dst, cap, src, srcSize);
+    if (ZL_isError(r)) return r;
+    if (t->wrapType && (info.type != (ZL_Type)t->wrapType || info.fixedWidth != t->wrapWidth))
+        return rep_err(ZL_ErrorCode_corruption);
+    t->type = info.type;
+    t->width = info.fixedWidth;
+    t->bytes = info.decompressedByteSize;
+    return rep_ok(t->bytes);
+}
+
+ZL_Report ZL_DCtx_decompressMultiTBuffer(ZL_DCtx* dctx, ZL_TypedBuffer* outputs[], size_t nbOutputs,
+                                         const void* src, size_t srcSize)
+{
+    if (!dctx || !outputs) return rep_err(ZL_ErrorCode_parameter_invalid);
+    // decompress2.c:839-846: the exact number of outputs of the frame
+    if (nbOutputs != 1) {
+        dctx_end(dctx);
+        return rep_err(ZL_ErrorCode_userBuffers_invalidNum);
+    }
+    ZL_Report r = ZL_DCtx_decompressTBuffer(dctx, outputs[0], src, srcSize);
+    return ZL_isError(r) ? r : rep_ok(1);
+}
+
+ZL_Type ZL_TypedBuffer_type(const ZL_TypedBuffer* t) { return t ? t->type : ZL_Type_serial; }
+size_t ZL_TypedBuffer_byteSize(const ZL_TypedBuffer* t) { return t ? t->bytes : 0; }
+const void* ZL_TypedBuffer_rPtr(const ZL_TypedBuffer* t)
+{
+    if (!t) return nullptr;
+    return t->wrapType ? t->wrap : (const void*)t->own.data();
+}
+size_t ZL_TypedBuffer_numElts(const ZL_TypedBuffer* t) { return t && t->width ? t->bytes / t->width : 0; }
+size_t ZL_TypedBuffer_eltWidth(const ZL_TypedBuffer* t) { return t ? t->width : 0; }
 
 ZL_Report ZL_getDecompressedSize(const void* src, size_t srcSize)
 {
@@ -992,6 +1166,54 @@ ZL_Report ZL_getDecompressedSize(const void* src, size_t srcSize)
     uint64_t s1 = r.varint();
     if (r.bad || s1 == 0) return rep_err(ZL_ErrorCode_corruption);
     return rep_ok(s1 - 1);
+}
+
+// decode_frameheader.c:375-470, 570-599 (frames of this library: one output)
+ZL_FrameInfo* ZL_FrameInfo_create(const void* src, size_t srcSize)
+{
+    ParsedFrame P;
+    if (host_frame_info(src, srcSize, P)) return nullptr;
+    return new (std::nothrow) ZL_FrameInfo{ frame_ztype(P.type), P.size };
+}
+void ZL_FrameInfo_free(ZL_FrameInfo* fi) { delete fi; }
+ZL_Report ZL_FrameInfo_getFormatVersion(const ZL_FrameInfo* fi)
+{
+    return fi ? rep_ok(OZL_FORMAT_VERSION) : rep_err(ZL_ErrorCode_GENERIC);
+}
+ZL_Report ZL_FrameInfo_getNumOutputs(const ZL_FrameInfo* fi) { return fi ? rep_ok(1) : rep_err(ZL_ErrorCode_GENERIC); }
+ZL_Report ZL_FrameInfo_getOutputType(const ZL_FrameInfo* fi, int id)
+{
+    if (!fi) return rep_err(ZL_ErrorCode_GENERIC);
+    return id < 1 ? rep_ok((size_t)fi->type) : rep_err(ZL_ErrorCode_outputID_invalid);
+}
+ZL_Report ZL_FrameInfo_getDecompressedSize(const ZL_FrameInfo* fi, int id)
+{
+    if (!fi) return rep_err(ZL_ErrorCode_GENERIC);
+    return id < 1 ? rep_ok(fi->size) : rep_err(ZL_ErrorCode_outputID_invalid);
+}
+// decode_frameheader.c:439-470: serial outputs only
+ZL_Report ZL_FrameInfo_getNumElts(const ZL_FrameInfo* fi, int id)
+{
+    if (!fi) return rep_err(ZL_ErrorCode_GENERIC);
+    if (id >= 1) return rep_err(ZL_ErrorCode_outputID_invalid);
+    if (fi->type != ZL_Type_serial) return rep_err(ZL_ErrorCode_temporaryLibraryLimitation);
+    return rep_ok(fi->size);
+}
+ZL_Report ZL_getNumOutputs(const void* src, size_t srcSize)
+{
+    ZL_FrameInfo* fi = ZL_FrameInfo_create(src, srcSize);
+    ZL_Report r = ZL_FrameInfo_getNumOutputs(fi);
+    ZL_FrameInfo_free(fi);
+    return r;
+}
+ZL_Report ZL_getOutputType(ZL_Type* stPtr, const void* src, size_t srcSize)
+{
+    if (!stPtr) return rep_err(ZL_ErrorCode_parameter_invalid);
+    ParsedFrame P;
+    int rc = host_frame_info(src, srcSize, P);
+    if (rc) return rep_err(rc);
+    *stPtr = frame_ztype(P.type);
+    return rep_ok(0);
 }
 
 }  // extern "C"

@@ -135,6 +135,51 @@ struct ENode {
 
 }  // namespace
 
+namespace {
+
+// Global compression parameters as requested on a compressor or a context
+// (src/openzl/compress/gcparams.c:15-100): 0 means "not set".
+struct GParams {
+    int sticky = 0, compressionLevel = 0, decompressionLevel = 0, formatVersion = 0, permissive = 0,
+        compressedChecksum = 0, contentChecksum = 0, minStreamSize = 0;
+    // gcparams.c:62-100
+    int set(ZL_CParam p, int v)
+    {
+        switch (p) {
+        case ZL_CParam_stickyParameters: sticky = v != 0; break;
+        case ZL_CParam_compressionLevel: compressionLevel = v; break;
+        case ZL_CParam_decompressionLevel: decompressionLevel = v; break;
+        case ZL_CParam_permissiveCompression: permissive = v; break;
+        case ZL_CParam_compressedChecksum: compressedChecksum = v; break;
+        case ZL_CParam_contentChecksum: contentChecksum = v; break;
+        case ZL_CParam_minStreamSize: minStreamSize = v; break;
+        case ZL_CParam_formatVersion:
+            if (!(v == 0 || (v >= ZL_MIN_FORMAT_VERSION && v <= ZL_MAX_FORMAT_VERSION)))
+                return ZL_ErrorCode_formatVersion_unsupported;
+            formatVersion = v;
+            break;
+        default: return ZL_ErrorCode_compressionParameter_invalid;
+        }
+        return 0;
+    }
+    int get(ZL_CParam p) const
+    {
+        switch (p) {
+        case ZL_CParam_stickyParameters: return sticky;
+        case ZL_CParam_compressionLevel: return compressionLevel;
+        case ZL_CParam_decompressionLevel: return decompressionLevel;
+        case ZL_CParam_permissiveCompression: return permissive;
+        case ZL_CParam_compressedChecksum: return compressedChecksum;
+        case ZL_CParam_contentChecksum: return contentChecksum;
+        case ZL_CParam_minStreamSize: return minStreamSize;
+        case ZL_CParam_formatVersion: return formatVersion;
+        default: return 0;
+        }
+    }
+};
+
+}  // namespace
+
 struct ZL_Compressor_s {
     std::vector<ZL_OpaquePtr> opaques;   // owned (segmenter descriptors), freed with the compressor
     ~ZL_Compressor_s()
@@ -142,10 +187,7 @@ struct ZL_Compressor_s {
         for (const ZL_OpaquePtr& o : opaques)
             if (o.freeFn) o.freeFn(o.freeOpaquePtr, o.ptr);
     }
-    int formatVersion = 0;
-    int contentChecksum = ZL_TernaryParam_auto;
-    int compressedChecksum = ZL_TernaryParam_auto;
-    int minStreamSize = OZL_MIN_STREAM_SIZE;
+    GParams params;
     std::vector<GDef> graphs;
     unsigned start = 0;
 };
@@ -176,6 +218,7 @@ struct EntBatch {
     size_t maxBlock[2] = { 1, 1 };
     ozl_ews ws[2];
     std::vector<ozl_eplan> plans[2];
+    int minSS = OZL_MIN_STREAM_SIZE;   // minStreamSize of the batch's sessions
     void add(EntReq&& r);
     int run(DevArena* arena, OzlHostStage* stage, hipStream_t st);
     int expand();
@@ -205,6 +248,9 @@ struct Session {
     hipStream_t st;
     int minStreamSize;
     std::vector<EStream> streams;
+    // below minStreamSize: stored (cctx.c:1127-1140; a negative threshold
+    // stores nothing)
+    bool small(size_t bytes) const { return minStreamSize > 0 && bytes < (size_t)minStreamSize; }
     std::vector<ENode> nodes;
     int err = 0;
     EntBatch* batch = nullptr;   // entropy calls deferred to this batch (none: run at once)
@@ -431,6 +477,7 @@ struct Session {
             return 0;
         }
         EntBatch local;
+        local.minSS = minStreamSize;
         local.add(std::move(r));
         int rc = local.run(arena, nullptr, st);
         return rc ? rc : local.expand();
@@ -475,7 +522,7 @@ struct Session {
         for (uint32_t c = 0; c < B; c++) {
             const ozl_eplan& p = plans[c];
             uint32_t csid = chunks[c];
-            bool tiny = streams[csid].n < (size_t)minStreamSize;
+            bool tiny = small(streams[csid].n);
             if (!tiny && (p.bk == OZL_BK_FSE || p.bk == OZL_BK_HUF)) continue;
             uint8_t* aux = ws.aux + (size_t)c * OZL_AUX_BYTES;
             uint8_t* pay = ws.payload + (size_t)c * ws.payloadCap;
@@ -508,11 +555,11 @@ struct Session {
         for (uint32_t c = 0; c < B; c++) {
             const ozl_eplan& p = plans[c];
             uint32_t csid = chunks[c];
-            if (streams[csid].n < (size_t)minStreamSize) continue;
+            if (small(streams[csid].n)) continue;
             if (p.bk != OZL_BK_FSE && p.bk != OZL_BK_HUF) continue;
             uint8_t* aux = ws.aux + (size_t)c * OZL_AUX_BYTES;
             if (p.bk == OZL_BK_FSE) {
-                if (streams[side[c]].bytes() < (size_t)minStreamSize) {
+                if (small(streams[side[c]].bytes())) {
                     int rc = storeStream(side[c]);
                     if (rc) return rc;
                 } else {
@@ -521,7 +568,7 @@ struct Session {
                 }
             } else {
                 uint32_t m = p.maxSym + 1u;
-                if (m < (uint32_t)minStreamSize) {
+                if (small(m)) {
                     int rc = storeStream(side[c]);
                     if (rc) return rc;
                 } else {
@@ -559,7 +606,7 @@ struct Session {
                         k.hdrLen = (uint8_t)(1 + nbBytes);
                         uint32_t nc = newStream(T_NUMERIC, 2, p.wMaxSym + 1u, aux + OZL_AUX_W_SIDE);
                         uint32_t wb = newStream(T_SERIAL, 1, p.wPayload, aux + OZL_AUX_W_PAYLOAD);
-                        if (streams[nc].bytes() < (size_t)minStreamSize) {
+                        if (small(streams[nc].bytes())) {
                             int rc = storeStream(nc);
                             if (rc) return rc;
                         } else {
@@ -582,7 +629,7 @@ struct Session {
     int runSuccessor(unsigned gid, uint32_t sid)
     {
         if (err) return err;
-        if (gid != GID_SERIAL_STORE && streams[sid].bytes() < (size_t)minStreamSize) return storeStream(sid);
+        if (gid != GID_SERIAL_STORE && small(streams[sid].bytes())) return storeStream(sid);
         return runSupervised(gid, sid);
     }
 
@@ -765,6 +812,7 @@ int EntBatch::run(DevArena* arena, OzlHostStage* stage, hipStream_t st)
         OZL_HIP_CHECK(hipMemcpyAsync(dO, h ? (const void*)h : (const void*)o.data(), o.size() * 8, hipMemcpyHostToDevice,
                                      st));
         ws[m] = ozl_ews_carve(wsMem, B, maxBlock[m]);
+        ws[m].minSS = minSS;
         int rc = ozl_entropy_encode_chunks(m, (const uint8_t*)(uintptr_t)lo, dO, dO + B, B, maxBlock[m], ws[m], st);
         if (rc) return rc;
     }
@@ -855,9 +903,7 @@ ZL_Result_ZL_EdgeList ZL_Edge_runTransposeSplit(ZL_Edge* e, const ZL_Graph* grap
 // ---------------------------------------------------------------- contexts
 struct ZL_CCtx_s {
     const ZL_Compressor_s* comp = nullptr;
-    int contentChecksum = ZL_TernaryParam_auto;
-    int compressedChecksum = ZL_TernaryParam_auto;
-    int formatVersion = 0;
+    GParams params;                         // requested on the context
     DevArena arena;
     hipStream_t st = nullptr;
     hipStream_t st2 = nullptr;              // content checksums, alongside the graph work
@@ -878,16 +924,26 @@ namespace {
 
 struct FrameParams {
     int contentChecksum, compressedChecksum, formatVersion, minStreamSize;
+    int sticky = 0;
 };
 
+// Applied parameters (cctx.c:350-364, gcparams.c:124-137): a context value
+// that is set wins, then the compressor's, then the default. Checksums are
+// written unless disabled (cctx.c:1480-1486). minStreamSize is kept signed:
+// a negative value never triggers the automatic store (cctx.c:1127-1140
+// compares the stream size with it as an int).
 FrameParams resolve(const ZL_CCtx_s* c)
 {
     FrameParams f;
-    auto pick = [](int cctx, int comp) { return cctx != ZL_TernaryParam_auto ? cctx : comp; };
-    f.contentChecksum = pick(c->contentChecksum, c->comp->contentChecksum) != ZL_TernaryParam_disable;
-    f.compressedChecksum = pick(c->compressedChecksum, c->comp->compressedChecksum) != ZL_TernaryParam_disable;
-    f.formatVersion = c->formatVersion ? c->formatVersion : c->comp->formatVersion;
-    f.minStreamSize = c->comp->minStreamSize;
+    auto pick = [](int cctx, int comp, int dflt) { return cctx ? cctx : comp ? comp : dflt; };
+    const GParams& a = c->params;
+    const GParams& b = c->comp->params;
+    f.contentChecksum = pick(a.contentChecksum, b.contentChecksum, ZL_TernaryParam_enable) != ZL_TernaryParam_disable;
+    f.compressedChecksum =
+        pick(a.compressedChecksum, b.compressedChecksum, ZL_TernaryParam_enable) != ZL_TernaryParam_disable;
+    f.formatVersion = pick(a.formatVersion, b.formatVersion, 0);
+    f.minStreamSize = pick(a.minStreamSize, b.minStreamSize, ZL_MINSTREAMSIZE_DEFAULT);
+    f.sticky = a.sticky;
     return f;
 }
 
@@ -1034,6 +1090,7 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
     // the kernels read the caller's buffers
     OzlStreamsDone done{ st, c->st2 };
     EntBatch eb;
+    eb.minSS = fp.minStreamSize;
     const int cc = fp.contentChecksum, kc = fp.compressedChecksum;
     std::vector<const uint8_t*> dIn(n);
     std::vector<size_t> inBytes(n);
@@ -1279,15 +1336,8 @@ void ZL_Compressor_free(ZL_Compressor* c) { delete c; }
 ZL_Report ZL_Compressor_setParameter(ZL_Compressor* c, ZL_CParam p, int v)
 {
     if (!c) return rep_err(ZL_ErrorCode_parameter_invalid);
-    switch (p) {
-    case ZL_CParam_formatVersion: c->formatVersion = v; break;
-    case ZL_CParam_contentChecksum: c->contentChecksum = v; break;
-    case ZL_CParam_compressedChecksum: c->compressedChecksum = v; break;
-    case ZL_CParam_minStreamSize: c->minStreamSize = v; break;
-    case ZL_CParam_compressionLevel: break;
-    default: return rep_err(ZL_ErrorCode_parameter_invalid);
-    }
-    return rep_ok(0);
+    int rc = c->params.set(p, v);
+    return rc ? rep_err(rc) : rep_ok(0);
 }
 
 static ZL_GraphID add_graph(ZL_Compressor* c, GDef g)
@@ -1468,12 +1518,18 @@ ZL_Report ZL_CCtx_refCompressor(ZL_CCtx* c, const ZL_Compressor* comp)
 ZL_Report ZL_CCtx_setParameter(ZL_CCtx* c, ZL_CParam p, int v)
 {
     if (!c) return rep_err(ZL_ErrorCode_parameter_invalid);
-    switch (p) {
-    case ZL_CParam_formatVersion: c->formatVersion = v; break;
-    case ZL_CParam_contentChecksum: c->contentChecksum = v; break;
-    case ZL_CParam_compressedChecksum: c->compressedChecksum = v; break;
-    default: return rep_err(ZL_ErrorCode_parameter_invalid);
-    }
+    int rc = c->params.set(p, v);
+    return rc ? rep_err(rc) : rep_ok(0);
+}
+
+int ZL_CCtx_getParameter(const ZL_CCtx* c, ZL_CParam p) { return c ? c->params.get(p) : 0; }
+
+// cctx.c:293-314: the requested parameters and the compressor reference go.
+ZL_Report ZL_CCtx_resetParameters(ZL_CCtx* c)
+{
+    if (!c) return rep_err(ZL_ErrorCode_parameter_invalid);
+    c->params = GParams();
+    c->comp = nullptr;
     return rep_ok(0);
 }
 
@@ -1482,11 +1538,34 @@ ZL_TypedRef* ZL_TypedRef_createStruct(const void* src, size_t w, size_t n) { ret
 ZL_TypedRef* ZL_TypedRef_createNumeric(const void* src, size_t w, size_t n) { return new ZL_TypedRef_s{ src, T_NUMERIC, w, n }; }
 void ZL_TypedRef_free(ZL_TypedRef* t) { delete t; }
 
+// Every compression call ends like the reference's (compress2.c:124-146):
+// unless the context's parameters are sticky, they are reset, compressor
+// reference included, whether the call succeeded or not.
+static int compress_call(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void* dst, size_t cap, size_t* sizes)
+{
+    if (!c) return ZL_ErrorCode_parameter_invalid;
+    // compress2.c:349: a compressor must be referenced
+    int rc = c->comp ? compress_frames(c, ins, n, dst, cap, sizes) : ZL_ErrorCode_compressionParameter_invalid;
+    if (!c->params.sticky) ZL_CCtx_resetParameters(c);
+    return rc;
+}
+
 ZL_Report ZL_CCtx_compressTypedRef(ZL_CCtx* c, void* dst, size_t dstCapacity, const ZL_TypedRef* in)
 {
     size_t size = 0;
-    int rc = compress_frames(c, &in, 1, dst, dstCapacity, &size);
+    int rc = compress_call(c, &in, 1, dst, dstCapacity, &size);
     return rc ? rep_err(rc) : rep_ok(size);
+}
+
+ZL_Report ZL_CCtx_compressMultiTypedRef(ZL_CCtx* c, void* dst, size_t dstCapacity, const ZL_TypedRef* inputs[],
+                                        size_t nbInputs)
+{
+    if (!c || !inputs || nbInputs == 0) return rep_err(ZL_ErrorCode_parameter_invalid);
+    if (nbInputs > 1) {
+        if (!c->params.sticky) ZL_CCtx_resetParameters(c);
+        return rep_err(ZL_ErrorCode_temporaryLibraryLimitation);
+    }
+    return ZL_CCtx_compressTypedRef(c, dst, dstCapacity, inputs[0]);
 }
 
 ZL_Report ozl_CCtx_compressBatch(ZL_CCtx* c, void* dst, size_t dstCapacity, size_t* frameSizes,
@@ -1494,7 +1573,7 @@ ZL_Report ozl_CCtx_compressBatch(ZL_CCtx* c, void* dst, size_t dstCapacity, size
 {
     if (!frameSizes || (!inputs && nbInputs)) return rep_err(ZL_ErrorCode_parameter_invalid);
     if (nbInputs == 0) return rep_ok(0);
-    int rc = compress_frames(c, inputs, nbInputs, dst, dstCapacity, frameSizes);
+    int rc = compress_call(c, inputs, nbInputs, dst, dstCapacity, frameSizes);
     if (rc) return rep_err(rc);
     size_t total = 0;
     for (size_t i = 0; i < nbInputs; i++) total += frameSizes[i];
@@ -1503,6 +1582,11 @@ ZL_Report ozl_CCtx_compressBatch(ZL_CCtx* c, void* dst, size_t dstCapacity, size
 
 ZL_Report ZL_CCtx_compress(ZL_CCtx* c, void* dst, size_t cap, const void* src, size_t n)
 {
+    if (c && !c->comp) {
+        // the reference's default is its generic LZ graph (compress2.c:377-379)
+        if (!c->params.sticky) ZL_CCtx_resetParameters(c);
+        return rep_err(ZL_ErrorCode_graph_invalid);
+    }
     ZL_TypedRef_s t{ src, T_SERIAL, 1, n };
     return ZL_CCtx_compressTypedRef(c, dst, cap, &t);
 }

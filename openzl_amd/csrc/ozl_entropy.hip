@@ -93,10 +93,10 @@ struct PlanLds {
 
 // Nested FSE graph on the Huffman weights (entropyCompressChunk in EBM_fse
 // mode on the num->serial converted weights stream).
-__device__ __forceinline__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux)
+__device__ __forceinline__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux, int minSS)
 {
     uint32_t m = p.maxSym + 1u;
-    if (m < OZL_MIN_STREAM_SIZE) { p.wbk = 255; return; }
+    if (ozl_small(m, minSS)) { p.wbk = 255; return; }
     for (int s = 0; s < 256; s++) L.wcount[s] = 0;
     for (uint32_t i = 0; i < m; i++) L.wcount[L.wts[i]]++;
     unsigned wms, wcard;
@@ -130,7 +130,7 @@ __device__ __forceinline__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* 
         size_t bits = ozl_fse_encode_seq(pay, OZL_AUX_BYTES - OZL_AUX_W_PAYLOAD, L.wts, m, L.wst, L.wtt, tl, 2);
         if (bits == 0) { p.status = OZL_ERR_GENERIC; return; }
         p.wPayload = (uint16_t)bits;
-        if (2u * (wms + 1u) < OZL_MIN_STREAM_SIZE) {
+        if (ozl_small(2u * (wms + 1u), minSS)) {
             for (unsigned s = 0; s <= wms; s++) { side[2 * s] = (uint8_t)wn[s]; side[2 * s + 1] = (uint8_t)((uint16_t)wn[s] >> 8); }
             p.wAux = (uint16_t)(2 * (wms + 1));
         } else {
@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, co
         uint8_t* aux = L.aux;
         const unsigned maxSym = ms, card = cd;
         int bk = ozl_decide_from_stats(tsum, n, maxSym, card, HUF);
-        if (n < OZL_MIN_STREAM_SIZE) bk = OZL_BK_STORE;
+        if (ozl_small(n, ws.minSS)) bk = OZL_BK_STORE;
         p.bk = (uint8_t)bk;
         p.maxSym = (uint8_t)maxSym;
         p.card = (uint16_t)card;
@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, co
             p.tl = (uint8_t)tl;
             p.nbStates = n < 1000 ? 2 : 4;
             if (tl > 11 || ozl_fse_normalize(L.norm, tl, L.count, n, maxSym) != 0) { p.status = OZL_ERR_GENERIC; break; }
-            if (2u * (maxSym + 1u) < OZL_MIN_STREAM_SIZE) {
+            if (ozl_small(2u * (maxSym + 1u), ws.minSS)) {
                 for (unsigned s = 0; s <= maxSym; s++) { aux[2 * s] = (uint8_t)L.norm[s]; aux[2 * s + 1] = (uint8_t)((uint16_t)L.norm[s] >> 8); }
                 p.aux = (uint16_t)(2 * (maxSym + 1));
             } else {
@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, co
                 L.wts[s] = w;
                 aux[s] = w;
             }
-            plan_weights(p, L, aux);
+            plan_weights(p, L, aux, ws.minSS);
             break;
         }
         default: break;

@@ -12,7 +12,12 @@ struct ozl_ews {
     uint8_t* payload;
     uint64_t* frameOff;  // B+1 (frame path only)
     size_t payloadCap;
+    int minSS;           // ZL_CParam_minStreamSize in force (<= 0: no automatic store)
 };
+
+// A stream of this many bytes is stored by the minStreamSize rule
+// (src/openzl/compress/cctx.c:1127-1140).
+__host__ __device__ inline bool ozl_small(uint32_t bytes, int minSS) { return minSS > 0 && bytes < (uint32_t)minSS; }
 
 // Payload capacity per block of at most maxBlock bytes.
 static inline size_t ozl_payload_cap(size_t maxBlock) { return ((maxBlock + maxBlock / 128 + 256) + 15) & ~(size_t)15; }
@@ -28,6 +33,7 @@ static inline ozl_ews ozl_ews_carve(void* workspace, uint32_t B, size_t maxBlock
     ws.aux = w; w += (size_t)B * OZL_AUX_BYTES;
     ws.payload = w;
     ws.frameOff = nullptr;
+    ws.minSS = OZL_MIN_STREAM_SIZE;
     return ws;
 }
 

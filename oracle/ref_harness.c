@@ -84,6 +84,12 @@ static ZL_Report fixedChunks(ZL_Segmenter* sctx)
 
 static ZL_GraphID buildGraph(ZL_Compressor* c, int kind);
 
+/* ZL_CParam_minStreamSize set on the compressor of every later refh_compress
+ * call (0: left unset, i.e. the default of 10) */
+static int g_minStream = 0;
+
+void refh_set_min_stream_size(int v) { g_minStream = v; }
+
 static ZL_GraphID buildSegmenter(ZL_Compressor* c)
 {
     static const ZL_Type anyType = (ZL_Type)(ZL_Type_serial | ZL_Type_struct | ZL_Type_numeric);
@@ -167,6 +173,7 @@ long long refh_compress(int kind, int type, int width, const void* src, size_t n
         ZL_Compressor_setParameter(c, ZL_CParam_contentChecksum, ZL_TernaryParam_disable);
     if (!compressedChecksum)
         ZL_Compressor_setParameter(c, ZL_CParam_compressedChecksum, ZL_TernaryParam_disable);
+    if (g_minStream) ZL_Compressor_setParameter(c, ZL_CParam_minStreamSize, g_minStream);
     ZL_GraphID g = buildGraph(c, kind);
     if (!ZL_GraphID_isValid(g)) goto out;
     if (ZL_isError(ZL_Compressor_selectStartingGraphID(c, g))) goto out;

@@ -75,6 +75,32 @@ def small_cases():
     return out
 
 
+def param_cases():
+    """Cases compressed with compressor parameters: (name, kind, type, width,
+    generator, kwargs, params). minStreamSize 0 is the default, 5 and 64 move
+    the automatic store, -1 turns it off (include/openzl/zl_compress.h:117-121);
+    the skewed 8-symbol Huffman input has an 8-byte weights stream, stored
+    below 10 and FSE-coded at 5 or with the store off."""
+    out = []
+    for ms in (0, 5, -1, 64):
+        tag = "m" if ms < 0 else ""
+        p = {"minStreamSize": ms}
+        out.append((f"ms{tag}{abs(ms)}_skew8_huf_3000", C.HUFFMAN, 0, 1, "zipf_bytes",
+                    {"n": 3000, "s": 2.0, "card": 8, "seed": 11}, p))
+        out.append((f"ms{tag}{abs(ms)}_skew8_huf_40", C.HUFFMAN, 0, 1, "zipf_bytes",
+                    {"n": 40, "s": 2.0, "card": 8, "seed": 12}, p))
+        for n in (1, 5, 17, 500):
+            out.append((f"ms{tag}{abs(ms)}_zipf_fse_{n}", C.FSE, 0, 1, "zipf_bytes", {"n": n, "s": 1.1, "seed": n}, p))
+            out.append((f"ms{tag}{abs(ms)}_zipf_huf_{n}", C.HUFFMAN, 0, 1, "zipf_bytes", {"n": n, "s": 1.1, "seed": n}, p))
+        out.append((f"ms{tag}{abs(ms)}_ts_delta_ts_fse_1000", C.DELTA_TS_FSE, 2, 4, "timestamps_u32", {"n": 1000}, p))
+        out.append((f"ms{tag}{abs(ms)}_col_delta_zz_bp_16", C.DELTA_ZZ_BP, 2, 2, "walk_column",
+                    {"n": 16, "width": 2, "b": 2, "seed": 16}, p))
+        out.append((f"ms{tag}{abs(ms)}_bf16_ts_huf_fse_3000", C.TS_HUF_FSE, 1, 2, "bf16_normal", {"n": 3000}, p))
+        out.append((f"ms{tag}{abs(ms)}_zipf_huf_100000", C.HUFFMAN, 0, 1, "zipf_bytes",
+                    {"n": 100000, "s": 1.1, "seed": 100000}, p))
+    return out
+
+
 def make_input(gen, kw):
     kw = dict(kw)
     if "dtype" in kw:
@@ -90,17 +116,25 @@ def write_small():
     os.makedirs(os.path.join(HERE, "small"), exist_ok=True)
     entries = []
     total = 0
-    for name, kind, typ, width, gen, kw, _ in small_cases():
+    cases = [c[:6] + (None,) for c in small_cases()] + param_cases()
+    for name, kind, typ, width, gen, kw, params in cases:
         x = make_input(gen, kw)
-        fr = C.ref_compress(kind, x, typ, width)
+        C.ref_set_min_stream_size((params or {}).get("minStreamSize", 0))
+        try:
+            fr = C.ref_compress(kind, x, typ, width)
+        finally:
+            C.ref_set_min_stream_size(0)
         back, t, w = C.ref_decompress(fr, x.nbytes + 64)
         assert back == x.tobytes(), name
         with open(os.path.join(HERE, "small", name + ".zl"), "wb") as f:
             f.write(fr)
         total += len(fr)
-        entries.append({"name": name, "kind": kind, "type": typ, "width": width, "gen": gen, "kwargs": kw,
-                        "input_bytes": int(x.nbytes), "input_sha256": sha(x.tobytes()),
-                        "frame_bytes": len(fr), "frame_sha256": sha(fr)})
+        e = {"name": name, "kind": kind, "type": typ, "width": width, "gen": gen, "kwargs": kw,
+             "input_bytes": int(x.nbytes), "input_sha256": sha(x.tobytes()),
+             "frame_bytes": len(fr), "frame_sha256": sha(fr)}
+        if params:
+            e["params"] = params
+        entries.append(e)
     with open(os.path.join(HERE, "small.json"), "w") as f:
         json.dump(entries, f, indent=1)
     print(f"small: {len(entries)} frames, {total} bytes")

@@ -161,6 +161,13 @@ STORE, FSE, HUFFMAN, BITPACK, DELTA_TS_FSE, DELTA_ZZ_BP, TS_HUF_FSE, DELTA_FSE, 
     SEGMENTED = range(14)
 
 
+def ref_set_min_stream_size(v):
+    """ZL_CParam_minStreamSize of the reference harness's later compressions (0 = default)."""
+    h = refh()
+    h.refh_set_min_stream_size.argtypes = [ctypes.c_int]
+    h.refh_set_min_stream_size(int(v))
+
+
 def ref_set_segmenter(kind, chunk_elts):
     """The graph and chunk size (elements) of the reference harness's SEGMENTED kind."""
     h = refh()

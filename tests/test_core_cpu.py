@@ -153,7 +153,8 @@ def test_dparse_recognises_golden_frames(cu):
     import struct
     seen = {"fse": 0, "huf": 0}
     for e in GC.small_entries():
-        if e["kind"] not in (1, 2) or e["input_bytes"] >= 100000:
+        # the batched decoder reads frames written with the default parameters
+        if e["kind"] not in (1, 2) or e["input_bytes"] >= 100000 or e.get("params"):
             continue
         x = GC.make_input(e["gen"], e["kwargs"])
         fr = np.frombuffer(GC.small_frame(e["name"]), np.uint8)

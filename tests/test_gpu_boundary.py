@@ -1,0 +1,190 @@
+"""The drop-in boundary through ctypes, on the GPU: the reference's
+decompression entry points (ZL_DCtx_decompress, ZL_decompress, typed buffers)
+on the golden reference frames, the context parameter life cycle of a
+compression (reset unless sticky, compressor reference dropped), the
+decompression checksum parameters, multi-input calls, and a corrupt header
+that announces a huge stream."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import golden_cases as GC
+
+pytestmark = pytest.mark.gpu
+
+ENTRIES = GC.small_entries()
+TYPES = {0: 1, 1: 2, 2: 4}
+
+
+@pytest.fixture(scope="module")
+def api():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from openzl_amd import api as A
+    A._api()
+    return A
+
+
+def _buf(b):
+    return ctypes.create_string_buffer(b, len(b))
+
+
+def test_dctx_decompress_and_zl_decompress_on_golden_frames(api):
+    """decompress2.c:1845-1882: serial frames decode, frames of other types are
+    GENERIC errors."""
+    L = api._api()
+    d = L.ZL_DCtx_create()
+    try:
+        for e in ENTRIES:
+            x = GC.make_input(e["gen"], e["kwargs"])
+            fr = GC.small_frame(e["name"])
+            print(f"ZL_DCtx_decompress {e['name']}", flush=True)
+            out = np.zeros(x.nbytes + 64, np.uint8)
+            for r in (L.ZL_DCtx_decompress(d, out.ctypes.data, out.size, _buf(fr), len(fr)),
+                      L.ZL_decompress(out.ctypes.data, out.size, _buf(fr), len(fr))):
+                if e["type"] == 0:
+                    assert r.code == 0, (e["name"], r.code)
+                    assert r.value == x.nbytes and out[:x.nbytes].tobytes() == x.tobytes(), e["name"]
+                else:
+                    assert r.code == 1, (e["name"], r.code)
+    finally:
+        L.ZL_DCtx_free(d)
+
+
+def test_typed_buffers_on_golden_frames(api):
+    """decompress2.c:1581-1700, stream.c:146-160: an empty typed buffer is
+    sized from the frame; a wrapped one must have the frame's type and width."""
+    L = api._api()
+    d = L.ZL_DCtx_create()
+    try:
+        for name in ("zipf_huf_65536", "ts_delta_ts_fse_30000", "bf16_ts_huf_fse_3000", "store_num_3000"):
+            e = next(e for e in ENTRIES if e["name"] == name)
+            x = GC.make_input(e["gen"], e["kwargs"])
+            fr = _buf(GC.small_frame(name))
+            n = len(GC.small_frame(name))
+            print(f"typed buffers {name}", flush=True)
+            t = L.ZL_TypedBuffer_create()
+            try:
+                r = L.ZL_DCtx_decompressTBuffer(d, t, fr, n)
+                assert r.code == 0 and r.value == x.nbytes
+                assert L.ZL_TypedBuffer_type(t) == TYPES[e["type"]]
+                assert L.ZL_TypedBuffer_byteSize(t) == x.nbytes
+                assert L.ZL_TypedBuffer_eltWidth(t) == (1 if e["type"] == 0 else e["width"])
+                assert L.ZL_TypedBuffer_numElts(t) == x.nbytes // L.ZL_TypedBuffer_eltWidth(t)
+                got = ctypes.string_at(L.ZL_TypedBuffer_rPtr(t), x.nbytes)
+                assert got == x.tobytes()
+            finally:
+                L.ZL_TypedBuffer_free(t)
+            # wrapped: right type and width, then a wrong one
+            out = np.zeros(x.nbytes + 16, np.uint8)
+            w = e["width"]
+            if e["type"] == 0:
+                good = L.ZL_TypedBuffer_createWrapSerial(out.ctypes.data, out.size)
+                bad = L.ZL_TypedBuffer_createWrapNumeric(out.ctypes.data, 1, out.size)
+            elif e["type"] == 1:
+                good = L.ZL_TypedBuffer_createWrapStruct(out.ctypes.data, w, out.size // w)
+                bad = L.ZL_TypedBuffer_createWrapSerial(out.ctypes.data, out.size)
+            else:
+                good = L.ZL_TypedBuffer_createWrapNumeric(out.ctypes.data, w, out.size // w)
+                bad = L.ZL_TypedBuffer_createWrapStruct(out.ctypes.data, w, out.size // w)
+            try:
+                arr = (ctypes.c_void_p * 1)(good)
+                r = L.ZL_DCtx_decompressMultiTBuffer(d, arr, 1, fr, n)
+                assert r.code == 0 and r.value == 1
+                assert out[:x.nbytes].tobytes() == x.tobytes()
+                assert L.ZL_DCtx_decompressTBuffer(d, bad, fr, n).code == 12
+                arr2 = (ctypes.c_void_p * 2)(good, bad)
+                assert L.ZL_DCtx_decompressMultiTBuffer(d, arr2, 2, fr, n).code == 8
+            finally:
+                L.ZL_TypedBuffer_free(good)
+                L.ZL_TypedBuffer_free(bad)
+    finally:
+        L.ZL_DCtx_free(d)
+
+
+def test_checksum_parameters_and_reset(api):
+    """A frame whose content checksum is wrong fails with
+    contentChecksumWrong; with ZL_DParam_checkContentChecksum disabled (and
+    the compressed checksum, which covers that field, disabled too) it
+    decodes; parameters are dropped after the call unless sticky."""
+    L = api._api()
+    e = next(e for e in ENTRIES if e["name"] == "zipf_fse_5000")
+    x = GC.make_input(e["gen"], e["kwargs"])
+    fr = bytearray(GC.small_frame(e["name"]))
+    # the last 9 bytes are: content checksum, compressed checksum, end marker
+    fr[-9] ^= 0x01
+    fb = _buf(bytes(fr))
+    out = np.zeros(x.nbytes + 64, np.uint8)
+    d = L.ZL_DCtx_create()
+    try:
+        r = L.ZL_DCtx_decompress(d, out.ctypes.data, out.size, fb, len(fr))
+        assert r.code in (13, 14)
+        for p in (2, 3):
+            assert L.ZL_DCtx_setParameter(d, p, 2).code == 0
+        r = L.ZL_DCtx_decompress(d, out.ctypes.data, out.size, fb, len(fr))
+        assert r.code == 0 and out[:x.nbytes].tobytes() == x.tobytes()
+        assert L.ZL_DCtx_getParameter(d, 2) == 0 and L.ZL_DCtx_getParameter(d, 3) == 0   # reset
+        assert L.ZL_DCtx_decompress(d, out.ctypes.data, out.size, fb, len(fr)).code in (13, 14)
+        # sticky: kept across calls
+        assert L.ZL_DCtx_setParameter(d, 1, 1).code == 0
+        for p in (2, 3):
+            L.ZL_DCtx_setParameter(d, p, 2)
+        for _ in range(2):
+            assert L.ZL_DCtx_decompress(d, out.ctypes.data, out.size, fb, len(fr)).code == 0
+    finally:
+        L.ZL_DCtx_free(d)
+
+
+def test_cctx_parameters_reset_after_compression(api):
+    """compress2.c:124-146: without stickyParameters a compression resets the
+    context (its compressor reference included): a second call without
+    refCompressor is compressionParameter_invalid. A context parameter
+    overrides the compressor's: contentChecksum disabled on the context
+    removes the 4-byte content checksum from the frame."""
+    L = api._api()
+    e = next(e for e in ENTRIES if e["name"] == "zipf_huf_5000")
+    x = GC.make_input(e["gen"], e["kwargs"])
+    comp = api.Compressor(api.HUFFMAN)
+    cctx = L.ZL_CCtx_create()
+    cap = L.ZL_compressBound(x.nbytes)
+    dst = np.zeros(cap, np.uint8)
+    tr = L.ZL_TypedRef_createSerial(x.ctypes.data, x.nbytes)
+    try:
+        assert L.ZL_CCtx_refCompressor(cctx, comp.c).code == 0
+        r = L.ZL_CCtx_compressTypedRef(cctx, dst.ctypes.data, cap, tr)
+        assert r.code == 0 and GC.sha256(dst[:r.value].tobytes()) == e["frame_sha256"]
+        assert L.ZL_CCtx_compressTypedRef(cctx, dst.ctypes.data, cap, tr).code == 20
+        assert L.ZL_CCtx_refCompressor(cctx, comp.c).code == 0
+        assert L.ZL_CCtx_setParameter(cctx, api.CP_CSUM, 2).code == 0
+        r2 = L.ZL_CCtx_compressTypedRef(cctx, dst.ctypes.data, cap, tr)
+        assert r2.code == 0 and r2.value == e["frame_bytes"] - 4
+        # multi-input entry point: one input is the single-input frame
+        assert L.ZL_CCtx_refCompressor(cctx, comp.c).code == 0
+        ins = (ctypes.c_void_p * 2)(tr, tr)
+        r3 = L.ZL_CCtx_compressMultiTypedRef(cctx, dst.ctypes.data, cap, ins, 1)
+        assert r3.code == 0 and GC.sha256(dst[:r3.value].tobytes()) == e["frame_sha256"]
+        assert L.ZL_CCtx_refCompressor(cctx, comp.c).code == 0
+        assert L.ZL_CCtx_compressMultiTypedRef(cctx, dst.ctypes.data, cap, ins, 2).code == 81
+    finally:
+        L.ZL_TypedRef_free(tr)
+        L.ZL_CCtx_free(cctx)
+
+
+def test_corrupt_constant_count_is_rejected_before_allocation(api):
+    """A constant node whose varint count announces ~2 M elements in a frame
+    of 200,000 bytes is corruption, reported before anything is allocated or
+    filled."""
+    L = api._api()
+    fr = bytearray(GC.small_frame("const_fse_200000"))
+    i = fr.find(b"\x80\x80\x02")   # varint(32768): one constant chunk's count
+    assert i > 0
+    fr[i:i + 3] = b"\xff\xff\x7f"
+    out = np.zeros(200064, np.uint8)
+    d = L.ZL_DCtx_create()
+    try:
+        r = L.ZL_DCtx_decompress(d, out.ctypes.data, out.size, _buf(bytes(fr)), len(fr))
+        assert r.code == 12, r.code
+    finally:
+        L.ZL_DCtx_free(d)

@@ -38,7 +38,7 @@ def test_small_engine_frames_are_golden(api):
     for e in ENTRIES:
         x = GC.make_input(e["gen"], e["kwargs"])
         print(f"golden engine {e['name']} kind={e['kind']} bytes={x.nbytes}", flush=True)
-        got = api.Compressor(e["kind"]).compress(x, TYPES[e["type"]], e["width"])
+        got = api.Compressor(e["kind"], params=e.get("params")).compress(x, TYPES[e["type"]], e["width"])
         if GC.sha256(got) != e["frame_sha256"]:
             bad.append((e["name"], len(got), e["frame_bytes"]))
             continue
