@@ -406,6 +406,16 @@ size_t ozl_entropy_decompress_workspace(uint32_t nbBlocks);
 int ozl_entropy_decompress_batch(const void* frames, const uint64_t* frameOffsets, uint32_t nbFrames, void* dst,
                                  const uint64_t* dstOffsets, int* status, void* workspace, size_t workspaceSize,
                                  void* stream);
+/* No counterpart in the reference: ozl_entropy_decompress_batch for callers
+ * that cannot allocate (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc). The workspace is the library's own device array of
+ * 9 MiB per GPU (held until the process ends), so at most
+ * ozl_entropy_decompress_noalloc_max_frames() = 16384 frames per call; more
+ * is internalBuffer_tooSmall. Calls that overlap on one GPU share that array
+ * and race. */
+int ozl_entropy_decompress_batch_noalloc(const void* frames, const uint64_t* frameOffsets, uint32_t nbFrames,
+                                         void* dst, const uint64_t* dstOffsets, int* status, void* stream);
+size_t ozl_entropy_decompress_noalloc_max_frames(void);
 
 /* Batched numeric columns, one frame per column, with the graph
  * ZL_NODE_DELTA_INT -> ZL_NODE_ZIGZAG -> ZL_GRAPH_BITPACK: the frame

@@ -24,6 +24,8 @@
 // loads issued one staging step ahead of use, so no global-memory latency
 // sits on a decoding lane's dependency chain. Decoded bytes leave in 4-byte
 // stores.
+#include <mutex>
+
 #include "ozl_internal.h"
 #include "ozl_core.h"
 #include "ozl_xxh3.h"
@@ -921,6 +923,63 @@ int ozl_entropy_decompress_batch(const void* frames, const uint64_t* foff, uint3
     OZL_HIP_CHECK(hipGetLastError());
     return OZL_OK;
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free batch decode
+// ozl_entropy_decompress_batch's workspace (one plan and one side table per
+// frame) for at most OZL_NOALLOC_MAX_FRAMES frames, as a file-scope device
+// array: each GPU the library runs on holds its own 9 MiB copy until the
+// process ends. Host code never passes the array's name to a kernel (on the
+// host it is only a stand-in for the symbol): the device address comes from
+// hipGetSymbolAddress, looked up once per device with that device current.
+#define OZL_NOALLOC_MAX_FRAMES 16384u
+#define OZL_NOALLOC_MAX_DEVICES 64
+constexpr size_t kNoAllocBytes = (size_t)OZL_NOALLOC_MAX_FRAMES * (sizeof(ozl_dplan) + OZL_DSIDE_BYTES) + 4096;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+std::mutex g_scratchMu;
+void* g_scratchPtr[OZL_NOALLOC_MAX_DEVICES];   // device address of `scratch` per device
+
+int noalloc_scratch(void** out)
+{
+    int dev = 0;
+    OZL_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= OZL_NOALLOC_MAX_DEVICES) return OZL_ERR_PARAMETER_INVALID;
+    std::lock_guard<std::mutex> g(g_scratchMu);
+    if (!g_scratchPtr[dev]) {
+        void* p = nullptr;
+        OZL_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return OZL_ERR_HIP;
+        g_scratchPtr[dev] = p;
+    }
+    *out = g_scratchPtr[dev];
+    return OZL_OK;
+}
+}  // namespace
+
+extern "C" {
+
+// ozl_entropy_decompress_batch with the library's own scratch instead of a
+// workspace argument: no allocation of any kind, for callers that cannot
+// allocate. nbFrames above OZL_NOALLOC_MAX_FRAMES is internalBuffer_tooSmall.
+// Calls that overlap on one GPU share its scratch and race.
+int ozl_entropy_decompress_batch_noalloc(const void* frames, const uint64_t* frameOffsets, uint32_t nbFrames,
+                                         void* dst, const uint64_t* dstOffsets, int* status, void* stream)
+{
+    if (nbFrames == 0) return OZL_OK;
+    if (nbFrames > OZL_NOALLOC_MAX_FRAMES) return OZL_ERR_INTERNAL_BUFFER;
+    void* ws = nullptr;
+    int rc = noalloc_scratch(&ws);
+    if (rc) return rc;
+    // every plan and side table the kernels read is written by k_dparse in
+    // the same call: nothing of the previous call's contents is read
+    return ozl_entropy_decompress_batch(frames, frameOffsets, nbFrames, dst, dstOffsets, status, ws, kNoAllocBytes,
+                                        stream);
+}
+
+size_t ozl_entropy_decompress_noalloc_max_frames(void) { return OZL_NOALLOC_MAX_FRAMES; }
 
 }  // extern "C"
 
