@@ -21,6 +21,7 @@
 #include "ozl_profile.h"
 #include "ozl_engine_util.h"
 #include "ozl_hparse.h"
+#include "ozl_entropy16_api.h"
 
 extern "C" {
 int ozl_delta_decode(const void* first, const void* deltas, size_t n, int width, void* dst, void* scratch, void* stream);
@@ -333,6 +334,7 @@ struct Decoder {
             return 0;
         }
         case OZL_TR_CONV_NUM_TO_STRUCT_LE:
+        case OZL_TR_CONV_STRUCT_TO_NUM_LE:
         case OZL_TR_CONV_STRUCT_TO_SERIAL:
         case OZL_TR_CONV_NUM_TO_SERIAL_LE: {
             // byte-identical: the regenerated stream aliases the decoded one,
@@ -353,6 +355,9 @@ struct Decoder {
                 const int w = 1 << nd.hdr[0];
                 if (s.bytes % w) return OZL_ERR_CORRUPTION;
                 width = w;
+            } else if (nd.trid == OZL_TR_CONV_STRUCT_TO_NUM_LE) {   // numeric -> struct, same width
+                if (s.type != T_NUMERIC_) return OZL_ERR_CORRUPTION;
+                type = T_STRUCT_;
             }
             in = s;
             in.type = type;
@@ -497,7 +502,25 @@ struct Decoder {
             }
             break;
         }
-        case OZL_TR_SPLITN: {  // decode_splitN_binding.c:16 -- concatenation
+        case OZL_TR_SPLITN:
+        case OZL_TR_SPLITN_STRUCT:
+        case OZL_TR_SPLITN_NUM: {  // decode_splitN_binding.c:16 -- concatenation
+            // segments of one width; with none, the width is the header's
+            // varint (encode_splitN_binding.c:122-130)
+            int w = 1, type = T_SERIAL_;
+            if (nd.trid != OZL_TR_SPLITN) {
+                type = nd.trid == OZL_TR_SPLITN_STRUCT ? T_STRUCT_ : T_NUMERIC_;
+                if (nd.nbVO) {
+                    w = S[o0].width;
+                    for (uint32_t k = 0; k < nd.nbVO; k++)
+                        if (S[o0 + k].width != w || S[o0 + k].type != type) return OZL_ERR_CORRUPTION;
+                } else {
+                    uint64_t v = 0;
+                    for (size_t k = 0; k < nd.hdr.size() && k < 9; k++) v |= (uint64_t)(nd.hdr[k] & 127) << (7 * k);
+                    if (v == 0 || v > 0xFFFF) return OZL_ERR_CORRUPTION;
+                    w = (int)v;
+                }
+            }
             size_t total = 0;
             for (uint32_t k = 0; k < nd.nbVO; k++) total += S[o0 + k].bytes;
             in.dev = out_alloc(nd, total);
@@ -515,11 +538,78 @@ struct Decoder {
             }
             gather(srcs, sz, offs, in.dev);
             in.bytes = total;
-            in.type = T_SERIAL_;
-            in.width = 1;
+            in.type = type;
+            in.width = w;
             in.ready = false;
             in.pending = true;   // the concatenation goes out with the queue
             return 0;
+        }
+        case OZL_TR_CONSTANT_FIXED: {  // decode_constant_binding.c:10-47
+            const DStream& v = S[o0];
+            uint64_t n = 0;
+            for (size_t k = 0; k < nd.hdr.size() && k < 10; k++) n |= (uint64_t)(nd.hdr[k] & 127) << (7 * k);
+            const int w = v.width;
+            if (v.type != T_STRUCT_ || w < 1 || v.bytes != (size_t)w || n < 1 || n > limit) return OZL_ERR_CORRUPTION;
+            in.dev = out_alloc(nd, n * w);
+            if (err) return err;
+            rc = ozl_fill_fixed(in.dev, v.dev, w, n, st);
+            if (rc) return rc;
+            in.bytes = n * w;
+            in.type = T_STRUCT_;
+            in.width = w;
+            break;
+        }
+        case OZL_TR_TOKENIZE_FIXED: {  // decode_tokenize_binding.c:12-37
+            const DStream& a = S[o0];
+            const DStream& ix = S[o0 + 1];
+            const int w = a.width, iw = ix.width;
+            if (a.type != T_STRUCT_ || ix.type != T_NUMERIC_ || w < 1 || (iw != 1 && iw != 2 && iw != 4))
+                return OZL_ERR_CORRUPTION;
+            const size_t card = a.bytes / w, n = ix.bytes / iw;
+            if (n > limit / (size_t)w) return OZL_ERR_CORRUPTION;
+            in.dev = out_alloc(nd, n * w);
+            int* dst = (int*)alloc(4);
+            if (err) return err;
+            OZL_HIP_CHECK(hipMemsetAsync(dst, 0, 4, st));
+            rc = ozl_tokenize_decode(a.dev, (uint32_t)card, w, ix.dev, iw, n, in.dev, dst, st);
+            if (rc) return rc;
+            q->statuses.push_back({ dst, 1 });
+            in.bytes = n * w;
+            in.type = T_STRUCT_;
+            in.width = w;
+            break;
+        }
+        case OZL_TR_HUFFMAN_STRUCT_V2: {  // decode_entropy_binding.c:415-472
+            const DStream& wv = S[o0];
+            const DStream& bt = S[o0 + 1];
+            if (wv.type != T_NUMERIC_ || wv.width != 1 || nd.hdr.size() < 2 || nd.hdr.size() > 9)
+                return OZL_ERR_CORRUPTION;
+            uint64_t n = 0;
+            for (size_t k = 1; k < nd.hdr.size(); k++) n |= (uint64_t)nd.hdr[k] << (8 * (k - 1));
+            if (n < 2 || n > limit / 2 || n > 0xFFFFFFFFull || bt.bytes > 0xFFFFFFFFull || wv.bytes > 65536)
+                return OZL_ERR_CORRUPTION;
+            in.dev = out_alloc(nd, 2 * n);
+            uint8_t* ws = alloc(2 * wv.bytes + 16);
+            ozl_h16dec* dIt = (ozl_h16dec*)alloc(sizeof(ozl_h16dec));
+            int* dst = (int*)alloc(4);
+            if (err) return err;
+            ozl_h16dec it;
+            it.weights = (uint64_t)(uintptr_t)wv.dev;
+            it.bits = (uint64_t)(uintptr_t)bt.dev;
+            it.out = (uint64_t)(uintptr_t)in.dev;
+            it.ws = (uint64_t)(uintptr_t)ws;
+            it.nbWeights = (uint32_t)wv.bytes;
+            it.bitsLen = (uint32_t)bt.bytes;
+            it.n = (uint32_t)n;
+            it.x4 = nd.hdr[0] & 1;
+            OZL_HIP_CHECK(hipMemcpyAsync(dIt, host_keep(&it, sizeof(it)), sizeof(it), hipMemcpyHostToDevice, st));
+            rc = ozl_e16_huf_decode(dIt, 1, dst, st);
+            if (rc) return rc;
+            q->statuses.push_back({ dst, 1 });
+            in.bytes = 2 * n;
+            in.type = T_STRUCT_;
+            in.width = 2;
+            break;
         }
         default: return OZL_ERR_CORRUPTION;
         }

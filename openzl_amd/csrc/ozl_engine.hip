@@ -19,6 +19,8 @@
 #include <algorithm>
 #include <memory>
 #include <numeric>
+#include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "openzl_amd.h"
@@ -27,6 +29,7 @@
 #include "ozl_xxh3.h"
 #include "ozl_frame.h"
 #include "ozl_entropy_api.h"
+#include "ozl_entropy16_api.h"
 #include "ozl_profile.h"
 #include "ozl_engine_util.h"
 
@@ -38,6 +41,7 @@ int ozl_transpose_split(const void* src, size_t n, int width, void* planes, void
 int ozl_max_async(const void* src, size_t n, int width, unsigned long long* dmax, void* stream);
 int ozl_bitpack_encode(const void* src, size_t n, int width, int nbBits, void* dst, void* stream);
 }
+int ozl_differs_async(const void* src, size_t n, int width, int* flag, hipStream_t st);
 
 namespace {
 
@@ -136,6 +140,15 @@ struct ENode {
 }  // namespace
 
 namespace {
+
+// 2-byte chunks whose reference choice is huffman_struct_v2
+// (encode_entropy_binding.c:676-682) are rejected with
+// temporaryLibraryLimitation while this is set: the table kernel
+// k_e16_huf_build faulted the GPU (memory aperture violation) on its first
+// run, and no cause has been shown on the CPU, where the same kernels over
+// the same input match the oracle bit for bit with no out-of-bounds access.
+// The decoder of huffman_struct_v2 frames is not affected.
+constexpr bool kHuf16EncodeOff = true;
 
 // Global compression parameters as requested on a compressor or a context
 // (src/openzl/compress/gcparams.c:15-100): 0 means "not set".
@@ -405,17 +418,22 @@ struct Session {
 
     // ZL_GRAPH_BITPACK: selector to bitpack_int (numeric) or bitpack_serial
     // (encode_bitpack_binding.c:92-165), then serial_store.
-    int runBitpack(uint32_t sid)
+    int runBitpack(uint32_t sid, long long knownMax = -1)
     {
         EStream s = streams[sid];
-        unsigned long long* dmax = (unsigned long long*)alloc(8);
-        if (err) return err;
-        OZL_HIP_CHECK(hipMemsetAsync(dmax, 0, 8, st));
-        int rc = ozl_max_async(s.dev, s.n, s.width, dmax, st);
-        if (rc) return rc;
+        int rc = 0;
         unsigned long long mx = 0;
-        OZL_HIP_CHECK(hipMemcpyAsync(&mx, dmax, 8, hipMemcpyDeviceToHost, st));
-        OZL_HIP_CHECK(hipStreamSynchronize(st));
+        if (knownMax >= 0) {
+            mx = (unsigned long long)knownMax;
+        } else {
+            unsigned long long* dmax = (unsigned long long*)alloc(8);
+            if (err) return err;
+            OZL_HIP_CHECK(hipMemsetAsync(dmax, 0, 8, st));
+            rc = ozl_max_async(s.dev, s.n, s.width, dmax, st);
+            if (rc) return rc;
+            OZL_HIP_CHECK(hipMemcpyAsync(&mx, dmax, 8, hipMemcpyDeviceToHost, st));
+            OZL_HIP_CHECK(hipStreamSynchronize(st));
+        }
         int nb = 1 + (mx ? ozl_hb64(mx) : 0);
         size_t size = (s.n * (size_t)nb + 7) / 8;
         uint8_t* d = alloc(size + 8);
@@ -443,6 +461,7 @@ struct Session {
     int runEntropy(uint32_t sid, int huf)
     {
         EStream s = streams[sid];
+        if (huf && s.width == 2 && s.type != T_SERIAL) return runEntropy2(sid);
         if (s.type != T_SERIAL || s.width != 1) {
             // width-1 numeric/struct inputs are converted at the top of the
             // Huffman graph (doEntropyConversion, :765-812)
@@ -625,6 +644,236 @@ struct Session {
         return 0;
     }
 
+    // ---------------------------------------------------------------- 2-byte entropy graph
+    // Streams whose largest value the 2-byte path already knows (from the
+    // chunk's histogram): their bitpack needs no device reduction.
+    std::unordered_map<uint32_t, uint32_t> knownMaxes;
+    long long knownMax(uint32_t sid) const
+    {
+        auto it = knownMaxes.find(sid);
+        return it == knownMaxes.end() ? -1 : (long long)it->second;
+    }
+    // streams that are known to hold one repeated element
+    std::unordered_set<uint32_t> knownConstant;
+
+    // ZL_GRAPH_CONSTANT (selector_constant.c:13-31): constant_serial or
+    // constant_fixed by type; the single element is the stream's first.
+    int runConstant(uint32_t sid)
+    {
+        const EStream s = streams[sid];
+        if (s.n < 1) return OZL_ERR_SRCSIZE_TOOSMALL;
+        if (!knownConstant.count(sid)) {
+            int* flag = (int*)alloc(4);
+            if (err) return err;
+            OZL_HIP_CHECK(hipMemsetAsync(flag, 0, 4, st));
+            int rc = ozl_differs_async(s.dev, s.n, s.width, flag, st);
+            if (rc) return rc;
+            int h = 0;
+            OZL_HIP_CHECK(hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, st));
+            OZL_HIP_CHECK(hipStreamSynchronize(st));
+            if (h) return OZL_ERR_NODE_INVALID_INPUT;   // encode_constant_binding.c:29-30
+        }
+        ENode& nd = newNode(s.type == T_SERIAL ? OZL_TR_CONSTANT_SERIAL : OZL_TR_CONSTANT_FIXED, sid);
+        nd.hdrLen = (uint8_t)ozl_varint(s.n, nd.hdr);
+        const uint32_t v = newStream(s.type == T_SERIAL ? T_SERIAL : T_STRUCT, s.width, 1, s.dev);
+        return runSuccessor(GID_SERIAL_STORE, v);
+    }
+
+    // ZL_GRAPH_HUFFMAN on struct / numeric inputs of 2-byte elements
+    // (encode_entropy_binding.c:612-682, 775-797, 833-851): num -> token
+    // conversion, 32 Ki-element chunks from 100,000 elements on (splitn_struct),
+    // then per chunk the reference's choice from its 16-bit histogram. The
+    // nodes each chunk creates and the destinations it sets follow the
+    // reference's order; the successors run afterwards in edge order.
+    int runEntropy2(uint32_t sid)
+    {
+        EStream s = streams[sid];
+        if (s.type == T_NUMERIC) {
+            newNode(OZL_TR_CONV_NUM_TO_STRUCT_LE, sid);
+            sid = newStream(T_STRUCT, 2, s.n, s.dev);
+            s = streams[sid];
+        }
+        const size_t kChunk = 1 << 15;
+        std::vector<uint32_t> chunks;
+        if (s.n < 100000) {
+            chunks.push_back(sid);
+        } else {
+            const size_t nb = (s.n + kChunk - 1) / kChunk;
+            ENode& nd = newNode(OZL_TR_SPLITN_STRUCT, sid);
+            nd.nbVO = (uint16_t)nb;
+            for (size_t c = 0; c < nb; c++) {
+                const size_t a = c * kChunk, b = a + kChunk < s.n ? a + kChunk : s.n;
+                chunks.push_back(newStream(T_STRUCT, 2, b - a, s.dev + 2 * a));
+            }
+        }
+        const uint32_t C = (uint32_t)chunks.size();
+        enum { D_STORE, D_CONSTANT, D_BITPACK, D_TOKENIZE, D_HUF16 };
+        struct Plan {
+            int d = D_STORE;
+            ozl_s16 st{};
+            uint8_t* alpha = nullptr;
+            uint8_t* idx = nullptr;
+            uint8_t* weights = nullptr;
+            uint8_t* bits = nullptr;
+            uint32_t bitsLen = 0;
+        };
+        std::vector<Plan> P(C);
+        // statistics and codecs in groups of chunks (each group's histograms
+        // take 256 KiB per chunk)
+        const uint32_t G = 512;
+        uint32_t* hist = (uint32_t*)alloc((size_t)(C < G ? C : G) * 65536 * 4);
+        ozl_c16* dC = (ozl_c16*)alloc((size_t)G * sizeof(ozl_c16));
+        ozl_s16* dS = (ozl_s16*)alloc((size_t)G * sizeof(ozl_s16));
+        ozl_tok16* dT = (ozl_tok16*)alloc((size_t)G * sizeof(ozl_tok16));
+        ozl_h16job* dH = (ozl_h16job*)alloc((size_t)G * sizeof(ozl_h16job));
+        ozl_h16res* dR = (ozl_h16res*)alloc((size_t)G * sizeof(ozl_h16res));
+        if (err) return err;
+        for (uint32_t g0 = 0; g0 < C; g0 += G) {
+            const uint32_t g1 = g0 + G < C ? g0 + G : C;
+            std::vector<ozl_c16> hc;
+            std::vector<uint32_t> who;
+            uint32_t maxN = 0;
+            for (uint32_t c = g0; c < g1; c++) {
+                const EStream& x = streams[chunks[c]];
+                if (x.n <= 1) continue;   // :614-616 STORE
+                hc.push_back({ (uint64_t)(uintptr_t)x.dev, (uint32_t)x.n, 0 });
+                who.push_back(c);
+                maxN = x.n > maxN ? (uint32_t)x.n : maxN;
+            }
+            const uint32_t K = (uint32_t)hc.size();
+            if (!K) continue;
+            std::vector<ozl_s16> hs(K);
+            OZL_HIP_CHECK(hipMemcpyAsync(dC, hc.data(), K * sizeof(ozl_c16), hipMemcpyHostToDevice, st));
+            int rc = ozl_e16_stats(dC, K, maxN, hist, dS, st);
+            if (rc) return rc;
+            OZL_HIP_CHECK(hipMemcpyAsync(hs.data(), dS, K * sizeof(ozl_s16), hipMemcpyDeviceToHost, st));
+            OZL_HIP_CHECK(hipStreamSynchronize(st));
+            std::vector<ozl_tok16> tj;
+            std::vector<ozl_h16job> hj;
+            std::vector<uint32_t> hwho;
+            for (uint32_t k = 0; k < K; k++) {
+                Plan& p = P[who[k]];
+                p.st = hs[k];
+                const size_t n = hc[k].n;
+                const unsigned mx = hs[k].maxSym, card = hs[k].card;
+                if (hs[k].maxCount == n) { p.d = D_CONSTANT; continue; }
+                double e = ozl_entropy_finish(hs[k].termSum, n);
+                e = 1 > e ? 1 : e;
+                const size_t entropySize = (size_t)(e * (double)n + 7) / 8;
+                const size_t hdr = mx / 4 > 100 ? mx / 4 : 100;
+                const size_t huf = entropySize + hdr, store = 2 * n;
+                const size_t nbBits = (size_t)ozl_next_pow2((uint64_t)mx + 1);
+                const size_t bp = (n * nbBits + 7) / 8 + 2;
+                if (bp <= huf && bp < store) { p.d = D_BITPACK; continue; }
+                if (e > 15 || huf >= store) { p.d = D_STORE; continue; }
+                uint32_t* h = hist + (size_t)k * 65536;
+                if (card < 256) {
+                    p.d = D_TOKENIZE;
+                    p.alpha = alloc(2 * (size_t)card);
+                    p.idx = alloc(n);
+                    if (err) return err;
+                    tj.push_back({ hc[k].src, (uint64_t)(uintptr_t)h, (uint64_t)(uintptr_t)p.alpha,
+                                   (uint64_t)(uintptr_t)p.idx, (uint32_t)n, card });
+                    continue;
+                }
+                // huffman_struct_v2 encode (ozl_e16_huf_encode) is switched
+                // off: its table kernel faulted the GPU on its first run and
+                // the cause is not yet shown; such inputs are rejected
+                if (kHuf16EncodeOff) return ZL_ErrorCode_temporaryLibraryLimitation;
+                p.d = D_HUF16;
+                const size_t cap = 2 * n + 32;
+                p.weights = alloc((size_t)mx + 1);
+                p.bits = alloc(cap + 16);
+                uint8_t* ws = alloc(ozl_e16_huf_ws(card, mx));
+                if (err) return err;
+                OZL_HIP_CHECK(hipMemsetAsync(p.bits, 0, cap + 16, st));
+                hj.push_back({ hc[k].src, (uint64_t)(uintptr_t)h, (uint64_t)(uintptr_t)p.weights,
+                               (uint64_t)(uintptr_t)p.bits, (uint64_t)(uintptr_t)ws, (uint32_t)n, mx, card,
+                               n > 1000 ? 1u : 0u });
+                hwho.push_back(who[k]);
+            }
+            if (!tj.empty()) {
+                OZL_HIP_CHECK(hipMemcpyAsync(dT, tj.data(), tj.size() * sizeof(ozl_tok16), hipMemcpyHostToDevice, st));
+                rc = ozl_e16_tokenize(dT, (uint32_t)tj.size(), st);
+                if (rc) return rc;
+            }
+            if (!hj.empty()) {
+                std::vector<ozl_h16res> hr(hj.size());
+                OZL_HIP_CHECK(hipMemcpyAsync(dH, hj.data(), hj.size() * sizeof(ozl_h16job), hipMemcpyHostToDevice, st));
+                rc = ozl_e16_huf_encode(dH, (uint32_t)hj.size(), dR, st);
+                if (rc) return rc;
+                OZL_HIP_CHECK(hipMemcpyAsync(hr.data(), dR, hr.size() * sizeof(ozl_h16res), hipMemcpyDeviceToHost, st));
+                OZL_HIP_CHECK(hipStreamSynchronize(st));
+                for (size_t q = 0; q < hj.size(); q++) {
+                    if (hr[q].status) return (int)hr[q].status;
+                    P[hwho[q]].bitsLen = hr[q].size;
+                }
+            }
+            // the next group reuses the histograms and the job uploads read
+            // this group's host vectors: the group ends here
+            OZL_HIP_CHECK(hipStreamSynchronize(st));
+        }
+        // nodes and destinations, chunk by chunk (:612-682)
+        std::vector<std::pair<uint32_t, unsigned>> succ;
+        for (uint32_t c = 0; c < C; c++) {
+            const uint32_t cs = chunks[c];
+            const Plan& p = P[c];
+            const size_t n = streams[cs].n;
+            switch (p.d) {
+            case D_STORE: succ.push_back({ cs, GID_STORE }); break;
+            case D_CONSTANT:
+                knownConstant.insert(cs);
+                succ.push_back({ cs, GID_CONSTANT });
+                break;
+            case D_BITPACK: {   // runBitpack: interpret as LE numeric, then ZL_GRAPH_BITPACK
+                newNode(OZL_TR_CONV_STRUCT_TO_NUM_LE, cs);
+                const uint32_t o = newStream(T_NUMERIC, 2, n, streams[cs].dev);
+                knownMaxes[o] = p.st.maxSym;
+                succ.push_back({ o, GID_BITPACK });
+                break;
+            }
+            case D_TOKENIZE: {
+                newNode(OZL_TR_TOKENIZE_FIXED, cs);
+                const uint32_t al = newStream(T_STRUCT, 2, p.st.card, p.alpha);
+                const uint32_t ix = newStream(T_NUMERIC, 1, n, p.idx);
+                if (ozl_next_pow2((uint64_t)p.st.maxSym + 1) < 16) {
+                    newNode(OZL_TR_CONV_STRUCT_TO_NUM_LE, al);
+                    const uint32_t o = newStream(T_NUMERIC, 2, p.st.card, p.alpha);
+                    knownMaxes[o] = p.st.maxSym;
+                    succ.push_back({ o, GID_BITPACK });
+                } else {
+                    succ.push_back({ al, GID_STORE });
+                }
+                succ.push_back({ ix, GID_HUF });
+                break;
+            }
+            case D_HUF16: {   // EI_huffman_struct_v2 (:286-367)
+                ENode& nd = newNode(OZL_TR_HUFFMAN_STRUCT_V2, cs);
+                const int nbBytes = (ozl_next_pow2((uint64_t)n + 1) + 7) / 8;
+                nd.hdr[0] = n > 1000 ? 1 : 0;
+                for (int q = 0; q < nbBytes; q++) nd.hdr[1 + q] = (uint8_t)(n >> (8 * q));
+                nd.hdrLen = (uint8_t)(1 + nbBytes);
+                const uint32_t w = newStream(T_NUMERIC, 1, (size_t)p.st.maxSym + 1, p.weights);
+                const uint32_t b = newStream(T_SERIAL, 1, p.bitsLen, p.bits);
+                succ.push_back({ w, GID_FSE });
+                succ.push_back({ b, GID_STORE });
+                break;
+            }
+            default: return OZL_ERR_LOGIC;
+            }
+        }
+        // successors in edge creation order (cctx.c:656-700)
+        std::stable_sort(succ.begin(), succ.end(), [](const std::pair<uint32_t, unsigned>& x,
+                                                      const std::pair<uint32_t, unsigned>& y) {
+            return x.first < y.first;
+        });
+        for (const auto& q : succ) {
+            int rc = runSuccessor(q.second, q.first);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+
     // ---------------------------------------------------------------- graph walk
     int runSuccessor(unsigned gid, uint32_t sid)
     {
@@ -648,7 +897,8 @@ struct Session {
         case GID_STORE: return storeStream(sid);
         case GID_FSE: return runEntropy(sid, 0);
         case GID_HUF: return runEntropy(sid, 1);
-        case GID_BITPACK: return runBitpack(sid);
+        case GID_BITPACK: return runBitpack(sid, knownMax(sid));
+        case GID_CONSTANT: return runConstant(sid);
         default: break;
         }
         const GDef* g = custom(gid);

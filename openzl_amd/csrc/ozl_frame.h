@@ -28,6 +28,7 @@ enum {
     OZL_TR_BITPACK_SERIAL = 27,
     OZL_TR_BITPACK_INT = 28,
     OZL_TR_FLATPACK = 29,
+    OZL_TR_TOKENIZE_FIXED = 36,
     OZL_TR_SPLITN = 40,
     OZL_TR_CONSTANT_SERIAL = 44,
     OZL_TR_CONSTANT_FIXED = 45,
@@ -35,6 +36,7 @@ enum {
     OZL_TR_SPLITN_NUM = 48,
     OZL_TR_FSE_V2 = 49,
     OZL_TR_HUFFMAN_V2 = 50,
+    OZL_TR_HUFFMAN_STRUCT_V2 = 51,
     OZL_TR_FSE_NCOUNT = 52,
 };
 

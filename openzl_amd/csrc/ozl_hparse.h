@@ -64,10 +64,11 @@ inline int nb_singletons(uint8_t trid)
 {
     switch (trid) {
     case OZL_TR_DELTA_INT: case OZL_TR_ZIGZAG: case OZL_TR_CONV_STRUCT_TO_SERIAL: case OZL_TR_CONV_NUM_TO_STRUCT_LE:
-    case OZL_TR_CONV_NUM_TO_SERIAL_LE: case OZL_TR_BITPACK_SERIAL: case OZL_TR_BITPACK_INT:
-    case OZL_TR_CONSTANT_SERIAL: case OZL_TR_FSE_NCOUNT: return 1;
-    case OZL_TR_FLATPACK: case OZL_TR_FSE_V2: case OZL_TR_HUFFMAN_V2: return 2;
-    case OZL_TR_TRANSPOSE_SPLIT: case OZL_TR_SPLITN: return 0;
+    case OZL_TR_CONV_STRUCT_TO_NUM_LE: case OZL_TR_CONV_NUM_TO_SERIAL_LE: case OZL_TR_BITPACK_SERIAL:
+    case OZL_TR_BITPACK_INT: case OZL_TR_CONSTANT_SERIAL: case OZL_TR_CONSTANT_FIXED: case OZL_TR_FSE_NCOUNT: return 1;
+    case OZL_TR_FLATPACK: case OZL_TR_FSE_V2: case OZL_TR_HUFFMAN_V2: case OZL_TR_TOKENIZE_FIXED:
+    case OZL_TR_HUFFMAN_STRUCT_V2: return 2;
+    case OZL_TR_TRANSPOSE_SPLIT: case OZL_TR_SPLITN: case OZL_TR_SPLITN_STRUCT: case OZL_TR_SPLITN_NUM: return 0;
     default: return -1;
     }
 }

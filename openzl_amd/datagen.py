@@ -74,6 +74,44 @@ def uniform_ints(n, hi, dtype, seed=1):
     return rng.integers(0, hi, n).astype(dtype)
 
 
+def u16_spread(n, card, hi, s=1.1, seed=1, perm=True):
+    """n uint16 values over `card` distinct values (each present at least
+    once, drawn from [0, hi) when perm, else 0..card-1), the rest Zipf(s)
+    over them, shuffled."""
+    rng = np.random.default_rng(seed)
+    vals = (rng.choice(hi, card, replace=False) if perm else np.arange(card)).astype(np.uint16)
+    p = 1.0 / np.arange(1, card + 1) ** s
+    p /= p.sum()
+    x = np.concatenate([vals[:min(card, n)], vals[rng.choice(card, max(n - card, 0), p=p)]])
+    rng.shuffle(x)
+    return x
+
+
+def u16_fib(scale=2000):
+    """Values k*scale repeated Fibonacci(k) times (26 values): an unlimited
+    Huffman tree of depth ~25."""
+    fib = [1, 1]
+    while len(fib) < 26:
+        fib.append(fib[-1] + fib[-2])
+    return np.repeat(np.arange(26, dtype=np.uint16) * scale, fib)
+
+
+def u16_mixed(seed=1):
+    """200,000 uint16 values whose 32,768-element chunks differ: constant,
+    small values (bitpack), 100 values (tokenize), 1,000 values (16-bit
+    Huffman), uniform noise (store), a height-limited tree, then the rest."""
+    rng = np.random.default_rng(seed)
+    c = 32768
+    parts = [np.full(c, 4242, np.uint16),
+             rng.integers(0, 512, c).astype(np.uint16),
+             u16_spread(c, 100, 4000, 1.1, seed + 1),
+             u16_spread(c, 1000, 10000, 1.1, seed + 2),
+             rng.integers(0, 65536, c).astype(np.uint16),
+             u16_spread(c, 300, 300, 2.0, seed + 3, perm=False),
+             u16_spread(200000 - 6 * c, 700, 65536, 1.3, seed + 4)]
+    return np.concatenate(parts)
+
+
 def zipf_table(seed, s=1.1, card=256):
     """uint32 inverse-CDF thresholds and the rank -> symbol permutation."""
     p = 1.0 / np.arange(1, card + 1, dtype=np.float64) ** s

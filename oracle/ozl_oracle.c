@@ -10,6 +10,7 @@
  */
 #include "ozl_oracle.h"
 
+#include <stdlib.h>
 #include <string.h>
 
 static int highbit32(uint32_t v) { return 31 - __builtin_clz(v); }
@@ -923,6 +924,316 @@ int orc_huf_decode(uint8_t* dst, size_t n, const uint8_t* src, size_t size,
         op += cnt;
     }
     return 0;
+}
+
+/* ===================== 2-byte entropy path ===================== */
+
+/* src/openzl/shared/histogram.c:37-82 ZS_Histogram_count16: 65,536 counts,
+ * the largest value present and the number of distinct values. */
+void orc_histogram16(uint32_t* count, unsigned* maxSym, unsigned* card, const uint16_t* src, size_t n)
+{
+    memset(count, 0, 65536 * sizeof(uint32_t));
+    unsigned mx = 0, cd = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (count[src[i]]++ == 0) cd++;
+        if (src[i] > mx) mx = src[i];
+    }
+    *maxSym = mx;
+    *card = cd;
+}
+
+/* src/openzl/codecs/entropy/encode_entropy_binding.c:612-682, the choice for a
+ * struct chunk of 2-byte elements inside ZL_GRAPH_HUFFMAN: STORE for <= 1
+ * element, CONSTANT, BITPACK (via interpret-as-LE) when the packed size beats
+ * both the entropy estimate plus header and the raw size, STORE when the
+ * estimate is not below the raw size or the entropy exceeds 15 bits,
+ * TOKENIZE below 256 distinct values, else huffman_struct_v2. */
+int orc_entropy2_decide(const uint16_t* src, size_t n, double* entropyOut)
+{
+    static uint32_t count[65536];
+    if (n <= 1) return ORC_E2_STORE;
+    unsigned mx, card;
+    orc_histogram16(count, &mx, &card, src, n);
+    if (count[mx] == n) return ORC_E2_CONSTANT;
+    double e = orc_entropy(count, mx, n);
+    e = 1 > e ? 1 : e;
+    if (entropyOut) *entropyOut = e;
+    size_t entropySize = (size_t)(e * (double)n + 7) / 8;
+    size_t hdr = mx / 4 > 100 ? mx / 4 : 100;
+    size_t huf = entropySize + hdr, store = 2 * n;
+    size_t nbBits = (size_t)orc_next_pow2((uint64_t)mx + 1);
+    size_t bp = (n * nbBits + 7) / 8 + 2;
+    if (bp <= huf && bp < store) return ORC_E2_BITPACK;
+    if (e > 15 || huf >= store) return ORC_E2_STORE;
+    if (card < 256) return ORC_E2_TOKENIZE;
+    return ORC_E2_HUF16;
+}
+
+/* src/openzl/codecs/tokenize/encode_tokenize_binding.c:124-216 (tokenize of
+ * a struct stream, never sorted): the alphabet lists the distinct values in
+ * order of first occurrence, each element becomes its alphabet index (1 byte
+ * wide for alphabets of at most 256 values). Returns the alphabet size, or 0
+ * when it exceeds 256. */
+size_t orc_tokenize2(uint16_t* alphabet, uint8_t* idx, const uint16_t* src, size_t n)
+{
+    static int32_t slot[65536];
+    memset(slot, -1, sizeof(slot));
+    size_t card = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (slot[src[i]] < 0) {
+            if (card == 256) return 0;
+            slot[src[i]] = (int32_t)card;
+            alphabet[card++] = src[i];
+        }
+        idx[i] = (uint8_t)slot[src[i]];
+    }
+    return card;
+}
+
+/* ---- 16-bit Huffman (src/openzl/codecs/entropy/encode_huffman_kernel.c) ---- */
+typedef struct { uint32_t count, parent; uint16_t symbol, nbBits; } h16node;
+
+static int h16_by_count(const void* a, const void* b)
+{
+    const h16node* x = (const h16node*)a;
+    const h16node* y = (const h16node*)b;
+    if (x->count != y->count) return x->count > y->count ? -1 : 1;
+    return x->symbol < y->symbol ? -1 : x->symbol > y->symbol;
+}
+
+/* :36-140 ZS_largeHuffmanSetMaxHeight: the depths above maxNbBits are cut
+ * down and the Kraft debt is repaid by deepening the cheapest shallower
+ * symbols, then any overshoot is given back. Returns the final height. */
+static int h16_set_max_height(h16node* node, int lastNonNull, int maxNbBits)
+{
+    const int largestBits = node[lastNonNull].nbBits;
+    if (largestBits <= maxNbBits) return largestBits;
+    int cost = 0;
+    const int base = 1 << (largestBits - maxNbBits);
+    int n = lastNonNull;
+    for (; node[n].nbBits > maxNbBits; n--) {
+        cost += base - (1 << (largestBits - node[n].nbBits));
+        node[n].nbBits = (uint16_t)maxNbBits;
+    }
+    while (node[n].nbBits == maxNbBits) n--;
+    cost >>= largestBits - maxNbBits;
+    const uint32_t none = 0xF0F0F0F0u;
+    uint32_t last[22];
+    for (int r = 0; r < 22; r++) last[r] = none;
+    {
+        int cur = maxNbBits;
+        for (int pos = n; pos >= 0; pos--) {
+            if (node[pos].nbBits >= cur) continue;
+            cur = node[pos].nbBits;
+            last[maxNbBits - cur] = (uint32_t)pos;
+        }
+    }
+    while (cost > 0) {
+        int k = highbit32((uint32_t)cost) + 1;
+        while (k > 1 && last[k] == none) k--;
+        for (; k > 1; k--) {
+            const uint32_t hi = last[k], lo = last[k - 1];
+            if (lo == none) break;
+            if (node[hi].count <= 2 * node[lo].count) break;
+        }
+        while (k <= 20 && last[k] == none) k++;
+        cost -= 1 << (k - 1);
+        if (last[k - 1] == none) last[k - 1] = last[k];
+        node[last[k]].nbBits++;
+        if (last[k] == 0) {
+            last[k] = none;
+        } else {
+            last[k]--;
+            if (node[last[k]].nbBits != maxNbBits - k) last[k] = none;
+        }
+    }
+    while (cost < 0) {
+        if (last[1] == none) {
+            while (node[n].nbBits == maxNbBits) n--;
+            node[n + 1].nbBits--;
+            last[1] = (uint32_t)(n + 1);
+            cost++;
+            continue;
+        }
+        node[last[1] + 1].nbBits--;
+        last[1]++;
+        cost++;
+    }
+    return maxNbBits;
+}
+
+/* :274-329 ZS_largeHuffmanBuildCTable(ctable, count, maxSym, 0): maxNbBits 20
+ * lowered to nextPow2(maxSym + 1) + 3; symbols sorted by decreasing count
+ * (equal counts keep symbol order, :145-229); two-queue tree (:231-273);
+ * height limit; canonical values per length in symbol order, in 16-bit
+ * arithmetic as the reference's uint16_t counters (:275-301). code[s] and
+ * nbBits[s] for s <= maxSym (nbBits 0 for absent symbols). Returns the table
+ * log (the largest code length), or -1. */
+int orc_huf16_build(uint16_t* code, uint8_t* nbBits, const uint32_t* count, unsigned maxSym)
+{
+    const int S = (int)maxSym + 1;
+    h16node* tab = (h16node*)calloc((size_t)(2 * S + 3), sizeof(h16node));
+    if (!tab) return -1;
+    h16node* node = tab + 1;
+    int maxNbBits = 20;
+    {
+        int bound = orc_next_pow2((uint64_t)S);
+        if (bound + 3 < maxNbBits) maxNbBits = bound + 3;
+    }
+    for (int s = 0; s < S; s++) { node[s].count = count[s]; node[s].symbol = (uint16_t)s; }
+    qsort(node, (size_t)S, sizeof(h16node), h16_by_count);
+    int nonNull = S - 1;
+    while (node[nonNull].count == 0) nonNull--;
+    if (nonNull < 1) { free(tab); return -1; }
+    {
+        int nb = S, lowS = nonNull, lowN = nb;
+        const int root = nb + lowS - 1;
+        node[nb].count = node[lowS].count + node[lowS - 1].count;
+        node[lowS].parent = node[lowS - 1].parent = (uint32_t)nb;
+        nb++;
+        lowS -= 2;
+        for (int k = nb; k <= root; k++) node[k].count = 1u << 30;
+        node[-1].count = 1u << 31;
+        for (; nb <= root; nb++) {
+            int a = node[lowS].count < node[lowN].count ? lowS-- : lowN++;
+            int b = node[lowS].count < node[lowN].count ? lowS-- : lowN++;
+            node[nb].count = node[a].count + node[b].count;
+            node[a].parent = node[b].parent = (uint32_t)nb;
+        }
+        node[root].nbBits = 0;
+        for (int k = root - 1; k >= S; k--) node[k].nbBits = (uint16_t)(node[node[k].parent].nbBits + 1);
+        for (int k = 0; k <= nonNull; k++) node[k].nbBits = (uint16_t)(node[node[k].parent].nbBits + 1);
+    }
+    maxNbBits = h16_set_max_height(node, nonNull, maxNbBits);
+    if (maxNbBits > 20) { free(tab); return -1; }
+    {
+        uint16_t perRank[21];
+        uint16_t start[21];
+        memset(perRank, 0, sizeof(perRank));
+        memset(start, 0, sizeof(start));
+        for (int k = 0; k <= nonNull; k++) perRank[node[k].nbBits]++;
+        uint16_t mn = 0;
+        for (int r = maxNbBits; r > 0; r--) { start[r] = mn; mn = (uint16_t)(mn + perRank[r]); mn >>= 1; }
+        for (int s = 0; s < S; s++) nbBits[s] = 0;
+        for (int k = 0; k < S; k++) nbBits[node[k].symbol] = (uint8_t)node[k].nbBits;
+        for (int s = 0; s < S; s++) code[s] = start[nbBits[s]]++;
+    }
+    free(tab);
+    return maxNbBits;
+}
+
+/* :331-452 ZS_largeHuffmanEncodeUsingCTable / X4 as called by
+ * EI_huffman_struct_v2 (encode_entropy_binding.c:286-367) with a capacity of
+ * 2n + 32 bytes: each stream is [LE32 element count][LE32 byte size][bits],
+ * the bits written from the stream's last element to its first, LSB-first,
+ * closed by a 1 bit; X4 cuts streams of n/4 + 1 elements (the last gets the
+ * rest). A stream that does not fit its remaining capacity fails
+ * (bitstream.h:154-245: at least 9 bytes of capacity, and the written bytes
+ * stay 8 bytes below the end). Returns the total size or 0. */
+size_t orc_huf16_encode(uint8_t* dst, size_t cap, const uint16_t* src, size_t n, const uint16_t* code,
+                        const uint8_t* nbBits, int x4)
+{
+    size_t pos = 0, off = 0;
+    const int ns = x4 ? 4 : 1;
+    const size_t seg = n / 4 + 1;
+    for (int k = 0; k < ns; k++) {
+        size_t cnt = x4 ? (seg < n - off ? seg : n - off) : n;
+        if (cap - pos < 8) return 0;
+        size_t avail = cap - pos - 8;
+        if (avail <= 8) return 0;
+        uint8_t* b = dst + pos + 8;
+        memset(b, 0, avail);
+        uint64_t bit = 0;
+        for (size_t i = cnt; i-- > 0;) {
+            uint16_t s = src[off + i];
+            unsigned nb = nbBits[s];
+            uint32_t c = (uint32_t)code[s] & ((1u << nb) - 1);
+            for (unsigned q = 0; q < nb; q++, bit++)
+                if ((c >> q) & 1) {
+                    if ((bit >> 3) >= avail) return 0;
+                    b[bit >> 3] |= (uint8_t)(1u << (bit & 7));
+                }
+        }
+        /* end mark; the stream's full bytes must stay below its end */
+        if ((bit >> 3) >= avail) return 0;
+        b[bit >> 3] |= (uint8_t)(1u << (bit & 7));
+        bit++;
+        if ((bit >> 3) >= avail - 8) return 0;
+        size_t bytes = (size_t)((bit + 7) >> 3);
+        for (int q = 0; q < 4; q++) {
+            dst[pos + q] = (uint8_t)(cnt >> (8 * q));
+            dst[pos + 4 + q] = (uint8_t)(bytes >> (8 * q));
+        }
+        pos += 8 + bytes;
+        off += cnt;
+    }
+    return pos;
+}
+
+/* Decoder of the streams above with the table of the weights
+ * (src/openzl/codecs/entropy/decode_entropy_binding.c:348-472 buildHUF16DTable
+ * and DI_huffman_struct_v2, decode_huffman_kernel.c:13-254): weights of at
+ * least two symbols, each <= 20, whose 2^(w-1) sum to a power of two (the
+ * table log); length = tableLog + 1 - w; symbols of one weight in symbol
+ * order. Streams are read from their end mark down; bits below a stream's
+ * start read as zero. Returns 0, or -1 on corruption. */
+int orc_huf16_decode(uint16_t* dst, size_t n, const uint8_t* src, size_t size, const uint8_t* weights,
+                     size_t nbWeights, int x4)
+{
+    if (nbWeights < 2 || nbWeights > 65536) return -1;
+    uint64_t sum = 0;
+    size_t nz = 0;
+    for (size_t s = 0; s < nbWeights; s++) {
+        if (weights[s] > 20) return -1;
+        if (weights[s]) { sum += (1ull << weights[s]) >> 1; nz++; }
+    }
+    if (nz < 2 || (sum & (sum - 1))) return -1;
+    const int tl = highbit64(sum);
+    if (tl > 20) return -1;
+    uint32_t* dt = (uint32_t*)malloc(sizeof(uint32_t) << tl);   /* symbol | nbBits << 16 */
+    if (!dt) return -1;
+    {
+        uint32_t start[21] = { 0 };
+        for (size_t s = 0; s < nbWeights; s++) start[weights[s]]++;
+        uint32_t next = 0;
+        for (int w = 1; w <= tl; w++) { uint32_t c = start[w]; start[w] = next; next += c << (w - 1); }
+        for (size_t s = 0; s < nbWeights; s++) {
+            const int w = weights[s];
+            if (!w) continue;
+            const uint32_t len = 1u << (w - 1);
+            for (uint32_t i = 0; i < len; i++) dt[start[w] + i] = (uint32_t)s | (uint32_t)(tl + 1 - w) << 16;
+            start[w] += len;
+        }
+    }
+    size_t ip = 0, op = 0;
+    const int ns = x4 ? 4 : 1;
+    int rc = 0;
+    for (int k = 0; k < ns && !rc; k++) {
+        if (size - ip < 8) { rc = -1; break; }
+        uint32_t cnt = 0, bytes = 0;
+        for (int q = 0; q < 4; q++) {
+            cnt |= (uint32_t)src[ip + q] << (8 * q);
+            bytes |= (uint32_t)src[ip + 4 + q] << (8 * q);
+        }
+        ip += 8;
+        if (bytes > size - ip || cnt > n - op || bytes == 0 || src[ip + bytes - 1] == 0) { rc = -1; break; }
+        const uint8_t* b = src + ip;
+        int64_t bit = (int64_t)(bytes - 1) * 8 + highbit32(b[bytes - 1]);
+        for (uint32_t i = 0; i < cnt; i++) {
+            uint32_t v = 0;
+            for (int q = 0; q < tl; q++) {
+                int64_t p = bit - 1 - q;
+                v = (v << 1) | (p >= 0 ? (uint32_t)((b[p >> 3] >> (p & 7)) & 1) : 0u);
+            }
+            dst[op + i] = (uint16_t)(dt[v] & 0xFFFF);
+            bit -= (int64_t)(dt[v] >> 16);
+        }
+        ip += bytes;
+        op += cnt;
+    }
+    if (!rc && (op != n || ip != size)) rc = -1;
+    free(dt);
+    return rc;
 }
 
 /* ============================ XXH3-64 ============================ */

@@ -83,6 +83,17 @@ size_t orc_huf_encode_4x(uint8_t* dst, size_t cap, const uint8_t* src, size_t n,
 int orc_huf_decode(uint8_t* dst, size_t n, const uint8_t* src, size_t size,
                    const uint8_t* weights, size_t nbWeights, int x4);
 
+/* ---- 2-byte inputs of ZL_GRAPH_HUFFMAN ---- */
+enum { ORC_E2_STORE = 0, ORC_E2_CONSTANT = 1, ORC_E2_BITPACK = 3, ORC_E2_TOKENIZE = 6, ORC_E2_HUF16 = 7 };
+void orc_histogram16(uint32_t* count, unsigned* maxSym, unsigned* card, const uint16_t* src, size_t n);
+int orc_entropy2_decide(const uint16_t* src, size_t n, double* entropyOut);
+size_t orc_tokenize2(uint16_t* alphabet, uint8_t* idx, const uint16_t* src, size_t n);
+int orc_huf16_build(uint16_t* code, uint8_t* nbBits, const uint32_t* count, unsigned maxSym);
+size_t orc_huf16_encode(uint8_t* dst, size_t cap, const uint16_t* src, size_t n, const uint16_t* code,
+                        const uint8_t* nbBits, int x4);
+int orc_huf16_decode(uint16_t* dst, size_t n, const uint8_t* src, size_t size, const uint8_t* weights,
+                     size_t nbWeights, int x4);
+
 /* ---- XXH3-64 (seed 0, default secret) ---- */
 uint64_t orc_xxh3_64(const void* data, size_t len);
 

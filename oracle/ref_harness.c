@@ -25,6 +25,9 @@
 #include "openzl/openzl.h"
 #include "openzl/zl_graph_api.h"
 #include "openzl/zl_segmenter.h"
+/* internal kernel headers of the 16-bit Huffman coder (src/) */
+#include "openzl/codecs/entropy/decode_huffman_kernel.h"
+#include "openzl/codecs/entropy/encode_huffman_kernel.h"
 
 /* Graph kinds (mirrored by openzl_amd.graphs on the Python side). */
 enum {
@@ -215,3 +218,48 @@ long long refh_decompress(const void* src, size_t srcSize, void* dst, size_t dst
 }
 
 size_t refh_compressBound(size_t n) { return ZL_compressBound(n); }
+
+/* The reference's 16-bit Huffman kernels, flat: the table of a histogram
+ * (ZS_largeHuffmanBuildCTable with maxNbBits 0, as EI_huffman_struct_v2 calls
+ * it; returns the table log or -1), the encoded streams of a source with that
+ * table (ZS_largeHuffmanEncodeUsingCTable / X4 into cap bytes; returns the
+ * size or 0), and the decode of such streams from the weights
+ * (ZS_largeHuffmanBuildDTable + ZS_largeHuffmanDecodeUsingDTable / X4;
+ * returns the decoded count or -1). */
+int refh_huf16_build(uint16_t* code, uint8_t* nbBits, const unsigned* count, unsigned maxSym)
+{
+    ZS_Huf16CElt* ct = (ZS_Huf16CElt*)calloc((size_t)maxSym + 1, sizeof(ZS_Huf16CElt));
+    if (!ct) return -1;
+    ZL_Report r = ZS_largeHuffmanBuildCTable(ct, count, (uint16_t)maxSym, 0);
+    if (ZL_isError(r)) { free(ct); return -1; }
+    for (unsigned s = 0; s <= maxSym; s++) { code[s] = ct[s].symbol; nbBits[s] = (uint8_t)ct[s].nbBits; }
+    free(ct);
+    return (int)ZL_validResult(r);
+}
+
+size_t refh_huf16_encode(uint8_t* dst, size_t cap, const uint16_t* src, size_t n, const uint16_t* code,
+                         const uint8_t* nbBits, unsigned maxSym, int tableLog, int x4)
+{
+    ZS_Huf16CElt* ct = (ZS_Huf16CElt*)calloc((size_t)maxSym + 1, sizeof(ZS_Huf16CElt));
+    if (!ct) return 0;
+    for (unsigned s = 0; s <= maxSym; s++) { ct[s].symbol = code[s]; ct[s].nbBits = nbBits[s]; }
+    ZL_WC wc = ZL_WC_wrap(dst, cap);
+    ZL_Report r = x4 ? ZS_largeHuffmanEncodeUsingCTableX4(&wc, src, n, ct, tableLog)
+                     : ZS_largeHuffmanEncodeUsingCTable(&wc, src, n, ct, tableLog);
+    free(ct);
+    return ZL_isError(r) ? 0 : ZL_WC_size(&wc);
+}
+
+long long refh_huf16_decode(uint16_t* dst, size_t cap, const uint8_t* src, size_t size, const uint8_t* weights,
+                            size_t nbWeights, int tableLog, int x4)
+{
+    if (!ZS_largeHuffmanValidWeights(weights, nbWeights, tableLog)) return -1;
+    ZS_Huf16DElt* dt = (ZS_Huf16DElt*)malloc(sizeof(ZS_Huf16DElt) << tableLog);
+    if (!dt) return -1;
+    ZS_largeHuffmanBuildDTable(dt, weights, nbWeights, tableLog);
+    ZL_RC rc = ZL_RC_wrap(src, size);
+    ZL_Report r = x4 ? ZS_largeHuffmanDecodeUsingDTableX4(dst, cap, &rc, dt, tableLog)
+                     : ZS_largeHuffmanDecodeUsingDTable(dst, cap, &rc, dt, tableLog);
+    free(dt);
+    return ZL_isError(r) ? -1 : (long long)ZL_validResult(r);
+}

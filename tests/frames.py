@@ -9,8 +9,8 @@ TRID_NAMES = {
     1: "delta_int", 3: "zigzag", 4: "transpose_split", 6: "convert_struct_to_serial",
     7: "convert_struct_to_num_le", 8: "convert_num_to_struct_le", 10: "convert_num_to_serial_le",
     27: "bitpack_serial", 28: "bitpack_int", 29: "flatpack", 40: "splitn", 44: "constant_serial",
-    45: "constant_fixed", 47: "splitn_struct", 48: "splitn_num", 49: "fse_v2", 50: "huffman_v2",
-    52: "fse_ncount",
+    36: "tokenize_fixed", 45: "constant_fixed", 47: "splitn_struct", 48: "splitn_num", 49: "fse_v2",
+    50: "huffman_v2", 51: "huffman_struct_v2", 52: "fse_ncount",
 }
 
 

@@ -75,6 +75,40 @@ def small_cases():
     return out
 
 
+def entropy2_cases():
+    """ZL_GRAPH_HUFFMAN on 2-byte inputs (struct, and numeric through the
+    num->token conversion), one case per branch of the reference's 2-byte
+    choice (encode_entropy_binding.c:612-682): constant, bitpack, store,
+    tokenize (alphabet bitpacked or, at 16 bits, stored; indices through the
+    1-byte Huffman graph), huffman_struct_v2 with 4 streams and with 1,
+    height-limited and wide (20,000-symbol) trees, and chunked inputs whose
+    chunks take different branches; ZL_GRAPH_FSE on a struct input (converted
+    to serial) for contrast."""
+    out = []
+    H = C.HUFFMAN
+    for typ, t in ((1, "s"), (2, "n")):
+        out.append((f"e2{t}_const_5000", H, typ, 2, "uniform_ints", {"n": 5000, "hi": 1, "dtype": "uint16"}))
+        out.append((f"e2{t}_bp_5000", H, typ, 2, "uniform_ints", {"n": 5000, "hi": 512, "dtype": "uint16", "seed": 5}))
+        out.append((f"e2{t}_rand_5000", H, typ, 2, "uniform_ints", {"n": 5000, "hi": 65536, "dtype": "uint16", "seed": 6}))
+        out.append((f"e2{t}_tok_5000", H, typ, 2, "u16_spread", {"n": 5000, "card": 100, "hi": 4000, "seed": 1}))
+        out.append((f"e2{t}_h16_30000", H, typ, 2, "u16_spread", {"n": 30000, "card": 1000, "hi": 10000, "seed": 3}))
+    out.append(("e2s_tok16_40000", H, 1, 2, "u16_spread", {"n": 40000, "card": 100, "hi": 65536, "seed": 2}))
+    out.append(("e2s_h16x1_1000", H, 1, 2, "u16_spread", {"n": 1000, "card": 300, "hi": 512, "s": 1.0, "seed": 6}))
+    out.append(("e2s_h16deep_32768", H, 1, 2, "u16_spread",
+                {"n": 32768, "card": 300, "hi": 300, "s": 2.0, "seed": 8, "perm": False}))
+    out.append(("e2s_h16fib", H, 1, 2, "u16_fib", {"scale": 2000}))
+    out.append(("e2s_h16wide_99999", H, 1, 2, "u16_spread", {"n": 99999, "card": 20000, "hi": 65536, "s": 1.2, "seed": 9}))
+    out.append(("e2s_h16mid_60000", H, 1, 2, "u16_spread", {"n": 60000, "card": 2000, "hi": 2000, "s": 1.6, "seed": 9,
+                                                          "perm": False}))
+    out.append(("e2s_mixed_200000", H, 1, 2, "u16_mixed", {"seed": 1}))
+    out.append(("e2n_mixed_200000", H, 2, 2, "u16_mixed", {"seed": 2}))
+    out.append(("e2s_chunks_131073", H, 1, 2, "u16_spread", {"n": 131073, "card": 900, "hi": 30000, "seed": 4}))
+    out.append(("e2s_five_5", H, 1, 2, "u16_spread", {"n": 5, "card": 3, "hi": 65536, "seed": 5}))
+    out.append(("e2s_two_200", H, 1, 2, "u16_spread", {"n": 200, "card": 2, "hi": 65536, "seed": 7}))
+    out.append(("e2s_fse_5000", C.FSE, 1, 2, "u16_spread", {"n": 5000, "card": 1000, "hi": 10000, "seed": 3}))
+    return out
+
+
 def param_cases():
     """Cases compressed with compressor parameters: (name, kind, type, width,
     generator, kwargs, params). minStreamSize 0 is the default, 5 and 64 move
@@ -98,6 +132,10 @@ def param_cases():
         out.append((f"ms{tag}{abs(ms)}_bf16_ts_huf_fse_3000", C.TS_HUF_FSE, 1, 2, "bf16_normal", {"n": 3000}, p))
         out.append((f"ms{tag}{abs(ms)}_zipf_huf_100000", C.HUFFMAN, 0, 1, "zipf_bytes",
                     {"n": 100000, "s": 1.1, "seed": 100000}, p))
+        out.append((f"ms{tag}{abs(ms)}_e2s_one_1", C.HUFFMAN, 1, 2, "uniform_ints",
+                    {"n": 1, "hi": 65536, "dtype": "uint16"}, p))
+        out.append((f"ms{tag}{abs(ms)}_e2s_tok_300", C.HUFFMAN, 1, 2, "u16_spread",
+                    {"n": 300, "card": 40, "hi": 4000, "seed": 10}, p))
     return out
 
 
@@ -116,7 +154,7 @@ def write_small():
     os.makedirs(os.path.join(HERE, "small"), exist_ok=True)
     entries = []
     total = 0
-    cases = [c[:6] + (None,) for c in small_cases()] + param_cases()
+    cases = [c[:6] + (None,) for c in small_cases()] + [c + (None,) for c in entropy2_cases()] + param_cases()
     for name, kind, typ, width, gen, kw, params in cases:
         x = make_input(gen, kw)
         C.ref_set_min_stream_size((params or {}).get("minStreamSize", 0))

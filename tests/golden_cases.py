@@ -46,3 +46,22 @@ def full_table(name):
 
 def frame_digest(fr):
     return int(np.frombuffer(hashlib.sha256(fr).digest()[:8], np.uint64)[0])
+
+
+def frame_transforms(frame):
+    """Transform ids of every node of a single-input frame."""
+    from frames import parse_frame
+    f = parse_frame(bytes(frame))
+    return {d["trid"] for ch in f["chunks"] for d in ch["decoders"]}
+
+
+# huffman_struct_v2 (wire transform 51), the 16-bit Huffman node of
+# ZL_GRAPH_HUFFMAN on 2-byte inputs: its GPU encoder is switched off (the
+# engine rejects such inputs with temporaryLibraryLimitation) after its table
+# kernel faulted the GPU with no cause shown yet; its decoder is tested.
+HUF16_TRID = 51
+HUF16_ENCODE_OFF = "huffman_struct_v2 GPU encode switched off: its table kernel faulted the GPU, cause not shown"
+
+
+def huf16_entries():
+    return {e["name"] for e in small_entries() if HUF16_TRID in frame_transforms(small_frame(e["name"]))}

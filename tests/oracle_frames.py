@@ -16,7 +16,8 @@ import numpy as np
 from frames import parse_frame
 from ozl_checkers import oracle, ptr
 
-SINGLETONS = {1: 1, 3: 1, 4: 0, 6: 1, 7: 1, 8: 1, 10: 1, 27: 1, 28: 1, 29: 2, 40: 0, 44: 1, 49: 2, 50: 2, 52: 1}
+SINGLETONS = {1: 1, 3: 1, 4: 0, 6: 1, 7: 1, 8: 1, 10: 1, 27: 1, 28: 1, 29: 2, 36: 2, 40: 0, 44: 1, 45: 1, 47: 0, 48: 0,
+              49: 2, 50: 2, 51: 2, 52: 1}
 
 
 class FrameError(ValueError):
@@ -107,6 +108,38 @@ def _decode_node(O, trid, hdr, ins):
         return out[:n].tobytes(), "s", 1
     if trid == 40:  # splitN: concatenation
         return b"".join(p[0] for p in ins), ins[0][1] if ins else "s", ins[0][2] if ins else 1
+    if trid in (47, 48):  # splitN of struct / numeric streams: concatenation
+        if not ins:
+            w, _ = _varint(hdr)
+            return b"", "t" if trid == 47 else "n", w
+        if any(p[2] != ins[0][2] for p in ins):
+            raise FrameError("splitN widths differ")
+        return b"".join(p[0] for p in ins), "t" if trid == 47 else "n", ins[0][2]
+    if trid == 45:  # constant_fixed (decode_constant_binding.c:10-47)
+        n, _ = _varint(hdr)
+        v, t, w = ins[0]
+        if len(v) != w or n < 1:
+            raise FrameError("constant")
+        return v * n, "t", w
+    if trid == 36:  # tokenize (decode_tokenize_binding.c:12-37)
+        alpha, _, w = ins[0]
+        idx, _, iw = ins[1]
+        a = np.frombuffer(alpha, np.uint8).reshape(-1, w)
+        ix = np.frombuffer(idx, {1: np.uint8, 2: np.uint16, 4: np.uint32}[iw]).astype(np.int64)
+        if ix.size and ix.max() >= a.shape[0]:
+            raise FrameError("tokenize index")
+        return a[ix].tobytes(), "t", w
+    if trid == 51:  # huffman_struct_v2 (decode_entropy_binding.c:415-472)
+        if not 2 <= len(hdr) <= 9:
+            raise FrameError("header")
+        x4 = hdr[0] & 1
+        n = int.from_bytes(hdr[1:], "little")
+        wts = np.frombuffer(ins[0][0], np.uint8)
+        bits = np.frombuffer(ins[1][0], np.uint8) if ins[1][0] else np.zeros(1, np.uint8)
+        out = np.empty(max(n, 1), np.uint16)
+        if n < 2 or O.orc_huf16_decode(ptr(out), n, ptr(bits), len(ins[1][0]), ptr(wts), len(wts), x4) != 0:
+            raise FrameError("huffman16")
+        return out[:n].tobytes(), "t", 2
     if trid == 44:  # constant_serial
         n, _ = _varint(hdr)
         if len(ins[0][0]) != 1:

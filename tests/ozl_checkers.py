@@ -79,6 +79,17 @@ def _setup_oracle(L):
     L.orc_fse_decode.argtypes = [P, SZ, P, SZ, P, U, U, U]
     L.orc_huf_optimal_tablelog.restype = U
     L.orc_huf_optimal_tablelog.argtypes = [U, SZ, U]
+    L.orc_histogram16.argtypes = [P, P, P, P, SZ]
+    L.orc_entropy2_decide.restype = I
+    L.orc_entropy2_decide.argtypes = [P, SZ, P]
+    L.orc_tokenize2.restype = SZ
+    L.orc_tokenize2.argtypes = [P, P, P, SZ]
+    L.orc_huf16_build.restype = I
+    L.orc_huf16_build.argtypes = [P, P, P, U]
+    L.orc_huf16_encode.restype = SZ
+    L.orc_huf16_encode.argtypes = [P, SZ, P, SZ, P, P, I]
+    L.orc_huf16_decode.restype = I
+    L.orc_huf16_decode.argtypes = [P, SZ, P, SZ, P, SZ, I]
     L.orc_huf_build_ctable.restype = I
     L.orc_huf_build_ctable.argtypes = [P, P, P, U, U]
     for f in ("orc_huf_encode_1x", "orc_huf_encode_4x"):
@@ -132,6 +143,12 @@ def _setup_ref(L):
 
 
 def _setup_refh(L):
+    L.refh_huf16_build.restype = I
+    L.refh_huf16_build.argtypes = [P, P, P, U]
+    L.refh_huf16_encode.restype = SZ
+    L.refh_huf16_encode.argtypes = [P, SZ, P, SZ, P, P, U, I, I]
+    L.refh_huf16_decode.restype = ctypes.c_longlong
+    L.refh_huf16_decode.argtypes = [P, SZ, P, SZ, P, SZ, I, I]
     L.refh_compress.restype = ctypes.c_longlong
     L.refh_compress.argtypes = [I, I, I, P, SZ, P, SZ, I, I]
     L.refh_decompress.restype = ctypes.c_longlong

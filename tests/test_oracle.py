@@ -354,3 +354,93 @@ def test_decision_matches_reference_frames():
 
 
 C_ORC = {"STORE": 0, "CONSTANT": 1, "FLATPACK": 2, "BITPACK": 3, "HUF": 4, "FSE": 5}
+
+
+# ---------------------------------------------------------------- 2-byte inputs
+def h16_cases():
+    """16-bit sources: Zipf over spread values, deep (height-limited) trees,
+    many ties, two symbols, a dominant symbol with singletons."""
+    rng = np.random.default_rng(77)
+    out = []
+
+    def spread(n, card, hi, s, perm=True):
+        vals = (rng.choice(hi, card, replace=False) if perm else np.arange(card)).astype(np.uint16)
+        p = 1.0 / np.arange(1, card + 1) ** s
+        p /= p.sum()
+        x = np.concatenate([vals, vals[rng.choice(card, n - card, p=p)]])
+        rng.shuffle(x)
+        return x
+    for n, card, hi, s in ((1000, 300, 512, 1.0), (600, 260, 300, 0.3), (32768, 300, 300, 2.0),
+                           (60000, 2000, 2000, 1.6), (30000, 1000, 10000, 1.1), (32768, 5000, 65536, 0.9),
+                           (99999, 20000, 65536, 1.2), (4096, 4000, 4096, 0.0), (50, 2, 65536, 0.0)):
+        out.append(spread(n, card, hi, s))
+    x = np.zeros(20000, np.uint16)
+    x[rng.choice(20000, 400, replace=False)] = np.arange(1, 401, dtype=np.uint16) * 97
+    out.append(x)
+    # Fibonacci-like counts: the unlimited tree is 30+ deep
+    fib = [1, 1]
+    while len(fib) < 26:
+        fib.append(fib[-1] + fib[-2])
+    out.append(np.repeat(np.arange(26, dtype=np.uint16) * 2000, fib))
+    return out
+
+
+@needs_ref
+def test_huf16_tables_encode_decode_match_reference():
+    h = C.refh()
+    for x in h16_cases():
+        count = np.zeros(65536, np.uint32)
+        ms, cd = ctypes.c_uint(0), ctypes.c_uint(0)
+        O.orc_histogram16(ptr(count), ctypes.byref(ms), ctypes.byref(cd), ptr(x), x.size)
+        assert ms.value == int(x.max()) and cd.value == len(np.unique(x))
+        assert np.array_equal(count[: ms.value + 1], np.bincount(x, minlength=ms.value + 1))
+        m = ms.value
+        ca = np.zeros(m + 1, np.uint16); na = np.zeros(m + 1, np.uint8)
+        cb = np.zeros(m + 1, np.uint16); nb = np.zeros(m + 1, np.uint8)
+        ta = O.orc_huf16_build(ptr(ca), ptr(na), ptr(count), m)
+        tb = h.refh_huf16_build(ptr(cb), ptr(nb), ptr(count), m)
+        assert ta == tb and ta > 0
+        assert np.array_equal(na, nb) and np.array_equal(ca, cb)
+        assert na.max() == ta and ((na > 0) == (count[: m + 1] > 0)).all()
+        for x4 in (0, 1):
+            for cap in (2 * x.size + 32, x.size // 3 + 40):
+                ea = np.zeros(cap + 64, np.uint8)
+                eb = np.zeros(cap + 64, np.uint8)
+                sa = O.orc_huf16_encode(ptr(ea), cap, ptr(x), x.size, ptr(ca), ptr(na), x4)
+                sb = h.refh_huf16_encode(ptr(eb), cap, ptr(x), x.size, ptr(cb), ptr(nb), m, tb, x4)
+                assert sa == sb
+                if not sa:
+                    continue
+                assert np.array_equal(ea[:sa], eb[:sb])
+                w = np.where(na > 0, ta + 1 - na.astype(np.int32), 0).astype(np.uint8)
+                back = np.zeros(x.size, np.uint16)
+                assert O.orc_huf16_decode(ptr(back), x.size, ptr(ea), sa, ptr(w), w.size, x4) == 0
+                assert np.array_equal(back, x)
+                rback = np.zeros(x.size, np.uint16)
+                assert h.refh_huf16_decode(ptr(rback), x.size, ptr(eb), sb, ptr(w), w.size, ta, x4) == x.size
+                assert np.array_equal(rback, x)
+
+
+@needs_ref
+def test_entropy16_matches_reference():
+    for x in h16_cases():
+        count = np.bincount(x, minlength=65536).astype(np.uint32)
+        ms = int(x.max())
+        assert O.orc_entropy(ptr(count), ms, x.size) == R.ZL_calculateEntropy(ptr(count), ms, x.size)
+
+
+def test_tokenize2_closed_form():
+    rng = np.random.default_rng(3)
+    for n, card in ((1, 1), (10, 3), (5000, 100), (40000, 255), (300, 256)):
+        vals = rng.choice(65536, card, replace=False).astype(np.uint16)
+        x = np.concatenate([vals, vals[rng.integers(0, card, n - card)]]) if n > card else vals[:n]
+        rng.shuffle(x)
+        a = np.zeros(256, np.uint16)
+        idx = np.zeros(x.size, np.uint8)
+        k = O.orc_tokenize2(ptr(a), ptr(idx), ptr(x), x.size)
+        u, first = np.unique(x, return_index=True)
+        order = u[np.argsort(first)]
+        assert k == len(order) and np.array_equal(a[:k], order)
+        assert np.array_equal(a[idx], x)
+    x = np.arange(300, dtype=np.uint16)
+    assert O.orc_tokenize2(ptr(np.zeros(300, np.uint16)), ptr(np.zeros(300, np.uint8)), ptr(x), 300) == 0
