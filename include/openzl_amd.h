@@ -84,6 +84,12 @@ static inline size_t ZL_validResult(ZL_Report r) { return r._value._value; }
 static inline ZL_ErrorCode ZL_errorCode(ZL_Report r) { return r._code; }
 /* include/openzl/zl_errors.h: ZL_ErrorCode_toString */
 const char* ZL_ErrorCode_toString(ZL_ErrorCode code);
+/* include/openzl/detail/zl_errors_detail.h:217-220: an error is its code and
+ * an info pointer (16 bytes, the error member of ZL_Report) */
+typedef struct {
+    ZL_ErrorCode _code;
+    void* _info;
+} ZL_Error;
 
 /* ------------------------------------------------------------ types and IDs */
 /* include/openzl/zl_data.h: ZL_Type bit flags */
@@ -160,6 +166,9 @@ ZL_GraphID ZL_Compressor_registerBruteForceSelectorGraph(ZL_Compressor* compress
                                                          size_t numSuccessors);
 /* include/openzl/zl_compressor.h:555 */
 ZL_Report ZL_Compressor_selectStartingGraphID(ZL_Compressor* compressor, ZL_GraphID graph);
+/* include/openzl/zl_compressor.h:162-164 */
+int ZL_Compressor_getParameter(const ZL_Compressor* compressor, ZL_CParam gcparam);
+const char* ZL_Compressor_getErrorContextString_fromError(const ZL_Compressor* compressor, ZL_Error error);
 
 /* Function graphs: include/openzl/zl_graph_api.h:58-120 (subset). */
 typedef ZL_Report (*ZL_FunctionGraphFn)(ZL_Graph* graph, ZL_Edge* inputs[], size_t nbInputs);
@@ -272,6 +281,21 @@ ZL_Report ZL_CCtx_setParameter(ZL_CCtx* cctx, ZL_CParam param, int value);
 int ZL_CCtx_getParameter(const ZL_CCtx* cctx, ZL_CParam param);
 /* include/openzl/zl_compress.h:168 */
 ZL_Report ZL_CCtx_resetParameters(ZL_CCtx* cctx);
+/* include/openzl/zl_compressor.h:411-422; only NULL is accepted by
+ * ZL_CCtx_selectStartingGraphID here (runtime graph parameters are outside
+ * the hot path) */
+typedef struct ZL_GraphParameters_s ZL_GraphParameters;
+/* include/openzl/zl_compressor.h:616-620 (cctx.c:269-279): the starting graph
+ * of the next compression, reset with the other requested parameters unless
+ * sticky; a non-NULL compressor becomes the referenced one; rgp != NULL is
+ * temporaryLibraryLimitation. */
+ZL_Report ZL_CCtx_selectStartingGraphID(ZL_CCtx* cctx, const ZL_Compressor* compressor, ZL_GraphID graphID,
+                                        const ZL_GraphParameters* rgp);
+/* include/openzl/zl_compress.h:213-223, zl_decompress.h:188-204,
+ * zl_compressor.h:115-117: the message of an error (here the error code's
+ * name: no dynamic context is recorded). */
+const char* ZL_CCtx_getErrorContextString(const ZL_CCtx* cctx, ZL_Report report);
+const char* ZL_CCtx_getErrorContextString_fromError(const ZL_CCtx* cctx, ZL_Error error);
 /* include/openzl/zl_compress.h:313-349 */
 ZL_TypedRef* ZL_TypedRef_createSerial(const void* src, size_t srcSize);
 ZL_TypedRef* ZL_TypedRef_createStruct(const void* start, size_t structWidth, size_t structCount);
@@ -320,6 +344,8 @@ void ZL_DCtx_free(ZL_DCtx* dctx);
 ZL_Report ZL_DCtx_setParameter(ZL_DCtx* dctx, ZL_DParam param, int value);
 int ZL_DCtx_getParameter(const ZL_DCtx* dctx, ZL_DParam param);
 ZL_Report ZL_DCtx_resetParameters(ZL_DCtx* dctx);
+const char* ZL_DCtx_getErrorContextString(const ZL_DCtx* dctx, ZL_Report report);
+const char* ZL_DCtx_getErrorContextString_fromError(const ZL_DCtx* dctx, ZL_Error error);
 /* include/openzl/zl_decompress.h:230 */
 ZL_Report ZL_DCtx_decompress(ZL_DCtx* dctx, void* dst, size_t dstCapacity, const void* compressed, size_t cSize);
 /* include/openzl/zl_decompress.h:388 */

@@ -61,6 +61,10 @@ def _api():
         "ZL_CCtx_setParameter": (ZLReport, [P, ctypes.c_int, ctypes.c_int]),
         "ZL_CCtx_getParameter": (ctypes.c_int, [P, ctypes.c_int]),
         "ZL_CCtx_resetParameters": (ZLReport, [P]),
+        "ZL_CCtx_selectStartingGraphID": (ZLReport, [P, P, GraphID, P]),
+        "ZL_CCtx_getErrorContextString": (ctypes.c_char_p, [P, ZLReport]),
+        "ZL_DCtx_getErrorContextString": (ctypes.c_char_p, [P, ZLReport]),
+        "ZL_Compressor_getParameter": (ctypes.c_int, [P, ctypes.c_int]),
         "ZL_CCtx_compressMultiTypedRef": (ZLReport, [P, P, SZ, ctypes.POINTER(P), SZ]),
         "ZL_DCtx_setParameter": (ZLReport, [P, ctypes.c_int, ctypes.c_int]),
         "ZL_DCtx_getParameter": (ctypes.c_int, [P, ctypes.c_int]),

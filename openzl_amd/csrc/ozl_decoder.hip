@@ -1114,6 +1114,9 @@ ZL_Report ZL_DCtx_resetParameters(ZL_DCtx* d)
     return rep_ok(0);
 }
 
+const char* ZL_DCtx_getErrorContextString(const ZL_DCtx*, ZL_Report r) { return ZL_ErrorCode_toString(r._code); }
+const char* ZL_DCtx_getErrorContextString_fromError(const ZL_DCtx*, ZL_Error e) { return ZL_ErrorCode_toString(e._code); }
+
 ZL_Report ZL_DCtx_decompressTyped(ZL_DCtx* dctx, ZL_OutputInfo* info, void* dst, size_t cap, const void* src,
                                   size_t srcSize)
 {

@@ -1154,6 +1154,7 @@ ZL_Result_ZL_EdgeList ZL_Edge_runTransposeSplit(ZL_Edge* e, const ZL_Graph* grap
 struct ZL_CCtx_s {
     const ZL_Compressor_s* comp = nullptr;
     GParams params;                         // requested on the context
+    unsigned startGraph = 0;                // ZL_CCtx_selectStartingGraphID (0: the compressor's)
     DevArena arena;
     hipStream_t st = nullptr;
     hipStream_t st2 = nullptr;              // content checksums, alongside the graph work
@@ -1359,7 +1360,7 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
     }
     // the graph walks: one chunk per input, or the chunks its segmenter cuts
     std::vector<ChunkUnit> units;
-    const unsigned start = c->comp->start ? c->comp->start : GID_STORE;
+    const unsigned start = c->startGraph ? c->startGraph : c->comp->start ? c->comp->start : GID_STORE;
     const GDef* sg = start >= GID_CUSTOM_BASE && start - GID_CUSTOM_BASE < c->comp->graphs.size()
                          ? &c->comp->graphs[start - GID_CUSTOM_BASE]
                          : nullptr;
@@ -1553,35 +1554,59 @@ int compress_frames(ZL_CCtx_s* c, const ZL_TypedRef* const* ins, size_t n, void*
 // ====================================================================== C ABI
 extern "C" {
 
+// src/openzl/common/errors.c:14-140 with the messages of
+// include/openzl/detail/zl_errors_detail.h:52-137
 const char* ZL_ErrorCode_toString(ZL_ErrorCode code)
 {
     switch (code) {
-    case ZL_ErrorCode_no_error: return "No error detected";
-    case ZL_ErrorCode_GENERIC: return "Error (generic)";
+    case ZL_ErrorCode_no_error: return "No Error";
+    case ZL_ErrorCode_GENERIC: return "Generic";
     case ZL_ErrorCode_srcSize_tooSmall: return "Source size too small";
     case ZL_ErrorCode_srcSize_tooLarge: return "Source size too large";
     case ZL_ErrorCode_dstCapacity_tooSmall: return "Destination capacity too small";
+    case ZL_ErrorCode_userBuffers_invalidNum: return "Nb of Typed Buffers provided is incorrect for this frame";
     case ZL_ErrorCode_header_unknown: return "Unknown header";
+    case ZL_ErrorCode_frameParameter_unsupported: return "Frame parameter unsupported";
     case ZL_ErrorCode_corruption: return "Corruption detected";
-    case ZL_ErrorCode_compressedChecksumWrong: return "Compressed checksum mismatch";
-    case ZL_ErrorCode_contentChecksumWrong: return "Content checksum mismatch";
+    case ZL_ErrorCode_compressedChecksumWrong: return "Compressed checksum mismatch (corruption after compression)";
+    case ZL_ErrorCode_contentChecksumWrong:
+        return "Content checksum mismatch (either corruption after compression or corruption during compression or "
+               "decompression)";
+    case ZL_ErrorCode_outputs_tooNumerous: return "Too many outputs: unsupported by claimed format version";
+    case ZL_ErrorCode_compressionParameter_invalid: return "Compression parameter invalid";
     case ZL_ErrorCode_parameter_invalid: return "Parameter is invalid";
-    case ZL_ErrorCode_graph_invalid: return "Graph is invalid";
-    case ZL_ErrorCode_successor_invalid: return "Successor is invalid";
-    case ZL_ErrorCode_inputType_unsupported: return "Input type not supported";
-    case ZL_ErrorCode_node_invalid_input: return "Node input is invalid";
+    case ZL_ErrorCode_outputID_invalid: return "Frame doesn't host this many outputs";
+    case ZL_ErrorCode_segmenter_inputNotConsumed: return "Segmenter did not consume entirely all inputs";
+    case ZL_ErrorCode_graph_invalid: return "Graph invalid";
+    case ZL_ErrorCode_graph_invalidNumInputs: return "Graph invalid nb inputs";
+    case ZL_ErrorCode_successor_invalid: return "Selected an invalid Successor Graph";
+    case ZL_ErrorCode_inputType_unsupported: return "Input Type not supported by selected Port";
+    case ZL_ErrorCode_node_invalid_input: return "Input does not respect conditions for this node";
+    case ZL_ErrorCode_node_invalid: return "Invalid Node ID";
     case ZL_ErrorCode_formatVersion_unsupported: return "Format version unsupported";
-    case ZL_ErrorCode_formatVersion_notSet: return "Format version not set";
-    case ZL_ErrorCode_allocation: return "Allocation error";
+    case ZL_ErrorCode_formatVersion_notSet:
+        return "Format version is not set; it must be set via the ZL_CParam_formatVersion parameter";
+    case ZL_ErrorCode_allocation: return "Allocation";
     case ZL_ErrorCode_internalBuffer_tooSmall: return "Internal buffer too small";
+    case ZL_ErrorCode_logicError: return "Internal logic error";
+    case ZL_ErrorCode_temporaryLibraryLimitation: return "Temporary OpenZL library limitation";
     default: return "Unspecified error code";
     }
+}
+
+const char* ZL_CCtx_getErrorContextString(const ZL_CCtx*, ZL_Report r) { return ZL_ErrorCode_toString(r._code); }
+const char* ZL_CCtx_getErrorContextString_fromError(const ZL_CCtx*, ZL_Error e) { return ZL_ErrorCode_toString(e._code); }
+const char* ZL_Compressor_getErrorContextString_fromError(const ZL_Compressor*, ZL_Error e)
+{
+    return ZL_ErrorCode_toString(e._code);
 }
 
 size_t ZL_compressBound(size_t n) { return n * 2 + 512 + 8; }
 
 ZL_Compressor* ZL_Compressor_create(void) { return new (std::nothrow) ZL_Compressor_s(); }
 void ZL_Compressor_free(ZL_Compressor* c) { delete c; }
+
+int ZL_Compressor_getParameter(const ZL_Compressor* c, ZL_CParam p) { return c ? c->params.get(p) : 0; }
 
 ZL_Report ZL_Compressor_setParameter(ZL_Compressor* c, ZL_CParam p, int v)
 {
@@ -1774,12 +1799,28 @@ ZL_Report ZL_CCtx_setParameter(ZL_CCtx* c, ZL_CParam p, int v)
 
 int ZL_CCtx_getParameter(const ZL_CCtx* c, ZL_CParam p) { return c ? c->params.get(p) : 0; }
 
+// cctx.c:269-279: the starting graph of the next compression, a requested
+// parameter like the others (reset with them unless sticky); a compressor
+// given here becomes the referenced one. Runtime graph parameters are not
+// supported (temporaryLibraryLimitation).
+ZL_Report ZL_CCtx_selectStartingGraphID(ZL_CCtx* c, const ZL_Compressor* comp, ZL_GraphID g,
+                                        const ZL_GraphParameters* rgp)
+{
+    if (!c) return rep_err(ZL_ErrorCode_parameter_invalid);
+    if (comp) c->comp = comp;
+    if (rgp) return rep_err(ZL_ErrorCode_temporaryLibraryLimitation);
+    if (g.gid == 0) return rep_err(ZL_ErrorCode_graph_invalid);   // as ZL_Compressor_selectStartingGraphID
+    c->startGraph = g.gid;
+    return rep_ok(0);
+}
+
 // cctx.c:293-314: the requested parameters and the compressor reference go.
 ZL_Report ZL_CCtx_resetParameters(ZL_CCtx* c)
 {
     if (!c) return rep_err(ZL_ErrorCode_parameter_invalid);
     c->params = GParams();
     c->comp = nullptr;
+    c->startGraph = 0;
     return rep_ok(0);
 }
 

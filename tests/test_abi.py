@@ -56,6 +56,9 @@ DROP_IN = [
     "ZL_FrameInfo_getOutputType", "ZL_FrameInfo_getDecompressedSize", "ZL_FrameInfo_getNumElts",
     "ZL_decompress", "ZL_getDecompressedSize", "ZL_getNumOutputs", "ZL_getOutputType", "ZL_compressBound",
     "ZL_ErrorCode_toString", "ZL_Edge_runNode", "ZL_Edge_setDestination", "ZL_Edge_getData",
+    "ZL_CCtx_selectStartingGraphID", "ZL_Compressor_getParameter", "ZL_CCtx_getErrorContextString",
+    "ZL_CCtx_getErrorContextString_fromError", "ZL_DCtx_getErrorContextString",
+    "ZL_DCtx_getErrorContextString_fromError", "ZL_Compressor_getErrorContextString_fromError",
 ]
 
 

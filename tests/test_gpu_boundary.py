@@ -167,6 +167,18 @@ def test_cctx_parameters_reset_after_compression(api):
         assert r3.code == 0 and GC.sha256(dst[:r3.value].tobytes()) == e["frame_sha256"]
         assert L.ZL_CCtx_refCompressor(cctx, comp.c).code == 0
         assert L.ZL_CCtx_compressMultiTypedRef(cctx, dst.ctypes.data, cap, ins, 2).code == 81
+        # cctx.c:269-279: a starting graph chosen on the context (with its
+        # compressor) wins over the compressor's, for one compression
+        fse = api.Compressor(api.FSE)
+        assert L.ZL_CCtx_selectStartingGraphID(cctx, fse.c, api.G_HUFFMAN, None).code == 0
+        r4 = L.ZL_CCtx_compressTypedRef(cctx, dst.ctypes.data, cap, tr)
+        assert r4.code == 0 and GC.sha256(dst[:r4.value].tobytes()) == e["frame_sha256"]
+        assert L.ZL_CCtx_refCompressor(cctx, fse.c).code == 0
+        r5 = L.ZL_CCtx_compressTypedRef(cctx, dst.ctypes.data, cap, tr)
+        f = next(f for f in ENTRIES if f["name"] == "zipf_fse_5000")
+        assert r5.code == 0 and GC.sha256(dst[:r5.value].tobytes()) == f["frame_sha256"]
+        assert L.ZL_CCtx_selectStartingGraphID(cctx, fse.c, api.G_HUFFMAN, ctypes.c_void_p(8)).code == 81
+        assert L.ZL_CCtx_getErrorContextString(cctx, r5).decode() == "No Error"
     finally:
         L.ZL_TypedRef_free(tr)
         L.ZL_CCtx_free(cctx)
