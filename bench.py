@@ -11,6 +11,14 @@ One step = FSE encode + Huffman encode + FSE decode + Huffman decode of the
 whole 1 GiB batch, inputs resident in HBM. value = (2 x 1 GiB) / step time.
 
 Also reported:
+  host buffers  the same steps with the input, frames and output in pinned
+                host memory (copies in and out timed, wall clock)
+  workloads     sub-records (config.workloads) of the other workloads of the
+                path, each timed with its own steps: the 16 MiB timestamps
+                frame and the 1 GiB bf16 tensor through the ZL_CCtx engine,
+                4,096 batched columns, learned selection over 8,192 chunks
+                against exhaustive trial; each with encode / decode GB/s, its
+                roofline fraction and traffic and a reference CPU baseline
   roofline      the kernel with the largest share of the timed region, its
                 algorithmic bytes per launch (input + payload bytes of its
                 blocks) over its mean duration (HIP events around every launch
@@ -45,16 +53,22 @@ from openzl_amd.datagen import zipf_blocks_np as gen_blocks_np, zipf_blocks_torc
 
 
 # ---------------------------------------------------------------- CPU baseline
-def cpu_baseline(sample_blocks, threads, budget_s=8.0):
+def host_threads():
+    """Every core this process may run on (no cap), and what the host has."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, aff), {"nproc": os.cpu_count(), "affinity": aff}
+
+
+def cpu_baseline(data, threads, budget_s=8.0):
     """The reference library on the host cores: each thread owns one
     compression and one decompression context and cycles over its share of
-    `sample_blocks` blocks of the same workload (FSE and Huffman frame of each
-    block, then decompression), until `budget_s` seconds have passed."""
+    the blocks of `data` (FSE and Huffman frame of each block, then
+    decompression), until `budget_s` seconds have passed."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import ozl_checkers as C
     if C.refh() is None:
         return None
-    data = gen_blocks_np(0, sample_blocks)
+    sample_blocks = data.size // BLOCK
     blocks = [data[i * BLOCK:(i + 1) * BLOCK] for i in range(sample_blocks)]
     h = C.refh()
     cap = h.refh_compressBound(BLOCK) + 4096
@@ -91,11 +105,22 @@ def cpu_baseline(sample_blocks, threads, budget_s=8.0):
         "unit": "GB/s",
         "cores": threads,
         "kind": "reference",
-        "sample": f"{sample_blocks} distinct 64 KiB blocks of the same Zipf(1.1) input, cycled; each block "
+        "sample": f"{sample_blocks} distinct 64 KiB blocks of the benchmark's own input, cycled; each block "
                   f"compressed with ZL_GRAPH_FSE and with ZL_GRAPH_HUFFMAN and decompressed again by the "
                   f"reference library built from its sources, one context pair per thread; "
                   f"{total / 1e9:.2f} GB (uncompressed, both codings) in {wall:.1f} s wall",
     }
+
+
+def load_traffic(name):
+    """The committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of a workload
+    (profiles/<name>_hbm_traffic.json, copy-calibrated bytes per kernel and
+    their total over one encode + decode), or None."""
+    path = os.path.join(ROOT, "profiles", f"{name}_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 # ---------------------------------------------------------------- batched columns
@@ -251,15 +276,18 @@ def bench_columns(args):
     enc_gbs = (U + C_bytes) / (enc_ms / 1000) / 1e9
     dec_gbs = (U + C_bytes) / (dec_ms / 1000) / 1e9
     both = 2 * (U + C_bytes) / ((enc_ms + dec_ms) / 1000) / 1e9
+    traffic = load_traffic("columns_4096")
     roofline = {"bound": "hbm", "kernel": "whole encode + decode graphs (all launches)", "achieved": round(both, 2),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(both / HBM_PEAK_GBS, 4), "traffic": None,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(both / HBM_PEAK_GBS, 4),
+                "traffic": traffic.get("total_bytes") if traffic else None, "algorithmic_bytes": 2 * (U + C_bytes),
                 "encode_algorithmic_GBps": round(enc_gbs, 2), "decode_algorithmic_GBps": round(dec_gbs, 2),
                 "kernel_ms_per_call": prof}
     cpu = None
     if args.cpu_sample > 0:
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-        threads = max(1, min(threads, 16))
-        cpu = cpu_baseline_columns(cols[:256], threads, args.cpu_seconds)
+        threads, hinfo = host_threads()
+        cpu = cpu_baseline_columns(cols, threads, args.cpu_seconds)
+        if cpu:
+            cpu.update(hinfo)
     value = 2 * U / (ms_step / 1000) / 1e9
     rec = {
         "metric": METRIC, "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -280,7 +308,7 @@ def bench_columns(args):
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(rec))
+    return rec
 
 
 def cpu_baseline_select(chunks, threads, budget_s):
@@ -397,14 +425,17 @@ def bench_select(args):
             prof[name] = round(float(ms), 4)
         L.ozl_profile_tag(b"")
     ml_gbs = (U + C_ml) / (ms_ml / 1000) / 1e9
+    traffic = load_traffic("select_8192")
     roofline = {"bound": "hbm", "kernel": "whole learned path (features, MLP, compression with the choice)",
                 "achieved": round(ml_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ml_gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel_ms_per_call": prof}
+                "frac": round(ml_gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get("total_bytes") if traffic else None,
+                "algorithmic_bytes": U + C_ml, "kernel_ms_per_call": prof}
     cpu = None
     if args.cpu_sample > 0:
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-        threads = max(1, min(threads, 16))
-        cpu = cpu_baseline_select(chunks[:512], threads, args.cpu_seconds)
+        threads, hinfo = host_threads()
+        cpu = cpu_baseline_select(chunks, threads, args.cpu_seconds)
+        if cpu:
+            cpu.update(hinfo)
     rec = {
         "metric": METRIC, "value": round(U / (ms_ml / 1000) / 1e9, 3), "unit": "GB/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_ml, 4), "higher_is_better": True,
@@ -426,7 +457,7 @@ def bench_select(args):
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(rec))
+    return rec
 
 
 def bench_engine(args):
@@ -475,11 +506,6 @@ def bench_engine(args):
     # host buffers (PCIe included): the reference frames' check and the side figure
     frames = comp.compress_batch(parts, typ, width)
     Cb = sum(len(f) for f in frames)
-    if C.refh() is not None:
-        if segment:
-            C.ref_set_segmenter(*segment)
-        assert frames[0] == C.ref_compress(kind, parts[0], {api.SERIAL: C.SERIAL, api.STRUCT: C.STRUCT,
-                                                           api.NUMERIC: C.NUMERIC}[typ], width)
     outs = dec.decompress_batch(frames, U + 64)
     assert all(o == p.tobytes() for o, p in zip(outs, parts))
     # device-resident buffers
@@ -508,6 +534,25 @@ def bench_engine(args):
     t2 = time.perf_counter()
     enc_ms = (t1 - t0) * 1000 / args.steps
     dec_ms = (t2 - t1) * 1000 / args.steps
+    # per-kernel milliseconds of one encode + decode (launch-time HIP events)
+    prof = {}
+    if not args.no_profile:
+        from openzl_amd import _lib
+        Lk = _lib.lib()
+        Lk.ozl_profile_begin()
+        Lk.ozl_profile_tag(b"enc/")
+        comp.compress_ptrs(ptrs, sizes_in, typ, width, dfr.data_ptr(), cap)
+        Lk.ozl_profile_tag(b"dec/")
+        dec.decompress_ptrs(dfr.data_ptr(), fsz, dout.data_ptr(), U + 64)
+        torch.cuda.synchronize()
+        bufp = ctypes.create_string_buffer(1 << 16)
+        Lk.ozl_profile_end.restype = ctypes.c_size_t
+        Lk.ozl_profile_end.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+        Lk.ozl_profile_end(bufp, len(bufp))
+        for line in bufp.value.decode().splitlines():
+            name, cnt, kms = line.split()
+            prof[name] = (int(cnt), float(kms))
+        Lk.ozl_profile_tag(b"")
     # host buffers: one step each way
     h0 = time.perf_counter()
     comp.compress_batch(parts, typ, width)
@@ -517,48 +562,72 @@ def bench_engine(args):
     host_ms = (h2 - h0) * 1000
     cpu = None
     if args.cpu_sample > 0 and C.refh() is not None:
-        # the reference library on the same frames: one thread for the single
-        # timestamps frame, all host cores (one context per thread) for the bf16 frames
-        threads = 1 if len(parts) == 1 else max(1, min(16, len(os.sched_getaffinity(0))))
+        # the reference library on the same frames, one context per thread:
+        # the bf16 frames on all host cores; the single timestamps frame on one
+        # thread (the reference codes one frame on one thread, so no number of
+        # cores makes one frame faster), with the throughput of all cores each
+        # coding its own copy reported beside it
+        nproc, hinfo = host_threads()
         ctyp = {api.SERIAL: C.SERIAL, api.STRUCT: C.STRUCT, api.NUMERIC: C.NUMERIC}[typ]
-        res = [0] * threads
-        t_start = time.perf_counter()
+        if segment:
+            C.ref_set_segmenter(*segment)
+        assert frames[0] == C.ref_compress(kind, parts[0], ctyp, width), "frame differs from the reference's"
 
-        def worker(tid):
-            i, nbytes = tid, 0
-            while time.perf_counter() - t_start < args.cpu_seconds:
-                p = parts[i % len(parts)]
-                f = C.ref_compress(kind, p, ctyp, width)
-                C.ref_decompress(f, p.nbytes + 64)
-                nbytes += p.nbytes
-                i += threads
-            res[tid] = nbytes
-        ths = [threading.Thread(target=worker, args=(t,)) for t in range(threads)]
-        for t in ths:
-            t.start()
-        for t in ths:
-            t.join()
-        wall = time.perf_counter() - t_start
-        cpu = {"value": round(sum(res) / wall / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "reference",
+        def run_ref(threads, budget):
+            res = [0] * threads
+            t_start = time.perf_counter()
+
+            def worker(tid):
+                i, nbytes = tid, 0
+                while time.perf_counter() - t_start < budget:
+                    p = parts[i % len(parts)]
+                    f = C.ref_compress(kind, p, ctyp, width)
+                    C.ref_decompress(f, p.nbytes + 64)
+                    nbytes += p.nbytes
+                    i += threads
+                res[tid] = nbytes
+            ths = [threading.Thread(target=worker, args=(t,)) for t in range(threads)]
+            for t in ths:
+                t.start()
+            for t in ths:
+                t.join()
+            wall = time.perf_counter() - t_start
+            return sum(res), wall
+
+        single = len(parts) == 1
+        threads = 1 if single else nproc
+        nbytes, wall = run_ref(threads, args.cpu_seconds)
+        cpu = {"value": round(nbytes / wall / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "reference",
                "sample": f"the same frames cycled, compressed and decompressed by the reference library "
-                         f"({threads} thread(s)); {sum(res) / 1e9:.3f} GB in {wall:.1f} s wall"}
+                         f"({threads} thread(s)); {nbytes / 1e9:.3f} GB in {wall:.1f} s wall", **hinfo}
+        if single and nproc > 1:
+            nb_all, wall_all = run_ref(nproc, args.cpu_seconds / 2)
+            cpu["all_cores_independent_frames_GBps"] = round(nb_all / wall_all / 1e9, 4)
     ms = enc_ms + dec_ms
     both = 2 * (U + Cb) / (ms / 1000) / 1e9
+    traffic = load_traffic(f"engine_{args.workload}")
+    dom = max(prof.items(), key=lambda kv: kv[1][1]) if prof else None
     rec = {
         "metric": METRIC, "value": round(U / (ms / 1000) / 1e9, 3), "unit": "GB/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded generator)",
         "config": {"workload": workload, "input_bytes": U, "frame_bytes": Cb, "ratio": round(U / Cb, 4),
                    "frames": len(frames), "encode_ms": round(enc_ms, 3), "decode_ms": round(dec_ms, 3),
+                   "encode_GBps": round(U / (enc_ms / 1000) / 1e9, 3),
+                   "decode_GBps": round(U / (dec_ms / 1000) / 1e9, 3),
                    "buffers": "inputs, frames and outputs in HBM (one batch call per direction)",
                    "host_buffers_GBps": round(U / (host_ms / 1000) / 1e9, 3),
                    "host_buffers_ms": round(host_ms, 3)},
         "roofline": {"bound": "hbm", "kernel": "whole engine calls (host-orchestrated graph walk, device-resident)",
                      "achieved": round(both, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(both / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(both / HBM_PEAK_GBS, 4),
+                     "traffic": traffic.get("total_bytes") if traffic else None,
+                     "algorithmic_bytes": 2 * (U + Cb),
+                     "dominant_kernel": dom[0] if dom else None,
+                     "kernel_ms_per_call": {k: round(v[1], 4) for k, v in sorted(prof.items())}},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(rec))
+    return rec
 
 
 # ---------------------------------------------------------------- main
@@ -568,7 +637,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--blocks", type=int, default=N_BLOCKS)
-    ap.add_argument("--cpu-sample", type=int, default=1024, help="distinct blocks in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=N_BLOCKS,
+                    help="distinct blocks of the input in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="the entropy stress line only (no sub-records of the other workloads)")
     ap.add_argument("--cpu-seconds", type=float, default=8.0, help="wall-clock budget of the CPU baseline")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -586,11 +658,11 @@ def main():
         sys.exit(2)
 
     if args.workload == "columns":
-        return bench_columns(args)
+        return print(json.dumps(bench_columns(args)))
     if args.workload == "select":
-        return bench_select(args)
+        return print(json.dumps(bench_select(args)))
     if args.workload in ("timestamps", "bf16", "bf16seg"):
-        return bench_engine(args)
+        return print(json.dumps(bench_engine(args)))
     import torch
     from openzl_amd import batch, _lib
 
@@ -701,30 +773,53 @@ def main():
                     "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": per_launch,
                     "kernel_ms_per_step": {k: round(v[1] / args.steps, 4) for k, v in sorted(prof.items())}}
 
-    # host-buffer variant (not `value`): pinned H2D / D2H bandwidth, no overlap
-    pin = torch.empty(1 << 28, dtype=torch.uint8, pin_memory=True)
-    dbuf = torch.empty(1 << 28, dtype=torch.uint8, device=dev)
-    t_a = time.perf_counter()
-    for _ in range(4):
-        dbuf.copy_(pin, non_blocking=True)
-    torch.cuda.synchronize()
-    h2d = 4 * (1 << 28) / (time.perf_counter() - t_a)
-    t_a = time.perf_counter()
-    for _ in range(4):
-        pin.copy_(dbuf, non_blocking=True)
-    torch.cuda.synchronize()
-    d2h = 4 * (1 << 28) / (time.perf_counter() - t_a)
-    C_tot = csize[batch.FSE] + csize[batch.HUFFMAN]
-    xfer_s = (2 * U + C_tot) / h2d + (2 * U + C_tot) / d2h
-    pcie_rate = 2 * U / (ms_step / 1000 + xfer_s) / 1e9
-    del pin, dbuf
+    # host buffers (not `value`): the input in pinned host memory, copied in,
+    # encoded, the frames copied out (their size read back first); then the
+    # frames copied in, decoded and the output copied out; wall clock
+    print("[bench] host-buffer steps (PCIe included)", file=sys.stderr, flush=True)
+    pin_in = torch.empty(U, dtype=torch.uint8, pin_memory=True)
+    pin_in.copy_(src)
+    pin_fr = {m: torch.empty(csize[m] + (1 << 20), dtype=torch.uint8, pin_memory=True) for m in enc}
+    pin_out = torch.empty(U, dtype=torch.uint8, pin_memory=True)
+    d_in = torch.empty_like(src)
+    d_fr = {m: torch.empty(csize[m] + (1 << 20), dtype=torch.uint8, device=dev) for m in enc}
+    host_ms = {"encode": 0.0, "decode": 0.0}
+    nh = 2
+    for it in range(nh + 1):
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        d_in.copy_(pin_in, non_blocking=True)
+        fr_h = {}
+        for m in (batch.FSE, batch.HUFFMAN):
+            dst, foff, _ = enc[m].run(m, d_in, off)
+            nbytes = int(foff[B].item())
+            pin_fr[m][:nbytes].copy_(dst[:nbytes], non_blocking=True)
+            fr_h[m] = (nbytes, foff)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        for m in (batch.FSE, batch.HUFFMAN):
+            nbytes, foff = fr_h[m]
+            d_fr[m][:nbytes].copy_(pin_fr[m][:nbytes], non_blocking=True)
+            dec[m].run(d_fr[m], foff, out[m], off)
+            pin_out.copy_(out[m], non_blocking=True)
+        torch.cuda.synchronize()
+        tc = time.perf_counter()
+        if it:   # the first pass warms the pinned buffers
+            host_ms["encode"] += (tb - ta) * 1000 / nh
+            host_ms["decode"] += (tc - tb) * 1000 / nh
+    assert torch.equal(out[batch.HUFFMAN], src) and bool(torch.equal(pin_out, pin_in))
+    host_rate = 2 * U / ((host_ms["encode"] + host_ms["decode"]) / 1000) / 1e9
+    del pin_fr, d_fr, d_in
 
     cpu = None
     if args.cpu_sample > 0:
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-        threads = max(1, min(threads, 16))
-        print(f"[bench] CPU baseline on {threads} threads", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.cpu_sample, threads, args.cpu_seconds)
+        threads, hinfo = host_threads()
+        nb = min(args.cpu_sample, B)
+        print(f"[bench] CPU baseline on {threads} threads over {nb} blocks", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(pin_in[:nb * BLOCK].numpy(), threads, args.cpu_seconds)
+        if cpu:
+            cpu.update(hinfo)
+    del pin_in, pin_out
 
     rec = {
         "metric": METRIC,
@@ -749,13 +844,61 @@ def main():
             "encode_GBps": round(2 * U / (enc_ms / 1000) / 1e9, 3),
             "decode_GBps": round(2 * U / (dec_ms / 1000) / 1e9, 3),
             "encode_ms_per_step": round(enc_ms, 3), "decode_ms_per_step": round(dec_ms, 3),
-            "host_buffer_GBps_no_overlap": round(pcie_rate, 3),
-            "pcie_h2d_GBps": round(h2d / 1e9, 2), "pcie_d2h_GBps": round(d2h / 1e9, 2),
+            "host_buffers_GBps": round(host_rate, 3),
+            "host_buffers_encode_ms": round(host_ms["encode"], 3),
+            "host_buffers_decode_ms": round(host_ms["decode"], 3),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if not args.no_configs:
+        del src, out, enc, dec, frames
+        torch.cuda.empty_cache()
+        rec["config"]["workloads"] = other_workloads(args)
     print(json.dumps(rec))
+
+
+def summary(sub):
+    """The figures of a workload's own record kept in the headline line."""
+    c = sub["config"]
+    keep = {k: v for k, v in c.items() if k.endswith(("GBps", "_ms", "ratio", "speedup", "agreement"))
+            or k.startswith(("ratio", "learned_", "exhaustive_", "selection_", "frame_bytes", "input_bytes"))}
+    r = sub.get("roofline") or {}
+    cpu = sub.get("cpu_baseline")
+    out = {"workload": c["workload"], "value": sub["value"], "unit": sub["unit"], "ms_per_step": sub["ms_per_step"],
+           "steps": sub["steps"], "dtype": sub["dtype"], **keep,
+           "roofline": {k: r.get(k) for k in ("bound", "kernel", "dominant_kernel", "achieved", "peak", "unit",
+                                              "frac", "traffic", "algorithmic_bytes") if k in r},
+           "cpu_baseline": cpu,
+           "vs_cpu": round(sub["value"] / cpu["value"], 2) if cpu and cpu.get("value") else None}
+    return out
+
+
+def other_workloads(args):
+    """The other workloads of the hot path, each timed by its own function
+    with fewer steps: the 16 MiB timestamps frame and the 1 GiB bf16 tensor
+    through the ZL_CCtx engine, the 4,096 batched columns, and learned
+    selection over 8,192 chunks against exhaustive trial."""
+    import copy
+    import torch
+    res = {}
+    for name, fn, wl in (("timestamps_16MiB", bench_engine, "timestamps"), ("columns_4096", bench_columns, None),
+                         ("bf16_1GiB", bench_engine, "bf16"), ("select_8192", bench_select, None)):
+        a = copy.copy(args)
+        a.steps = max(1, min(args.steps, 5))
+        a.warmup = 1
+        if wl:
+            a.workload = wl
+        print(f"[bench] workload {name}", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        try:
+            res[name] = summary(fn(a))
+        except Exception as e:   # the headline line is still printed
+            res[name] = {"error": f"{type(e).__name__}: {e}"}
+        res[name]["wall_s"] = round(time.perf_counter() - t0, 1)
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    return res
 
 
 if __name__ == "__main__":
