@@ -235,6 +235,50 @@ void* ZL_Segmenter_getScratchSpace(ZL_Segmenter* segCtx, size_t size);
 /* include/openzl/zl_segmenter.h:300-330; rGraphParams must be NULL */
 ZL_Report ZL_Segmenter_processChunk(ZL_Segmenter* segCtx, const size_t numElts[], size_t numInputs,
                                     ZL_GraphID startingGraphID, const void* rGraphParams);
+/* Selector graphs: include/openzl/zl_selector.h:156-222. The selector
+ * function runs on the host during the graph walk and returns the successor
+ * of its whole input (any graph of the compressor accepting the input's type);
+ * customGraphs are handed back to it. The input's bytes are in device memory
+ * (ZL_Input_ptr). localParams are not supported (must be zero). */
+typedef struct ZL_Selector_s ZL_Selector;
+typedef ZL_GraphID (*ZL_SelectorFn)(const ZL_Selector* selectorAPI, const ZL_Input* input,
+                                    const ZL_GraphID* customGraphs, size_t nbCustomGraphs);
+typedef struct {
+    ZL_SelectorFn selector_f;
+    ZL_Type inStreamType;
+    const ZL_GraphID* customGraphs;
+    size_t nbCustomGraphs;
+    char localParams[48];             /* layout of ZL_LocalParams; must be zero */
+    const char* name;
+    ZL_OpaquePtr opaque;
+} ZL_SelectorDesc;
+/* include/openzl/zl_selector.h:222 (the compressor owns desc->opaque from
+ * here on, even when registration fails) */
+ZL_GraphID ZL_Compressor_registerSelectorGraph(ZL_Compressor* compressor, const ZL_SelectorDesc* desc);
+/* include/openzl/zl_selector.h:235 */
+const void* ZL_Selector_getOpaquePtr(const ZL_Selector* selector);
+/* include/openzl/zl_input.h:107: the first byte of the input (device memory here) */
+const void* ZL_Input_ptr(const ZL_Input* input);
+
+/* The learned selector as a selector function (the counterpart of the
+ * reference's ML selector, src/openzl/compress/selectors/ml/mlselector.c:
+ * ZL_Compressor_registerMLSelectorGraph with a model of its own): the 40
+ * features and the bf16 MLP of ozl_select_features / ozl_select_mlp run on
+ * the device over the input, and the argmax picks customGraphs[choice]. The
+ * selector's opaque pointer is a model from ozl_learned_model_create, and
+ * customGraphs are the 8 candidates in the order of ozl_select_compress
+ * (STORE, FSE, BITPACK, DELTA_INT -> FSE, DELTA_INT -> ZIGZAG -> BITPACK,
+ * transpose_split -> FSE, transpose_split -> HUFFMAN, DELTA_INT ->
+ * transpose_split -> FSE). With no model or fewer candidates than the choice
+ * it returns customGraphs[0]. No counterpart in the reference. */
+ZL_GraphID ozl_learned_selector(const ZL_Selector* selectorAPI, const ZL_Input* input, const ZL_GraphID* customGraphs,
+                                size_t nbCustomGraphs);
+/* A device copy of the MLP parameters (ozl_select_mlp_params_bytes() bytes
+ * at hostParams), or NULL; ozl_learned_model_free is its ZL_OpaquePtr free
+ * function. No counterpart in the reference. */
+void* ozl_learned_model_create(const void* hostParams, size_t bytes);
+void ozl_learned_model_free(void* freeOpaquePtr, void* model);
+
 typedef struct {
     ZL_Edge** edges;
     size_t nbEdges;

@@ -3,6 +3,7 @@ program calls). Builds the graphs of the benchmark configurations exactly as a
 reference user would (include/openzl/zl_compressor.h), compresses host
 buffers and decompresses frames; every codec runs on the GPU."""
 import ctypes
+import os
 
 import numpy as np
 
@@ -24,12 +25,25 @@ class NodeID(ctypes.Structure):
     _fields_ = [("nid", ctypes.c_uint)]
 
 
+class OpaquePtr(ctypes.Structure):
+    _fields_ = [("ptr", ctypes.c_void_p), ("freeOpaquePtr", ctypes.c_void_p), ("freeFn", ctypes.c_void_p)]
+
+
+class SelectorDesc(ctypes.Structure):
+    """ZL_SelectorDesc (include/openzl/zl_selector.h:174-193)."""
+    _fields_ = [("selector_f", ctypes.c_void_p), ("inStreamType", ctypes.c_int),
+                ("customGraphs", ctypes.c_void_p), ("nbCustomGraphs", ctypes.c_size_t),
+                ("localParams", ctypes.c_char * 48), ("name", ctypes.c_char_p), ("opaque", OpaquePtr)]
+
+
 class OutputInfo(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int), ("fixedWidth", ctypes.c_uint32), ("decompressedByteSize", ctypes.c_uint64),
                 ("numElts", ctypes.c_uint64)]
 
 
 SERIAL, STRUCT, NUMERIC = 1, 2, 4
+ZL_GRAPH_INVALID, ZL_ALLOCATION = 30, 70
+LEARNED_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "selector_mlp.bin")
 # ZL_CParam values (include/openzl/zl_compress.h:64-131)
 CP_STICKY, CP_LEVEL, CP_DLEVEL, CP_FORMAT, CP_PERMISSIVE, CP_KSUM, CP_CSUM, CP_MINSTREAM = 1, 2, 3, 4, 5, 6, 7, 11
 G_STORE, G_FSE, G_HUFFMAN, G_BITPACK = GraphID(2), GraphID(3), GraphID(4), GraphID(8)
@@ -37,7 +51,9 @@ N_DELTA, N_TRANSPOSE_SPLIT, N_ZIGZAG = NodeID(2), NodeID(3), NodeID(4)
 
 # graph kinds, same numbering as the reference harness (oracle/ref_harness.c)
 STORE, FSE, HUFFMAN, BITPACK, DELTA_TS_FSE, DELTA_ZZ_BP, TS_HUF_FSE, DELTA_FSE, TS_FSE, TS_HUF, ZZ_BP, DELTA_BP, SELECT8, \
-    SEGMENTED = range(14)
+    SEGMENTED, LEARNED8 = range(15)
+# the learned selector's candidates, in the model's output order
+SELECT_CANDIDATES = (STORE, FSE, BITPACK, DELTA_FSE, DELTA_ZZ_BP, TS_FSE, TS_HUF, DELTA_TS_FSE)
 
 _setup = False
 
@@ -62,6 +78,8 @@ def _api():
         "ZL_CCtx_getParameter": (ctypes.c_int, [P, ctypes.c_int]),
         "ZL_CCtx_resetParameters": (ZLReport, [P]),
         "ZL_CCtx_selectStartingGraphID": (ZLReport, [P, P, GraphID, P]),
+        "ZL_Compressor_registerSelectorGraph": (GraphID, [P, P]),
+        "ozl_learned_model_create": (P, [P, SZ]),
         "ZL_CCtx_getErrorContextString": (ctypes.c_char_p, [P, ZLReport]),
         "ZL_DCtx_getErrorContextString": (ctypes.c_char_p, [P, ZLReport]),
         "ZL_Compressor_getParameter": (ctypes.c_int, [P, ctypes.c_int]),
@@ -191,8 +209,10 @@ class Compressor:
               "compressedChecksum": CP_KSUM}
 
     def __init__(self, kind, candidates=None, content_checksum=True, compressed_checksum=True, segment=None,
-                 params=None):
-        """params: further compressor parameters by name (PARAMS)."""
+                 params=None, model=None):
+        """params: further compressor parameters by name (PARAMS); model: the
+        learned selector's parameter bytes for LEARNED8 (default: the
+        packaged model)."""
         L = _api()
         self.L = L
         self.c = L.ZL_Compressor_create()
@@ -203,7 +223,24 @@ class Compressor:
             _chk(L.ZL_Compressor_setParameter(self.c, 7, 2), "contentChecksum")
         if not compressed_checksum:
             _chk(L.ZL_Compressor_setParameter(self.c, 6, 2), "compressedChecksum")
-        if candidates is not None:
+        if kind == LEARNED8:
+            # the learned selector over the 8 candidates, the model owned by
+            # the compressor (freed with it)
+            self._gids = (GraphID * len(SELECT_CANDIDATES))(*[build_graph(L, self.c, k) for k in SELECT_CANDIDATES])
+            raw = np.fromfile(LEARNED_MODEL, dtype=np.uint8) if model is None else np.ascontiguousarray(model)
+            m = L.ozl_learned_model_create(raw.ctypes.data, raw.nbytes)
+            if not m:
+                raise ZLError(ZL_ALLOCATION, "ozl_learned_model_create")
+            d = SelectorDesc()
+            d.selector_f = ctypes.cast(L.ozl_learned_selector, ctypes.c_void_p).value
+            d.inStreamType = SERIAL | NUMERIC
+            d.customGraphs = ctypes.cast(self._gids, ctypes.c_void_p).value
+            d.nbCustomGraphs = len(SELECT_CANDIDATES)
+            d.opaque = OpaquePtr(m, None, ctypes.cast(L.ozl_learned_model_free, ctypes.c_void_p).value)
+            g = L.ZL_Compressor_registerSelectorGraph(self.c, ctypes.byref(d))
+            if g.gid == 0:
+                raise ZLError(ZL_GRAPH_INVALID, "ZL_Compressor_registerSelectorGraph")
+        elif candidates is not None:
             gids = (GraphID * len(candidates))(*[build_graph(L, self.c, k) for k in candidates])
             g = L.ZL_Compressor_registerBruteForceSelectorGraph(self.c, gids, len(candidates))
         else:

@@ -49,7 +49,7 @@ enum { NID_DELTA = 2, NID_TRANSPOSE_SPLIT = 3, NID_ZIGZAG = 4, NID_CONV_STRUCT_T
        NID_CONV_NUM_TO_STRUCT = 11, NID_CONV_NUM_TO_SERIAL = 12 };
 enum { GID_SERIAL_STORE = 1, GID_STORE = 2, GID_FSE = 3, GID_HUF = 4, GID_CONSTANT = 6, GID_BITPACK = 8,
        GID_FLATPACK = 9, GID_CUSTOM_BASE = 4096 };
-enum { GK_STATIC, GK_TRANSPOSE, GK_FUNCTION, GK_BRUTE, GK_SEGMENTER };
+enum { GK_STATIC, GK_TRANSPOSE, GK_FUNCTION, GK_BRUTE, GK_SEGMENTER, GK_SELECTOR };
 enum { T_SERIAL = 1, T_STRUCT = 2, T_NUMERIC = 4 };
 
 ZL_Report rep_ok(size_t v)
@@ -78,6 +78,7 @@ struct GDef {
     int mask;
     ZL_SegmenterFn segFn = nullptr;
     const void* opaque = nullptr;
+    ZL_SelectorFn selFn = nullptr;
 };
 
 // grow-only device arena, reset per compression
@@ -240,6 +241,10 @@ struct EntBatch {
 struct ZL_Input_s {
     Session* s;
     uint32_t sid;
+};
+struct ZL_Selector_s {
+    Session* s;
+    const GDef* g;
 };
 struct ZL_Edge_s {
     ZL_Graph* g;
@@ -919,8 +924,24 @@ struct Session {
         }
         case GK_FUNCTION: return runFunction(g, sid);
         case GK_BRUTE: return runBrute(g, sid);
+        case GK_SELECTOR: return runSelector(g, sid);
         default: return OZL_ERR_GRAPH_INVALID;
         }
+    }
+
+    // A selector graph (include/openzl/zl_selector.h:160-222): the selector
+    // function picks the successor of the whole input, which then runs as if
+    // it had been started on it (selectors leave nothing in the frame).
+    int runSelector(const GDef* g, uint32_t sid)
+    {
+        ZL_Selector_s sel{ this, g };
+        ZL_Input_s in{ this, sid };
+        std::vector<ZL_GraphID> cg;
+        for (unsigned x : g->succ) cg.push_back(ZL_GraphID{ x });
+        const ZL_GraphID r = g->selFn(&sel, &in, cg.empty() ? nullptr : cg.data(), cg.size());
+        if (err) return err;
+        if (!graphMask(r.gid)) return OZL_ERR_GRAPH_INVALID;
+        return runSuccessor(r.gid, sid);
     }
 
     int runFunction(const GDef* g, uint32_t sid)
@@ -1771,6 +1792,76 @@ ZL_GraphID ZL_Compressor_registerFunctionGraph(ZL_Compressor* c, const ZL_Functi
 {
     if (!c || !d || !d->graph_f || d->nbInputs != 1 || !d->inputTypeMasks) return ZL_GRAPH_ILLEGAL;
     return add_graph(c, GDef{ GK_FUNCTION, 0, {}, d->graph_f, (int)d->inputTypeMasks[0] });
+}
+
+ZL_GraphID ZL_Compressor_registerSelectorGraph(ZL_Compressor* c, const ZL_SelectorDesc* d)
+{
+    if (c && d && d->opaque.freeFn) c->opaques.push_back(d->opaque);   // owned even when registration fails
+    if (!c || !d || !d->selector_f || !d->inStreamType) return ZL_GraphID{ 0 };
+    if (d->nbCustomGraphs && !d->customGraphs) return ZL_GraphID{ 0 };
+    for (size_t i = 0; i < sizeof(d->localParams); i++)
+        if (d->localParams[i]) return ZL_GraphID{ 0 };
+    GDef g;
+    g.kind = GK_SELECTOR;
+    g.node = 0;
+    g.fn = nullptr;
+    g.mask = (int)d->inStreamType & (T_SERIAL | T_STRUCT | T_NUMERIC);
+    g.selFn = d->selector_f;
+    g.opaque = d->opaque.ptr;
+    for (size_t i = 0; i < d->nbCustomGraphs; i++) g.succ.push_back(d->customGraphs[i].gid);
+    if (!g.mask) return ZL_GraphID{ 0 };
+    return add_graph(c, std::move(g));
+}
+
+const void* ZL_Selector_getOpaquePtr(const ZL_Selector* sel) { return sel ? sel->g->opaque : nullptr; }
+
+const void* ZL_Input_ptr(const ZL_Input* in) { return in->s->streams[in->sid].dev; }
+
+// The learned selector (features + MLP on the device over the input).
+ZL_GraphID ozl_learned_selector(const ZL_Selector* sel, const ZL_Input* in, const ZL_GraphID* cg, size_t ncg)
+{
+    const ZL_GraphID first = ncg ? cg[0] : ZL_GraphID{ GID_STORE };
+    if (!sel || !in || !cg || ncg == 0 || !sel->g->opaque) return first;
+    Session* s = sel->s;
+    const EStream x = s->streams[in->sid];
+    if (s->err || x.bytes() == 0) return first;
+    // spans (2 u64), width, 40 features, choice
+    uint8_t* w = s->alloc(512);
+    if (!w) return first;
+    uint64_t* spans = reinterpret_cast<uint64_t*>(w);
+    uint8_t* width = w + 16;
+    float* feats = reinterpret_cast<float*>(w + 64);
+    uint8_t* choice = w + 256;
+    const uint64_t hs[2] = { 0, (uint64_t)x.bytes() };
+    const uint8_t hw = (uint8_t)x.width;
+    uint8_t hc = 0;
+    if (hipMemcpyAsync(spans, hs, sizeof(hs), hipMemcpyHostToDevice, s->st) != hipSuccess ||
+        hipMemcpyAsync(width, &hw, 1, hipMemcpyHostToDevice, s->st) != hipSuccess ||
+        ozl_select_features(x.dev, spans, width, 1, feats, s->st) != OZL_OK ||
+        ozl_select_mlp(feats, 1, sel->g->opaque, choice, nullptr, s->st) != OZL_OK ||
+        hipMemcpyAsync(&hc, choice, 1, hipMemcpyDeviceToHost, s->st) != hipSuccess ||
+        hipStreamSynchronize(s->st) != hipSuccess) {
+        s->err = OZL_ERR_HIP;
+        return first;
+    }
+    return hc < ncg ? cg[hc] : first;
+}
+
+void* ozl_learned_model_create(const void* hostParams, size_t bytes)
+{
+    if (!hostParams || bytes != ozl_select_mlp_params_bytes()) return nullptr;
+    void* d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, hostParams, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        return nullptr;
+    }
+    return d;
+}
+
+void ozl_learned_model_free(void*, void* model)
+{
+    if (model) (void)hipFree(model);
 }
 
 ZL_Report ZL_Compressor_selectStartingGraphID(ZL_Compressor* c, ZL_GraphID g)

@@ -11,7 +11,8 @@ Outputs (all under tests/golden/):
                         width, the generator call that rebuilds the input, the
                         input's sha256, and the reference frame's size and sha256
   small/<name>.zl       the reference frame of each small case
-  full.json             full-size workloads (with --full): per-frame sizes and
+  full.json             full-size workloads (with --full; --select-full adds
+                        only the 8,192-chunk selector batch): per-frame sizes and
                         sha256 prefixes of the reference frames, plus the
                         sha256 of all frames concatenated in order
   full/<name>.npy       per-frame (size, 8-byte sha256 prefix as u64) tables
@@ -243,15 +244,45 @@ def write_full():
     print(f"full: {time.time() - t0:.0f} s")
 
 
+def write_select_full():
+    """The learned-selection workload's 8,192 chunks (datagen.selector_chunks,
+    the benchmark's seed) through the reference's brute-force selector over
+    the eight candidates: per-frame digests, added to full.json."""
+    path = os.path.join(HERE, "full.json")
+    with open(path) as f:
+        meta = json.load(f)
+    seed = G.SEEDS[5]
+    _, chunks = G.selector_chunks(8192, seed=seed)
+    hin = hashlib.sha256()
+    for c in chunks:
+        hin.update(c.tobytes())
+    with ThreadPoolExecutor(8) as ex:
+        frames = list(ex.map(lambda c: C.ref_compress(C.SELECT8, c, 2, c.dtype.itemsize), chunks))
+    tab, whole = digest_table(frames)
+    np.save(os.path.join(HERE, "full", "select_8192_brute_force.npy"), tab)
+    meta["select_8192_brute_force"] = {"gen": "selector_chunks", "kwargs": {"n_chunks": 8192, "seed": seed},
+                                       "kind": C.SELECT8, "type": 2, "width": 0, "input_sha256": hin.hexdigest(),
+                                       "frames": len(frames), "total_frame_bytes": int(tab[:, 0].sum()),
+                                       "sha256_all": whole}
+    with open(path, "w") as f:
+        json.dump(meta, f, indent=1)
+    print(f"select_8192_brute_force: {len(frames)} frames, {int(tab[:, 0].sum())} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--full", action="store_true")
+    ap.add_argument("--select-full", action="store_true",
+                    help="only add the 8,192-chunk brute-force selector digests to full.json")
     a = ap.parse_args()
     if C.refh() is None:
         sys.exit("oracle/_ref/libref_harness.so is not built (make -C oracle ref)")
+    if a.select_full:
+        return write_select_full()
     write_small()
     if a.full:
         write_full()
+    write_select_full()
 
 
 if __name__ == "__main__":

@@ -59,6 +59,7 @@ DROP_IN = [
     "ZL_CCtx_selectStartingGraphID", "ZL_Compressor_getParameter", "ZL_CCtx_getErrorContextString",
     "ZL_CCtx_getErrorContextString_fromError", "ZL_DCtx_getErrorContextString",
     "ZL_DCtx_getErrorContextString_fromError", "ZL_Compressor_getErrorContextString_fromError",
+    "ZL_Compressor_registerSelectorGraph", "ZL_Selector_getOpaquePtr", "ZL_Input_ptr",
 ]
 
 
@@ -70,8 +71,8 @@ def test_header_declares_the_drop_in_list():
 
 def test_descriptor_layouts_match_the_reference():
     """Offsets of the descriptor fields a caller fills in: the reference's
-    declarations (include/openzl/zl_graph_api.h:74-95, zl_segmenter.h:70-108)
-    compiled with the same compiler."""
+    declarations (include/openzl/zl_graph_api.h:74-95, zl_segmenter.h:70-108,
+    zl_selector.h:174-193) compiled with the same compiler."""
     import subprocess
     import tempfile
     src = r'''
@@ -81,6 +82,9 @@ def test_descriptor_layouts_match_the_reference():
 int main(void) {
   printf("%zu %zu %zu %zu %zu\n", sizeof(ZL_SegmenterDesc), offsetof(ZL_SegmenterDesc, lastInputIsVariable),
          offsetof(ZL_SegmenterDesc, customGraphs), offsetof(ZL_SegmenterDesc, opaque), sizeof(bool));
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(ZL_SelectorDesc), offsetof(ZL_SelectorDesc, inStreamType),
+         offsetof(ZL_SelectorDesc, customGraphs), offsetof(ZL_SelectorDesc, nbCustomGraphs),
+         offsetof(ZL_SelectorDesc, localParams), offsetof(ZL_SelectorDesc, name), offsetof(ZL_SelectorDesc, opaque));
   return 0; }
 '''
     with tempfile.TemporaryDirectory() as d:
@@ -88,10 +92,15 @@ int main(void) {
         open(c, "w").write(src)
         exe = os.path.join(d, "t")
         subprocess.run(["gcc", "-std=gnu11", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
-        size, lv, cg, op, bsz = map(int, subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split())
+        lines = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.splitlines()
+    size, lv, cg, op, bsz = map(int, lines[0].split())
     # name, segmenterFn, inputTypeMasks, numInputs (8 bytes each), then the
     # bool, padded to the next pointer
     assert bsz == 1 and lv == 32 and cg == 40 and op == 40 + 8 + 8 + 48 and size == op + 24
+    # ZL_SelectorDesc (zl_selector.h:174-193, as the reference's header
+    # compiles with gcc): selector_f, the ZL_Type enum padded to 8, the graph
+    # list, ZL_LocalParams (3 x pointer + size), name, ZL_OpaquePtr
+    assert list(map(int, lines[1].split())) == [112, 8, 16, 24, 32, 80, 88]
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
