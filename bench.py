@@ -605,7 +605,7 @@ def bench_engine(args):
             cpu["all_cores_independent_frames_GBps"] = round(nb_all / wall_all / 1e9, 4)
     ms = enc_ms + dec_ms
     both = 2 * (U + Cb) / (ms / 1000) / 1e9
-    traffic = load_traffic(f"engine_{args.workload}")
+    traffic = load_traffic({"timestamps": "engine_timestamps_16MiB", "bf16": "engine_bf16_1GiB"}.get(args.workload, ""))
     dom = max(prof.items(), key=lambda kv: kv[1][1]) if prof else None
     rec = {
         "metric": METRIC, "value": round(U / (ms / 1000) / 1e9, 3), "unit": "GB/s", "n_gpus": 1,
