@@ -538,6 +538,7 @@ struct ozl_huf_scratch {
     int stk[32];
     uint32_t rankLast[14];
     uint16_t perRank[13], valPerRank[13];
+    int nonNull;    // last leaf with a nonzero count (set by ozl_huf_build)
 };
 
 OZL_HD uint32_t ozl_huf_bucket(uint32_t c) { return c < 166 ? c : (uint32_t)ozl_hb32(c) + 158; }
@@ -588,16 +589,22 @@ OZL_HD void ozl_huf_sort_range(ozl_hnode* a, int lo0, int hi0, int* stk)
 // HUF_buildCTable_wksp (huf_compress.c:731-764): sort (:596-641), tree
 // (:657-704), height limit (:352-500), canonical codes (:706-729).
 // Returns the max code length or -1.
+// `zeroed`: the caller has cleared W->tbl and W->base (a wave does it in
+// parallel on the device). `lengthsOnly`: stop once the leaves W->tbl[1 + k]
+// (k <= W->nonNull, in sorted order) hold their final code lengths; the
+// caller derives nbBits and values from them (canonical codes, :706-729).
 OZL_HD int ozl_huf_build(uint8_t* nbBits, uint16_t* values, const uint32_t* count, unsigned maxSym,
-                         unsigned maxNbBits, ozl_huf_scratch* W)
+                         unsigned maxNbBits, ozl_huf_scratch* W, bool zeroed = false, bool lengthsOnly = false)
 {
     ozl_hnode* tbl = W->tbl;
     ozl_hnode* node = tbl + 1;
-    for (int k = 0; k < 2 * 256 + 2; k++) { tbl[k].count = 0; tbl[k].parent = 0; tbl[k].byte = 0; tbl[k].nbBits = 0; }
+    if (!zeroed)
+        for (int k = 0; k < 2 * 256 + 2; k++) { tbl[k].count = 0; tbl[k].parent = 0; tbl[k].byte = 0; tbl[k].nbBits = 0; }
     {
         uint16_t* base = W->base;
         uint16_t* curr = W->curr;
-        for (int r = 0; r < 192; r++) base[r] = 0;
+        if (!zeroed)
+            for (int r = 0; r < 192; r++) base[r] = 0;
         for (unsigned s = 0; s <= maxSym; s++) base[ozl_huf_bucket(count[s])]++;
         for (int r = 191; r > 0; r--) base[r - 1] = (uint16_t)(base[r - 1] + base[r]);
         for (int r = 0; r < 192; r++) curr[r] = base[r];
@@ -690,6 +697,8 @@ OZL_HD int ozl_huf_build(uint8_t* nbBits, uint16_t* values, const uint32_t* coun
         largest = target;
     }
     if (largest > 12) return -1;
+    W->nonNull = nonNull;
+    if (lengthsOnly) return (int)largest;
     uint16_t* perRank = W->perRank;
     uint16_t* valPerRank = W->valPerRank;
     for (int r = 0; r < 13; r++) { perRank[r] = 0; valPerRank[r] = 0; }

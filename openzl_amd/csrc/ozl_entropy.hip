@@ -93,14 +93,17 @@ struct PlanLds {
 
 // Nested FSE graph on the Huffman weights (entropyCompressChunk in EBM_fse
 // mode on the num->serial converted weights stream).
+// L.wcount holds the histogram of the m weights (every other bin zero).
 __device__ __forceinline__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* aux, int minSS)
 {
     uint32_t m = p.maxSym + 1u;
     if (ozl_small(m, minSS)) { p.wbk = 255; return; }
-    for (int s = 0; s < 256; s++) L.wcount[s] = 0;
-    for (uint32_t i = 0; i < m; i++) L.wcount[L.wts[i]]++;
-    unsigned wms, wcard;
-    int wbk = ozl_decide_from_hist(L.wcount, m, 0, &wms, &wcard);
+    // ozl_decide_from_hist over the 13 bins weights can fill
+    unsigned wms = 0, wcard = 0;
+    for (unsigned v = 0; v <= 12; v++)
+        if (L.wcount[v]) { wms = v; wcard++; }
+    int wbk = m <= 1 ? OZL_BK_STORE : wcard == 1 ? OZL_BK_CONSTANT
+                                               : ozl_decide(ozl_entropy(L.wcount, wms, m), wms, wcard, m, 0);
     p.wbk = (uint8_t)wbk;
     p.wMaxSym = (uint8_t)wms;
     uint8_t* side = aux + OZL_AUX_W_SIDE;
@@ -195,6 +198,15 @@ __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, co
             ms = o > ms ? o : ms;
         }
     }
+    if (HUF) {
+        // the Huffman build's node table and buckets and the weights'
+        // histogram start at zero: cleared by the wave
+        uint32_t* tb = reinterpret_cast<uint32_t*>(L.hs.tbl);
+        for (int i = threadIdx.x; i < 2 * (2 * 256 + 2); i += 64) tb[i] = 0;
+        for (int i = threadIdx.x; i < 192; i += 64) L.hs.base[i] = 0;
+        for (int i = threadIdx.x; i < 256; i += 64) L.wcount[i] = 0;
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
         // the plan is built in place in LDS (a local copy handed to
         // plan_weights by reference lived in scratch memory)
@@ -240,22 +252,84 @@ __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, co
         }
         case OZL_BK_HUF: {
             unsigned tl = ozl_fse_optimal_tablelog(OZL_HUF_DEFAULT_TABLELOG, n, maxSym, 1);
-            int mx = ozl_huf_build(L.hlen, L.hval, L.count, maxSym, tl, &L.hs);
+            // code lengths of the sorted leaves; the wave derives the codes
+            int mx = ozl_huf_build(L.hlen, L.hval, L.count, maxSym, tl, &L.hs, true, true);
             if (mx < 0) { p.status = OZL_ERR_GENERIC; break; }
             p.tl = (uint8_t)mx;
             p.nbStates = n > 1000 ? 1 : 0;
-            for (unsigned s = 0; s <= maxSym; s++) {
-                uint8_t w = L.hlen[s] ? (uint8_t)(mx + 1 - L.hlen[s]) : 0;
-                L.wts[s] = w;
-                aux[s] = w;
-            }
-            plan_weights(p, L, aux, ws.minSS);
-            break;
+            break;   // canonical codes, weights and their nested graph follow
         }
         default: break;
         }
     }
     __syncthreads();
+    if (HUF && L.plan.bk == OZL_BK_HUF && L.plan.status == 0) {
+        // canonical codes (huf_compress.c:706-729) by the wave: leaves per
+        // length, the first value of each length, every symbol's length, and
+        // its value = the first value of its length + its rank among the
+        // symbols of that length in symbol order
+        const ozl_hnode* leaf = L.hs.tbl + 1;
+        const int nonNull = L.hs.nonNull;
+        const uint32_t largest = L.plan.tl, m = L.plan.maxSym + 1u;
+        const uint32_t lane = threadIdx.x;
+        const uint64_t below = (1ull << lane) - 1ull;
+        uint32_t perRank[13];
+        for (uint32_t r = 0; r <= 12; r++) perRank[r] = 0;
+        for (int k0 = 0; k0 <= nonNull; k0 += 64) {
+            const int k = k0 + (int)lane;
+            const uint32_t len = k <= nonNull ? leaf[k].nbBits : 0u;
+            for (uint32_t r = 1; r <= largest; r++) perRank[r] += (uint32_t)__popcll(__ballot(len == r));
+        }
+        uint32_t first[13];
+        {
+            uint16_t mn = 0;
+            for (uint32_t r = 0; r <= 12; r++) first[r] = 0;
+            for (int r = (int)largest; r > 0; r--) {
+                first[r] = mn;
+                mn = (uint16_t)(mn + (uint16_t)perRank[r]);
+                mn >>= 1;
+            }
+        }
+        for (uint32_t k = lane; k < m; k += 64) L.hlen[leaf[k].byte] = leaf[k].nbBits;
+        __syncthreads();
+        uint32_t runs[13];
+        for (uint32_t r = 0; r <= 12; r++) runs[r] = 0;
+        for (uint32_t s0 = 0; s0 < m; s0 += 64) {
+            const uint32_t sy = s0 + lane;
+            const uint32_t len = sy < m ? L.hlen[sy] : 0u;
+            uint32_t v = 0;
+            for (uint32_t r = 1; r <= largest; r++) {
+                const uint64_t mk = __ballot(len == r);
+                if (len == r) v = (uint16_t)(first[r] + runs[r] + (uint32_t)__popcll(mk & below));
+                runs[r] += (uint32_t)__popcll(mk);
+            }
+            if (sy < m) {
+                L.hval[sy] = (uint16_t)v;
+                const uint8_t w = len ? (uint8_t)(largest + 1 - len) : 0;
+                L.wts[sy] = w;
+                L.aux[sy] = w;
+            }
+        }
+        __syncthreads();
+        // histogram of the weights (values 0..12: code lengths are at most
+        // 11) by ballots of the wave
+        uint32_t seen = 0;
+        for (uint32_t v = 0; v <= 12; v++) {
+            uint32_t c = 0;
+            for (uint32_t i0 = 0; i0 < m; i0 += 64) {
+                const uint32_t i = i0 + threadIdx.x;
+                c += (uint32_t)__popcll(__ballot(i < m && L.wts[i] == v));
+            }
+            seen += c;
+            if (threadIdx.x == 0) L.wcount[v] = c;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (seen != m) L.plan.status = OZL_ERR_GENERIC;
+            else plan_weights(L.plan, L, L.aux, ws.minSS);
+        }
+        __syncthreads();
+    }
     const ozl_eplan& p = L.plan;
     if (p.bk == OZL_BK_FSE && p.status == 0) {
         ozl_wave::wave_fse_ctable(reinterpret_cast<uint16_t*>(tab), reinterpret_cast<uint32_t*>(tab + 8192), L.norm,
