@@ -84,8 +84,14 @@ struct PlanLds {
     uint8_t aux[OZL_AUX_BYTES];
     uint16_t cumul[18];   // nested table build workspace (13 weight symbols)
     uint8_t fmap[256];
+    uint32_t wWave;       // the weights' FSE bitstream is left to the wave
     ozl_eplan plan;
 };
+
+// The weights' nested FSE graph keeps tableLog 5 (<= 256 weights of value
+// <= 12, fse_compress.c:379-396); the wave encoder's transition table covers
+// that size.
+#define OZL_WEIGHTS_WAVE_TABLELOG 5
 
 // The table area of a block holds its histogram (u32[256]) at this offset
 // between k_hist and k_plan; the FSE tables end below it.
@@ -130,9 +136,13 @@ __device__ __forceinline__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* 
         if (tl > OZL_WEIGHTS_MAX_TABLELOG || wms > 12) { p.status = OZL_ERR_GENERIC; return; }
         if (ozl_fse_normalize(wn, tl, L.wcount, m, wms) != 0) { p.status = OZL_ERR_GENERIC; return; }
         if (ozl_fse_build_ctable(L.wst, L.wtt, L.cells, wn, wms, tl, L.cumul)) { p.status = OZL_ERR_GENERIC; return; }
-        size_t bits = ozl_fse_encode_seq(pay, OZL_AUX_BYTES - OZL_AUX_W_PAYLOAD, L.wts, m, L.wst, L.wtt, tl, 2);
-        if (bits == 0) { p.status = OZL_ERR_GENERIC; return; }
-        p.wPayload = (uint16_t)bits;
+        if (tl <= OZL_WEIGHTS_WAVE_TABLELOG && m > 2) {
+            L.wWave = 1;   // wave_weights_fse writes the payload
+        } else {
+            size_t bits = ozl_fse_encode_seq(pay, OZL_AUX_BYTES - OZL_AUX_W_PAYLOAD, L.wts, m, L.wst, L.wtt, tl, 2);
+            if (bits == 0) { p.status = OZL_ERR_GENERIC; return; }
+            p.wPayload = (uint16_t)bits;
+        }
         if (ozl_small(2u * (wms + 1u), minSS)) {
             for (unsigned s = 0; s <= wms; s++) { side[2 * s] = (uint8_t)wn[s]; side[2 * s + 1] = (uint8_t)((uint16_t)wn[s] >> 8); }
             p.wAux = (uint16_t)(2 * (wms + 1));
@@ -143,6 +153,95 @@ __device__ __forceinline__ void plan_weights(ozl_eplan& p, PlanLds& L, uint8_t* 
     }
     default: break;
     }
+}
+
+// The weights' FSE bitstream by the wave: the bytes ozl_fse_encode_seq
+// writes for m > 2 weights with two states (fse_compress.c:568-627 over
+// the weights stream). Position p is encoded by the state of positions of its
+// parity, from the last position down; every emission (the state's low bits
+// before its transition) has a fixed place in the stream: main emissions for
+// p = m-3 .. 0, the final states of positions 1 and 0 (tableLog bits each),
+// then a 1 bit. Two lanes run the two state chains over a transition table
+// of (state, weight) pairs; the wave adds up the bit offsets and places the
+// bits. Needs tl <= OZL_WEIGHTS_WAVE_TABLELOG; the table, emissions and
+// words reuse the Huffman build's node table, which is dead by then.
+__device__ __forceinline__ void wave_weights_fse(PlanLds& L, uint8_t* pay)
+{
+    const uint32_t lane = threadIdx.x;
+    const uint32_t m = L.plan.maxSym + 1u, tl = L.plan.wTl, wms = L.plan.wMaxSym;
+    const uint32_t ts = 1u << tl;
+    uint16_t* ct = reinterpret_cast<uint16_t*>(L.hs.tbl);   // [state - ts][weight]: next | nbBits << 8
+    uint16_t* em = ct + 16 * (1u << OZL_WEIGHTS_WAVE_TABLELOG);   // [m + 1]: value | nbBits << 8
+    uint32_t* words = reinterpret_cast<uint32_t*>(em + 264);      // the stream, 32 bits a word
+    static_assert(sizeof(uint16_t) * (16 * 32 + 264) + 4 * 48 <= sizeof(ozl_huf_scratch::tbl), "weights area");
+    for (uint32_t i = lane; i < 16 * ts; i += 64) {
+        const uint32_t u = i >> 4, w = i & 15u;
+        uint16_t e = 0;
+        if (w <= wms && L.wn[w] != 0) {
+            const uint32_t t = L.wtt[w], x = ts + u;
+            const uint32_t nb = (x + ozl_tt_dnb(t)) >> 16;
+            e = (uint16_t)(L.wst[(int)(x >> nb) + ozl_tt_dfs(t)] | (nb << 8));
+        }
+        ct[i] = e;
+    }
+    for (uint32_t i = lane; i < 48; i += 64) words[i] = 0;
+    __syncthreads();
+    if (lane < 2) {
+        // lane c: the state started on position m-1-c
+        int p = (int)m - 1 - (int)lane;
+        const uint32_t t = L.wtt[L.wts[p]];
+        const uint32_t dnb = ozl_tt_dnb(t);
+        const uint32_t nb0 = (dnb + (1u << 15)) >> 16;
+        const uint32_t v = (nb0 << 16) - dnb;
+        uint32_t x = L.wst[(int)(v >> nb0) + ozl_tt_dfs(t)];
+        p -= 2;
+        uint32_t w = p >= 0 ? L.wts[p] : 0u;
+        for (; p >= 0; p -= 2) {
+            const uint32_t wNext = p >= 2 ? L.wts[p - 2] : 0u;
+            const uint32_t e = ct[((x - ts) << 4) + w];
+            const uint32_t nb = e >> 8;
+            em[m - 3 - (uint32_t)p] = (uint16_t)(__builtin_amdgcn_ubfe(x, 0, nb) | (nb << 8));
+            x = e & 255u;
+            w = wNext;
+        }
+        const uint32_t odd = (m - 1u - lane) & 1u;
+        em[odd ? m - 2 : m - 1] = (uint16_t)((x & (ts - 1u)) | (tl << 8));
+        if (lane == 0) em[m] = (uint16_t)(1u | (1u << 8));
+    }
+    __syncthreads();
+    // five emissions a lane (m + 1 <= 257), offsets by a wave scan
+    uint32_t loc[5], nbs = 0;
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const uint32_t e = lane * 5u + (uint32_t)k;
+        loc[k] = e <= m ? em[e] : 0u;
+        nbs += loc[k] >> 8;
+    }
+    uint32_t inc = nbs;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d, 64);
+        if ((int)lane >= d) inc += y;
+    }
+    uint32_t P = inc - nbs;
+    const uint32_t totalBits = __shfl(inc, 63, 64);
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const uint32_t nb = loc[k] >> 8, val = loc[k] & 255u, sh = P & 31u;
+        if (nb) {
+            atomicOr(&words[P >> 5], val << sh);
+            if (sh + nb > 32) atomicOr(&words[(P >> 5) + 1], val >> (32 - sh));
+        }
+        P += nb;
+    }
+    __syncthreads();
+    const uint32_t bytes = (totalBits + 7) >> 3;
+    if (bytes + 8 > OZL_AUX_BYTES - OZL_AUX_W_PAYLOAD) {
+        if (lane == 0) L.plan.status = OZL_ERR_GENERIC;
+        return;
+    }
+    for (uint32_t i = lane; i < bytes; i += 64) pay[i] = (uint8_t)(words[i >> 2] >> (8 * (i & 3)));
+    if (lane == 0) L.plan.wPayload = (uint16_t)bytes;
 }
 
 // Histogram of every block (one 256-thread workgroup per block, LDS
@@ -325,10 +424,15 @@ __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, co
         }
         __syncthreads();
         if (threadIdx.x == 0) {
+            L.wWave = 0;
             if (seen != m) L.plan.status = OZL_ERR_GENERIC;
             else plan_weights(L.plan, L, L.aux, ws.minSS);
         }
         __syncthreads();
+        if (L.wWave && L.plan.status == 0) {
+            wave_weights_fse(L, L.aux + OZL_AUX_W_PAYLOAD);
+            __syncthreads();
+        }
     }
     const ozl_eplan& p = L.plan;
     if (p.bk == OZL_BK_FSE && p.status == 0) {
