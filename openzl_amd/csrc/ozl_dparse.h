@@ -40,15 +40,115 @@ struct ozl_dplan {
     uint64_t outPtr;           // decoded output (absolute device address)
 };
 
+// Bytes of a frame kept in the parser's copy (pn <= OZL_DPREFIX).
+#define OZL_DPREFIX 1024
+
 // A frame seen through a copy of its first `pn` bytes (in LDS on the device):
-// reads inside the copy use it, the rest go to the frame itself.
+// reads inside the copy use it, the rest go to the frame itself. When `w` is
+// set, the copy is 4-byte aligned with OZL_DPREFIX + 8 readable bytes, and
+// the bit streams that lie inside it are read 32 bits at a time.
 struct ozl_fview {
     const uint8_t* g;  // the frame
     const uint8_t* p;  // copy of its first pn bytes
     size_t pn;
+    const uint32_t* w = nullptr;   // p as words
     OZL_HDM uint8_t operator[](size_t i) const { return i < pn ? p[i] : g[i]; }
     OZL_HDM const uint8_t* at(size_t off, size_t len) const { return off + len <= pn ? p + off : g + off; }
+    OZL_HDM bool words(size_t off, size_t len) const { return w && off + len <= pn; }
+    // the 32 bits of the copy from bit b on (LSB-first); the word index wraps
+    // inside the copy, so reads past a stream's end stay in bounds
+    OZL_HDM uint32_t bits32(size_t b) const
+    {
+        const uint32_t i = (uint32_t)(b >> 5) & (OZL_DPREFIX / 4 - 1);
+        const uint64_t x = (uint64_t)w[i] | ((uint64_t)w[i + 1] << 32);
+        return (uint32_t)(x >> (b & 31));
+    }
 };
+
+// ozl_fse_read_ncount (ozl_core.h) for an NCount lying inside the copy:
+// the same fields and checks, each field read from a 32-bit window (the
+// reference reads past the end as zeros, here as the frame's next bytes;
+// either way a read past the end fails the final check). norm must hold
+// zeros on entry.
+OZL_HDN long ozl_read_ncount_w(int16_t* norm, unsigned* maxSymOut, unsigned* tlOut, const ozl_fview& fr, size_t off,
+                               size_t size)
+{
+    const size_t b0 = off * 8, total = size * 8;
+    size_t pos = 0;
+    const unsigned tl = (fr.bits32(b0) & 15u) + OZL_FSE_MIN_TABLELOG;
+    pos = 4;
+    if (tl > OZL_FSE_MAX_TABLELOG) return -1;
+    int remaining = (1 << tl) + 1, threshold = 1 << tl, nb = (int)tl + 1;
+    unsigned symbol = 0;
+    bool prev0 = false;
+    while (remaining > 1 && symbol <= 255) {
+        if (prev0) {
+            unsigned n0 = symbol;
+            for (;;) {
+                const uint32_t r = fr.bits32(b0 + pos) & 3u;
+                pos += 2;
+                n0 += r;
+                if (r != 3) break;
+                if (n0 > 255) return -1;
+            }
+            if (n0 > 255) return -1;
+            symbol = n0;
+        }
+        // nb - 1 low bits, and the next bit when they reach max
+        const int max = (2 * threshold - 1) - remaining;
+        const uint32_t win = fr.bits32(b0 + pos);
+        int count = (int)(win & ((1u << (nb - 1)) - 1u));
+        if (count < max) pos += (size_t)(nb - 1);
+        else {
+            count = (int)(win & ((1u << nb) - 1u));
+            if (count >= threshold) count -= max;
+            pos += (size_t)nb;
+        }
+        count--;
+        remaining -= count < 0 ? -count : count;
+        norm[symbol++] = (int16_t)count;
+        prev0 = count == 0;
+        while (remaining < threshold) { nb--; threshold >>= 1; }
+    }
+    if (remaining != 1 || pos > total || symbol == 0) return -1;
+    *maxSymOut = symbol - 1;
+    *tlOut = tl;
+    return (long)((pos + 7) / 8);
+}
+
+// ozl_fse_decode_seq (ozl_core.h) for a stream lying inside the copy, with
+// NS interleaved states: bits [pos - nb, pos) read as one field.
+template <int NS>
+OZL_HDN int ozl_fse_decode_w(uint8_t* dst, size_t n, const ozl_fview& fr, size_t off, size_t size, const uint32_t* dt,
+                             unsigned tl)
+{
+    if (size == 0) return -1;
+    const uint32_t last = fr[off + size - 1];
+    if (last == 0) return -1;
+    const size_t b0 = off * 8;
+    long pos = (long)(size - 1) * 8 + ozl_hb32(last);
+    auto rdb = [&](unsigned nb) -> uint32_t {
+        pos -= (long)nb;
+        return pos < 0 ? 0u : fr.bits32(b0 + (size_t)pos) & ((1u << nb) - 1u);
+    };
+    // state p mod NS of symbol p sits in x0 when p is decoded
+    uint32_t x0 = rdb(tl), x1 = NS > 1 ? rdb(tl) : 0u, x2 = NS > 2 ? rdb(tl) : 0u, x3 = NS > 3 ? rdb(tl) : 0u;
+    if (pos < 0) return -1;
+    for (size_t p = 0; p < n; p++) {
+        const uint32_t e = dt[x0];
+        dst[p] = (uint8_t)e;
+        if (p + NS < n) {
+            const uint32_t r = rdb((e >> 8) & 255);
+            if (pos < 0) return -1;
+            x0 = (e >> 16) + r;
+        }
+        const uint32_t t = x0;
+        if (NS == 2) { x0 = x1; x1 = t; }
+        else if (NS == 3) { x0 = x1; x1 = x2; x2 = t; }
+        else if (NS == 4) { x0 = x1; x1 = x2; x2 = x3; x3 = t; }
+    }
+    return pos == 0 ? 0 : -1;
+}
 
 struct ozl_reader {
     ozl_fview p;
@@ -112,9 +212,26 @@ struct ozl_dlds {
     uint8_t expect[512];
 };
 
+// An NCount read while the header was parsed (kept so that the tables
+// pass does not read it again).
+struct ozl_ncread {
+    size_t off, size;
+    long used;
+    unsigned ms, tl;
+};
+
+// NCount of `size` bytes at `off` into norm: through the copy's words when it
+// lies there (norm then holds zeros on entry), else byte by byte.
+OZL_HDN long ozl_dread_ncount(int16_t* norm, unsigned* ms, unsigned* tl, const ozl_fview& fr, size_t off, size_t size,
+                              bool normZero)
+{
+    if (normZero && fr.words(off, size)) return ozl_read_ncount_w(norm, ms, tl, fr, off, size);
+    return ozl_fse_read_ncount(norm, ms, tl, fr.at(off, size), size);
+}
+
 // Decodes the Huffman weights stream described by the recognised layout.
 OZL_HDN int ozl_decode_weights(const ozl_fview& fr, const ozl_fnode* nodes, int nn, const ozl_fstream* st,
-                                const uint32_t* soff, uint32_t m, ozl_dlds& L)
+                                const uint32_t* soff, uint32_t m, ozl_dlds& L, const ozl_ncread* nc)
 {
     // stream ids follow ozl_entropy_layout: 1 weights, 3 serial weights, 4.. nested outputs
     if (nn == 2) {  // conversion then STORE
@@ -163,7 +280,11 @@ OZL_HDN int ozl_decode_weights(const ozl_fview& fr, const ozl_fnode* nodes, int 
         unsigned ms, tl;
         int16_t* wn = L.norm;  // free in Huffman frames
         if (nn == 4 && nodes[3].trid == OZL_TR_FSE_NCOUNT) {  // NCount encoded
-            if (ozl_fse_read_ncount(wn, &ms, &tl, fr.at(soff[6], st[6].size), st[6].size) < 0) return -1;
+            if (nc && nc->off == soff[6] && nc->size == st[6].size) {   // read by the header pass
+                if (nc->used < 0) return -1;
+                ms = nc->ms;
+                tl = nc->tl;
+            } else if (ozl_fse_read_ncount(wn, &ms, &tl, fr.at(soff[6], st[6].size), st[6].size) < 0) return -1;
         } else {        // raw int16 counts, stored after a num->serial conversion
             uint32_t cnt = st[6].size / 2;
             if (cnt == 0) return -1;
@@ -180,7 +301,15 @@ OZL_HDN int ozl_decode_weights(const ozl_fview& fr, const ozl_fnode* nodes, int 
         if (tl < OZL_FSE_MIN_TABLELOG || tl > OZL_FSE_MAX_TABLELOG) return -1;
         if (tl > OZL_WEIGHTS_MAX_TABLELOG) return -1;
         if (ozl_fse_build_dtable(L.wdt, L.cells, wn, ms, tl, L.x.next)) return -1;
-        return ozl_fse_decode_seq(L.w, m, fr.at(soff[5], st[5].size), st[5].size, L.wdt, tl, nodes[2].hdr[0]);
+        const unsigned ns = nodes[2].hdr[0];
+        if (fr.words(soff[5], st[5].size) && ns >= 1 && ns <= 4) {
+            const size_t o = soff[5], z = st[5].size;
+            return ns == 1 ? ozl_fse_decode_w<1>(L.w, m, fr, o, z, L.wdt, tl)
+                 : ns == 2 ? ozl_fse_decode_w<2>(L.w, m, fr, o, z, L.wdt, tl)
+                 : ns == 3 ? ozl_fse_decode_w<3>(L.w, m, fr, o, z, L.wdt, tl)
+                           : ozl_fse_decode_w<4>(L.w, m, fr, o, z, L.wdt, tl);
+        }
+        return ozl_fse_decode_seq(L.w, m, fr.at(soff[5], st[5].size), st[5].size, L.wdt, tl, ns);
     }
     return -1;
 }
@@ -189,7 +318,11 @@ OZL_HDN int ozl_decode_weights(const ozl_fview& fr, const ozl_fnode* nodes, int 
 // holds the normalized counts (d.aux = maxSymbol, d.tl = tableLog); for
 // Huffman, L.wlen holds the code lengths (d.tl = maximum length). The tables
 // themselves are built by the decoding kernels.
-OZL_HDN void ozl_dparse_frame(const ozl_fview& fr, size_t flen, ozl_dlds& L, ozl_dplan& d)
+// `normZero`: L.norm holds zeros on entry. `weightsOut`: when set, a Huffman
+// frame stops once L.w holds its weights, their number goes to *weightsOut and
+// the caller derives the code lengths (ozl_huf_from_weights) and d.tl.
+OZL_HDN void ozl_dparse_frame(const ozl_fview& fr, size_t flen, ozl_dlds& L, ozl_dplan& d, bool normZero = false,
+                              uint32_t* weightsOut = nullptr)
 {
     d = ozl_dplan{};
     d.status = OZL_ERR_CORRUPTION;
@@ -350,6 +483,8 @@ OZL_HDN void ozl_dparse_frame(const ozl_fview& fr, size_t flen, ozl_dlds& L, ozl
     }
     // stored stream positions: header stream first, then stored streams in list order
     size_t dataPos = hsPos + hsLen;
+    ozl_ncread nc = {};
+    bool haveNc = false;
     if (ok) {
         // flatpack weights: recover m from the terminating bit now that positions are known
         if (p.bk == OZL_BK_HUF && p.wbk == OZL_BK_FLATPACK) {
@@ -364,18 +499,15 @@ OZL_HDN void ozl_dparse_frame(const ozl_fview& fr, size_t flen, ozl_dlds& L, ozl
         bool mainNc = p.bk == OZL_BK_FSE && nodes[1].trid == OZL_TR_FSE_NCOUNT;
         bool nestedNc = p.bk == OZL_BK_HUF && p.wbk == OZL_BK_FSE && nn == 4 && nodes[3].trid == OZL_TR_FSE_NCOUNT;
         if ((mainNc || nestedNc) && dataPos + storedSize[0] > flen) ok = false;
-        if (ok && mainNc) {
-            // maxSym from the NCount itself
-            unsigned ms, tl;
-            long used = ozl_fse_read_ncount(L.norm, &ms, &tl, fr.at(dataPos, storedSize[0]), storedSize[0]);
-            if (used < 0) ok = false;
-            else p.maxSym = (uint8_t)ms;
-        }
-        if (ok && nestedNc) {
-            unsigned ms, tl;
-            long used = ozl_fse_read_ncount(L.norm, &ms, &tl, fr.at(dataPos, storedSize[0]), storedSize[0]);
-            if (used < 0) ok = false;
-            else p.wMaxSym = (uint8_t)ms;
+        if (ok && (mainNc || nestedNc)) {
+            // maxSym from the NCount itself; the counts stay for the tables
+            nc.off = dataPos;
+            nc.size = storedSize[0];
+            nc.used = ozl_dread_ncount(L.norm, &nc.ms, &nc.tl, fr, dataPos, storedSize[0], normZero);
+            haveNc = true;
+            if (nc.used < 0) ok = false;
+            else if (mainNc) p.maxSym = (uint8_t)nc.ms;
+            else p.wMaxSym = (uint8_t)nc.ms;
         }
     }
     int* order = L.x.order;
@@ -417,7 +549,13 @@ OZL_HDN void ozl_dparse_frame(const ozl_fview& fr, size_t flen, ozl_dlds& L, ozl
             d.payOff = soff[2]; d.payLen = st[2].size; d.nbStates = p.nbStates;
             unsigned ms, tl;
             if (nodes[1].trid == OZL_TR_FSE_NCOUNT) {
-                if (ozl_fse_read_ncount(L.norm, &ms, &tl, fr.at(soff[3], st[3].size), st[3].size) < 0) { d.status = OZL_ERR_CORRUPTION; break; }
+                if (haveNc && nc.off == soff[3] && nc.size == st[3].size) {
+                    ms = nc.ms;   // read by the header pass (which succeeded)
+                    tl = nc.tl;
+                } else if (ozl_fse_read_ncount(L.norm, &ms, &tl, fr.at(soff[3], st[3].size), st[3].size) < 0) {
+                    d.status = OZL_ERR_CORRUPTION;
+                    break;
+                }
             } else {  // raw int16 counts after a num->serial conversion
                 uint32_t cnt = st[3].size / 2, sum = 0;
                 for (int s = 0; s < 256; s++) L.norm[s] = 0;
@@ -439,7 +577,8 @@ OZL_HDN void ozl_dparse_frame(const ozl_fview& fr, size_t flen, ozl_dlds& L, ozl
             d.payOff = soff[2]; d.payLen = st[2].size; d.nbStates = p.nbStates;
             if (d.nbStates > 1 || (d.nbStates == 1 && d.n < 12)) { d.status = OZL_ERR_CORRUPTION; break; }
             uint32_t m = p.maxSym + 1u;
-            if (ozl_decode_weights(fr, nodes, nn, st, soff, m, L)) { d.status = OZL_ERR_CORRUPTION; break; }
+            if (ozl_decode_weights(fr, nodes, nn, st, soff, m, L, haveNc ? &nc : nullptr)) { d.status = OZL_ERR_CORRUPTION; break; }
+            if (weightsOut) { *weightsOut = m; break; }
             int tl = ozl_huf_from_weights(L.wlen, L.wval, L.w, m, L.x.rk);
             if (tl < 1 || tl > 11) { d.status = OZL_ERR_CORRUPTION; break; }
             d.tl = (uint8_t)tl;

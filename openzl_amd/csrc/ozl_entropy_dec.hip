@@ -47,37 +47,83 @@ using namespace ozl_wave;
 
 constexpr int kT = 256;
 constexpr int kParseT = 64;
-constexpr int kPrefix = 1024;
+constexpr int kPrefix = OZL_DPREFIX;
 
 __global__ void __launch_bounds__(kParseT) k_dparse(const uint8_t* __restrict__ frames, const uint64_t* __restrict__ foff,
                                                     uint32_t B, uint8_t* __restrict__ dst, const uint64_t* __restrict__ doff,
                                                     ozl_dws ws)
 {
     __shared__ ozl_dlds L;
-    __shared__ uint8_t pre[kPrefix];
-    __shared__ int sBk;
+    __shared__ __attribute__((aligned(16))) uint8_t pre[kPrefix + 16];
+    __shared__ ozl_dplan sD;
+    __shared__ uint32_t sM;
     uint32_t b = blockIdx.x;
     if (b >= B) return;
     const uint8_t* fr = frames + foff[b];
     size_t flen = foff[b + 1] - foff[b];
     size_t pn = flen < (size_t)kPrefix ? flen : (size_t)kPrefix;
-    for (size_t i = threadIdx.x; i < pn; i += kParseT) pre[i] = fr[i];
+    // the frame's first KiB as words (zeros past the frame), zero counts
+    {
+        uint32_t* pw = reinterpret_cast<uint32_t*>(pre);
+        for (uint32_t i = threadIdx.x; i < (kPrefix + 16) / 4; i += kParseT) {
+            uint32_t x = 0;
+            for (uint32_t k = 0; k < 4; k++) {
+                const size_t j = 4 * (size_t)i + k;
+                if (j < pn) x |= (uint32_t)fr[j] << (8 * k);
+            }
+            pw[i] = x;
+        }
+        for (int s = threadIdx.x; s < 256; s += kParseT) L.norm[s] = 0;
+    }
+    if (threadIdx.x == 0) sM = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
         ozl_dplan d;
-        ozl_fview v = { fr, pre, pn };
-        ozl_dparse_frame(v, flen, L, d);
+        ozl_fview v = { fr, pre, pn, reinterpret_cast<const uint32_t*>(pre) };
+        ozl_dparse_frame(v, flen, L, d, true, &sM);
         d.srcPtr = (uint64_t)(uintptr_t)(fr + d.payOff);
         d.outPtr = (uint64_t)(uintptr_t)(dst + doff[b]);
         if (d.status == 0 && doff[b + 1] - doff[b] != d.n) d.status = OZL_ERR_DSTCAPACITY_TOOSMALL;
-        ws.plans[b] = d;
-        sBk = d.status == 0 ? d.bk : 255;
+        sD = d;
     }
     __syncthreads();
+    if (sD.status == 0 && sD.bk == OZL_BK_HUF) {
+        // code lengths from the m weights (ozl_huf_from_weights): a weight
+        // above 12 or a sum of 2^(w-1) that is not a power of two is corrupt
+        const uint32_t m = sM;
+        uint32_t tot = 0, bad = 0;
+        uint8_t wl[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t sy = threadIdx.x + 64u * k;
+            const uint32_t w = sy < m ? L.w[sy] : 0u;
+            wl[k] = (uint8_t)w;
+            bad |= w > 12 ? 1u : 0u;
+            tot += w && w <= 12 ? 1u << (w - 1) : 0u;
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            tot += __shfl_xor(tot, d, 64);
+            bad |= __shfl_xor(bad, d, 64);
+        }
+        const uint32_t tl = tot ? (uint32_t)ozl_hb32(tot) : 0u;
+        const bool okW = !bad && tot && (tot & (tot - 1)) == 0 && tl >= 1 && tl <= 11;
+        if (okW) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) L.wlen[threadIdx.x + 64 * k] = wl[k] ? (uint8_t)(tl + 1 - wl[k]) : 0;
+        }
+        if (threadIdx.x == 0) {
+            if (okW) sD.tl = (uint8_t)tl;
+            else sD.status = OZL_ERR_CORRUPTION;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ws.plans[b] = sD;
+    const int bk = sD.status == 0 ? sD.bk : 255;
     uint8_t* side = ws.side + (size_t)b * OZL_DSIDE_BYTES;
-    if (sBk == OZL_BK_FSE)
+    if (bk == OZL_BK_FSE)
         for (int s = threadIdx.x; s < 256; s += kParseT) reinterpret_cast<int16_t*>(side)[s] = L.norm[s];
-    else if (sBk == OZL_BK_HUF)
+    else if (bk == OZL_BK_HUF)
         for (int s = threadIdx.x; s < 256; s += kParseT) side[s] = L.wlen[s];
 }
 

@@ -71,12 +71,26 @@ size_t cu_flatpack(uint8_t* alphabet, uint8_t* packed, size_t* packedSize, const
 // Frame recognition of the batched entropy decoder (ozl_dparse.h): returns
 // the plan's status and fills `plan` (the ozl_dplan bytes) and `side` (FSE
 // normalized counts int16[256] or Huffman code lengths u8[256]).
-int cu_dparse(const uint8_t* fr, size_t flen, void* plan, uint8_t* side)
+// words = 1: the frame's first OZL_DPREFIX bytes are seen as aligned words
+// (zeros past the frame) and the NCount / nested FSE streams inside them are
+// read 32 bits at a time, as on the device; 0: byte by byte.
+int cu_dparse(const uint8_t* fr, size_t flen, void* plan, uint8_t* side, int words)
 {
     static ozl_dlds L;
+    alignas(16) static uint8_t pre[OZL_DPREFIX + 16];
     ozl_dplan d;
-    ozl_fview v = { fr, fr, flen };
-    ozl_dparse_frame(v, flen, L, d);
+    memset(&L, 0x5A, sizeof(L));
+    if (words) {
+        size_t pn = flen < OZL_DPREFIX ? flen : OZL_DPREFIX;
+        memset(pre, 0, sizeof(pre));
+        memcpy(pre, fr, pn);
+        for (int s = 0; s < 256; s++) L.norm[s] = 0;
+        ozl_fview v = { fr, pre, pn, reinterpret_cast<const uint32_t*>(pre) };
+        ozl_dparse_frame(v, flen, L, d, true);
+    } else {
+        ozl_fview v = { fr, fr, flen };
+        ozl_dparse_frame(v, flen, L, d);
+    }
     memcpy(plan, &d, sizeof(d));
     if (d.status == 0 && d.bk == OZL_BK_FSE) memcpy(side, L.norm, 512);
     if (d.status == 0 && d.bk == OZL_BK_HUF) memcpy(side, L.wlen, 256);

@@ -148,7 +148,7 @@ def test_dparse_recognises_golden_frames(cu):
     import golden_cases as GC
     P, SZ, U = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint
     cu.cu_dparse.restype = ctypes.c_int
-    cu.cu_dparse.argtypes = [P, SZ, P, P]
+    cu.cu_dparse.argtypes = [P, SZ, P, P, ctypes.c_int]
     cu.cu_dplan_size.restype = SZ
     import struct
     seen = {"fse": 0, "huf": 0}
@@ -160,7 +160,7 @@ def test_dparse_recognises_golden_frames(cu):
         fr = np.frombuffer(GC.small_frame(e["name"]), np.uint8)
         plan = np.zeros(cu.cu_dplan_size(), np.uint8)
         side = np.zeros(512, np.uint8)
-        st = cu.cu_dparse(ptr(fr), fr.size, ptr(plan), ptr(side))
+        st = cu.cu_dparse(ptr(fr), fr.size, ptr(plan), ptr(side), 1)
         assert st == 0, e["name"]
         n, bk, tl, nbs, _ = struct.unpack_from("<IBBBB", plan, 0)
         payOff, payLen, _, _, aux = struct.unpack_from("<5I", plan, 8)
@@ -179,3 +179,58 @@ def test_dparse_recognises_golden_frames(cu):
             assert kraft == 2 ** tl
             seen["huf"] += 1
     assert seen["fse"] >= 5 and seen["huf"] >= 5
+
+
+def _dparse(cu, fr, words):
+    P, SZ = ctypes.c_void_p, ctypes.c_size_t
+    cu.cu_dparse.restype = ctypes.c_int
+    cu.cu_dparse.argtypes = [P, SZ, P, P, ctypes.c_int]
+    cu.cu_dplan_size.restype = SZ
+    plan = np.zeros(cu.cu_dplan_size(), np.uint8)
+    side = np.zeros(512, np.uint8)
+    st = cu.cu_dparse(ptr(fr), fr.size, ptr(plan), ptr(side), words)
+    return st, plan, side
+
+
+def test_dparse_word_readers_match_byte_readers(cu):
+    """The parser's 32-bit-window readers (NCount, nested weights FSE stream:
+    ozl_read_ncount_w / ozl_fse_decode_w, used on the device) against its
+    byte-by-byte readers (ozl_fse_read_ncount / ozl_fse_decode_seq, pinned to
+    the reference) on the reference's FSE / Huffman frames and on copies with
+    single bit flips, truncations and byte swaps in the header, NCount and
+    weights area: the same status, plan and tables every time."""
+    import golden_cases as GC
+    rng = np.random.default_rng(21)
+    frames = []
+    for e in GC.small_entries():
+        if e["kind"] not in (1, 2) or e["type"] != 0 or e["input_bytes"] >= 100000 or e.get("params"):
+            continue
+        frames.append(np.frombuffer(GC.small_frame(e["name"]), np.uint8).copy())
+    assert len(frames) >= 10
+    n_ok = n_bad = 0
+    for fr in frames:
+        variants = [fr]
+        head = min(fr.size, 400)
+        for _ in range(40):
+            v = fr.copy()
+            i = int(rng.integers(0, head))
+            kind = int(rng.integers(0, 3))
+            if kind == 0:
+                v[i] ^= np.uint8(1 << int(rng.integers(0, 8)))
+            elif kind == 1:
+                v = v[: max(1, fr.size - int(rng.integers(1, 16)))].copy()
+            else:
+                j = int(rng.integers(0, head))
+                v[i], v[j] = v[j], v[i]
+            variants.append(v)
+        for v in variants:
+            a = _dparse(cu, v, 1)
+            b = _dparse(cu, v, 0)
+            assert a[0] == b[0]
+            assert np.array_equal(a[1], b[1])
+            if a[0] == 0:
+                assert np.array_equal(a[2], b[2])
+                n_ok += 1
+            else:
+                n_bad += 1
+    assert n_ok >= len(frames) and n_bad >= 50
