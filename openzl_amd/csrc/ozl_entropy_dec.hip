@@ -447,8 +447,9 @@ __device__ inline uint32_t fse_quad_round(const uint16_t* DT, const uint8_t* SY,
     const uint32_t e = DT[st], y = SY[st];
     const bool upd = !TAIL || p + 4 < n;
     const uint32_t nb = upd ? e & 15u : 0u;
-    // nb in bytes q..3 (24-bit multiply, full rate), summed over the quad
-    uint32_t v = (__umul24(nb, 0x10101u) | (nb << 24)) << sq;
+    // nb in bytes q..3 (a byte broadcast by v_perm: a multiply here would
+    // be a quarter-rate v_mul_lo_u32 on the chain), summed over the quad
+    uint32_t v = __builtin_amdgcn_perm(0u, nb, 0u) << sq;
     v += quad_xor1(v);
     const uint32_t M = v + quad_xor2(v);          // byte k: bits of states 0..k
     const uint32_t incl = __builtin_amdgcn_ubfe(M, sq, 8);
