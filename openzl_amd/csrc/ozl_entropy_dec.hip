@@ -295,9 +295,10 @@ constexpr int kQLook = 192;     // quads: bytes kept staged below the window
 constexpr int kQPhase = 16;     // quads: rounds between staging steps (<= 88 bytes consumed)
 
 struct FseDecLds {
+    // the rings first: their addresses fit the LDS instructions' offsets
+    uint32_t ring[kFQ][kFRingW + 4];  // dwords kFRingW, kFRingW+1 mirror 0, 1
     uint16_t dt[kFQ][2048];
     uint8_t sym[kFQ][2048];
-    uint32_t ring[kFQ][kFRingW + 4];  // dwords kFRingW, kFRingW+1 mirror 0, 1
     FseBuildHdr bs;
     ozl_dplan plan[kFQ];
 };
@@ -360,6 +361,16 @@ __device__ inline void qstage_step(uint32_t* ring, QStage& g, uint64_t winLo, ui
 {
     qstage_put(ring, g);
     qstage_load(g, winLo, q);
+}
+
+// the 64 stream bits above absolute bit address Qm (mod 2^32): the window of
+// a reader at Qm + 64, with the dword index taken by one bit-field extract
+__device__ inline uint64_t qwindow_m(const uint32_t* ring, uint32_t Qm)
+{
+    static_assert(kFRingW == 64, "ring index: 6 bits");
+    const uint32_t j = __builtin_amdgcn_ubfe(Qm, 5, 6);
+    const uint32_t d0 = ring[j], d1 = ring[j + 1], d2 = ring[j + 2];
+    return ((uint64_t)__builtin_amdgcn_alignbit(d2, d1, Qm) << 32) | __builtin_amdgcn_alignbit(d1, d0, Qm);
 }
 
 // the 64 stream bits below absolute bit address Q (mod 2^32)
@@ -439,10 +450,11 @@ __device__ inline void put4_dwords(uint64_t a, uint32_t stride, uint32_t d0, uin
 }
 
 // one round of a 4-state frame's quad: returns this lane's symbol; TAIL
-// rounds keep the states of the last 4 symbols (fse_decompress.c:287-336)
+// rounds keep the states of the last 4 symbols (fse_decompress.c:287-336).
+// W holds the 64 bits above bit address Qm (the reader sits at Qm + 64).
 template <bool TAIL>
 __device__ inline uint32_t fse_quad_round(const uint16_t* DT, const uint8_t* SY, const uint32_t* ring, uint32_t& st,
-                                          uint64_t& W, uint32_t& Q, uint32_t sq, uint32_t p, uint32_t n)
+                                          uint64_t& W, uint32_t& Qm, uint32_t sq, uint32_t p, uint32_t n)
 {
     const uint32_t e = DT[st], y = SY[st];
     const bool upd = !TAIL || p + 4 < n;
@@ -455,8 +467,8 @@ __device__ inline uint32_t fse_quad_round(const uint16_t* DT, const uint8_t* SY,
     const uint32_t incl = __builtin_amdgcn_ubfe(M, sq, 8);
     const uint32_t bits = __builtin_amdgcn_ubfe((uint32_t)(W >> ((64 - incl) & 63)), 0, nb);
     st = upd ? (e >> 4) + bits : st;
-    Q -= M >> 24;
-    W = qwindow(ring, Q);
+    Qm -= M >> 24;
+    W = qwindow_m(ring, Qm);
     return y;
 }
 
@@ -515,8 +527,8 @@ __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
     // initial states, state 0 in the highest tl bits
     uint32_t st = __builtin_amdgcn_ubfe((uint32_t)(W >> (64 - (q + 1) * tl)), 0, tl);
     Pb -= 4 * tl;
-    uint32_t Q = (uint32_t)Pb;
-    W = qwindow(ring, Q);
+    uint32_t Q = (uint32_t)Pb - 64;   // window base: the reader sits at Q + 64
+    W = qwindow_m(ring, Q);
     // a group is 4 rounds (16 symbols, one dword per lane after the
     // transpose); the groups whose states all update run 4 per staging step,
     // and their dwords are stored at the next step between its ring stores
