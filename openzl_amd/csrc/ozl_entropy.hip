@@ -509,6 +509,8 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
     const uint16_t* ST = L.st[g] - 4096;   // indexed by (state >> nb) + (tt >> 18)
     const uint32_t* TT = L.tt[g];
     g_uint32* outw = (g_uint32*)(ws.payload + (size_t)b * ws.payloadCap);
+    // payload words; word capWords + 1 is a sink the payload never reaches
+    // (its tail ends in word capWords at the latest)
     const uint32_t capWords = (uint32_t)(ws.payloadCap / 4) - 2;
     const bool mine = (uint32_t)q < S;
     // init: the one position among the last S whose state index is q
@@ -546,8 +548,8 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
         const uint32_t val = __builtin_amdgcn_ubfe(state, 0, nb);
         const uint32_t ns = ST[doit ? (state >> nb) + (t >> 18) : 4096u];
         state = doit ? ns : state;
-        // nb in bytes kq..3 (24-bit multiply, full rate), summed over the quad
-        uint32_t v = (__umul24(nb, 0x10101u) | (nb << 24)) << sq;
+        // nb in bytes kq..3 (a byte broadcast by v_perm), summed over the quad
+        uint32_t v = __builtin_amdgcn_perm(0u, nb, 0u) << sq;
         v += ozl_wave::quad_xor1(v);
         const uint32_t M = v + ozl_wave::quad_xor2(v);  // byte k: bits of slots 0..k
         const uint32_t excl = __builtin_amdgcn_ubfe(M, sq, 8) - nb;
@@ -560,7 +562,9 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
         const uint64_t T = (((uint64_t)rh << 32) | rl) << ab;
         const uint32_t w0 = a0 | (uint32_t)T, w1 = (uint32_t)(T >> 32), w2 = __builtin_amdgcn_ubfe(rh, 32 - ab, ab);
         const uint32_t nAcc = ab + (M >> 24), ne = nAcc >> 5;
-        if ((uint32_t)q < ne && wpos + q < capWords) outw[wpos + q] = q == 0 ? w0 : w1;
+        // every lane stores: a completed word to its place, else to the
+        // block's sink word past the payload (no branch on the chain)
+        outw[(uint32_t)q < ne && wpos + q < capWords ? wpos + q : capWords + 1] = q == 0 ? w0 : w1;
         a0 = ne == 0 ? w0 : ne == 1 ? w1 : w2;
         ab = nAcc & 31;
         wpos += ne;
