@@ -735,6 +735,21 @@ __global__ void __launch_bounds__(64) k_huf_decode(uint32_t B, ozl_dws ws)
         win_refill32(w, d);
         return e & 255u;
     };
+    // two symbols per refill: the window holds >= 33 bits whenever a pair
+    // starts (>= 57 after the set-up; a pair takes <= 22 and a refill adds
+    // 32 when <= 32 remain), so both lookups see tl valid bits
+    auto sym2 = [&]() -> uint32_t {
+        const uint32_t d = ring_dword<kHRing>(ring, (uint64_t)((int64_t)lo + w.lb) - 4);
+        const uint32_t e0 = LUT[win_hi(w) >> (32 - tl)];
+        const uint32_t l0 = e0 >> 8;
+        w.acc <<= l0;
+        const uint32_t e1 = LUT[win_hi(w) >> (32 - tl)];
+        const uint32_t l1 = e1 >> 8;
+        w.acc <<= l1;
+        w.valid -= (int)(l0 + l1);
+        win_refill32(w, d);
+        return (e0 & 255u) | ((e1 & 255u) << 8);
+    };
     // 16 rounds per staging step, 4 steps per iteration; the iteration's 64
     // bytes are stored at the next iteration's first step, between its ring
     // stores and its loads, so that waiting for a step's loads never waits
@@ -767,10 +782,8 @@ __global__ void __launch_bounds__(64) k_huf_decode(uint32_t B, ozl_dws ws)
             uint32_t x[4];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                uint32_t v = sym();
-                v |= sym() << 8;
-                v |= sym() << 16;
-                v |= sym() << 24;
+                uint32_t v = sym2();
+                v |= sym2() << 16;
                 x[j] = v;
             }
             ob[q4] = make_uint4(x[0], x[1], x[2], x[3]);
