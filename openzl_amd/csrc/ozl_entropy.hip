@@ -85,6 +85,7 @@ struct PlanLds {
     uint16_t cumul[18];   // nested table build workspace (13 weight symbols)
     uint8_t fmap[256];
     uint32_t wWave;       // the weights' FSE bitstream is left to the wave
+    uint32_t nPart[4];    // FSE normalisation: lane 0's partial results
     ozl_eplan plan;
 };
 
@@ -244,6 +245,189 @@ __device__ __forceinline__ void wave_weights_fse(PlanLds& L, uint8_t* pay)
     if (lane == 0) L.plan.wPayload = (uint16_t)bytes;
 }
 
+// ozl_fse_normalize (FSE_normalizeCount with low-probability counts,
+// fse_compress.c:487-549) by the wave: every symbol's count is its own
+// (lane s mod 64 takes symbols s, s+64, ...); the counts' sum, the constant
+// test and the largest count (first symbol among equal ones, as the
+// sequential strict comparison keeps) are wave reductions. Lane 0 then
+// applies the correction to the largest, or runs ozl_fse_normalize_m2 when
+// the reference does. Returns ozl_fse_normalize's result in every lane.
+__device__ int wave_fse_normalize(PlanLds& L, uint32_t total, uint32_t maxSym, uint32_t tl)
+{
+    const uint32_t lane = threadIdx.x;
+    if (tl < OZL_FSE_MIN_TABLELOG || tl > OZL_FSE_MAX_TABLELOG || tl < ozl_fse_min_tablelog(total, maxSym)) return -1;
+    const uint64_t scale = 62 - tl;
+    const uint64_t step = (1ull << 62) / total;
+    const uint64_t vStep = 1ull << (scale - 20);
+    const uint32_t lowThreshold = total >> tl;
+    int used = 0;             // counts given out (a -1 takes one)
+    uint32_t constant = 0, key = 255;   // key: largest count << 8 | (255 - symbol)
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t sy = lane + 64u * k;
+        if (sy > maxSym) continue;
+        const uint32_t c = L.count[sy];
+        int16_t v;
+        if (c == total) constant = 1;
+        if (c == 0) v = 0;
+        else if (c <= lowThreshold) v = -1;
+        else {
+            const uint64_t cs = (uint64_t)c * step;
+            int p = (int)(cs >> scale);
+            if (p < 8) p += cs - ((uint64_t)p << scale) > vStep * ozl_fse_rtb(p) ? 1 : 0;
+            v = (int16_t)p;
+            const uint32_t kk = ((uint32_t)p << 8) | (255u - sy);
+            key = kk > key ? kk : key;
+        }
+        used += v < 0 ? 1 : v;
+        L.norm[sy] = v;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        used += __shfl_xor(used, d, 64);
+        constant |= __shfl_xor(constant, d, 64);
+        const uint32_t o = __shfl_xor(key, d, 64);
+        key = o > key ? o : key;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        int rc = 0;
+        if (constant) rc = 1;
+        else {
+            const int still = (1 << tl) - used;
+            const uint32_t largest = 255u - (key & 255u);
+            if (-still >= (L.norm[largest] >> 1)) rc = ozl_fse_normalize_m2(L.norm, tl, L.count, total, maxSym);
+            else L.norm[largest] = (int16_t)(L.norm[largest] + still);
+        }
+        L.nPart[0] = (uint32_t)rc;
+    }
+    __syncthreads();
+    return (int)L.nPart[0];
+}
+
+// ozl_fse_write_ncount (FSE_writeNCount_generic, fse_compress.c:246-357) by
+// the wave. The header is the LSB-first concatenation of: tableLog - 5 in 4
+// bits; for each symbol up to the one that brings `remaining` to 1, its
+// field, except zeros that follow a zero, whose run is a repeat field written
+// before the next nonzero symbol (16 one-bits per 24 zeros, 2 one-bits per 3
+// more, then the rest in 2 bits). A symbol's field depends only on
+// remaining before it (2^tl + 1 minus the counts before it), which a wave
+// scan gives: threshold = the largest power of two <= remaining (at most
+// 2^tl), max = 2 threshold - 1 - remaining, value count + 1 (+ max when it
+// reaches threshold), tl + 1 - log2(2^tl / threshold) bits less one when
+// the value is below max. Lane l takes symbols 4l..4l+3; the fields' offsets
+// come from a wave scan of their widths and the bits are OR-ed into `words`
+// (LDS, OZL_NCOUNT_WORDS, cleared here). Returns the header's bytes (written
+// to dst), or 0 where the reference writer returns 0.
+#define OZL_NCOUNT_WORDS 128
+__device__ uint32_t wave_fse_ncount(PlanLds& L, uint8_t* dst, uint32_t maxSym, uint32_t tl, uint32_t* words)
+{
+    const uint32_t lane = threadIdx.x;
+    const int ts = 1 << tl;
+    for (uint32_t i = lane; i < OZL_NCOUNT_WORDS; i += 64) words[i] = 0;
+    // the lane's counts, their weights (|count|), zero flags and the last
+    // nonzero symbol among them
+    int v[4];
+    int aLoc = 0, lnz = -1;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t sy = 4 * lane + (uint32_t)k;
+        v[k] = sy <= maxSym ? L.norm[sy] : 0;
+        aLoc += v[k] < 0 ? -v[k] : v[k];
+        if (sy <= maxSym && v[k] != 0) lnz = (int)sy;
+    }
+    // exclusive scans: weights before the lane, last nonzero symbol before it
+    int aInc = aLoc, zInc = lnz;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int ya = __shfl_up(aInc, d, 64), yz = __shfl_up(zInc, d, 64);
+        if ((int)lane >= d) {
+            aInc += ya;
+            zInc = yz > zInc ? yz : zInc;
+        }
+    }
+    int A = aInc - aLoc;                          // weights before symbol 4l
+    int lastNz = __shfl_up(zInc, 1, 64);          // last nonzero symbol before 4l
+    if (lane == 0) lastNz = -1;
+    // previous symbol's count (symbol 4l - 1)
+    int prevV = __shfl_up(v[3], 1, 64);
+    if (lane == 0) prevV = 1;                     // symbol 0: no zero before it
+    // fields
+    uint32_t fw[4], fv[4], rw[4], rr[4];
+    uint32_t wLoc = 0;
+    bool bad = false, reached = false;            // remaining went below 1 / reached 1
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t sy = 4 * lane + (uint32_t)k;
+        const int R = ts + 1 - A;                 // remaining before sy
+        const bool live = sy <= maxSym && R > 1;  // the writer's loop reaches sy
+        const int pv = k == 0 ? prevV : v[k - 1];
+        const bool cont = live && v[k] == 0 && sy > 0 && pv == 0;   // inside a zero run
+        fw[k] = fv[k] = rw[k] = rr[k] = 0;
+        if (live && !cont) {
+            if (v[k] != 0 && sy > 0 && pv == 0) {
+                // repeat field of the zeros since the run's first one
+                const uint32_t r = sy - 2 - (uint32_t)lastNz;
+                rw[k] = 16 * (r / 24) + 2 * ((r % 24) / 3) + 2;
+                rr[k] = r % 3;
+            }
+            const int thr = R > ts ? ts : 1 << ozl_hb32((uint32_t)R);
+            const int nb = ozl_hb32((uint32_t)thr) + 1;
+            const int mx = 2 * thr - 1 - R;
+            int c = v[k] + 1;
+            if (c >= thr) c += mx;
+            fv[k] = (uint32_t)c;
+            fw[k] = (uint32_t)(nb - (c < mx ? 1 : 0));
+            const int R2 = R - (v[k] < 0 ? -v[k] : v[k]);
+            if (R2 < 1) bad = true;
+            if (R2 == 1) reached = true;
+        }
+        if (sy <= maxSym && v[k] != 0) lastNz = (int)sy;
+        A += v[k] < 0 ? -v[k] : v[k];
+        wLoc += rw[k] + fw[k];
+    }
+    // the writer stops where remaining first drops to 1 or below: below 1,
+    // or never reaching 1 within the alphabet, it returns 0
+    const bool ok = __ballot(reached) != 0 && __ballot(bad) == 0;
+    uint32_t wInc = wLoc;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(wInc, d, 64);
+        if ((int)lane >= d) wInc += y;
+    }
+    const uint32_t totalBits = 4 + __shfl(wInc, 63, 64);
+    if (!ok || totalBits > 32 * OZL_NCOUNT_WORDS - 32) return 0;
+    __syncthreads();
+    auto put = [&](uint32_t P, uint32_t val, uint32_t w) {   // w <= 32, val < 2^w
+        if (!w) return;
+        const uint32_t sh = P & 31;
+        atomicOr(&words[P >> 5], val << sh);
+        if (sh + w > 32) atomicOr(&words[(P >> 5) + 1], val >> (32 - sh));
+    };
+    if (lane == 0) put(0, tl - OZL_FSE_MIN_TABLELOG, 4);
+    uint32_t P = 4 + wInc - wLoc;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (rw[k]) {
+            uint32_t ones = rw[k] - 2;
+            while (ones) {
+                const uint32_t c = ones < 32 ? ones : 32;
+                put(P, c == 32 ? 0xFFFFFFFFu : (1u << c) - 1u, c);
+                P += c;
+                ones -= c;
+            }
+            put(P, rr[k], 2);
+            P += 2;
+        }
+        put(P, fv[k], fw[k]);
+        P += fw[k];
+    }
+    __syncthreads();
+    const uint32_t bytes = (totalBits + 7) / 8;
+    for (uint32_t i = lane; i < bytes; i += 64) dst[i] = (uint8_t)(words[i >> 2] >> (8 * (i & 3)));
+    return bytes;
+}
+
 // Histogram of every block (one 256-thread workgroup per block, LDS
 // atomics on 16 interleaved copies), parked in the block's table area.
 __global__ void __launch_bounds__(kT) k_hist(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
@@ -340,14 +524,8 @@ __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, co
             unsigned tl = ozl_fse_optimal_tablelog(OZL_FSE_DEFAULT_TABLELOG, n, maxSym, 2);
             p.tl = (uint8_t)tl;
             p.nbStates = n < 1000 ? 2 : 4;
-            if (tl > 11 || ozl_fse_normalize(L.norm, tl, L.count, n, maxSym) != 0) { p.status = OZL_ERR_GENERIC; break; }
-            if (ozl_small(2u * (maxSym + 1u), ws.minSS)) {
-                for (unsigned s = 0; s <= maxSym; s++) { aux[2 * s] = (uint8_t)L.norm[s]; aux[2 * s + 1] = (uint8_t)((uint16_t)L.norm[s] >> 8); }
-                p.aux = (uint16_t)(2 * (maxSym + 1));
-            } else {
-                p.aux = (uint16_t)ozl_fse_write_ncount(aux, L.norm, maxSym, tl);
-            }
-            break;
+            if (tl > 11) p.status = OZL_ERR_GENERIC;
+            break;   // normalisation and NCount by the wave, below
         }
         case OZL_BK_HUF: {
             unsigned tl = ozl_fse_optimal_tablelog(OZL_HUF_DEFAULT_TABLELOG, n, maxSym, 1);
@@ -362,6 +540,27 @@ __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, co
         }
     }
     __syncthreads();
+    if (!HUF && L.plan.bk == OZL_BK_FSE && L.plan.status == 0) {
+        const uint32_t maxSym = L.plan.maxSym, tl = L.plan.tl;
+        const int rc = wave_fse_normalize(L, n, maxSym, tl);
+        if (rc != 0) {
+            if (threadIdx.x == 0) L.plan.status = OZL_ERR_GENERIC;
+        } else if (ozl_small(2u * (maxSym + 1u), ws.minSS)) {
+            // raw int16 counts (stored after a num->serial conversion)
+            for (uint32_t s = threadIdx.x; s <= maxSym; s += 64) {
+                L.aux[2 * s] = (uint8_t)L.norm[s];
+                L.aux[2 * s + 1] = (uint8_t)((uint16_t)L.norm[s] >> 8);
+            }
+            if (threadIdx.x == 0) L.plan.aux = (uint16_t)(2 * (maxSym + 1));
+        } else {
+            // the words live in the table build's area, which is free until
+            // wave_fse_ctable below
+            static_assert(OZL_NCOUNT_WORDS * 4 <= sizeof(ozl_wave::FseBuild), "ncount words");
+            const uint32_t bytes = wave_fse_ncount(L, L.aux, maxSym, tl, reinterpret_cast<uint32_t*>(&L.fb));
+            if (threadIdx.x == 0) L.plan.aux = (uint16_t)bytes;
+        }
+        __syncthreads();
+    }
     if (HUF && L.plan.bk == OZL_BK_HUF && L.plan.status == 0) {
         // canonical codes (huf_compress.c:706-729) by the wave: leaves per
         // length, the first value of each length, every symbol's length, and
