@@ -185,7 +185,7 @@ struct DQueue {
     std::vector<FillItem> fills;
     std::vector<std::pair<int*, uint32_t>> statuses;
     std::vector<std::vector<uint8_t>> keep;   // host bytes asynchronous copies read (when pinned staging fails)
-    std::vector<Decoder*> members;
+    std::vector<DStream*> pend;                // streams produced by queued work
     std::vector<uint64_t> gSrc, gSize, gDst;  // queued copies (device addresses)
     int err = 0;
 
@@ -315,6 +315,7 @@ struct Decoder {
             in.type = T_SERIAL_;
             in.width = 1;
             in.pending = true;
+            q->pend.push_back(&in);
             ozl_ditem it;
             memset(&it, 0, sizeof(it));
             it.side = (uint64_t)(uintptr_t)side.dev;
@@ -362,6 +363,7 @@ struct Decoder {
             in = s;
             in.type = type;
             in.width = width;
+            if (in.pending) q->pend.push_back(&in);
             return 0;
         }
         default: break;
@@ -498,6 +500,7 @@ struct Decoder {
                 q->fills.push_back({ (uint64_t)(uintptr_t)in.dev, n, (uint64_t)(uintptr_t)v.dev });
                 in.ready = false;
                 in.pending = true;
+                q->pend.push_back(&in);
                 return 0;
             }
             break;
@@ -542,6 +545,7 @@ struct Decoder {
             in.width = w;
             in.ready = false;
             in.pending = true;   // the concatenation goes out with the queue
+            q->pend.push_back(&in);
             return 0;
         }
         case OZL_TR_CONSTANT_FIXED: {  // decode_constant_binding.c:10-47
@@ -664,10 +668,21 @@ int DQueue::flush()
         // launched in stream order without waiting: later nodes that read
         // these outputs are enqueued after them. Items of one coder sit
         // together (each coder's kernel packs consecutive items into a
-        // workgroup), longest first, so the long state chains start first.
-        std::stable_sort(items.begin(), items.end(), [](const ozl_ditem& a, const ozl_ditem& b) {
-            return a.huf != b.huf ? a.huf < b.huf : a.n > b.n;
-        });
+        // workgroup), longer ones first (by powers of two: a counting sort,
+        // since a call may queue tens of thousands), so the long state chains
+        // start first.
+        {
+            auto key = [](const ozl_ditem& it) {
+                const uint32_t lg = it.n ? 31u - (uint32_t)__builtin_clz(it.n) : 0u;
+                return (it.huf ? 32u : 0u) + (31u - lg);
+            };
+            uint32_t start[65] = { 0 };
+            for (const ozl_ditem& it : items) start[key(it) + 1]++;
+            for (int k = 0; k < 64; k++) start[k + 1] += start[k];
+            std::vector<ozl_ditem> sorted(items.size());
+            for (const ozl_ditem& it : items) sorted[start[key(it)]++] = it;
+            items.swap(sorted);
+        }
         const uint32_t B = (uint32_t)items.size();
         ozl_ditem* dItems = (ozl_ditem*)alloc(B * sizeof(ozl_ditem));
         int* dStatus = (int*)alloc(B * 4);
@@ -683,9 +698,9 @@ int DQueue::flush()
     }
     // everything pending is now enqueued (conversions alias their producer's
     // buffer, so they were pending too)
-    for (Decoder* m : members)
-        for (DStream& x : m->S)
-            if (x.pending) { x.pending = false; x.ready = true; }
+    for (DStream* x : pend)
+        if (x->pending) { x->pending = false; x->ready = true; }
+    pend.clear();
     return 0;
 }
 
@@ -806,7 +821,7 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
             ozl_parallel_for(L.size(), [&](size_t k) {
                 FrameJob& j = J[L[k]];
                 bool longer = false;
-                prc[k] = parse_window(hw + wOff[k], j.pos, wLen[k], j.size, j.P, j.pos, j.done, longer);
+                prc[k] = parse_window(hw + wOff[k], j.pos, wLen[k], j.size, j.P, j.pos, j.done, longer, true);
                 lg[k] = longer;
             });
             for (size_t k = 0; k < L.size(); k++) {
@@ -843,7 +858,9 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
             OZL_HIP_CHECK(hipMemcpyAsync(j.dFrame, j.src, j.size, hipMemcpyHostToDevice, st));
         }
     }
-    for (ChunkJob& u : U) {
+    // each chunk's decoder (host work only; one chunk per worker)
+    ozl_parallel_for(U.size(), [&](size_t ui) {
+        ChunkJob& u = U[ui];
         const ParsedChunk& C = *u.C;
         const FrameJob& j = J[u.frame];
         u.D.reset(new Decoder());
@@ -853,7 +870,6 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         D.st = st;
         D.S.assign(C.nbStreams, DStream());
         D.limit = j.P.size + ((size_t)1 << 17);
-        Q.members.push_back(&D);
         size_t pos = C.dataStart;
         for (size_t k = 0; k < C.storedId.size(); k++) {
             DStream& x = D.S[C.storedId[k]];
@@ -872,24 +888,28 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         }
         for (size_t k = C.nodes.size(); k-- > 0;) u.todo.push_back((uint32_t)k);
         if (!j.dev) memcpy(u.tail, j.src + C.csumPos, C.end - C.csumPos);
-    }
+    });
+    // a device frame's checksum tails, 16 bytes a chunk; after the last
+    // chunk of a frame whose end marker the parse did not read, that byte too
     std::vector<uint8_t> tails;
     uint8_t* dTails = nullptr;
     if (devIn && !U.empty()) {
         const size_t nu = U.size();
         std::vector<uint64_t> m(3 * nu);
         for (size_t k = 0; k < nu; k++) {
-            m[k] = (uint64_t)(uintptr_t)(J[U[k].frame].src + U[k].C->csumPos);
-            m[nu + k] = U[k].C->end - U[k].C->csumPos;
-            m[2 * nu + k] = k * 8;
+            const FrameJob& j = J[U[k].frame];
+            const bool withEnd = j.P.endDeferred && U[k].C == &j.P.chunks.back();
+            m[k] = (uint64_t)(uintptr_t)(j.src + U[k].C->csumPos);
+            m[nu + k] = U[k].C->end - U[k].C->csumPos + (withEnd ? 1 : 0);
+            m[2 * nu + k] = k * 16;
         }
-        dTails = dc->alloc(nu * 8 + 16);
+        dTails = dc->alloc(nu * 16 + 16);
         uint64_t* dm = (uint64_t*)dc->alloc(m.size() * 8 + 16);
         if (!dTails || !dm) return OZL_ERR_ALLOCATION;
         OZL_HIP_CHECK(hipMemcpyAsync(dm, Q.host_keep(m.data(), m.size() * 8), m.size() * 8, hipMemcpyHostToDevice, st));
-        int rc = ozl_gather_sized((const uint8_t* const*)dm, dm + nu, dm + 2 * nu, (int)nu, 8, dTails, st);
+        int rc = ozl_gather_sized((const uint8_t* const*)dm, dm + nu, dm + 2 * nu, (int)nu, 9, dTails, st);
         if (rc) return rc;
-        tails.resize(nu * 8);
+        tails.resize(nu * 16);
     }
     // compressed checksums (they read only the frames) on the second stream
     std::vector<uint64_t> kJobs;
@@ -1013,11 +1033,16 @@ int decompress_frames(ZL_DCtx_s* dc, const uint8_t* src, const size_t* frameSize
         OZL_HIP_CHECK(hipMemcpyAsync((uint8_t*)dst + u.outOff, out.dev, out.bytes,
                                      dstDev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     }
-    if (devIn && !U.empty()) OZL_HIP_CHECK(hipMemcpyAsync(tails.data(), dTails, U.size() * 8, hipMemcpyDeviceToHost, st));
+    if (devIn && !U.empty()) OZL_HIP_CHECK(hipMemcpyAsync(tails.data(), dTails, U.size() * 16, hipMemcpyDeviceToHost, st));
     int rc = Q.check_statuses();   // synchronises the stream
     if (rc) return rc;
     if (devIn)
-        for (size_t k = 0; k < U.size(); k++) memcpy(U[k].tail, tails.data() + k * 8, U[k].C->end - U[k].C->csumPos);
+        for (size_t k = 0; k < U.size(); k++) {
+            const size_t tl = U[k].C->end - U[k].C->csumPos;
+            memcpy(U[k].tail, tails.data() + k * 16, tl);
+            const FrameJob& j = J[U[k].frame];
+            if (j.P.endDeferred && U[k].C == &j.P.chunks.back() && tails[k * 16 + tl] != 0) return OZL_ERR_CORRUPTION;
+        }
     for (size_t k = 0; k < kIdx.size(); k++) {
         const ChunkJob& u = U[kIdx[k]];
         if ((uint32_t)hK[k] != ozl_le32(u.tail + ((J[u.frame].P.flags & 1) ? 4 : 0))) return OZL_ERR_COMPRESSED_CHECKSUM;

@@ -26,23 +26,15 @@ bool ozl_is_device_ptr(const void* p);
 #include <thread>
 #include <vector>
 
-// f(0) .. f(n-1) on up to eight host threads (pure host work: no HIP calls).
+// f(0) .. f(n-1) on the library's host worker threads and the caller (pure
+// host work: no HIP calls). The workers (up to 15 besides the caller) are
+// started once and kept; calls from several threads take turns, and a call
+// made from inside f runs inline.
+void ozl_pool_run(size_t n, void (*fn)(void*, size_t), void* ctx);
 template <class F>
 void ozl_parallel_for(size_t n, F f)
 {
-    const unsigned hc = std::thread::hardware_concurrency();
-    const size_t T = std::min<size_t>(n, std::min<size_t>(8, hc ? hc : 1));
-    if (T <= 1 || n < 4) {
-        for (size_t i = 0; i < n; i++) f(i);
-        return;
-    }
-    std::atomic<size_t> next(0);
-    std::vector<std::thread> th;
-    for (size_t t = 0; t < T; t++)
-        th.emplace_back([&]() {
-            for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
-        });
-    for (auto& x : th) x.join();
+    ozl_pool_run(n, [](void* c, size_t i) { (*static_cast<F*>(c))(i); }, &f);
 }
 
 // Waits for up to three streams when it goes out of scope.

@@ -885,11 +885,9 @@ namespace {
 __global__ void __launch_bounds__(64) k_prep_items(const ozl_ditem* __restrict__ items, uint32_t B, ozl_dws ws)
 {
     __shared__ int16_t norm[256];
-    __shared__ uint8_t w[256];
     __shared__ uint8_t len[256];
-    __shared__ uint16_t val[256];
-    __shared__ uint16_t rk[28];
-    __shared__ int sBk;
+    __shared__ __attribute__((aligned(16))) uint8_t pre[OZL_DPREFIX + 16];
+    __shared__ ozl_dplan sD;
     uint32_t b = blockIdx.x;
     if (b >= B) return;
     ozl_ditem it = items[b];
@@ -897,6 +895,26 @@ __global__ void __launch_bounds__(64) k_prep_items(const ozl_ditem* __restrict__
     // came from a byte load it treated as uniform; reading it through
     // readfirstlane makes the comparison a scalar one.
     const uint32_t huf = __builtin_amdgcn_readfirstlane((uint32_t)it.huf);
+    const uint8_t* side = (const uint8_t*)(uintptr_t)it.side;
+    const uint32_t sideN = it.sideN;
+    const bool nc = !huf && (sideN & 0x80000000u);
+    // the side stream (NCount header, raw int16 counts or weights) copied to
+    // LDS by the wave as words, zeros past it, when it fits the copy
+    const uint64_t sideBytes = huf ? (sideN < 256u ? sideN : 256u) : nc ? (sideN & 0x7FFFFFFFu) : 2ull * sideN;
+    const bool inLds = sideBytes <= OZL_DPREFIX;
+    {
+        uint32_t* pw = reinterpret_cast<uint32_t*>(pre);
+        for (uint32_t i = threadIdx.x; i < (OZL_DPREFIX + 16) / 4; i += 64) {
+            uint32_t x = 0;
+            for (uint32_t k = 0; k < 4; k++) {
+                const uint64_t j = 4ull * i + k;
+                if (inLds && j < sideBytes) x |= (uint32_t)side[j] << (8 * k);
+            }
+            pw[i] = x;
+        }
+        for (int s = threadIdx.x; s < 256; s += 64) norm[s] = 0;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         ozl_dplan d = {};
         d.n = it.n;
@@ -905,24 +923,23 @@ __global__ void __launch_bounds__(64) k_prep_items(const ozl_ditem* __restrict__
         d.srcPtr = it.bits;
         d.outPtr = it.out;
         d.bk = huf ? OZL_BK_HUF : OZL_BK_FSE;
-        const uint8_t* side = (const uint8_t*)(uintptr_t)it.side;
         if (!huf) {
-            uint32_t sum = 0;
+            uint32_t sum = 0, cnt = sideN;
             bool bad = false;
-            uint32_t sideN = it.sideN;
-            for (int s = 0; s < 256; s++) norm[s] = 0;
-            if (sideN & 0x80000000u) {
+            const uint8_t* sp = inLds ? pre : side;
+            if (nc) {
                 // the side stream is still an NCount header (fse_compress.c:246-357)
-                const uint32_t len = sideN & 0x7FFFFFFFu;
+                const uint32_t nlen = sideN & 0x7FFFFFFFu;
                 unsigned ms = 0, ntl = 0;
-                const long r = ozl_fse_read_ncount(norm, &ms, &ntl, side, len);
-                bad = r < 0 || (uint32_t)r != len;
-                sideN = bad ? 0u : ms + 1;
-                for (uint32_t s = 0; !bad && s < sideN; s++) sum += norm[s] == -1 ? 1 : (uint32_t)norm[s];
+                const ozl_fview fv = { side, pre, inLds ? nlen : 0u, reinterpret_cast<const uint32_t*>(pre) };
+                const long r = ozl_dread_ncount(norm, &ms, &ntl, fv, 0, nlen, true);
+                bad = r < 0 || (uint32_t)r != nlen;
+                cnt = bad ? 0u : ms + 1;
+                for (uint32_t s = 0; !bad && s < cnt; s++) sum += norm[s] == -1 ? 1 : (uint32_t)norm[s];
             } else {
-                bad = sideN == 0 || sideN > 256;
-                for (uint32_t s = 0; !bad && s < sideN; s++) {
-                    norm[s] = (int16_t)(side[2 * s] | (side[2 * s + 1] << 8));
+                bad = cnt == 0 || cnt > 256;
+                for (uint32_t s = 0; !bad && s < cnt; s++) {
+                    norm[s] = (int16_t)(sp[2 * s] | (sp[2 * s + 1] << 8));
                     if (norm[s] < -1) bad = true;
                     sum += norm[s] == -1 ? 1 : (uint32_t)norm[s];
                 }
@@ -933,22 +950,48 @@ __global__ void __launch_bounds__(64) k_prep_items(const ozl_ditem* __restrict__
                 d.status = OZL_ERR_CORRUPTION;
             else {
                 d.tl = (uint8_t)tl;
-                d.aux = sideN - 1;
+                d.aux = cnt - 1;
             }
-        } else {
-            for (uint32_t s = 0; s < it.sideN && s < 256; s++) w[s] = side[s];
-            int tl = it.sideN <= 256 ? ozl_huf_from_weights(len, val, w, it.sideN, rk) : -1;
-            if (tl < 1 || tl > 11 || it.nbStates > 1 || (it.nbStates == 1 && it.n < 12)) d.status = OZL_ERR_CORRUPTION;
-            else d.tl = (uint8_t)tl;
+        } else if (sideN > 256 || it.nbStates > 1 || (it.nbStates == 1 && it.n < 12)) {
+            d.status = OZL_ERR_CORRUPTION;
         }
-        ws.plans[b] = d;
-        sBk = d.status == 0 ? d.bk : 255;
+        sD = d;
     }
     __syncthreads();
+    if (huf && sD.status == 0) {
+        // code lengths from the weights (ozl_huf_from_weights): a weight
+        // above 12 or a sum of 2^(w-1) that is not a power of two is corrupt
+        uint32_t tot = 0, bad = 0;
+        uint8_t wl[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t sy = threadIdx.x + 64u * k;
+            const uint32_t wv = sy < sideN ? pre[sy] : 0u;
+            wl[k] = (uint8_t)wv;
+            bad |= wv > 12 ? 1u : 0u;
+            tot += wv && wv <= 12 ? 1u << (wv - 1) : 0u;
+        }
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1) {
+            tot += __shfl_xor(tot, dd, 64);
+            bad |= __shfl_xor(bad, dd, 64);
+        }
+        const uint32_t tl = tot ? (uint32_t)ozl_hb32(tot) : 0u;
+        const bool okW = !bad && tot && (tot & (tot - 1)) == 0 && tl >= 1 && tl <= 11;
+#pragma unroll
+        for (int k = 0; k < 4; k++) len[threadIdx.x + 64 * k] = okW && wl[k] ? (uint8_t)(tl + 1 - wl[k]) : 0;
+        if (threadIdx.x == 0) {
+            if (okW) sD.tl = (uint8_t)tl;
+            else sD.status = OZL_ERR_CORRUPTION;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ws.plans[b] = sD;
+    const int bk = sD.status == 0 ? sD.bk : 255;
     uint8_t* sd = ws.side + (size_t)b * OZL_DSIDE_BYTES;
-    if (sBk == OZL_BK_FSE)
+    if (bk == OZL_BK_FSE)
         for (int s = threadIdx.x; s < 256; s += 64) reinterpret_cast<int16_t*>(sd)[s] = norm[s];
-    else if (sBk == OZL_BK_HUF)
+    else if (bk == OZL_BK_HUF)
         for (int s = threadIdx.x; s < 256; s += 64) sd[s] = len[s];
 }
 

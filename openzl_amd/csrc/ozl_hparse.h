@@ -53,9 +53,33 @@ struct HostReader {
     }
 };
 
+// A node's header bytes: held in place up to 22 bytes (every header of the
+// codecs here), in a vector beyond (a frame carries thousands of nodes).
+struct NodeHeader {
+    uint8_t n = 0;
+    uint8_t b[22];
+    std::vector<uint8_t> big;
+    size_t size() const { return n < sizeof(b) + 1 ? n : big.size(); }
+    bool empty() const { return size() == 0; }
+    const uint8_t* data() const { return n <= sizeof(b) ? b : big.data(); }
+    uint8_t operator[](size_t k) const { return data()[k]; }
+    void assign(const uint8_t* first, const uint8_t* last)
+    {
+        const size_t len = (size_t)(last - first);
+        if (len <= sizeof(b)) {
+            n = (uint8_t)len;
+            for (size_t k = 0; k < len; k++) b[k] = first[k];
+            big.clear();
+        } else {
+            n = (uint8_t)(sizeof(b) + 1);
+            big.assign(first, last);
+        }
+    }
+};
+
 struct DNode {
     uint8_t trid;
-    std::vector<uint8_t> hdr;
+    NodeHeader hdr;
     uint32_t nbVO;
     uint32_t firstOut, nbOut, in;
 };
@@ -90,6 +114,7 @@ struct ParsedFrame {
     size_t headerEnd = 0;
     std::vector<ParsedChunk> chunks;
     size_t end = 0;   // position of the end-of-frame marker
+    bool endDeferred = false;   // the marker (the frame's last byte) is still to be checked as zero
 };
 
 // a read ran past the host window before the end of the frame
@@ -201,9 +226,12 @@ inline int parse_chunk(HostReader& r, size_t frameSize, int flags, ParsedChunk& 
 // (the whole frame, or a window of a device frame): parses the header when
 // pos is 0, then chunks until the end marker, or until the window runs out
 // (kNeedMore: the next window starts at pos; `longer` when a chunk's headers
-// did not fit a window that starts at the chunk).
+// did not fit a window that starts at the chunk). With deferEnd, a chunk that
+// ends on the frame's last byte, outside the window, ends the frame: that
+// byte can only be the end marker (a chunk there would not fit), and the
+// caller checks that it is zero (P.endDeferred).
 inline int parse_window(const uint8_t* w, size_t base, size_t len, size_t frameSize, ParsedFrame& P, size_t& pos,
-                 bool& done, bool& longer)
+                 bool& done, bool& longer, bool deferEnd = false)
 {
     longer = false;
     HostReader r = { w, base, 0, base + len, false };
@@ -232,6 +260,12 @@ inline int parse_window(const uint8_t* w, size_t base, size_t len, size_t frameS
         if (rc) return rc;
         pos = C.end;
         P.chunks.push_back(std::move(C));
+        if (deferEnd && pos >= base + len && pos + 1 == frameSize) {
+            P.end = pos;
+            P.endDeferred = true;
+            done = true;
+            return 0;
+        }
     }
 }
 

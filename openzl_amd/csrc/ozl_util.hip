@@ -4,6 +4,9 @@
 #include "ozl_xxh3.h"
 #include "ozl_gmem.h"
 #include "ozl_profile.h"
+#include <condition_variable>
+#include <mutex>
+
 #include "ozl_engine_util.h"
 
 namespace {
@@ -169,6 +172,82 @@ __global__ void __launch_bounds__(64) k_put_tails(uint8_t* __restrict__ dst, con
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------- host worker threads
+namespace {
+struct HostPool {
+    std::mutex callMu;   // one job at a time
+    std::mutex mu;
+    std::condition_variable cv, done;
+    std::vector<std::thread> th;
+    void (*fn)(void*, size_t) = nullptr;
+    void* ctx = nullptr;
+    size_t n = 0;
+    std::atomic<size_t> next{ 0 };
+    size_t active = 0;   // workers still on the job
+    uint64_t gen = 0;
+    bool quit = false;
+    ~HostPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            quit = true;
+        }
+        cv.notify_all();
+        for (auto& t : th) t.join();
+    }
+};
+thread_local bool tInHostPool = false;
+
+void host_worker(HostPool* P)
+{
+    tInHostPool = true;
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> lk(P->mu);
+    for (;;) {
+        P->cv.wait(lk, [&]() { return P->quit || P->gen != seen; });
+        if (P->quit) return;
+        seen = P->gen;
+        void (*fn)(void*, size_t) = P->fn;
+        void* ctx = P->ctx;
+        const size_t n = P->n;
+        lk.unlock();
+        for (size_t i; (i = P->next.fetch_add(1)) < n;) fn(ctx, i);
+        lk.lock();
+        if (--P->active == 0) P->done.notify_all();
+    }
+}
+}  // namespace
+
+void ozl_pool_run(size_t n, void (*fn)(void*, size_t), void* ctx)
+{
+    if (tInHostPool || n < 4) {
+        for (size_t i = 0; i < n; i++) fn(ctx, i);
+        return;
+    }
+    static HostPool P;
+    std::lock_guard<std::mutex> call(P.callMu);
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        if (P.th.empty()) {
+            const unsigned hc = std::thread::hardware_concurrency();
+            const unsigned W = (hc ? (hc < 16 ? hc : 16) : 1) - 1;
+            for (unsigned t = 0; t < W; t++) P.th.emplace_back(host_worker, &P);
+        }
+        P.fn = fn;
+        P.ctx = ctx;
+        P.n = n;
+        P.next = 0;
+        P.active = P.th.size();
+        P.gen++;
+    }
+    P.cv.notify_all();
+    tInHostPool = true;
+    for (size_t i; (i = P.next.fetch_add(1)) < n;) fn(ctx, i);
+    tInHostPool = false;
+    std::unique_lock<std::mutex> lk(P.mu);
+    P.done.wait(lk, [&]() { return P.active == 0; });
+}
 
 size_t ozl_xxh3_scratch_size(size_t len) { return ((len / 1024) + 1) * 64 + 256; }
 
