@@ -861,7 +861,8 @@ __global__ void __launch_bounds__(kT) k_huf_encode(const uint8_t* __restrict__ s
                                                    uint32_t B, ozl_ews ws)
 {
     __shared__ uint32_t sCode[256];  // code value | length << 16
-    __shared__ uint32_t sBits[(kHufMaxSeg * 11) / 32 + 64];
+    constexpr uint32_t kBitWords = (kHufMaxSeg * 11) / 32 + 64;
+    __shared__ uint32_t sBits[kBitWords + kT];   // the stream's words, then one sink word per thread
     __shared__ uint32_t sScan[4];
     uint32_t b = blockIdx.x;
     if (b >= B) return;
@@ -903,32 +904,40 @@ __global__ void __launch_bounds__(kT) k_huf_encode(const uint8_t* __restrict__ s
         for (uint32_t w = threadIdx.x; w < words; w += kT) sBits[w] = 0;
         __syncthreads();
         if (mine) {
-            const uint32_t w0 = bit >> 5;
+            // the thread's first word is shared with the thread before (kept
+            // in `first`, OR-ed at the end), its last with the one after;
+            // the words between are its own. Every step stores: a completed
+            // word to its place, else to the thread's sink word (no branch).
+            const uint32_t w0 = bit >> 5, sink = kBitWords + threadIdx.x;
             uint64_t acc = 0;
-            uint32_t nacc = bit & 31, wpos = w0;
-            auto put = [&](uint32_t c) {
-                acc |= (uint64_t)(c & 0xFFFFu) << nacc;
-                nacc += c >> 16;
-                if (nacc >= 32) {
-                    if (wpos == w0) atomicOr(&sBits[wpos], (uint32_t)acc);
-                    else sBits[wpos] = (uint32_t)acc;
-                    wpos++;
-                    acc >>= 32;
-                    nacc -= 32;
-                }
+            uint32_t nacc = bit & 31, wpos = w0, first = 0;
+            auto flush = [&]() {   // nacc < 54 on entry, < 32 after
+                const bool full = nacc >= 32;
+                const uint32_t lo32 = (uint32_t)acc;
+                sBits[full && wpos != w0 ? wpos : sink] = lo32;
+                first = full && wpos == w0 ? lo32 : first;
+                wpos += full ? 1u : 0u;
+                acc = full ? acc >> 32 : acc;
+                nacc -= full ? 32u : 0u;
+            };
+            auto put2 = [&](uint32_t c0, uint32_t c1) {   // two codes of <= 11 bits
+                acc |= (uint64_t)(c0 & 0xFFFFu) << nacc;
+                nacc += c0 >> 16;
+                acc |= (uint64_t)(c1 & 0xFFFFu) << nacc;
+                nacc += c1 >> 16;
+                flush();
             };
             for (uint32_t e = hi; e > lo; e -= 4) {
                 if (e >= lo + 4) {
                     const uint32_t d = *reinterpret_cast<const uint32_t*>(base + e - 4);
-                    put(sCode[d >> 24]);
-                    put(sCode[(d >> 16) & 255]);
-                    put(sCode[(d >> 8) & 255]);
-                    put(sCode[d & 255]);
+                    put2(sCode[d >> 24], sCode[(d >> 16) & 255]);
+                    put2(sCode[(d >> 8) & 255], sCode[d & 255]);
                 } else {
-                    for (uint32_t q = e; q > lo; q--) put(sCode[base[q - 1]]);
+                    for (uint32_t q = e; q > lo; q--) put2(sCode[base[q - 1]], 0u);
                     break;
                 }
             }
+            if (wpos != w0) atomicOr(&sBits[w0], first);
             if (nacc) atomicOr(&sBits[wpos], (uint32_t)acc);
         }
         if (threadIdx.x == 0) atomicOr(&sBits[total >> 5], 1u << (total & 31));  // end mark

@@ -158,6 +158,35 @@ def test_batch_device_buffers(api):
         assert dout[:m].cpu().numpy().tobytes() == arrs[-1].tobytes(), name
 
 
+def test_device_frames_end_marker(api):
+    """Device-resident frames: the end-of-frame byte after the last chunk is
+    read with the checksum tails (not in a header round of its own); a
+    nonzero one is rejected as corruption, in any frame of a batch."""
+    import torch
+    A = api
+    dec = A.Decompressor()
+    for name, kind, typ, width, arrs in batch_cases()[:3]:
+        comp = A.Compressor(kind)
+        at = {0: A.SERIAL, 1: A.STRUCT, 2: A.NUMERIC}[typ]
+        frames = comp.compress_batch(arrs, at, width)
+        sizes = [len(f) for f in frames]
+        total = sum(a.nbytes for a in arrs)
+        for bad in range(len(frames)):
+            print(f"end marker {name} frame {bad}", flush=True)
+            blob = bytearray(b"".join(frames))
+            end = sum(sizes[:bad + 1]) - 1
+            assert blob[end] == 0
+            blob[end] = 1
+            dfr = torch.from_numpy(np.frombuffer(bytes(blob), np.uint8).copy()).cuda()
+            dout = torch.zeros(total + 64, dtype=torch.uint8, device="cuda")
+            with pytest.raises(A.ZLError):
+                dec.decompress_ptrs(dfr.data_ptr(), sizes, dout.data_ptr(), total + 64)
+        dfr = torch.from_numpy(np.frombuffer(b"".join(frames), np.uint8).copy()).cuda()
+        dout = torch.zeros(total + 64, dtype=torch.uint8, device="cuda")
+        assert dec.decompress_ptrs(dfr.data_ptr(), sizes, dout.data_ptr(), total + 64) == [a.nbytes for a in arrs]
+        assert dout[:total].cpu().numpy().tobytes() == b"".join(a.tobytes() for a in arrs), name
+
+
 def segmented_cases():
     """(inner kind, type, width, chunk elements, array): frames of several chunks."""
     from openzl_amd import datagen as G
