@@ -834,6 +834,10 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
 // last words, which it shares with its neighbours); the buffer leaves in
 // dwords.
 constexpr int kHufMaxSeg = 25008;  // (100000+3)/4 symbols per stream
+// dwords a thread of k_huf_encode holds: a stream of <= kHufMaxSeg symbols
+// over 256 threads gives each at most 100 symbols
+constexpr int kHufRegDwords = 25;
+static_assert(((kHufMaxSeg + 255) / 256 + 3) / 4 <= kHufRegDwords, "register dwords per thread");
 
 template <typename T>
 __device__ T block_scan_excl(T v, T* lds, T& total)
@@ -887,17 +891,21 @@ __global__ void __launch_bounds__(kT) k_huf_encode(const uint8_t* __restrict__ s
         const uint32_t hiR = threadIdx.x * per, loR = hiR + per;
         const uint32_t hi = hiR < len ? len - hiR : 0u, lo = loR < len ? len - loR : 0u;
         const uint8_t* base = in + s0;
+        // the thread's dwords (positions hi-4(i+1), coded from the top) stay
+        // in registers between the two passes; the < 4 bytes below them are
+        // read again
+        const uint32_t nq = (hi - lo) / 4, low = lo + (hi - lo) % 4;   // [lo, low): tail bytes
+        uint32_t dq[kHufRegDwords];
         uint32_t mine = 0;
-        for (uint32_t e = hi; e > lo; e -= 4) {
-            if (e >= lo + 4) {
-                const uint32_t d = *reinterpret_cast<const uint32_t*>(base + e - 4);
+#pragma unroll
+        for (int i = 0; i < kHufRegDwords; i++) {
+            const uint32_t d = (uint32_t)i < nq ? *reinterpret_cast<const uint32_t*>(base + hi - 4 * (i + 1)) : 0u;
+            dq[i] = d;
+            if ((uint32_t)i < nq)
                 mine += (sCode[d >> 24] >> 16) + (sCode[(d >> 16) & 255] >> 16) + (sCode[(d >> 8) & 255] >> 16) +
                         (sCode[d & 255] >> 16);
-            } else {
-                for (uint32_t q = e; q > lo; q--) mine += sCode[base[q - 1]] >> 16;
-                break;
-            }
         }
+        for (uint32_t q = low; q > lo; q--) mine += sCode[base[q - 1]] >> 16;
         uint32_t total;
         const uint32_t bit = block_scan_excl<uint32_t>(mine, sScan, total);
         const uint32_t words = (total + 1 + 31) / 32;
@@ -927,16 +935,15 @@ __global__ void __launch_bounds__(kT) k_huf_encode(const uint8_t* __restrict__ s
                 nacc += c1 >> 16;
                 flush();
             };
-            for (uint32_t e = hi; e > lo; e -= 4) {
-                if (e >= lo + 4) {
-                    const uint32_t d = *reinterpret_cast<const uint32_t*>(base + e - 4);
+#pragma unroll
+            for (int i = 0; i < kHufRegDwords; i++) {
+                if ((uint32_t)i < nq) {
+                    const uint32_t d = dq[i];
                     put2(sCode[d >> 24], sCode[(d >> 16) & 255]);
                     put2(sCode[(d >> 8) & 255], sCode[d & 255]);
-                } else {
-                    for (uint32_t q = e; q > lo; q--) put2(sCode[base[q - 1]], 0u);
-                    break;
                 }
             }
+            for (uint32_t q = low; q > lo; q--) put2(sCode[base[q - 1]], 0u);
             if (wpos != w0) atomicOr(&sBits[w0], first);
             if (nacc) atomicOr(&sBits[wpos], (uint32_t)acc);
         }
