@@ -294,6 +294,9 @@ constexpr int kFPhase = 8;      // one-lane frames: rounds between staging steps
 constexpr int kQLook = 192;     // quads: bytes kept staged below the window
 constexpr int kQPhase = 16;     // quads: rounds between staging steps (<= 88 bytes consumed)
 
+// Frame f's decoding table starts at byte 4096 f of dt and its symbols at
+// byte 2048 f of sym, so a lane keeps one LDS byte offset per state: 4096 f +
+// 2 state indexes the table, half of it the symbols.
 struct FseDecLds {
     // the rings first: their addresses fit the LDS instructions' offsets
     uint32_t ring[kFQ][kFRingW + 4];  // dwords kFRingW, kFRingW+1 mirror 0, 1
@@ -413,11 +416,11 @@ __device__ bool fse_decode_lane(const ozl_dplan& P, const uint16_t* DT, const ui
             const uint32_t d = DT[st[j]];
             if (p < n) out[p] = SY[st[j]];
             // a state updates unless its symbol is among the last S
-            const uint32_t nb = p + S < n ? d & 15u : 0u;
+            const uint32_t nb = p + S < n ? d >> 12 : 0u;
             const uint32_t bits = __builtin_amdgcn_ubfe(win_hi(w), 32 - nb, nb);
             w.acc <<= nb;
             w.valid -= (int)nb;
-            if (p + S < n) st[j] = (d >> 4) + bits;
+            if (p + S < n) st[j] = ((d & 0xFFFu) >> 1) + bits;
             if (j & 1) win_refill32(w, ring_dword<kFRing>(ring, s + (uint64_t)(int64_t)w.lb - 4));
         }
         if (S & 1) win_refill32(w, ring_dword<kFRing>(ring, s + (uint64_t)(int64_t)w.lb - 4));
@@ -449,25 +452,63 @@ __device__ inline void put4_dwords(uint64_t a, uint32_t stride, uint32_t d0, uin
     put_dword(a + 3 * stride, d3, wide);
 }
 
+// 4x4 transpose of dwords across a quad: lane q's register k becomes lane
+// k's register q (two exchange stages, lanes xor 1 then xor 2)
+__device__ inline void quad_transpose4(uint32_t& r0, uint32_t& r1, uint32_t& r2, uint32_t& r3, uint32_t q)
+{
+    const bool o1 = q & 1, o2 = q & 2;
+    uint32_t t = quad_xor1(o1 ? r0 : r1);
+    r0 = o1 ? t : r0;
+    r1 = o1 ? r1 : t;
+    t = quad_xor1(o1 ? r2 : r3);
+    r2 = o1 ? t : r2;
+    r3 = o1 ? r3 : t;
+    t = quad_xor2(o2 ? r0 : r2);
+    r0 = o2 ? t : r0;
+    r2 = o2 ? r2 : t;
+    t = quad_xor2(o2 ? r1 : r3);
+    r1 = o2 ? t : r1;
+    r3 = o2 ? r3 : t;
+}
+
+// 16 bytes to global memory (dword stores when the destination is not
+// 16-aligned, byte stores when it is not 4-aligned)
+__device__ inline void put_16(uint64_t a, uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3, bool wide)
+{
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    if (wide && (a & 15) == 0) {
+        u32x4 v = { d0, d1, d2, d3 };
+        *(__attribute__((address_space(1))) u32x4*)(uintptr_t)a = v;
+    }
+    else put4_dwords(a, 4, d0, d1, d2, d3, wide);
+}
+
 // one round of a 4-state frame's quad: returns this lane's symbol; TAIL
 // rounds keep the states of the last 4 symbols (fse_decompress.c:287-336).
-// W holds the 64 bits above bit address Qm (the reader sits at Qm + 64).
+// W holds the 64 bits above bit address Qm (the reader sits at Qm + 64); the
+// lane's state is the LDS byte offset o = 4096 f + 2 state. The quad sums, in
+// bytes, 64 minus the bit counts of the states (lane 0 adds the 64s, each
+// lane subtracts its count from bytes q..3, a byte broadcast by v_perm), so
+// byte k holds 64 - (bits of states 0..k): shifted right by that (mod 64),
+// the window has this lane's bits at bit 0. The table's 2 * newStateBase has
+// zeros where they go, and 4096 f is above it, so the new offset is ORs.
 template <bool TAIL>
-__device__ inline uint32_t fse_quad_round(const uint16_t* DT, const uint8_t* SY, const uint32_t* ring, uint32_t& st,
-                                          uint64_t& W, uint32_t& Qm, uint32_t sq, uint32_t p, uint32_t n)
+__device__ inline uint32_t fse_quad_round(const FseDecLds& L, const uint32_t* ring, uint32_t& o, uint64_t& W, uint32_t& Qm,
+                                          uint32_t sq, uint32_t selq, uint32_t cq, uint32_t fb, uint32_t p, uint32_t n)
 {
-    const uint32_t e = DT[st], y = SY[st];
+    const uint32_t e = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const uint8_t*>(&L.dt[0][0]) + o);
+    const uint32_t y = (&L.sym[0][0])[o >> 1];
     const bool upd = !TAIL || p + 4 < n;
-    const uint32_t nb = upd ? e & 15u : 0u;
-    // nb in bytes q..3 (a byte broadcast by v_perm: a multiply here would
-    // be a quarter-rate v_mul_lo_u32 on the chain), summed over the quad
-    uint32_t v = __builtin_amdgcn_perm(0u, nb, 0u) << sq;
+    const uint32_t nb = upd ? e >> 12 : 0u;
+    uint32_t v = cq - __builtin_amdgcn_perm(0u, nb, selq);
     v += quad_xor1(v);
-    const uint32_t M = v + quad_xor2(v);          // byte k: bits of states 0..k
-    const uint32_t incl = __builtin_amdgcn_ubfe(M, sq, 8);
-    const uint32_t bits = __builtin_amdgcn_ubfe((uint32_t)(W >> ((64 - incl) & 63)), 0, nb);
-    st = upd ? (e >> 4) + bits : st;
-    Qm -= M >> 24;
+    const uint32_t M = v + quad_xor2(v);   // byte k: 64 - bits of states 0..k
+    const uint32_t X = (uint32_t)(W >> (__builtin_amdgcn_ubfe(M, sq, 8) & 63));
+    uint32_t b2 = (e & 0xFFFu) | fb;
+    __asm__("" : "+v"(b2));   // keeps b2 one value, off the chain (one v_lshl_or_b32 below)
+    const uint32_t no = (__builtin_amdgcn_ubfe(X, 0, nb) << 1) | b2;
+    o = upd ? no : o;
+    Qm += (M >> 24) - 64u;
     W = qwindow_m(ring, Qm);
     return y;
 }
@@ -499,14 +540,16 @@ __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
     const uint32_t q = lane & 3;
     if (f >= kFQ || !((factMask >> f) & 1)) return;
     const ozl_dplan P = L.plan[f];
-    const uint16_t* DT = L.dt[f];
-    const uint8_t* SY = L.sym[f];
     uint32_t* ring = L.ring[f];
     if (P.nbStates != 4) {
-        if (q == 0 && !fse_decode_lane(P, DT, SY, ring)) ws.plans[b0 + f].status = OZL_ERR_CORRUPTION;
+        if (q == 0 && !fse_decode_lane(P, L.dt[f], L.sym[f], ring)) ws.plans[b0 + f].status = OZL_ERR_CORRUPTION;
         return;
     }
     const uint32_t n = P.n, tl = P.tl, sq = 8 * q;
+    // v_perm selector of the count in bytes q..3 (0: byte 0, 0x0c: zero)
+    const uint32_t selq = 0x0c0c0c0cu & ~(0xFFFFFFFFu << sq);
+    const uint32_t cq = q == 0 ? 0x40404040u : 0u;
+    const uint32_t fb = 4096u * (uint32_t)f;
     const uint64_t s = P.srcPtr, top = s + P.payLen;
     uint8_t* out = reinterpret_cast<uint8_t*>((uintptr_t)P.outPtr);
     const bool wide = (P.outPtr & 3) == 0;
@@ -525,7 +568,7 @@ __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
     Pb -= 8 - (uint32_t)ozl_hb32(topByte | 1);
     W = qwindow(ring, (uint32_t)Pb);
     // initial states, state 0 in the highest tl bits
-    uint32_t st = __builtin_amdgcn_ubfe((uint32_t)(W >> (64 - (q + 1) * tl)), 0, tl);
+    uint32_t o = fb + 2 * __builtin_amdgcn_ubfe((uint32_t)(W >> (64 - (q + 1) * tl)), 0, tl);
     Pb -= 4 * tl;
     uint32_t Q = (uint32_t)Pb - 64;   // window base: the reader sits at Q + 64
     W = qwindow_m(ring, Q);
@@ -535,19 +578,22 @@ __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
     // and its loads, so that waiting for a step's loads never waits for
     // output stores (both count in vmcnt, which drains in order)
     auto group = [&]() -> uint32_t {
-        uint32_t acc = fse_quad_round<false>(DT, SY, ring, st, W, Q, sq, 0, n);
-        acc |= fse_quad_round<false>(DT, SY, ring, st, W, Q, sq, 0, n) << 8;
-        acc |= fse_quad_round<false>(DT, SY, ring, st, W, Q, sq, 0, n) << 16;
-        acc |= fse_quad_round<false>(DT, SY, ring, st, W, Q, sq, 0, n) << 24;
+        uint32_t acc = fse_quad_round<false>(L, ring, o, W, Q, sq, selq, cq, fb, 0, n);
+        acc |= fse_quad_round<false>(L, ring, o, W, Q, sq, selq, cq, fb, 0, n) << 8;
+        acc |= fse_quad_round<false>(L, ring, o, W, Q, sq, selq, cq, fb, 0, n) << 16;
+        acc |= fse_quad_round<false>(L, ring, o, W, Q, sq, selq, cq, fb, 0, n) << 24;
         const uint32_t c = __builtin_amdgcn_perm(quad_xor1(acc), acc, selA);
         return __builtin_amdgcn_perm(quad_xor2(c), c, selB);
     };
     const uint64_t oq = P.outPtr + 4 * q;
     const uint32_t G = ok && n > 19 ? (n - 20) / 16 + 1 : 0u, G4 = G & ~3u;
     uint32_t o0 = 0, o1 = 0, o2 = 0, o3 = 0;
+    // after 4 groups, lane q holds dword q of each; transposed, it holds the
+    // 16 bytes of group q, stored in one instruction
+    const uint64_t o16 = P.outPtr + 16 * q;
     for (uint32_t gi = 0; gi < G4; gi += 4) {
         qstage_put(ring, g);
-        if (gi) put4_dwords(oq + 16 * (uint64_t)(gi - 4), 16, o0, o1, o2, o3, wide);
+        if (gi) put_16(o16 + 16 * (uint64_t)(gi - 4), o0, o1, o2, o3, wide);
         qstage_load(g, (Pb >> 3) - 12, q);
         wave_sync();
         const uint32_t Q0 = Q;
@@ -555,9 +601,10 @@ __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
         o1 = group();
         o2 = group();
         o3 = group();
+        quad_transpose4(o0, o1, o2, o3, q);
         Pb -= Q0 - Q;
     }
-    if (G4) put4_dwords(oq + 16 * (uint64_t)(G4 - 4), 16, o0, o1, o2, o3, wide);
+    if (G4) put_16(o16 + 16 * (uint64_t)(G4 - 4), o0, o1, o2, o3, wide);
     for (uint32_t gi = G4; gi < G; gi++) {
         if ((gi & 3) == 0) {
             qstage_step(ring, g, (Pb >> 3) - 12, q);
@@ -575,7 +622,7 @@ __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
             wave_sync();
         }
         const uint32_t p = 4 * r + q, Q0 = Q;
-        const uint32_t y = fse_quad_round<true>(DT, SY, ring, st, W, Q, sq, p, n);
+        const uint32_t y = fse_quad_round<true>(L, ring, o, W, Q, sq, selq, cq, fb, p, n);
         Pb -= Q0 - Q;
         if (p < n) *(g_uint8*)(uintptr_t)(out + p) = (uint8_t)y;
     }

@@ -107,8 +107,10 @@ static __device__ void wave_fse_spread(uint8_t* cell, const int16_t* normSrc, un
     wave_sync();
 }
 
-// Decoding table in two arrays: sym[u] the cell's symbol, dt[u] = nbBits |
-// newStateBase << 4 (both fit 16 bits for tableLog <= 11).
+// Decoding table in two arrays: sym[u] the cell's symbol, dt[u] = nbBits << 12 |
+// 2 * newStateBase (both fit 16 bits for tableLog <= 11; the low nbBits + 1
+// bits of 2 * newStateBase are zero, so a decoder adds the read bits, times
+// two, with an OR).
 static __device__ void wave_fse_dtable16(uint16_t* dt, uint8_t* sym, const int16_t* normSrc, unsigned maxSym, unsigned tl,
                                          FseBuildHdr& S)
 {
@@ -125,7 +127,7 @@ static __device__ void wave_fse_dtable16(uint16_t* dt, uint8_t* sym, const int16
         if (act) {
             uint32_t ns = base + (uint32_t)__popcll(m & lt);
             uint32_t nb = tl - (uint32_t)ozl_hb32(ns);
-            dt[u] = (uint16_t)(nb | (((ns << nb) - T) << 4));
+            dt[u] = (uint16_t)((nb << 12) | (((ns << nb) - T) << 1));
         }
         wave_sync();
         if (act && (m & lt) == 0) S.next[s] = (uint16_t)(base + (uint32_t)__popcll(m));
