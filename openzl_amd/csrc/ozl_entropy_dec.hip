@@ -277,16 +277,21 @@ __device__ inline uint32_t win_hi(const Win& w) { return (uint32_t)(w.acc >> 32)
 // A quad of lanes decodes one frame of 4 interleaved states
 // (fse_decompress.c:255-338), lane q holding state q: per round every lane
 // looks its state up, the quad adds up its 4 bit counts (each lane's count
-// copied into bytes q..3 of a word, the words summed over the quad with two
-// DPP exchanges, so that byte k holds the bits of states 0..k), and lane q takes
-// its bits below those of the states before it from a 64-bit window that every
-// lane reads from the frame's LDS ring at the quad's bit position. After 4
-// rounds the quad's 16 symbols are transposed across its lanes (two DPP byte
-// permutes) into one dword per lane. Frames of 2 states are decoded by lane 0
-// of their quad, state after state. 6 frames per one-wave workgroup: tables
-// (16-bit entries and the symbol of every cell) and rings take 40 KB, so 4
-// workgroups (one per SIMD) share a CU.
-constexpr int kFQ = 6;
+// subtracted from bytes q..3 of a word, the words summed over the quad with
+// two DPP exchanges, so that byte k holds 63 - the bits of states 0..k), and
+// lane q takes its bits below those of the states before it from a 64-bit
+// window that every lane reads from the frame's LDS ring at the quad's bit
+// position. After 4 rounds the quad's 16 symbols are transposed across its
+// lanes (two DPP byte permutes) into one dword per lane, and after 16 rounds
+// the dwords once more, so that a lane stores 16 consecutive bytes. Frames of
+// 2 states are decoded by lane 0
+// of their quad, state after state. 8 frames per one-wave workgroup: the
+// 16-bit table entries carry an identifier of the cell's symbol in the bits
+// their state base leaves zero (ozl_wave::wave_fse_dtable_ids), so a frame
+// takes its 4 KB table, a 256-byte symbol table and its ring; 4 workgroups
+// (one per SIMD) share a CU, and the 16,384 frames of a 1 GiB batch of
+// 64 KiB blocks run in two rounds of workgroups.
+constexpr int kFQ = 8;
 constexpr int kFRing = 256;
 constexpr int kFRingW = kFRing / 4;
 constexpr int kFLook = 128;     // one-lane frames
@@ -294,17 +299,24 @@ constexpr int kFPhase = 8;      // one-lane frames: rounds between staging steps
 constexpr int kQLook = 192;     // quads: bytes kept staged below the window
 constexpr int kQPhase = 16;     // quads: rounds between staging steps (<= 88 bytes consumed)
 
-// Frame f's decoding table starts at byte 4096 f of dt and its symbols at
-// byte 2048 f of sym, so a lane keeps one LDS byte offset per state: 4096 f +
-// 2 state indexes the table, half of it the symbols.
+// Frame f's decoding table starts at byte 4096 f of dt, so a lane keeps one
+// LDS byte offset per state: 4096 f + 2 state.
 struct FseDecLds {
     // the rings first: their addresses fit the LDS instructions' offsets
     uint32_t ring[kFQ][kFRingW + 4];  // dwords kFRingW, kFRingW+1 mirror 0, 1
     uint16_t dt[kFQ][2048];
-    uint8_t sym[kFQ][2048];
+    uint8_t sys[kFQ][256];
     FseBuildHdr bs;
     ozl_dplan plan[kFQ];
 };
+
+// the symbol of table entry e (wave_fse_dtable_ids): m1 = 2^(nbBits+1) - 1
+__device__ inline uint32_t fse_entry_symbol(const uint8_t* sys, uint32_t e, uint32_t nb, uint32_t m1)
+{
+    const uint32_t id = e & m1;
+    const uint32_t ys = sys[(m1 - 1 + id) & 255];   // in the table whatever nb (read, then chosen)
+    return nb >= 7 ? id : ys;
+}
 
 // A frame's quad stages its stream into the ring 128 bytes per step, lane q
 // loading blocks q and q+4 of the 16-byte blocks below the staged range; each
@@ -388,7 +400,7 @@ __device__ inline uint64_t qwindow(const uint32_t* ring, uint32_t Q)
 // the states take their bits from the window in state order; the window is
 // refilled after each pair of states so that it always holds the <= 22 bits a
 // pair reads.
-__device__ bool fse_decode_lane(const ozl_dplan& P, const uint16_t* DT, const uint8_t* SY, uint32_t* ring)
+__device__ bool fse_decode_lane(const ozl_dplan& P, const uint16_t* DT, const uint8_t* SYS, uint32_t* ring)
 {
     const uint32_t S = P.nbStates, n = P.n, tl = P.tl;
     const uint64_t s = P.srcPtr;
@@ -414,13 +426,14 @@ __device__ bool fse_decode_lane(const ozl_dplan& P, const uint16_t* DT, const ui
         for (uint32_t j = 0; j < S; j++) {
             const uint32_t p = r * S + j;
             const uint32_t d = DT[st[j]];
-            if (p < n) out[p] = SY[st[j]];
+            const uint32_t nbd = d >> 12, m1 = (2u << nbd) - 1u;
+            if (p < n) out[p] = (uint8_t)fse_entry_symbol(SYS, d, nbd, m1);
             // a state updates unless its symbol is among the last S
-            const uint32_t nb = p + S < n ? d >> 12 : 0u;
+            const uint32_t nb = p + S < n ? nbd : 0u;
             const uint32_t bits = __builtin_amdgcn_ubfe(win_hi(w), 32 - nb, nb);
             w.acc <<= nb;
             w.valid -= (int)nb;
-            if (p + S < n) st[j] = ((d & 0xFFFu) >> 1) + bits;
+            if (p + S < n) st[j] = ((d & 0xFFFu & ~m1) >> 1) + bits;
             if (j & 1) win_refill32(w, ring_dword<kFRing>(ring, s + (uint64_t)(int64_t)w.lb - 4));
         }
         if (S & 1) win_refill32(w, ring_dword<kFRing>(ring, s + (uint64_t)(int64_t)w.lb - 4));
@@ -487,30 +500,31 @@ __device__ inline void put_16(uint64_t a, uint32_t d0, uint32_t d1, uint32_t d2,
 // rounds keep the states of the last 4 symbols (fse_decompress.c:287-336).
 // W holds the 64 bits above bit address Qm (the reader sits at Qm + 64); the
 // lane's state is the LDS byte offset o = 4096 f + 2 state. The quad sums, in
-// bytes, 64 minus the bit counts of the states (lane 0 adds the 64s, each
+// bytes, 63 minus the bit counts of the states (lane 0 adds the 63s, each
 // lane subtracts its count from bytes q..3, a byte broadcast by v_perm), so
-// byte k holds 64 - (bits of states 0..k): shifted right by that (mod 64),
-// the window has this lane's bits at bit 0. The table's 2 * newStateBase has
-// zeros where they go, and 4096 f is above it, so the new offset is ORs.
+// byte k holds 63 - (bits of states 0..k): shifted right by that (mod 64),
+// the window has this lane's bits at bits 1..nbBits, where the entry's
+// 2 * newStateBase has zeros (and its symbol identifier, masked off), and
+// 4096 f is above it, so the new offset is an AND-OR.
 template <bool TAIL>
 __device__ inline uint32_t fse_quad_round(const FseDecLds& L, const uint32_t* ring, uint32_t& o, uint64_t& W, uint32_t& Qm,
-                                          uint32_t sq, uint32_t selq, uint32_t cq, uint32_t fb, uint32_t p, uint32_t n)
+                                          uint32_t sq, uint32_t selq, uint32_t cq, uint32_t fb, const uint8_t* sys,
+                                          uint32_t p, uint32_t n)
 {
     const uint32_t e = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const uint8_t*>(&L.dt[0][0]) + o);
-    const uint32_t y = (&L.sym[0][0])[o >> 1];
     const bool upd = !TAIL || p + 4 < n;
-    const uint32_t nb = upd ? e >> 12 : 0u;
+    const uint32_t nbe = e >> 12, m1 = (2u << nbe) - 1u;
+    const uint32_t nb = upd ? nbe : 0u;
     uint32_t v = cq - __builtin_amdgcn_perm(0u, nb, selq);
     v += quad_xor1(v);
-    const uint32_t M = v + quad_xor2(v);   // byte k: 64 - bits of states 0..k
+    const uint32_t M = v + quad_xor2(v);   // byte k: 63 - bits of states 0..k
     const uint32_t X = (uint32_t)(W >> (__builtin_amdgcn_ubfe(M, sq, 8) & 63));
-    uint32_t b2 = (e & 0xFFFu) | fb;
-    __asm__("" : "+v"(b2));   // keeps b2 one value, off the chain (one v_lshl_or_b32 below)
-    const uint32_t no = (__builtin_amdgcn_ubfe(X, 0, nb) << 1) | b2;
+    const uint32_t b2 = (e & (0xFFFu ^ m1)) | fb;
+    const uint32_t no = (X & (m1 ^ 1u)) | b2;
     o = upd ? no : o;
-    Qm += (M >> 24) - 64u;
+    Qm += (M >> 24) - 63u;
     W = qwindow_m(ring, Qm);
-    return y;
+    return fse_entry_symbol(sys, e, nbe, m1);
 }
 
 __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
@@ -532,8 +546,11 @@ __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
     if (!factMask) return;
     for (int f = 0; f < kFQ; f++) {
         if (!((factMask >> f) & 1)) continue;
-        wave_fse_dtable16(L.dt[f], L.sym[f], reinterpret_cast<const int16_t*>(ws.side + (size_t)(b0 + f) * OZL_DSIDE_BYTES),
-                          L.plan[f].aux, L.plan[f].tl, L.bs);
+        // the spread's cells in the table's upper half, its scratch in the lower
+        uint8_t* dtb = reinterpret_cast<uint8_t*>(L.dt[f]);
+        wave_fse_dtable_ids(L.dt[f], L.sys[f], dtb + 2048, dtb,
+                            reinterpret_cast<const int16_t*>(ws.side + (size_t)(b0 + f) * OZL_DSIDE_BYTES), L.plan[f].aux,
+                            L.plan[f].tl, L.bs);
     }
     wave_sync();
     const int f = lane >> 2;
@@ -542,13 +559,14 @@ __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
     const ozl_dplan P = L.plan[f];
     uint32_t* ring = L.ring[f];
     if (P.nbStates != 4) {
-        if (q == 0 && !fse_decode_lane(P, L.dt[f], L.sym[f], ring)) ws.plans[b0 + f].status = OZL_ERR_CORRUPTION;
+        if (q == 0 && !fse_decode_lane(P, L.dt[f], L.sys[f], ring)) ws.plans[b0 + f].status = OZL_ERR_CORRUPTION;
         return;
     }
     const uint32_t n = P.n, tl = P.tl, sq = 8 * q;
     // v_perm selector of the count in bytes q..3 (0: byte 0, 0x0c: zero)
     const uint32_t selq = 0x0c0c0c0cu & ~(0xFFFFFFFFu << sq);
-    const uint32_t cq = q == 0 ? 0x40404040u : 0u;
+    const uint32_t cq = q == 0 ? 0x3F3F3F3Fu : 0u;
+    const uint8_t* sys = L.sys[f];
     const uint32_t fb = 4096u * (uint32_t)f;
     const uint64_t s = P.srcPtr, top = s + P.payLen;
     uint8_t* out = reinterpret_cast<uint8_t*>((uintptr_t)P.outPtr);
@@ -578,10 +596,10 @@ __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
     // and its loads, so that waiting for a step's loads never waits for
     // output stores (both count in vmcnt, which drains in order)
     auto group = [&]() -> uint32_t {
-        uint32_t acc = fse_quad_round<false>(L, ring, o, W, Q, sq, selq, cq, fb, 0, n);
-        acc |= fse_quad_round<false>(L, ring, o, W, Q, sq, selq, cq, fb, 0, n) << 8;
-        acc |= fse_quad_round<false>(L, ring, o, W, Q, sq, selq, cq, fb, 0, n) << 16;
-        acc |= fse_quad_round<false>(L, ring, o, W, Q, sq, selq, cq, fb, 0, n) << 24;
+        uint32_t acc = fse_quad_round<false>(L, ring, o, W, Q, sq, selq, cq, fb, sys, 0, n);
+        acc |= fse_quad_round<false>(L, ring, o, W, Q, sq, selq, cq, fb, sys, 0, n) << 8;
+        acc |= fse_quad_round<false>(L, ring, o, W, Q, sq, selq, cq, fb, sys, 0, n) << 16;
+        acc |= fse_quad_round<false>(L, ring, o, W, Q, sq, selq, cq, fb, sys, 0, n) << 24;
         const uint32_t c = __builtin_amdgcn_perm(quad_xor1(acc), acc, selA);
         return __builtin_amdgcn_perm(quad_xor2(c), c, selB);
     };
@@ -622,7 +640,7 @@ __global__ void __launch_bounds__(64) k_fse_decode(uint32_t B, ozl_dws ws)
             wave_sync();
         }
         const uint32_t p = 4 * r + q, Q0 = Q;
-        const uint32_t y = fse_quad_round<true>(L, ring, o, W, Q, sq, selq, cq, fb, p, n);
+        const uint32_t y = fse_quad_round<true>(L, ring, o, W, Q, sq, selq, cq, fb, sys, p, n);
         Pb -= Q0 - Q;
         if (p < n) *(g_uint8*)(uintptr_t)(out + p) = (uint8_t)y;
     }

@@ -53,19 +53,18 @@ struct FseBuild : FseBuildHdr {
     uint8_t cell[2048];  // spread, when the caller keeps none of its own
 };
 
-// Symbol spread (fse_compress.c:93-155, fse_decompress.c:93-160) with one
-// wave: low-probability symbols take the top cells in symbol order; the k-th
-// occurrence of the other symbols (in symbol order) goes to the k-th position
-// of the sequence j*step mod T that is not above the low-probability cells.
-// Writes the symbol of every cell to `cell`, leaves the counts in S.norm
-// (zero above maxSym) and, per symbol, the exclusive prefix of the counts in
-// S.next (-1 counting as 1) when `prefixAll`, else the count itself (-1 as 1).
-static __device__ void wave_fse_spread(uint8_t* cell, const int16_t* normSrc, unsigned maxSym, unsigned tl, FseBuildHdr& S,
-                                       bool prefixAll)
+// Counts of a spread (fse_compress.c:93-155, fse_decompress.c:93-160):
+// leaves the counts in S.norm (zero above maxSym), per symbol the exclusive
+// prefix of the positive counts in S.cum and, in S.next, the exclusive
+// prefix of all counts (-1 counting as 1) when `prefixAll`, else the count
+// itself (-1 as 1); writes the low-probability symbols (count -1) to the top
+// cells of `cell` in symbol order and returns how many there are.
+static __device__ unsigned wave_fse_counts(uint8_t* cell, const int16_t* normSrc, unsigned maxSym, unsigned tl,
+                                           FseBuildHdr& S, bool prefixAll)
 {
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanes_below();
-    const unsigned T = 1u << tl, mask = T - 1, step = (T >> 1) + (T >> 3) + 3;
+    const unsigned T = 1u << tl;
     for (int s = lane; s < 256; s += 64) S.norm[s] = s <= (int)maxSym ? normSrc[s] : (int16_t)0;
     wave_sync();
     unsigned low = 0, carry = 0, carryAll = 0;
@@ -88,6 +87,21 @@ static __device__ void wave_fse_spread(uint8_t* cell, const int16_t* normSrc, un
         carryAll += __shfl(y, 63, 64);
     }
     wave_sync();
+    return low;
+}
+
+// Symbol spread with one wave: low-probability symbols take the top cells in
+// symbol order (wave_fse_counts); the k-th occurrence of the other symbols (in
+// symbol order) goes to the k-th position of the sequence j*step mod T that is
+// not above the low-probability cells. Writes the symbol of every cell to
+// `cell` and leaves S as wave_fse_counts does.
+static __device__ void wave_fse_spread(uint8_t* cell, const int16_t* normSrc, unsigned maxSym, unsigned tl, FseBuildHdr& S,
+                                       bool prefixAll)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanes_below();
+    const unsigned T = 1u << tl, mask = T - 1, step = (T >> 1) + (T >> 3) + 3;
+    const unsigned low = wave_fse_counts(cell, normSrc, maxSym, tl, S, prefixAll);
     const unsigned high = T - 1 - low;
     unsigned run = 0;
     for (unsigned j0 = 0; j0 < T; j0 += 64) {
@@ -107,56 +121,133 @@ static __device__ void wave_fse_spread(uint8_t* cell, const int16_t* normSrc, un
     wave_sync();
 }
 
-// Decoding table in two arrays: sym[u] the cell's symbol, dt[u] = nbBits << 12 |
-// 2 * newStateBase (both fit 16 bits for tableLog <= 11; the low nbBits + 1
-// bits of 2 * newStateBase are zero, so a decoder adds the read bits, times
-// two, with an OR).
-static __device__ void wave_fse_dtable16(uint16_t* dt, uint8_t* sym, const int16_t* normSrc, unsigned maxSym, unsigned tl,
-                                         FseBuildHdr& S)
+// The same spread without a search per cell: sk[k] (T bytes of scratch) gets
+// the symbol of occurrence k, as the running maximum over markers sk[cum[s]] =
+// s of the symbols with a positive count (cum increases with the symbol), then
+// each cell reads its occurrence's symbol.
+static __device__ void wave_fse_spread_sk(uint8_t* cell, uint8_t* sk, const int16_t* normSrc, unsigned maxSym,
+                                          unsigned tl, FseBuildHdr& S)
 {
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanes_below();
-    const unsigned T = 1u << tl;
-    wave_fse_spread(sym, normSrc, maxSym, tl, S, false);
-    for (unsigned u0 = 0; u0 < T; u0 += 64) {
-        unsigned u = u0 + lane;
-        bool act = u < T;
-        uint32_t s = act ? sym[u] : 0u;
-        uint64_t m = match8(s, act);
-        uint32_t base = act ? S.next[s] : 0u;
-        if (act) {
-            uint32_t ns = base + (uint32_t)__popcll(m & lt);
-            uint32_t nb = tl - (uint32_t)ozl_hb32(ns);
-            dt[u] = (uint16_t)((nb << 12) | (((ns << nb) - T) << 1));
-        }
-        wave_sync();
-        if (act && (m & lt) == 0) S.next[s] = (uint16_t)(base + (uint32_t)__popcll(m));
-        wave_sync();
+    const unsigned T = 1u << tl, mask = T - 1, step = (T >> 1) + (T >> 3) + 3;
+    const unsigned low = wave_fse_counts(cell, normSrc, maxSym, tl, S, false);
+    uint32_t* skw = reinterpret_cast<uint32_t*>(sk);
+    for (unsigned i = lane; i < T / 4; i += 64) skw[i] = 0;
+    wave_sync();
+    for (int c = 0; c < 4; c++) {
+        const int s = c * 64 + lane;
+        if (S.norm[s] > 0) sk[S.cum[s]] = (uint8_t)s;
     }
+    wave_sync();
+    // lane l takes dwords [D l, D l + D) (D = T / 256, at least 1)
+    const unsigned D = T >= 256 ? T / 256 : 1u;
+    const bool mine = D * 4 * (unsigned)lane < T;
+    uint32_t w[8];
+    uint32_t segMax = 0;
+#pragma unroll
+    for (unsigned i = 0; i < 8; i++) {
+        w[i] = (i < D && mine) ? skw[D * lane + i] : 0u;
+        const uint32_t a = w[i] & 255, b = (w[i] >> 8) & 255, c = (w[i] >> 16) & 255, d = w[i] >> 24;
+        const uint32_t m0 = a > b ? a : b, m1 = c > d ? c : d;
+        const uint32_t mm = m0 > m1 ? m0 : m1;
+        segMax = segMax > mm ? segMax : mm;
+    }
+    // maximum over the lanes below (exclusive)
+    uint32_t incl = segMax;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t x = __shfl_up(incl, d, 64);
+        if (lane >= d) incl = incl > x ? incl : x;
+    }
+    uint32_t run = __shfl_up(incl, 1, 64);
+    if (lane == 0) run = 0;
+#pragma unroll
+    for (unsigned i = 0; i < 8; i++) {
+        uint32_t o = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t x = (w[i] >> (8 * k)) & 255;
+            run = run > x ? run : x;
+            o |= run << (8 * k);
+        }
+        if (i < D && mine) skw[D * lane + i] = o;
+    }
+    wave_sync();
+    const unsigned high = T - 1 - low;
+    unsigned base = 0;
+    for (unsigned j0 = 0; j0 < T; j0 += 64) {
+        const unsigned j = j0 + lane;
+        const unsigned pos = (j * step) & mask;
+        const bool v = j < T && pos <= high;
+        const uint64_t m = __ballot(v);
+        if (v) cell[pos] = sk[base + (unsigned)__popcll(m & lt)];
+        base += (unsigned)__popcll(m);
+    }
+    wave_sync();
 }
 
-// FSE_buildDTable (fse_decompress.c:73-176): every cell's next state is its
-// rank among the cells of its symbol (in cell order) plus the symbol's count.
-// Entry: symbol | nbBits << 8 | newStateBase << 16.
-static __device__ void wave_fse_dtable(uint32_t* dt, const int16_t* normSrc, unsigned maxSym, unsigned tl, FseBuild& S)
+// FSE_buildDTable (fse_decompress.c:73-176) with symbol identifiers: every
+// cell's next state is its rank among the cells of its symbol (in cell order)
+// plus the symbol's count; dt[u] = nbBits << 12 | 2 * newStateBase | id, with
+// id in the low nbBits + 1 bits, which 2 * newStateBase leaves zero. A cell
+// of nbBits = k has its next state in [2^(tl-k), 2^(tl-k+1)), so its symbol's
+// count exceeds 2^(tl-k-1) and fewer than 2^(k+1) symbols have such cells: id
+// is the symbol's rank among them (in symbol order) for k <= 6, whose symbol
+// sys[2^(k+1) - 2 + id] holds (254 bytes), and the symbol itself for k >= 7.
+// `cell` (T bytes) may be the upper half of a 2048-entry dt: a cell's symbol
+// is read before the entries written over it. sk is T bytes of scratch
+// (the lower half of such a dt).
+static __device__ void wave_fse_dtable_ids(uint16_t* dt, uint8_t* sys, uint8_t* cell, uint8_t* sk, const int16_t* normSrc,
+                                           unsigned maxSym, unsigned tl, FseBuildHdr& S)
 {
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanes_below();
     const unsigned T = 1u << tl;
-    wave_fse_spread(S.cell, normSrc, maxSym, tl, S, false);
+    wave_fse_spread_sk(cell, sk, normSrc, maxSym, tl, S);
+    // identifiers per symbol: S.cum[s] = id at the larger nbBits | id at the
+    // smaller one << 8 (S.cum is free after the spread)
+    uint32_t base[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) base[k] = 0;
+    for (int c = 0; c < 4; c++) {
+        const int s = c * 64 + lane;
+        const int v = S.norm[s];
+        const uint32_t n = v < 0 ? 1u : (uint32_t)v;
+        const uint32_t kHi = n ? tl - (uint32_t)ozl_hb32(n | 1) : 99u;
+        const uint32_t kLo = (n & (n - 1)) ? kHi - 1 : 99u;
+        uint32_t ids = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 7; k++) {
+            const bool mem = kHi == k || kLo == k;
+            const uint64_t m = __ballot(mem);
+            const uint32_t id = base[k] + (uint32_t)__popcll(m & lt);
+            if (mem) {
+                ids |= k == kHi ? id : id << 8;
+                sys[((2u << k) - 2 + id) & 255] = (uint8_t)s;
+            }
+            base[k] += (uint32_t)__popcll(m);
+        }
+        S.cum[s] = (uint16_t)ids;
+    }
+    wave_sync();
     for (unsigned u0 = 0; u0 < T; u0 += 64) {
         unsigned u = u0 + lane;
         bool act = u < T;
-        uint32_t s = act ? S.cell[u] : 0u;
+        uint32_t s = act ? cell[u] : 0u;
         uint64_t m = match8(s, act);
-        uint32_t base = act ? S.next[s] : 0u;
+        uint32_t nxt = act ? S.next[s] : 0u;
+        uint32_t ids = act ? S.cum[s] : 0u;
+        int v = act ? S.norm[s] : 0;
         if (act) {
-            uint32_t ns = base + (uint32_t)__popcll(m & lt);
-            uint32_t nb = tl - (uint32_t)ozl_hb32(ns);
-            dt[u] = s | (nb << 8) | (((ns << nb) - T) << 16);
+            const uint32_t ns = nxt + (uint32_t)__popcll(m & lt);
+            const uint32_t nb = tl - (uint32_t)ozl_hb32(ns);
+            const uint32_t n = v < 0 ? 1u : (uint32_t)v;
+            const uint32_t kHi = tl - (uint32_t)ozl_hb32(n);
+            const uint32_t id = nb >= 7 ? s : nb == kHi ? (ids & 255) : (ids >> 8);
+            dt[u] = (uint16_t)((nb << 12) | (((ns << nb) - T) << 1) | id);
         }
         wave_sync();
-        if (act && (m & lt) == 0) S.next[s] = (uint16_t)(base + (uint32_t)__popcll(m));
+        if (act && (m & lt) == 0) S.next[s] = (uint16_t)(nxt + (uint32_t)__popcll(m));
         wave_sync();
     }
 }
