@@ -15,6 +15,9 @@
 //   k_pack_encode   flatpack / bitpack payloads, one output byte per lane
 //   k_content_hash  one wavefront per block (ozl_xxh3_wave)
 //   k_frame_size -> k_scan -> k_frame_write -> k_frame_checksum
+#include <stddef.h>
+#include <type_traits>
+
 #include "ozl_internal.h"
 #include "ozl_core.h"
 #include "ozl_xxh3.h"
@@ -663,12 +666,21 @@ __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, co
 // loaded 16 rounds ahead of use (an unrolled register queue), so the global
 // load latency stays off the state chain. Tables take 5 KB per block (40 KB
 // per workgroup, 4 workgroups per CU).
+//
+// In LDS a symbol's transform is repacked so that a round's state update is
+// four instructions: the high half holds c = (maxBitsOut - 1) << (tl + 1) +
+// 2T - minStatePlus, so that ((state << 16) + tt) >> (tl + 17) is
+// maxBitsOut - (state < minStatePlus), the number of bits out (state lies in
+// [T, 2T) and minStatePlus in [T, 2T]); the low half holds the LDS byte offset
+// of the block's state table entry deltaFindState (positive: the tables follow
+// the 8 KB of transforms and deltaFindState >= -T), to which 2 * (state >>
+// nbBitsOut) is added.
 constexpr int kFseGroups = 8;
 constexpr int kFsePipe = 16;
 
 struct FseEncLds {
-    uint16_t st[kFseGroups][2048];
     uint32_t tt[kFseGroups][256];
+    uint16_t st[kFseGroups][2048];
 };
 
 typedef __attribute__((address_space(1))) const uint8_t gc_uint8;
@@ -677,6 +689,15 @@ typedef __attribute__((address_space(1))) uint32_t g_uint32;
 __device__ inline uint32_t sel4(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t i)
 {
     return i == 0 ? a0 : i == 1 ? a1 : i == 2 ? a2 : a3;
+}
+
+// the LDS form of a symbol transform (see above) for block slot f
+__device__ inline uint32_t tt_lds(uint32_t p, uint32_t tl, uint32_t f)
+{
+    const uint32_t T = 1u << tl;
+    const uint32_t c = (((p & 15u) - 1u) << (tl + 1)) + 2 * T - ((p >> 4) & 0x3FFFu);
+    const uint32_t a = (uint32_t)offsetof(FseEncLds, st) + 4096u * f + 2u * ((p >> 18) - 4096u);
+    return (c << 16) | (a & 0xFFFFu);
 }
 
 __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
@@ -699,21 +720,26 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
         const uint4* tab = reinterpret_cast<const uint4*>(ws.tables + (size_t)(blockIdx.x * kFseGroups + f) * OZL_TAB_BYTES);
         const uint32_t nS = (1u << tlf) / 8, nT = (msf + 4) / 4;
         for (uint32_t i = lane; i < nS; i += 64) reinterpret_cast<uint4*>(L.st[f])[i] = tab[i];
-        for (uint32_t i = lane; i < nT; i += 64) reinterpret_cast<uint4*>(L.tt[f])[i] = tab[512 + i];
+        for (uint32_t i = lane; i < nT; i += 64) {
+            const uint4 x = tab[512 + i];
+            reinterpret_cast<uint4*>(L.tt[f])[i] = make_uint4(tt_lds(x.x, tlf, f), tt_lds(x.y, tlf, f),
+                                                             tt_lds(x.z, tlf, f), tt_lds(x.w, tlf, f));
+        }
     }
     ozl_wave::wave_sync();
     if (!active) return;
     gc_uint8* in = (gc_uint8*)(src + off[b]);
     const uint32_t n = p.n, S = p.nbStates, tl = p.tl;
-    const uint16_t* ST = L.st[g] - 4096;   // indexed by (state >> nb) + (tt >> 18)
     const uint32_t* TT = L.tt[g];
-    g_uint32* outw = (g_uint32*)(ws.payload + (size_t)b * ws.payloadCap);
-    // payload words; word capWords + 1 is a sink the payload never reaches
-    // (its tail ends in word capWords at the latest)
+    // payload words of the workgroup's blocks from one base (32-bit offsets):
+    // this block's words start at gw; word capWords + 1 is a sink the payload
+    // never reaches (its tail ends in word capWords at the latest)
+    g_uint32* outw = (g_uint32*)(ws.payload + (size_t)blockIdx.x * kFseGroups * ws.payloadCap);
+    const uint32_t gw = (uint32_t)g * (uint32_t)(ws.payloadCap / 4);
     const uint32_t capWords = (uint32_t)(ws.payloadCap / 4) - 2;
     const bool mine = (uint32_t)q < S;
     // init: the one position among the last S whose state index is q
-    // (FSE_initCState2)
+    // (FSE_initCState2), from the block's transforms in their packed form
     uint32_t state = 0;
     if (mine) {
         uint32_t pinit = 0;
@@ -721,15 +747,18 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
             uint32_t pp = n - 1 - k;
             if ((S - 1 - (pp % S)) % S == (uint32_t)q) pinit = pp;
         }
-        uint32_t t = TT[in[pinit]];
+        const uint32_t* ttg = reinterpret_cast<const uint32_t*>(ws.tables + (size_t)b * OZL_TAB_BYTES + 8192);
+        uint32_t t = ttg[in[pinit]];
         uint32_t dnb = ozl_tt_dnb(t);
         uint32_t nbOut = (dnb + (1u << 15)) >> 16;
         uint32_t v = (nbOut << 16) - dnb;
-        state = ST[(int)(v >> nbOut) + (int)(t >> 18)];
+        state = L.st[g][(int)(v >> nbOut) + ozl_tt_dfs(t)];
     }
     const uint32_t nEmit = n - S;
     const uint32_t kq = mine ? (uint32_t)(q + n) % S : (uint32_t)q;  // this lane's slot in a round
     const uint32_t sq = 8 * kq;
+    // v_perm selector of the bit count in bytes kq..3 (0: byte 0, 0x0c: zero)
+    const uint32_t selq = 0x0c0c0c0cu & ~(0xFFFFFFFFu << sq);
     const uint32_t rounds = (nEmit + S - 1) / S;
     uint32_t qbuf[kFsePipe];
 #pragma unroll
@@ -740,15 +769,21 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
     uint32_t tn = TT[qbuf[0]];          // transform of the next round's symbol
     uint32_t a0 = 0, ab = 0, wpos = 0;  // pending bits (ab < 32) and words written
     // one emission per lane: the state update, then the quad appends the
-    // round's bits in slot order
-    auto emit = [&](uint32_t t, bool doit) {
-        // nbBitsOut = (state + deltaNbBits) >> 16 = maxBits - (state < minStatePlus)
-        const uint32_t nb = doit ? (t & 15u) - (state < ((t >> 4) & 0x3FFFu) ? 1u : 0u) : 0u;
+    // round's bits in slot order. FAST rounds store every lane's word at
+    // wpos + q, complete or not (a word keeps being rewritten until it is
+    // complete, and a batch of them checks the capacity once); the others
+    // store completed words only, the rest to the sink.
+    auto emit = [&](uint32_t t, bool doit, auto fastTag) {
+        constexpr bool FAST = decltype(fastTag)::value;
+        const uint32_t nbx = ((state << 16) + t) >> (tl + 17);
+        const uint32_t nb = FAST || doit ? nbx : 0u;
         const uint32_t val = __builtin_amdgcn_ubfe(state, 0, nb);
-        const uint32_t ns = ST[doit ? (state >> nb) + (t >> 18) : 4096u];
-        state = doit ? ns : state;
+        // (a lane without an emission reads the first table entry instead)
+        const uint32_t at = FAST || doit ? ((state >> nb) << 1) + (t & 0xFFFFu) : (uint32_t)offsetof(FseEncLds, st);
+        const uint32_t ns = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const uint8_t*>(&L) + at);
+        state = FAST || doit ? ns : state;
         // nb in bytes kq..3 (a byte broadcast by v_perm), summed over the quad
-        uint32_t v = __builtin_amdgcn_perm(0u, nb, 0u) << sq;
+        uint32_t v = __builtin_amdgcn_perm(0u, nb, selq);
         v += ozl_wave::quad_xor1(v);
         const uint32_t M = v + ozl_wave::quad_xor2(v);  // byte k: bits of slots 0..k
         const uint32_t excl = __builtin_amdgcn_ubfe(M, sq, 8) - nb;
@@ -761,9 +796,9 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
         const uint64_t T = (((uint64_t)rh << 32) | rl) << ab;
         const uint32_t w0 = a0 | (uint32_t)T, w1 = (uint32_t)(T >> 32), w2 = __builtin_amdgcn_ubfe(rh, 32 - ab, ab);
         const uint32_t nAcc = ab + (M >> 24), ne = nAcc >> 5;
-        // every lane stores: a completed word to its place, else to the
-        // block's sink word past the payload (no branch on the chain)
-        outw[(uint32_t)q < ne && wpos + q < capWords ? wpos + q : capWords + 1] = q == 0 ? w0 : w1;
+        // (a 32-bit byte offset from the workgroup's base: one address VGPR)
+        const uint32_t wi = FAST ? wpos + (uint32_t)q : (uint32_t)q < ne && wpos + q < capWords ? wpos + q : capWords + 1;
+        *(g_uint32*)((__attribute__((address_space(1))) uint8_t*)outw + 4u * (gw + wi)) = q == 0 ? w0 : w1;
         a0 = ne == 0 ? w0 : ne == 1 ? w1 : w2;
         ab = nAcc & 31;
         wpos += ne;
@@ -772,15 +807,17 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
     if (S == 4) {
         // rounds whose 4 emissions all exist, while the queue's next 16 do too:
         // no per-lane conditions, queue loads at immediate offsets
+        // (at most 2 words complete per round: a batch needs 2 * kFsePipe + 4
+        // words of room for its stores)
         const uint32_t RF = nEmit / 4;
-        for (; r0 + 2 * kFsePipe <= RF; r0 += kFsePipe) {
+        for (; r0 + 2 * kFsePipe <= RF && wpos + 2 * kFsePipe + 4 <= capWords; r0 += kFsePipe) {
             gc_uint8* pb = in + (nEmit - 1 - 4 * (r0 + kFsePipe) - kq);
 #pragma unroll
             for (int k = 0; k < kFsePipe; k++) {
                 const uint32_t t = tn;
                 tn = TT[qbuf[(k + 1) % kFsePipe]];
                 qbuf[k] = pb[-4 * k];
-                emit(t, true);
+                emit(t, true, std::true_type{});
             }
         }
     }
@@ -792,7 +829,7 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
             tn = TT[qbuf[(k + 1) % kFsePipe]];
             const uint32_t eNext = e + kFsePipe * S;
             qbuf[k] = in[(mine && eNext < nEmit) ? nEmit - 1 - eNext : 0u];
-            emit(t, mine && e < nEmit);
+            emit(t, mine && e < nEmit, std::false_type{});
         }
     }
     // the states in index order, then the end mark
@@ -806,7 +843,7 @@ __global__ void __launch_bounds__(64) k_fse_encode(const uint8_t* __restrict__ s
         acc |= (uint64_t)sv << accBits;
         accBits += tl;
         if (accBits >= 32) {
-            if (wpos < capWords) outw[wpos] = (uint32_t)acc;
+            if (wpos < capWords) outw[gw + wpos] = (uint32_t)acc;
             wpos++;
             acc >>= 32;
             accBits -= 32;
