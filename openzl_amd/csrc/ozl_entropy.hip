@@ -2,7 +2,9 @@
 //
 // Pipeline for B blocks (each < 100000 bytes, so the graphs do not split it,
 // src/openzl/codecs/entropy/encode_entropy_binding.c:491-519):
-//   k_hist          one workgroup per block: LDS histogram
+//   k_hist          one workgroup per block: LDS histogram (k_hist_hash in
+//                   the batch API: the histogram and the content checksum
+//                   in one read of the block)
 //   k_plan          one wave per block: one lane decides the backend
 //                   (:601-753) and builds the FSE tables or Huffman code, the
 //                   NCount, and the nested graph of the Huffman weights
@@ -13,7 +15,6 @@
 //   k_huf_encode    one workgroup per block: code lengths, a block-wide scan of
 //                   bit offsets in reverse symbol order, bits OR-ed into LDS
 //   k_pack_encode   flatpack / bitpack payloads, one output byte per lane
-//   k_content_hash  one wavefront per block (ozl_xxh3_wave)
 //   k_frame_size -> k_scan -> k_frame_write -> k_frame_checksum
 #include <stddef.h>
 #include <type_traits>
@@ -443,6 +444,120 @@ __global__ void __launch_bounds__(kT) k_hist(const uint8_t* __restrict__ src, co
     block_hist(src + off[b], (uint32_t)(offEnd[b] - off[b]), hc, count);
     uint32_t* cg = reinterpret_cast<uint32_t*>(ws.tables + (size_t)b * OZL_TAB_BYTES + OZL_TAB_COUNT_OFF);
     for (int s = threadIdx.x; s < 256; s += kT) cg[s] = count[s];
+}
+
+// Histogram and content checksum of every block in one read of the input
+// (the batch API; the engine hashes whole inputs): one 256-thread workgroup
+// per block. Each thread takes 16-byte pieces t, t + 256, ...: 16 LDS atomics
+// for the histogram and, when the block starts 16-byte aligned, the
+// piece's XXH3 terms, so that wave w covers 1 KiB blocks w, w + 4, ... and
+// reduces each block's stripe sums (ozl_xxh3_sums_of) into LDS; wave 0 then
+// runs the scramble chain over them and the tail stripes. Unaligned blocks
+// hash with ozl_xxh3_wave after the histogram. The hash is parked in
+// hashOut[b] (the status array, which the frame checksum kernel overwrites
+// after the frame writer has read it).
+__global__ void __launch_bounds__(kT) k_hist_hash(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
+                                                  uint32_t B, ozl_ews ws, uint32_t* __restrict__ hashOut)
+{
+    __shared__ __attribute__((aligned(16))) uint32_t hc[kHistCopies * 256];
+    __shared__ uint32_t count[256];
+    __shared__ uint64_t bsum[100][8];   // blocks are below 100000 bytes: < 98 full KiB
+    const uint32_t b = blockIdx.x;
+    if (b >= B) return;
+    const uint8_t* in = src + off[b];
+    const uint32_t n = (uint32_t)(off[b + 1] - off[b]);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const bool aligned = ((uintptr_t)in & 15) == 0;
+    const uint32_t nbBlocks = n > 240 ? (n - 1) / 1024 : 0u;
+    for (int i = tid; i < kHistCopies * 256; i += kT) hc[i] = 0;
+    __syncthreads();
+    uint32_t* h = hc + (tid & (kHistCopies - 1));
+    if (aligned) {
+        const ozl_xxh3_lane XL = ozl_xxh3_lane_init();
+        const uint32_t nv = n / 16;
+        // whole waves of pieces: a wave's 64 pieces are one 1 KiB block; four
+        // pieces per thread are loaded before any is used
+        constexpr int kU = 4;
+        for (uint32_t c0 = (uint32_t)(tid - lane); c0 < nv; c0 += kU * kT) {
+            uint4 q[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const uint32_t c = c0 + u * kT + lane;
+                q[u] = c < nv ? *reinterpret_cast<const uint4*>(in + 16 * (size_t)c) : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const uint32_t cu = c0 + u * kT;
+                if (cu + lane < nv) {
+                    const uint32_t w[4] = { q[u].x, q[u].y, q[u].z, q[u].w };
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+#pragma unroll
+                        for (int bb = 0; bb < 4; bb++) atomicAdd(&h[kHistCopies * ((w[k] >> (8 * bb)) & 255)], 1u);
+                }
+                const uint32_t blk = cu / 64;
+                if (blk < nbBlocks) {
+                    uint64_t sa, sb;
+                    ozl_xxh3_sums_of(q[u].x | ((uint64_t)q[u].y << 32), q[u].z | ((uint64_t)q[u].w << 32), XL, sa, sb);
+                    if (lane < 4) {
+                        bsum[blk][2 * lane] = sa;
+                        bsum[blk][2 * lane + 1] = sb;
+                    }
+                }
+            }
+        }
+        for (uint32_t i = 16 * nv + tid; i < n; i += kT) atomicAdd(&h[kHistCopies * in[i]], 1u);
+    } else {
+        uint32_t head = (uint32_t)((16 - ((uintptr_t)in & 15)) & 15);
+        if (head > n) head = n;
+        for (uint32_t i = tid; i < head; i += kT) atomicAdd(&h[kHistCopies * in[i]], 1u);
+        const uint32_t nv = (n - head) / 16;
+        const uint4* v = reinterpret_cast<const uint4*>(in + head);
+        for (uint32_t i = tid; i < nv; i += kT) {
+            const uint4 q = v[i];
+            const uint32_t w[4] = { q.x, q.y, q.z, q.w };
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int bb = 0; bb < 4; bb++) atomicAdd(&h[kHistCopies * ((w[k] >> (8 * bb)) & 255)], 1u);
+        }
+        for (uint32_t i = head + nv * 16 + tid; i < n; i += kT) atomicAdd(&h[kHistCopies * in[i]], 1u);
+    }
+    __syncthreads();
+    for (int sy = tid; sy < 256; sy += kT) {
+        const uint4* r = reinterpret_cast<const uint4*>(hc + kHistCopies * sy);
+        uint32_t t = 0;
+#pragma unroll
+        for (int c = 0; c < kHistCopies / 4; c++) t += r[c].x + r[c].y + r[c].z + r[c].w;
+        count[sy] = t;
+    }
+    __syncthreads();
+    uint32_t* cg = reinterpret_cast<uint32_t*>(ws.tables + (size_t)b * OZL_TAB_BYTES + OZL_TAB_COUNT_OFF);
+    for (int sy = tid; sy < 256; sy += kT) cg[sy] = count[sy];
+    if (tid >= 64) return;
+    uint64_t hv;
+    if (n <= 240) {
+        hv = ozl_xxh3_short(in, n);
+    } else if (!aligned) {
+        hv = ozl_xxh3_wave(in, n, nullptr, 0);
+    } else {
+        // the chain over the block sums: lane j keeps accumulators 2j, 2j+1
+        const int j = lane & 3;
+        const ozl_xxh3_lane XL = ozl_xxh3_lane_init();
+        uint64_t A = ozl_xxh3_init_acc(2 * j), Bc = ozl_xxh3_init_acc(2 * j + 1);
+        for (uint32_t k = 0; k < nbBlocks; k++) {
+            A = ozl_xxh3_scr(A + bsum[k][2 * j], XL.c0);
+            Bc = ozl_xxh3_scr(Bc + bsum[k][2 * j + 1], XL.c1);
+        }
+        uint64_t acc = 0;
+        for (int qq = 0; qq < 4; qq++) {
+            const uint64_t a = __shfl(A, qq, 64), bq = __shfl(Bc, qq, 64);
+            if (lane == 2 * qq) acc = a;
+            if (lane == 2 * qq + 1) acc = bq;
+        }
+        hv = ozl_xxh3_finish(in, n, nbBlocks, acc);
+    }
+    if (lane == 0) hashOut[b] = (uint32_t)hv;
 }
 
 // One wave per block: decision / normalisation / Huffman code / nested
@@ -1067,16 +1182,6 @@ __global__ void __launch_bounds__(kT) k_pack_encode(const uint8_t* __restrict__ 
 }
 
 // ------------------------------------------------------------------ checksums and frames
-__global__ void __launch_bounds__(64) k_content_hash(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
-                                                     uint32_t B, ozl_ews ws)
-{
-    __shared__ uint64_t lds[8 * 128];
-    uint32_t b = blockIdx.x;
-    if (b >= B) return;
-    uint64_t h = ozl_xxh3_wave(src + off[b], off[b + 1] - off[b], lds, 128);
-    if (threadIdx.x == 0) ws.plans[b].contentHash = (uint32_t)h;
-}
-
 __device__ inline size_t stored_bytes(const ozl_fstream* st, const int* order, int nbStored)
 {
     size_t t = 0;
@@ -1117,7 +1222,8 @@ __global__ void __launch_bounds__(kT) k_scan_u64(uint64_t* v, uint32_t B)
 }
 
 __global__ void __launch_bounds__(kT) k_frame_write(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
-                                                    uint32_t B, ozl_ews ws, uint8_t* __restrict__ dst)
+                                                    uint32_t B, ozl_ews ws, uint8_t* __restrict__ dst,
+                                                    const uint32_t* __restrict__ contentHash)
 {
     __shared__ uint8_t hdr[512];
     __shared__ ozl_fstream st[12];
@@ -1156,7 +1262,7 @@ __global__ void __launch_bounds__(kT) k_frame_write(const uint8_t* __restrict__ 
         pos += s.size;
     }
     if (threadIdx.x == 0) {
-        uint32_t h = p.contentHash;
+        const uint32_t h = contentHash[b];
         for (int k = 0; k < 4; k++) out[pos + k] = (uint8_t)(h >> (8 * k));
     }
 }
@@ -1181,6 +1287,24 @@ __global__ void __launch_bounds__(64) k_frame_checksum(uint32_t B, ozl_ews ws, u
         for (int k = 0; k < 4; k++) fr[fh + clen + k] = (uint8_t)(h >> (8 * k));
         fr[flen - 1] = 0;
     }
+}
+
+// plan, coder and packed payloads of blocks whose histograms are in place
+int entropy_encode_after_hist(int hufMode, const uint8_t* s, const uint64_t* off, const uint64_t* offEnd, uint32_t B,
+                              const ozl_ews& ws, hipStream_t st)
+{
+    if (hufMode) OZL_LAUNCH("k_plan", k_plan<1>, dim3(B), dim3(64), 0, st, s, off, offEnd, B, ws);
+    else OZL_LAUNCH("k_plan", k_plan<0>, dim3(B), dim3(64), 0, st, s, off, offEnd, B, ws);
+    if (hufMode) OZL_LAUNCH("k_huf_encode", k_huf_encode, dim3(B), dim3(kT), 0, st, s, off, B, ws);
+    else OZL_LAUNCH("k_fse_encode", k_fse_encode, dim3((B + kFseGroups - 1) / kFseGroups), dim3(64), 0, st, s, off, B, ws);
+    OZL_LAUNCH("k_pack_encode", k_pack_encode, dim3(B), dim3(kT), 0, st, s, off, B, ws);
+    OZL_HIP_CHECK(hipGetLastError());
+    return OZL_OK;
+}
+int entropy_encode_after_hist(int hufMode, const uint8_t* s, const uint64_t* off, uint32_t B, const ozl_ews& ws,
+                              hipStream_t st)
+{
+    return entropy_encode_after_hist(hufMode, s, off, off + 1, B, ws, st);
 }
 
 }  // namespace
@@ -1218,12 +1342,14 @@ int ozl_entropy_compress_batch(int hufMode, const void* src, const uint64_t* off
     ozl_ews ws = ozl_ews_carve(workspace, B, maxBlock);
     ws.frameOff = frameOff;
     const uint8_t* s = (const uint8_t*)src;
-    int rc = ozl_entropy_encode_chunks(hufMode, s, off, off + 1, B, maxBlock, ws, st);
+    // the content checksums wait in status until the frame writer has read them
+    OZL_LAUNCH("k_hist_hash", k_hist_hash, dim3(B), dim3(kT), 0, st, s, off, B, ws, (uint32_t*)status);
+    int rc = entropy_encode_after_hist(hufMode, s, off, B, ws, st);
     if (rc) return rc;
-    OZL_LAUNCH("k_content_hash", k_content_hash, dim3(B), dim3(64), 0, st, s, off, B, ws);
     OZL_LAUNCH("k_frame_size", k_frame_size, dim3((B + 63) / 64), dim3(64), 0, st, B, ws);
     OZL_LAUNCH("k_scan_u64", k_scan_u64, dim3(1), dim3(kT), 0, st, frameOff, B);
-    OZL_LAUNCH("k_frame_write", k_frame_write, dim3(B), dim3(kT), 0, st, s, off, B, ws, (uint8_t*)dst);
+    OZL_LAUNCH("k_frame_write", k_frame_write, dim3(B), dim3(kT), 0, st, s, off, B, ws, (uint8_t*)dst,
+               (const uint32_t*)status);
     OZL_LAUNCH("k_frame_checksum", k_frame_checksum, dim3(B), dim3(64), 0, st, B, ws, (uint8_t*)dst, status);
     OZL_HIP_CHECK(hipGetLastError());
     return OZL_OK;
@@ -1237,11 +1363,5 @@ int ozl_entropy_encode_chunks(int hufMode, const uint8_t* s, const uint64_t* off
     if (B == 0) return OZL_OK;
     if (maxBlock >= 100000) return OZL_ERR_PARAMETER_INVALID;
     OZL_LAUNCH("k_hist", k_hist, dim3(B), dim3(kT), 0, st, s, off, offEnd, B, ws);
-    if (hufMode) OZL_LAUNCH("k_plan", k_plan<1>, dim3(B), dim3(64), 0, st, s, off, offEnd, B, ws);
-    else OZL_LAUNCH("k_plan", k_plan<0>, dim3(B), dim3(64), 0, st, s, off, offEnd, B, ws);
-    if (hufMode) OZL_LAUNCH("k_huf_encode", k_huf_encode, dim3(B), dim3(kT), 0, st, s, off, B, ws);
-    else OZL_LAUNCH("k_fse_encode", k_fse_encode, dim3((B + kFseGroups - 1) / kFseGroups), dim3(64), 0, st, s, off, B, ws);
-    OZL_LAUNCH("k_pack_encode", k_pack_encode, dim3(B), dim3(kT), 0, st, s, off, B, ws);
-    OZL_HIP_CHECK(hipGetLastError());
-    return OZL_OK;
+    return entropy_encode_after_hist(hufMode, s, off, offEnd, B, ws, st);
 }

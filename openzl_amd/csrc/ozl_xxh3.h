@@ -243,6 +243,26 @@ __device__ inline int ozl_xxh3_mode(const uint8_t* in, size_t len, size_t b)
     if (((uintptr_t)in & 15) == 0) return 1;
     return (b + 1) * 1024 + 16 <= len ? 2 : 0;
 }
+// v + v of lane l ^ 16, and of lane l ^ 32: v_permlane16_swap /
+// v_permlane32_swap with both operands v hand back the two rows (halves) in
+// swapped order, so their sum is the pair's sum in every lane (no LDS)
+__device__ inline uint32_t ozl_add_xor16(uint32_t v)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return r[0] + r[1];
+}
+__device__ inline uint64_t ozl_add_xor16_64(uint64_t v)
+{
+    const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)v, (uint32_t)v, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(v >> 32), (uint32_t)(v >> 32), false, false);
+    return (((uint64_t)hi[0] << 32) | lo[0]) + (((uint64_t)hi[1] << 32) | lo[1]);
+}
+__device__ inline uint64_t ozl_add_xor32_64(uint64_t v)
+{
+    const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)v, (uint32_t)v, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(v >> 32), (uint32_t)(v >> 32), false, false);
+    return (((uint64_t)hi[0] << 32) | lo[0]) + (((uint64_t)hi[1] << 32) | lo[1]);
+}
 // sums over the 16 stripes of a 1 KiB block for accumulators 2j and 2j+1,
 // in every lane of the pair's group, from the words ozl_xxh3_block_load read
 __device__ inline void ozl_xxh3_sums_of(uint64_t v0, uint64_t v1, const ozl_xxh3_lane& L, uint64_t& sa, uint64_t& sb)
@@ -254,10 +274,10 @@ __device__ inline void ozl_xxh3_sums_of(uint64_t v0, uint64_t v1, const ozl_xxh3
     sb += ozl_dpp64<0x124>(sb);
     sa += ozl_dpp64<0x128>(sa);  // row_ror:8
     sb += ozl_dpp64<0x128>(sb);
-    sa += __shfl_xor(sa, 16, 64);
-    sb += __shfl_xor(sb, 16, 64);
-    sa += __shfl_xor(sa, 32, 64);
-    sb += __shfl_xor(sb, 32, 64);
+    sa = ozl_add_xor16_64(sa);
+    sb = ozl_add_xor16_64(sb);
+    sa = ozl_add_xor32_64(sa);
+    sb = ozl_add_xor32_64(sb);
 }
 __device__ inline void ozl_xxh3_block_sums(const uint8_t* blk, int mode, const ozl_xxh3_lane& L, uint64_t& sa,
                                            uint64_t& sb)

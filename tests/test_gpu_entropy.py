@@ -83,6 +83,23 @@ def test_batch_frames_decode_with_reference(env, mode):
         assert out == b.tobytes()
 
 
+@pytest.mark.parametrize("mode", [0, 1], ids=["fse", "huffman"])
+def test_batch_frames_of_aligned_blocks_match_reference(env, mode):
+    """Blocks that start 16-byte aligned take the histogram + content-checksum
+    pass that hashes 1 KiB blocks across a workgroup's waves: sizes around the
+    short-input limit (240), one XXH3 block (1024) and the tail stripes."""
+    torch, batch = env
+    rng = np.random.default_rng(11)
+    sizes = [16, 240, 256, 1008, 1024, 1040, 2048, 3072, 4096, 16384, 65536, 65552, 99984, 32, 1029]
+    blocks = [zipf_bytes(rng, n, 1.1) if i % 3 else rng.integers(0, 7, n).astype(np.uint8)
+              for i, n in enumerate(sizes)]
+    print(f"aligned entropy batch mode={mode} blocks={len(blocks)}", flush=True)
+    frames = run_batch(torch, batch, blocks, mode)
+    kind = C.FSE if mode == 0 else C.HUFFMAN
+    bad = [(i, b.size) for i, (b, f) in enumerate(zip(blocks, frames)) if f != C.ref_compress(kind, b, C.SERIAL, 1)]
+    assert not bad, bad
+
+
 def gpu_decode(torch, batch, frames, sizes):
     foffs = np.concatenate([[0], np.cumsum([len(f) for f in frames])]).astype(np.int64)
     doffs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
