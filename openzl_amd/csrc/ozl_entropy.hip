@@ -15,7 +15,8 @@
 //   k_huf_encode    one workgroup per block: code lengths, a block-wide scan of
 //                   bit offsets in reverse symbol order, bits OR-ed into LDS
 //   k_pack_encode   flatpack / bitpack payloads, one output byte per lane
-//   k_frame_size -> k_scan -> k_frame_write -> k_frame_checksum
+//   k_frame_size -> k_scan -> k_frame_write_hash (frames and compressed
+//                   checksums in one pass)
 #include <stddef.h>
 #include <type_traits>
 
@@ -1221,71 +1222,169 @@ __global__ void __launch_bounds__(kT) k_scan_u64(uint64_t* v, uint32_t B)
     if (threadIdx.x == 0) v[B] = carry;
 }
 
-__global__ void __launch_bounds__(kT) k_frame_write(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
-                                                    uint32_t B, ozl_ews ws, uint8_t* __restrict__ dst,
-                                                    const uint32_t* __restrict__ contentHash)
+// Frame writer and compressed checksum in one pass over each frame (one
+// 256-thread workgroup per frame). The checksum region (chunk header, stored
+// streams, content checksum) is cut into 16-byte pieces: wave w takes the
+// region's 1 KiB blocks w, w + 4, ..., lane l piece l of the block, gathers
+// it from its source (the header and the content checksum in LDS, a stream
+// in the input, payload or auxiliary area: two aligned 16-byte loads and a
+// funnel shift when the piece lies inside one workspace stream, bytes
+// otherwise), stores it into the frame and adds its XXH3 terms to the
+// block's stripe sums (ozl_xxh3_sums_of). After a barrier wave 0 runs the
+// scramble chain, hashes the tail stripes from the written frame, and writes
+// the checksum and the end byte. The content checksums arrive in `status`
+// (k_hist_hash), which ends up holding each block's status.
+constexpr int kMaxSeg = 14;
+
+struct FrameWriteLds {
+    uint8_t hdr[512];          // frame header + chunk header
+    uint32_t segStart[kMaxSeg];  // region offsets of the streams, then the content checksum
+    uint64_t segSrc[kMaxSeg];
+    uint32_t segSafe[kMaxSeg];   // bytes of the stream a 32-byte over-read may start below
+    uint32_t nSeg, fh, ch, clen, hash;
+    uint64_t bsum[100][8];
+};
+
+// byte x of the checksum region
+__device__ inline uint32_t region_byte(const FrameWriteLds& F, uint32_t x)
 {
-    __shared__ uint8_t hdr[512];
-    __shared__ ozl_fstream st[12];
-    __shared__ int order[12];
-    __shared__ int sInfo[3];
-    uint32_t b = blockIdx.x;
-    if (b >= B) return;
-    ozl_eplan p = ws.plans[b];
-    if (p.status != 0) return;
-    if (threadIdx.x == 0) {
-        ozl_fnode nodes[8];
-        int ns, nbStored;
-        int nn = ozl_entropy_layout(&p, nodes, st, &ns);
-        size_t fh = ozl_write_frame_header(hdr, OZL_T_SERIAL, p.n, 1, 1);
-        size_t ch = ozl_write_chunk_header(hdr + fh, nodes, nn, st, ns, order, &nbStored);
-        sInfo[0] = (int)(fh + ch);
-        sInfo[1] = nbStored;
-        sInfo[2] = (int)fh;
-    }
-    __syncthreads();
-    uint8_t* out = dst + ws.frameOff[b];
-    int hlen = sInfo[0];
-    for (int i = threadIdx.x; i < hlen; i += kT) out[i] = hdr[i];
-    size_t pos = hlen;
-    const uint8_t* in = src + off[b];
-    const uint8_t* aux = ws.aux + (size_t)b * OZL_AUX_BYTES;
-    const uint8_t* pay = ws.payload + (size_t)b * ws.payloadCap;
-    for (int k = 0; k < sInfo[1]; k++) {
-        const ozl_fstream s = st[order[k]];
-        const uint8_t* from = s.src == OZL_SRC_INPUT ? in : s.src == OZL_SRC_PAYLOAD ? pay : aux + s.srcOff;
-        // dword copies (unaligned both sides), then the tail bytes
-        uint8_t* to = out + pos;
-        const uint32_t nd = s.size / 4;
-        for (uint32_t i = threadIdx.x; i < nd; i += kT) ozl_gst4(to + 4 * (size_t)i, ozl_gld4(from + 4 * (size_t)i));
-        for (uint32_t i = 4 * nd + threadIdx.x; i < s.size; i += kT) ozl_gst1(to + i, ozl_gld1(from + i));
-        pos += s.size;
-    }
-    if (threadIdx.x == 0) {
-        const uint32_t h = contentHash[b];
-        for (int k = 0; k < 4; k++) out[pos + k] = (uint8_t)(h >> (8 * k));
-    }
+    if (x < F.ch) return F.hdr[F.fh + x];
+    if (x >= F.clen - 4) return (F.hash >> (8 * (x - (F.clen - 4)))) & 255;
+    uint32_t k = 0;
+    for (uint32_t i = 1; i < F.nSeg; i++) k = x >= F.segStart[i] ? i : k;
+    return ozl_gld1(reinterpret_cast<const uint8_t*>((uintptr_t)F.segSrc[k]) + (x - F.segStart[k]));
 }
 
-// compressed checksum over [chunk start, end of content checksum) and the
-// end-of-frame byte; one wavefront per frame
-__global__ void __launch_bounds__(64) k_frame_checksum(uint32_t B, ozl_ews ws, uint8_t* __restrict__ dst, int* status)
+// the 16 region bytes at x (x + 16 <= clen) as two little-endian words
+__device__ inline void region_piece(const FrameWriteLds& F, uint32_t x, uint64_t& v0, uint64_t& v1)
 {
-    __shared__ uint64_t lds[8 * 128];
-    uint32_t b = blockIdx.x;
+    uint32_t k = 0;
+    for (uint32_t i = 1; i < F.nSeg; i++) k = x >= F.segStart[i] ? i : k;
+    const bool inside = x >= F.ch && x < F.clen - 4 && x + 16 <= (k + 1 < F.nSeg ? F.segStart[k + 1] : F.clen - 4) &&
+                        x - F.segStart[k] < F.segSafe[k];
+    uint32_t d0, d1, d2, d3;
+    if (inside) {
+        const uintptr_t ad = (uintptr_t)F.segSrc[k] + (x - F.segStart[k]);
+        const uint8_t* base = reinterpret_cast<const uint8_t*>(ad & ~(uintptr_t)15);
+        const uint32_t sh = (uint32_t)(ad & 15), bs = 8 * (sh & 3);
+        const ozl_u32x4 lo = ozl_gld16(base), hi = ozl_gld16(base + 16);
+        const uint32_t d[8] = { lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w };
+        switch (sh >> 2) {
+        case 0: ozl_xxh3_pick<0>(d, bs, d0, d1, d2, d3); break;
+        case 1: ozl_xxh3_pick<1>(d, bs, d0, d1, d2, d3); break;
+        case 2: ozl_xxh3_pick<2>(d, bs, d0, d1, d2, d3); break;
+        default: ozl_xxh3_pick<3>(d, bs, d0, d1, d2, d3); break;
+        }
+    } else {
+        uint32_t w[4] = { 0, 0, 0, 0 };
+        for (uint32_t i = 0; i < 16; i++) w[i >> 2] |= region_byte(F, x + i) << (8 * (i & 3));
+        d0 = w[0]; d1 = w[1]; d2 = w[2]; d3 = w[3];
+    }
+    v0 = d0 | ((uint64_t)d1 << 32);
+    v1 = d2 | ((uint64_t)d3 << 32);
+}
+
+__global__ void __launch_bounds__(kT) k_frame_write_hash(const uint8_t* __restrict__ src, const uint64_t* __restrict__ off,
+                                                         uint32_t B, ozl_ews ws, uint8_t* __restrict__ dst,
+                                                         int* __restrict__ status)
+{
+    __shared__ FrameWriteLds F;
+    __shared__ ozl_fstream st[12];
+    __shared__ int order[12];
+    const uint32_t b = blockIdx.x;
     if (b >= B) return;
-    ozl_eplan p = ws.plans[b];
-    if (p.status != 0) { if (threadIdx.x == 0) status[b] = (int)p.status; return; }
-    if (threadIdx.x == 0) status[b] = 0;
-    uint8_t fhbuf[24];
-    size_t fh = ozl_write_frame_header(fhbuf, OZL_T_SERIAL, p.n, 1, 1);
+    const ozl_eplan p = ws.plans[b];
+    const int tid = threadIdx.x, lane = tid & 63;
+    if (p.status != 0) {
+        if (tid == 0) status[b] = (int)p.status;
+        return;
+    }
+    if (tid == 0) {
+        ozl_fnode nodes[8];
+        int ns, nbStored;
+        const int nn = ozl_entropy_layout(&p, nodes, st, &ns);
+        const size_t fh = ozl_write_frame_header(F.hdr, OZL_T_SERIAL, p.n, 1, 1);
+        const size_t ch = ozl_write_chunk_header(F.hdr + fh, nodes, nn, st, ns, order, &nbStored);
+        F.fh = (uint32_t)fh;
+        F.ch = (uint32_t)ch;
+        const uint8_t* in = src + off[b];
+        const uint8_t* aux = ws.aux + (size_t)b * OZL_AUX_BYTES;
+        const uint8_t* pay = ws.payload + (size_t)b * ws.payloadCap;
+        uint32_t pos = (uint32_t)ch, k = 0;
+        for (int i = 0; i < nbStored; i++) {
+            const ozl_fstream s = st[order[i]];
+            if (s.size == 0) continue;
+            const uint8_t* from = s.src == OZL_SRC_INPUT ? in : s.src == OZL_SRC_PAYLOAD ? pay : aux + s.srcOff;
+            F.segStart[k] = pos;
+            F.segSrc[k] = (uint64_t)(uintptr_t)from;
+            // workspace streams may be read up to 31 bytes past their end
+            // (the workspace has room); the caller's input may not
+            F.segSafe[k] = s.src == OZL_SRC_INPUT ? (s.size > 32 ? s.size - 32 : 0u) : s.size;
+            pos += s.size;
+            k++;
+        }
+        if (k == 0) {   // no stored stream: the region is the header and the checksum
+            F.segStart[0] = pos;
+            F.segSrc[0] = 0;
+            F.segSafe[0] = 0;
+            k = 1;
+        }
+        F.nSeg = k;
+        F.clen = pos + 4;
+        F.hash = (uint32_t)status[b];
+    }
+    __syncthreads();
     uint8_t* fr = dst + ws.frameOff[b];
-    size_t flen = ws.frameOff[b + 1] - ws.frameOff[b];
-    size_t clen = flen - fh - 4 - 1;
-    uint64_t h = ozl_xxh3_wave(fr + fh, clen, lds, 128);
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < 4; k++) fr[fh + clen + k] = (uint8_t)(h >> (8 * k));
-        fr[flen - 1] = 0;
+    const uint32_t fh = F.fh, clen = F.clen;
+    for (uint32_t i = tid; i < fh; i += kT) fr[i] = F.hdr[i];
+    uint8_t* rg = fr + fh;
+    const uint32_t nbBlocks = clen > 240 ? (clen - 1) / 1024 : 0u;
+    const ozl_xxh3_lane XL = ozl_xxh3_lane_init();
+    for (uint32_t blk = (uint32_t)(tid >> 6); 1024 * blk < clen; blk += kT / 64) {
+        const uint32_t x = 1024 * blk + 16 * (uint32_t)lane;
+        uint64_t v0 = 0, v1 = 0;
+        if (x + 16 <= clen) {
+            region_piece(F, x, v0, v1);
+            typedef __attribute__((address_space(1))) ozl_u32x4 g_u32x4;
+            ozl_u32x4 v;
+            v.x = (uint32_t)v0; v.y = (uint32_t)(v0 >> 32); v.z = (uint32_t)v1; v.w = (uint32_t)(v1 >> 32);
+            *(g_u32x4*)(uintptr_t)(rg + x) = v;
+        } else {
+            for (uint32_t i = x; i < clen && i < x + 16; i++) ozl_gst1(rg + i, region_byte(F, i));
+        }
+        if (blk < nbBlocks) {
+            uint64_t sa, sb;
+            ozl_xxh3_sums_of(v0, v1, XL, sa, sb);
+            if (lane < 4) {
+                F.bsum[blk][2 * lane] = sa;
+                F.bsum[blk][2 * lane + 1] = sb;
+            }
+        }
+    }
+    __syncthreads();
+    if (tid >= 64) return;
+    uint64_t hv;
+    if (clen <= 240) {
+        hv = ozl_xxh3_short(rg, clen);
+    } else {
+        const int j = lane & 3;
+        uint64_t A = ozl_xxh3_init_acc(2 * j), Bc = ozl_xxh3_init_acc(2 * j + 1);
+        for (uint32_t k = 0; k < nbBlocks; k++) {
+            A = ozl_xxh3_scr(A + F.bsum[k][2 * j], XL.c0);
+            Bc = ozl_xxh3_scr(Bc + F.bsum[k][2 * j + 1], XL.c1);
+        }
+        uint64_t acc = 0;
+        for (int qq = 0; qq < 4; qq++) {
+            const uint64_t a = __shfl(A, qq, 64), bq = __shfl(Bc, qq, 64);
+            if (lane == 2 * qq) acc = a;
+            if (lane == 2 * qq + 1) acc = bq;
+        }
+        hv = ozl_xxh3_finish(rg, clen, nbBlocks, acc);
+    }
+    if (lane == 0) {
+        for (int k = 0; k < 4; k++) rg[clen + k] = (uint8_t)(hv >> (8 * k));
+        rg[clen + 4] = 0;
+        status[b] = 0;
     }
 }
 
@@ -1348,9 +1447,8 @@ int ozl_entropy_compress_batch(int hufMode, const void* src, const uint64_t* off
     if (rc) return rc;
     OZL_LAUNCH("k_frame_size", k_frame_size, dim3((B + 63) / 64), dim3(64), 0, st, B, ws);
     OZL_LAUNCH("k_scan_u64", k_scan_u64, dim3(1), dim3(kT), 0, st, frameOff, B);
-    OZL_LAUNCH("k_frame_write", k_frame_write, dim3(B), dim3(kT), 0, st, s, off, B, ws, (uint8_t*)dst,
-               (const uint32_t*)status);
-    OZL_LAUNCH("k_frame_checksum", k_frame_checksum, dim3(B), dim3(64), 0, st, B, ws, (uint8_t*)dst, status);
+    OZL_LAUNCH("k_frame_write_hash", k_frame_write_hash, dim3(B), dim3(kT), 0, st, s, off, B, ws, (uint8_t*)dst,
+               status);
     OZL_HIP_CHECK(hipGetLastError());
     return OZL_OK;
 }
