@@ -587,37 +587,39 @@ OZL_HD void ozl_huf_sort_range(ozl_hnode* a, int lo0, int hi0, int* stk)
 }
 
 // HUF_buildCTable_wksp (huf_compress.c:731-764): sort (:596-641), tree
-// (:657-704), height limit (:352-500), canonical codes (:706-729).
-// Returns the max code length or -1.
-// `zeroed`: the caller has cleared W->tbl and W->base (a wave does it in
-// parallel on the device). `lengthsOnly`: stop once the leaves W->tbl[1 + k]
-// (k <= W->nonNull, in sorted order) hold their final code lengths; the
-// caller derives nbBits and values from them (canonical codes, :706-729).
-OZL_HD int ozl_huf_build(uint8_t* nbBits, uint16_t* values, const uint32_t* count, unsigned maxSym,
-                         unsigned maxNbBits, ozl_huf_scratch* W, bool zeroed = false, bool lengthsOnly = false)
+// (:657-704), height limit (:352-500), canonical codes (:706-729), in phases
+// that a kernel can interleave with wave-parallel versions of the sort's
+// placement and of the depths (they give the same arrays).
+
+// Sort, first part: every symbol's count and value to its place (buckets in
+// descending order, symbol order within a bucket); leaves W->base / W->curr
+// as the quicksort pass expects them. W->tbl and W->base start at zero.
+OZL_HD void ozl_huf_place(const uint32_t* count, unsigned maxSym, ozl_huf_scratch* W)
+{
+    ozl_hnode* node = W->tbl + 1;
+    uint16_t* base = W->base;
+    uint16_t* curr = W->curr;
+    for (unsigned s = 0; s <= maxSym; s++) base[ozl_huf_bucket(count[s])]++;
+    for (int r = 191; r > 0; r--) base[r - 1] = (uint16_t)(base[r - 1] + base[r]);
+    for (int r = 0; r < 192; r++) curr[r] = base[r];
+    for (unsigned s = 0; s <= maxSym; s++) {
+        uint32_t r = ozl_huf_bucket(count[s]) + 1;
+        uint16_t pos = curr[r]++;
+        node[pos].count = count[s];
+        node[pos].byte = (uint8_t)s;
+    }
+}
+
+// Sort, second part: the buckets of large counts sorted by count
+// (HUF_simpleQuickSort); then the tree of the nonzero leaves. Returns the
+// last nonzero leaf.
+OZL_HD int ozl_huf_sort_tree(unsigned maxSym, ozl_huf_scratch* W)
 {
     ozl_hnode* tbl = W->tbl;
     ozl_hnode* node = tbl + 1;
-    if (!zeroed)
-        for (int k = 0; k < 2 * 256 + 2; k++) { tbl[k].count = 0; tbl[k].parent = 0; tbl[k].byte = 0; tbl[k].nbBits = 0; }
-    {
-        uint16_t* base = W->base;
-        uint16_t* curr = W->curr;
-        if (!zeroed)
-            for (int r = 0; r < 192; r++) base[r] = 0;
-        for (unsigned s = 0; s <= maxSym; s++) base[ozl_huf_bucket(count[s])]++;
-        for (int r = 191; r > 0; r--) base[r - 1] = (uint16_t)(base[r - 1] + base[r]);
-        for (int r = 0; r < 192; r++) curr[r] = base[r];
-        for (unsigned s = 0; s <= maxSym; s++) {
-            uint32_t r = ozl_huf_bucket(count[s]) + 1;
-            uint16_t pos = curr[r]++;
-            node[pos].count = count[s];
-            node[pos].byte = (uint8_t)s;
-        }
-        for (int r = 166; r < 191; r++) {
-            int sz = curr[r] - base[r];
-            if (sz > 1) ozl_huf_sort_range(node + base[r], 0, sz - 1, W->stk);
-        }
+    for (int r = 166; r < 191; r++) {
+        int sz = W->curr[r] - W->base[r];
+        if (sz > 1) ozl_huf_sort_range(node + W->base[r], 0, sz - 1, W->stk);
     }
     const int startNode = 256;
     int nonNull = (int)maxSym;
@@ -636,9 +638,26 @@ OZL_HD int ozl_huf_build(uint8_t* nbBits, uint16_t* values, const uint32_t* coun
         node[n1].parent = node[n2].parent = (uint16_t)nodeNb;
         nodeNb++;
     }
+    W->nonNull = nonNull;
+    return nonNull;
+}
+
+// Depth of every node of the tree (root 256 + nonNull - 1).
+OZL_HD void ozl_huf_depths(ozl_huf_scratch* W)
+{
+    ozl_hnode* node = W->tbl + 1;
+    const int nonNull = W->nonNull, startNode = 256, nodeRoot = startNode + nonNull - 1;
     node[nodeRoot].nbBits = 0;
     for (int k = nodeRoot - 1; k >= startNode; k--) node[k].nbBits = (uint8_t)(node[node[k].parent].nbBits + 1);
     for (int k = 0; k <= nonNull; k++) node[k].nbBits = (uint8_t)(node[node[k].parent].nbBits + 1);
+}
+
+// Height limit (HUF_setMaxHeight) on the leaves' depths; returns the largest
+// code length, or -1.
+OZL_HD int ozl_huf_limit(unsigned maxNbBits, ozl_huf_scratch* W)
+{
+    ozl_hnode* node = W->tbl + 1;
+    const int nonNull = W->nonNull;
     unsigned largest = node[nonNull].nbBits;
     unsigned target = maxNbBits;
     if (largest > target) {
@@ -697,7 +716,29 @@ OZL_HD int ozl_huf_build(uint8_t* nbBits, uint16_t* values, const uint32_t* coun
         largest = target;
     }
     if (largest > 12) return -1;
-    W->nonNull = nonNull;
+    return (int)largest;
+}
+
+// Returns the max code length or -1.
+// `zeroed`: the caller has cleared W->tbl and W->base (a wave does it in
+// parallel on the device). `lengthsOnly`: stop once the leaves W->tbl[1 + k]
+// (k <= W->nonNull, in sorted order) hold their final code lengths; the
+// caller derives nbBits and values from them (canonical codes, :706-729).
+OZL_HD int ozl_huf_build(uint8_t* nbBits, uint16_t* values, const uint32_t* count, unsigned maxSym,
+                         unsigned maxNbBits, ozl_huf_scratch* W, bool zeroed = false, bool lengthsOnly = false)
+{
+    ozl_hnode* tbl = W->tbl;
+    ozl_hnode* node = tbl + 1;
+    if (!zeroed) {
+        for (int k = 0; k < 2 * 256 + 2; k++) { tbl[k].count = 0; tbl[k].parent = 0; tbl[k].byte = 0; tbl[k].nbBits = 0; }
+        for (int r = 0; r < 192; r++) W->base[r] = 0;
+    }
+    ozl_huf_place(count, maxSym, W);
+    const int nonNull = ozl_huf_sort_tree(maxSym, W);
+    ozl_huf_depths(W);
+    const int lim = ozl_huf_limit(maxNbBits, W);
+    if (lim < 0) return -1;
+    const unsigned largest = (unsigned)lim;
     if (lengthsOnly) return (int)largest;
     uint16_t* perRank = W->perRank;
     uint16_t* valPerRank = W->valPerRank;

@@ -561,6 +561,89 @@ __global__ void __launch_bounds__(kT) k_hist_hash(const uint8_t* __restrict__ sr
     if (lane == 0) hashOut[b] = (uint32_t)hv;
 }
 
+// The placement of ozl_huf_place by the wave: the same node order (buckets
+// of descending rank, symbol order within a bucket) and the same W.base /
+// W.curr. W.tbl and W.base start at zero, and so do the 192 words of `hist`.
+// Within a round of 64 symbols, a symbol's rank in its bucket is the number
+// of lower lanes whose bucket matches all 8 bits of its own.
+__device__ inline void wave_huf_place(const uint32_t* count, uint32_t maxSym, ozl_huf_scratch& W, uint32_t* hist)
+{
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t s = lane; s <= maxSym; s += 64) atomicAdd(&hist[ozl_huf_bucket(count[s])], 1u);
+    __syncthreads();
+    {
+        // base[r] = symbols of bucket >= r; lane i holds buckets 3i .. 3i+2
+        const uint32_t h0 = hist[3 * lane], h1 = hist[3 * lane + 1], h2 = hist[3 * lane + 2];
+        uint32_t t = h0 + h1 + h2;
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_down(t, d, 64);
+            if (lane + d < 64) t += o;
+        }
+        const uint16_t b0 = (uint16_t)t, b1 = (uint16_t)(t - h0), b2 = (uint16_t)(t - h0 - h1);
+        W.base[3 * lane] = b0;
+        W.base[3 * lane + 1] = b1;
+        W.base[3 * lane + 2] = b2;
+        W.curr[3 * lane] = b0;
+        W.curr[3 * lane + 1] = b1;
+        W.curr[3 * lane + 2] = b2;
+    }
+    __syncthreads();
+    ozl_hnode* node = W.tbl + 1;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t s0 = 0; s0 <= maxSym; s0 += 64) {
+        const uint32_t s = s0 + lane;
+        const bool act = s <= maxSym;
+        const uint32_t c = act ? count[s] : 0u;
+        const uint32_t r = ozl_huf_bucket(c) + 1;
+        const uint64_t m = ozl_wave::match8(r, act);
+        if (act) {
+            const uint32_t pos = W.curr[r] + (uint32_t)__popcll(m & below);
+            node[pos].count = c;
+            node[pos].byte = (uint8_t)s;
+        }
+        __syncthreads();
+        if (act && (m >> lane) == 1ull) W.curr[r] = (uint16_t)(W.curr[r] + __popcll(m));
+        __syncthreads();
+    }
+}
+
+// The depths of ozl_huf_depths by the wave: pointer jumping over the
+// internal nodes (word j of `P` = depth << 16 | the internal ancestor it
+// has reached, for node 256 + j; the root points at itself with depth 0),
+// then every leaf one below its parent. 256 words of `P`.
+__device__ inline void wave_huf_depths(ozl_huf_scratch& W, uint32_t* P)
+{
+    ozl_hnode* node = W.tbl + 1;
+    const int nonNull = W.nonNull;
+    const uint32_t root = (uint32_t)nonNull - 1u, lane = threadIdx.x;
+    uint32_t v[4];
+    for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t j = lane + 64 * i;
+        v[i] = j < root ? (1u << 16) | (uint32_t)(node[256 + j].parent - 256) : root;
+    }
+    // depths of internal nodes are below 255: at most 8 jumps
+    for (;;) {
+        bool more = false;
+        for (uint32_t i = 0; i < 4; i++) {
+            P[lane + 64 * i] = v[i];
+            more |= (v[i] & 0xFFFFu) != root;
+        }
+        __syncthreads();
+        if (!__any(more)) break;
+        for (uint32_t i = 0; i < 4; i++) {
+            const uint32_t w = P[v[i] & 0xFFFFu];
+            v[i] = ((v[i] & 0xFFFF0000u) + (w & 0xFFFF0000u)) | (w & 0xFFFFu);
+        }
+        __syncthreads();
+    }
+    for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t j = lane + 64 * i;
+        if (j <= root) node[256 + j].nbBits = (uint8_t)(v[i] >> 16);
+        if ((int)j <= nonNull) node[j].nbBits = (uint8_t)((P[node[j].parent - 256] >> 16) + 1);
+    }
+    __syncthreads();
+}
+
 // One wave per block: decision / normalisation / Huffman code / nested
 // weights graph (lane 0, ozl_core.h), FSE encoding table (the wave,
 // ozl_wave::wave_fse_ctable, straight into the block's table area). Small LDS
@@ -646,15 +729,11 @@ __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, co
             if (tl > 11) p.status = OZL_ERR_GENERIC;
             break;   // normalisation and NCount by the wave, below
         }
-        case OZL_BK_HUF: {
-            unsigned tl = ozl_fse_optimal_tablelog(OZL_HUF_DEFAULT_TABLELOG, n, maxSym, 1);
-            // code lengths of the sorted leaves; the wave derives the codes
-            int mx = ozl_huf_build(L.hlen, L.hval, L.count, maxSym, tl, &L.hs, true, true);
-            if (mx < 0) { p.status = OZL_ERR_GENERIC; break; }
-            p.tl = (uint8_t)mx;
+        case OZL_BK_HUF:
+            // the code length limit; the code itself follows
+            p.tl = (uint8_t)ozl_fse_optimal_tablelog(OZL_HUF_DEFAULT_TABLELOG, n, maxSym, 1);
             p.nbStates = n > 1000 ? 1 : 0;
-            break;   // canonical codes, weights and their nested graph follow
-        }
+            break;
         default: break;
         }
     }
@@ -677,6 +756,22 @@ __global__ void __launch_bounds__(64) k_plan(const uint8_t* __restrict__ src, co
             static_assert(OZL_NCOUNT_WORDS * 4 <= sizeof(ozl_wave::FseBuild), "ncount words");
             const uint32_t bytes = wave_fse_ncount(L, L.aux, maxSym, tl, reinterpret_cast<uint32_t*>(&L.fb));
             if (threadIdx.x == 0) L.plan.aux = (uint16_t)bytes;
+        }
+        __syncthreads();
+    }
+    if (HUF && L.plan.bk == OZL_BK_HUF) {
+        // code lengths of the sorted leaves (huf_compress.c:731-764): the
+        // sort's placement and the depths by the wave, the sort of the large
+        // buckets, the tree and the height limit by lane 0
+        wave_huf_place(L.count, L.plan.maxSym, L.hs, L.wcount);
+        if (threadIdx.x == 0) ozl_huf_sort_tree(L.plan.maxSym, &L.hs);
+        __syncthreads();
+        wave_huf_depths(L.hs, L.wcount);
+        for (int i = threadIdx.x; i < 256; i += 64) L.wcount[i] = 0;
+        if (threadIdx.x == 0) {
+            const int mx = ozl_huf_limit(L.plan.tl, &L.hs);
+            if (mx < 0) L.plan.status = OZL_ERR_GENERIC;
+            else L.plan.tl = (uint8_t)mx;
         }
         __syncthreads();
     }
