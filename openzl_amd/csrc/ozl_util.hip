@@ -61,66 +61,90 @@ __device__ inline void xxh3_block_sums(const uint8_t* in, size_t nbBlocks, uint6
     sums[blk * 8 + i] = s;
 }
 
-// ... then the scramble chain, the tail and the merge, with one wave per
-// accumulator word: every value is uniform
-// across the wave, so the scramble chain runs on the scalar unit (dependent
-// scalar instructions issue back to back, the 64-bit multiply takes two
-// 32-bit ones) and the block sums come through scalar loads, read eight
-// blocks ahead. Eight waves per buffer; wave 0 merges.
+// ... then the scramble chain, the tail and the merge by one wave: lane i
+// (mod 8) runs accumulator word i. The chain's vector steps are a 64-bit add
+// (one instruction), a shift and a three-way xor, and one 32 x 32 -> 64-bit
+// multiply-add for the product with the 32-bit prime; the block sums come
+// through per-lane loads issued a batch of 32 blocks ahead, which covers the
+// memory latency (loads complete in order).
 __constant__ uint64_t kXxh3Init[8] = { OZL_P32_3, OZL_P64_1, OZL_P64_2, OZL_P64_3,
                                        OZL_P64_4, OZL_P32_2, OZL_P64_5, OZL_P32_1 };
 
-__device__ inline void xxh3_finish_waves(const uint8_t* __restrict__ in, size_t len, const uint64_t* __restrict__ sums,
-                                         uint64_t* __restrict__ out, uint64_t* accs)
+__device__ __attribute__((always_inline)) inline void xxh3_finish_wave(const uint8_t* __restrict__ in, size_t len, const uint64_t* __restrict__ sums,
+                                        uint64_t* __restrict__ out, uint64_t* accs)
 {
     if (len <= 240) {
         if (threadIdx.x == 0) *out = ozl_xxh3_short(in, len);
         return;
     }
-    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int w = (int)(threadIdx.x & 7);
     uint64_t acc = kXxh3Init[w];
     const size_t nbBlocks = (len - 1) / 1024;
     const uint64_t sec = ozl_sec64(192 - 64 + 8 * w);
     const uint64_t* sp = sums + w;
-    constexpr int kA = 8;
+    const uint32_t secLo = (uint32_t)sec, secHi = (uint32_t)(sec >> 32);
+    // acc = ((acc + v) ^ ((acc + v) >> 47) ^ sec) * P32_1 in 32-bit halves:
+    // the high half's product runs beside the low half's shift and xor
     auto scr = [&](uint64_t v) {
-        uint64_t a2 = acc + v;
-        a2 ^= a2 >> 47;
-        a2 ^= sec;
-        acc = a2 * OZL_P32_1;
+        const uint64_t a = acc + v;
+        const uint32_t lo = (uint32_t)a, hi = (uint32_t)(a >> 32);
+        const uint32_t l2 = lo ^ (hi >> 15) ^ secLo;
+        const uint32_t h2 = (hi ^ secHi) * (uint32_t)OZL_P32_1;
+        acc = (uint64_t)l2 * (uint32_t)OZL_P32_1 + ((uint64_t)h2 << 32);
     };
-    size_t b = 0;
-    if (nbBlocks >= (size_t)kA) {
-        // lane k of the wave holds the sum of block b + k. The sums come
-        // through vector loads, which complete in order: the next batch's
-        // load stays in flight while this batch is scrambled (a scalar load
-        // could only be waited for together with every other one).
-        // (every lane loads, lane k and k + 8j the same sum: no branch)
-        const size_t lane = threadIdx.x & (kA - 1);
-        uint64_t cur = sp[8 * lane];
-        for (; b + 2 * kA <= nbBlocks; b += kA) {
-            const uint64_t nxt = sp[8 * (b + kA + lane)];
+    constexpr int kB = 32;
+    // the block count is uniform: kept in a scalar register, so that the
+    // batch loop's branches are scalar ones
+    const uint32_t nb = (uint32_t)__builtin_amdgcn_readfirstlane((int)nbBlocks);
+    if (nb > 0) {
+        // two batches of sums in flight at a time: a batch is scrambled while
+        // the next one loads. Loads of blocks past the end read the last one
+        // again (in bounds, never scrambled): no branch around a load.
+        const uint32_t last = nb - 1;
+        uint64_t A[kB], B[kB];
+        auto load = [&](uint64_t* d, uint32_t b0) {
 #pragma unroll
-            for (int k = 0; k < kA; k++) {
-                const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)cur, k);
-                const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(cur >> 32), k);
-                scr(((uint64_t)hi << 32) | lo);
+            for (int k = 0; k < kB; k++) {
+                const uint32_t x = b0 + (uint32_t)k;
+                d[k] = sp[8 * (size_t)(x < last ? x : last)];
             }
-            cur = nxt;
-        }
+        };
+        auto run = [&](const uint64_t* v) {
 #pragma unroll
-        for (int k = 0; k < kA; k++) {
-            const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)cur, k);
-            const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(cur >> 32), k);
-            scr(((uint64_t)hi << 32) | lo);
+            for (int k = 0; k < kB; k++) scr(v[k]);
+        };
+        load(A, 0);
+        uint32_t b = 0;
+        for (;;) {
+            if (b + kB > nb) break;
+            load(B, b + kB);
+            run(A);
+            b += kB;
+            if (b + kB > nb) {
+#pragma unroll
+                for (int k = 0; k < kB; k++) A[k] = B[k];
+                break;
+            }
+            load(A, b + kB);
+            run(B);
+            b += kB;
         }
-        b += kA;
+        // the last nb mod 32 blocks (in A), in pieces of 16, 8, 4, 2, 1
+        const uint32_t r = nb - b;
+#pragma unroll
+        for (int piece = 16; piece >= 1; piece >>= 1) {
+            if (r & piece) {
+#pragma unroll
+                for (int k = 0; k < piece; k++) scr(A[k]);
+#pragma unroll
+                for (int k = 0; k + piece < kB; k++) A[k] = A[k + piece];
+            }
+        }
     }
-    for (; b < nbBlocks; b++) scr(sp[8 * b]);
     const size_t nbStripes = ((len - 1) - 1024 * nbBlocks) / 64;
     for (size_t s = 0; s < nbStripes; s++) acc += ozl_stripe_term(in + nbBlocks * 1024 + s * 64, (int)s * 8, w);
     acc += ozl_stripe_term(in + len - 64, 192 - 64 - 7, w);
-    if ((threadIdx.x & 63) == 0) accs[w] = acc;
+    if (threadIdx.x < 8) accs[w] = acc;
     __syncthreads();
     if (threadIdx.x == 0) *out = ozl_xxh3_merge(accs, len);
 }
@@ -131,11 +155,11 @@ __global__ void __launch_bounds__(kT) k_xxh3_blocks(const uint8_t* __restrict__ 
     xxh3_block_sums(in, nbBlocks, sums, (size_t)blockIdx.x * (kT / 8) + (threadIdx.x >> 3), threadIdx.x & 7);
 }
 
-__global__ void __launch_bounds__(512) k_xxh3_finish(const uint8_t* __restrict__ in, size_t len,
-                                                     const uint64_t* __restrict__ sums, uint64_t* __restrict__ out)
+__global__ void __launch_bounds__(64) k_xxh3_finish(const uint8_t* __restrict__ in, size_t len,
+                                                    const uint64_t* __restrict__ sums, uint64_t* __restrict__ out)
 {
     __shared__ uint64_t accs[8];
-    xxh3_finish_waves(in, len, sums, out, accs);
+    xxh3_finish_wave(in, len, sums, out, accs);
 }
 
 // many buffers: grid.y = buffer (block sums), grid.x = buffer (chains)
@@ -147,12 +171,12 @@ __global__ void __launch_bounds__(kT) k_xxh3_blocks_n(const ozl_xjob* __restrict
                     (size_t)blockIdx.x * (kT / 8) + (threadIdx.x >> 3), threadIdx.x & 7);
 }
 
-__global__ void __launch_bounds__(512) k_xxh3_finish_n(const ozl_xjob* __restrict__ jobs,
-                                                       const uint64_t* __restrict__ sums, uint64_t* __restrict__ out)
+__global__ void __launch_bounds__(64) k_xxh3_finish_n(const ozl_xjob* __restrict__ jobs,
+                                                      const uint64_t* __restrict__ sums, uint64_t* __restrict__ out)
 {
     __shared__ uint64_t accs[8];
     const ozl_xjob j = jobs[blockIdx.x];
-    xxh3_finish_waves((const uint8_t*)(uintptr_t)j.ptr, j.len, sums + j.sumOff, out + blockIdx.x, accs);
+    xxh3_finish_wave((const uint8_t*)(uintptr_t)j.ptr, j.len, sums + j.sumOff, out + blockIdx.x, accs);
 }
 
 // frame tails: the low 4 bytes of hashes[k] (little endian) at dst + at[k]
@@ -258,7 +282,7 @@ int ozl_xxh3_device(const uint8_t* data, size_t len, uint64_t* dOut, void* scrat
     if (scratchSize < ozl_xxh3_scratch_size(len)) return OZL_ERR_INTERNAL_BUFFER;
     uint64_t* sums = (uint64_t*)scratch;
     if (nbBlocks) OZL_LAUNCH("k_xxh3_blocks", k_xxh3_blocks, dim3((unsigned)((nbBlocks + 31) / 32)), dim3(kT), 0, st, data, nbBlocks, sums);
-    OZL_LAUNCH("k_xxh3_finish", k_xxh3_finish, dim3(1), dim3(512), 0, st, data, len, sums, dOut);
+    OZL_LAUNCH("k_xxh3_finish", k_xxh3_finish, dim3(1), dim3(64), 0, st, data, len, sums, dOut);
     OZL_HIP_CHECK(hipGetLastError());
     return OZL_OK;
 }
@@ -279,7 +303,7 @@ int ozl_xxh3_device_batch(const ozl_xjob* jobs, uint32_t count, size_t maxLen, u
             OZL_LAUNCH("k_xxh3_blocks_n", k_xxh3_blocks_n, dim3((unsigned)((maxBlocks + 31) / 32), cnt), dim3(kT), 0, st,
                        jobs + y0, sums);
         }
-    OZL_LAUNCH("k_xxh3_finish_n", k_xxh3_finish_n, dim3(count), dim3(512), 0, st, jobs, sums, dOut);
+    OZL_LAUNCH("k_xxh3_finish_n", k_xxh3_finish_n, dim3(count), dim3(64), 0, st, jobs, sums, dOut);
     OZL_HIP_CHECK(hipGetLastError());
     return OZL_OK;
 }
