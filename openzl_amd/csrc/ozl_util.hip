@@ -51,14 +51,33 @@ __global__ void __launch_bounds__(kT) k_gather(const uint8_t* const* __restrict_
 }
 
 // XXH3 of a long buffer: per 1 KiB block, the 8 accumulator sums of its 16
-// stripes (independent across blocks) ...
-__device__ inline void xxh3_block_sums(const uint8_t* in, size_t nbBlocks, uint64_t* sums, size_t blk, int i)
+// stripes (independent across blocks), by one wave per 8 blocks: lane l
+// reads bytes 16 l .. 16 l + 15 of a block (stripe l / 4, words 2 (l % 4)
+// and 2 (l % 4) + 1) with 16-byte loads, all 8 blocks' loads in flight
+// together, and the 16 stripes' terms are summed across lanes
+// (ozl_xxh3_sums_of) ...
+static_assert(kT == 256, "a block-sum workgroup covers 32 blocks: 4 waves of 8");
+__device__ inline void xxh3_block_sums(const uint8_t* in, size_t len, uint64_t* sums, size_t b0)
 {
-    if (blk >= nbBlocks) return;
-    const uint8_t* p = in + blk * 1024;
-    uint64_t s = 0;
-    for (int st = 0; st < 16; st++) s += ozl_stripe_term(p + st * 64, st * 8, i);
-    sums[blk * 8 + i] = s;
+    const size_t nbBlocks = len > 240 ? (len - 1) / 1024 : 0;
+    if (b0 >= nbBlocks) return;
+    const ozl_xxh3_lane L = ozl_xxh3_lane_init();
+    const uint32_t m = nbBlocks - b0 < 8 ? (uint32_t)(nbBlocks - b0) : 8u;
+    uint64_t v0[8], v1[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++)
+        if (k < m) ozl_xxh3_block_load(in + (b0 + k) * 1024, ozl_xxh3_mode(in, len, b0 + k), v0[k], v1[k]);
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++) {
+        if (k < m) {
+            uint64_t sa, sb;
+            ozl_xxh3_sums_of(v0[k], v1[k], L, sa, sb);
+            if ((threadIdx.x & 63) < 4) {
+                sums[(b0 + k) * 8 + 2 * L.j] = sa;
+                sums[(b0 + k) * 8 + 2 * L.j + 1] = sb;
+            }
+        }
+    }
 }
 
 // ... then the scramble chain, the tail and the merge by one wave: lane i
@@ -149,10 +168,10 @@ __device__ __attribute__((always_inline)) inline void xxh3_finish_wave(const uin
     if (threadIdx.x == 0) *out = ozl_xxh3_merge(accs, len);
 }
 
-__global__ void __launch_bounds__(kT) k_xxh3_blocks(const uint8_t* __restrict__ in, size_t nbBlocks,
+__global__ void __launch_bounds__(kT) k_xxh3_blocks(const uint8_t* __restrict__ in, size_t len,
                                                     uint64_t* __restrict__ sums)
 {
-    xxh3_block_sums(in, nbBlocks, sums, (size_t)blockIdx.x * (kT / 8) + (threadIdx.x >> 3), threadIdx.x & 7);
+    xxh3_block_sums(in, len, sums, (size_t)blockIdx.x * 32 + 8 * (threadIdx.x >> 6));
 }
 
 __global__ void __launch_bounds__(64) k_xxh3_finish(const uint8_t* __restrict__ in, size_t len,
@@ -166,9 +185,7 @@ __global__ void __launch_bounds__(64) k_xxh3_finish(const uint8_t* __restrict__ 
 __global__ void __launch_bounds__(kT) k_xxh3_blocks_n(const ozl_xjob* __restrict__ jobs, uint64_t* __restrict__ sums)
 {
     const ozl_xjob j = jobs[blockIdx.y];
-    const size_t nbBlocks = j.len > 240 ? (j.len - 1) / 1024 : 0;
-    xxh3_block_sums((const uint8_t*)(uintptr_t)j.ptr, nbBlocks, sums + j.sumOff,
-                    (size_t)blockIdx.x * (kT / 8) + (threadIdx.x >> 3), threadIdx.x & 7);
+    xxh3_block_sums((const uint8_t*)(uintptr_t)j.ptr, j.len, sums + j.sumOff, (size_t)blockIdx.x * 32 + 8 * (threadIdx.x >> 6));
 }
 
 __global__ void __launch_bounds__(64) k_xxh3_finish_n(const ozl_xjob* __restrict__ jobs,
@@ -281,7 +298,7 @@ int ozl_xxh3_device(const uint8_t* data, size_t len, uint64_t* dOut, void* scrat
     size_t nbBlocks = len > 240 ? (len - 1) / 1024 : 0;
     if (scratchSize < ozl_xxh3_scratch_size(len)) return OZL_ERR_INTERNAL_BUFFER;
     uint64_t* sums = (uint64_t*)scratch;
-    if (nbBlocks) OZL_LAUNCH("k_xxh3_blocks", k_xxh3_blocks, dim3((unsigned)((nbBlocks + 31) / 32)), dim3(kT), 0, st, data, nbBlocks, sums);
+    if (nbBlocks) OZL_LAUNCH("k_xxh3_blocks", k_xxh3_blocks, dim3((unsigned)((nbBlocks + 31) / 32)), dim3(kT), 0, st, data, len, sums);
     OZL_LAUNCH("k_xxh3_finish", k_xxh3_finish, dim3(1), dim3(64), 0, st, data, len, sums, dOut);
     OZL_HIP_CHECK(hipGetLastError());
     return OZL_OK;
