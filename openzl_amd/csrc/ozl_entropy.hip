@@ -1130,30 +1130,54 @@ __global__ void __launch_bounds__(kT) k_huf_encode(const uint8_t* __restrict__ s
     const uint32_t seg = x4 ? (n + 3) / 4 : n;
     const int nbSeg = x4 ? 4 : 1;
     uint32_t opos = x4 ? 6 : 0;
-    __syncthreads();
-    for (int k = 0; k < nbSeg; k++) {
+    // this lane's symbols of segment k: positions [lo, hi) of the stream,
+    // coded downward; its dwords (positions hi-4(i+1), coded from the top)
+    // stay in registers between the two passes, and the < 4 bytes below them
+    // ([lo, low)) are read again
+    struct SegGeom {
+        uint32_t len, hi, lo, nq, low;
+        const uint8_t* base;
+    };
+    auto geom = [&](int k) {
+        SegGeom g;
         const uint32_t s0 = k * seg;
-        const uint32_t len = (k < nbSeg - 1) ? seg : n - s0;
-        const uint32_t per = ((len + kT - 1) / kT + 3) & ~3u;
-        // this lane's symbols: positions [lo, hi) of the stream, coded downward
+        g.len = (k < nbSeg - 1) ? seg : n - s0;
+        const uint32_t per = ((g.len + kT - 1) / kT + 3) & ~3u;
         const uint32_t hiR = threadIdx.x * per, loR = hiR + per;
-        const uint32_t hi = hiR < len ? len - hiR : 0u, lo = loR < len ? len - loR : 0u;
-        const uint8_t* base = in + s0;
-        // the thread's dwords (positions hi-4(i+1), coded from the top) stay
-        // in registers between the two passes; the < 4 bytes below them are
-        // read again
-        const uint32_t nq = (hi - lo) / 4, low = lo + (hi - lo) % 4;   // [lo, low): tail bytes
-        uint32_t dq[kHufRegDwords];
+        g.hi = hiR < g.len ? g.len - hiR : 0u;
+        g.lo = loR < g.len ? g.len - loR : 0u;
+        g.base = in + s0;
+        g.nq = (g.hi - g.lo) / 4;
+        g.low = g.lo + (g.hi - g.lo) % 4;
+        return g;
+    };
+    auto load = [&](const SegGeom& g, uint32_t* d) {
+#pragma unroll
+        for (int i = 0; i < kHufRegDwords; i++)
+            d[i] = (uint32_t)i < g.nq ? *reinterpret_cast<const uint32_t*>(g.base + g.hi - 4 * (i + 1)) : 0u;
+    };
+    // the next segment's dwords are loaded while this one is packed and
+    // stored
+    uint32_t dq[kHufRegDwords], dn[kHufRegDwords];
+    load(geom(0), dq);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (k >= nbSeg) break;
+        const SegGeom g = geom(k);
+        const uint32_t len = g.len, hi = g.hi, lo = g.lo, nq = g.nq, low = g.low;
+        const uint8_t* base = g.base;
+        (void)hi;
         uint32_t mine = 0;
 #pragma unroll
         for (int i = 0; i < kHufRegDwords; i++) {
-            const uint32_t d = (uint32_t)i < nq ? *reinterpret_cast<const uint32_t*>(base + hi - 4 * (i + 1)) : 0u;
-            dq[i] = d;
+            const uint32_t d = dq[i];
             if ((uint32_t)i < nq)
                 mine += (sCode[d >> 24] >> 16) + (sCode[(d >> 16) & 255] >> 16) + (sCode[(d >> 8) & 255] >> 16) +
                         (sCode[d & 255] >> 16);
         }
         for (uint32_t q = low; q > lo; q--) mine += sCode[base[q - 1]] >> 16;
+        if (k + 1 < nbSeg) load(geom(k + 1), dn);
         uint32_t total;
         const uint32_t bit = block_scan_excl<uint32_t>(mine, sScan, total);
         const uint32_t words = (total + 1 + 31) / 32;
@@ -1231,6 +1255,8 @@ __global__ void __launch_bounds__(kT) k_huf_encode(const uint8_t* __restrict__ s
         }
         opos += bytes;
         __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kHufRegDwords; i++) dq[i] = dn[i];
     }
     if (threadIdx.x == 0) ws.plans[b].payload = opos;
 }
