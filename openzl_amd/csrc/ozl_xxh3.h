@@ -299,20 +299,49 @@ __device__ inline uint64_t ozl_xxh3_init_acc(int i)
          : i == 4 ? OZL_P64_4 : i == 5 ? OZL_P32_2 : i == 6 ? OZL_P64_5 : OZL_P32_1;
 }
 // lanes 0..7 hold accumulator `lane` after the full blocks; adds the tail
-// stripes and the last stripe and returns the hash in every lane
+// stripes and the last stripe and returns the hash in every lane. Every lane
+// of the wave runs it: lane l reads the 16-byte piece l of the tail (stripe
+// l / 4, words 2 (l % 4) and 2 (l % 4) + 1) and lanes 0..3 the pieces of the
+// last stripe, with dword loads (any alignment) issued together; the terms
+// are summed across lanes like a block's.
 __device__ inline uint64_t ozl_xxh3_finish(const uint8_t* in, size_t len, size_t nbBlocks, uint64_t acc)
 {
     const int lane = threadIdx.x & 63;
-    if (lane < 8) {
-        size_t nbStripes = ((len - 1) - 1024 * nbBlocks) / 64;
-        for (size_t s = 0; s < nbStripes; s++) acc += ozl_stripe_term(in + nbBlocks * 1024 + s * 64, (int)s * 8, lane);
-        acc += ozl_stripe_term(in + len - 64, 192 - 64 - 7, lane);
+    const uint32_t st = (uint32_t)lane >> 2, j = (uint32_t)lane & 3;
+    const uint32_t nbStripes = (uint32_t)(((len - 1) - 1024 * nbBlocks) / 64);   // < 16
+    uint32_t d[4] = { 0, 0, 0, 0 }, e[4] = { 0, 0, 0, 0 };
+    const uint8_t* p = in + nbBlocks * 1024 + 16 * (size_t)lane;
+    const uint8_t* q = in + len - 64 + 16 * (size_t)j;
+    if (st < nbStripes)
+        for (int k = 0; k < 4; k++) d[k] = ozl_gld4(p + 4 * k);
+    if (lane < 4)
+        for (int k = 0; k < 4; k++) e[k] = ozl_gld4(q + 4 * k);
+    uint64_t ta = 0, tb = 0;   // terms of accumulators 2j and 2j + 1
+    if (st < nbStripes) {
+        const uint64_t w0 = d[0] | ((uint64_t)d[1] << 32), w1 = d[2] | ((uint64_t)d[3] << 32);
+        const uint64_t x0 = w0 ^ ozl_sec64(8 * (int)st + 16 * (int)j), x1 = w1 ^ ozl_sec64(8 * (int)st + 16 * (int)j + 8);
+        ta = w1 + (uint64_t)(uint32_t)x0 * (x0 >> 32);
+        tb = w0 + (uint64_t)(uint32_t)x1 * (x1 >> 32);
     }
+    if (lane < 4) {
+        const uint64_t w0 = e[0] | ((uint64_t)e[1] << 32), w1 = e[2] | ((uint64_t)e[3] << 32);
+        const uint64_t x0 = w0 ^ ozl_sec64(192 - 64 - 7 + 16 * (int)j), x1 = w1 ^ ozl_sec64(192 - 64 - 7 + 16 * (int)j + 8);
+        ta += w1 + (uint64_t)(uint32_t)x0 * (x0 >> 32);
+        tb += w0 + (uint64_t)(uint32_t)x1 * (x1 >> 32);
+    }
+    // sum over the lanes of the same j (rows of 16, then the four rows)
+    ta += ozl_dpp64<0x124>(ta);  // row_ror:4
+    tb += ozl_dpp64<0x124>(tb);
+    ta += ozl_dpp64<0x128>(ta);  // row_ror:8
+    tb += ozl_dpp64<0x128>(tb);
+    ta = ozl_add_xor32_64(ozl_add_xor16_64(ta));
+    tb = ozl_add_xor32_64(ozl_add_xor16_64(tb));
     uint64_t a[8];
-    for (int i = 0; i < 8; i++) a[i] = __shfl(acc, i, 64);
+    for (int i = 0; i < 8; i++) a[i] = __shfl(acc, i, 64) + __shfl((i & 1) ? tb : ta, i >> 1, 64);
     return ozl_xxh3_merge(a, len);
 }
 
+// XXH3 of a buffer by one wave (many waves per CU keep the loads in flight)
 __device__ inline uint64_t ozl_xxh3_wave(const uint8_t* in, size_t len, uint64_t* lds, int chainCap)
 {
     (void)lds;
