@@ -17,7 +17,7 @@
 //                  table from the code lengths, then one lane per Huffman
 //                  stream decodes (huf_decompress.c:587-787)
 //   k_pack_decode  store / constant / flatpack / bitpack
-//   k_dverify      one wavefront per frame: content and compressed checksums
+//   k_dverify      two wavefronts per frame: compressed and content checksums
 //
 // Bit streams are read backwards through 64-bit windows over a ring in LDS
 // per stream, which the stream's lanes refill from HBM with aligned 16-byte
@@ -906,23 +906,33 @@ __global__ void __launch_bounds__(kT) k_pack_decode(const uint8_t* __restrict__ 
     }
 }
 
-__global__ void __launch_bounds__(64) k_dverify(const uint8_t* __restrict__ frames, const uint64_t* __restrict__ foff,
-                                                uint32_t B, const uint8_t* __restrict__ dst,
-                                                const uint64_t* __restrict__ doff, ozl_dws ws, int* status)
+// two waves per frame: wave 0 hashes the compressed region, wave 1 the
+// decoded output (no LDS, so that many frames' waves share a CU and keep
+// their loads in flight)
+__global__ void __launch_bounds__(128) k_dverify(const uint8_t* __restrict__ frames, const uint64_t* __restrict__ foff,
+                                                 uint32_t B, const uint8_t* __restrict__ dst,
+                                                 const uint64_t* __restrict__ doff, ozl_dws ws, int* status)
 {
-    __shared__ uint64_t lds[8 * 128];
+    __shared__ uint32_t hsh[2];
     uint32_t b = blockIdx.x;
     if (b >= B) return;
     ozl_dplan d = ws.plans[b];
     int st = (int)d.status;
+    const uint8_t* fr = frames + foff[b];
+    const int w = (int)(threadIdx.x >> 6);
     if (st == 0) {
-        const uint8_t* fr = frames + foff[b];
-        uint64_t hc = ozl_xxh3_wave(fr + d.chunkOff, d.csumOff + 4 - d.chunkOff, lds, 128);
-        uint64_t hd = ozl_xxh3_wave(dst + doff[b], d.n, lds, 128);
-        if ((uint32_t)hc != ozl_le32(fr + d.csumOff + 4)) st = OZL_ERR_COMPRESSED_CHECKSUM;
-        else if ((uint32_t)hd != ozl_le32(fr + d.csumOff)) st = OZL_ERR_CONTENT_CHECKSUM;
+        const uint64_t h = w == 0 ? ozl_xxh3_wave(fr + d.chunkOff, d.csumOff + 4 - d.chunkOff, nullptr, 0)
+                                  : ozl_xxh3_wave(dst + doff[b], d.n, nullptr, 0);
+        if ((threadIdx.x & 63) == 0) hsh[w] = (uint32_t)h;
     }
-    if (threadIdx.x == 0) status[b] = st;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (st == 0) {
+            if (hsh[0] != ozl_le32(fr + d.csumOff + 4)) st = OZL_ERR_COMPRESSED_CHECKSUM;
+            else if (hsh[1] != ozl_le32(fr + d.csumOff)) st = OZL_ERR_CONTENT_CHECKSUM;
+        }
+        status[b] = st;
+    }
 }
 
 // ---------------------------------------------------------------- items of the graph decoder
@@ -1099,7 +1109,7 @@ int ozl_entropy_decompress_batch(const void* frames, const uint64_t* foff, uint3
     OZL_LAUNCH("k_fse_decode", k_fse_decode, dim3((B + kFQ - 1) / kFQ), dim3(64), 0, st, B, ws);
     OZL_LAUNCH("k_huf_decode", k_huf_decode, dim3((B + kHF - 1) / kHF), dim3(64), 0, st, B, ws);
     OZL_LAUNCH("k_pack_decode", k_pack_decode, dim3(B), dim3(kT), 0, st, f, foff, B, o, doff, ws);
-    OZL_LAUNCH("k_dverify", k_dverify, dim3(B), dim3(64), 0, st, f, foff, B, (const uint8_t*)o, doff, ws, status);
+    OZL_LAUNCH("k_dverify", k_dverify, dim3(B), dim3(128), 0, st, f, foff, B, (const uint8_t*)o, doff, ws, status);
     OZL_HIP_CHECK(hipGetLastError());
     return OZL_OK;
 }

@@ -744,7 +744,6 @@ __global__ void __launch_bounds__(64) k_sel_hash(uint32_t K, SelWs ws, uint8_t* 
                                                  const uint64_t* __restrict__ frameOff, int compressed,
                                                  int* __restrict__ status)
 {
-    __shared__ uint64_t lds[8 * 128];
     const uint32_t c = blockIdx.x;
     if (c >= K) return;
     const SelChunk& H = ws.ch[c];
@@ -756,8 +755,7 @@ __global__ void __launch_bounds__(64) k_sel_hash(uint32_t K, SelWs ws, uint8_t* 
     uint8_t* fr = dst + frameOff[c];
     const size_t flen = frameOff[c + 1] - frameOff[c];
     if (!compressed) {
-        const uint64_t h = ozl_xxh3_wave(reinterpret_cast<const uint8_t*>((uintptr_t)H.src), (size_t)H.n * H.w, lds,
-                                         128);
+        const uint64_t h = ozl_xxh3_wave(reinterpret_cast<const uint8_t*>((uintptr_t)H.src), (size_t)H.n * H.w, nullptr, 0);
         if (threadIdx.x == 0)
             for (int k = 0; k < 4; k++) fr[flen - 9 + k] = (uint8_t)(h >> (8 * k));
         return;
@@ -765,7 +763,7 @@ __global__ void __launch_bounds__(64) k_sel_hash(uint32_t K, SelWs ws, uint8_t* 
     uint8_t fhb[24];
     const size_t fh = ozl_write_frame_header(fhb, OZL_T_NUMERIC, (uint64_t)H.n * H.w, 1, 1);
     const size_t clen = flen - fh - 4 - 1;
-    const uint64_t h = ozl_xxh3_wave(fr + fh, clen, lds, 128);
+    const uint64_t h = ozl_xxh3_wave(fr + fh, clen, nullptr, 0);
     if (threadIdx.x == 0) {
         for (int k = 0; k < 4; k++) fr[fh + clen + k] = (uint8_t)(h >> (8 * k));
         fr[flen - 1] = 0;
